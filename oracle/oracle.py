@@ -1,0 +1,168 @@
+"""Oracle (test infrastructure only): ctypes front-end to oracle/liboracle.so, the plain-C
+FP64 restatement of the reference's TG substep (oracle/tg_oracle.c), plus the seeded initial
+field used by tests and the bench.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may import this module,
+and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import setup as _setup
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("L", C.c_int), ("M", C.c_int), ("N", C.c_int),
+                ("dxi", C.c_double), ("dzi", C.c_double), ("dxqi", C.c_double),
+                ("dzqi", C.c_double), ("volm", C.c_double),
+                ("dyfi", _dp), ("dyci", _dp), ("wff", _dp), ("wfb", _dp),
+                ("amvr", _dp), ("acvr", _dp), ("apvr", _dp),
+                ("amph", _dp), ("acph", _dp), ("apph", _dp),
+                ("ak1", _dp), ("ak3", _dp), ("tgam", _dp), ("troh", _dp), ("talp", _dp)]
+
+
+class _State(C.Structure):
+    _fields_ = [("q", _dp), ("pr", _dp), ("convh0", _dp), ("qtmp", _dp), ("dph", _dp),
+                ("rhsllphi", _dp), ("rhs", _dp), ("dpgrns", C.c_double)]
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")],
+                   cwd=os.path.dirname(HERE), check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        for name, args in (("or_substep", [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double]),
+                           ("or_convection", [C.c_void_p, C.c_void_p]),
+                           ("or_divg", [C.c_void_p, C.c_void_p, C.c_int, C.c_double]),
+                           ("or_poisson", [C.c_void_p, _dp, _dp]),
+                           ("or_tdma_cyc", [C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                            C.c_long, _dp]),
+                           ("or_tdma_noncyc", [C.c_int, _dp, _dp, _dp, _dp, C.c_long, _dp]),
+                           ("or_dft", [C.c_int, C.c_int, _dp, _dp, _dp, _dp])):
+            fn = getattr(_lib, name)
+            fn.argtypes = args
+            fn.restype = None
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class OracleTG:
+    """Channel state + reference-order substep on the CPU (FP64).
+
+    Arrays are C-order views of the reference's Fortran layout:
+    ``q[c, k, J, i]`` (J = 0..M+1), ``pr[k, J, i]``, ``convh0[c, k, J-1, i]``.
+    """
+
+    def __init__(self, mesh: _setup.Mesh, REN: float):
+        self.mesh = mesh
+        self.cvisc = 1.0 / REN
+        L, M, N = mesh.L, mesh.M, mesh.N
+        self._keep = []
+
+        def arr(x):
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            self._keep.append(a)
+            return _p(a)
+
+        self._m = _Mesh(L, M, N, mesh.DXI, mesh.DZI, mesh.DXQI, mesh.DZQI, mesh.VOLM,
+                        arr(mesh.DYFI), arr(mesh.DYCI), arr(mesh.WFF), arr(mesh.WFB),
+                        arr(mesh.AMVR), arr(mesh.ACVR), arr(mesh.APVR),
+                        arr(mesh.AMPH), arr(mesh.ACPH), arr(mesh.APPH),
+                        arr(mesh.AK1), arr(mesh.AK3), arr(_setup.TGAM), arr(_setup.TROH),
+                        arr(_setup.TALP))
+        self.q = np.zeros((3, N, M + 2, L))
+        self.pr = np.zeros((N, M + 2, L))
+        self.convh0 = np.zeros((3, N, M, L))
+        self.qtmp = np.zeros((N, M + 2, L))
+        self.dph = np.zeros((N, M + 2, L))
+        self.rhsllphi = np.zeros((N, M, L))
+        self.rhs = np.zeros((N, M, L))
+        self._s = _State(_p(self.q), _p(self.pr), _p(self.convh0), _p(self.qtmp),
+                         _p(self.dph), _p(self.rhsllphi), _p(self.rhs), 0.0)
+
+    def set_fields(self, q, pr):
+        self.q[...] = q
+        self.pr[...] = pr
+
+    def substep(self, ns: int, dt: float):
+        lib().or_substep(C.byref(self._m), C.byref(self._s), ns, dt, self.cvisc)
+
+    def step(self, dt: float):
+        for ns in (1, 2, 3):
+            self.substep(ns, dt)
+
+    def convection(self):
+        lib().or_convection(C.byref(self._m), C.byref(self._s))
+        return self.qtmp[:, 1:-1].copy(), self.dph[:, 1:-1].copy(), self.rhsllphi.copy()
+
+    def poisson(self, rhs_cells):
+        L, M, N = self.mesh.L, self.mesh.M, self.mesh.N
+        src = np.ascontiguousarray(rhs_cells, dtype=np.float64)
+        out = np.zeros((N, M + 2, L))
+        lib().or_poisson(C.byref(self._m), _p(src), _p(out))
+        return out[:, 1:M + 1].copy()
+
+    @property
+    def dpgrns(self):
+        return self._s.dpgrns
+
+
+def tdma_cyc(a, b, c, r):
+    r = np.array(r, dtype=np.float64)
+    work = np.zeros(3 * len(r))
+    lib().or_tdma_cyc(len(r), a, b, c, _p(r), 1, _p(work))
+    return r
+
+
+def tdma_noncyc(a, b, c, r):
+    a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c))
+    r = np.array(r, dtype=np.float64)
+    work = np.zeros(2 * len(r))
+    lib().or_tdma_noncyc(len(r), _p(a), _p(b), _p(c), _p(r), 1, _p(work))
+    return r
+
+
+def dft(x, sign):
+    x = np.asarray(x, dtype=np.complex128)
+    xr, xi = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+    yr, yi = np.zeros(len(x)), np.zeros(len(x))
+    lib().or_dft(len(x), sign, _p(xr), _p(xi), _p(yr), _p(yi))
+    return yr + 1j * yi
+
+
+def initial_field(mesh: _setup.Mesh, seed: int, amp: float = 0.1):
+    """Seeded initial field: Poiseuille profile (LAMPOISLPROF) in u plus uniform random
+    perturbations U(-amp, amp) in u, v, w (bulk velocity is 1), v = 0 on the walls, wall
+    ghost rows set as BC_WALL_Q_tg.  Returns (q[3,N,M+2,L], pr[N,M+2,L]); not yet
+    divergence-free — the NS = 0 substep projects it, as CALC_INITIALIZATION_tg does
+    (INITIAL_FL_FLD_TG.f90:109-121).  Counter-based generator (numpy Philox) so a field is
+    reproducible from (seed, shape) alone."""
+    L, M, N = mesh.L, mesh.M, mesh.N
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    q = np.zeros((3, N, M + 2, L))
+    pert = rng.uniform(-amp, amp, size=(3, N, M, L))
+    q[0, :, 1:M + 1, :] = mesh.VINI[None, :, None] + pert[0]
+    q[1, :, 2:M + 1, :] = pert[1, :, 1:, :]
+    q[2, :, 1:M + 1, :] = pert[2]
+    pr = np.zeros((N, M + 2, L))
+    return q, pr

@@ -1,0 +1,227 @@
+!-----------------------------------------------------------------------------------------------
+! ref_harness: test infrastructure only.  Drives single routines of the unmodified CHAPSim 1.0
+! reference (linked from oracle/_ref/obj by oracle/ref.mk) on inputs given as raw files, and
+! dumps what they compute, so the C restatement in oracle/ and the HIP path can be checked
+! against the reference itself.
+!
+! Set-up repeats the reference main program's sequence (DNS_THERMAL.f90:59-130): READINI,
+! decomposition, MEM_ALLOCAT, index maps, RK coefficients, mesh, Poiseuille profile, TDMA
+! coefficients and FFT99_POIS3D_INIT(ITG).  Then commands are read from ./harness.cmd, one
+! per line:
+!   dumpsetup FILE          rank 0 writes the set-up arrays (layout in oracle/refrun.py)
+!   load FILE               every rank reads global U,V,W,P (each L x (M+2) x N, y-halo
+!                           rows 0..M+1, Fortran order, float64) and keeps its y-slab
+!   substep NS DT           DT = given, CVISC = 1/REN, CALL SOLVERRK3_MOM_tg(NS)
+!   dump FILE               each rank writes FILE.<rank>: Q_tg and PR_tg rows 0..N2DO+1
+!   conv FILE               CONVECTION_X/Y/Z_tg, then dumps QTMP, DPH, RHSLLPHI rows 1..N2DO
+!   divg NS DT FILE         DIVG_tg(NS) and dumps RHSLLPHI_tg rows 1..N2DO
+!   poisson FILEIN FILE     RHSLLPHI_tg from a global L x M x N file, FFT99 Poisson solve,
+!                           dumps DPH_tg rows 1..N2DO
+!   rhsmom NS DT FILE       conv + RHS_CvLpGpS_tg/MOMFA_tg for IDR=1..3 (as SOLVERRK3_MOM_tg
+!                           does), dumps Q_tg rows 0..N2DO+1 and RHS_tg rows 1..N2DO
+! Must run on 2 or more MPI ranks (the reference deadlocks on one, INTFC_VARS.f90:57-79).
+!-----------------------------------------------------------------------------------------------
+PROGRAM ref_harness
+    USE init_info
+    USE mesh_info
+    USE flow_info
+    USE FFT99_info1
+    IMPLICIT NONE
+
+    CHARACTER(LEN=512) :: line, cmd, fname, fname2
+    INTEGER(4) :: ios, ns, idr
+    REAL(WP) :: dtin
+
+    CALL MKDIR
+    CALL Start_MPI
+    IF (MYID == 0) THEN
+        LOGFIL_PG = TRIM(FilePath0) // 'CHAPSim_harness.log'
+        OPEN (logflg_pg, FILE = TRIM(LOGFIL_PG), action = 'write')
+    END IF
+    IF (MYID == 0) CALL READINI
+    CALL BCAST_READINI
+    CALL mesh_Ydecomp
+    CALL mesh_Zdecomp
+    CALL MEM_ALLOCAT
+    CALL INDEXL2G
+    CALL INDEXSET
+    IF (MYID == 0) CALL CONS_RKCOEF
+    CALL BCAST_RKCOEF
+    IF (MYID == 0) CALL CONSPARA
+    CALL BCAST_CONSPARA
+    IF (MYID == 0) CALL COORDJR
+    CALL BCAST_COORDJR
+    IF (MYID == 0) CALL LAMPOISLPROF
+    CALL BCAST_LAMPOISLPROF
+    IF (MYID == 0) CALL TDMA_COEF
+    CALL BCAST_TDMA_COEF
+    CALL FFT99_POIS3D_INIT(ITG)
+    CALL MPI_BARRIER(ICOMM, IERROR)
+
+    CONVH0_tg = 0.0_WP
+    CVISC = 1.0_WP / REN
+
+    OPEN(77, FILE = 'harness.cmd', STATUS = 'old', ACTION = 'read')
+    DO
+        READ(77, '(A)', IOSTAT = ios) line
+        IF (ios /= 0) EXIT
+        IF (LEN_TRIM(line) == 0) CYCLE
+        READ(line, *) cmd
+        SELECT CASE (TRIM(cmd))
+        CASE ('dumpsetup')
+            READ(line, *) cmd, fname
+            IF (MYID == 0) CALL H_DUMPSETUP(TRIM(fname))
+        CASE ('load')
+            READ(line, *) cmd, fname
+            CALL H_LOAD(TRIM(fname))
+        CASE ('substep')
+            READ(line, *) cmd, ns, dtin
+            DT = dtin
+            CVISC = 1.0_WP / REN
+            CALL SOLVERRK3_MOM_tg(ns)
+        CASE ('dump')
+            READ(line, *) cmd, fname
+            CALL H_DUMPQ(TRIM(fname))
+        CASE ('conv')
+            READ(line, *) cmd, fname
+            CALL CONVECTION_X_tg
+            CALL CONVECTION_Y_tg
+            CALL CONVECTION_Z_tg
+            CALL H_DUMPCONV(TRIM(fname))
+        CASE ('divg')
+            READ(line, *) cmd, ns, dtin, fname
+            DT = dtin
+            CALL DIVG_tg(ns)
+            CALL H_DUMP3(TRIM(fname), RHSLLPHI_tg, 1)
+        CASE ('poisson')
+            READ(line, *) cmd, fname2, fname
+            CALL H_LOADRHS(TRIM(fname2))
+            CALL FFT99_POIS3D_periodicxz(ITG)
+            CALL H_DUMP3H(TRIM(fname), DPH_tg)
+        CASE ('rhsmom')
+            READ(line, *) cmd, ns, dtin, fname
+            DT = dtin
+            CVISC = 1.0_WP / REN
+            CALL CONVECTION_X_tg
+            CALL CONVECTION_Y_tg
+            CALL CONVECTION_Z_tg
+            DO idr = 1, 3
+                CALL RHS_CvLpGpS_tg(ns, idr)
+                CALL MOMFA_tg(ns, idr)
+            END DO
+            CALL H_DUMPQ(TRIM(fname))
+            CALL H_DUMP3(TRIM(fname) // '.rhs', RHS_tg, 1)
+        CASE DEFAULT
+            IF (MYID == 0) WRITE(*, *) 'ref_harness: unknown command ', TRIM(cmd)
+            CALL MPI_ABORT(ICOMM, 2, IERROR)
+        END SELECT
+        CALL MPI_BARRIER(ICOMM, IERROR)
+    END DO
+    CLOSE(77)
+
+    CALL MPI_FINALIZE(IERROR)
+
+CONTAINS
+
+    SUBROUTINE H_OPENRANK(fn, unit)
+        CHARACTER(*), INTENT(IN) :: fn
+        INTEGER(4), INTENT(IN) :: unit
+        CHARACTER(LEN=16) :: rk
+        WRITE(rk, '(I0)') MYID
+        OPEN(unit, FILE = fn // '.' // TRIM(rk), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+             STATUS = 'replace', ACTION = 'write')
+        WRITE(unit) INT(MYID, 4), INT(JDSWT(MYID), 4), INT(N2DO(MYID), 4), &
+                    INT(NCL1_tg, 4), INT(NCL2, 4), INT(NCL3, 4)
+    END SUBROUTINE
+
+    SUBROUTINE H_DUMPSETUP(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        OPEN(78, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'replace', &
+             ACTION = 'write')
+        WRITE(78) INT(NCL1_tg, 4), INT(NCL2, 4), INT(NCL3, 4)
+        WRITE(78) IPV_tg(1:NCL1_tg), IMV_tg(1:NCL1_tg), KPV(1:NCL3), KMV(1:NCL3), KSYM(1:NCL3)
+        WRITE(78) JGPV(0:NCL2), JGMV(1:NND2)
+        WRITE(78) DX, DZ, DXI, DZI, DXQI, DZQI, VOLM_tg, Area_inlet, AMVR1, APVRN
+        WRITE(78) TGAM(0:3), TROH(0:3), TALP(0:3)
+        WRITE(78) YND(1:NND2), YCC(1:NCL2), DYFI(1:NCL2), DYCI(1:NND2)
+        WRITE(78) YCL2ND_WFF(1:NND2), YCL2ND_WFB(1:NND2)
+        WRITE(78) AMVR(1:NND2, 1:3), ACVR(1:NND2, 1:3), APVR(1:NND2, 1:3)
+        WRITE(78) AMPH(1:NCL2), ACPH(1:NCL2), APPH(1:NCL2)
+        WRITE(78) AK1(1:NCL1_tg), AK3(1:NCL3)
+        WRITE(78) Vini(1:NCL2)
+        CLOSE(78)
+    END SUBROUTINE
+
+    SUBROUTINE H_LOAD(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: G(:, :, :, :)
+        INTEGER(4) :: J, JG
+        ALLOCATE(G(NCL1_tg, 0:NCL2 + 1, NCL3, 4))
+        OPEN(79, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', &
+             ACTION = 'read')
+        READ(79) G
+        CLOSE(79)
+        DO J = 0, N2DO(MYID) + 1
+            JG = JDSWT(MYID) - 1 + J
+            Q_tg(:, J, :, 1) = G(:, JG, :, 1)
+            Q_tg(:, J, :, 2) = G(:, JG, :, 2)
+            Q_tg(:, J, :, 3) = G(:, JG, :, 3)
+            PR_tg(:, J, :)   = G(:, JG, :, 4)
+        END DO
+        DEALLOCATE(G)
+    END SUBROUTINE
+
+    SUBROUTINE H_LOADRHS(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: G(:, :, :)
+        INTEGER(4) :: J, JG
+        ALLOCATE(G(NCL1_tg, NCL2, NCL3))
+        OPEN(79, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', &
+             ACTION = 'read')
+        READ(79) G
+        CLOSE(79)
+        DO J = 1, N2DO(MYID)
+            JG = JDSWT(MYID) - 1 + J
+            RHSLLPHI_tg(:, J, :) = G(:, JG, :)
+        END DO
+        DEALLOCATE(G)
+    END SUBROUTINE
+
+    SUBROUTINE H_DUMPQ(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        INTEGER(4) :: n
+        CALL H_OPENRANK(fn, 80)
+        DO n = 1, 3
+            WRITE(80) Q_tg(:, 0:N2DO(MYID) + 1, :, n)
+        END DO
+        WRITE(80) PR_tg(:, 0:N2DO(MYID) + 1, :)
+        CLOSE(80)
+    END SUBROUTINE
+
+    SUBROUTINE H_DUMPCONV(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        CALL H_OPENRANK(fn, 80)
+        WRITE(80) QTMP_tg(:, 1:N2DO(MYID), :)
+        WRITE(80) DPH_tg(:, 1:N2DO(MYID), :)
+        WRITE(80) RHSLLPHI_tg(:, 1:N2DO(MYID), :)
+        CLOSE(80)
+    END SUBROUTINE
+
+    SUBROUTINE H_DUMP3(fn, A, j0)
+        CHARACTER(*), INTENT(IN) :: fn
+        INTEGER(4), INTENT(IN) :: j0
+        REAL(WP), INTENT(IN) :: A(NCL1_tg, j0:N2DO(0) + j0 - 1, NCL3)
+        CALL H_OPENRANK(fn, 80)
+        WRITE(80) A(:, 1:N2DO(MYID), :)
+        CLOSE(80)
+    END SUBROUTINE
+
+    SUBROUTINE H_DUMP3H(fn, A)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), INTENT(IN) :: A(NCL1_tg, 0:N2DO(0) + 1, NCL3)
+        CALL H_OPENRANK(fn, 80)
+        WRITE(80) A(:, 1:N2DO(MYID), :)
+        CLOSE(80)
+    END SUBROUTINE
+
+END PROGRAM ref_harness

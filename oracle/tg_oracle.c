@@ -1,0 +1,471 @@
+/*
+ * Oracle — TEST INFRASTRUCTURE ONLY.  Plain-C, FP64 restatement of the reference's
+ * fractional-step RK3/Crank-Nicolson substep for the periodic plane channel on the
+ * turbulence-generator domain (SOLVERRK3_MOM_tg, SOLVERRK3_MOM_tg.f90:13-56).  Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, and only as the
+ * checker; nothing in the product links or calls it.
+ *
+ * Layout (same as the reference, MEMALLOCT.f90:96-103, x fastest):
+ *   halo fields   Q(i, J, k, c) for J = 0..M+1   -> q[((c*N + k)*(M+2) + J)*L + i]
+ *   cell fields   RHS(i, J, k)  for J = 1..M     -> rhs[(k*M + J-1)*L + i]
+ * i and k are 0-based here (reference I = i+1, K = k+1); J keeps the reference's value.
+ * RCCI1/RCCI2/RNDI1/RNDI2 are 1 for the channel (COORDJR.f90:39-42) and are dropped.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
+ * Pinned by tests/test_oracle_ref.py (against oracle/_ref, the compiled reference) and by
+ * tests/test_oracle_math.py (dense solves, DFT definitions, manufactured Poisson solutions).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    int L, M, N;
+    double dxi, dzi, dxqi, dzqi, volm;
+    const double *dyfi;                /* [M]   DYFI(J) at [J-1]              */
+    const double *dyci;                /* [M+2] DYCI(J) at [J]                */
+    const double *wff, *wfb;           /* [M+2] YCL2ND_WFF/WFB(J) at [J]      */
+    const double *amvr, *acvr, *apvr;  /* [3][M+2] A*VR(J, IDR) at [IDR-1][J] */
+    const double *amph, *acph, *apph;  /* [M]   A*PH(J) at [J-1]              */
+    const double *ak1, *ak3;           /* [L], [N] modified wavenumbers       */
+    const double *tgam, *troh, *talp;  /* [4] RK coefficients, index NS       */
+} or_mesh;
+
+typedef struct {
+    double *q;        /* [3][N][M+2][L] Q_tg          */
+    double *pr;       /* [N][M+2][L]    PR_tg         */
+    double *convh0;   /* [3][N][M][L]   CONVH0_tg     */
+    double *qtmp;     /* [N][M+2][L]    QTMP_tg       */
+    double *dph;      /* [N][M+2][L]    DPH_tg        */
+    double *rhsllphi; /* [N][M][L]      RHSLLPHI_tg   */
+    double *rhs;      /* [N][M][L]      RHS_tg        */
+    double dpgrns;    /* last mass-flux correction (RHS_CvLpGpS_tg.f90:179-191) */
+} or_state;
+
+#define LL (m->L)
+#define MM (m->M)
+#define NN (m->N)
+#define HX(a, i, J, k) (a)[((size_t)(k) * (MM + 2) + (J)) * LL + (i)]
+#define QC(c, i, J, k) s->q[(((size_t)(c) * NN + (k)) * (MM + 2) + (J)) * LL + (i)]
+#define CX(a, i, J, k) (a)[((size_t)(k) * MM + (J) - 1) * LL + (i)]
+#define H0C(c, i, J, k) s->convh0[(((size_t)(c) * NN + (k)) * MM + (J) - 1) * LL + (i)]
+
+static inline int ip_(const or_mesh *m, int i) { return i == m->L - 1 ? 0 : i + 1; }
+static inline int im_(const or_mesh *m, int i) { return i == 0 ? m->L - 1 : i - 1; }
+static inline int kp_(const or_mesh *m, int k) { return k == m->N - 1 ? 0 : k + 1; }
+static inline int km_(const or_mesh *m, int k) { return k == 0 ? m->N - 1 : k - 1; }
+
+/* CONVECTION_X_tg (CONVECTION_X.f90:13-64), _Y_tg (CONVECTION_Y.f90:13-84, channel branch),
+ * _Z_tg (CONVECTION_Z.f90:13-73): divergence-form staggered convection into QTMP, DPH,
+ * RHSLLPHI.  XND2CL = YND2CL = ZND2CL = 0.5 (COORDJR.f90:466-468). */
+void or_convection(const or_mesh *m, or_state *s)
+{
+    const double h = 0.5;
+    memset(s->qtmp, 0, sizeof(double) * (size_t)NN * (MM + 2) * LL);
+    memset(s->dph, 0, sizeof(double) * (size_t)NN * (MM + 2) * LL);
+    memset(s->rhsllphi, 0, sizeof(double) * (size_t)NN * MM * LL);
+    for (int J = 1; J <= MM; ++J) {
+        const int JM = J - 1, JP = J + 1, JJP = J + 1;
+        const double coe1 = h * h * m->dxi;
+        const double coe2 = m->dyfi[J - 1] * h;
+        const double coe3 = h * h * m->dzi;
+        for (int k = 0; k < NN; ++k) {
+            const int km = km_(m, k), kp = kp_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                double H11 = ((QC(0, ip, J, k) + QC(0, i, J, k)) * (QC(0, ip, J, k) + QC(0, i, J, k)) -
+                              (QC(0, im, J, k) + QC(0, i, J, k)) * (QC(0, im, J, k) + QC(0, i, J, k))) * coe1;
+                double H12 = ((QC(1, i, JP, k) + QC(1, im, JP, k)) *
+                                  (m->wff[JJP] * QC(0, i, JP, k) + m->wfb[JJP] * QC(0, i, J, k)) -
+                              (QC(1, i, J, k) + QC(1, im, J, k)) *
+                                  (m->wff[J] * QC(0, i, J, k) + m->wfb[J] * QC(0, i, JM, k))) * coe2;
+                double H13 = ((QC(2, i, J, kp) + QC(2, im, J, kp)) * (QC(0, i, J, kp) + QC(0, i, J, k)) -
+                              (QC(2, i, J, k) + QC(2, im, J, k)) * (QC(0, i, J, k) + QC(0, i, J, km))) * coe3;
+                HX(s->qtmp, i, J, k) = -(H11 + H12 + H13);
+            }
+        }
+    }
+    for (int J = 2; J <= MM; ++J) {   /* NYI = 2 on the bottom rank */
+        const int JM = J - 1, JP = J + 1;
+        const double coe1 = m->dxi * h;
+        const double coe2 = m->dyci[J] * h * h;
+        const double coe31 = m->dzi * h;
+        for (int k = 0; k < NN; ++k) {
+            const int km = km_(m, k), kp = kp_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                double H21 = ((m->wff[J] * QC(0, ip, J, k) + m->wfb[J] * QC(0, ip, JM, k)) *
+                                  (QC(1, ip, J, k) + QC(1, i, J, k)) -
+                              (m->wff[J] * QC(0, i, J, k) + m->wfb[J] * QC(0, i, JM, k)) *
+                                  (QC(1, i, J, k) + QC(1, im, J, k))) * coe1;
+                double H22 = ((QC(1, i, JP, k) + QC(1, i, J, k)) * (QC(1, i, JP, k) + QC(1, i, J, k)) -
+                              (QC(1, i, J, k) + QC(1, i, JM, k)) * (QC(1, i, J, k) + QC(1, i, JM, k))) * coe2;
+                double H23 = ((m->wff[J] * QC(2, i, J, kp) + m->wfb[J] * QC(2, i, JM, kp)) *
+                                  (QC(1, i, J, kp) + QC(1, i, J, k)) -
+                              (m->wff[J] * QC(2, i, J, k) + m->wfb[J] * QC(2, i, JM, k)) *
+                                  (QC(1, i, J, k) + QC(1, i, J, km))) * coe31;
+                HX(s->dph, i, J, k) = -(H21 + H22 + H23);
+            }
+        }
+    }
+    for (int J = 1; J <= MM; ++J) {
+        const int JM = J - 1, JP = J + 1, JJP = J + 1;
+        const double coe1 = h * h * m->dxi;
+        const double coe2 = m->dyfi[J - 1] * h;
+        const double coe3 = h * h * m->dzi;
+        for (int k = 0; k < NN; ++k) {
+            const int km = km_(m, k), kp = kp_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                double H31 = ((QC(0, ip, J, k) + QC(0, ip, J, km)) * (QC(2, ip, J, k) + QC(2, i, J, k)) -
+                              (QC(0, i, J, k) + QC(0, i, J, km)) * (QC(2, i, J, k) + QC(2, im, J, k))) * coe1;
+                double H32 = ((QC(1, i, JP, k) + QC(1, i, JP, km)) *
+                                  (m->wff[JJP] * QC(2, i, JP, k) + m->wfb[JJP] * QC(2, i, J, k)) -
+                              (QC(1, i, J, k) + QC(1, i, J, km)) *
+                                  (m->wff[J] * QC(2, i, J, k) + m->wfb[J] * QC(2, i, JM, k))) * coe2;
+                double H33 = ((QC(2, i, J, kp) + QC(2, i, J, k)) * (QC(2, i, J, kp) + QC(2, i, J, k)) -
+                              (QC(2, i, J, k) + QC(2, i, J, km)) * (QC(2, i, J, k) + QC(2, i, J, km))) * coe3;
+                CX(s->rhsllphi, i, J, k) = -(H31 + H32 + H33);
+            }
+        }
+    }
+}
+
+/* RHS_CvLpGpS_tg (RHS_CvLpGpS_tg.f90:12-204), channel, iFlowDriven = 1, NFLOW = 1.
+ * Rows J >= NII; for IDR = 2 the bottom row J = 1 of RHS_tg is left as it was. */
+void or_rhs(const or_mesh *m, or_state *s, int ns, int idr, double dt, double cvisc)
+{
+    const int c = idr - 1;
+    const int nii = (idr == 2) ? 2 : 1;
+    const double *conv = idr == 1 ? s->qtmp : (idr == 2 ? s->dph : NULL);
+    const double coe1 = m->talp[ns] * cvisc;   /* M_inlet / D_inlet = 1 (modules.f90:584-586) */
+    for (int k = 0; k < NN; ++k) {
+        const int km = km_(m, k), kp = kp_(m, k);
+        for (int J = nii; J <= MM; ++J) {
+            const double ap = m->apvr[c * (MM + 2) + J], ac = m->acvr[c * (MM + 2) + J],
+                         am = m->amvr[c * (MM + 2) + J];
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                const double convh = conv ? HX(conv, i, J, k) : CX(s->rhsllphi, i, J, k);
+                double rhsc = m->tgam[ns] * convh + m->troh[ns] * H0C(c, i, J, k);
+                double rhsl = coe1 * ((QC(c, ip, J, k) - 2.0 * QC(c, i, J, k) + QC(c, im, J, k)) * m->dxqi +
+                                      (QC(c, i, J, kp) - 2.0 * QC(c, i, J, k) + QC(c, i, J, km)) * m->dzqi +
+                                      (QC(c, i, J + 1, k) * ap + QC(c, i, J, k) * ac + QC(c, i, J - 1, k) * am));
+                H0C(c, i, J, k) = convh;
+                CX(s->rhs, i, J, k) = (rhsc + rhsl) * dt;
+            }
+        }
+    }
+    const double coe2 = m->talp[ns] * dt;
+    for (int k = 0; k < NN; ++k) {
+        const int km = km_(m, k);
+        for (int J = nii; J <= MM; ++J) {
+            const double sucacj = m->dyci[J];
+            for (int i = 0; i < LL; ++i) {
+                double pgm;
+                if (idr == 1) pgm = (HX(s->pr, i, J, k) - HX(s->pr, im_(m, i), J, k)) * m->dxi * coe2;
+                else if (idr == 2) pgm = (HX(s->pr, i, J, k) - HX(s->pr, i, J - 1, k)) * sucacj * coe2;
+                else pgm = (HX(s->pr, i, J, k) - HX(s->pr, i, J, km)) * m->dzi * coe2;
+                CX(s->rhs, i, J, k) = CX(s->rhs, i, J, k) - pgm;
+            }
+        }
+    }
+    if (idr == 1) {
+        double intg = 0.0;   /* summed I, J, K order as RHS_CvLpGpS_tg.f90:171-178 */
+        for (int i = 0; i < LL; ++i)
+            for (int J = 1; J <= MM; ++J)
+                for (int k = 0; k < NN; ++k) intg = intg + CX(s->rhs, i, J, k) / m->dyfi[J - 1];
+        const double dpgrns = intg / m->volm;
+        s->dpgrns = dpgrns;
+        for (int k = 0; k < NN; ++k)
+            for (int i = 0; i < LL; ++i)
+                for (int J = 1; J <= MM; ++J) CX(s->rhs, i, J, k) = CX(s->rhs, i, J, k) - dpgrns;
+    }
+}
+
+/* TDMAIJI_CYC (TDMAIJI_CYC.f90:67-157) for one line of length n with constant-in-line
+ * coefficients a (sub), b (diag), c (super), in place on r (stride rs).  Work: 3n doubles. */
+void or_tdma_cyc(int n, double a, double b, double c, double *r, long rs, double *work)
+{
+    double *H0 = work, *G1 = work + n, *G2 = work + 2 * n;
+    H0[0] = c / b;
+    G1[0] = r[0] / b;
+    G2[0] = -a / b;
+    for (int i = 1; i <= n - 3; ++i) {
+        double pp = 1.0 / (b - H0[i - 1] * a);
+        double qq1 = G1[i - 1] * a, qq2 = G2[i - 1] * a;
+        H0[i] = c * pp;
+        G1[i] = (r[i * rs] - qq1) * pp;
+        G2[i] = -qq2 * pp;
+    }
+    {
+        int i = n - 2;
+        double pp = 1.0 / (b - H0[i - 1] * a);
+        double qq1 = G1[i - 1] * a, qq2 = G2[i - 1] * a;
+        G1[i] = (r[i * rs] - qq1) * pp;
+        G2[i] = (-c - qq2) * pp;
+    }
+    for (int i = n - 3; i >= 0; --i) {
+        G1[i] = -H0[i] * G1[i + 1] + G1[i];
+        G2[i] = -H0[i] * G2[i + 1] + G2[i];
+    }
+    {
+        int i = n - 1;
+        double pp = r[i * rs] - c * G1[0] - a * G1[i - 1];
+        double qq1 = b + c * G2[0] + a * G2[i - 1];
+        r[i * rs] = pp / qq1;
+    }
+    for (int i = n - 2; i >= 0; --i) r[i * rs] = G1[i] + G2[i] * r[(n - 1) * rs];
+}
+
+/* TDMAIJJ_nonCYC (TDMAIJJ_nonCYC.f90:67-123) for one line with per-row coefficients a,b,c
+ * and zero boundary values BCJ, in place on r (stride rs).  Work: 2n doubles. */
+void or_tdma_noncyc(int n, const double *a, const double *b, const double *c, double *r, long rs,
+                    double *work)
+{
+    double *H = work, *G = work + n;
+    H[0] = c[0] / b[0];
+    G[0] = r[0] / b[0];
+    for (int j = 1; j <= n - 2; ++j) {
+        double pp = 1.0 / (b[j] - H[j - 1] * a[j]);
+        H[j] = c[j] * pp;
+        G[j] = (r[j * rs] - a[j] * G[j - 1]) * pp;
+    }
+    {
+        int j = n - 1;
+        double pp = 1.0 / (b[j] - H[j - 1] * a[j]);
+        G[j] = (r[j * rs] - a[j] * G[j - 1]) * pp;
+        r[j * rs] = G[j];
+    }
+    for (int j = n - 2; j >= 0; --j) r[j * rs] = G[j] - H[j] * r[(j + 1) * rs];
+}
+
+/* MOMFA_tg (MOMFA_tg.f90:15-47): x (MOMFA1_X_tg :51-106) and z (MOMFA2_Z_tg :110-169) cyclic
+ * solves over rows J >= N2I, the y solve (MOMFA3_Y_tg :172-239) over all rows J = 1..M, then
+ * Q += RHS for rows J >= N2I. */
+void or_momfa(const or_mesh *m, or_state *s, int ns, int idr, double dt, double cvisc)
+{
+    const int c = idr - 1;
+    const int n2i = (idr == 2) ? 2 : 1;
+    const double facoe = 0.5 * m->talp[ns] * dt * cvisc;
+    int nmax = LL > NN ? LL : NN;
+    if (MM + 2 > nmax) nmax = MM + 2;
+    double *work = (double *)malloc(sizeof(double) * 3 * nmax);
+    double *a = (double *)malloc(sizeof(double) * (MM + 2) * 3);
+    double *b = a + MM + 2, *cc = b + MM + 2;
+    {   /* x: a = c = -FACOE*DXQI, b = 1 - a - c */
+        const double co = -facoe * m->dxqi;
+        const double bb = 1.0 - co - co;
+        for (int k = 0; k < NN; ++k)
+            for (int J = n2i; J <= MM; ++J) or_tdma_cyc(LL, co, bb, co, &CX(s->rhs, 0, J, k), 1, work);
+    }
+    {   /* z: a = c = -FACOE*DZQI (times RNDI2/RCCI2 = 1) */
+        const double co = 1.0 * (-facoe * m->dzqi);
+        const double bb = 1.0 - co - co;
+        for (int i = 0; i < LL; ++i)
+            for (int J = n2i; J <= MM; ++J)
+                or_tdma_cyc(NN, co, bb, co, &CX(s->rhs, i, J, 0), (long)MM * LL, work);
+    }
+    {   /* y: b = 1 - FACOE*ACVR, a = -FACOE*AMVR, c = -FACOE*APVR over J = 1..NCL2 */
+        for (int J = 1; J <= MM; ++J) {
+            b[J - 1] = 1.0 - facoe * m->acvr[c * (MM + 2) + J];
+            cc[J - 1] = -facoe * m->apvr[c * (MM + 2) + J];
+            a[J - 1] = -facoe * m->amvr[c * (MM + 2) + J];
+        }
+        for (int k = 0; k < NN; ++k)
+            for (int i = 0; i < LL; ++i) or_tdma_noncyc(MM, a, b, cc, &CX(s->rhs, i, 1, k), LL, work);
+    }
+    for (int k = 0; k < NN; ++k)
+        for (int J = n2i; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) QC(c, i, J, k) = CX(s->rhs, i, J, k) + QC(c, i, J, k);
+    free(work);
+    free(a);
+}
+
+/* BC_WALL_Q_tg (BC_WALL.f90:212-255, :646-657): ghost u, w = 0; v = 0 on both walls. */
+void or_bc_q(const or_mesh *m, or_state *s)
+{
+    for (int k = 0; k < NN; ++k)
+        for (int i = 0; i < LL; ++i) {
+            QC(0, i, MM + 1, k) = 0.0; QC(2, i, MM + 1, k) = 0.0; QC(1, i, MM + 1, k) = 0.0;
+            QC(0, i, 0, k) = 0.0; QC(2, i, 0, k) = 0.0; QC(1, i, 1, k) = 0.0; QC(1, i, 0, k) = 0.0;
+        }
+}
+
+/* BC_WALL_PR / BC_WALL_DPH (BC_WALL.f90:285-411): ghost rows copy the first/last cell. */
+void or_bc_copy(const or_mesh *m, double *a)
+{
+    for (int k = 0; k < NN; ++k)
+        for (int i = 0; i < LL; ++i) {
+            HX(a, i, MM + 1, k) = HX(a, i, MM, k);
+            HX(a, i, 0, k) = HX(a, i, 1, k);
+        }
+}
+
+/* DIVG_tg (DIVG.f90:15-51). */
+void or_divg(const or_mesh *m, or_state *s, int ns, double dt)
+{
+    const double coe0 = 1.0 / (dt * m->talp[ns]);
+    for (int J = 1; J <= MM; ++J) {
+        const double coe1 = m->dxi, coe2 = m->dyfi[J - 1];
+        for (int k = 0; k < NN; ++k) {
+            const int kp = kp_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                double d = (QC(0, ip_(m, i), J, k) - QC(0, i, J, k)) * coe1 +
+                           (QC(1, i, J + 1, k) - QC(1, i, J, k)) * coe2 +
+                           (QC(2, i, J, kp) - QC(2, i, J, k)) * m->dzi;
+                CX(s->rhsllphi, i, J, k) = d * coe0;
+            }
+        }
+    }
+}
+
+/* Complex DFT by mixed-radix decimation in time; sign = -1 forward, +1 backward,
+ * unnormalised: X[k] = sum_j x[j] exp(sign * 2 pi i j k / n).  Twiddles are evaluated
+ * directly with cos/sin of the reduced angle, so no recurrence error accumulates. */
+static void dft_rec(int n, int sign, const double *xr, const double *xi, int stride,
+                    double *yr, double *yi, double *tr, double *ti)
+{
+    if (n == 1) { yr[0] = xr[0]; yi[0] = xi[0]; return; }
+    int p = 2;
+    while (n % p) ++p;
+    const int mlen = n / p;
+    for (int r = 0; r < p; ++r)
+        dft_rec(mlen, sign, xr + (size_t)r * stride, xi + (size_t)r * stride, stride * p,
+                tr + (size_t)r * mlen, ti + (size_t)r * mlen, yr + (size_t)r * mlen, yi + (size_t)r * mlen);
+    const double tp = 2.0 * 3.14159265358979323846 / n;
+    for (int k = 0; k < n; ++k) {
+        double sr = 0.0, si = 0.0;
+        for (int r = 0; r < p; ++r) {
+            long e = ((long)r * k) % n;
+            double cr = cos(tp * e), ci = sign * sin(tp * e);
+            double ar = tr[(size_t)r * mlen + k % mlen], ai = ti[(size_t)r * mlen + k % mlen];
+            sr += ar * cr - ai * ci;
+            si += ar * ci + ai * cr;
+        }
+        yr[k] = sr; yi[k] = si;
+    }
+}
+
+void or_dft(int n, int sign, const double *xr, const double *xi, double *yr, double *yi)
+{
+    double *t = (double *)malloc(sizeof(double) * 4 * n);
+    double *ar = (double *)malloc(sizeof(double) * 2 * n);
+    memcpy(ar, xr, sizeof(double) * n);
+    memcpy(ar + n, xi, sizeof(double) * n);
+    dft_rec(n, sign, ar, ar + n, 1, yr, yi, t, t + n);
+    (void)t;
+    free(ar);
+    free(t);
+}
+
+/* FFT99_POIS3D_periodicxz (possion3d_FFT99.f90:271-505), channel (non-cyclic y):
+ * x forward real transform normalised by 1/L (FFT99 ISIGN=-1, FFT99.f90:95-103), z forward
+ * complex transform times 1/N (CFFT99 ISIGN=-1, CFFT99.f90:171-200), (kx,kz)=(0,0) gauge:
+ * RHS(0,0,J=1) = 0 and row 1 decoupled (possion3d_FFT99.f90:224-231, :341-344), per-mode
+ * TDMAIJJ_nonCYC in y for the real and imaginary parts, then the inverse transforms
+ * (the imaginary parts of kx = 0 and kx = L/2 are not used, FFT99.f90:84-86). */
+void or_poisson(const or_mesh *m, const double *rhs_cells, double *dph_halo)
+{
+    const int L = LL, M = MM, N = NN, H = L / 2 + 1;
+    double *sr = (double *)calloc((size_t)H * M * N, sizeof(double));   /* [k][J-1][kx] */
+    double *si = (double *)calloc((size_t)H * M * N, sizeof(double));
+    int nmax = L > N ? L : N;
+    double *ar = (double *)malloc(sizeof(double) * nmax), *ai = (double *)malloc(sizeof(double) * nmax);
+    double *br = (double *)malloc(sizeof(double) * nmax), *bi = (double *)malloc(sizeof(double) * nmax);
+#define SP(a, kx, J, k) (a)[((size_t)(k) * M + (J) - 1) * H + (kx)]
+    for (int J = 1; J <= M; ++J) {
+        for (int k = 0; k < N; ++k) {
+            for (int i = 0; i < L; ++i) { ar[i] = rhs_cells[((size_t)k * M + J - 1) * L + i]; ai[i] = 0.0; }
+            or_dft(L, -1, ar, ai, br, bi);
+            for (int kx = 0; kx < H; ++kx) { SP(sr, kx, J, k) = br[kx] / L; SP(si, kx, J, k) = bi[kx] / L; }
+        }
+        for (int kx = 0; kx < H; ++kx) {
+            for (int k = 0; k < N; ++k) { ar[k] = SP(sr, kx, J, k); ai[k] = SP(si, kx, J, k); }
+            or_dft(N, -1, ar, ai, br, bi);
+            for (int k = 0; k < N; ++k) { SP(sr, kx, J, k) = br[k] * (1.0 / N); SP(si, kx, J, k) = bi[k] * (1.0 / N); }
+        }
+    }
+    SP(sr, 0, 1, 0) = 0.0;
+    SP(si, 0, 1, 0) = 0.0;
+    double *a = (double *)malloc(sizeof(double) * M * 3), *b = a + M, *c = b + M;
+    double *work = (double *)malloc(sizeof(double) * 2 * M);
+    for (int k = 0; k < N; ++k)
+        for (int kx = 0; kx < H; ++kx) {
+            for (int J = 1; J <= M; ++J) {
+                a[J - 1] = m->amph[J - 1];
+                c[J - 1] = m->apph[J - 1];
+                b[J - 1] = m->acph[J - 1] + (-m->ak1[kx] / 1.0 - m->ak3[k]);
+            }
+            if (kx == 0 && k == 0) { a[0] = 0.0; c[0] = 0.0; b[0] = m->acph[0] + (-m->ak1[0] / 1.0 - m->ak3[0]); }
+            or_tdma_noncyc(M, a, b, c, &SP(sr, kx, 1, k), H, work);
+            or_tdma_noncyc(M, a, b, c, &SP(si, kx, 1, k), H, work);
+        }
+    for (int J = 1; J <= M; ++J) {
+        for (int kx = 0; kx < H; ++kx) {
+            for (int k = 0; k < N; ++k) { ar[k] = SP(sr, kx, J, k); ai[k] = SP(si, kx, J, k); }
+            or_dft(N, +1, ar, ai, br, bi);
+            for (int k = 0; k < N; ++k) { SP(sr, kx, J, k) = br[k]; SP(si, kx, J, k) = bi[k]; }
+        }
+        for (int k = 0; k < N; ++k) {
+            for (int kx = 0; kx < H; ++kx) { ar[kx] = SP(sr, kx, J, k); ai[kx] = SP(si, kx, J, k); }
+            ai[0] = 0.0;
+            ai[L / 2] = 0.0;
+            for (int kx = H; kx < L; ++kx) { ar[kx] = ar[L - kx]; ai[kx] = -ai[L - kx]; }
+            or_dft(L, +1, ar, ai, br, bi);
+            for (int i = 0; i < L; ++i) dph_halo[((size_t)k * (M + 2) + J) * L + i] = br[i];
+        }
+    }
+#undef SP
+    free(sr); free(si); free(ar); free(ai); free(br); free(bi); free(a); free(work);
+}
+
+/* VELOUPDT_tg (VELO_CORRECTION.f90:14-100). */
+void or_veloupdt(const or_mesh *m, or_state *s, int ns, double dt)
+{
+    double coe1 = dt * m->talp[ns] * m->dxi;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i)
+                QC(0, i, J, k) = QC(0, i, J, k) - (HX(s->dph, i, J, k) - HX(s->dph, im_(m, i), J, k)) * coe1;
+    coe1 = dt * m->talp[ns] * m->dzi;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i)
+                QC(2, i, J, k) = QC(2, i, J, k) - (HX(s->dph, i, J, k) - HX(s->dph, i, J, km_(m, k))) * coe1;
+    for (int J = 2; J <= MM; ++J) {
+        coe1 = dt * m->talp[ns] * m->dyci[J];
+        for (int k = 0; k < NN; ++k)
+            for (int i = 0; i < LL; ++i)
+                QC(1, i, J, k) = QC(1, i, J, k) - (HX(s->dph, i, J, k) - HX(s->dph, i, J - 1, k)) * coe1;
+    }
+}
+
+/* PRCALC_tg (PRESSURE_CORRECTION.f90:15-36): p += phi - (alpha dt / 2Re) RHSLLPHI. */
+void or_prcalc(const or_mesh *m, or_state *s, int ns, double dt, double cvisc)
+{
+    const double be = 0.5 * m->talp[ns] * dt * cvisc;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i)
+                HX(s->pr, i, J, k) = HX(s->pr, i, J, k) + HX(s->dph, i, J, k) - be * CX(s->rhsllphi, i, J, k);
+}
+
+/* SOLVERRK3_MOM_tg (SOLVERRK3_MOM_tg.f90:13-56), one substep NS in {0,1,2,3}. */
+void or_substep(const or_mesh *m, or_state *s, int ns, double dt, double cvisc)
+{
+    or_convection(m, s);
+    for (int idr = 1; idr <= 3; ++idr) {
+        or_rhs(m, s, ns, idr, dt, cvisc);
+        or_momfa(m, s, ns, idr, dt, cvisc);
+    }
+    or_bc_q(m, s);
+    or_divg(m, s, ns, dt);
+    memset(s->dph, 0, sizeof(double) * (size_t)NN * (MM + 2) * LL);
+    or_poisson(m, s->rhsllphi, s->dph);
+    or_bc_copy(m, s->dph);
+    or_veloupdt(m, s, ns, dt);
+    or_bc_q(m, s);
+    or_prcalc(m, s, ns, dt, cvisc);
+    or_bc_copy(m, s->pr);
+}

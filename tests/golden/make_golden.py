@@ -1,0 +1,71 @@
+"""Writes the golden fixtures in tests/golden/ from the compiled reference (oracle/_ref,
+built by ``make -f oracle/ref.mk`` from the reference's Fortran sources).
+
+Each ``tg_<L>x<M>x<N>.npz`` holds, for one channel mesh:
+  setup arrays as dumped by the reference (prefix ``s_``),
+  the seeded initial field (``q_init``, ``pr_init``; oracle.initial_field),
+  the reference convection terms of that field (``conv``),
+  the field after the NS=0 projection substep (``q_ns0``, ``pr_ns0``),
+  the field after one RK3 step (NS=1,2,3) from there (``q_step1``, ``pr_step1``),
+  a reference Poisson solve of a seeded compatible RHS (``poisson_rhs``, ``poisson_phi``).
+Arrays are C-order views of the reference's Fortran layout (z, y, x).
+
+Run: python tests/golden/make_golden.py   (needs oracle/_ref built; about a minute)
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle, refrun, setup  # noqa: E402
+
+CASES = [
+    # L, M, N, STR2, REN, dt, seed
+    (16, 16, 16, 1.8, 2800.0, 1.0e-3, 11),
+    (20, 18, 30, 2.2, 6875.0, 5.0e-4, 12),
+]
+
+
+def compatible_rhs(mesh, seed):
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    r = rng.standard_normal((mesh.N, mesh.M, mesh.L))
+    w = (1.0 / mesh.DYFI)[None, :, None]
+    r -= np.sum(r * w) / (np.sum(w) * mesh.L * mesh.N)
+    return r
+
+
+def make_case(L, M, N, STR2, REN, dt, seed):
+    h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt)
+    try:
+        s = h.setup()
+        mesh = setup.Mesh(L, M, N, STR2=STR2)
+        q, pr = oracle.initial_field(mesh, seed)
+        h.write_fields("init.bin", q[0], q[1], q[2], pr)
+        rhs = compatible_rhs(mesh, seed + 1000)
+        h.write_cells("prhs.bin", rhs)
+        h.run(["load init.bin", "conv c0", "substep 0 %r" % dt, "dump s0",
+               "substep 1 %r" % dt, "substep 2 %r" % dt, "substep 3 %r" % dt, "dump s1",
+               "poisson prhs.bin phi"])
+        out = {f"s_{k}": np.asarray(v) for k, v in s.items()}
+        out.update(q_init=q, pr_init=pr, conv=h.read_cells("c0", 3),
+                   poisson_rhs=rhs, poisson_phi=h.read_cells("phi"),
+                   params=np.array([L, M, N, STR2, REN, dt, seed], dtype=np.float64))
+        u, v, w, p = h.read_q("s0")
+        out.update(q_ns0=np.stack([u, v, w]), pr_ns0=p)
+        u, v, w, p = h.read_q("s1")
+        out.update(q_step1=np.stack([u, v, w]), pr_step1=p)
+    finally:
+        h.close()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"tg_{L}x{M}x{N}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    for case in CASES:
+        make_case(*case)
