@@ -1,0 +1,285 @@
+"""Benchmark of the RK3/Crank-Nicolson channel time step on one MI355X.
+
+Workload: single periodic channel, flow only, Re_tau = 395 (REN = 6875),
+grid 256 x 192 x 256 (x, y, z), box 4 pi x 2 x 2 pi, y tanh-stretched with beta = 2.0,
+dt = 5e-4.  Synthetic start: Poiseuille + 5 % seeded perturbation, projected by the NS = 0
+substep.  A "step" is one full RK3 step (three substeps SOLVERRK3_MOM_tg(1..3)) of the whole
+grid with all fields resident in HBM.
+
+Prints one JSON line.  ``value`` is grid-point RK3-steps/s of the form named in
+config.form (plain FP64 unless --form mixed).  ``roofline`` is the dominant kernel's
+algorithmic HBM bytes per launch over its average launch time, measured with HIP events on
+the solver's stream inside the timed region.  ``cpu_baseline`` is the reference Fortran/MPI
+program (oracle/_ref/chapsim, built from the reference sources) timed on this host's cores
+on a bounded sample of the same workload (a few steps), or the C oracle on one core if the
+reference build is absent.
+
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--form plain|mixed]
+                       [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+CFG = dict(L=256, M=192, N=256, HX=4 * np.pi, HZ=2 * np.pi, STR2=2.0, REN=6875.0, dt=5.0e-4)
+
+
+def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
+    """Algorithmic HBM bytes of one launch: every array element the kernel must read or
+    write, counted once.  F = L*M*N cells; fields u,v,w,p,phi,div are FP64; RHS/history and
+    spectral arrays use the work precision (wbytes)."""
+    F = float(L * M * N)
+    S = float((L // 2 + 1) * M * N)          # complex spectral values
+    table = {
+        # R u,v,w,p (4F, FP64) + H0 x3; W RHS x3 + H0 x3
+        "conv_rhs": 4 * F * 8 + 3 * F * wbytes + 6 * F * wbytes,
+        # R RHS x3, W RHS x3 (v skips row 1)
+        "cn_x": 6 * F * wbytes,
+        "cn_z": 6 * F * wbytes,
+        # R RHS x3, R+W q x3
+        "cn_y": 3 * F * wbytes + 6 * F * 8,
+        # R u,v,w; W div (FP64) + spectrum
+        "divg_fftx": 3 * F * 8 + F * 8 + S * 2 * wbytes,
+        "fftz_fwd": 2 * S * 2 * wbytes,
+        # R spectrum + pivots (FP64), W spectrum
+        "ytdma": 2 * S * 2 * wbytes + S * 8,
+        "fftz_inv": 2 * S * 2 * wbytes,
+        # R spectrum, W phi
+        "fftx_inv": S * 2 * wbytes + F * 8,
+        # R phi, div, u, v, w, p; W u, v, w, p
+        "update": 10 * F * 8,
+    }
+    return table[kernel]
+
+
+KERNELS = ["conv_rhs", "cn_x", "cn_z", "cn_y", "divg_fftx", "fftz_fwd", "ytdma", "fftz_inv",
+           "fftx_inv", "update"]
+
+
+def load_pkg():
+    import __graft_entry__
+    return __graft_entry__.load()
+
+
+def run_gpu(pkg, form: str, steps: int, warmup: int):
+    c = CFG
+    coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
+    mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
+    ch = pkg.ChannelTG(coeffs, mode)
+    q, pr = pkg.poiseuille_perturbed(coeffs, seed=395, amp=0.05)
+    ch.upload(q, pr)
+    cv = 1.0 / c["REN"]
+    dt = c["dt"]
+    ch.SOLVERRK3_MOM_tg(0, dt, cv)
+    q0, p0 = ch.download()          # projected start, shared with the CPU baseline
+    # profiling pre-pass: which kernel dominates a step
+    ch.profile_reset(None)
+    for _ in range(2):
+        ch.rk3_step(dt, cv)
+    ch.sync()
+    per = {}
+    for k in KERNELS:
+        avg, n = ch.profile_get(k)
+        per[k] = avg * n / 2.0          # ms per step
+    dom = max(per, key=per.get)
+    for _ in range(warmup):
+        ch.rk3_step(dt, cv)
+    # timed region; events only around the dominant kernel
+    ch.profile_reset(dom)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    ch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ch.rk3_step(dt, cv)
+    ch.sync()
+    t1 = time.perf_counter()
+    dom_avg_ms, dom_n = ch.profile_get(dom)
+    cfl = ch.cfl(dt)
+    div = ch.max_div()
+    ch.close()
+    ms = (t1 - t0) * 1e3 / steps
+    return dict(ms_per_step=ms, per_kernel_ms=per, dominant=dom, dom_avg_ms=dom_avg_ms,
+                dom_count=dom_n, cfl=cfl, max_div=div, wbytes=4 if form == "mixed" else 8,
+                start=(q0, p0))
+
+
+def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = None):
+    """Time the reference program on this host: restart from the projected start field
+    (the reference checks max div < 1e-6 on restart, FLOWSTART.f90:41-46) written in its
+    restart format (processing/WRT_VARS_TG.f90:12-104), `sample_steps` steps, median of the
+    per-step CpuT monitor column (processing/PP_SCN_MONITOR.f90:236, RK loop only, max over
+    ranks)."""
+    from oracle import refrun
+    if not os.path.exists(refrun.PROGRAM) or not os.path.exists(refrun.MPIRUN):
+        return None
+    c = CFG
+    L, M, N = c["L"], c["M"], c["N"]
+    ncpu = len(os.sched_getaffinity(0))
+    P = cores or max(2, min(16, ncpu))
+    P = min(P, M, N)
+    work = tempfile.mkdtemp(prefix="chs_cpu_")
+    try:
+        ini = refrun.ini_text(L, M, N, HX=c["HX"], HZ=c["HZ"], STR2=c["STR2"], REN=c["REN"],
+                              DT=c["dt"], NTSTF=sample_steps, iIniField_tg=2, TimeReStart_tg=0.0)
+        with open(os.path.join(work, "readdata.ini"), "w") as fh:
+            fh.write(ini)
+        os.makedirs(os.path.join(work, "1_instant_rawdata"))
+        q, pr = start
+        hdr_i = np.array([L, M, N, 0], dtype=np.int32)
+        hdr_r = np.array([0.0, c["REN"], c["dt"]], dtype=np.float64)
+        for name, arr in (("U", q[0]), ("V", q[1]), ("W", q[2]), ("P", pr)):
+            fn = os.path.join(work, "1_instant_rawdata", f"DNS_perixz_INSTANT_T{0.0:.9E}_{name}.D")
+            with open(fn, "wb") as fh:
+                fh.write(hdr_i.tobytes())
+                fh.write(hdr_r.tobytes())
+                fh.write(np.ascontiguousarray(arr[:, 1:M + 1, :]).tobytes())
+        cmd = f"ulimit -s unlimited 2>/dev/null || ulimit -s $(ulimit -H -s); " \
+              f"{refrun.MPIRUN} -np {P} {refrun.PROGRAM}"
+        t0 = time.perf_counter()
+        res = subprocess.run(["bash", "-c", cmd], cwd=work, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        logs = [f for f in os.listdir(os.path.join(work, "0_log_monitors")) if f.startswith("monitor.tg")]
+        if res.returncode != 0 or not logs:
+            return dict(error=f"reference run failed rc={res.returncode}: {res.stderr[-400:]}")
+        cput = []
+        with open(os.path.join(work, "0_log_monitors", logs[0])) as fh:
+            for line in fh:
+                if line.lstrip().startswith("#") or not line.strip():
+                    continue
+                parts = line.split()
+                if re.fullmatch(r"\d+", parts[0]):
+                    cput.append(float(parts[2]))
+        if not cput:
+            return dict(error="no monitor lines")
+        s = float(np.median(cput))
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        cpu_model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        return dict(value=L * M * N / s, unit="grid-point RK3-steps/s", cores=P, kind="reference",
+                    sample=f"reference CHAPSim (oracle/_ref/chapsim, amdflang -O2) mpirun -np {P} "
+                           f"on {cpu_model or 'host'}; {L}x{M}x{N}, {len(cput)} RK3 steps from a "
+                           f"restart field; median per-step CpuT {s:.3f} s; run wall {wall:.1f} s")
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def cpu_baseline_port():
+    """C oracle (one core) on one substep of the same grid."""
+    from oracle import oracle, setup
+    c = CFG
+    mesh = setup.Mesh(c["L"], c["M"], c["N"], HX=c["HX"], HZ=c["HZ"], STR2=c["STR2"])
+    q, pr = oracle.initial_field(mesh, 395, 0.05)
+    o = oracle.OracleTG(mesh, c["REN"])
+    o.set_fields(q, pr)
+    t0 = time.perf_counter()
+    o.substep(1, c["dt"])
+    s = 3 * (time.perf_counter() - t0)
+    return dict(value=c["L"] * c["M"] * c["N"] / s, unit="grid-point RK3-steps/s", cores=1,
+                kind="port", sample=f"C oracle, one substep at {c['L']}x{c['M']}x{c['N']}, x3")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device copy (calibrates HBM byte counters)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": x.numel() * 8, "launches": 3}))
+        return
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the workload is one channel on one "
+              "MI355X); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    pkg = load_pkg()
+    r = run_gpu(pkg, args.form, args.steps, args.warmup)
+    c = CFG
+    F = c["L"] * c["M"] * c["N"]
+    value = F / (r["ms_per_step"] * 1e-3)
+    dom = r["dominant"]
+    b = alg_bytes(dom, c["L"], c["M"], c["N"], r["wbytes"])
+    achieved = b / (r["dom_avg_ms"] * 1e-3) / 1e9 if r["dom_avg_ms"] > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(f"{args.form}:{dom}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline_reference(r["start"])
+        except Exception as e:  # pragma: no cover - host-dependent
+            cpu = dict(error=str(e))
+        if cpu is None or "error" in cpu:
+            err = cpu.get("error") if cpu else "reference build absent"
+            cpu = cpu_baseline_port()
+            cpu["note"] = f"reference run unavailable: {err}"
+    step_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) for k in KERNELS) * 3
+    out = {
+        "metric": "grid-point RK3-steps/s, mixed vs plain FP64 at stated error; % of HBM peak",
+        "value": value,
+        "unit": "grid-point RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.form == "plain" else "f32+f64",
+        "data": "synthetic: Poiseuille + 5% seeded (Philox) perturbation, projected by NS=0",
+        "config": {"workload": "periodic channel Re_tau=395 (REN=6875), 256x192x256, "
+                               "4pi x 2 x 2pi, tanh beta=2.0, dt=5e-4, RK3+CN, flow only",
+                   "form": "plain_fp64" if args.form == "plain" else "mixed",
+                   "grid": [c["L"], c["M"], c["N"]]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": b, "avg_launch_ms": r["dom_avg_ms"],
+                     "launches": r["dom_count"]},
+        "step_roofline": {"alg_bytes_per_step": step_bytes,
+                          "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9,
+                          "frac": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "per_kernel_ms_per_step": r["per_kernel_ms"],
+        "checks": {"cfl_dt": r["cfl"], "max_div": r["max_div"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1353 @@
+// libchapsim_hip: MI355X (gfx950) implementation of the CHAPSim 1.0 fractional-step
+// RK3 / Crank-Nicolson substep for the periodic plane channel (SOLVERRK3_MOM_tg,
+// SOLVERRK3_MOM_tg.f90:13-56) behind the C-ABI in include/chapsim_hip.h.
+//
+// Device layout (one channel): x fastest, then y, then z (the reference's order,
+// MEMALLOCT.f90:96-103).  Halo fields (velocity, pressure, phi) keep the y-ghost rows
+// J = 0 and J = M+1; cell fields (RHS, history, divergence) keep J = 1..M.  Spectral data
+// after the x- and z-transforms is complex [kz][J][kx], kx = 0..L/2.
+//
+// Per substep (kernels in launch order):
+//   k_conv_rhs      convection (CONVECTION_X/Y/Z_tg) + viscous + pressure gradient + RK
+//                   combine (RHS_CvLpGpS_tg) for u, v, w at once; FP64 block partial sums
+//                   of the u-RHS for the constant-mass-flux correction
+//   k_reduce        deterministic FP64 sum of the partials -> DPGRNS
+//   k_cn_x, k_cn_z  Crank-Nicolson x and z solves (MOMFA1_X_tg, MOMFA2_Z_tg) applied as
+//                   the exact inverse of the constant-coefficient cyclic operator
+//   k_cn_y          y solve (MOMFA3_Y_tg) with the exact banded inverse, Q += dQ, wall BCs
+//   k_divg_fftx     divergence (DIVG_tg) + forward real x-FFT of the Poisson RHS
+//   k_fftz          forward / inverse complex z-FFT
+//   k_ytdma         per-wavenumber y-Thomas (TDMAIJJ_nonCYC in FFT99_POIS3D_periodicxz)
+//   k_fftx_inv      inverse real x-FFT -> phi, phi ghost rows (BC_WALL_DPH_tg)
+//   k_update        VELOUPDT_tg + PRCALC_tg + wall BCs
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/chapsim_hip.h"
+#include "chs_fft.h"
+
+using chs::cpx;
+
+#define CHS_VERSION "chapsim_hip 0.1 (gfx950)"
+
+namespace {
+
+// ---------------------------------------------------------------------------------------
+// Device-side geometry passed by value to every kernel.
+struct Geo {
+    int L, M, N, H;              // H = L/2 + 1
+    double dxi, dzi, dxqi, dzqi;
+    const double *dyfi;          // [M]     DYFI(J) at J-1
+    const double *dyci;          // [M+2]   DYCI(J) at J
+    const double *wff, *wfb;     // [M+2]   at J
+    const double *amvr, *acvr, *apvr;   // [3][M+2] at [c][J]
+};
+
+__device__ __forceinline__ size_t hix(const Geo &g, int i, int J, int k)
+{
+    return ((size_t)k * (g.M + 2) + J) * g.L + i;
+}
+__device__ __forceinline__ size_t cix(const Geo &g, int i, int J, int k)
+{
+    return ((size_t)k * g.M + (J - 1)) * g.L + i;
+}
+
+// RK scalars of one substep.
+struct Sub {
+    double tgam, troh, talp, dt, cvisc;
+};
+
+// ---------------------------------------------------------------------------------------
+// k_conv_rhs: one thread per cell (i, J, k).  Writes rhs[c] = (gamma H + zeta H0 + alpha nu
+// Lap q_c) dt - alpha dt grad_c p (RHS_CvLpGpS_tg.f90:95-160) and H0[c] = H; v only for
+// J >= 2 (NII = 2 on the bottom rank).  Also block partial sums of rhs_u / DYFI
+// (RHS_CvLpGpS_tg.f90:171-178) in FP64.
+template <typename W>
+__global__ __launch_bounds__(256) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
+                                                  const double *__restrict__ pr,
+                                                  W *__restrict__ h0, W *__restrict__ rhs,
+                                                  double *__restrict__ partial)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    const size_t hs = (size_t)g.N * (g.M + 2) * g.L;   // halo field size
+    const size_t cs = (size_t)g.N * g.M * g.L;         // cell field size
+    double mf = 0.0;
+    if (i < g.L && J <= g.M) {
+        const int L = g.L;
+        const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+        const int kp = (k == g.N - 1) ? 0 : k + 1, km = (k == 0) ? g.N - 1 : k - 1;
+        const double *u = q, *v = q + hs, *w = q + 2 * hs;
+        auto U = [&](int ii, int jj, int kk) { return (W)u[hix(g, ii, jj, kk)]; };
+        auto V = [&](int ii, int jj, int kk) { return (W)v[hix(g, ii, jj, kk)]; };
+        auto Wv = [&](int ii, int jj, int kk) { return (W)w[hix(g, ii, jj, kk)]; };
+        auto P = [&](int ii, int jj, int kk) { return (W)pr[hix(g, ii, jj, kk)]; };
+        const W h = W(0.5);
+        const W dxi = (W)g.dxi, dzi = (W)g.dzi, dxqi = (W)g.dxqi, dzqi = (W)g.dzqi;
+        const W wffJ = (W)g.wff[J], wfbJ = (W)g.wfb[J], wffP = (W)g.wff[J + 1], wfbP = (W)g.wfb[J + 1];
+        const W u0 = U(i, J, k), v0 = V(i, J, k), w0 = Wv(i, J, k);
+        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)s.dt;
+        const W coev = (W)(s.talp * s.cvisc), coep = (W)(s.talp * s.dt);
+        const size_t c0 = cix(g, i, J, k);
+        // ---- u (CONVECTION_X.f90:31-59) ----
+        {
+            const W coe1 = h * h * dxi, coe2 = (W)g.dyfi[J - 1] * h, coe3 = h * h * dzi;
+            const W uip = U(ip, J, k), uim = U(im, J, k);
+            const W H11 = ((uip + u0) * (uip + u0) - (uim + u0) * (uim + u0)) * coe1;
+            const W H12 = ((V(i, J + 1, k) + V(im, J + 1, k)) * (wffP * U(i, J + 1, k) + wfbP * u0) -
+                           (v0 + V(im, J, k)) * (wffJ * u0 + wfbJ * U(i, J - 1, k))) * coe2;
+            const W ukp = U(i, J, kp), ukm = U(i, J, km);
+            const W H13 = ((Wv(i, J, kp) + Wv(im, J, kp)) * (ukp + u0) - (w0 + Wv(im, J, k)) * (u0 + ukm)) * coe3;
+            const W conv = -(H11 + H12 + H13);
+            const W rhsc = tgam * conv + troh * h0[c0];
+            const W lap = (uip - W(2) * u0 + uim) * dxqi + (ukp - W(2) * u0 + ukm) * dzqi +
+                          (U(i, J + 1, k) * (W)g.apvr[J] + u0 * (W)g.acvr[J] + U(i, J - 1, k) * (W)g.amvr[J]);
+            h0[c0] = conv;
+            const W pgm = (P(i, J, k) - P(im, J, k)) * dxi * coep;
+            const W r = (rhsc + coev * lap) * dt - pgm;
+            rhs[c0] = r;
+            mf = (double)r / g.dyfi[J - 1];
+        }
+        // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 ----
+        if (J >= 2) {
+            const W coe1 = dxi * h, coe2 = (W)g.dyci[J] * h * h, coe31 = dzi * h;
+            const W vip = V(ip, J, k), vim = V(im, J, k), vjp = V(i, J + 1, k), vjm = V(i, J - 1, k);
+            const W vkp = V(i, J, kp), vkm = V(i, J, km);
+            const W H21 = ((wffJ * U(ip, J, k) + wfbJ * U(ip, J - 1, k)) * (vip + v0) -
+                           (wffJ * u0 + wfbJ * U(i, J - 1, k)) * (v0 + vim)) * coe1;
+            const W H22 = ((vjp + v0) * (vjp + v0) - (v0 + vjm) * (v0 + vjm)) * coe2;
+            const W H23 = ((wffJ * Wv(i, J, kp) + wfbJ * Wv(i, J - 1, kp)) * (vkp + v0) -
+                           (wffJ * w0 + wfbJ * Wv(i, J - 1, k)) * (v0 + vkm)) * coe31;
+            const W conv = -(H21 + H22 + H23);
+            const size_t cc = cs + c0;
+            const W rhsc = tgam * conv + troh * h0[cc];
+            const int o = g.M + 2;
+            const W lap = (vip - W(2) * v0 + vim) * dxqi + (vkp - W(2) * v0 + vkm) * dzqi +
+                          (vjp * (W)g.apvr[o + J] + v0 * (W)g.acvr[o + J] + vjm * (W)g.amvr[o + J]);
+            h0[cc] = conv;
+            const W pgm = (P(i, J, k) - P(i, J - 1, k)) * (W)g.dyci[J] * coep;
+            rhs[cc] = (rhsc + coev * lap) * dt - pgm;
+        }
+        // ---- w (CONVECTION_Z.f90:33-69) ----
+        {
+            const W coe1 = h * h * dxi, coe2 = (W)g.dyfi[J - 1] * h, coe3 = h * h * dzi;
+            const W wip = Wv(ip, J, k), wim = Wv(im, J, k), wkp = Wv(i, J, kp), wkm = Wv(i, J, km);
+            const W H31 = ((U(ip, J, k) + U(ip, J, km)) * (wip + w0) - (u0 + U(i, J, km)) * (w0 + wim)) * coe1;
+            const W H32 = ((V(i, J + 1, k) + V(i, J + 1, km)) * (wffP * Wv(i, J + 1, k) + wfbP * w0) -
+                           (v0 + V(i, J, km)) * (wffJ * w0 + wfbJ * Wv(i, J - 1, k))) * coe2;
+            const W H33 = ((wkp + w0) * (wkp + w0) - (w0 + wkm) * (w0 + wkm)) * coe3;
+            const W conv = -(H31 + H32 + H33);
+            const size_t cc = 2 * cs + c0;
+            const W rhsc = tgam * conv + troh * h0[cc];
+            const int o = 2 * (g.M + 2);
+            const W lap = (wip - W(2) * w0 + wim) * dxqi + (wkp - W(2) * w0 + wkm) * dzqi +
+                          (Wv(i, J + 1, k) * (W)g.apvr[o + J] + w0 * (W)g.acvr[o + J] +
+                           Wv(i, J - 1, k) * (W)g.amvr[o + J]);
+            h0[cc] = conv;
+            const W pgm = (P(i, J, k) - P(i, J, km)) * dzi * coep;
+            rhs[cc] = (rhsc + coev * lap) * dt - pgm;
+        }
+    }
+    // block reduction of the mass-flux partial (fixed order -> deterministic)
+    __shared__ double red[256];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    red[t] = mf;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Sum of n partials in a fixed order by one block; out[0] = sum * scale.
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int n, double scale,
+                                                double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double a = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) a += partial[t];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+// ---------------------------------------------------------------------------------------
+// Crank-Nicolson line solves as exact inverses.
+// x: (1 - beta dxx) on a periodic line is circulant; its inverse is circulant with first
+// column gx (computed on the host in long double).  Entries decay geometrically; D is the
+// half-width beyond which all entries are below 2^-64 of the diagonal (D = L/2 keeps all).
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, int D, int full,
+                                              W *__restrict__ rhs, const double *__restrict__ dpg,
+                                              int comp0)
+{
+    // one block per x-line set: blockIdx.x -> (J, k) line of component (comp0 + blockIdx.z)
+    extern __shared__ unsigned char smem_raw[];
+    W *line = reinterpret_cast<W *>(smem_raw);
+    const int c = comp0 + blockIdx.z;
+    const int J = blockIdx.x + 1, k = blockIdx.y;
+    if (c == 1 && J == 1) return;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
+    const size_t cs = (size_t)g.N * g.M * g.L;
+    W *r = rhs + c * cs + cix(g, 0, J, k);
+    const W sub = (c == 0) ? (W)dpg[0] : W(0);
+    for (int i = threadIdx.x; i < g.L; i += blockDim.x) line[i] = r[i] - sub;
+    __syncthreads();
+    const int L = g.L;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        W acc = W(0);
+        if (full) {
+            for (int l = 0; l < L; ++l) {
+                int d = i - l;
+                if (d < 0) d += L;
+                acc += gx[d] * line[l];
+            }
+        } else {
+            acc = gx[0] * line[i];
+            for (int d = 1; d <= D; ++d) {
+                int a = i + d, b = i - d;
+                if (a >= L) a -= L;
+                if (b < 0) b += L;
+                acc += gx[d] * (line[a] + line[b]);
+            }
+        }
+        r[i] = acc;
+    }
+}
+
+// z: same circulant inverse along k (stride (M)*L in the cell layout), one thread per (i, J).
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_z(Geo g, const W *__restrict__ gz, int D, int full,
+                                              const W *__restrict__ rin, W *__restrict__ rout,
+                                              int comp0)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    const int c = comp0;
+    if (i >= g.L || J > g.M) return;
+    if (c == 1 && J == 1) return;
+    const size_t cs = (size_t)g.N * g.M * g.L;
+    const W *r = rin + c * cs;
+    const int N = g.N;
+    W acc = W(0);
+    if (full) {
+        for (int l = 0; l < N; ++l) {
+            int d = k - l;
+            if (d < 0) d += N;
+            acc += gz[d] * r[cix(g, i, J, l)];
+        }
+    } else {
+        acc = gz[0] * r[cix(g, i, J, k)];
+        for (int d = 1; d <= D; ++d) {
+            int a = k + d, b = k - d;
+            if (a >= N) a -= N;
+            if (b < 0) b += N;
+            acc += gz[d] * (r[cix(g, i, J, a)] + r[cix(g, i, J, b)]);
+        }
+    }
+    rout[c * cs + cix(g, i, J, k)] = acc;
+}
+
+// y: banded exact inverse by[J][d], d = -D..D (row J = 1..M), Q_c += dQ for rows J >= N2I,
+// then the wall BCs of BC_WALL_Q_tg (BC_WALL.f90:229-251).  For u the increment of row
+// J = 1 is also stored into row 1 of v's RHS, which the reference's shared RHS_tg array
+// carries into v's y-solve (RHS_CvLpGpS_tg.f90:45-46 leaves that row, MOMFA3_Y_tg solves it).
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, int D,
+                                              const W *__restrict__ rin, W *__restrict__ vrow1,
+                                              double *__restrict__ q, int c)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= g.L) return;
+    const size_t cs = (size_t)g.N * g.M * g.L;
+    const size_t hs = (size_t)g.N * (g.M + 2) * g.L;
+    const W *r = rin + c * cs;
+    double *qc = q + c * hs;
+    const int M = g.M, w = 2 * D + 1;
+    const int n2i = (c == 1) ? 2 : 1;
+    for (int J = n2i; J <= M; ++J) {
+        const int lo = J - D < 1 ? 1 : J - D, hi = J + D > M ? M : J + D;
+        W acc = W(0);
+        for (int l = lo; l <= hi; ++l) acc += by[(size_t)(J - 1) * w + (l - J + D)] * r[cix(g, i, l, k)];
+        qc[hix(g, i, J, k)] += (double)acc;
+        if (c == 0 && J == 1 && vrow1) vrow1[cix(g, i, 1, k)] = acc;
+    }
+    // BC_WALL_Q (bottom and top walls)
+    qc[hix(g, i, M + 1, k)] = 0.0;
+    qc[hix(g, i, 0, k)] = 0.0;
+    if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Poisson.  k_divg_fftx: per block `nlc` pairs of x-lines.  Divergence (DIVG_tg,
+// DIVG.f90:31-46) of the two real lines of a pair goes to div[] (kept for PRCALC_tg) and
+// into one complex line; after the FFT the two half spectra are separated and scaled by
+// 1/(L N) (the 1/L of FFT99 and the 1/N applied after CFFT99).
+struct PoisArgs {
+    chs::FftPlan px, pz;
+    const void *twx, *twz;   // cpx<T>[L], cpx<T>[N]
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_divg_fftx(Geo g, PoisArgs pa, double coe0, const double *__restrict__ q,
+                                                   double *__restrict__ div, cpx<T> *__restrict__ spec,
+                                                   int nlc, const double *__restrict__ src)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int L = g.L, M = g.M, H = g.H;
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem_raw);
+    cpx<T> *a = tw + L;
+    cpx<T> *b = a + nlc * L;
+    const size_t hs = (size_t)g.N * (M + 2) * L;
+    const double *u = q, *v = q + hs, *w = q + 2 * hs;
+    const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twx);
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const int nlines = M * g.N;   // real lines (J, k)
+    const int pair0 = blockIdx.x * nlc;
+    for (int t = threadIdx.x; t < nlc * L; t += blockDim.x) {
+        const int p = t / L, i = t - p * L;
+        T val[2] = {T(0), T(0)};
+        for (int h = 0; h < 2; ++h) {
+            const int ln = 2 * (pair0 + p) + h;
+            if (ln < nlines && src) {
+                const int k = ln / M, J = ln - k * M + 1;
+                val[h] = (T)src[cix(g, i, J, k)];
+            } else if (ln < nlines) {
+                const int k = ln / M, J = ln - k * M + 1;
+                const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
+                const double d = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
+                                 (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * g.dyfi[J - 1] +
+                                 (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
+                const double dv = d * coe0;
+                div[cix(g, i, J, k)] = dv;
+                val[h] = (T)dv;
+            }
+        }
+        a[p * L + i] = {val[0], val[1]};
+    }
+    __syncthreads();
+    cpx<T> *res = chs::fft_lds<T>(a, b, L, nlc, pa.px, tw, -1, threadIdx.x, blockDim.x);
+    const T scale = T(1.0 / ((double)L * (double)g.N));
+    for (int t = threadIdx.x; t < nlc * H; t += blockDim.x) {
+        const int p = t / H, kx = t - p * H;
+        const cpx<T> Z = res[p * L + kx];
+        const cpx<T> Zc = res[p * L + ((L - kx) % L)];
+        // A = (Z + conj Zc)/2, B = (Z - conj Zc)/(2i)
+        const cpx<T> A = {(Z.x + Zc.x) * T(0.5) * scale, (Z.y - Zc.y) * T(0.5) * scale};
+        const cpx<T> B = {(Z.y + Zc.y) * T(0.5) * scale, (Zc.x - Z.x) * T(0.5) * scale};
+        for (int h = 0; h < 2; ++h) {
+            const int ln = 2 * (pair0 + p) + h;
+            if (ln < nlines) {
+                const int k = ln / M, J = ln - k * M + 1;
+                spec[((size_t)k * M + (J - 1)) * H + kx] = h == 0 ? A : B;
+            }
+        }
+    }
+}
+
+// Complex z-transform of TB consecutive kx columns at one J (both directions).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__restrict__ spec, int TB, int sign)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int N = g.N, M = g.M, H = g.H;
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem_raw);
+    cpx<T> *a = tw + N;
+    cpx<T> *b = a + TB * N;
+    const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twz);
+    for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
+    const int kx0 = blockIdx.x * TB, J = blockIdx.y + 1;
+    for (int t = threadIdx.x; t < TB * N; t += blockDim.x) {
+        const int kz = t / TB, c = t - kz * TB;
+        const int kx = kx0 + c;
+        a[c * N + kz] = kx < H ? spec[((size_t)kz * M + (J - 1)) * H + kx] : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    cpx<T> *res = chs::fft_lds<T>(a, b, N, TB, pa.pz, tw, sign, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < TB * N; t += blockDim.x) {
+        const int kz = t / TB, c = t - kz * TB;
+        const int kx = kx0 + c;
+        if (kx < H) spec[((size_t)kz * M + (J - 1)) * H + kx] = res[c * N + kz];
+    }
+}
+
+// Per-mode y-Thomas (TDMAIJJ_nonCYC.f90:84-120 on the FFT99 coefficients,
+// possion3d_FFT99.f90:213-231): pp[kz][J][kx] = 1/(b_J - h_{J-1} a_J) precomputed in FP64,
+// h_J = c_J pp_J, and the (kx,kz) = (0,0) mode has RHS(J=1) = 0 and row 1 decoupled.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__ amph, const double *__restrict__ apph,
+                                               const double *__restrict__ pp, cpx<T> *__restrict__ spec)
+{
+    const int kx = blockIdx.x * 256 + threadIdx.x;
+    const int kz = blockIdx.y;
+    if (kx >= g.H) return;
+    const int M = g.M, H = g.H;
+    const bool zero = (kx == 0 && kz == 0);
+    cpx<T> *s = spec + (size_t)kz * M * H + kx;
+    const double *p = pp + (size_t)kz * M * H + kx;
+    // forward: G_J = (R_J - a_J G_{J-1}) pp_J
+    double gr = 0.0, gi = 0.0;
+    for (int J = 1; J <= M; ++J) {
+        cpx<T> r = s[(size_t)(J - 1) * H];
+        double rr = (double)r.x, ri = (double)r.y;
+        if (zero && J == 1) { rr = 0.0; ri = 0.0; }
+        const double a = (J == 1) ? 0.0 : amph[J - 1];
+        const double pj = p[(size_t)(J - 1) * H];
+        gr = (rr - a * gr) * pj;
+        gi = (ri - a * gi) * pj;
+        s[(size_t)(J - 1) * H] = {(T)gr, (T)gi};
+    }
+    // backward: X_J = G_J - h_J X_{J+1}
+    double xr = gr, xi = gi;
+    for (int J = M - 1; J >= 1; --J) {
+        const double c = (zero && J == 1) ? 0.0 : apph[J - 1];
+        const double h = c * p[(size_t)(J - 1) * H];
+        cpx<T> gg = s[(size_t)(J - 1) * H];
+        xr = (double)gg.x - h * xr;
+        xi = (double)gg.y - h * xi;
+        s[(size_t)(J - 1) * H] = {(T)xr, (T)xi};
+    }
+}
+
+// Inverse real x-transform of pairs of lines -> phi (halo field, rows 1..M) and the
+// Neumann ghost rows of BC_WALL_DPH_tg (BC_WALL.f90:369-386).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
+                                                  double *__restrict__ phi, int nlc)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int L = g.L, M = g.M, H = g.H;
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem_raw);
+    cpx<T> *a = tw + L;
+    cpx<T> *b = a + nlc * L;
+    const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twx);
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const int nlines = M * g.N;
+    const int pair0 = blockIdx.x * nlc;
+    for (int t = threadIdx.x; t < nlc * L; t += blockDim.x) {
+        const int p = t / L, kx = t - p * L;
+        // Z[kx] = A[kx] + i B[kx] with Hermitian extension; imaginary parts at kx = 0, L/2
+        // are not used (FFT99.f90:84-86).
+        const int src = kx < H ? kx : L - kx;
+        cpx<T> AB[2];
+        for (int h = 0; h < 2; ++h) {
+            const int ln = 2 * (pair0 + p) + h;
+            cpx<T> z = {T(0), T(0)};
+            if (ln < nlines) {
+                const int k = ln / M, J = ln - k * M + 1;
+                z = spec[((size_t)k * M + (J - 1)) * H + src];
+                if (src == 0 || src == L / 2) z.y = T(0);
+                if (kx >= H) z.y = -z.y;
+            }
+            AB[h] = z;
+        }
+        a[p * L + kx] = {AB[0].x - AB[1].y, AB[0].y + AB[1].x};
+    }
+    __syncthreads();
+    cpx<T> *res = chs::fft_lds<T>(a, b, L, nlc, pa.px, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nlc * L; t += blockDim.x) {
+        const int p = t / L, i = t - p * L;
+        for (int h = 0; h < 2; ++h) {
+            const int ln = 2 * (pair0 + p) + h;
+            if (ln < nlines) {
+                const int k = ln / M, J = ln - k * M + 1;
+                const double val = h == 0 ? (double)res[p * L + i].x : (double)res[p * L + i].y;
+                phi[hix(g, i, J, k)] = val;
+                if (J == 1) phi[hix(g, i, 0, k)] = val;
+                if (J == M) phi[hix(g, i, M + 1, k)] = val;
+            }
+        }
+    }
+}
+
+// VELOUPDT_tg (VELO_CORRECTION.f90:31-71) + PRCALC_tg (PRESSURE_CORRECTION.f90:26-36) +
+// BC_WALL_Q_tg / BC_WALL_PR_TG ghost rows.
+__global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__restrict__ phi,
+                                                const double *__restrict__ div, double *__restrict__ q,
+                                                double *__restrict__ pr)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    if (i >= g.L || J > g.M) return;
+    const size_t hs = (size_t)g.N * (g.M + 2) * g.L;
+    const int im = (i == 0) ? g.L - 1 : i - 1, km = (k == 0) ? g.N - 1 : k - 1;
+    const double p0 = phi[hix(g, i, J, k)];
+    const double cx = s.dt * s.talp * g.dxi, cz = s.dt * s.talp * g.dzi;
+    const size_t h0 = hix(g, i, J, k);
+    q[h0] = q[h0] - (p0 - phi[hix(g, im, J, k)]) * cx;
+    q[2 * hs + h0] = q[2 * hs + h0] - (p0 - phi[hix(g, i, J, km)]) * cz;
+    if (J >= 2) {
+        const double cy = s.dt * s.talp * g.dyci[J];
+        q[hs + h0] = q[hs + h0] - (p0 - phi[hix(g, i, J - 1, k)]) * cy;
+    }
+    const double be = 0.5 * s.talp * s.dt * s.cvisc;
+    const double pn = pr[h0] + p0 - be * div[cix(g, i, J, k)];
+    pr[h0] = pn;
+    if (J == 1) {
+        pr[hix(g, i, 0, k)] = pn;
+        q[hix(g, i, 0, k)] = 0.0;
+        q[2 * hs + hix(g, i, 0, k)] = 0.0;
+        q[hs + hix(g, i, 0, k)] = 0.0;
+        q[hs + h0] = 0.0;
+    }
+    if (J == g.M) {
+        const size_t ht = hix(g, i, g.M + 1, k);
+        pr[ht] = pn;
+        q[ht] = 0.0;
+        q[hs + ht] = 0.0;
+        q[2 * hs + ht] = 0.0;
+    }
+}
+
+// Diagnostics: CFL_tg (CFL.f90:27-46) max over cells, and DIVGCK_tg max |div| (DIVGCK.f90).
+__global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict__ q, double *__restrict__ part,
+                                                 int which)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    double val = 0.0;
+    if (i < g.L && J <= g.M) {
+        const size_t hs = (size_t)g.N * (g.M + 2) * g.L;
+        const int ip = (i == g.L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
+        const double *u = q, *v = q + hs, *w = q + 2 * hs;
+        if (which == 0) {
+            const double a = fabs(u[hix(g, i, J, k)] + u[hix(g, ip, J, k)]) * g.dxi;
+            const double b = fabs(v[hix(g, i, J, k)] + v[hix(g, i, J + 1, k)]) * g.dyfi[J - 1];
+            const double c = fabs(w[hix(g, i, J, k)] + w[hix(g, i, J, kp)]) * g.dzi;
+            val = (a + b + c) * 0.5;
+        } else {
+            val = fabs((u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
+                       (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * g.dyfi[J - 1] +
+                       (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi);
+        }
+    }
+    __shared__ double red[256];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    red[t] = val;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] = fmax(red[t], red[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double a = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) a = fmax(a, part[t]);
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+}  // namespace
+
+// =========================================================================================
+// Host side
+// =========================================================================================
+namespace {
+
+thread_local std::string g_err;
+
+bool fft_ok(int n, chs::FftPlan &p)
+{
+    p.n = n;
+    p.npass = 0;
+    int m = n;
+    while (m % 4 == 0) { p.radix[p.npass++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.radix[p.npass++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.radix[p.npass++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.radix[p.npass++] = 5; m /= 5; }
+    return m == 1 && p.npass <= 16;
+}
+
+struct CnTab {
+    bool valid = false;
+    double dt = 0, cvisc = 0;
+    int Dx = 0, Dz = 0, Dy[3] = {0, 0, 0};
+    int fullx = 0, fullz = 0;
+    void *gx = nullptr, *gz = nullptr, *by[3] = {nullptr, nullptr, nullptr};
+    size_t bycap[3] = {0, 0, 0};
+};
+
+}  // namespace
+
+struct chs_ctx {
+    int L = 0, M = 0, N = 0, H = 0, mode = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // host copies (index conventions of Geo)
+    std::vector<double> dyfi, dyci, wff, wfb, amvr, acvr, apvr, amph, acph, apph, ak1, ak3;
+    double tgam[4], troh[4], talp[4];
+    double dxi, dzi, dxqi, dzqi, volm;
+    // device
+    double *cf = nullptr;   // packed coefficients
+    Geo geo;
+    const double *d_amph = nullptr, *d_apph = nullptr;
+    double *q = nullptr, *pr = nullptr, *phi = nullptr, *div = nullptr;
+    void *h0 = nullptr, *rhs = nullptr, *rhs2 = nullptr, *spec = nullptr;
+    double *pp = nullptr, *partial = nullptr, *scal = nullptr;
+    int npartial = 0;
+    void *twx = nullptr, *twz = nullptr;
+    chs::FftPlan px, pz;
+    CnTab cn[4];
+    bool have_fields = false;
+    // profiling of the dominant kernel
+    bool prof = false;
+    std::string prof_only;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+};
+
+namespace {
+
+int fail(chs_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(ctx, CHS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t wsize(const chs_ctx *c) { return c->mode == CHS_MIXED ? sizeof(float) : sizeof(double); }
+
+// Exact inverse of the periodic constant-coefficient tridiagonal (a, b, a), first column.
+void circulant_inverse(int n, long double a, long double b, std::vector<long double> &g)
+{
+    // Solve A g = e0 with the cyclic Thomas algorithm of TDMAIJI_CYC in long double.
+    std::vector<long double> H0(n), G1(n), G2(n), r(n, 0.0L);
+    r[0] = 1.0L;
+    const long double c = a;
+    H0[0] = c / b;
+    G1[0] = r[0] / b;
+    G2[0] = -a / b;
+    for (int i = 1; i <= n - 3; ++i) {
+        long double pp = 1.0L / (b - H0[i - 1] * a);
+        H0[i] = c * pp;
+        G1[i] = (r[i] - G1[i - 1] * a) * pp;
+        G2[i] = -(G2[i - 1] * a) * pp;
+    }
+    {
+        int i = n - 2;
+        long double pp = 1.0L / (b - H0[i - 1] * a);
+        G1[i] = (r[i] - G1[i - 1] * a) * pp;
+        G2[i] = (-c - G2[i - 1] * a) * pp;
+    }
+    for (int i = n - 3; i >= 0; --i) {
+        G1[i] = -H0[i] * G1[i + 1] + G1[i];
+        G2[i] = -H0[i] * G2[i + 1] + G2[i];
+    }
+    g.assign(n, 0.0L);
+    long double xn = (r[n - 1] - c * G1[0] - a * G1[n - 2]) / (b + c * G2[0] + a * G2[n - 2]);
+    g[n - 1] = xn;
+    for (int i = n - 2; i >= 0; --i) g[i] = G1[i] + G2[i] * xn;
+}
+
+// Half-width of the circulant inverse kept so that the dropped entries sum below tol*g0.
+int circulant_width(const std::vector<long double> &g, long double tol)
+{
+    const int n = (int)g.size();
+    long double dropped = 0.0L;
+    int D = n / 2;
+    // entries at cyclic distance d appear at indices d and n-d
+    for (int d = n / 2; d >= 1; --d) {
+        long double e = fabsl(g[d]) + (d != n - d ? fabsl(g[n - d]) : 0.0L);
+        if (dropped + e > tol * fabsl(g[0])) break;
+        dropped += e;
+        D = d - 1;
+    }
+    return D;
+}
+
+// Exact inverse of the non-periodic tridiagonal (rows j = 0..n-1; a_j sub, b_j diag,
+// c_j super), dense, in long double (columns by the Thomas algorithm).
+void tridiag_inverse(int n, const std::vector<long double> &a, const std::vector<long double> &b,
+                     const std::vector<long double> &c, std::vector<long double> &inv)
+{
+    inv.assign((size_t)n * n, 0.0L);
+    std::vector<long double> H(n), P(n), G(n);
+    H[0] = c[0] / b[0];
+    P[0] = 1.0L / b[0];
+    for (int j = 1; j < n; ++j) {
+        P[j] = 1.0L / (b[j] - H[j - 1] * a[j]);
+        H[j] = c[j] * P[j];
+    }
+    for (int col = 0; col < n; ++col) {
+        long double gprev = 0.0L;
+        for (int j = 0; j < n; ++j) {
+            long double r = (j == col) ? 1.0L : 0.0L;
+            G[j] = (j == 0) ? r * P[0] : (r - a[j] * gprev) * P[j];
+            gprev = G[j];
+        }
+        long double x = G[n - 1];
+        inv[(size_t)(n - 1) * n + col] = x;
+        for (int j = n - 2; j >= 0; --j) {
+            x = G[j] - H[j] * x;
+            inv[(size_t)j * n + col] = x;
+        }
+    }
+}
+
+int build_cn(chs_ctx *c, int ns, double dt, double cvisc)
+{
+    CnTab &t = c->cn[ns];
+    if (t.valid && t.dt == dt && t.cvisc == cvisc) return 0;
+    // Tables of an earlier (dt, cvisc) may still be read by enqueued kernels.
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool mixed = c->mode == CHS_MIXED;
+    const long double tol = mixed ? 1e-12L : 1e-19L;
+    const long double beta = 0.5L * (long double)c->talp[ns] * (long double)dt * (long double)cvisc;
+    auto upload = [&](void **dst, size_t *cap, const std::vector<long double> &src) -> int {
+        const size_t need = src.size() * wsize(c);
+        if (!*dst || (cap && *cap < need)) {
+            if (*dst) HIPCHK(c, hipFree(*dst));
+            HIPCHK(c, hipMalloc(dst, need));
+            if (cap) *cap = need;
+        }
+        if (mixed) {
+            std::vector<float> h(src.begin(), src.end());
+            HIPCHK(c, hipMemcpyAsync(*dst, h.data(), need, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            std::vector<double> h(src.begin(), src.end());
+            HIPCHK(c, hipMemcpyAsync(*dst, h.data(), need, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        return 0;
+    };
+    // x (MOMFA1_X_tg, MOMFA_tg.f90:72-82): a = c = -FACOE*DXQI, b = 1 - a - c
+    {
+        long double co = -beta * (long double)c->dxqi;
+        std::vector<long double> g;
+        circulant_inverse(c->L, co, 1.0L - co - co, g);
+        t.Dx = circulant_width(g, tol);
+        t.fullx = (2 * t.Dx + 1 >= c->L) ? 1 : 0;
+        if (t.gx) { HIPCHK(c, hipFree(t.gx)); t.gx = nullptr; }
+        if (int e = upload(&t.gx, nullptr, g)) return e;
+    }
+    // z (MOMFA2_Z_tg, MOMFA_tg.f90:129-146)
+    {
+        long double co = 1.0L * (-beta * (long double)c->dzqi);
+        std::vector<long double> g;
+        circulant_inverse(c->N, co, 1.0L - co - co, g);
+        t.Dz = circulant_width(g, tol);
+        t.fullz = (2 * t.Dz + 1 >= c->N) ? 1 : 0;
+        if (t.gz) { HIPCHK(c, hipFree(t.gz)); t.gz = nullptr; }
+        if (int e = upload(&t.gz, nullptr, g)) return e;
+    }
+    // y (MOMFA3_Y_tg, MOMFA_tg.f90:206-216): rows J = 1..NCL2
+    const int M = c->M;
+    for (int comp = 0; comp < 3; ++comp) {
+        std::vector<long double> a(M), b(M), cc(M), inv;
+        for (int J = 1; J <= M; ++J) {
+            const size_t o = (size_t)comp * (M + 2) + J;
+            b[J - 1] = 1.0L - beta * (long double)c->acvr[o];
+            cc[J - 1] = -beta * (long double)c->apvr[o];
+            a[J - 1] = -beta * (long double)c->amvr[o];
+        }
+        a[0] = 0.0L;       // multiplies BCJ(:,1) = 0 (TDMAIJJ_nonCYC.f90:87)
+        cc[M - 1] = 0.0L;  // multiplies BCJ(:,2) = 0 (TDMAIJJ_nonCYC.f90:102)
+        tridiag_inverse(M, a, b, cc, inv);
+        int D = 0;
+        for (int J = 0; J < M; ++J) {
+            long double diag = fabsl(inv[(size_t)J * M + J]);
+            long double dropped = 0.0L;
+            int dJ = M - 1;
+            for (int d = M - 1; d >= 1; --d) {
+                long double e = 0.0L;
+                if (J + d < M) e += fabsl(inv[(size_t)J * M + J + d]);
+                if (J - d >= 0) e += fabsl(inv[(size_t)J * M + J - d]);
+                if (dropped + e > tol * diag) break;
+                dropped += e;
+                dJ = d - 1;
+            }
+            if (dJ > D) D = dJ;
+        }
+        const int w = 2 * D + 1;
+        std::vector<long double> band((size_t)M * w, 0.0L);
+        for (int J = 0; J < M; ++J)
+            for (int d = -D; d <= D; ++d)
+                if (J + d >= 0 && J + d < M) band[(size_t)J * w + d + D] = inv[(size_t)J * M + J + d];
+        t.Dy[comp] = D;
+        if (int e = upload(&t.by[comp], &t.bycap[comp], band)) return e;
+    }
+    t.dt = dt;
+    t.cvisc = cvisc;
+    t.valid = true;
+    return 0;
+}
+
+template <typename T>
+int upload_twiddles(chs_ctx *c, int n, void **dst)
+{
+    std::vector<cpx<T>> h(n);
+    for (int m = 0; m < n; ++m) {
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+        h[m] = {(T)cosl(ang), (T)sinl(ang)};
+    }
+    HIPCHK(c, hipMalloc(dst, sizeof(cpx<T>) * n));
+    HIPCHK(c, hipMemcpyAsync(*dst, h.data(), sizeof(cpx<T>) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int nlc_for(int L) { int n = 1024 / L; return n < 1 ? 1 : n; }
+
+// kx columns per z-FFT block: largest power of two <= 16 whose twiddles + two buffers fit
+// in 64 KiB of LDS.
+int tb_for(int N, size_t csz)
+{
+    int tb = 16;
+    while (tb > 1 && (size_t)(N + 2 * tb * N) * csz > 65536) tb >>= 1;
+    return tb;
+}
+
+struct EvScope {
+    chs_ctx *c;
+    const char *name;
+    hipEvent_t e1 = nullptr;
+    EvScope(chs_ctx *cc, const char *nm) : c(cc), name(nm)
+    {
+        if (!c->prof || (!c->prof_only.empty() && c->prof_only != nm)) return;
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e1, c->stream);
+    }
+    ~EvScope()
+    {
+        if (!c->prof || !e1) return;
+        hipEvent_t e2;
+        (void)hipEventCreate(&e2);
+        (void)hipEventRecord(e2, c->stream);
+        c->ev_used.push_back({name, {e1, e2}});
+    }
+};
+
+template <typename W>
+int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
+{
+    using T = W;
+    if (int e = build_cn(c, ns, dt, cvisc)) return e;
+    const Geo &g = c->geo;
+    const Sub s = {c->tgam[ns], c->troh[ns], c->talp[ns], dt, cvisc};
+    const int L = c->L, M = c->M, N = c->N, H = c->H;
+    const size_t cs = (size_t)N * M * L;
+    W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
+    dim3 blk(64, 4), grd((L + 63) / 64, (M + 3) / 4, N);
+    {
+        EvScope ev(c, "conv_rhs");
+        hipLaunchKernelGGL(k_conv_rhs<W>, grd, blk, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial);
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, c->npartial,
+                       1.0 / c->volm, c->scal);
+    CnTab &t = c->cn[ns];
+    {
+        EvScope ev(c, "cn_x");
+        hipLaunchKernelGGL(k_cn_x<W>, dim3(M, N, 3), dim3(256), sizeof(W) * L, c->stream, g,
+                           (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, 0);
+    }
+    {
+        EvScope ev(c, "cn_z");
+        for (int comp = 0; comp < 3; ++comp)
+            hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz,
+                               rhs, rhs2, comp);
+    }
+    {
+        EvScope ev(c, "cn_y");
+        for (int comp = 0; comp < 3; ++comp)
+            hipLaunchKernelGGL(k_cn_y<W>, dim3((L + 63) / 64, N), dim3(64), 0, c->stream, g,
+                               (const W *)t.by[comp], t.Dy[comp], rhs2, comp == 0 ? rhs2 + cs : nullptr,
+                               c->q, comp);
+    }
+    PoisArgs pa{c->px, c->pz, c->twx, c->twz};
+    const int nlc = nlc_for(L);
+    const int npairs = (M * N + 1) / 2;
+    const size_t shx = sizeof(cpx<T>) * (L + 2 * nlc * L);
+    {
+        EvScope ev(c, "divg_fftx");
+        hipLaunchKernelGGL(k_divg_fftx<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa,
+                           1.0 / (dt * c->talp[ns]), c->q, c->div, (cpx<T> *)c->spec, nlc,
+                           (const double *)nullptr);
+    }
+    const int TB = tb_for(N, sizeof(cpx<T>));
+    const size_t shz = sizeof(cpx<T>) * (N + 2 * TB * N);
+    {
+        EvScope ev(c, "fftz_fwd");
+        hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
+                           (cpx<T> *)c->spec, TB, -1);
+    }
+    {
+        EvScope ev(c, "ytdma");
+        hipLaunchKernelGGL(k_ytdma<T>, dim3((H + 255) / 256, N), dim3(256), 0, c->stream, g, c->d_amph,
+                           c->d_apph, c->pp, (cpx<T> *)c->spec);
+    }
+    {
+        EvScope ev(c, "fftz_inv");
+        hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
+                           (cpx<T> *)c->spec, TB, +1);
+    }
+    {
+        EvScope ev(c, "fftx_inv");
+        hipLaunchKernelGGL(k_fftx_inv<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa,
+                           (const cpx<T> *)c->spec, c->phi, nlc);
+    }
+    {
+        EvScope ev(c, "update");
+        hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr);
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// Transforms, y-Thomas and inverse transforms of the cell field in c->div -> c->phi.
+template <typename T>
+void poisson_launch(chs_ctx *c, const double *src)
+{
+    const Geo &g = c->geo;
+    const int L = c->L, M = c->M, N = c->N, H = c->H;
+    PoisArgs pa{c->px, c->pz, c->twx, c->twz};
+    const int nlc = nlc_for(L);
+    const int npairs = (M * N + 1) / 2;
+    const size_t shx = sizeof(cpx<T>) * (L + 2 * nlc * L);
+    hipLaunchKernelGGL(k_divg_fftx<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa, 1.0,
+                       c->q, c->div, (cpx<T> *)c->spec, nlc, src);
+    const int TB = tb_for(N, sizeof(cpx<T>));
+    const size_t shz = sizeof(cpx<T>) * (N + 2 * TB * N);
+    hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
+                       (cpx<T> *)c->spec, TB, -1);
+    hipLaunchKernelGGL(k_ytdma<T>, dim3((H + 255) / 256, N), dim3(256), 0, c->stream, g, c->d_amph, c->d_apph,
+                       c->pp, (cpx<T> *)c->spec);
+    hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
+                       (cpx<T> *)c->spec, TB, +1);
+    hipLaunchKernelGGL(k_fftx_inv<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa,
+                       (const cpx<T> *)c->spec, c->phi, nlc);
+}
+
+template <typename T>
+int poisson_impl(chs_ctx *c, const double *rhs, double *phi)
+{
+    const int L = c->L, M = c->M, N = c->N;
+    const size_t cs = (size_t)N * M * L, hs = (size_t)N * (M + 2) * L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double *src = nullptr;
+    HIPCHK(c, hipMalloc(&src, sizeof(double) * cs));
+    hipError_t e = hipMemcpyAsync(src, rhs, sizeof(double) * cs, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        poisson_launch<T>(c, src);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    std::vector<double> h(hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), c->phi, sizeof(double) * hs, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(src);
+    if (e != hipSuccess) return fail(c, CHS_E_HIP, std::string("chs_poisson: ") + hipGetErrorString(e));
+    for (int k = 0; k < N; ++k)
+        for (int J = 1; J <= M; ++J)
+            std::memcpy(phi + ((size_t)k * M + J - 1) * L, h.data() + ((size_t)k * (M + 2) + J) * L,
+                        sizeof(double) * L);
+    return 0;
+}
+
+}  // namespace
+
+// =========================================================================================
+// C-ABI
+// =========================================================================================
+extern "C" {
+
+const char *chs_version(void) { return CHS_VERSION; }
+
+const char *chs_last_error(const chs_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int chs_channel_setup(int L, int M, int N, double HX, double HZ, double STR2, double *scal, double *dyfi,
+                      double *dyci, double *wff, double *wfb, double *amvr, double *acvr, double *apvr,
+                      double *amph, double *acph, double *apph, double *ak1, double *ak3, double *tgam,
+                      double *troh, double *talp)
+{
+    if (L < 4 || M < 4 || N < 4 || !(HX > 0) || !(HZ > 0)) return fail(nullptr, CHS_E_ARG, "bad mesh size");
+    // CONSPARA.f90:44-73
+    const double dx = HX / (double)L, dz = HZ / (double)N;
+    const double dxi = 1.0 / dx, dzi = 1.0 / dz;
+    // COORDJR.f90:44-55: YND(J) = tanh(STR2 (x2 - 1/2)) / tanh(STR2/2), x2 = (J-1)/(NND2-1)
+    std::vector<double> ynd(M + 1), ycc(M);
+    for (int J = 1; J <= M + 1; ++J) {
+        const double x2 = (double)(J - 1) / (double)M;
+        ynd[J - 1] = std::tanh(STR2 * (x2 - 0.5)) / std::tanh(STR2 * 0.5);
+    }
+    for (int J = 1; J <= M; ++J) ycc[J - 1] = (ynd[J - 1] + ynd[J]) * 0.5;
+    // COORDJR.f90:121-135
+    for (int J = 1; J <= M; ++J) dyfi[J - 1] = 1.0 / (ynd[J] - ynd[J - 1]);
+    for (int J = 2; J <= M; ++J) dyci[J - 1] = 1.0 / ((ynd[J] - ynd[J - 2]) * 0.5);
+    dyci[0] = 2.0 / (ynd[1] - ynd[0]);
+    dyci[M] = 2.0 / (ynd[M] - ynd[M - 1]);
+    // COORDJR.f90:425-442
+    for (int J = 2; J <= M; ++J) {
+        wff[J - 1] = (ynd[J - 1] - ycc[J - 2]) / (ycc[J - 1] - ycc[J - 2]);
+        wfb[J - 1] = (ycc[J - 1] - ynd[J - 1]) / (ycc[J - 1] - ycc[J - 2]);
+    }
+    wfb[0] = 1.0; wff[0] = 0.0; wfb[M] = 0.0; wff[M] = 1.0;
+    // COORDJR.f90:380-388
+    double volm = 0.0;
+    for (int k = 0; k < N; ++k)
+        for (int J = 1; J <= M; ++J)
+            for (int i = 0; i < L; ++i) volm = volm + 1.0 / dyfi[J - 1];
+    // TDMA_COEF.f90:44-65
+    for (int J = 1; J <= M; ++J) {
+        amph[J - 1] = dyfi[J - 1] * dyci[J - 1];
+        apph[J - 1] = dyfi[J - 1] * dyci[J];
+        if (J == 1) amph[J - 1] = dyfi[J - 1] * dyci[J - 1] * 0.5;
+        if (J + 1 == M + 1) apph[J - 1] = dyfi[J - 1] * dyci[J] * 0.5;
+    }
+    for (int J = 1; J <= M; ++J) acph[J - 1] = -(amph[J - 1] + apph[J - 1]);
+    acph[0] = amph[0] + acph[0];
+    amph[0] = 0.0;
+    acph[M - 1] = apph[M - 1] + acph[M - 1];
+    apph[M - 1] = 0.0;
+    // TDMA_COEF.f90:67-104, A*VR(1:NND2, 1:3) column-major
+    const int S = M + 1;
+    for (int n = 0; n < 3 * S; ++n) amvr[n] = acvr[n] = apvr[n] = 0.0;
+    auto DYFI = [&](int J) { return dyfi[J - 1]; };
+    auto DYCI = [&](int J) { return dyci[J - 1]; };
+    for (int idr = 1; idr <= 3; ++idr) {
+        double *am = amvr + (idr - 1) * S - 1, *ac = acvr + (idr - 1) * S - 1, *ap = apvr + (idr - 1) * S - 1;
+        if (idr != 2) {
+            for (int J = 2; J <= M - 1; ++J) {
+                am[J] = DYFI(J) * DYCI(J);
+                ap[J] = DYFI(J) * DYCI(J + 1);
+                ac[J] = -(am[J] + ap[J]);
+            }
+            double ucaj = 4.0 / (2.0 / DYCI(2) + 1.0 / DYFI(1));
+            am[1] = 0.0;
+            ap[1] = ucaj * DYCI(2);
+            ac[1] = -ucaj * (DYCI(2) + 2.0 * DYFI(1));
+            ucaj = 4.0 / (2.0 / DYCI(M) + 1.0 / DYFI(M));
+            am[M] = ucaj * DYCI(M);
+            ap[M] = 0.0;
+            ac[M] = -ucaj * (2.0 * DYFI(M) + DYCI(M));
+        } else {
+            for (int J = 2; J <= M; ++J) {
+                am[J] = DYFI(J - 1) * DYCI(J);
+                ap[J] = DYFI(J) * DYCI(J);
+                ac[J] = -(am[J] + ap[J]);
+            }
+            am[1] = ap[1] = 0.0;
+            ac[1] = 1.0;
+            am[M + 1] = ap[M + 1] = 0.0;
+            ac[M + 1] = 1.0;
+        }
+    }
+    // possion3d_FFT99.f90:171-199
+    const double pi = 2.0 * std::asin(1.0);
+    for (int K = 1; K <= N; ++K) {
+        double an = (K <= N / 2) ? (double)(K - 1) * 2.0 * pi : -(double)(N - K + 1) * 2.0 * pi;
+        ak3[K - 1] = 2.0 * (1.0 - std::cos(an / (double)N)) * (dzi * dzi);
+    }
+    for (int I = 1; I <= L; ++I) {
+        double ap = (I <= L / 2) ? (double)(I - 1) * 2.0 * pi : -(double)(L - I + 1) * 2.0 * pi;
+        ak1[I - 1] = 2.0 * (1.0 - std::cos(ap / (double)L)) * (dxi * dxi);
+    }
+    // CONS_RKCOEF.f90:21-51
+    const double tg[4] = {1.0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+    const double tr[4] = {0.0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
+    for (int n = 0; n < 4; ++n) { tgam[n] = tg[n]; troh[n] = tr[n]; talp[n] = tg[n] + tr[n]; }
+    scal[0] = dxi; scal[1] = dzi; scal[2] = dxi * dxi; scal[3] = dzi * dzi; scal[4] = volm;
+    return 0;
+}
+
+int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
+{
+    if (!cf || !out) return fail(nullptr, CHS_E_ARG, "null argument");
+    *out = nullptr;
+    if (mode != CHS_PLAIN_FP64 && mode != CHS_MIXED) return fail(nullptr, CHS_E_ARG, "bad mode");
+    const int L = cf->L, M = cf->M, N = cf->N;
+    chs::FftPlan px, pz;
+    if (L < 4 || M < 4 || N < 4 || (L & 1) || (N & 1) || !fft_ok(L, px) || !fft_ok(N, pz))
+        return fail(nullptr, CHS_E_ARG, "L, N must be even with factors 2,3,5 only; M >= 4");
+    chs_ctx *c = new chs_ctx();
+    c->L = L; c->M = M; c->N = N; c->H = L / 2 + 1; c->mode = mode;
+    c->px = px; c->pz = pz;
+    c->dxi = cf->dxi; c->dzi = cf->dzi; c->dxqi = cf->dxqi; c->dzqi = cf->dzqi; c->volm = cf->volm;
+    c->dyfi.assign(cf->dyfi, cf->dyfi + M);
+    c->dyci.assign(M + 2, 0.0);
+    c->wff.assign(M + 2, 0.0);
+    c->wfb.assign(M + 2, 0.0);
+    for (int J = 1; J <= M + 1; ++J) {
+        c->dyci[J] = cf->dyci[J - 1];
+        c->wff[J] = cf->wff[J - 1];
+        c->wfb[J] = cf->wfb[J - 1];
+    }
+    c->amvr.assign(3 * (M + 2), 0.0);
+    c->acvr.assign(3 * (M + 2), 0.0);
+    c->apvr.assign(3 * (M + 2), 0.0);
+    for (int comp = 0; comp < 3; ++comp)
+        for (int J = 1; J <= M + 1; ++J) {
+            c->amvr[comp * (M + 2) + J] = cf->amvr[comp * (M + 1) + J - 1];
+            c->acvr[comp * (M + 2) + J] = cf->acvr[comp * (M + 1) + J - 1];
+            c->apvr[comp * (M + 2) + J] = cf->apvr[comp * (M + 1) + J - 1];
+        }
+    c->amph.assign(cf->amph, cf->amph + M);
+    c->acph.assign(cf->acph, cf->acph + M);
+    c->apph.assign(cf->apph, cf->apph + M);
+    c->ak1.assign(cf->ak1, cf->ak1 + L);
+    c->ak3.assign(cf->ak3, cf->ak3 + N);
+    for (int n = 0; n < 4; ++n) { c->tgam[n] = cf->tgam[n]; c->troh[n] = cf->troh[n]; c->talp[n] = cf->talp[n]; }
+
+    auto bad = [&](int code) { chs_destroy(c); return code; };
+#define CK(expr)                                                                        \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fail(nullptr, CHS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return bad(CHS_E_HIP);                                                      \
+        }                                                                               \
+    } while (0)
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // packed coefficients
+    std::vector<double> pack;
+    auto put = [&](const std::vector<double> &v) { size_t o = pack.size(); pack.insert(pack.end(), v.begin(), v.end()); return o; };
+    size_t o_dyfi = put(c->dyfi), o_dyci = put(c->dyci), o_wff = put(c->wff), o_wfb = put(c->wfb);
+    size_t o_am = put(c->amvr), o_ac = put(c->acvr), o_ap = put(c->apvr);
+    size_t o_amph = put(c->amph), o_apph = put(c->apph);
+    CK(hipMalloc(&c->cf, sizeof(double) * pack.size()));
+    CK(hipMemcpyAsync(c->cf, pack.data(), sizeof(double) * pack.size(), hipMemcpyHostToDevice, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    Geo &g = c->geo;
+    g.L = L; g.M = M; g.N = N; g.H = c->H;
+    g.dxi = c->dxi; g.dzi = c->dzi; g.dxqi = c->dxqi; g.dzqi = c->dzqi;
+    g.dyfi = c->cf + o_dyfi; g.dyci = c->cf + o_dyci; g.wff = c->cf + o_wff; g.wfb = c->cf + o_wfb;
+    g.amvr = c->cf + o_am; g.acvr = c->cf + o_ac; g.apvr = c->cf + o_ap;
+    c->d_amph = c->cf + o_amph; c->d_apph = c->cf + o_apph;
+
+    const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L;
+    const size_t ws = wsize(c);
+    CK(hipMalloc(&c->q, sizeof(double) * 3 * hs));
+    CK(hipMalloc(&c->pr, sizeof(double) * hs));
+    CK(hipMalloc(&c->phi, sizeof(double) * hs));
+    CK(hipMalloc(&c->div, sizeof(double) * cs));
+    CK(hipMalloc(&c->h0, ws * 3 * cs));
+    CK(hipMalloc(&c->rhs, ws * 3 * cs));
+    CK(hipMalloc(&c->rhs2, ws * 3 * cs));
+    CK(hipMalloc(&c->spec, 2 * ws * (size_t)c->H * M * N));
+    CK(hipMalloc(&c->pp, sizeof(double) * (size_t)c->H * M * N));
+    c->npartial = ((L + 63) / 64) * ((M + 3) / 4) * N;
+    CK(hipMalloc(&c->partial, sizeof(double) * c->npartial));
+    CK(hipMalloc(&c->scal, sizeof(double) * 16));
+    CK(hipMemsetAsync(c->q, 0, sizeof(double) * 3 * hs, c->stream));
+    CK(hipMemsetAsync(c->pr, 0, sizeof(double) * hs, c->stream));
+    CK(hipMemsetAsync(c->phi, 0, sizeof(double) * hs, c->stream));
+    CK(hipMemsetAsync(c->h0, 0, ws * 3 * cs, c->stream));
+    CK(hipMemsetAsync(c->rhs, 0, ws * 3 * cs, c->stream));
+    CK(hipMemsetAsync(c->rhs2, 0, ws * 3 * cs, c->stream));
+    CK(hipMemsetAsync(c->scal, 0, sizeof(double) * 16, c->stream));
+    // Poisson pivots (TDMAIJJ_nonCYC.f90:84-106 on TDMA_PHI_COEF, possion3d_FFT99.f90:213-231)
+    {
+        const int H = c->H;
+        std::vector<double> pp((size_t)H * M * N);
+        std::vector<double> hh(M);
+        for (int kz = 0; kz < N; ++kz)
+            for (int kx = 0; kx < H; ++kx) {
+                double hprev = 0.0;
+                for (int J = 1; J <= M; ++J) {
+                    double a = c->amph[J - 1], cc = c->apph[J - 1];
+                    double b = c->acph[J - 1] + (-c->ak1[kx] / 1.0 - c->ak3[kz]);
+                    if (kx == 0 && kz == 0 && J == 1) { a = 0.0; cc = 0.0; b = c->acph[0] + (-c->ak1[0] / 1.0 - c->ak3[0]); }
+                    double p = (J == 1) ? 1.0 / b : 1.0 / (b - hprev * a);
+                    hprev = cc * p;
+                    pp[((size_t)kz * M + (J - 1)) * H + kx] = p;
+                }
+            }
+        CK(hipMemcpyAsync(c->pp, pp.data(), sizeof(double) * pp.size(), hipMemcpyHostToDevice, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+    }
+    int e = (mode == CHS_MIXED) ? upload_twiddles<float>(c, L, &c->twx) : upload_twiddles<double>(c, L, &c->twx);
+    if (!e) e = (mode == CHS_MIXED) ? upload_twiddles<float>(c, N, &c->twz) : upload_twiddles<double>(c, N, &c->twz);
+    if (e) return bad(e);
+#undef CK
+    *out = c;
+    return 0;
+}
+
+void chs_destroy(chs_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->cf, c->q, c->pr, c->phi, c->div, c->h0, c->rhs, c->rhs2, c->spec, c->pp,
+                    c->partial, c->scal, c->twx, c->twz};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &t : c->cn) {
+        if (t.gx) (void)hipFree(t.gx);
+        if (t.gz) (void)hipFree(t.gz);
+        for (auto *b : t.by)
+            if (b) (void)hipFree(b);
+    }
+    for (auto &u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int chs_upload_fields(chs_ctx *c, const double *q, const double *pr, const double *convh0)
+{
+    if (!c || !q || !pr) return fail(c, CHS_E_ARG, "null argument");
+    const size_t hs = (size_t)c->N * (c->M + 2) * c->L, cs = (size_t)c->N * c->M * c->L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->q, q, sizeof(double) * 3 * hs, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pr, pr, sizeof(double) * hs, hipMemcpyHostToDevice, c->stream));
+    if (convh0) {
+        if (c->mode == CHS_MIXED) {
+            std::vector<float> h(convh0, convh0 + 3 * cs);
+            HIPCHK(c, hipMemcpyAsync(c->h0, h.data(), sizeof(float) * 3 * cs, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, hipMemcpyAsync(c->h0, convh0, sizeof(double) * 3 * cs, hipMemcpyHostToDevice, c->stream));
+        }
+    } else {
+        HIPCHK(c, hipMemsetAsync(c->h0, 0, wsize(c) * 3 * cs, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_fields = true;
+    return 0;
+}
+
+int chs_download_fields(chs_ctx *c, double *q, double *pr, double *convh0)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    const size_t hs = (size_t)c->N * (c->M + 2) * c->L, cs = (size_t)c->N * c->M * c->L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (q) HIPCHK(c, hipMemcpyAsync(q, c->q, sizeof(double) * 3 * hs, hipMemcpyDeviceToHost, c->stream));
+    if (pr) HIPCHK(c, hipMemcpyAsync(pr, c->pr, sizeof(double) * hs, hipMemcpyDeviceToHost, c->stream));
+    if (convh0) {
+        if (c->mode == CHS_MIXED) {
+            std::vector<float> h(3 * cs);
+            HIPCHK(c, hipMemcpyAsync(h.data(), c->h0, sizeof(float) * 3 * cs, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (size_t n = 0; n < 3 * cs; ++n) convh0[n] = h[n];
+        } else {
+            HIPCHK(c, hipMemcpyAsync(convh0, c->h0, sizeof(double) * 3 * cs, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chs_substep_tg(chs_ctx *c, int ns, double dt, double cvisc)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    if (ns < 0 || ns > 3) return fail(c, CHS_E_ARG, "NS must be 0..3");
+    if (!(dt > 0) || !(cvisc > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
+    if (!c->have_fields) return fail(c, CHS_E_STATE, "no fields uploaded");
+    return c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt, cvisc) : substep_impl<double>(c, ns, dt, cvisc);
+}
+
+int chs_step_tg(chs_ctx *c, double dt, double cvisc)
+{
+    for (int ns = 1; ns <= 3; ++ns)
+        if (int e = chs_substep_tg(c, ns, dt, cvisc)) return e;
+    return 0;
+}
+
+int chs_poisson(chs_ctx *c, const double *rhs, double *phi)
+{
+    if (!c || !rhs || !phi) return fail(c, CHS_E_ARG, "null argument");
+    return c->mode == CHS_MIXED ? poisson_impl<float>(c, rhs, phi) : poisson_impl<double>(c, rhs, phi);
+}
+
+int chs_cfl_tg(chs_ctx *c, double dt, double *cflmm)
+{
+    if (!c || !cflmm) return fail(c, CHS_E_ARG, "null argument");
+    const Geo &g = c->geo;
+    dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
+    hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 0);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, c->npartial, c->scal + 1);
+    double v = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&v, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *cflmm = v * dt;
+    return 0;
+}
+
+int chs_max_div(chs_ctx *c, double *maxdiv)
+{
+    if (!c || !maxdiv) return fail(c, CHS_E_ARG, "null argument");
+    const Geo &g = c->geo;
+    dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
+    hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 1);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, c->npartial, c->scal + 2);
+    double v = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&v, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *maxdiv = v;
+    return 0;
+}
+
+int chs_sync(chs_ctx *c)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chs_device_fields(chs_ctx *c, double **q, double **pr)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    if (q) *q = c->q;
+    if (pr) *pr = c->pr;
+    return 0;
+}
+
+int chs_profile_reset(chs_ctx *c, const char *only)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
+    c->ev_used.clear();
+    c->prof = true;
+    c->prof_only = only ? only : "";
+    return 0;
+}
+
+int chs_profile_get(chs_ctx *c, const char *kernel, double *avg_ms, int *count)
+{
+    if (!c || !kernel || !avg_ms || !count) return fail(c, CHS_E_ARG, "null argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    int n = 0;
+    for (auto &u : c->ev_used)
+        if (u.first == kernel) {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, u.second.first, u.second.second));
+            tot += ms;
+            ++n;
+        }
+    *avg_ms = n ? tot / n : 0.0;
+    *count = n;
+    return 0;
+}
+
+}  // extern "C"

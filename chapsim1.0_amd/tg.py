@@ -1,0 +1,193 @@
+"""Host mirror of the reference's TG-domain momentum substep interface over the C-ABI of
+libchapsim_hip.so (include/chapsim_hip.h).
+
+The reference calls ``SOLVERRK3_MOM_tg(NS)`` with everything else in module globals
+(SOLVERRK3_MOM_tg.f90:13, SOLVE.f90:114-134).  ``ChannelTG`` holds the same state on the
+GPU and exposes the same steps: ``SOLVERRK3_MOM_tg(ns, dt, cvisc)`` for one substep
+(ns = 0 is the projection of CALC_INITIALIZATION_tg), ``rk3_step`` for the stage loop of
+SOLVE, ``cfl`` / ``max_div`` for CFL_tg / DIVGCK_tg.  Errors are raised as ``ChsError``
+(the reference STOPs through ERRHDL, WRTHDL.f90:48-62).
+
+Arrays use C-order numpy views of the reference's Fortran layout:
+``q[c, k, J, i]`` = Q_tg(i+1, J, k+1, c+1) with J = 0..M+1, ``pr[k, J, i]``,
+``convh0[c, k, J-1, i]``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchapsim_hip.so")
+
+PLAIN_FP64 = 0
+MIXED = 1
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+# Every entry point declared in include/chapsim_hip.h: name -> (restype, argtypes).
+SIGNATURES = {
+    "chs_version": (C.c_char_p, []),
+    "chs_last_error": (C.c_char_p, [C.c_void_p]),
+    "chs_channel_setup": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+                          + [_dp] * 16),
+    "chs_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "chs_destroy": (None, [C.c_void_p]),
+    "chs_upload_fields": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "chs_download_fields": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "chs_substep_tg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
+    "chs_step_tg": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "chs_poisson": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "chs_cfl_tg": (C.c_int, [C.c_void_p, C.c_double, _dp]),
+    "chs_max_div": (C.c_int, [C.c_void_p, _dp]),
+    "chs_sync": (C.c_int, [C.c_void_p]),
+    "chs_device_fields": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp)]),
+    "chs_profile_reset": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "chs_profile_get": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_int)]),
+}
+
+
+class ChsError(RuntimeError):
+    pass
+
+
+class _Coeffs(C.Structure):
+    _fields_ = [("L", C.c_int), ("M", C.c_int), ("N", C.c_int),
+                ("dxi", C.c_double), ("dzi", C.c_double), ("dxqi", C.c_double),
+                ("dzqi", C.c_double), ("volm", C.c_double)] + \
+               [(n, _dp) for n in ("dyfi", "dyci", "wff", "wfb", "amvr", "acvr", "apvr",
+                                   "amph", "acph", "apph", "ak1", "ak3", "tgam", "troh", "talp")]
+
+
+def lib():
+    """Load libchapsim_hip.so; raises if it is not built (there is no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChsError(f"{LIB_PATH} is missing: build it (make -f chapsim1.0_amd/Makefile)")
+        lb = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lb, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lb
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        msg = lib().chs_last_error(ctx)
+        raise ChsError(f"chapsim_hip error {rc}: {msg.decode() if msg else ''}")
+
+
+def channel_setup(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8):
+    """Mesh and coefficients of the reference's set-up, computed by chs_channel_setup."""
+    scal = np.zeros(5)
+    out = dict(dyfi=np.zeros(M), dyci=np.zeros(M + 1), wff=np.zeros(M + 1), wfb=np.zeros(M + 1),
+               amvr=np.zeros(3 * (M + 1)), acvr=np.zeros(3 * (M + 1)), apvr=np.zeros(3 * (M + 1)),
+               amph=np.zeros(M), acph=np.zeros(M), apph=np.zeros(M), ak1=np.zeros(L),
+               ak3=np.zeros(N), tgam=np.zeros(4), troh=np.zeros(4), talp=np.zeros(4))
+    keys = ("dyfi", "dyci", "wff", "wfb", "amvr", "acvr", "apvr", "amph", "acph", "apph",
+            "ak1", "ak3", "tgam", "troh", "talp")
+    _check(lib().chs_channel_setup(L, M, N, HX, HZ, STR2, _p(scal), *[_p(out[k]) for k in keys]))
+    out.update(L=L, M=M, N=N, dxi=scal[0], dzi=scal[1], dxqi=scal[2], dzqi=scal[3], volm=scal[4])
+    return out
+
+
+class ChannelTG:
+    """One periodic channel resident on the GPU (TG domain, flow only)."""
+
+    def __init__(self, coeffs: dict, mode: int = PLAIN_FP64):
+        self.L, self.M, self.N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
+        self.mode = mode
+        self._arrays = {k: np.ascontiguousarray(v, dtype=np.float64)
+                        for k, v in coeffs.items() if isinstance(v, np.ndarray)}
+        a = self._arrays
+        self._c = _Coeffs(self.L, self.M, self.N, coeffs["dxi"], coeffs["dzi"], coeffs["dxqi"],
+                          coeffs["dzqi"], coeffs["volm"],
+                          *[_p(a[k]) for k in ("dyfi", "dyci", "wff", "wfb", "amvr", "acvr",
+                                               "apvr", "amph", "acph", "apph", "ak1", "ak3",
+                                               "tgam", "troh", "talp")])
+        h = C.c_void_p()
+        _check(lib().chs_create(C.byref(self._c), mode, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -----------------------------------------------------------------------
+    def upload(self, q, pr, convh0=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        pr = np.ascontiguousarray(pr, dtype=np.float64)
+        L, M, N = self.L, self.M, self.N
+        if q.shape != (3, N, M + 2, L) or pr.shape != (N, M + 2, L):
+            raise ChsError("field shapes must be q[3,N,M+2,L], pr[N,M+2,L]")
+        h0 = None
+        if convh0 is not None:
+            h0 = np.ascontiguousarray(convh0, dtype=np.float64)
+            if h0.shape != (3, N, M, L):
+                raise ChsError("convh0 shape must be [3,N,M,L]")
+        _check(lib().chs_upload_fields(self._h, _p(q), _p(pr), _p(h0) if h0 is not None else None),
+               self._h)
+
+    def download(self, with_history=False):
+        L, M, N = self.L, self.M, self.N
+        q = np.zeros((3, N, M + 2, L))
+        pr = np.zeros((N, M + 2, L))
+        h0 = np.zeros((3, N, M, L)) if with_history else None
+        _check(lib().chs_download_fields(self._h, _p(q), _p(pr), _p(h0) if with_history else None),
+               self._h)
+        return (q, pr, h0) if with_history else (q, pr)
+
+    # -- the reference's call structure ----------------------------------------------
+    def SOLVERRK3_MOM_tg(self, ns: int, dt: float, cvisc: float):
+        _check(lib().chs_substep_tg(self._h, int(ns), float(dt), float(cvisc)), self._h)
+
+    def rk3_step(self, dt: float, cvisc: float):
+        _check(lib().chs_step_tg(self._h, float(dt), float(cvisc)), self._h)
+
+    def poisson(self, rhs):
+        L, M, N = self.L, self.M, self.N
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        if rhs.shape != (N, M, L):
+            raise ChsError("rhs shape must be [N,M,L]")
+        phi = np.zeros((N, M, L))
+        _check(lib().chs_poisson(self._h, _p(rhs), _p(phi)), self._h)
+        return phi
+
+    def cfl(self, dt: float) -> float:
+        v = C.c_double()
+        _check(lib().chs_cfl_tg(self._h, float(dt), C.byref(v)), self._h)
+        return v.value
+
+    def max_div(self) -> float:
+        v = C.c_double()
+        _check(lib().chs_max_div(self._h, C.byref(v)), self._h)
+        return v.value
+
+    def sync(self):
+        _check(lib().chs_sync(self._h), self._h)
+
+    def profile_reset(self, only: str | None = None):
+        _check(lib().chs_profile_reset(self._h, only.encode() if only else None), self._h)
+
+    def profile_get(self, kernel: str):
+        avg = C.c_double()
+        n = C.c_int()
+        _check(lib().chs_profile_get(self._h, kernel.encode(), C.byref(avg), C.byref(n)), self._h)
+        return avg.value, n.value

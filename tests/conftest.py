@@ -19,3 +19,9 @@ def golden():
     def load(name):
         return dict(np.load(os.path.join(ROOT, "tests", "golden", name)))
     return load
+
+
+@pytest.fixture(scope="session")
+def pkg():
+    import __graft_entry__
+    return __graft_entry__.load()

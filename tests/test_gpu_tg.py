@@ -1,0 +1,113 @@
+"""GPU parity of the plain FP64 substep against the oracle (and the reference's golden
+vectors).  Tolerances: velocity 1e-10 relative (max-norm), pressure 1e-9 relative
+(the pressure gauge mode is solved by an ill-conditioned Neumann Thomas sweep),
+divergence after projection < 1e-10."""
+import numpy as np
+import pytest
+
+from oracle import oracle, setup
+
+pytestmark = pytest.mark.gpu
+
+MESHES = [  # L, M, N, STR2, REN, dt
+    (16, 16, 16, 1.8, 2800.0, 1.0e-3),
+    (20, 18, 30, 2.2, 6875.0, 5.0e-4),
+    (64, 64, 64, 1.8, 2800.0, 1.0e-3),
+    (48, 36, 40, 2.0, 4000.0, 8.0e-4),
+]
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def make(pkg, L, M, N, STR2, REN, seed=3):
+    mesh = setup.Mesh(L, M, N, STR2=STR2)
+    q, pr = oracle.initial_field(mesh, seed)
+    ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=STR2), pkg.PLAIN_FP64)
+    ch.upload(q, pr)
+    o = oracle.OracleTG(mesh, REN)
+    o.set_fields(q, pr)
+    return mesh, ch, o
+
+
+@pytest.mark.parametrize("L,M,N,STR2,REN,dt", MESHES)
+def test_projection_and_steps_match_oracle(pkg, L, M, N, STR2, REN, dt):
+    print(f"case {L}x{M}x{N}", flush=True)
+    mesh, ch, o = make(pkg, L, M, N, STR2, REN)
+    cv = 1.0 / REN
+    ch.SOLVERRK3_MOM_tg(0, dt, cv)
+    o.substep(0, dt)
+    gq, gp = ch.download()
+    for c in range(3):
+        assert rel(gq[c], o.q[c]) < 1e-10, ("ns0", c, rel(gq[c], o.q[c]))
+    assert rel(gp, o.pr) < 1e-9
+    assert ch.max_div() < 1e-10
+    for step in range(3):
+        for ns in (1, 2, 3):
+            ch.SOLVERRK3_MOM_tg(ns, dt, cv)
+            o.substep(ns, dt)
+            gq, gp, gh = ch.download(with_history=True)
+            for c in range(3):
+                assert rel(gq[c], o.q[c]) < 1e-10, (step, ns, c, rel(gq[c], o.q[c]))
+            assert rel(gp, o.pr) < 1e-9, (step, ns, rel(gp, o.pr))
+            assert rel(gh[0], o.convh0[0]) < 1e-10
+    assert ch.max_div() < 1e-10
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["tg_16x16x16.npz", "tg_20x18x30.npz"])
+def test_matches_reference_golden(pkg, golden, name):
+    g = golden(name)
+    L, M, N, STR2, REN, dt = (g["params"][i] for i in range(6))
+    L, M, N = int(L), int(M), int(N)
+    ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=STR2), pkg.PLAIN_FP64)
+    ch.upload(g["q_init"], g["pr_init"])
+    ch.SOLVERRK3_MOM_tg(0, dt, 1.0 / REN)
+    gq, gp = ch.download()
+    for c in range(3):
+        assert rel(gq[c], g["q_ns0"][c]) < 1e-10
+    assert rel(gp, g["pr_ns0"]) < 1e-9
+    ch.rk3_step(dt, 1.0 / REN)
+    gq, gp = ch.download()
+    for c in range(3):
+        assert rel(gq[c], g["q_step1"][c]) < 1e-10
+    assert rel(gp, g["pr_step1"]) < 1e-9
+    phi = ch.poisson(g["poisson_rhs"])
+    assert rel(phi, g["poisson_phi"]) < 1e-10
+    ch.close()
+
+
+@pytest.mark.parametrize("L,M,N", [(16, 16, 16), (30, 20, 24), (128, 96, 128)])
+def test_poisson_matches_oracle(pkg, L, M, N):
+    mesh = setup.Mesh(L, M, N, STR2=2.2)
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal((N, M, L))
+    w = (1.0 / mesh.DYFI)[None, :, None]
+    r -= np.sum(r * w) / (np.sum(w) * L * N)
+    o = oracle.OracleTG(mesh, 2800.0)
+    ref = o.poisson(r)
+    ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=2.2), pkg.PLAIN_FP64)
+    phi = ch.poisson(r)
+    assert rel(phi, ref) < 1e-10
+    ch.close()
+
+
+def test_mass_flux_and_walls(pkg):
+    L, M, N, REN, dt = 32, 32, 32, 2800.0, 1e-3
+    mesh, ch, _ = make(pkg, L, M, N, 1.8, REN)
+    ch.SOLVERRK3_MOM_tg(0, dt, 1.0 / REN)
+    q0, _ = ch.download()
+    bulk0 = np.sum(q0[0][:, 1:M + 1] / mesh.DYFI[None, :, None])
+    for _ in range(3):
+        ch.rk3_step(dt, 1.0 / REN)
+    q, p = ch.download()
+    bulk = np.sum(q[0][:, 1:M + 1] / mesh.DYFI[None, :, None])
+    # constant mass flux (iFlowDriven = 1): the explicit increment has zero volume mean
+    # (RHS_CvLpGpS_tg.f90:165-201); the wall-bounded CN y-solve then moves the bulk by O(beta)
+    assert abs(bulk - bulk0) / abs(bulk0) < 1e-9
+    assert np.all(q[:, :, 0, :] == 0) and np.all(q[:, :, M + 1, :] == 0)
+    assert np.all(q[1, :, 1, :] == 0)                    # v on the bottom wall
+    np.testing.assert_array_equal(p[:, 0, :], p[:, 1, :])
+    np.testing.assert_array_equal(p[:, M + 1, :], p[:, M, :])
+    ch.close()
