@@ -22,20 +22,20 @@ template <typename T> __device__ __forceinline__ cpx<T> cmuli(cpx<T> a, int sign
 }
 
 // In-register DFT of length R (2, 3, 4, 5) with exponent sign `sign`.
-template <typename T> __device__ __forceinline__ void bfly(cpx<T> *v, int R, int sign)
+template <typename T, int R> __device__ __forceinline__ void bfly(cpx<T> *v, int sign)
 {
-    if (R == 2) {
+    if constexpr (R == 2) {
         cpx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
-    } else if (R == 4) {
+    } else if constexpr (R == 4) {
         cpx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
         cpx<T> b0 = cadd(v[1], v[3]), b1 = cmuli(csub(v[1], v[3]), sign);
         v[0] = cadd(a0, b0);
         v[2] = csub(a0, b0);
         v[1] = cadd(a1, b1);
         v[3] = csub(a1, b1);
-    } else if (R == 3) {
+    } else if constexpr (R == 3) {
         const T c1 = T(-0.5), s1 = T(sign) * T(0.86602540378443864676372317075294);
         cpx<T> t = cadd(v[1], v[2]);
         cpx<T> d = csub(v[1], v[2]);
@@ -69,26 +69,26 @@ struct FftPlan {
     int radix[16];
 };
 
-// Transforms `nlines` lines of length plan.n stored at a[line * ls + idx] (LDS).  b is a
-// scratch buffer of the same shape.  tw holds exp(-2 pi i m / n), m = 0..n-1 (LDS).
-// Returns the buffer that holds the result (a or b).  All threads of the block must call.
-template <typename T>
-__device__ cpx<T> *fft_lds(cpx<T> *a, cpx<T> *b, int ls, int nlines, const FftPlan &plan,
-                           const cpx<T> *tw, int sign, int tid, int nthr)
+// Largest number of butterflies one thread holds in registers during an in-place pass.
+constexpr int kMaxBfly = 4;
+
+template <typename T, int R>
+__device__ __forceinline__ void fft_pass_inplace(cpx<T> *a, int ls, int nlines, int n, int Ns,
+                                                 const cpx<T> *tw, int sign, int tid, int nthr)
 {
-    const int n = plan.n;
-    int Ns = 1;
-    for (int p = 0; p < plan.npass; ++p) {
-        const int R = plan.radix[p];
-        const int nb = n / R;
-        const int tstride = n / (Ns * R);
-        for (int t = tid; t < nb * nlines; t += nthr) {
+    const int nb = n / R;
+    const int tstride = n / (Ns * R);
+    const int total = nb * nlines;
+    cpx<T> v[kMaxBfly][R];
+#pragma unroll
+    for (int b = 0; b < kMaxBfly; ++b) {
+        const int t = tid + b * nthr;
+        if (t < total) {
             const int line = t / nb;
             const int j = t - line * nb;
             const int k = j % Ns;
             const cpx<T> *src = a + line * ls;
-            cpx<T> *dst = b + line * ls;
-            cpx<T> v[5];
+#pragma unroll
             for (int r = 0; r < R; ++r) {
                 cpx<T> x = src[j + r * nb];
                 if (r > 0) {
@@ -96,19 +96,47 @@ __device__ cpx<T> *fft_lds(cpx<T> *a, cpx<T> *b, int ls, int nlines, const FftPl
                     if (sign > 0) w.y = -w.y;
                     x = cmul(x, w);
                 }
-                v[r] = x;
+                v[b][r] = x;
             }
-            bfly(v, R, sign);
-            const int idxD = (j / Ns) * Ns * R + k;
-            for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
         }
-        __syncthreads();
-        cpx<T> *tmp = a;
-        a = b;
-        b = tmp;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxBfly; ++b) {
+        const int t = tid + b * nthr;
+        if (t < total) {
+            const int line = t / nb;
+            const int j = t - line * nb;
+            const int k = j % Ns;
+            bfly<T, R>(v[b], sign);
+            cpx<T> *dst = a + line * ls;
+            const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place Stockham transform of `nlines` lines of length plan.n stored at
+// a[line * ls + idx] in LDS; tw holds exp(-2 pi i m / n), m = 0..n-1 (LDS).  Each pass reads
+// all butterfly inputs into registers, synchronises, then writes.  Requires
+// nlines * n / (smallest radix) <= kMaxBfly * nthr.  All threads of the block must call.
+template <typename T>
+__device__ void fft_lds_inplace(cpx<T> *a, int ls, int nlines, const FftPlan &plan,
+                                const cpx<T> *tw, int sign, int tid, int nthr)
+{
+    int Ns = 1;
+    for (int p = 0; p < plan.npass; ++p) {
+        const int R = plan.radix[p];
+        switch (R) {
+        case 4: fft_pass_inplace<T, 4>(a, ls, nlines, plan.n, Ns, tw, sign, tid, nthr); break;
+        case 2: fft_pass_inplace<T, 2>(a, ls, nlines, plan.n, Ns, tw, sign, tid, nthr); break;
+        case 3: fft_pass_inplace<T, 3>(a, ls, nlines, plan.n, Ns, tw, sign, tid, nthr); break;
+        default: fft_pass_inplace<T, 5>(a, ls, nlines, plan.n, Ns, tw, sign, tid, nthr); break;
+        }
         Ns *= R;
     }
-    return a;
 }
 
 }  // namespace chs

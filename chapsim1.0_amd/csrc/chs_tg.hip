@@ -72,87 +72,96 @@ template <typename W>
 __global__ __launch_bounds__(256) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
                                                   const double *__restrict__ pr,
                                                   W *__restrict__ h0, W *__restrict__ rhs,
-                                                  double *__restrict__ partial)
+                                                  double *__restrict__ partial, int KC)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z;
     const size_t hs = (size_t)g.N * (g.M + 2) * g.L;   // halo field size
     const size_t cs = (size_t)g.N * g.M * g.L;         // cell field size
     double mf = 0.0;
+    // each thread marches over KC consecutive z-planes so the k-neighbours it reads were
+    // loaded by its own block one plane earlier (cache hits instead of HBM re-reads)
+    const int kbeg = blockIdx.z * KC, kend = min(g.N, kbeg + KC);
+    for (int k = kbeg; k < kend; ++k)
     if (i < g.L && J <= g.M) {
         const int L = g.L;
         const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
         const int kp = (k == g.N - 1) ? 0 : k + 1, km = (k == 0) ? g.N - 1 : k - 1;
         const double *u = q, *v = q + hs, *w = q + 2 * hs;
-        auto U = [&](int ii, int jj, int kk) { return (W)u[hix(g, ii, jj, kk)]; };
-        auto V = [&](int ii, int jj, int kk) { return (W)v[hix(g, ii, jj, kk)]; };
-        auto Wv = [&](int ii, int jj, int kk) { return (W)w[hix(g, ii, jj, kk)]; };
-        auto P = [&](int ii, int jj, int kk) { return (W)pr[hix(g, ii, jj, kk)]; };
-        const W h = W(0.5);
-        const W dxi = (W)g.dxi, dzi = (W)g.dzi, dxqi = (W)g.dxqi, dzqi = (W)g.dzqi;
-        const W wffJ = (W)g.wff[J], wfbJ = (W)g.wfb[J], wffP = (W)g.wff[J + 1], wfbP = (W)g.wfb[J + 1];
-        const W u0 = U(i, J, k), v0 = V(i, J, k), w0 = Wv(i, J, k);
+        // State is FP64.  Sums and differences of state values (and the viscous and
+        // pressure-gradient stencils, whose rows cancel) are formed in FP64; the convection
+        // products run in the work precision W.
+        auto U = [&](int ii, int jj, int kk) { return u[hix(g, ii, jj, kk)]; };
+        auto V = [&](int ii, int jj, int kk) { return v[hix(g, ii, jj, kk)]; };
+        auto Wv = [&](int ii, int jj, int kk) { return w[hix(g, ii, jj, kk)]; };
+        auto P = [&](int ii, int jj, int kk) { return pr[hix(g, ii, jj, kk)]; };
+        const double wffJ = g.wff[J], wfbJ = g.wfb[J], wffP = g.wff[J + 1], wfbP = g.wfb[J + 1];
+        const double u0 = U(i, J, k), v0 = V(i, J, k), w0 = Wv(i, J, k), p0 = P(i, J, k);
         const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)s.dt;
-        const W coev = (W)(s.talp * s.cvisc), coep = (W)(s.talp * s.dt);
+        const double coev = s.talp * s.cvisc, coep = s.talp * s.dt;
         const size_t c0 = cix(g, i, J, k);
         // ---- u (CONVECTION_X.f90:31-59) ----
         {
-            const W coe1 = h * h * dxi, coe2 = (W)g.dyfi[J - 1] * h, coe3 = h * h * dzi;
-            const W uip = U(ip, J, k), uim = U(im, J, k);
-            const W H11 = ((uip + u0) * (uip + u0) - (uim + u0) * (uim + u0)) * coe1;
-            const W H12 = ((V(i, J + 1, k) + V(im, J + 1, k)) * (wffP * U(i, J + 1, k) + wfbP * u0) -
-                           (v0 + V(im, J, k)) * (wffJ * u0 + wfbJ * U(i, J - 1, k))) * coe2;
-            const W ukp = U(i, J, kp), ukm = U(i, J, km);
-            const W H13 = ((Wv(i, J, kp) + Wv(im, J, kp)) * (ukp + u0) - (w0 + Wv(im, J, k)) * (u0 + ukm)) * coe3;
+            const W coe1 = (W)(0.25 * g.dxi), coe2 = (W)(g.dyfi[J - 1] * 0.5), coe3 = (W)(0.25 * g.dzi);
+            const double uip = U(ip, J, k), uim = U(im, J, k), ujp = U(i, J + 1, k), ujm = U(i, J - 1, k);
+            const double ukp = U(i, J, kp), ukm = U(i, J, km);
+            const W a1 = (W)(uip + u0), a2 = (W)(uim + u0);
+            const W H11 = (a1 * a1 - a2 * a2) * coe1;
+            const W H12 = ((W)(V(i, J + 1, k) + V(im, J + 1, k)) * (W)(wffP * ujp + wfbP * u0) -
+                           (W)(v0 + V(im, J, k)) * (W)(wffJ * u0 + wfbJ * ujm)) * coe2;
+            const W H13 = ((W)(Wv(i, J, kp) + Wv(im, J, kp)) * (W)(ukp + u0) -
+                           (W)(w0 + Wv(im, J, k)) * (W)(u0 + ukm)) * coe3;
             const W conv = -(H11 + H12 + H13);
             const W rhsc = tgam * conv + troh * h0[c0];
-            const W lap = (uip - W(2) * u0 + uim) * dxqi + (ukp - W(2) * u0 + ukm) * dzqi +
-                          (U(i, J + 1, k) * (W)g.apvr[J] + u0 * (W)g.acvr[J] + U(i, J - 1, k) * (W)g.amvr[J]);
+            const double lap = (uip - 2.0 * u0 + uim) * g.dxqi + (ukp - 2.0 * u0 + ukm) * g.dzqi +
+                               (ujp * g.apvr[J] + u0 * g.acvr[J] + ujm * g.amvr[J]);
             h0[c0] = conv;
-            const W pgm = (P(i, J, k) - P(im, J, k)) * dxi * coep;
-            const W r = (rhsc + coev * lap) * dt - pgm;
+            const W pgm = (W)((p0 - P(im, J, k)) * g.dxi * coep);
+            const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
             rhs[c0] = r;
-            mf = (double)r / g.dyfi[J - 1];
+            mf += (double)r / g.dyfi[J - 1];
         }
         // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 ----
         if (J >= 2) {
-            const W coe1 = dxi * h, coe2 = (W)g.dyci[J] * h * h, coe31 = dzi * h;
-            const W vip = V(ip, J, k), vim = V(im, J, k), vjp = V(i, J + 1, k), vjm = V(i, J - 1, k);
-            const W vkp = V(i, J, kp), vkm = V(i, J, km);
-            const W H21 = ((wffJ * U(ip, J, k) + wfbJ * U(ip, J - 1, k)) * (vip + v0) -
-                           (wffJ * u0 + wfbJ * U(i, J - 1, k)) * (v0 + vim)) * coe1;
-            const W H22 = ((vjp + v0) * (vjp + v0) - (v0 + vjm) * (v0 + vjm)) * coe2;
-            const W H23 = ((wffJ * Wv(i, J, kp) + wfbJ * Wv(i, J - 1, kp)) * (vkp + v0) -
-                           (wffJ * w0 + wfbJ * Wv(i, J - 1, k)) * (v0 + vkm)) * coe31;
+            const W coe1 = (W)(g.dxi * 0.5), coe2 = (W)(g.dyci[J] * 0.25), coe31 = (W)(g.dzi * 0.5);
+            const double vip = V(ip, J, k), vim = V(im, J, k), vjp = V(i, J + 1, k), vjm = V(i, J - 1, k);
+            const double vkp = V(i, J, kp), vkm = V(i, J, km);
+            const W H21 = ((W)(wffJ * U(ip, J, k) + wfbJ * U(ip, J - 1, k)) * (W)(vip + v0) -
+                           (W)(wffJ * u0 + wfbJ * U(i, J - 1, k)) * (W)(v0 + vim)) * coe1;
+            const W b1 = (W)(vjp + v0), b2 = (W)(v0 + vjm);
+            const W H22 = (b1 * b1 - b2 * b2) * coe2;
+            const W H23 = ((W)(wffJ * Wv(i, J, kp) + wfbJ * Wv(i, J - 1, kp)) * (W)(vkp + v0) -
+                           (W)(wffJ * w0 + wfbJ * Wv(i, J - 1, k)) * (W)(v0 + vkm)) * coe31;
             const W conv = -(H21 + H22 + H23);
             const size_t cc = cs + c0;
             const W rhsc = tgam * conv + troh * h0[cc];
             const int o = g.M + 2;
-            const W lap = (vip - W(2) * v0 + vim) * dxqi + (vkp - W(2) * v0 + vkm) * dzqi +
-                          (vjp * (W)g.apvr[o + J] + v0 * (W)g.acvr[o + J] + vjm * (W)g.amvr[o + J]);
+            const double lap = (vip - 2.0 * v0 + vim) * g.dxqi + (vkp - 2.0 * v0 + vkm) * g.dzqi +
+                               (vjp * g.apvr[o + J] + v0 * g.acvr[o + J] + vjm * g.amvr[o + J]);
             h0[cc] = conv;
-            const W pgm = (P(i, J, k) - P(i, J - 1, k)) * (W)g.dyci[J] * coep;
-            rhs[cc] = (rhsc + coev * lap) * dt - pgm;
+            const W pgm = (W)((p0 - P(i, J - 1, k)) * g.dyci[J] * coep);
+            rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
         }
         // ---- w (CONVECTION_Z.f90:33-69) ----
         {
-            const W coe1 = h * h * dxi, coe2 = (W)g.dyfi[J - 1] * h, coe3 = h * h * dzi;
-            const W wip = Wv(ip, J, k), wim = Wv(im, J, k), wkp = Wv(i, J, kp), wkm = Wv(i, J, km);
-            const W H31 = ((U(ip, J, k) + U(ip, J, km)) * (wip + w0) - (u0 + U(i, J, km)) * (w0 + wim)) * coe1;
-            const W H32 = ((V(i, J + 1, k) + V(i, J + 1, km)) * (wffP * Wv(i, J + 1, k) + wfbP * w0) -
-                           (v0 + V(i, J, km)) * (wffJ * w0 + wfbJ * Wv(i, J - 1, k))) * coe2;
-            const W H33 = ((wkp + w0) * (wkp + w0) - (w0 + wkm) * (w0 + wkm)) * coe3;
+            const W coe1 = (W)(0.25 * g.dxi), coe2 = (W)(g.dyfi[J - 1] * 0.5), coe3 = (W)(0.25 * g.dzi);
+            const double wip = Wv(ip, J, k), wim = Wv(im, J, k), wkp = Wv(i, J, kp), wkm = Wv(i, J, km);
+            const double wjp = Wv(i, J + 1, k), wjm = Wv(i, J - 1, k);
+            const W H31 = ((W)(U(ip, J, k) + U(ip, J, km)) * (W)(wip + w0) -
+                           (W)(u0 + U(i, J, km)) * (W)(w0 + wim)) * coe1;
+            const W H32 = ((W)(V(i, J + 1, k) + V(i, J + 1, km)) * (W)(wffP * wjp + wfbP * w0) -
+                           (W)(v0 + V(i, J, km)) * (W)(wffJ * w0 + wfbJ * wjm)) * coe2;
+            const W c1 = (W)(wkp + w0), c2 = (W)(w0 + wkm);
+            const W H33 = (c1 * c1 - c2 * c2) * coe3;
             const W conv = -(H31 + H32 + H33);
             const size_t cc = 2 * cs + c0;
             const W rhsc = tgam * conv + troh * h0[cc];
             const int o = 2 * (g.M + 2);
-            const W lap = (wip - W(2) * w0 + wim) * dxqi + (wkp - W(2) * w0 + wkm) * dzqi +
-                          (Wv(i, J + 1, k) * (W)g.apvr[o + J] + w0 * (W)g.acvr[o + J] +
-                           Wv(i, J - 1, k) * (W)g.amvr[o + J]);
+            const double lap = (wip - 2.0 * w0 + wim) * g.dxqi + (wkp - 2.0 * w0 + wkm) * g.dzqi +
+                               (wjp * g.apvr[o + J] + w0 * g.acvr[o + J] + wjm * g.amvr[o + J]);
             h0[cc] = conv;
-            const W pgm = (P(i, J, k) - P(i, J, km)) * dzi * coep;
-            rhs[cc] = (rhsc + coev * lap) * dt - pgm;
+            const W pgm = (W)((p0 - P(i, J, km)) * g.dzi * coep);
+            rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
         }
     }
     // block reduction of the mass-flux partial (fixed order -> deterministic)
@@ -191,38 +200,43 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
 template <typename W>
 __global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, int D, int full,
                                               W *__restrict__ rhs, const double *__restrict__ dpg,
-                                              int comp0)
+                                              int NLX)
 {
-    // one block per x-line set: blockIdx.x -> (J, k) line of component (comp0 + blockIdx.z)
+    // block: NLX consecutive x-lines (J, k) of component blockIdx.y
     extern __shared__ unsigned char smem_raw[];
     W *line = reinterpret_cast<W *>(smem_raw);
-    const int c = comp0 + blockIdx.z;
-    const int J = blockIdx.x + 1, k = blockIdx.y;
-    if (c == 1 && J == 1) return;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
+    const int c = blockIdx.y;
+    const int L = g.L, nl = g.M * g.N;
+    const int l0 = blockIdx.x * NLX;
     const size_t cs = (size_t)g.N * g.M * g.L;
-    W *r = rhs + c * cs + cix(g, 0, J, k);
+    W *r = rhs + c * cs + (size_t)l0 * L;     // line l = k*M + J-1 starts at l*L
     const W sub = (c == 0) ? (W)dpg[0] : W(0);
-    for (int i = threadIdx.x; i < g.L; i += blockDim.x) line[i] = r[i] - sub;
+    const int nel = min(NLX, nl - l0) * L;
+    for (int t = threadIdx.x; t < nel; t += blockDim.x) line[t] = r[t] - sub;
     __syncthreads();
-    const int L = g.L;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) {
-        W acc = W(0);
+    for (int t = threadIdx.x; t < nel; t += blockDim.x) {
+        const int ln = t / L, i = t - ln * L;
+        const int J = (l0 + ln) % g.M + 1;
+        if (c == 1 && J == 1) continue;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
+        const W *lp = line + ln * L;
+        W acc;
         if (full) {
+            acc = W(0);
             for (int l = 0; l < L; ++l) {
                 int d = i - l;
                 if (d < 0) d += L;
-                acc += gx[d] * line[l];
+                acc += gx[d] * lp[l];
             }
         } else {
-            acc = gx[0] * line[i];
+            acc = gx[0] * lp[i];
             for (int d = 1; d <= D; ++d) {
                 int a = i + d, b = i - d;
                 if (a >= L) a -= L;
                 if (b < 0) b += L;
-                acc += gx[d] * (line[a] + line[b]);
+                acc += gx[d] * (lp[a] + lp[b]);
             }
         }
-        r[i] = acc;
+        r[t] = acc;
     }
 }
 
@@ -270,205 +284,223 @@ __global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, i
                                               double *__restrict__ q, int c)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.y;
-    if (i >= g.L) return;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    if (i >= g.L || J > g.M) return;
     const size_t cs = (size_t)g.N * g.M * g.L;
     const size_t hs = (size_t)g.N * (g.M + 2) * g.L;
     const W *r = rin + c * cs;
     double *qc = q + c * hs;
     const int M = g.M, w = 2 * D + 1;
-    const int n2i = (c == 1) ? 2 : 1;
-    for (int J = n2i; J <= M; ++J) {
+    if (!(c == 1 && J == 1)) {   // v: rows J >= N2I = 2
         const int lo = J - D < 1 ? 1 : J - D, hi = J + D > M ? M : J + D;
+        const W *bj = by + (size_t)(J - 1) * w;
         W acc = W(0);
-        for (int l = lo; l <= hi; ++l) acc += by[(size_t)(J - 1) * w + (l - J + D)] * r[cix(g, i, l, k)];
+        for (int l = lo; l <= hi; ++l) acc += bj[l - J + D] * r[cix(g, i, l, k)];
         qc[hix(g, i, J, k)] += (double)acc;
         if (c == 0 && J == 1 && vrow1) vrow1[cix(g, i, 1, k)] = acc;
     }
     // BC_WALL_Q (bottom and top walls)
-    qc[hix(g, i, M + 1, k)] = 0.0;
-    qc[hix(g, i, 0, k)] = 0.0;
-    if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
+    if (J == 1) {
+        qc[hix(g, i, 0, k)] = 0.0;
+        if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
+    }
+    if (J == M) qc[hix(g, i, M + 1, k)] = 0.0;
 }
 
 // ---------------------------------------------------------------------------------------
-// Poisson.  k_divg_fftx: per block `nlc` pairs of x-lines.  Divergence (DIVG_tg,
-// DIVG.f90:31-46) of the two real lines of a pair goes to div[] (kept for PRCALC_tg) and
-// into one complex line; after the FFT the two half spectra are separated and scaled by
-// 1/(L N) (the 1/L of FFT99 and the 1/N applied after CFFT99).
+// Poisson.  Spectral data is complex, layout [J][kx][kz] (kz fastest), so z-lines are
+// contiguous and the per-mode y-Thomas reads consecutive kz across a wavefront.
 struct PoisArgs {
     chs::FftPlan px, pz;
     const void *twx, *twz;   // cpx<T>[L], cpx<T>[N]
 };
 
+__device__ __forceinline__ size_t six(const Geo &g, int J, int kx, int kz)
+{
+    return ((size_t)(J - 1) * g.H + kx) * g.N + kz;
+}
+
+// k_divg_fftx: block = row J and TK consecutive z-planes k0..k0+TK-1, i.e. TK x-lines taken
+// as TK/2 complex lines (planes k, k+1 packed as real and imaginary part).  Divergence
+// (DIVG_tg, DIVG.f90:31-46; or the cell field `src` if given) goes to div[] (kept for
+// PRCALC_tg) and into LDS; after the forward FFT the two half spectra are separated, scaled
+// by 1/(L N) (the 1/L of FFT99 and the 1/N applied after CFFT99, possion3d_FFT99.f90:316,
+// :327-334) and written as spec[J][kx][k0..k0+TK-1] (TK consecutive values per kx).
 template <typename T>
 __global__ __launch_bounds__(256) void k_divg_fftx(Geo g, PoisArgs pa, double coe0, const double *__restrict__ q,
                                                    double *__restrict__ div, cpx<T> *__restrict__ spec,
-                                                   int nlc, const double *__restrict__ src)
+                                                   int TK, const double *__restrict__ src)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int L = g.L, M = g.M, H = g.H;
+    const int L = g.L, H = g.H, ls = L + 1;
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem_raw);
     cpx<T> *a = tw + L;
-    cpx<T> *b = a + nlc * L;
-    const size_t hs = (size_t)g.N * (M + 2) * L;
-    const double *u = q, *v = q + hs, *w = q + 2 * hs;
     const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twx);
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
-    const int nlines = M * g.N;   // real lines (J, k)
-    const int pair0 = blockIdx.x * nlc;
-    for (int t = threadIdx.x; t < nlc * L; t += blockDim.x) {
+    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK;
+    const int np = TK / 2;
+    const size_t hs = (size_t)g.N * (g.M + 2) * L;
+    const double *u = q, *v = q + hs, *w = q + 2 * hs;
+    for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
         const int p = t / L, i = t - p * L;
-        T val[2] = {T(0), T(0)};
+        T val[2];
         for (int h = 0; h < 2; ++h) {
-            const int ln = 2 * (pair0 + p) + h;
-            if (ln < nlines && src) {
-                const int k = ln / M, J = ln - k * M + 1;
-                val[h] = (T)src[cix(g, i, J, k)];
-            } else if (ln < nlines) {
-                const int k = ln / M, J = ln - k * M + 1;
+            const int k = k0 + 2 * p + h;
+            double dv;
+            if (src) {
+                dv = src[cix(g, i, J, k)];
+            } else {
                 const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
                 const double d = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
                                  (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * g.dyfi[J - 1] +
                                  (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
-                const double dv = d * coe0;
+                dv = d * coe0;
                 div[cix(g, i, J, k)] = dv;
-                val[h] = (T)dv;
             }
+            val[h] = (T)dv;
         }
-        a[p * L + i] = {val[0], val[1]};
+        a[p * ls + i] = {val[0], val[1]};
     }
     __syncthreads();
-    cpx<T> *res = chs::fft_lds<T>(a, b, L, nlc, pa.px, tw, -1, threadIdx.x, blockDim.x);
+    chs::fft_lds_inplace<T>(a, ls, np, pa.px, tw, -1, threadIdx.x, blockDim.x);
     const T scale = T(1.0 / ((double)L * (double)g.N));
-    for (int t = threadIdx.x; t < nlc * H; t += blockDim.x) {
-        const int p = t / H, kx = t - p * H;
-        const cpx<T> Z = res[p * L + kx];
-        const cpx<T> Zc = res[p * L + ((L - kx) % L)];
+    for (int t = threadIdx.x; t < H * TK; t += blockDim.x) {
+        const int kx = t / TK, ln = t - kx * TK;
+        const int p = ln >> 1;
+        const cpx<T> Z = a[p * ls + kx];
+        const cpx<T> Zc = a[p * ls + ((L - kx) % L)];
         // A = (Z + conj Zc)/2, B = (Z - conj Zc)/(2i)
-        const cpx<T> A = {(Z.x + Zc.x) * T(0.5) * scale, (Z.y - Zc.y) * T(0.5) * scale};
-        const cpx<T> B = {(Z.y + Zc.y) * T(0.5) * scale, (Zc.x - Z.x) * T(0.5) * scale};
-        for (int h = 0; h < 2; ++h) {
-            const int ln = 2 * (pair0 + p) + h;
-            if (ln < nlines) {
-                const int k = ln / M, J = ln - k * M + 1;
-                spec[((size_t)k * M + (J - 1)) * H + kx] = h == 0 ? A : B;
-            }
-        }
+        cpx<T> o;
+        if ((ln & 1) == 0) o = {(Z.x + Zc.x) * T(0.5) * scale, (Z.y - Zc.y) * T(0.5) * scale};
+        else o = {(Z.y + Zc.y) * T(0.5) * scale, (Zc.x - Z.x) * T(0.5) * scale};
+        spec[six(g, J, kx, k0 + ln)] = o;
     }
 }
 
-// Complex z-transform of TB consecutive kx columns at one J (both directions).
+// Complex z-transform of NL contiguous lines [J][kx][:] (both directions).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__restrict__ spec, int TB, int sign)
+__global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__restrict__ spec, int NL, int sign)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int N = g.N, M = g.M, H = g.H;
+    const int N = g.N, ls = N + 1;
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem_raw);
     cpx<T> *a = tw + N;
-    cpx<T> *b = a + TB * N;
     const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twz);
     for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
-    const int kx0 = blockIdx.x * TB, J = blockIdx.y + 1;
-    for (int t = threadIdx.x; t < TB * N; t += blockDim.x) {
-        const int kz = t / TB, c = t - kz * TB;
-        const int kx = kx0 + c;
-        a[c * N + kz] = kx < H ? spec[((size_t)kz * M + (J - 1)) * H + kx] : cpx<T>{T(0), T(0)};
+    const int nlines = g.M * g.H;
+    const int l0 = blockIdx.x * NL;
+    const int nl = min(NL, nlines - l0);
+    cpx<T> *base = spec + (size_t)l0 * N;
+    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+        const int ln = t / N, kz = t - ln * N;
+        a[ln * ls + kz] = base[t];
     }
     __syncthreads();
-    cpx<T> *res = chs::fft_lds<T>(a, b, N, TB, pa.pz, tw, sign, threadIdx.x, blockDim.x);
-    for (int t = threadIdx.x; t < TB * N; t += blockDim.x) {
-        const int kz = t / TB, c = t - kz * TB;
-        const int kx = kx0 + c;
-        if (kx < H) spec[((size_t)kz * M + (J - 1)) * H + kx] = res[c * N + kz];
+    chs::fft_lds_inplace<T>(a, ls, nl, pa.pz, tw, sign, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+        const int ln = t / N, kz = t - ln * N;
+        base[t] = a[ln * ls + kz];
     }
 }
 
 // Per-mode y-Thomas (TDMAIJJ_nonCYC.f90:84-120 on the FFT99 coefficients,
-// possion3d_FFT99.f90:213-231): pp[kz][J][kx] = 1/(b_J - h_{J-1} a_J) precomputed in FP64,
-// h_J = c_J pp_J, and the (kx,kz) = (0,0) mode has RHS(J=1) = 0 and row 1 decoupled.
+// possion3d_FFT99.f90:213-231): pp[J][kx][kz] = 1/(b_J - h_{J-1} a_J) precomputed in FP64,
+// h_J = c_J pp_J; the (kx,kz) = (0,0) mode has RHS(J=1) = 0 and row 1 decoupled.  Loads are
+// issued four rows ahead of the recurrence.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__ amph, const double *__restrict__ apph,
                                                const double *__restrict__ pp, cpx<T> *__restrict__ spec)
 {
-    const int kx = blockIdx.x * 256 + threadIdx.x;
-    const int kz = blockIdx.y;
-    if (kx >= g.H) return;
-    const int M = g.M, H = g.H;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int H = g.H, N = g.N, M = g.M;
+    if (t >= H * N) return;
+    const int kx = t / N, kz = t - kx * N;
     const bool zero = (kx == 0 && kz == 0);
-    cpx<T> *s = spec + (size_t)kz * M * H + kx;
-    const double *p = pp + (size_t)kz * M * H + kx;
-    // forward: G_J = (R_J - a_J G_{J-1}) pp_J
+    const size_t st = (size_t)H * N;
+    cpx<T> *s = spec + (size_t)kx * N + kz;
+    const double *p = pp + (size_t)kx * N + kz;
     double gr = 0.0, gi = 0.0;
-    for (int J = 1; J <= M; ++J) {
-        cpx<T> r = s[(size_t)(J - 1) * H];
-        double rr = (double)r.x, ri = (double)r.y;
-        if (zero && J == 1) { rr = 0.0; ri = 0.0; }
-        const double a = (J == 1) ? 0.0 : amph[J - 1];
-        const double pj = p[(size_t)(J - 1) * H];
-        gr = (rr - a * gr) * pj;
-        gi = (ri - a * gi) * pj;
-        s[(size_t)(J - 1) * H] = {(T)gr, (T)gi};
+    for (int J0 = 1; J0 <= M; J0 += 4) {
+        cpx<T> r[4];
+        double pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (J0 + u <= M) {
+                r[u] = s[(size_t)(J0 + u - 1) * st];
+                pj[u] = p[(size_t)(J0 + u - 1) * st];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int J = J0 + u;
+            if (J <= M) {
+                double rr = (double)r[u].x, ri = (double)r[u].y;
+                if (zero && J == 1) { rr = 0.0; ri = 0.0; }
+                const double a = (J == 1) ? 0.0 : amph[J - 1];
+                gr = (rr - a * gr) * pj[u];
+                gi = (ri - a * gi) * pj[u];
+                s[(size_t)(J - 1) * st] = {(T)gr, (T)gi};
+            }
+        }
     }
-    // backward: X_J = G_J - h_J X_{J+1}
     double xr = gr, xi = gi;
-    for (int J = M - 1; J >= 1; --J) {
-        const double c = (zero && J == 1) ? 0.0 : apph[J - 1];
-        const double h = c * p[(size_t)(J - 1) * H];
-        cpx<T> gg = s[(size_t)(J - 1) * H];
-        xr = (double)gg.x - h * xr;
-        xi = (double)gg.y - h * xi;
-        s[(size_t)(J - 1) * H] = {(T)xr, (T)xi};
+    for (int J0 = M - 1; J0 >= 1; J0 -= 4) {
+        cpx<T> gg[4];
+        double pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (J0 - u >= 1) {
+                gg[u] = s[(size_t)(J0 - u - 1) * st];
+                pj[u] = p[(size_t)(J0 - u - 1) * st];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int J = J0 - u;
+            if (J >= 1) {
+                const double c = (zero && J == 1) ? 0.0 : apph[J - 1];
+                const double h = c * pj[u];
+                xr = (double)gg[u].x - h * xr;
+                xi = (double)gg[u].y - h * xi;
+                s[(size_t)(J - 1) * st] = {(T)xr, (T)xi};
+            }
+        }
     }
 }
 
-// Inverse real x-transform of pairs of lines -> phi (halo field, rows 1..M) and the
-// Neumann ghost rows of BC_WALL_DPH_tg (BC_WALL.f90:369-386).
+// Inverse real x-transform of TK z-planes at row J -> phi (halo field) and the Neumann
+// ghost rows of BC_WALL_DPH_tg (BC_WALL.f90:369-386).  Z = A + i B with the Hermitian
+// extension; imaginary parts at kx = 0 and L/2 are not used (FFT99.f90:84-86).
 template <typename T>
 __global__ __launch_bounds__(256) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
-                                                  double *__restrict__ phi, int nlc)
+                                                  double *__restrict__ phi, int TK)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int L = g.L, M = g.M, H = g.H;
+    const int L = g.L, M = g.M, H = g.H, ls = L + 1;
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem_raw);
     cpx<T> *a = tw + L;
-    cpx<T> *b = a + nlc * L;
     const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twx);
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
-    const int nlines = M * g.N;
-    const int pair0 = blockIdx.x * nlc;
-    for (int t = threadIdx.x; t < nlc * L; t += blockDim.x) {
-        const int p = t / L, kx = t - p * L;
-        // Z[kx] = A[kx] + i B[kx] with Hermitian extension; imaginary parts at kx = 0, L/2
-        // are not used (FFT99.f90:84-86).
-        const int src = kx < H ? kx : L - kx;
-        cpx<T> AB[2];
-        for (int h = 0; h < 2; ++h) {
-            const int ln = 2 * (pair0 + p) + h;
-            cpx<T> z = {T(0), T(0)};
-            if (ln < nlines) {
-                const int k = ln / M, J = ln - k * M + 1;
-                z = spec[((size_t)k * M + (J - 1)) * H + src];
-                if (src == 0 || src == L / 2) z.y = T(0);
-                if (kx >= H) z.y = -z.y;
-            }
-            AB[h] = z;
-        }
-        a[p * L + kx] = {AB[0].x - AB[1].y, AB[0].y + AB[1].x};
+    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK;
+    const int np = TK / 2;
+    for (int t = threadIdx.x; t < H * np; t += blockDim.x) {
+        const int kx = t / np, p = t - kx * np;
+        cpx<T> A = spec[six(g, J, kx, k0 + 2 * p)];
+        cpx<T> B = spec[six(g, J, kx, k0 + 2 * p + 1)];
+        if (kx == 0 || kx == L / 2) { A.y = T(0); B.y = T(0); }
+        a[p * ls + kx] = {A.x - B.y, A.y + B.x};
+        if (kx > 0 && kx < L / 2) a[p * ls + L - kx] = {A.x + B.y, B.x - A.y};
     }
     __syncthreads();
-    cpx<T> *res = chs::fft_lds<T>(a, b, L, nlc, pa.px, tw, +1, threadIdx.x, blockDim.x);
-    for (int t = threadIdx.x; t < nlc * L; t += blockDim.x) {
+    chs::fft_lds_inplace<T>(a, ls, np, pa.px, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
         const int p = t / L, i = t - p * L;
+        const cpx<T> z = a[p * ls + i];
         for (int h = 0; h < 2; ++h) {
-            const int ln = 2 * (pair0 + p) + h;
-            if (ln < nlines) {
-                const int k = ln / M, J = ln - k * M + 1;
-                const double val = h == 0 ? (double)res[p * L + i].x : (double)res[p * L + i].y;
-                phi[hix(g, i, J, k)] = val;
-                if (J == 1) phi[hix(g, i, 0, k)] = val;
-                if (J == M) phi[hix(g, i, M + 1, k)] = val;
-            }
+            const int k = k0 + 2 * p + h;
+            const double val = h == 0 ? (double)z.x : (double)z.y;
+            phi[hix(g, i, J, k)] = val;
+            if (J == 1) phi[hix(g, i, 0, k)] = val;
+            if (J == M) phi[hix(g, i, M + 1, k)] = val;
         }
     }
 }
@@ -616,6 +648,10 @@ struct chs_ctx {
     // profiling of the dominant kernel
     bool prof = false;
     std::string prof_only;
+    // hipGraph of one RK3 step (three substeps), re-captured when dt or cvisc change
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    double gdt = 0.0, gcv = 0.0;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 };
 
@@ -817,16 +853,19 @@ int upload_twiddles(chs_ctx *c, int n, void **dst)
     return 0;
 }
 
-int nlc_for(int L) { int n = 1024 / L; return n < 1 ? 1 : n; }
-
-// kx columns per z-FFT block: largest power of two <= 16 whose twiddles + two buffers fit
-// in 64 KiB of LDS.
-int tb_for(int N, size_t csz)
+// z-planes per x-transform block: TK x-lines = TK/2 complex lines, (TK/2)*L <= 2048 complex
+// values (the in-place FFT's register budget), TK a power of two dividing N, at most 16
+// (16 complex values = 256 contiguous bytes per kx on the spectral write).
+int tk_for(int L, int N)
 {
-    int tb = 16;
-    while (tb > 1 && (size_t)(N + 2 * tb * N) * csz > 65536) tb >>= 1;
-    return tb;
+    int tk = 16;
+    while (tk > 2 && (N % tk != 0 || (tk / 2) * L > 2048)) tk >>= 1;
+    return tk;
 }
+// contiguous z-lines per z-transform block
+int nlz_for(int N) { int n = 2048 / N; return n < 1 ? 1 : n; }
+constexpr int kKC = 8;   // z-planes marched per conv_rhs thread
+
 
 struct EvScope {
     chs_ctx *c;
@@ -848,104 +887,81 @@ struct EvScope {
     }
 };
 
-template <typename W>
-int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
+template <typename T>
+void poisson_launch(chs_ctx *c, double coe0, const double *src)
 {
-    using T = W;
-    if (int e = build_cn(c, ns, dt, cvisc)) return e;
     const Geo &g = c->geo;
-    const Sub s = {c->tgam[ns], c->troh[ns], c->talp[ns], dt, cvisc};
     const int L = c->L, M = c->M, N = c->N, H = c->H;
-    const size_t cs = (size_t)N * M * L;
-    W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
-    dim3 blk(64, 4), grd((L + 63) / 64, (M + 3) / 4, N);
-    {
-        EvScope ev(c, "conv_rhs");
-        hipLaunchKernelGGL(k_conv_rhs<W>, grd, blk, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial);
-    }
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, c->npartial,
-                       1.0 / c->volm, c->scal);
-    CnTab &t = c->cn[ns];
-    {
-        EvScope ev(c, "cn_x");
-        hipLaunchKernelGGL(k_cn_x<W>, dim3(M, N, 3), dim3(256), sizeof(W) * L, c->stream, g,
-                           (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, 0);
-    }
-    {
-        EvScope ev(c, "cn_z");
-        for (int comp = 0; comp < 3; ++comp)
-            hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz,
-                               rhs, rhs2, comp);
-    }
-    {
-        EvScope ev(c, "cn_y");
-        for (int comp = 0; comp < 3; ++comp)
-            hipLaunchKernelGGL(k_cn_y<W>, dim3((L + 63) / 64, N), dim3(64), 0, c->stream, g,
-                               (const W *)t.by[comp], t.Dy[comp], rhs2, comp == 0 ? rhs2 + cs : nullptr,
-                               c->q, comp);
-    }
     PoisArgs pa{c->px, c->pz, c->twx, c->twz};
-    const int nlc = nlc_for(L);
-    const int npairs = (M * N + 1) / 2;
-    const size_t shx = sizeof(cpx<T>) * (L + 2 * nlc * L);
+    const int TK = tk_for(L, N), NL = nlz_for(N);
+    const size_t shx = sizeof(cpx<T>) * (L + (TK / 2) * (L + 1));
+    const size_t shz = sizeof(cpx<T>) * (N + NL * (N + 1));
+    cpx<T> *spec = (cpx<T> *)c->spec;
     {
         EvScope ev(c, "divg_fftx");
-        hipLaunchKernelGGL(k_divg_fftx<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa,
-                           1.0 / (dt * c->talp[ns]), c->q, c->div, (cpx<T> *)c->spec, nlc,
-                           (const double *)nullptr);
+        hipLaunchKernelGGL(k_divg_fftx<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa, coe0, c->q,
+                           c->div, spec, TK, src);
     }
-    const int TB = tb_for(N, sizeof(cpx<T>));
-    const size_t shz = sizeof(cpx<T>) * (N + 2 * TB * N);
     {
         EvScope ev(c, "fftz_fwd");
-        hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
-                           (cpx<T> *)c->spec, TB, -1);
+        hipLaunchKernelGGL(k_fftz<T>, dim3((M * H + NL - 1) / NL), dim3(256), shz, c->stream, g, pa, spec, NL, -1);
     }
     {
         EvScope ev(c, "ytdma");
-        hipLaunchKernelGGL(k_ytdma<T>, dim3((H + 255) / 256, N), dim3(256), 0, c->stream, g, c->d_amph,
-                           c->d_apph, c->pp, (cpx<T> *)c->spec);
+        hipLaunchKernelGGL(k_ytdma<T>, dim3((H * N + 255) / 256), dim3(256), 0, c->stream, g, c->d_amph,
+                           c->d_apph, c->pp, spec);
     }
     {
         EvScope ev(c, "fftz_inv");
-        hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
-                           (cpx<T> *)c->spec, TB, +1);
+        hipLaunchKernelGGL(k_fftz<T>, dim3((M * H + NL - 1) / NL), dim3(256), shz, c->stream, g, pa, spec, NL, +1);
     }
     {
         EvScope ev(c, "fftx_inv");
-        hipLaunchKernelGGL(k_fftx_inv<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa,
-                           (const cpx<T> *)c->spec, c->phi, nlc);
+        hipLaunchKernelGGL(k_fftx_inv<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa,
+                           (const cpx<T> *)spec, c->phi, TK);
     }
+}
+
+template <typename W>
+int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
+{
+    if (int e = build_cn(c, ns, dt, cvisc)) return e;
+    const Geo &g = c->geo;
+    const Sub s = {c->tgam[ns], c->troh[ns], c->talp[ns], dt, cvisc};
+    const int L = c->L, M = c->M, N = c->N;
+    const size_t cs = (size_t)N * M * L;
+    W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
+    const dim3 blk(64, 4), grd((L + 63) / 64, (M + 3) / 4, N);
+    const dim3 grdc((L + 63) / 64, (M + 3) / 4, (N + kKC - 1) / kKC);
+    const int npart = grdc.x * grdc.y * grdc.z;
+    {
+        EvScope ev(c, "conv_rhs");
+        hipLaunchKernelGGL(k_conv_rhs<W>, grdc, blk, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, kKC);
+    }
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, npart, 1.0 / c->volm, c->scal);
+    CnTab &t = c->cn[ns];
+    {
+        const int NLX = (1024 / L) < 1 ? 1 : 1024 / L;
+        EvScope ev(c, "cn_x");
+        hipLaunchKernelGGL(k_cn_x<W>, dim3((M * N + NLX - 1) / NLX, 3), dim3(256), sizeof(W) * NLX * L, c->stream,
+                           g, (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, NLX);
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+        EvScope ev(c, "cn_z");
+        hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz, rhs, rhs2, comp);
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+        EvScope ev(c, "cn_y");
+        hipLaunchKernelGGL(k_cn_y<W>, grd, blk, 0, c->stream, g, (const W *)t.by[comp], t.Dy[comp], rhs2,
+                           comp == 0 ? rhs2 + cs : nullptr, c->q, comp);
+    }
+    poisson_launch<W>(c, 1.0 / (dt * c->talp[ns]), nullptr);
     {
         EvScope ev(c, "update");
         hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr);
     }
     HIPCHK(c, hipGetLastError());
     return 0;
-}
-
-// Transforms, y-Thomas and inverse transforms of the cell field in c->div -> c->phi.
-template <typename T>
-void poisson_launch(chs_ctx *c, const double *src)
-{
-    const Geo &g = c->geo;
-    const int L = c->L, M = c->M, N = c->N, H = c->H;
-    PoisArgs pa{c->px, c->pz, c->twx, c->twz};
-    const int nlc = nlc_for(L);
-    const int npairs = (M * N + 1) / 2;
-    const size_t shx = sizeof(cpx<T>) * (L + 2 * nlc * L);
-    hipLaunchKernelGGL(k_divg_fftx<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa, 1.0,
-                       c->q, c->div, (cpx<T> *)c->spec, nlc, src);
-    const int TB = tb_for(N, sizeof(cpx<T>));
-    const size_t shz = sizeof(cpx<T>) * (N + 2 * TB * N);
-    hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
-                       (cpx<T> *)c->spec, TB, -1);
-    hipLaunchKernelGGL(k_ytdma<T>, dim3((H + 255) / 256, N), dim3(256), 0, c->stream, g, c->d_amph, c->d_apph,
-                       c->pp, (cpx<T> *)c->spec);
-    hipLaunchKernelGGL(k_fftz<T>, dim3((H + TB - 1) / TB, M), dim3(256), shz, c->stream, g, pa,
-                       (cpx<T> *)c->spec, TB, +1);
-    hipLaunchKernelGGL(k_fftx_inv<T>, dim3((npairs + nlc - 1) / nlc), dim3(256), shx, c->stream, g, pa,
-                       (const cpx<T> *)c->spec, c->phi, nlc);
 }
 
 template <typename T>
@@ -958,7 +974,7 @@ int poisson_impl(chs_ctx *c, const double *rhs, double *phi)
     HIPCHK(c, hipMalloc(&src, sizeof(double) * cs));
     hipError_t e = hipMemcpyAsync(src, rhs, sizeof(double) * cs, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        poisson_launch<T>(c, src);
+        poisson_launch<T>(c, 1.0, src);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1087,8 +1103,9 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
     if (mode != CHS_PLAIN_FP64 && mode != CHS_MIXED) return fail(nullptr, CHS_E_ARG, "bad mode");
     const int L = cf->L, M = cf->M, N = cf->N;
     chs::FftPlan px, pz;
-    if (L < 4 || M < 4 || N < 4 || (L & 1) || (N & 1) || !fft_ok(L, px) || !fft_ok(N, pz))
-        return fail(nullptr, CHS_E_ARG, "L, N must be even with factors 2,3,5 only; M >= 4");
+    if (L < 4 || M < 4 || N < 4 || (L & 1) || (N & 1) || L > 2048 || N > 2048 || !fft_ok(L, px) ||
+        !fft_ok(N, pz))
+        return fail(nullptr, CHS_E_ARG, "L, N must be even, <= 2048, with factors 2,3,5 only; M >= 4");
     chs_ctx *c = new chs_ctx();
     c->L = L; c->M = M; c->N = N; c->H = L / 2 + 1; c->mode = mode;
     c->px = px; c->pz = pz;
@@ -1155,7 +1172,7 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
     CK(hipMalloc(&c->rhs2, ws * 3 * cs));
     CK(hipMalloc(&c->spec, 2 * ws * (size_t)c->H * M * N));
     CK(hipMalloc(&c->pp, sizeof(double) * (size_t)c->H * M * N));
-    c->npartial = ((L + 63) / 64) * ((M + 3) / 4) * N;
+    c->npartial = ((L + 63) / 64) * ((M + 3) / 4) * N;   // >= conv_rhs and k_cfl_div grids
     CK(hipMalloc(&c->partial, sizeof(double) * c->npartial));
     CK(hipMalloc(&c->scal, sizeof(double) * 16));
     CK(hipMemsetAsync(c->q, 0, sizeof(double) * 3 * hs, c->stream));
@@ -1179,7 +1196,7 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
                     if (kx == 0 && kz == 0 && J == 1) { a = 0.0; cc = 0.0; b = c->acph[0] + (-c->ak1[0] / 1.0 - c->ak3[0]); }
                     double p = (J == 1) ? 1.0 / b : 1.0 / (b - hprev * a);
                     hprev = cc * p;
-                    pp[((size_t)kz * M + (J - 1)) * H + kx] = p;
+                    pp[((size_t)(J - 1) * H + kx) * N + kz] = p;
                 }
             }
         CK(hipMemcpyAsync(c->pp, pp.data(), sizeof(double) * pp.size(), hipMemcpyHostToDevice, c->stream));
@@ -1208,6 +1225,7 @@ void chs_destroy(chs_ctx *c)
             if (b) (void)hipFree(b);
     }
     for (auto &u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1267,8 +1285,47 @@ int chs_substep_tg(chs_ctx *c, int ns, double dt, double cvisc)
 
 int chs_step_tg(chs_ctx *c, double dt, double cvisc)
 {
-    for (int ns = 1; ns <= 3; ++ns)
-        if (int e = chs_substep_tg(c, ns, dt, cvisc)) return e;
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    if (!c->use_graph || c->prof) {
+        for (int ns = 1; ns <= 3; ++ns)
+            if (int e = chs_substep_tg(c, ns, dt, cvisc)) return e;
+        return 0;
+    }
+    if (!(dt > 0) || !(cvisc > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
+    if (!c->have_fields) return fail(c, CHS_E_STATE, "no fields uploaded");
+    if (!c->gexec || c->gdt != dt || c->gcv != cvisc) {
+        // tables first: building them synchronises the stream, which capture forbids
+        for (int ns = 1; ns <= 3; ++ns)
+            if (int e = build_cn(c, ns, dt, cvisc)) return e;
+        if (c->gexec) {
+            HIPCHK(c, hipGraphExecDestroy(c->gexec));
+            c->gexec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int e = 0;
+        for (int ns = 1; ns <= 3 && !e; ++ns)
+            e = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt, cvisc) : substep_impl<double>(c, ns, dt, cvisc);
+        hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (e) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return e;
+        }
+        if (ce != hipSuccess) return fail(c, CHS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        ce = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ce != hipSuccess) return fail(c, CHS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ce));
+        c->gdt = dt;
+        c->gcv = cvisc;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    return 0;
+}
+
+int chs_set_graph(chs_ctx *c, int enable)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    c->use_graph = enable != 0;
     return 0;
 }
 

@@ -40,6 +40,7 @@ SIGNATURES = {
     "chs_download_fields": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "chs_substep_tg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
     "chs_step_tg": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "chs_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_poisson": (C.c_int, [C.c_void_p, _dp, _dp]),
     "chs_cfl_tg": (C.c_int, [C.c_void_p, C.c_double, _dp]),
     "chs_max_div": (C.c_int, [C.c_void_p, _dp]),
@@ -47,6 +48,11 @@ SIGNATURES = {
     "chs_device_fields": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp)]),
     "chs_profile_reset": (C.c_int, [C.c_void_p, C.c_char_p]),
     "chs_profile_get": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_int)]),
+    "chs_prop_table_setup": (C.c_int, [C.c_int, _dp, C.c_double, _dp, _dp, C.POINTER(C.c_int), _dp]),
+    "chs_prop_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "chs_prop_destroy": (None, [C.c_void_p]),
+    "chs_prop_last_error": (C.c_char_p, [C.c_void_p]),
+    "chs_prop_update_from_dh": (C.c_int, [C.c_void_p, C.c_long, _dp, _dp]),
 }
 
 
@@ -160,6 +166,10 @@ class ChannelTG:
 
     def rk3_step(self, dt: float, cvisc: float):
         _check(lib().chs_step_tg(self._h, float(dt), float(cvisc)), self._h)
+
+    def set_graph(self, enable: bool):
+        """Replay rk3_step from a hipGraph captured per (dt, cvisc)."""
+        _check(lib().chs_set_graph(self._h, 1 if enable else 0), self._h)
 
     def poisson(self, rhs):
         L, M, N = self.L, self.M, self.N

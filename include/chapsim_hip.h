@@ -27,7 +27,9 @@ enum {
     CHS_E_HIP = -2,        /* HIP runtime error                                          */
     CHS_E_NOMEM = -3,      /* device allocation failed                                   */
     CHS_E_NONFINITE = -4,  /* non-finite value found by chs_check_finite                 */
-    CHS_E_STATE = -5       /* call out of order (e.g. no fields uploaded)                */
+    CHS_E_STATE = -5,      /* call out of order (e.g. no fields uploaded)                */
+    CHS_E_RANGE = -6       /* value outside the property table (the reference STOPs,
+                              thermo_properties_table_searching.f90:688-701)               */
 };
 
 /* Arithmetic form of the substep. */
@@ -98,8 +100,12 @@ int chs_download_fields(chs_ctx *ctx, double *q, double *pr, double *convh0);
  * the context's stream; returns without waiting. */
 int chs_substep_tg(chs_ctx *ctx, int ns, double dt, double cvisc);
 
-/* One RK3 step: NS = 1, 2, 3 (SOLVE.f90:114-134, TG branch). */
+/* One RK3 step: NS = 1, 2, 3 (SOLVE.f90:114-134, TG branch).  With graphs enabled the
+ * step's launches are captured once per (dt, cvisc) into a hipGraph and replayed. */
 int chs_step_tg(chs_ctx *ctx, double dt, double cvisc);
+
+/* Enable (1) or disable (0, default) hipGraph replay of chs_step_tg. */
+int chs_set_graph(chs_ctx *ctx, int enable);
 
 /* Standalone pressure Poisson solve of FFT99_POIS3D_periodicxz (possion3d_FFT99.f90:
  * 271-505) on the context's mesh.  rhs: RHSLLPHI(1:L,1:M,1:N); phi: DPH(1:L,1:M,1:N)
@@ -122,6 +128,41 @@ int chs_device_fields(chs_ctx *ctx, double **q, double **pr);
  * average device time (ms) of one named kernel's launches and their count. */
 int chs_profile_reset(chs_ctx *ctx, const char *only);
 int chs_profile_get(chs_ctx *ctx, const char *kernel, double *avg_ms, int *count);
+
+/* ---- Thermophysical property tables (heated channel, table mode) ---------------------- */
+
+/* Nondimensional property table and cubic-spline coefficients as the reference's set-up
+ * holds them (thermal_info, modules.f90:520-690): LIST_DH, LIST_H, LIST_T, LIST_D, LIST_M,
+ * LIST_K, LIST_CP, LIST_B (n each, sorted by T) and SplineCoeff_{DHH,HT,HD,HM,HK,HCp,HB}_
+ * {B,C,D}; imin_dh .. imax_h are IMIN_DH, IMAX_DH, IMIN_H, IMAX_H (1-based). */
+typedef struct {
+    int n, imin_dh, imax_dh, imin_h, imax_h;
+    const double *dh, *h, *t, *d, *m, *k, *cp, *b;
+    const double *dhh_b, *dhh_c, *dhh_d, *ht_b, *ht_c, *ht_d, *hd_b, *hd_c, *hd_d;
+    const double *hm_b, *hm_c, *hm_d, *hk_b, *hk_c, *hk_d, *hcp_b, *hcp_c, *hcp_d;
+    const double *hb_b, *hb_c, *hb_d;
+} chs_prop_table;
+
+typedef struct chs_prop chs_prop;
+
+/* Host restatement of Property_Table_Nondimensionalization + SPLINE_COEFFICIENTS_FOR_DH/H
+ * (thermo_properties_table_searching.f90:1206-1346, :977-1129, CUBIC_SPLINE :842-936).
+ * rows: n x 8 row-major dimensional table as the file holds it (P H T D M K CP BETA).
+ * Out: list[8*n] = DH,H,T,D,M,K,CP,B; spl[21*n] = DHH,HT,HD,HM,HK,HCp,HB x (B,C,D);
+ * idx[4] = IMIN_DH, IMAX_DH, IMIN_H, IMAX_H; ref[7] = T0,H0,D0,M0,K0,CP0,B0. */
+int chs_prop_table_setup(int n, const double *rows, double T0, double *list, double *spl, int *idx,
+                         double *ref);
+
+/* Copy a table to the current GPU. */
+int chs_prop_create(const chs_prop_table *t, chs_prop **out);
+void chs_prop_destroy(chs_prop *p);
+const char *chs_prop_last_error(const chs_prop *p);
+
+/* THERM_PROP_UPDATE_FROM_DH (THERM_PROP_UPDATE.f90:14-67, table branch) for n values of
+ * rho*h: out[7*n] = H | T | D | M | K | Cp | B, bit-identical to the reference (including
+ * its D <- H->M, M <- H->K, K <- H->D spline assignment, lines 49-51).  Returns CHS_E_RANGE
+ * if a value lies outside the table by more than 1e-8.  Host arrays; synchronous. */
+int chs_prop_update_from_dh(chs_prop *p, long n, const double *dh, double *out);
 
 #ifdef __cplusplus
 }

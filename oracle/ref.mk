@@ -43,7 +43,7 @@ SRC5 := dsyev2 dsyevc3 dsyevd3 dsyevh3 dsyevj3 dsyevq3 dsyevv3 dsytrd3 slvsec3 s
 OBJS := $(addprefix $(OBJDIR)/,$(addsuffix .o,$(SRC1) $(SRC2) $(SRC3))) $(OBJDIR)/fftpack.o \
         $(addprefix $(OBJDIR)/,$(addsuffix .o,$(SRC5)))
 
-all: $(OUT)/chapsim $(OUT)/ref_harness
+all: $(OUT)/chapsim $(OUT)/ref_harness $(OUT)/chapsim_dropin
 
 $(OBJDIR):
 	mkdir -p $(OBJDIR)
@@ -64,6 +64,17 @@ $(OUT)/chapsim: $(OBJDIR)/.stamp
 $(OUT)/ref_harness: $(OBJDIR)/.stamp oracle/ref_harness.f90
 	$(FC) $(FFLAGS) -c oracle/ref_harness.f90 -o $(OBJDIR)/ref_harness.o
 	$(FC) -o $@ $(OBJDIR)/ref_harness.o $(OBJS) $(LDFLAGS)
+
+# The reference program with SOLVERRK3_MOM_tg.o replaced by the GPU drop-in
+# (chapsim1.0_amd/fortran/solverrk3_mom_tg_hip.f90 over libchapsim_hip.so): demonstrates and
+# tests the Fortran boundary end to end.
+PKG     := chapsim1.0_amd
+DROPOBJ := $(filter-out $(OBJDIR)/SOLVERRK3_MOM_tg.o,$(OBJS))
+$(OUT)/chapsim_dropin: $(OBJDIR)/.stamp $(PKG)/fortran/chapsim_hip_mod.f90 $(PKG)/fortran/solverrk3_mom_tg_hip.f90 $(PKG)/libchapsim_hip.so
+	$(FC) $(FFLAGS) -c $(PKG)/fortran/chapsim_hip_mod.f90 -o $(OBJDIR)/chapsim_hip_mod.o
+	$(FC) $(FFLAGS) -c $(PKG)/fortran/solverrk3_mom_tg_hip.f90 -o $(OBJDIR)/solverrk3_mom_tg_hip.o
+	$(FC) -o $@ $(OBJDIR)/DNS_THERMAL.o $(DROPOBJ) $(OBJDIR)/chapsim_hip_mod.o \
+	    $(OBJDIR)/solverrk3_mom_tg_hip.o -L$(PKG) -lchapsim_hip -Wl,-rpath,'$$ORIGIN/../../$(PKG)' $(LDFLAGS)
 
 clean:
 	rm -rf $(OUT)

@@ -19,12 +19,18 @@
 !                           dumps DPH_tg rows 1..N2DO
 !   rhsmom NS DT FILE       conv + RHS_CvLpGpS_tg/MOMFA_tg for IDR=1..3 (as SOLVERRK3_MOM_tg
 !                           does), dumps Q_tg rows 0..N2DO+1 and RHS_tg rows 1..N2DO
+!   thermoinit              rank 0: thermal_init (table read, sort, nondimensionalisation,
+!                           splines; Flow_State.f90:16-95), needs iThermoDynamics = 1
+!   dumptable FILE          rank 0: nondimensional table and spline coefficients
+!   proplookup FIN FOUT     rank 0: THERM_PROP_UPDATE_FROM_DH for every DH in FIN (int32 n,
+!                           then n float64); FOUT = H, T, D, M, K, Cp, B (7 x n float64)
 ! Must run on 2 or more MPI ranks (the reference deadlocks on one, INTFC_VARS.f90:57-79).
 !-----------------------------------------------------------------------------------------------
 PROGRAM ref_harness
     USE init_info
     USE mesh_info
     USE flow_info
+    USE thermal_info
     USE FFT99_info1
     IMPLICIT NONE
 
@@ -55,10 +61,10 @@ PROGRAM ref_harness
     CALL BCAST_LAMPOISLPROF
     IF (MYID == 0) CALL TDMA_COEF
     CALL BCAST_TDMA_COEF
-    CALL FFT99_POIS3D_INIT(ITG)
+    IF (TgFlowFlg) CALL FFT99_POIS3D_INIT(ITG)
     CALL MPI_BARRIER(ICOMM, IERROR)
 
-    CONVH0_tg = 0.0_WP
+    IF (TgFlowFlg) CONVH0_tg = 0.0_WP
     CVISC = 1.0_WP / REN
 
     OPEN(77, FILE = 'harness.cmd', STATUS = 'old', ACTION = 'read')
@@ -111,6 +117,14 @@ PROGRAM ref_harness
             END DO
             CALL H_DUMPQ(TRIM(fname))
             CALL H_DUMP3(TRIM(fname) // '.rhs', RHS_tg, 1)
+        CASE ('thermoinit')
+            IF (MYID == 0) CALL thermal_init
+        CASE ('dumptable')
+            READ(line, *) cmd, fname
+            IF (MYID == 0) CALL H_DUMPTABLE(TRIM(fname))
+        CASE ('proplookup')
+            READ(line, *) cmd, fname2, fname
+            IF (MYID == 0) CALL H_PROPLOOKUP(TRIM(fname2), TRIM(fname))
         CASE DEFAULT
             IF (MYID == 0) WRITE(*, *) 'ref_harness: unknown command ', TRIM(cmd)
             CALL MPI_ABORT(ICOMM, 2, IERROR)
@@ -222,6 +236,43 @@ CONTAINS
         CALL H_OPENRANK(fn, 80)
         WRITE(80) A(:, 1:N2DO(MYID), :)
         CLOSE(80)
+    END SUBROUTINE
+
+    SUBROUTINE H_DUMPTABLE(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        OPEN(78, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'replace', &
+             ACTION = 'write')
+        WRITE(78) INT(N_LIST, 4), INT(IMIN_DH, 4), INT(IMAX_DH, 4), INT(IMIN_H, 4), INT(IMAX_H, 4)
+        WRITE(78) LIST_DH, LIST_H, LIST_T, LIST_D, LIST_M, LIST_K, LIST_CP, LIST_B
+        WRITE(78) SplineCoeff_DHH_B, SplineCoeff_DHH_C, SplineCoeff_DHH_D
+        WRITE(78) SplineCoeff_HT_B, SplineCoeff_HT_C, SplineCoeff_HT_D
+        WRITE(78) SplineCoeff_HD_B, SplineCoeff_HD_C, SplineCoeff_HD_D
+        WRITE(78) SplineCoeff_HM_B, SplineCoeff_HM_C, SplineCoeff_HM_D
+        WRITE(78) SplineCoeff_HK_B, SplineCoeff_HK_C, SplineCoeff_HK_D
+        WRITE(78) SplineCoeff_HCp_B, SplineCoeff_HCp_C, SplineCoeff_HCp_D
+        WRITE(78) SplineCoeff_HB_B, SplineCoeff_HB_C, SplineCoeff_HB_D
+        WRITE(78) T0, H0, D0, M0, K0, CP0, B0
+        CLOSE(78)
+    END SUBROUTINE
+
+    SUBROUTINE H_PROPLOOKUP(fin, fout)
+        CHARACTER(*), INTENT(IN) :: fin, fout
+        INTEGER(4) :: n, i
+        REAL(WP), ALLOCATABLE :: dh(:), o(:, :)
+        OPEN(79, FILE = fin, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', ACTION = 'read')
+        READ(79) n
+        ALLOCATE(dh(n), o(n, 7))
+        READ(79) dh
+        CLOSE(79)
+        DO i = 1, n
+            CALL THERM_PROP_UPDATE_FROM_DH(dh(i), o(i, 1), o(i, 2), o(i, 3), o(i, 4), o(i, 5), &
+                                           o(i, 6), o(i, 7))
+        END DO
+        OPEN(80, FILE = fout, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'replace', &
+             ACTION = 'write')
+        WRITE(80) o
+        CLOSE(80)
+        DEALLOCATE(dh, o)
     END SUBROUTINE
 
 END PROGRAM ref_harness

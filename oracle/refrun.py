@@ -27,21 +27,30 @@ def available() -> bool:
 
 def ini_text(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, DT=1e-3,
              NTSTF=1, iIniField_tg=0, CFLGV=1.0e6, TSTOP=1.0e6, dtSave1=1.0e6,
-             iterMonitor=1, iRandomType=2, VPER=0.6, SVPER=0.3, TimeReStart_tg=0.0) -> str:
-    """Channel, turbulence-generator domain (iDomain=1), flow only, as key lines.
+             iterMonitor=1, iRandomType=2, VPER=0.6, SVPER=0.3, TimeReStart_tg=0.0,
+             iDomain=1, iThermoDynamics=0, T0=680.15, Ti=680.15, L0=1.5e-3,
+             iThermalWallType=(2, 2), thermalWallBC=(640.15, 660.15), iVisScheme=1) -> str:
+    """Channel as key lines: iDomain=1 puts the mesh on the turbulence-generator domain
+    (NCL1_tg, HX_tg), iDomain=2 on the main (IO) domain (NCL1_io, HX_io).
 
     Output intervals are set far beyond the run so no instant/averaged files are written.
     """
     f = lambda x: repr(float(x))
+    Ltg, Lio = (L, 0) if iDomain == 1 else (0, L)
+    HXtg, HXio = (HX, 0.0) if iDomain == 1 else (0.0, HX)
     lines = [
-        "[flowtype]", "iCase= 1", "iDomain= 1", "iThermoDynamics= 0", "iCHT= 0",
-        "[geometry]", f"HX_tg= {f(HX)}", "HX_io= 0.0", f"HZ= {f(HZ)}", "HRb= -1.0", "HRt= 1.0",
-        "[mesh]", f"NCL1_tg= {L}", "NCL1_io= 0", f"NCL3= {N}", f"NCL2= {M}", f"STR2= {f(STR2)}",
+        "[flowtype]", "iCase= 1", f"iDomain= {iDomain}", f"iThermoDynamics= {iThermoDynamics}",
+        "iCHT= 0",
+        "[geometry]", f"HX_tg= {f(HXtg)}", f"HX_io= {f(HXio)}", f"HZ= {f(HZ)}", "HRb= -1.0",
+        "HRt= 1.0",
+        "[mesh]", f"NCL1_tg= {Ltg}", f"NCL1_io= {Lio}", f"NCL3= {N}", f"NCL2= {M}",
+        f"STR2= {f(STR2)}",
         "[boundary]", "BCX12_tg= 3, 3", "BCX12_io= 3, 3", "BCZ12= 3, 3", "BCY12= 1, 1",
         "[fluid]", f"REN= {f(REN)}", f"ReIni= {f(REN)}", "TLgRe= 0.0", "iFlowDriven= 1",
         "Cf_Given= 8.18E-3",
-        "[thermohydraulics]", "iFluidMedia= 1", "iGravity= 0", "L0= 1.5E-3", "T0= 680.15",
-        "Ti= 680.15", "iThermalWallType= 2, 2", "thermalWallBC_Dim= 640.15, 660.15",
+        "[thermohydraulics]", "iFluidMedia= 1", "iGravity= 0", f"L0= {f(L0)}", f"T0= {f(T0)}",
+        f"Ti= {f(Ti)}", f"iThermalWallType= {iThermalWallType[0]}, {iThermalWallType[1]}",
+        f"thermalWallBC_Dim= {f(thermalWallBC[0])}, {f(thermalWallBC[1])}",
         "[conjugateHeatTransfer]", "HY_Solid= 0.1, 0.1", "HXs_Solid= 1.0, 1.0",
         "HXe_Solid= 3.0, 3.0", "NCL2_Solid= 8, 8", "Cp_Solid= 502.4, 502.4",
         "D_Soild= 8000.0, 8000.0", "K_Solid= 16.2, 16.2",
@@ -51,7 +60,7 @@ def ini_text(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, DT=1e-3,
         "iIniFieldTime= 0",
         "[numerics]", f"NTSTF= {NTSTF}", f"TSTOP= {f(TSTOP)}", "timeFlowStart= 0.0",
         "timeThermoStart= 1.0E6", f"DT= {f(DT)}", "DtMin= 1.0E-9", f"CFLGV= {f(CFLGV)}",
-        "[methods]", "iVisScheme= 1", "iWeightedPre= 0",
+        "[methods]", f"iVisScheme= {iVisScheme}", "iWeightedPre= 0",
         "[statistics]", f"dtSave1= {f(dtSave1)}", f"tRunAve= {f(dtSave1)}",
         "tRunAve_Reset= 0.0", f"dtAveView= {f(dtSave1)}", f"dtRawView= {f(dtSave1)}",
         f"iterMonitor= {iterMonitor}", "MGRID= 4", "JINI= 2",
@@ -176,6 +185,32 @@ class Harness:
         """Returns u, v, w, p as [N, M+2, L] arrays."""
         a = _gather(self.path(name), self.nranks, 4, halo=True)
         return a[0], a[1], a[2], a[3]
+
+    def read_table(self, name):
+        """Nondimensional property table + spline coefficients written by ``dumptable``."""
+        with open(self.path(name), "rb") as fh:
+            hdr = np.frombuffer(fh.read(20), dtype=np.int32)
+            n = int(hdr[0])
+            rest = np.frombuffer(fh.read(), dtype=np.float64)
+        t = {"N": n, "IMIN_DH": int(hdr[1]), "IMAX_DH": int(hdr[2]), "IMIN_H": int(hdr[3]),
+             "IMAX_H": int(hdr[4])}
+        keys = ["DH", "H", "T", "D", "M", "K", "CP", "B"]
+        for s in ("DHH", "HT", "HD", "HM", "HK", "HCp", "HB"):
+            keys += [f"{s}_B", f"{s}_C", f"{s}_D"]
+        for i, k in enumerate(keys):
+            t[k] = rest[i * n:(i + 1) * n].copy()
+        ref = rest[len(keys) * n:]
+        for i, k in enumerate(("T0", "H0", "D0", "M0", "K0", "CP0", "B0")):
+            t[k] = float(ref[i])
+        return t
+
+    def prop_lookup(self, dh, fin="dh.bin", fout="props.bin", table_file=None):
+        """THERM_PROP_UPDATE_FROM_DH of the reference for each DH; returns [7, n]."""
+        dh = np.ascontiguousarray(dh, dtype=np.float64)
+        with open(self.path(fin), "wb") as fh:
+            fh.write(np.array([len(dh)], dtype=np.int32).tobytes())
+            fh.write(dh.tobytes())
+        return fin, fout
 
     def read_cells(self, name, nvars=1):
         a = _gather(self.path(name), self.nranks, nvars, halo=False)

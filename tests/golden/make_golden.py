@@ -66,6 +66,39 @@ def make_case(L, M, N, STR2, REN, dt, seed):
     print("wrote", path, os.path.getsize(path))
 
 
+def make_props(T0=665.0, seed=21):
+    """Property-table fixture: the reference's nondimensional table and splines for the
+    synthetic 2048-point table (chapsim_amd.thermo.synthetic_table), and its
+    THERM_PROP_UPDATE_FROM_DH outputs for table nodes, range ends, points just inside the
+    1e-8 tolerance and seeded random values."""
+    sys.path.insert(0, os.path.join(ROOT))
+    import __graft_entry__
+    __graft_entry__.load()
+    from chapsim_amd import thermo
+    rows = thermo.synthetic_table()
+    h = refrun.Harness(8, 8, 8, iDomain=2, iThermoDynamics=1, T0=T0, Ti=T0,
+                       iThermalWallType=(1, 1), thermalWallBC=(1.0e3, -1.0e3))
+    try:
+        thermo.write_table_file(h.path("NIST_WATER_23.5MP.DAT"), rows)
+        h.run(["thermoinit", "dumptable table.bin"])
+        t = h.read_table("table.bin")
+        lo, hi = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+        rng = np.random.Generator(np.random.Philox(key=seed))
+        dh = np.concatenate([t["DH"][::7], [lo, hi, lo - 1e-10, hi + 1e-10],
+                             rng.uniform(lo, hi, 4000)])
+        h.prop_lookup(dh)
+        h.run(["thermoinit", "proplookup dh.bin props.bin"])
+        props = np.fromfile(h.path("props.bin"), dtype=np.float64).reshape(7, len(dh))
+    finally:
+        h.close()
+    out = {f"t_{k}": np.asarray(v) for k, v in t.items()}
+    out.update(rows=rows, T0=np.array(T0), dh=dh, props=props)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "props_2048.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
     for case in CASES:
         make_case(*case)
+    make_props()
