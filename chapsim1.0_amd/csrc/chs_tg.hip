@@ -545,6 +545,55 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__re
     }
 }
 
+// Residual of the discrete Poisson operator the FFT99 solve inverts (possion3d_FFT99.f90:
+// 202-231: x and z second differences, y-stencil AMPH/ACPH/APPH with the Neumann folding of
+// TDMA_COEF.f90:44-65): r = src - L phi, all FP64, on cell fields; block max |r| partials.
+__global__ __launch_bounds__(256) void k_poisson_resid(Geo g, const double *__restrict__ amph,
+                                                       const double *__restrict__ acph,
+                                                       const double *__restrict__ apph,
+                                                       const double *__restrict__ src,
+                                                       const double *__restrict__ phi, double *__restrict__ r,
+                                                       double *__restrict__ part)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    double val = 0.0;
+    if (i < g.L && J <= g.M) {
+        const int ip = (i == g.L - 1) ? 0 : i + 1, im = (i == 0) ? g.L - 1 : i - 1;
+        const int kp = (k == g.N - 1) ? 0 : k + 1, km = (k == 0) ? g.N - 1 : k - 1;
+        const double p0 = phi[cix(g, i, J, k)];
+        double lap = (phi[cix(g, ip, J, k)] - 2.0 * p0 + phi[cix(g, im, J, k)]) * g.dxqi +
+                     (phi[cix(g, i, J, kp)] - 2.0 * p0 + phi[cix(g, i, J, km)]) * g.dzqi + acph[J - 1] * p0;
+        if (J > 1) lap += amph[J - 1] * phi[cix(g, i, J - 1, k)];
+        if (J < g.M) lap += apph[J - 1] * phi[cix(g, i, J + 1, k)];
+        const double rr = src[cix(g, i, J, k)] - lap;
+        if (r) r[cix(g, i, J, k)] = rr;
+        val = fabs(rr);
+    }
+    __shared__ double red[256];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    red[t] = val;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] = fmax(red[t], red[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// acc (cells) = [acc +] phi interior rows (halo field)
+__global__ __launch_bounds__(256) void k_phi_accum(Geo g, const double *__restrict__ phi, double *__restrict__ acc,
+                                                   int add)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    if (i >= g.L || J > g.M) return;
+    const double v = phi[hix(g, i, J, k)];
+    acc[cix(g, i, J, k)] = add ? acc[cix(g, i, J, k)] + v : v;
+}
+
 // Diagnostics: CFL_tg (CFL.f90:27-46) max over cells, and DIVGCK_tg max |div| (DIVGCK.f90).
 __global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict__ q, double *__restrict__ part,
                                                  int which)
@@ -636,7 +685,7 @@ struct chs_ctx {
     // device
     double *cf = nullptr;   // packed coefficients
     Geo geo;
-    const double *d_amph = nullptr, *d_apph = nullptr;
+    const double *d_amph = nullptr, *d_apph = nullptr, *cf_acph = nullptr;
     double *q = nullptr, *pr = nullptr, *phi = nullptr, *div = nullptr;
     void *h0 = nullptr, *rhs = nullptr, *rhs2 = nullptr, *spec = nullptr;
     double *pp = nullptr, *partial = nullptr, *scal = nullptr;
@@ -964,6 +1013,35 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     return 0;
 }
 
+// Poisson solve of a device RHS (cells) into a device phi (cells) with `nref` refinement
+// sweeps: r = rhs - L phi in FP64, phi += solve(r).  resmax (optional) gets max |r| after
+// the last sweep.
+template <typename T>
+int poisson_device_impl(chs_ctx *c, const double *rhs, double *phi, int nref, double *resmax)
+{
+    const Geo &g = c->geo;
+    const dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
+    const int npart = grd.x * grd.y * grd.z;
+    const double *acph = c->cf_acph;
+    poisson_launch<T>(c, 1.0, rhs);
+    hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 0);
+    for (int r = 0; r < nref; ++r) {
+        hipLaunchKernelGGL(k_poisson_resid, grd, blk, 0, c->stream, g, c->d_amph, acph, c->d_apph, rhs, phi,
+                           c->div, c->partial);
+        poisson_launch<T>(c, 1.0, c->div);
+        hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 1);
+    }
+    if (resmax) {
+        hipLaunchKernelGGL(k_poisson_resid, grd, blk, 0, c->stream, g, c->d_amph, acph, c->d_apph, rhs, phi,
+                           (double *)nullptr, c->partial);
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, npart, c->scal + 3);
+        HIPCHK(c, hipMemcpyAsync(resmax, c->scal + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int poisson_impl(chs_ctx *c, const double *rhs, double *phi)
 {
@@ -1150,7 +1228,7 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
     auto put = [&](const std::vector<double> &v) { size_t o = pack.size(); pack.insert(pack.end(), v.begin(), v.end()); return o; };
     size_t o_dyfi = put(c->dyfi), o_dyci = put(c->dyci), o_wff = put(c->wff), o_wfb = put(c->wfb);
     size_t o_am = put(c->amvr), o_ac = put(c->acvr), o_ap = put(c->apvr);
-    size_t o_amph = put(c->amph), o_apph = put(c->apph);
+    size_t o_amph = put(c->amph), o_apph = put(c->apph), o_acph = put(c->acph);
     CK(hipMalloc(&c->cf, sizeof(double) * pack.size()));
     CK(hipMemcpyAsync(c->cf, pack.data(), sizeof(double) * pack.size(), hipMemcpyHostToDevice, c->stream));
     CK(hipStreamSynchronize(c->stream));
@@ -1159,7 +1237,7 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
     g.dxi = c->dxi; g.dzi = c->dzi; g.dxqi = c->dxqi; g.dzqi = c->dzqi;
     g.dyfi = c->cf + o_dyfi; g.dyci = c->cf + o_dyci; g.wff = c->cf + o_wff; g.wfb = c->cf + o_wfb;
     g.amvr = c->cf + o_am; g.acvr = c->cf + o_ac; g.apvr = c->cf + o_ap;
-    c->d_amph = c->cf + o_amph; c->d_apph = c->cf + o_apph;
+    c->d_amph = c->cf + o_amph; c->d_apph = c->cf + o_apph; c->cf_acph = c->cf + o_acph;
 
     const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L;
     const size_t ws = wsize(c);
@@ -1333,6 +1411,13 @@ int chs_poisson(chs_ctx *c, const double *rhs, double *phi)
 {
     if (!c || !rhs || !phi) return fail(c, CHS_E_ARG, "null argument");
     return c->mode == CHS_MIXED ? poisson_impl<float>(c, rhs, phi) : poisson_impl<double>(c, rhs, phi);
+}
+
+int chs_poisson_device(chs_ctx *c, const double *rhs, double *phi, int nref, double *resmax)
+{
+    if (!c || !rhs || !phi || nref < 0 || nref > 8) return fail(c, CHS_E_ARG, "bad argument");
+    return c->mode == CHS_MIXED ? poisson_device_impl<float>(c, rhs, phi, nref, resmax)
+                                : poisson_device_impl<double>(c, rhs, phi, nref, resmax);
 }
 
 int chs_cfl_tg(chs_ctx *c, double dt, double *cflmm)

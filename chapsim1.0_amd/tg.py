@@ -42,6 +42,7 @@ SIGNATURES = {
     "chs_step_tg": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "chs_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_poisson": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "chs_poisson_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp]),
     "chs_cfl_tg": (C.c_int, [C.c_void_p, C.c_double, _dp]),
     "chs_max_div": (C.c_int, [C.c_void_p, _dp]),
     "chs_sync": (C.c_int, [C.c_void_p]),
@@ -179,6 +180,14 @@ class ChannelTG:
         phi = np.zeros((N, M, L))
         _check(lib().chs_poisson(self._h, _p(rhs), _p(phi)), self._h)
         return phi
+
+    def poisson_device(self, rhs_ptr: int, phi_ptr: int, nref: int = 0, want_residual=True):
+        """Solve on device arrays (cells, [N,M,L] float64) given as raw device pointers;
+        returns max |rhs - L phi| (FP64) if want_residual."""
+        res = C.c_double(-1.0)
+        _check(lib().chs_poisson_device(self._h, C.c_void_p(rhs_ptr), C.c_void_p(phi_ptr), int(nref),
+                                        C.byref(res) if want_residual else None), self._h)
+        return res.value
 
     def cfl(self, dt: float) -> float:
         v = C.c_double()

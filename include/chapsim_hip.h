@@ -112,6 +112,12 @@ int chs_set_graph(chs_ctx *ctx, int enable);
  * (interior rows).  Host arrays; synchronous. */
 int chs_poisson(chs_ctx *ctx, const double *rhs, double *phi);
 
+/* Same solve on device arrays (cells, N x M x L as RHSLLPHI) on the context's stream, with
+ * nref (0..8) refinement sweeps: r = rhs - L phi in FP64, phi += solve(r).  resmax, if not
+ * NULL, receives max |rhs - L phi| after the last sweep (then the call waits).  This is
+ * the standalone pressure-Poisson workload and the mixed form's low-precision correction. */
+int chs_poisson_device(chs_ctx *ctx, const double *rhs, double *phi, int nref, double *resmax);
+
 /* Per-step diagnostics (CFL_tg, CFL.f90:13-57, and DIVGCK_tg, DIVGCK.f90:15-66), computed
  * on the device from the current fields; synchronous (a scalar read back). */
 int chs_cfl_tg(chs_ctx *ctx, double dt, double *cflmm);
