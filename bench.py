@@ -15,7 +15,11 @@ on a bounded sample of the same workload (a few steps), or the C oracle on one c
 reference build is absent.
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--form plain|mixed]
-                       [--no-cpu-baseline]
+                       [--no-cpu-baseline] [--graph]
+Other workloads (each prints its own JSON line; not the default run):
+  --workload channel64   64^3 channel, Re_tau 180, dt 1e-3
+  --workload poisson     standalone pressure Poisson 512x384x512, beta 2.2 (config 3)
+  --workload ensemble    32 channels of seeded uneven sizes, one hipGraph per step (config 4)
 """
 from __future__ import annotations
 
@@ -36,6 +40,37 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 CFG = dict(L=256, M=192, N=256, HX=4 * np.pi, HZ=2 * np.pi, STR2=2.0, REN=6875.0, dt=5.0e-4)
+CFG64 = dict(L=64, M=64, N=64, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, dt=1.0e-3)
+
+
+def ren_from_retau(re_tau):
+    """Bulk Reynolds number on the half height from Re_tau by Dean's correlation
+    Re_tau = 0.09 (2 REN)^0.88 (REN = 2800 <-> Re_tau ~ 180, as the reference's templates)."""
+    return (re_tau / 0.09) ** (1.0 / 0.88) / 2.0
+
+
+def ensemble_cases(n=32, seed=4):
+    """Seeded channel sizes for the ensemble workload: L, N in [64, 192] and M in [64, 128],
+    L and N even with no prime factor above 5 (FFT99's condition, lib/fft99/FFTFAX.f90:56-91),
+    Re_tau uniform in [180, 395]."""
+    ok = [v for v in range(64, 193, 2) if _smooth(v)]
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    cases = []
+    for _ in range(n):
+        L = int(rng.choice(ok))
+        N = int(rng.choice(ok))
+        M = int(rng.integers(64, 129))
+        rt = float(rng.uniform(180.0, 395.0))
+        cases.append(dict(L=L, M=M, N=N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8 + 0.2 * (rt - 180) / 215,
+                          REN=ren_from_retau(rt), dt=5.0e-4, re_tau=rt))
+    return cases
+
+
+def _smooth(v):
+    for p in (2, 3, 5):
+        while v % p == 0:
+            v //= p
+    return v == 1
 
 
 def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
@@ -47,11 +82,14 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
     table = {
         # R u,v,w,p (4F, FP64) + H0 x3; W RHS x3 + H0 x3
         "conv_rhs": 4 * F * 8 + 3 * F * wbytes + 6 * F * wbytes,
-        # R RHS x3, W RHS x3 (v skips row 1)
+        # R RHS x3, W RHS x3 (v skips row 1); one launch for the three components
         "cn_x": 6 * F * wbytes,
+        # fused z+y stencils + Q update, three components: R RHS x3, R+W q x3
+        "cn_zy": 3 * F * wbytes + 6 * F * 8,
+        # z stencils, one launch for the three components: R + W RHS x3
         "cn_z": 6 * F * wbytes,
-        # R RHS x3, R+W q x3
-        "cn_y": 3 * F * wbytes + 6 * F * 8,
+        # y stencil + Q update, one launch per component: R RHS, R + W q
+        "cn_y": F * wbytes + 2 * F * 8,
         # R u,v,w; W div (FP64) + spectrum
         "divg_fftx": 3 * F * 8 + F * 8 + S * 2 * wbytes,
         "fftz_fwd": 2 * S * 2 * wbytes,
@@ -62,12 +100,14 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
         "fftx_inv": S * 2 * wbytes + F * 8,
         # R phi, div, u, v, w, p; W u, v, w, p
         "update": 10 * F * 8,
+        # refinement correction: R phi, u, v, w, p; W u, v, w, p
+        "refine_update": 9 * F * 8,
     }
     return table[kernel]
 
 
-KERNELS = ["conv_rhs", "cn_x", "cn_z", "cn_y", "divg_fftx", "fftz_fwd", "ytdma", "fftz_inv",
-           "fftx_inv", "update"]
+KERNELS = ["conv_rhs", "cn_x", "cn_zy", "cn_z", "cn_y", "divg_fftx", "fftz_fwd", "ytdma",
+           "fftz_inv", "fftx_inv", "update", "refine_update"]
 
 
 def load_pkg():
@@ -75,8 +115,8 @@ def load_pkg():
     return __graft_entry__.load()
 
 
-def run_gpu(pkg, form: str, steps: int, warmup: int):
-    c = CFG
+def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False):
+    c = c or CFG
     coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
     ch = pkg.ChannelTG(coeffs, mode)
@@ -87,6 +127,8 @@ def run_gpu(pkg, form: str, steps: int, warmup: int):
     ch.SOLVERRK3_MOM_tg(0, dt, cv)
     q0, p0 = ch.download()          # projected start, shared with the CPU baseline
     # profiling pre-pass: which kernel dominates a step
+    if form == "mixed":
+        ch.set_refinement(0, 2)
     ch.profile_reset(None)
     for _ in range(2):
         ch.rk3_step(dt, cv)
@@ -96,16 +138,16 @@ def run_gpu(pkg, form: str, steps: int, warmup: int):
         avg, n = ch.profile_get(k)
         per[k] = avg * n / 2.0          # ms per step
     dom = max(per, key=per.get)
+    pre_avg, pre_n = ch.profile_get(dom)
     for _ in range(warmup):
         ch.rk3_step(dt, cv)
-    # timed region; events only around the dominant kernel
-    ch.profile_reset(dom)
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    if graph:                       # replayed hipGraph: no per-kernel events in the timed region
+        ch.set_graph(True)
+        ch.rk3_step(dt, cv)
+        ch.sync()
+        ch.profile_reset("none")
+    else:                           # timed region; events only around the dominant kernel
+        ch.profile_reset(dom)
     ch.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -113,6 +155,8 @@ def run_gpu(pkg, form: str, steps: int, warmup: int):
     ch.sync()
     t1 = time.perf_counter()
     dom_avg_ms, dom_n = ch.profile_get(dom)
+    if graph:                       # kernel time from the profiling pre-pass instead
+        dom_avg_ms, dom_n = pre_avg, pre_n
     cfl = ch.cfl(dt)
     div = ch.max_div()
     ch.close()
@@ -122,7 +166,7 @@ def run_gpu(pkg, form: str, steps: int, warmup: int):
                 start=(q0, p0))
 
 
-def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = None):
+def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = None, c=None):
     """Time the reference program on this host: restart from the projected start field
     (the reference checks max div < 1e-6 on restart, FLOWSTART.f90:41-46) written in its
     restart format (processing/WRT_VARS_TG.f90:12-104), `sample_steps` steps, median of the
@@ -131,7 +175,7 @@ def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = Non
     from oracle import refrun
     if not os.path.exists(refrun.PROGRAM) or not os.path.exists(refrun.MPIRUN):
         return None
-    c = CFG
+    c = c or CFG
     L, M, N = c["L"], c["M"], c["N"]
     ncpu = len(os.sched_getaffinity(0))
     P = cores or max(2, min(16, ncpu))
@@ -203,6 +247,81 @@ def cpu_baseline_port():
                 kind="port", sample=f"C oracle, one substep at {c['L']}x{c['M']}x{c['N']}, x3")
 
 
+def bench_poisson(pkg, args):
+    """Standalone pressure Poisson (config 3): 512x384x512, beta 2.2, seeded Gaussian RHS with
+    the dy-weighted mean removed.  Times `steps` solves of the plain FP64 form and of the
+    mixed form with args.nref FP64 refinement sweeps; reports the mixed form's error vs the
+    FP64 solution and both residuals."""
+    L, M, N = 512, 384, 512
+    co = pkg.channel_setup(L, M, N, 4 * np.pi, 2 * np.pi, 2.2)
+    rng = np.random.Generator(np.random.Philox(key=3))
+    r = rng.standard_normal((N, M, L))
+    w = (1.0 / co["dyfi"])[None, :, None]
+    r -= np.sum(r * w) / (np.sum(w) * L * N)
+    d_r = pkg.DeviceArray.from_host(r)
+    rmax = float(np.max(np.abs(r)))
+    del r
+    out = {}
+    phis = {}
+    for form, mode, nref in (("plain_fp64", pkg.PLAIN_FP64, 0), ("mixed", pkg.MIXED, args.nref)):
+        ch = pkg.ChannelTG(co, mode)
+        d_phi = pkg.DeviceArray(d_r.shape)
+        for _ in range(max(1, args.warmup)):
+            res = ch.poisson_device(d_r.ptr, d_phi.ptr, nref, True)
+        ch.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ch.poisson_device(d_r.ptr, d_phi.ptr, nref, False)
+        ch.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res = ch.poisson_device(d_r.ptr, d_phi.ptr, nref, True)
+        phis[form] = d_phi.to_host()
+        d_phi.free()
+        out[form] = {"ms_per_solve": ms, "grid_point_solves_per_s": L * M * N / (ms * 1e-3),
+                     "refinement_sweeps": nref, "max_residual_rel": res / rmax}
+        ch.close()
+    d_r.free()
+    err = float(np.max(np.abs(phis["mixed"] - phis["plain_fp64"])) / np.max(np.abs(phis["plain_fp64"])))
+    out["mixed"]["max_rel_error_vs_fp64"] = err
+    print(json.dumps({"metric": "grid-point Poisson solves/s", "workload":
+                      "standalone Poisson 512x384x512, tanh beta=2.2, periodic x,z, Neumann y",
+                      "value": out["mixed"]["grid_point_solves_per_s"], "unit": "grid-point solves/s",
+                      "n_gpus": 1, "steps": args.steps, "forms": out,
+                      "speedup_mixed_vs_plain": out["plain_fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"]}))
+
+
+def bench_ensemble(pkg, args):
+    """32 independent channels of seeded uneven sizes (config 4), one hipGraph per RK3 step
+    of the whole ensemble; value = total grid-point RK3-steps/s."""
+    cases = ensemble_cases()
+    mode = pkg.MIXED if args.form == "mixed" else pkg.PLAIN_FP64
+    cos = [pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"]) for c in cases]
+    ens = pkg.Ensemble(cos, mode)
+    dt = np.array([c["dt"] for c in cases])
+    cv = np.array([1.0 / c["REN"] for c in cases])
+    for i, (ch, co) in enumerate(zip(ens.channels, cos)):
+        q, p = pkg.poiseuille_perturbed(co, seed=1000 + i, amp=0.1)
+        ch.upload(q, p)
+        ch.SOLVERRK3_MOM_tg(0, dt[i], cv[i])
+    for _ in range(args.warmup):
+        ens.step(dt, cv)
+    ens.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ens.step(dt, cv)
+    ens.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    pts = sum(c["L"] * c["M"] * c["N"] for c in cases)
+    divs = [ch.max_div() for ch in ens.channels]
+    ens.close()
+    print(json.dumps({"metric": "grid-point RK3-steps/s (ensemble total)", "value": pts / (ms * 1e-3),
+                      "unit": "grid-point RK3-steps/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": ms, "channels": len(cases),
+                      "grid_points": pts, "form": args.form, "max_div": max(divs),
+                      "config": {"workload": "32 channels, L,N in [64,192], M in [64,128], "
+                                             "Re_tau U[180,395], one hipGraph per RK3 step"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +329,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay each RK3 step from a hipGraph")
+    ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble"],
+                    default="channel")
+    ap.add_argument("--nref", type=int, default=1, help="refinement sweeps (poisson, mixed)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -227,8 +350,12 @@ def main():
               "MI355X); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()
-    r = run_gpu(pkg, args.form, args.steps, args.warmup)
-    c = CFG
+    if args.workload == "poisson":
+        return bench_poisson(pkg, args)
+    if args.workload == "ensemble":
+        return bench_ensemble(pkg, args)
+    c = CFG64 if args.workload == "channel64" else CFG
+    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph)
     F = c["L"] * c["M"] * c["N"]
     value = F / (r["ms_per_step"] * 1e-3)
     dom = r["dominant"]
@@ -242,7 +369,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline_reference(r["start"])
+            cpu = cpu_baseline_reference(r["start"], c=c)
         except Exception as e:  # pragma: no cover - host-dependent
             cpu = dict(error=str(e))
         if cpu is None or "error" in cpu:
@@ -263,8 +390,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.form == "plain" else "f32+f64",
         "data": "synthetic: Poiseuille + 5% seeded (Philox) perturbation, projected by NS=0",
-        "config": {"workload": "periodic channel Re_tau=395 (REN=6875), 256x192x256, "
-                               "4pi x 2 x 2pi, tanh beta=2.0, dt=5e-4, RK3+CN, flow only",
+        "config": {"workload": (f"periodic channel REN={c['REN']:g}, {c['L']}x{c['M']}x{c['N']}, "
+                                f"4pi x 2 x 2pi, tanh beta={c['STR2']:g}, dt={c['dt']:g}, "
+                                "RK3+CN, flow only"),
+                   "graph": bool(args.graph),
                    "form": "plain_fp64" if args.form == "plain" else "mixed",
                    "grid": [c["L"], c["M"], c["N"]]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -5,5 +5,6 @@ The package directory is ``chapsim1.0_amd``; it is imported under the name ``cha
 to ``libchapsim_hip.so``), ``fortran/`` (ISO_C_BINDING module and drop-in substep),
 ``tg.py`` (Python host mirror of the reference's substep interface).
 """
-from .tg import MIXED, PLAIN_FP64, ChannelTG, ChsError, channel_setup, lib  # noqa: F401
+from .tg import (MIXED, PLAIN_FP64, ChannelTG, ChsError, DeviceArray, Ensemble,  # noqa: F401
+                 channel_setup, lib)
 from .init import poiseuille_perturbed  # noqa: F401

@@ -75,7 +75,8 @@ __global__ __launch_bounds__(256) void k_conv_rhs(Geo g, Sub s, const double *__
                                                   double *__restrict__ partial, int KC)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    // blockDim = (64, 4): a wavefront is one row J, so J-indexed coefficients are uniform
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
     const size_t hs = (size_t)g.N * (g.M + 2) * g.L;   // halo field size
     const size_t cs = (size_t)g.N * g.M * g.L;         // cell field size
     double mf = 0.0;
@@ -308,6 +309,169 @@ __global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, i
     if (J == M) qc[hix(g, i, M + 1, k)] = 0.0;
 }
 
+// Fused z + y Crank-Nicolson stencils, Q += dQ and wall BCs for all three components.
+// Block: an x-tile of TX columns at one z-plane k, all rows J = 1..M.  For each component
+// the z-stencil (circulant inverse) of the x-solved RHS goes to LDS column by column, then the
+// banded y-inverse is applied from LDS.  Components run u, v, w in turn so v's row J = 1
+// (the u increment left in the reference's shared RHS_tg, RHS_CvLpGpS_tg.f90:45-46) is taken
+// from the u pass of the same block.
+struct CnArgs {
+    const void *gz;
+    const void *by[3];
+    int Dz, fullz, Dy[3];
+};
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_zy(Geo g, CnArgs a, const W *__restrict__ rin, double *__restrict__ q,
+                                               int TX)
+{
+    extern __shared__ unsigned char smem_raw[];
+    W *col = reinterpret_cast<W *>(smem_raw);   // [M][TX]
+    W *du1 = col + (size_t)g.M * TX;             // [TX] u increment at J = 1
+    const int i0 = blockIdx.x * TX, k = blockIdx.y;
+    const int M = g.M, N = g.N;
+    const size_t cs = (size_t)N * M * g.L, hs = (size_t)N * (M + 2) * g.L;
+    const int nt = blockDim.x;
+    const W *gz = (const W *)a.gz;
+    for (int c = 0; c < 3; ++c) {
+        const W *r = rin + c * cs;
+        // z-stencil into LDS (rows J >= N2I; for v row 1 is u's increment)
+        for (int t = threadIdx.x; t < M * TX; t += nt) {
+            const int J = t / TX + 1, ii = t - (J - 1) * TX, i = i0 + ii;
+            W acc = W(0);
+            if (i < g.L) {
+                if (c == 1 && J == 1) {
+                    acc = du1[ii];
+                } else if (a.fullz) {
+                    for (int l = 0; l < N; ++l) {
+                        int d = k - l;
+                        if (d < 0) d += N;
+                        acc += gz[d] * r[cix(g, i, J, l)];
+                    }
+                } else {
+                    acc = gz[0] * r[cix(g, i, J, k)];
+                    for (int d = 1; d <= a.Dz; ++d) {
+                        int ka = k + d, kb = k - d;
+                        if (ka >= N) ka -= N;
+                        if (kb < 0) kb += N;
+                        acc += gz[d] * (r[cix(g, i, J, ka)] + r[cix(g, i, J, kb)]);
+                    }
+                }
+            }
+            col[(J - 1) * TX + ii] = acc;
+        }
+        __syncthreads();
+        // y-stencil from LDS, Q += dQ, wall BCs (BC_WALL.f90:229-251)
+        const W *by = (const W *)a.by[c];
+        const int D = a.Dy[c], w = 2 * D + 1;
+        double *qc = q + c * hs;
+        for (int t = threadIdx.x; t < M * TX; t += nt) {
+            const int J = t / TX + 1, ii = t - (J - 1) * TX, i = i0 + ii;
+            if (i >= g.L) continue;
+            if (!(c == 1 && J == 1)) {
+                const int lo = J - D < 1 ? 1 : J - D, hi = J + D > M ? M : J + D;
+                const W *bj = by + (size_t)(J - 1) * w;
+                W acc = W(0);
+                for (int l = lo; l <= hi; ++l) acc += bj[l - J + D] * col[(l - 1) * TX + ii];
+                qc[hix(g, i, J, k)] += (double)acc;
+                if (c == 0 && J == 1) du1[ii] = acc;
+            }
+            if (J == 1) {
+                qc[hix(g, i, 0, k)] = 0.0;
+                if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
+            }
+            if (J == M) qc[hix(g, i, M + 1, k)] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+// Marching forms of the z and y Crank-Nicolson stencils with the band half-width D a
+// compile-time constant (D <= 8), so each input is loaded once and kept in a register window.
+//
+// z: one thread per (i, J) marches KZ consecutive planes k; circulant inverse gz (cyclic k).
+template <typename W, int D>
+__global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__ gz, const W *__restrict__ rin,
+                                                    W *__restrict__ rout, int KZ)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
+    const int c = blockIdx.z % 3, kc = blockIdx.z / 3;
+    if (i >= g.L || J > g.M) return;
+    if (c == 1 && J == 1) return;
+    const int N = g.N;
+    const size_t cs = (size_t)N * g.M * g.L;
+    const size_t ks = (size_t)g.M * g.L;          // stride between z-planes
+    const W *r = rin + c * cs + cix(g, i, J, 0);
+    W *o = rout + c * cs + cix(g, i, J, 0);
+    W coef[D + 1];
+#pragma unroll
+    for (int d = 0; d <= D; ++d) coef[d] = gz[d];
+    const int k0 = kc * KZ, k1 = min(N, k0 + KZ);
+    W win[2 * D + 1];
+#pragma unroll
+    for (int d = -D; d < D; ++d) {
+        int kk = k0 + d;
+        kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
+        win[d + D] = r[(size_t)kk * ks];
+    }
+    for (int k = k0; k < k1; ++k) {
+        int kk = k + D;
+        if (kk >= N) kk -= N;
+        win[2 * D] = r[(size_t)kk * ks];
+        W acc = coef[0] * win[D];
+#pragma unroll
+        for (int d = 1; d <= D; ++d) acc += coef[d] * (win[D + d] + win[D - d]);
+        o[(size_t)k * ks] = acc;
+#pragma unroll
+        for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+    }
+}
+
+// y: one thread per (i, k) marches a chunk of KY rows; banded inverse by[J][d] (row J is
+// uniform across the wavefront, so its coefficients come through scalar loads); Q += dQ and
+// wall BCs (BC_WALL.f90:229-251).  For u the increment of row 1 is stored into row 1 of
+// v's input (the reference's shared RHS_tg, RHS_CvLpGpS_tg.f90:45-46).
+template <typename W, int D>
+__global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__ by, W *__restrict__ rin,
+                                                    double *__restrict__ q, int c, int KY)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int k = blockIdx.y * 4 + threadIdx.y;
+    const int jc = blockIdx.z;
+    if (i >= g.L || k >= g.N) return;
+    const int M = g.M, w = 2 * D + 1;
+    const size_t cs = (size_t)g.N * M * g.L, hs = (size_t)g.N * (M + 2) * g.L;
+    const W *r = rin + c * cs + cix(g, i, 1, k);   // row J at r[(J-1)*L]
+    double *qc = q + c * hs;
+    const size_t L = g.L;
+    const int J0 = (c == 1 && jc == 0) ? 2 : jc * KY + 1;
+    const int J1 = min(M, jc * KY + KY);
+    W win[2 * D + 1];
+#pragma unroll
+    for (int d = -D; d < D; ++d) {
+        const int jj = J0 + d;
+        win[d + D] = (jj >= 1 && jj <= M) ? r[(size_t)(jj - 1) * L] : W(0);
+    }
+    for (int J = J0; J <= J1; ++J) {
+        const int jj = J + D;
+        win[2 * D] = jj <= M ? r[(size_t)(jj - 1) * L] : W(0);
+        const W *bj = by + (size_t)(J - 1) * w;
+        W acc = W(0);
+#pragma unroll
+        for (int d = 0; d < w; ++d) acc += bj[d] * win[d];   // out-of-range rows hold 0 and 0 coefficients
+        qc[hix(g, i, J, k)] += (double)acc;
+        if (c == 0 && J == 1) rin[cs + cix(g, i, 1, k)] = acc;
+#pragma unroll
+        for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+    }
+    if (jc == 0) {
+        qc[hix(g, i, 0, k)] = 0.0;
+        if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
+    }
+    if (J1 == M) qc[hix(g, i, M + 1, k)] = 0.0;
+}
+
 // ---------------------------------------------------------------------------------------
 // Poisson.  Spectral data is complex, layout [J][kx][kz] (kz fastest), so z-lines are
 // contiguous and the per-mode y-Thomas reads consecutive kz across a wavefront.
@@ -410,16 +574,22 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
 // issued four rows ahead of the recurrence.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__ amph, const double *__restrict__ apph,
-                                               const double *__restrict__ pp, cpx<T> *__restrict__ spec)
+                                               const double *__restrict__ pp, const float *__restrict__ pp32,
+                                               cpx<T> *__restrict__ spec)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int H = g.H, N = g.N, M = g.M;
     if (t >= H * N) return;
     const int kx = t / N, kz = t - kx * N;
     const bool zero = (kx == 0 && kz == 0);
+    // FP32 pivots for all but the lowest modes (kx <= 1, |kz| <= 1), whose Neumann/near-
+    // Neumann y-systems are ill-conditioned on the stretched mesh
+    const bool low = !pp32 || (kx <= 1 && (kz <= 1 || kz == N - 1));
     const size_t st = (size_t)H * N;
     cpx<T> *s = spec + (size_t)kx * N + kz;
     const double *p = pp + (size_t)kx * N + kz;
+    const float *p32 = pp32 ? pp32 + (size_t)kx * N + kz : nullptr;
+    auto piv = [&](int J) { return low ? p[(size_t)(J - 1) * st] : (double)p32[(size_t)(J - 1) * st]; };
     double gr = 0.0, gi = 0.0;
     for (int J0 = 1; J0 <= M; J0 += 4) {
         cpx<T> r[4];
@@ -428,7 +598,7 @@ __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__
         for (int u = 0; u < 4; ++u)
             if (J0 + u <= M) {
                 r[u] = s[(size_t)(J0 + u - 1) * st];
-                pj[u] = p[(size_t)(J0 + u - 1) * st];
+                pj[u] = piv(J0 + u);
             }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -451,7 +621,7 @@ __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__
         for (int u = 0; u < 4; ++u)
             if (J0 - u >= 1) {
                 gg[u] = s[(size_t)(J0 - u - 1) * st];
-                pj[u] = p[(size_t)(J0 - u - 1) * st];
+                pj[u] = piv(J0 - u);
             }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -509,7 +679,7 @@ __global__ __launch_bounds__(256) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<
 // BC_WALL_Q_tg / BC_WALL_PR_TG ghost rows.
 __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__restrict__ phi,
                                                 const double *__restrict__ div, double *__restrict__ q,
-                                                double *__restrict__ pr)
+                                                double *__restrict__ pr, int refine)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
@@ -527,7 +697,9 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__re
         q[hs + h0] = q[hs + h0] - (p0 - phi[hix(g, i, J - 1, k)]) * cy;
     }
     const double be = 0.5 * s.talp * s.dt * s.cvisc;
-    const double pn = pr[h0] + p0 - be * div[cix(g, i, J, k)];
+    // a refinement sweep adds its correction phi to p; the viscous term of PRCALC_tg uses the
+    // first solve's RHS (the divergence of the predicted velocity) only
+    const double pn = refine ? pr[h0] + p0 : pr[h0] + p0 - be * div[cix(g, i, J, k)];
     pr[h0] = pn;
     if (J == 1) {
         pr[hix(g, i, 0, k)] = pn;
@@ -689,6 +861,7 @@ struct chs_ctx {
     double *q = nullptr, *pr = nullptr, *phi = nullptr, *div = nullptr;
     void *h0 = nullptr, *rhs = nullptr, *rhs2 = nullptr, *spec = nullptr;
     double *pp = nullptr, *partial = nullptr, *scal = nullptr;
+    float *pp32 = nullptr;   // FP32 copy of the Poisson pivots (mixed form, high modes)
     int npartial = 0;
     void *twx = nullptr, *twz = nullptr;
     chs::FftPlan px, pz;
@@ -699,6 +872,8 @@ struct chs_ctx {
     std::string prof_only;
     // hipGraph of one RK3 step (three substeps), re-captured when dt or cvisc change
     bool use_graph = false;
+    int nref_step = 0, nref_projection = 0;   // refinement sweeps (mixed form default 0 / 2)
+    bool own_stream = true;   // false for channels of an ensemble (stream owned by chs_ens)
     hipGraphExec_t gexec = nullptr;
     double gdt = 0.0, gcv = 0.0;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
@@ -958,7 +1133,7 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src)
     {
         EvScope ev(c, "ytdma");
         hipLaunchKernelGGL(k_ytdma<T>, dim3((H * N + 255) / 256), dim3(256), 0, c->stream, g, c->d_amph,
-                           c->d_apph, c->pp, spec);
+                           c->d_apph, c->pp, c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr, spec);
     }
     {
         EvScope ev(c, "fftz_inv");
@@ -968,6 +1143,29 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src)
         EvScope ev(c, "fftx_inv");
         hipLaunchKernelGGL(k_fftx_inv<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa,
                            (const cpx<T> *)spec, c->phi, TK);
+    }
+}
+
+template <typename W>
+void cn_z_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *gz, const W *rin, W *rout, int KZ)
+{
+    switch (D) {
+#define CHS_ZCASE(d) case d: hipLaunchKernelGGL((k_cn_z_march<W, d>), grid, blk, 0, st, g, gz, rin, rout, KZ); break;
+        CHS_ZCASE(0) CHS_ZCASE(1) CHS_ZCASE(2) CHS_ZCASE(3) CHS_ZCASE(4) CHS_ZCASE(5) CHS_ZCASE(6)
+        CHS_ZCASE(7) CHS_ZCASE(8)
+#undef CHS_ZCASE
+    }
+}
+
+template <typename W>
+void cn_y_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *by, W *rin, double *q, int comp,
+                   int KY)
+{
+    switch (D) {
+#define CHS_YCASE(d) case d: hipLaunchKernelGGL((k_cn_y_march<W, d>), grid, blk, 0, st, g, by, rin, q, comp, KY); break;
+        CHS_YCASE(0) CHS_YCASE(1) CHS_YCASE(2) CHS_YCASE(3) CHS_YCASE(4) CHS_YCASE(5) CHS_YCASE(6)
+        CHS_YCASE(7) CHS_YCASE(8)
+#undef CHS_YCASE
     }
 }
 
@@ -995,19 +1193,42 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
         hipLaunchKernelGGL(k_cn_x<W>, dim3((M * N + NLX - 1) / NLX, 3), dim3(256), sizeof(W) * NLX * L, c->stream,
                            g, (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, NLX);
     }
-    for (int comp = 0; comp < 3; ++comp) {
+    // z then y CN stencils: marching kernels for band half-widths up to 8 (the configs need
+    // 3..7), the direct-load kernels otherwise
+    const int KZ = 16, KY = 32;
+    const bool zmarch = !t.fullz && t.Dz <= 8;
+    if (zmarch) {
         EvScope ev(c, "cn_z");
-        hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz, rhs, rhs2, comp);
+        const dim3 gz((L + 63) / 64, (M + 3) / 4, 3 * ((N + KZ - 1) / KZ));
+        cn_z_dispatch<W>(t.Dz, gz, blk, c->stream, g, (const W *)t.gz, rhs, rhs2, KZ);
+    } else {
+        for (int comp = 0; comp < 3; ++comp) {
+            EvScope ev(c, "cn_z");
+            hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz, rhs, rhs2, comp);
+        }
     }
     for (int comp = 0; comp < 3; ++comp) {
         EvScope ev(c, "cn_y");
-        hipLaunchKernelGGL(k_cn_y<W>, grd, blk, 0, c->stream, g, (const W *)t.by[comp], t.Dy[comp], rhs2,
-                           comp == 0 ? rhs2 + cs : nullptr, c->q, comp);
+        if (t.Dy[comp] <= 8) {
+            const dim3 gy((L + 63) / 64, (N + 3) / 4, (M + KY - 1) / KY);
+            cn_y_dispatch<W>(t.Dy[comp], gy, blk, c->stream, g, (const W *)t.by[comp], rhs2, c->q, comp, KY);
+        } else {
+            hipLaunchKernelGGL(k_cn_y<W>, grd, blk, 0, c->stream, g, (const W *)t.by[comp], t.Dy[comp], rhs2,
+                               comp == 0 ? rhs2 + cs : nullptr, c->q, comp);
+        }
     }
     poisson_launch<W>(c, 1.0 / (dt * c->talp[ns]), nullptr);
     {
         EvScope ev(c, "update");
-        hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr);
+        hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr, 0);
+    }
+    // FP64-residual refinement (mixed form): the divergence of the corrected velocity drives
+    // another solve and correction
+    const int nref = ns == 0 ? c->nref_projection : c->nref_step;
+    for (int r = 0; r < nref; ++r) {
+        poisson_launch<W>(c, 1.0 / (dt * c->talp[ns]), nullptr);
+        EvScope ev(c, "refine_update");
+        hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr, 1);
     }
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1174,7 +1395,11 @@ int chs_channel_setup(int L, int M, int N, double HX, double HZ, double STR2, do
     return 0;
 }
 
-int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
+static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_ctx **out);
+
+int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out) { return create_impl(cf, mode, nullptr, out); }
+
+static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_ctx **out)
 {
     if (!cf || !out) return fail(nullptr, CHS_E_ARG, "null argument");
     *out = nullptr;
@@ -1186,6 +1411,7 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
         return fail(nullptr, CHS_E_ARG, "L, N must be even, <= 2048, with factors 2,3,5 only; M >= 4");
     chs_ctx *c = new chs_ctx();
     c->L = L; c->M = M; c->N = N; c->H = L / 2 + 1; c->mode = mode;
+    if (mode == CHS_MIXED) c->nref_projection = 2;
     c->px = px; c->pz = pz;
     c->dxi = cf->dxi; c->dzi = cf->dzi; c->dxqi = cf->dxqi; c->dzqi = cf->dzqi; c->volm = cf->volm;
     c->dyfi.assign(cf->dyfi, cf->dyfi + M);
@@ -1222,7 +1448,12 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
             return bad(CHS_E_HIP);                                                      \
         }                                                                               \
     } while (0)
-    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (shared) {
+        c->stream = shared;
+        c->own_stream = false;
+    } else {
+        CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
     // packed coefficients
     std::vector<double> pack;
     auto put = [&](const std::vector<double> &v) { size_t o = pack.size(); pack.insert(pack.end(), v.begin(), v.end()); return o; };
@@ -1279,6 +1510,12 @@ int chs_create(const chs_coeffs *cf, int mode, chs_ctx **out)
             }
         CK(hipMemcpyAsync(c->pp, pp.data(), sizeof(double) * pp.size(), hipMemcpyHostToDevice, c->stream));
         CK(hipStreamSynchronize(c->stream));
+        if (mode == CHS_MIXED) {
+            std::vector<float> p32(pp.begin(), pp.end());
+            CK(hipMalloc(&c->pp32, sizeof(float) * p32.size()));
+            CK(hipMemcpyAsync(c->pp32, p32.data(), sizeof(float) * p32.size(), hipMemcpyHostToDevice, c->stream));
+            CK(hipStreamSynchronize(c->stream));
+        }
     }
     int e = (mode == CHS_MIXED) ? upload_twiddles<float>(c, L, &c->twx) : upload_twiddles<double>(c, L, &c->twx);
     if (!e) e = (mode == CHS_MIXED) ? upload_twiddles<float>(c, N, &c->twz) : upload_twiddles<double>(c, N, &c->twz);
@@ -1293,7 +1530,7 @@ void chs_destroy(chs_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cf, c->q, c->pr, c->phi, c->div, c->h0, c->rhs, c->rhs2, c->spec, c->pp,
-                    c->partial, c->scal, c->twx, c->twz};
+                    c->pp32, c->partial, c->scal, c->twx, c->twz};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->cn) {
@@ -1304,7 +1541,7 @@ void chs_destroy(chs_ctx *c)
     }
     for (auto &u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -1400,6 +1637,36 @@ int chs_step_tg(chs_ctx *c, double dt, double cvisc)
     return 0;
 }
 
+int chs_set_refinement(chs_ctx *c, int nref_step, int nref_projection)
+{
+    if (!c || nref_step < 0 || nref_step > 4 || nref_projection < 0 || nref_projection > 4)
+        return fail(c, CHS_E_ARG, "refinement sweeps must be 0..4");
+    c->nref_step = nref_step;
+    c->nref_projection = nref_projection;
+    c->gdt = 0.0;   // force re-capture of a step graph
+    return 0;
+}
+
+void *chs_dev_malloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    return p;
+}
+
+int chs_dev_free(void *p)
+{
+    if (p && hipFree(p) != hipSuccess) return fail(nullptr, CHS_E_HIP, "hipFree failed");
+    return 0;
+}
+
+int chs_memcpy(void *dst, const void *src, size_t bytes, int to_device)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    return 0;
+}
+
 int chs_set_graph(chs_ctx *c, int enable)
 {
     if (!c) return fail(c, CHS_E_ARG, "null ctx");
@@ -1489,6 +1756,110 @@ int chs_profile_get(chs_ctx *c, const char *kernel, double *avg_ms, int *count)
         }
     *avg_ms = n ? tot / n : 0.0;
     *count = n;
+    return 0;
+}
+
+// ---- Ensemble of independent channels (one hipGraph per RK3 step of all channels) ----
+
+struct chs_ens {
+    std::vector<chs_ctx *> ch;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<double> gdt, gcv;
+    std::string err;
+};
+
+int chs_ens_create(int n, const chs_coeffs *const *coeffs, int mode, chs_ens **out)
+{
+    if (n < 1 || !coeffs || !out) return fail(nullptr, CHS_E_ARG, "chs_ens_create: bad argument");
+    *out = nullptr;
+    chs_ens *e = new chs_ens();
+    hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete e;
+        return fail(nullptr, CHS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    for (int i = 0; i < n; ++i) {
+        chs_ctx *c = nullptr;
+        int rc = create_impl(coeffs[i], mode, e->stream, &c);
+        if (rc) {
+            chs_ens_destroy(e);
+            return rc;
+        }
+        e->ch.push_back(c);
+    }
+    *out = e;
+    return 0;
+}
+
+void chs_ens_destroy(chs_ens *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto *c : e->ch) chs_destroy(c);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int chs_ens_channel(chs_ens *e, int i, chs_ctx **ctx)
+{
+    if (!e || i < 0 || i >= (int)e->ch.size() || !ctx) return fail(nullptr, CHS_E_ARG, "chs_ens_channel: bad index");
+    *ctx = e->ch[i];
+    return 0;
+}
+
+int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
+{
+    if (!e || !dt || !cvisc) return fail(nullptr, CHS_E_ARG, "chs_ens_step: bad argument");
+    const int n = (int)e->ch.size();
+    bool same = e->gexec && (int)e->gdt.size() == n;
+    for (int i = 0; same && i < n; ++i) same = e->gdt[i] == dt[i] && e->gcv[i] == cvisc[i];
+    if (!same) {
+        for (int i = 0; i < n; ++i) {
+            chs_ctx *c = e->ch[i];
+            if (!c->have_fields) return fail(c, CHS_E_STATE, "chs_ens_step: channel without fields");
+            if (!(dt[i] > 0) || !(cvisc[i] > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
+            for (int ns = 1; ns <= 3; ++ns)
+                if (int rc = build_cn(c, ns, dt[i], cvisc[i])) return rc;
+        }
+        if (e->gexec) {
+            (void)hipGraphExecDestroy(e->gexec);
+            e->gexec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+        if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(he));
+        int rc = 0;
+        // RK stages outermost so the channels' kernels of one stage are adjacent in the graph
+        for (int ns = 1; ns <= 3 && !rc; ++ns)
+            for (int i = 0; i < n && !rc; ++i) {
+                chs_ctx *c = e->ch[i];
+                rc = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt[i], cvisc[i])
+                                          : substep_impl<double>(c, ns, dt[i], cvisc[i]);
+            }
+        he = hipStreamEndCapture(e->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+        he = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
+        e->gdt.assign(dt, dt + n);
+        e->gcv.assign(cvisc, cvisc + n);
+    }
+    hipError_t he = hipGraphLaunch(e->gexec, e->stream);
+    if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));
+    return 0;
+}
+
+int chs_ens_sync(chs_ens *e)
+{
+    if (!e) return fail(nullptr, CHS_E_ARG, "null ensemble");
+    hipError_t he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(he));
     return 0;
 }
 

@@ -41,6 +41,10 @@ SIGNATURES = {
     "chs_substep_tg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
     "chs_step_tg": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "chs_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "chs_set_refinement": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "chs_dev_malloc": (C.c_void_p, [C.c_size_t]),
+    "chs_dev_free": (C.c_int, [C.c_void_p]),
+    "chs_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "chs_poisson": (C.c_int, [C.c_void_p, _dp, _dp]),
     "chs_poisson_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp]),
     "chs_cfl_tg": (C.c_int, [C.c_void_p, C.c_double, _dp]),
@@ -49,6 +53,11 @@ SIGNATURES = {
     "chs_device_fields": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp)]),
     "chs_profile_reset": (C.c_int, [C.c_void_p, C.c_char_p]),
     "chs_profile_get": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_int)]),
+    "chs_ens_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "chs_ens_destroy": (None, [C.c_void_p]),
+    "chs_ens_channel": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "chs_ens_step": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "chs_ens_sync": (C.c_int, [C.c_void_p]),
     "chs_prop_table_setup": (C.c_int, [C.c_int, _dp, C.c_double, _dp, _dp, C.POINTER(C.c_int), _dp]),
     "chs_prop_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "chs_prop_destroy": (None, [C.c_void_p]),
@@ -108,28 +117,37 @@ def channel_setup(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8):
     return out
 
 
+def _coeff_struct(coeffs: dict, keep: list):
+    arrays = {k: np.ascontiguousarray(v, dtype=np.float64)
+              for k, v in coeffs.items() if isinstance(v, np.ndarray)}
+    keep.append(arrays)
+    return _Coeffs(int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"]), coeffs["dxi"],
+                   coeffs["dzi"], coeffs["dxqi"], coeffs["dzqi"], coeffs["volm"],
+                   *[_p(arrays[k]) for k in ("dyfi", "dyci", "wff", "wfb", "amvr", "acvr",
+                                             "apvr", "amph", "acph", "apph", "ak1", "ak3",
+                                             "tgam", "troh", "talp")])
+
+
 class ChannelTG:
     """One periodic channel resident on the GPU (TG domain, flow only)."""
 
-    def __init__(self, coeffs: dict, mode: int = PLAIN_FP64):
+    def __init__(self, coeffs: dict, mode: int = PLAIN_FP64, handle=None, owner=None):
         self.L, self.M, self.N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
         self.mode = mode
-        self._arrays = {k: np.ascontiguousarray(v, dtype=np.float64)
-                        for k, v in coeffs.items() if isinstance(v, np.ndarray)}
-        a = self._arrays
-        self._c = _Coeffs(self.L, self.M, self.N, coeffs["dxi"], coeffs["dzi"], coeffs["dxqi"],
-                          coeffs["dzqi"], coeffs["volm"],
-                          *[_p(a[k]) for k in ("dyfi", "dyci", "wff", "wfb", "amvr", "acvr",
-                                               "apvr", "amph", "acph", "apph", "ak1", "ak3",
-                                               "tgam", "troh", "talp")])
+        self._keep = []
+        self._owner = owner
+        if handle is not None:          # a channel of an Ensemble (destroyed by it)
+            self._h = handle
+            return
+        self._c = _coeff_struct(coeffs, self._keep)
         h = C.c_void_p()
         _check(lib().chs_create(C.byref(self._c), mode, C.byref(h)))
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and self._owner is None:
             lib().chs_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -172,6 +190,9 @@ class ChannelTG:
         """Replay rk3_step from a hipGraph captured per (dt, cvisc)."""
         _check(lib().chs_set_graph(self._h, 1 if enable else 0), self._h)
 
+    def set_refinement(self, nref_step: int, nref_projection: int):
+        _check(lib().chs_set_refinement(self._h, int(nref_step), int(nref_projection)), self._h)
+
     def poisson(self, rhs):
         L, M, N = self.L, self.M, self.N
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
@@ -210,3 +231,77 @@ class ChannelTG:
         n = C.c_int()
         _check(lib().chs_profile_get(self._h, kernel.encode(), C.byref(avg), C.byref(n)), self._h)
         return avg.value, n.value
+
+
+class Ensemble:
+    """Independent channels of uneven size advanced together: one hipGraph replays one RK3
+    step of every channel (chs_ens_step)."""
+
+    def __init__(self, coeff_list, mode: int = PLAIN_FP64):
+        self._keep = []
+        structs = [_coeff_struct(c, self._keep) for c in coeff_list]
+        self._structs = structs
+        ptrs = (C.c_void_p * len(structs))(*[C.cast(C.pointer(s), C.c_void_p) for s in structs])
+        h = C.c_void_p()
+        _check(lib().chs_ens_create(len(structs), ptrs, mode, C.byref(h)))
+        self._h = h
+        self.channels = []
+        for i, co in enumerate(coeff_list):
+            ch = C.c_void_p()
+            _check(lib().chs_ens_channel(self._h, i, C.byref(ch)))
+            self.channels.append(ChannelTG(co, mode, handle=ch, owner=self))
+
+    def step(self, dt, cvisc):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        cv = np.ascontiguousarray(cvisc, dtype=np.float64)
+        _check(lib().chs_ens_step(self._h, _p(dt), _p(cv)))
+
+    def sync(self):
+        _check(lib().chs_ens_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for ch in self.channels:
+                ch._h = None
+            lib().chs_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A float64 device buffer from chs_dev_malloc (for device-pointer entry points)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.ptr = lib().chs_dev_malloc(self.nbytes)
+        if not self.ptr:
+            raise ChsError("chs_dev_malloc failed")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        _check(lib().chs_memcpy(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), d.nbytes, 1))
+        return d
+
+    def to_host(self):
+        a = np.empty(self.shape)
+        _check(lib().chs_memcpy(a.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 0))
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib().chs_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -17,6 +17,8 @@
 #ifndef CHAPSIM_HIP_H
 #define CHAPSIM_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -107,6 +109,17 @@ int chs_step_tg(chs_ctx *ctx, double dt, double cvisc);
 /* Enable (1) or disable (0, default) hipGraph replay of chs_step_tg. */
 int chs_set_graph(chs_ctx *ctx, int enable);
 
+/* FP64-residual refinement sweeps of the pressure correction after each substep (0..4): the
+ * divergence of the corrected velocity drives another Poisson solve and correction.  Default
+ * 0 per substep and 2 for the NS = 0 projection in the mixed form, 0 / 0 in plain FP64. */
+int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
+
+/* Device memory helpers for hosts without their own allocator (e.g. the standalone Poisson
+ * workload): plain hipMalloc / hipFree / synchronous hipMemcpy (to_device 1: host->device). */
+void *chs_dev_malloc(size_t bytes);
+int chs_dev_free(void *p);
+int chs_memcpy(void *dst, const void *src, size_t bytes, int to_device);
+
 /* Standalone pressure Poisson solve of FFT99_POIS3D_periodicxz (possion3d_FFT99.f90:
  * 271-505) on the context's mesh.  rhs: RHSLLPHI(1:L,1:M,1:N); phi: DPH(1:L,1:M,1:N)
  * (interior rows).  Host arrays; synchronous. */
@@ -134,6 +147,21 @@ int chs_device_fields(chs_ctx *ctx, double **q, double **pr);
  * average device time (ms) of one named kernel's launches and their count. */
 int chs_profile_reset(chs_ctx *ctx, const char *only);
 int chs_profile_get(chs_ctx *ctx, const char *kernel, double *avg_ms, int *count);
+
+/* ---- Ensembles of independent channels ----------------------------------------------- */
+
+typedef struct chs_ens chs_ens;
+
+/* n channels (each its own mesh, chs_coeffs) sharing one stream.  Each channel is a chs_ctx
+ * (chs_ens_channel) for upload/download/diagnostics. */
+int chs_ens_create(int n, const chs_coeffs *const *coeffs, int mode, chs_ens **out);
+void chs_ens_destroy(chs_ens *e);
+int chs_ens_channel(chs_ens *e, int i, chs_ctx **ctx);
+
+/* One RK3 step of every channel (dt[i], cvisc[i]) replayed from a single hipGraph, captured
+ * once per set of (dt, cvisc); stage-major order (all channels' NS = 1, then NS = 2, 3). */
+int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc);
+int chs_ens_sync(chs_ens *e);
 
 /* ---- Thermophysical property tables (heated channel, table mode) ---------------------- */
 

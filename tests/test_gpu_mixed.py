@@ -7,9 +7,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def run(pkg, mode, L, M, N, STR2, REN, dt, steps, seed=9):
+def run(pkg, mode, L, M, N, STR2, REN, dt, steps, seed=9, nref_step=0):
     co = pkg.channel_setup(L, M, N, STR2=STR2)
     ch = pkg.ChannelTG(co, mode)
+    if mode == pkg.MIXED:
+        ch.set_refinement(nref_step, 2)
     q, p = pkg.poiseuille_perturbed(co, seed, 0.1)
     ch.upload(q, p)
     ch.SOLVERRK3_MOM_tg(0, dt, 1.0 / REN)
@@ -37,3 +39,11 @@ def test_mixed_vs_plain(pkg, L, M, N, STR2, REN, dt):
     assert e_mean < 1e-5
     assert e_vel < 1e-4
     assert max(dm) < 1e-9
+
+
+def test_mixed_refinement_sweep_lowers_divergence(pkg):
+    _, _, d0, _ = run(pkg, pkg.MIXED, 64, 48, 64, 1.8, 2800.0, 1e-3, 3, nref_step=0)
+    _, _, d1, _ = run(pkg, pkg.MIXED, 64, 48, 64, 1.8, 2800.0, 1e-3, 3, nref_step=1)
+    print(f"mixed max div per step, 0 sweeps {d0}, 1 sweep {d1}")
+    assert max(d1[1:]) < max(d0[1:])
+    assert max(d1) < 1e-12
