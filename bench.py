@@ -93,8 +93,8 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
         # R u,v,w; W div (FP64) + spectrum
         "divg_fftx": 3 * F * 8 + F * 8 + S * 2 * wbytes,
         "fftz_fwd": 2 * S * 2 * wbytes,
-        # R spectrum + pivots (FP64), W spectrum
-        "ytdma": 2 * S * 2 * wbytes + S * 8,
+        # R spectrum + pivots (work precision; FP64 for six low modes), W spectrum
+        "ytdma": 2 * S * 2 * wbytes + S * wbytes,
         "fftz_inv": 2 * S * 2 * wbytes,
         # R spectrum, W phi
         "fftx_inv": S * 2 * wbytes + F * 8,
@@ -115,7 +115,7 @@ def load_pkg():
     return __graft_entry__.load()
 
 
-def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False):
+def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False):
     c = c or CFG
     coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
@@ -126,17 +126,22 @@ def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False):
     dt = c["dt"]
     ch.SOLVERRK3_MOM_tg(0, dt, cv)
     q0, p0 = ch.download()          # projected start, shared with the CPU baseline
-    # profiling pre-pass: which kernel dominates a step
     if form == "mixed":
         ch.set_refinement(0, 2)
+    tuning = None
+    if autotune:
+        from chapsim_amd import tune
+        tuning = tune.autotune(ch, dt, cv, steps=3, verbose=True)["best"]
+    # profiling pre-pass: which kernel dominates a step
     ch.profile_reset(None)
     for _ in range(2):
         ch.rk3_step(dt, cv)
     ch.sync()
-    per = {}
+    per, launches = {}, {}
     for k in KERNELS:
         avg, n = ch.profile_get(k)
         per[k] = avg * n / 2.0          # ms per step
+        launches[k] = n / 2.0
     dom = max(per, key=per.get)
     pre_avg, pre_n = ch.profile_get(dom)
     for _ in range(warmup):
@@ -161,9 +166,9 @@ def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False):
     div = ch.max_div()
     ch.close()
     ms = (t1 - t0) * 1e3 / steps
-    return dict(ms_per_step=ms, per_kernel_ms=per, dominant=dom, dom_avg_ms=dom_avg_ms,
-                dom_count=dom_n, cfl=cfl, max_div=div, wbytes=4 if form == "mixed" else 8,
-                start=(q0, p0))
+    return dict(ms_per_step=ms, per_kernel_ms=per, launches=launches, dominant=dom,
+                dom_avg_ms=dom_avg_ms, dom_count=dom_n, cfl=cfl, max_div=div,
+                wbytes=4 if form == "mixed" else 8, start=(q0, p0), tuning=tuning)
 
 
 def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = None, c=None):
@@ -330,6 +335,8 @@ def main():
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay each RK3 step from a hipGraph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="search the kernels' launch shapes (chapsim_amd.tune) before timing")
     ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble"],
                     default="channel")
     ap.add_argument("--nref", type=int, default=1, help="refinement sweeps (poisson, mixed)")
@@ -355,7 +362,7 @@ def main():
     if args.workload == "ensemble":
         return bench_ensemble(pkg, args)
     c = CFG64 if args.workload == "channel64" else CFG
-    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph)
+    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune)
     F = c["L"] * c["M"] * c["N"]
     value = F / (r["ms_per_step"] * 1e-3)
     dom = r["dominant"]
@@ -376,7 +383,8 @@ def main():
             err = cpu.get("error") if cpu else "reference build absent"
             cpu = cpu_baseline_port()
             cpu["note"] = f"reference run unavailable: {err}"
-    step_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) for k in KERNELS) * 3
+    step_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) * n
+                     for k, n in r["launches"].items())
     out = {
         "metric": "grid-point RK3-steps/s, mixed vs plain FP64 at stated error; % of HBM peak",
         "value": value,
@@ -394,6 +402,7 @@ def main():
                                 f"4pi x 2 x 2pi, tanh beta={c['STR2']:g}, dt={c['dt']:g}, "
                                 "RK3+CN, flow only"),
                    "graph": bool(args.graph),
+                   "tuning": r["tuning"] or "defaults",
                    "form": "plain_fp64" if args.form == "plain" else "mixed",
                    "grid": [c["L"], c["M"], c["N"]]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

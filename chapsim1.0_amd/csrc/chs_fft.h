@@ -69,8 +69,32 @@ struct FftPlan {
     int radix[16];
 };
 
-// Largest number of butterflies one thread holds in registers during an in-place pass.
-constexpr int kMaxBfly = 4;
+// Global -> LDS staging of n items by a block: ld(t) reads item t, st(t, v) stores it.  The U
+// reads of one thread are all issued before the first store, so U memory latencies overlap
+// instead of following one another.
+template <int U, typename V, typename Ld, typename St>
+__device__ __forceinline__ void stage_items(int n, Ld ld, St st)
+{
+    const int nthr = blockDim.x * blockDim.y;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int t0 = tid; t0 < n; t0 += U * nthr) {
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * nthr;
+            if (t < n) v[u] = ld(t);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * nthr;
+            if (t < n) st(t, v[u]);
+        }
+    }
+}
+
+// Largest number of line points per thread of a block (nlines * n <= kMaxPts * nthr); a
+// radix-R pass then holds ceil(kMaxPts / R) butterflies of one thread in registers.
+constexpr int kMaxPts = 8;
 
 template <typename T, int R>
 __device__ __forceinline__ void fft_pass_inplace(cpx<T> *a, int ls, int nlines, int n, int Ns,
@@ -79,9 +103,10 @@ __device__ __forceinline__ void fft_pass_inplace(cpx<T> *a, int ls, int nlines, 
     const int nb = n / R;
     const int tstride = n / (Ns * R);
     const int total = nb * nlines;
-    cpx<T> v[kMaxBfly][R];
+    constexpr int kB = (kMaxPts + R - 1) / R;
+    cpx<T> v[kB][R];
 #pragma unroll
-    for (int b = 0; b < kMaxBfly; ++b) {
+    for (int b = 0; b < kB; ++b) {
         const int t = tid + b * nthr;
         if (t < total) {
             const int line = t / nb;
@@ -102,7 +127,7 @@ __device__ __forceinline__ void fft_pass_inplace(cpx<T> *a, int ls, int nlines, 
     }
     __syncthreads();
 #pragma unroll
-    for (int b = 0; b < kMaxBfly; ++b) {
+    for (int b = 0; b < kB; ++b) {
         const int t = tid + b * nthr;
         if (t < total) {
             const int line = t / nb;
@@ -121,7 +146,7 @@ __device__ __forceinline__ void fft_pass_inplace(cpx<T> *a, int ls, int nlines, 
 // In-place Stockham transform of `nlines` lines of length plan.n stored at
 // a[line * ls + idx] in LDS; tw holds exp(-2 pi i m / n), m = 0..n-1 (LDS).  Each pass reads
 // all butterfly inputs into registers, synchronises, then writes.  Requires
-// nlines * n / (smallest radix) <= kMaxBfly * nthr.  All threads of the block must call.
+// nlines * n <= kMaxPts * nthr.  All threads of the block must call.
 template <typename T>
 __device__ void fft_lds_inplace(cpx<T> *a, int ls, int nlines, const FftPlan &plan,
                                 const cpx<T> *tw, int sign, int tid, int nthr)

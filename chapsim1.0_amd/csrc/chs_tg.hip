@@ -64,113 +64,124 @@ struct Sub {
 };
 
 // ---------------------------------------------------------------------------------------
-// k_conv_rhs: one thread per cell (i, J, k).  Writes rhs[c] = (gamma H + zeta H0 + alpha nu
-// Lap q_c) dt - alpha dt grad_c p (RHS_CvLpGpS_tg.f90:95-160) and H0[c] = H; v only for
-// J >= 2 (NII = 2 on the bottom rank).  Also block partial sums of rhs_u / DYFI
-// (RHS_CvLpGpS_tg.f90:171-178) in FP64.
-template <typename W>
-__global__ __launch_bounds__(256) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
-                                                  const double *__restrict__ pr,
-                                                  W *__restrict__ h0, W *__restrict__ rhs,
-                                                  double *__restrict__ partial, int KC)
+// k_conv_rhs: one thread per column (i, J), marching KC z-planes.  Writes rhs[c] = (gamma H +
+// zeta H0 + alpha nu Lap q_c) dt - alpha dt grad_c p (RHS_CvLpGpS_tg.f90:95-160) and H0[c] = H;
+// v only for J >= 2 (NII = 2 on the bottom rank).  Also block partial sums of rhs_u / DYFI
+// (RHS_CvLpGpS_tg.f90:171-178) in FP64.  Values a column needs again on the next plane (its
+// own u, v, w, p and v(J+1) of plane k, and the plane k+1 values read for the z-differences)
+// are carried in registers, so each plane costs 15 loads of plane k, one of plane k-1, 5 of
+// plane k+1 and the 3 history values.
+template <typename W, int BY>
+__global__ __launch_bounds__(64 * BY) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
+                                                      const double *__restrict__ pr,
+                                                      W *__restrict__ h0, W *__restrict__ rhs,
+                                                      double *__restrict__ partial, int KC)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    // blockDim = (64, 4): a wavefront is one row J, so J-indexed coefficients are uniform
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
-    const size_t hs = (size_t)g.N * (g.M + 2) * g.L;   // halo field size
-    const size_t cs = (size_t)g.N * g.M * g.L;         // cell field size
+    // blockDim = (64, BY): a wavefront is one row J, so J-indexed coefficients are uniform
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * BY + threadIdx.y + 1);
     double mf = 0.0;
-    // each thread marches over KC consecutive z-planes so the k-neighbours it reads were
-    // loaded by its own block one plane earlier (cache hits instead of HBM re-reads)
     const int kbeg = blockIdx.z * KC, kend = min(g.N, kbeg + KC);
-    for (int k = kbeg; k < kend; ++k)
     if (i < g.L && J <= g.M) {
-        const int L = g.L;
+        const int L = g.L, N = g.N;
+        const size_t hs = (size_t)N * (g.M + 2) * L;   // halo field size
+        const size_t cs = (size_t)N * g.M * L;         // cell field size
         const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
-        const int kp = (k == g.N - 1) ? 0 : k + 1, km = (k == 0) ? g.N - 1 : k - 1;
         const double *u = q, *v = q + hs, *w = q + 2 * hs;
+        auto at = [&](const double *f, int ii, int jj, int kk) { return f[hix(g, ii, jj, kk)]; };
         // State is FP64.  Sums and differences of state values (and the viscous and
         // pressure-gradient stencils, whose rows cancel) are formed in FP64; the convection
         // products run in the work precision W.
-        auto U = [&](int ii, int jj, int kk) { return u[hix(g, ii, jj, kk)]; };
-        auto V = [&](int ii, int jj, int kk) { return v[hix(g, ii, jj, kk)]; };
-        auto Wv = [&](int ii, int jj, int kk) { return w[hix(g, ii, jj, kk)]; };
-        auto P = [&](int ii, int jj, int kk) { return pr[hix(g, ii, jj, kk)]; };
         const double wffJ = g.wff[J], wfbJ = g.wfb[J], wffP = g.wff[J + 1], wfbP = g.wfb[J + 1];
-        const double u0 = U(i, J, k), v0 = V(i, J, k), w0 = Wv(i, J, k), p0 = P(i, J, k);
+        const double dyf = g.dyfi[J - 1], dyc = g.dyci[J];
+        const int o1 = g.M + 2, o2 = 2 * (g.M + 2);
+        const double apu = g.apvr[J], acu = g.acvr[J], amu = g.amvr[J];
+        const double apv = g.apvr[o1 + J], acv = g.acvr[o1 + J], amv = g.amvr[o1 + J];
+        const double apw = g.apvr[o2 + J], acw = g.acvr[o2 + J], amw = g.amvr[o2 + J];
         const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)s.dt;
         const double coev = s.talp * s.cvisc, coep = s.talp * s.dt;
-        const size_t c0 = cix(g, i, J, k);
-        // ---- u (CONVECTION_X.f90:31-59) ----
-        {
-            const W coe1 = (W)(0.25 * g.dxi), coe2 = (W)(g.dyfi[J - 1] * 0.5), coe3 = (W)(0.25 * g.dzi);
-            const double uip = U(ip, J, k), uim = U(im, J, k), ujp = U(i, J + 1, k), ujm = U(i, J - 1, k);
-            const double ukp = U(i, J, kp), ukm = U(i, J, km);
-            const W a1 = (W)(uip + u0), a2 = (W)(uim + u0);
-            const W H11 = (a1 * a1 - a2 * a2) * coe1;
-            const W H12 = ((W)(V(i, J + 1, k) + V(im, J + 1, k)) * (W)(wffP * ujp + wfbP * u0) -
-                           (W)(v0 + V(im, J, k)) * (W)(wffJ * u0 + wfbJ * ujm)) * coe2;
-            const W H13 = ((W)(Wv(i, J, kp) + Wv(im, J, kp)) * (W)(ukp + u0) -
-                           (W)(w0 + Wv(im, J, k)) * (W)(u0 + ukm)) * coe3;
-            const W conv = -(H11 + H12 + H13);
-            const W rhsc = tgam * conv + troh * h0[c0];
-            const double lap = (uip - 2.0 * u0 + uim) * g.dxqi + (ukp - 2.0 * u0 + ukm) * g.dzqi +
-                               (ujp * g.apvr[J] + u0 * g.acvr[J] + ujm * g.amvr[J]);
-            h0[c0] = conv;
-            const W pgm = (W)((p0 - P(im, J, k)) * g.dxi * coep);
-            const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
-            rhs[c0] = r;
-            mf += (double)r / g.dyfi[J - 1];
-        }
-        // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 ----
-        if (J >= 2) {
-            const W coe1 = (W)(g.dxi * 0.5), coe2 = (W)(g.dyci[J] * 0.25), coe31 = (W)(g.dzi * 0.5);
-            const double vip = V(ip, J, k), vim = V(im, J, k), vjp = V(i, J + 1, k), vjm = V(i, J - 1, k);
-            const double vkp = V(i, J, kp), vkm = V(i, J, km);
-            const W H21 = ((W)(wffJ * U(ip, J, k) + wfbJ * U(ip, J - 1, k)) * (W)(vip + v0) -
-                           (W)(wffJ * u0 + wfbJ * U(i, J - 1, k)) * (W)(v0 + vim)) * coe1;
-            const W b1 = (W)(vjp + v0), b2 = (W)(v0 + vjm);
-            const W H22 = (b1 * b1 - b2 * b2) * coe2;
-            const W H23 = ((W)(wffJ * Wv(i, J, kp) + wfbJ * Wv(i, J - 1, kp)) * (W)(vkp + v0) -
-                           (W)(wffJ * w0 + wfbJ * Wv(i, J - 1, k)) * (W)(v0 + vkm)) * coe31;
-            const W conv = -(H21 + H22 + H23);
-            const size_t cc = cs + c0;
-            const W rhsc = tgam * conv + troh * h0[cc];
-            const int o = g.M + 2;
-            const double lap = (vip - 2.0 * v0 + vim) * g.dxqi + (vkp - 2.0 * v0 + vkm) * g.dzqi +
-                               (vjp * g.apvr[o + J] + v0 * g.acvr[o + J] + vjm * g.amvr[o + J]);
-            h0[cc] = conv;
-            const W pgm = (W)((p0 - P(i, J - 1, k)) * g.dyci[J] * coep);
-            rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
-        }
-        // ---- w (CONVECTION_Z.f90:33-69) ----
-        {
-            const W coe1 = (W)(0.25 * g.dxi), coe2 = (W)(g.dyfi[J - 1] * 0.5), coe3 = (W)(0.25 * g.dzi);
-            const double wip = Wv(ip, J, k), wim = Wv(im, J, k), wkp = Wv(i, J, kp), wkm = Wv(i, J, km);
-            const double wjp = Wv(i, J + 1, k), wjm = Wv(i, J - 1, k);
-            const W H31 = ((W)(U(ip, J, k) + U(ip, J, km)) * (W)(wip + w0) -
-                           (W)(u0 + U(i, J, km)) * (W)(w0 + wim)) * coe1;
-            const W H32 = ((W)(V(i, J + 1, k) + V(i, J + 1, km)) * (W)(wffP * wjp + wfbP * w0) -
-                           (W)(v0 + V(i, J, km)) * (W)(wffJ * w0 + wfbJ * wjm)) * coe2;
-            const W c1 = (W)(wkp + w0), c2 = (W)(w0 + wkm);
-            const W H33 = (c1 * c1 - c2 * c2) * coe3;
-            const W conv = -(H31 + H32 + H33);
-            const size_t cc = 2 * cs + c0;
-            const W rhsc = tgam * conv + troh * h0[cc];
-            const int o = 2 * (g.M + 2);
-            const double lap = (wip - 2.0 * w0 + wim) * g.dxqi + (wkp - 2.0 * w0 + wkm) * g.dzqi +
-                               (wjp * g.apvr[o + J] + w0 * g.acvr[o + J] + wjm * g.amvr[o + J]);
-            h0[cc] = conv;
-            const W pgm = (W)((p0 - P(i, J, km)) * g.dzi * coep);
-            rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
+        const W cu1 = (W)(0.25 * g.dxi), cu2 = (W)(dyf * 0.5), cu3 = (W)(0.25 * g.dzi);
+        const W cv1 = (W)(g.dxi * 0.5), cv2 = (W)(dyc * 0.25), cv3 = (W)(g.dzi * 0.5);
+        // carried: plane k-1 (_m) and plane k (_c)
+        const int km0 = (kbeg == 0) ? N - 1 : kbeg - 1;
+        double u_m = at(u, i, J, km0), v_m = at(v, i, J, km0), w_m = at(w, i, J, km0);
+        double p_m = at(pr, i, J, km0), vjp_m = at(v, i, J + 1, km0);
+        double u_c = at(u, i, J, kbeg), v_c = at(v, i, J, kbeg), w_c = at(w, i, J, kbeg);
+        double wim_c = at(w, im, J, kbeg), wjm_c = at(w, i, J - 1, kbeg);
+        for (int k = kbeg; k < kend; ++k) {
+            const int kp = (k == N - 1) ? 0 : k + 1, km = (k == 0) ? N - 1 : k - 1;
+            // plane k+1
+            const double u_p = at(u, i, J, kp), v_p = at(v, i, J, kp), w_p = at(w, i, J, kp);
+            const double wim_p = at(w, im, J, kp), wjm_p = at(w, i, J - 1, kp);
+            // plane k
+            const double uip = at(u, ip, J, k), uim = at(u, im, J, k), ujp = at(u, i, J + 1, k);
+            const double ujm = at(u, i, J - 1, k);
+            const double vjp = at(v, i, J + 1, k), vimjp = at(v, im, J + 1, k), vim = at(v, im, J, k);
+            const double p0 = at(pr, i, J, k), pim = at(pr, im, J, k);
+            const double wip = at(w, ip, J, k), wjp = at(w, i, J + 1, k);
+            const size_t c0 = cix(g, i, J, k);
+            // ---- u (CONVECTION_X.f90:31-59) ----
+            {
+                const W a1 = (W)(uip + u_c), a2 = (W)(uim + u_c);
+                const W H11 = (a1 * a1 - a2 * a2) * cu1;
+                const W H12 = ((W)(vjp + vimjp) * (W)(wffP * ujp + wfbP * u_c) -
+                               (W)(v_c + vim) * (W)(wffJ * u_c + wfbJ * ujm)) * cu2;
+                const W H13 = ((W)(w_p + wim_p) * (W)(u_p + u_c) - (W)(w_c + wim_c) * (W)(u_c + u_m)) * cu3;
+                const W conv = -(H11 + H12 + H13);
+                const W rhsc = tgam * conv + troh * h0[c0];
+                const double lap = (uip - 2.0 * u_c + uim) * g.dxqi + (u_p - 2.0 * u_c + u_m) * g.dzqi +
+                                   (ujp * apu + u_c * acu + ujm * amu);
+                h0[c0] = conv;
+                const W pgm = (W)((p0 - pim) * g.dxi * coep);
+                const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
+                rhs[c0] = r;
+                mf += (double)r / dyf;
+            }
+            // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 ----
+            if (J >= 2) {
+                const double vip = at(v, ip, J, k), vjm = at(v, i, J - 1, k);
+                const double uipjm = at(u, ip, J - 1, k), pjm = at(pr, i, J - 1, k);
+                const W H21 = ((W)(wffJ * uip + wfbJ * uipjm) * (W)(vip + v_c) -
+                               (W)(wffJ * u_c + wfbJ * ujm) * (W)(v_c + vim)) * cv1;
+                const W b1 = (W)(vjp + v_c), b2 = (W)(v_c + vjm);
+                const W H22 = (b1 * b1 - b2 * b2) * cv2;
+                const W H23 = ((W)(wffJ * w_p + wfbJ * wjm_p) * (W)(v_p + v_c) -
+                               (W)(wffJ * w_c + wfbJ * wjm_c) * (W)(v_c + v_m)) * cv3;
+                const W conv = -(H21 + H22 + H23);
+                const size_t cc = cs + c0;
+                const W rhsc = tgam * conv + troh * h0[cc];
+                const double lap = (vip - 2.0 * v_c + vim) * g.dxqi + (v_p - 2.0 * v_c + v_m) * g.dzqi +
+                                   (vjp * apv + v_c * acv + vjm * amv);
+                h0[cc] = conv;
+                const W pgm = (W)((p0 - pjm) * dyc * coep);
+                rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
+            }
+            // ---- w (CONVECTION_Z.f90:33-69) ----
+            {
+                const W H31 = ((W)(uip + at(u, ip, J, km)) * (W)(wip + w_c) - (W)(u_c + u_m) * (W)(w_c + wim_c)) * cu1;
+                const W H32 = ((W)(vjp + vjp_m) * (W)(wffP * wjp + wfbP * w_c) -
+                               (W)(v_c + v_m) * (W)(wffJ * w_c + wfbJ * wjm_c)) * cu2;
+                const W c1 = (W)(w_p + w_c), c2 = (W)(w_c + w_m);
+                const W H33 = (c1 * c1 - c2 * c2) * cu3;
+                const W conv = -(H31 + H32 + H33);
+                const size_t cc = 2 * cs + c0;
+                const W rhsc = tgam * conv + troh * h0[cc];
+                const double lap = (wip - 2.0 * w_c + wim_c) * g.dxqi + (w_p - 2.0 * w_c + w_m) * g.dzqi +
+                                   (wjp * apw + w_c * acw + wjm_c * amw);
+                h0[cc] = conv;
+                const W pgm = (W)((p0 - p_m) * g.dzi * coep);
+                rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
+            }
+            u_m = u_c; v_m = v_c; w_m = w_c; p_m = p0; vjp_m = vjp;
+            u_c = u_p; v_c = v_p; w_c = w_p; wim_c = wim_p; wjm_c = wjm_p;
         }
     }
     // block reduction of the mass-flux partial (fixed order -> deterministic)
-    __shared__ double red[256];
+    __shared__ double red[64 * BY];
     const int t = threadIdx.y * 64 + threadIdx.x;
     red[t] = mf;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = 32 * BY; o > 0; o >>= 1) {
         if (t < o) red[t] += red[t + o];
         __syncthreads();
     }
@@ -492,7 +503,7 @@ __device__ __forceinline__ size_t six(const Geo &g, int J, int kx, int kz)
 // by 1/(L N) (the 1/L of FFT99 and the 1/N applied after CFFT99, possion3d_FFT99.f90:316,
 // :327-334) and written as spec[J][kx][k0..k0+TK-1] (TK consecutive values per kx).
 template <typename T>
-__global__ __launch_bounds__(256) void k_divg_fftx(Geo g, PoisArgs pa, double coe0, const double *__restrict__ q,
+__global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, double coe0, const double *__restrict__ q,
                                                    double *__restrict__ div, cpx<T> *__restrict__ spec,
                                                    int TK, const double *__restrict__ src)
 {
@@ -506,26 +517,36 @@ __global__ __launch_bounds__(256) void k_divg_fftx(Geo g, PoisArgs pa, double co
     const int np = TK / 2;
     const size_t hs = (size_t)g.N * (g.M + 2) * L;
     const double *u = q, *v = q + hs, *w = q + 2 * hs;
-    for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
-        const int p = t / L, i = t - p * L;
-        T val[2];
-        for (int h = 0; h < 2; ++h) {
-            const int k = k0 + 2 * p + h;
-            double dv;
-            if (src) {
-                dv = src[cix(g, i, J, k)];
-            } else {
-                const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
-                const double d = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
-                                 (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * g.dyfi[J - 1] +
-                                 (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
-                dv = d * coe0;
-                div[cix(g, i, J, k)] = dv;
+    struct D2 { double d[2]; };
+    const double dyf = g.dyfi[J - 1];
+    chs::stage_items<4, D2>(
+        np * L,
+        [&](int t) {
+            const int p = t / L, i = t - p * L;
+            D2 r;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = k0 + 2 * p + h;
+                if (src) {
+                    r.d[h] = src[cix(g, i, J, k)];
+                } else {
+                    const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
+                    const double d = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
+                                     (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * dyf +
+                                     (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
+                    r.d[h] = d * coe0;
+                }
             }
-            val[h] = (T)dv;
-        }
-        a[p * ls + i] = {val[0], val[1]};
-    }
+            return r;
+        },
+        [&](int t, const D2 &r) {
+            const int p = t / L, i = t - p * L;
+            if (!src) {
+                div[cix(g, i, J, k0 + 2 * p)] = r.d[0];
+                div[cix(g, i, J, k0 + 2 * p + 1)] = r.d[1];
+            }
+            a[p * ls + i] = {(T)r.d[0], (T)r.d[1]};
+        });
     __syncthreads();
     chs::fft_lds_inplace<T>(a, ls, np, pa.px, tw, -1, threadIdx.x, blockDim.x);
     const T scale = T(1.0 / ((double)L * (double)g.N));
@@ -556,10 +577,12 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
     const int l0 = blockIdx.x * NL;
     const int nl = min(NL, nlines - l0);
     cpx<T> *base = spec + (size_t)l0 * N;
-    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
-        const int ln = t / N, kz = t - ln * N;
-        a[ln * ls + kz] = base[t];
-    }
+    chs::stage_items<4, cpx<T>>(
+        nl * N, [&](int t) { return base[t]; },
+        [&](int t, const cpx<T> &x) {
+            const int ln = t / N, kz = t - ln * N;
+            a[ln * ls + kz] = x;
+        });
     __syncthreads();
     chs::fft_lds_inplace<T>(a, ls, nl, pa.pz, tw, sign, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
@@ -570,8 +593,91 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
 
 // Per-mode y-Thomas (TDMAIJJ_nonCYC.f90:84-120 on the FFT99 coefficients,
 // possion3d_FFT99.f90:213-231): pp[J][kx][kz] = 1/(b_J - h_{J-1} a_J) precomputed in FP64,
-// h_J = c_J pp_J; the (kx,kz) = (0,0) mode has RHS(J=1) = 0 and row 1 decoupled.  Loads are
-// issued four rows ahead of the recurrence.
+// h_J = c_J pp_J; the (kx,kz) = (0,0) mode has RHS(J=1) = 0 and row 1 decoupled.  P is the
+// pivot type of the line (FP64, or FP32 for the well-conditioned modes of the mixed form);
+// the arithmetic is FP64 either way.
+template <typename T, typename P>
+__device__ __forceinline__ void ytdma_line(const int M, const double *__restrict__ amph,
+                                           const double *__restrict__ apph, const P *__restrict__ p,
+                                           cpx<T> *__restrict__ s, const size_t st, const bool zero)
+{
+    // Rows go in batches of B; the loads of the next batch are issued before the current
+    // batch's recurrence (double buffer in registers), and pivots stay in their storage type
+    // until used, so no load waits on a conversion.
+    constexpr int B = 8;
+    cpx<T> rc[B], rn[B];
+    P pc[B], pn[B];
+    double gr = 0.0, gi = 0.0;
+    auto fwd = [&](int J, cpx<T> r, double pj) {
+        double rr = (double)r.x, ri = (double)r.y;
+        if (zero && J == 1) { rr = 0.0; ri = 0.0; }
+        const double a = (J == 1) ? 0.0 : amph[J - 1];
+        gr = (rr - a * gr) * pj;
+        gi = (ri - a * gi) * pj;
+        s[(size_t)(J - 1) * st] = {(T)gr, (T)gi};
+    };
+    // forward elimination, rows 1..M: full batches, then the remainder row by row
+    const int nf = M / B;
+    if (nf > 0) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            rc[u] = s[(size_t)u * st];
+            pc[u] = p[(size_t)u * st];
+        }
+    }
+    for (int b = 0; b < nf; ++b) {
+        const int J0 = 1 + b * B;
+        const int Jn = (b + 1 < nf) ? J0 + B : J0;   // the last batch re-reads its own rows
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            rn[u] = s[(size_t)(Jn + u - 1) * st];
+            pn[u] = p[(size_t)(Jn + u - 1) * st];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) fwd(J0 + u, rc[u], (double)pc[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            rc[u] = rn[u];
+            pc[u] = pn[u];
+        }
+    }
+    for (int J = nf * B + 1; J <= M; ++J) fwd(J, s[(size_t)(J - 1) * st], (double)p[(size_t)(J - 1) * st]);
+    // back substitution x_J = g_J - c_J p_J x_{J+1}, rows M-1 down to 1 (x_M = g_M)
+    double xr = gr, xi = gi;
+    auto bwd = [&](int J, cpx<T> gg, double pj) {
+        const double c = (zero && J == 1) ? 0.0 : apph[J - 1];
+        const double h = c * pj;
+        xr = (double)gg.x - h * xr;
+        xi = (double)gg.y - h * xi;
+        s[(size_t)(J - 1) * st] = {(T)xr, (T)xi};
+    };
+    const int nb = (M - 1) / B;
+    if (nb > 0) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            rc[u] = s[(size_t)(M - 2 - u) * st];
+            pc[u] = p[(size_t)(M - 2 - u) * st];
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        const int J0 = M - 1 - b * B;                 // rows J0, J0-1, ..., J0-B+1
+        const int Jn = (b + 1 < nb) ? J0 - B : J0;
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            rn[u] = s[(size_t)(Jn - u - 1) * st];
+            pn[u] = p[(size_t)(Jn - u - 1) * st];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) bwd(J0 - u, rc[u], (double)pc[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            rc[u] = rn[u];
+            pc[u] = pn[u];
+        }
+    }
+    for (int J = M - 1 - nb * B; J >= 1; --J) bwd(J, s[(size_t)(J - 1) * st], (double)p[(size_t)(J - 1) * st]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__ amph, const double *__restrict__ apph,
                                                const double *__restrict__ pp, const float *__restrict__ pp32,
@@ -582,66 +688,19 @@ __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__
     if (t >= H * N) return;
     const int kx = t / N, kz = t - kx * N;
     const bool zero = (kx == 0 && kz == 0);
+    const size_t st = (size_t)H * N, off = (size_t)kx * N + kz;
     // FP32 pivots for all but the lowest modes (kx <= 1, |kz| <= 1), whose Neumann/near-
     // Neumann y-systems are ill-conditioned on the stretched mesh
     const bool low = !pp32 || (kx <= 1 && (kz <= 1 || kz == N - 1));
-    const size_t st = (size_t)H * N;
-    cpx<T> *s = spec + (size_t)kx * N + kz;
-    const double *p = pp + (size_t)kx * N + kz;
-    const float *p32 = pp32 ? pp32 + (size_t)kx * N + kz : nullptr;
-    auto piv = [&](int J) { return low ? p[(size_t)(J - 1) * st] : (double)p32[(size_t)(J - 1) * st]; };
-    double gr = 0.0, gi = 0.0;
-    for (int J0 = 1; J0 <= M; J0 += 4) {
-        cpx<T> r[4];
-        double pj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (J0 + u <= M) {
-                r[u] = s[(size_t)(J0 + u - 1) * st];
-                pj[u] = piv(J0 + u);
-            }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int J = J0 + u;
-            if (J <= M) {
-                double rr = (double)r[u].x, ri = (double)r[u].y;
-                if (zero && J == 1) { rr = 0.0; ri = 0.0; }
-                const double a = (J == 1) ? 0.0 : amph[J - 1];
-                gr = (rr - a * gr) * pj[u];
-                gi = (ri - a * gi) * pj[u];
-                s[(size_t)(J - 1) * st] = {(T)gr, (T)gi};
-            }
-        }
-    }
-    double xr = gr, xi = gi;
-    for (int J0 = M - 1; J0 >= 1; J0 -= 4) {
-        cpx<T> gg[4];
-        double pj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (J0 - u >= 1) {
-                gg[u] = s[(size_t)(J0 - u - 1) * st];
-                pj[u] = piv(J0 - u);
-            }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int J = J0 - u;
-            if (J >= 1) {
-                const double c = (zero && J == 1) ? 0.0 : apph[J - 1];
-                const double h = c * pj[u];
-                xr = (double)gg[u].x - h * xr;
-                xi = (double)gg[u].y - h * xi;
-                s[(size_t)(J - 1) * st] = {(T)xr, (T)xi};
-            }
-        }
-    }
+    if (low) ytdma_line<T, double>(M, amph, apph, pp + off, spec + off, st, zero);
+    else ytdma_line<T, float>(M, amph, apph, pp32 + off, spec + off, st, zero);
 }
 
 // Inverse real x-transform of TK z-planes at row J -> phi (halo field) and the Neumann
 // ghost rows of BC_WALL_DPH_tg (BC_WALL.f90:369-386).  Z = A + i B with the Hermitian
 // extension; imaginary parts at kx = 0 and L/2 are not used (FFT99.f90:84-86).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
+__global__ __launch_bounds__(256, 4) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
                                                   double *__restrict__ phi, int TK)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -652,14 +711,19 @@ __global__ __launch_bounds__(256) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
     const int J = blockIdx.x + 1, k0 = blockIdx.y * TK;
     const int np = TK / 2;
-    for (int t = threadIdx.x; t < H * np; t += blockDim.x) {
-        const int kx = t / np, p = t - kx * np;
-        cpx<T> A = spec[six(g, J, kx, k0 + 2 * p)];
-        cpx<T> B = spec[six(g, J, kx, k0 + 2 * p + 1)];
-        if (kx == 0 || kx == L / 2) { A.y = T(0); B.y = T(0); }
-        a[p * ls + kx] = {A.x - B.y, A.y + B.x};
-        if (kx > 0 && kx < L / 2) a[p * ls + L - kx] = {A.x + B.y, B.x - A.y};
-    }
+    struct C2 { cpx<T> A, B; };
+    chs::stage_items<4, C2>(
+        H * np,
+        [&](int t) {
+            const int kx = t / np, p = t - kx * np;
+            return C2{spec[six(g, J, kx, k0 + 2 * p)], spec[six(g, J, kx, k0 + 2 * p + 1)]};
+        },
+        [&](int t, C2 c) {
+            const int kx = t / np, p = t - kx * np;
+            if (kx == 0 || kx == L / 2) { c.A.y = T(0); c.B.y = T(0); }
+            a[p * ls + kx] = {c.A.x - c.B.y, c.A.y + c.B.x};
+            if (kx > 0 && kx < L / 2) a[p * ls + L - kx] = {c.A.x + c.B.y, c.B.x - c.A.y};
+        });
     __syncthreads();
     chs::fft_lds_inplace<T>(a, ls, np, pa.px, tw, +1, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
@@ -756,14 +820,38 @@ __global__ __launch_bounds__(256) void k_poisson_resid(Geo g, const double *__re
 
 // acc (cells) = [acc +] phi interior rows (halo field)
 __global__ __launch_bounds__(256) void k_phi_accum(Geo g, const double *__restrict__ phi, double *__restrict__ acc,
-                                                   int add)
+                                                   int add, const double *__restrict__ shift)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
     const int k = blockIdx.z;
     if (i >= g.L || J > g.M) return;
-    const double v = phi[hix(g, i, J, k)];
+    double v = phi[hix(g, i, J, k)];
+    if (shift) v -= shift[0];
     acc[cix(g, i, J, k)] = add ? acc[cix(g, i, J, k)] + v : v;
+}
+
+// Gauge of the pressure correction: the FFT solve fixes phi_hat_00(J=1) = 0, i.e. a zero mean
+// over the J = 1 plane (possion3d_FFT99.f90:225-231, :444-446). The residual cannot see a constant, so the
+// FP32 rounding of that mean by the first mixed-precision solve survives every refinement sweep;
+// out[0] = mean over J = 1 of (acc + correction), which the last accumulate subtracts.
+__global__ __launch_bounds__(256) void k_gauge_mean(Geo g, const double *__restrict__ acc,
+                                                    const double *__restrict__ phi, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int n = g.L * g.N;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) {
+        const int i = t % g.L, k = t / g.L;
+        s += acc[cix(g, i, 1, k)] + phi[hix(g, i, 1, k)];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] / (double)n;
 }
 
 // Diagnostics: CFL_tg (CFL.f90:27-46) max over cells, and DIVGCK_tg max |div| (DIVGCK.f90).
@@ -866,6 +954,16 @@ struct chs_ctx {
     void *twx = nullptr, *twz = nullptr;
     chs::FftPlan px, pz;
     CnTab cn[4];
+    // launch shapes (chs_set_tuning); 0 = derived from the mesh
+    // (defaults: the fastest shapes on MI355X for a 256 x 192 x 256 channel; cnx_nlx 0 means
+    // 2048 / L lines)
+    struct {
+        int conv_by = 4, conv_kc = 16;  // k_conv_rhs rows per block, z-planes per thread
+        int cnx_nlx = 0;                // k_cn_x lines per block
+        int cnz_kz = 16, cny_ky = 48;   // march lengths of the z and y CN stencils (FP64)
+        int fftx_tk = 0, fftz_nl = 0;   // z-planes per x-transform block, z-lines per block
+    } tune;
+    int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
     bool have_fields = false;
     // profiling of the dominant kernel
     bool prof = false;
@@ -1088,7 +1186,6 @@ int tk_for(int L, int N)
 }
 // contiguous z-lines per z-transform block
 int nlz_for(int N) { int n = 2048 / N; return n < 1 ? 1 : n; }
-constexpr int kKC = 8;   // z-planes marched per conv_rhs thread
 
 
 struct EvScope {
@@ -1117,7 +1214,8 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src)
     const Geo &g = c->geo;
     const int L = c->L, M = c->M, N = c->N, H = c->H;
     PoisArgs pa{c->px, c->pz, c->twx, c->twz};
-    const int TK = tk_for(L, N), NL = nlz_for(N);
+    const int TK = c->tune.fftx_tk > 0 ? c->tune.fftx_tk : tk_for(L, N);
+    const int NL = c->tune.fftz_nl > 0 ? c->tune.fftz_nl : nlz_for(N);
     const size_t shx = sizeof(cpx<T>) * (L + (TK / 2) * (L + 1));
     const size_t shz = sizeof(cpx<T>) * (N + NL * (N + 1));
     cpx<T> *spec = (cpx<T> *)c->spec;
@@ -1179,23 +1277,28 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     const size_t cs = (size_t)N * M * L;
     W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
     const dim3 blk(64, 4), grd((L + 63) / 64, (M + 3) / 4, N);
-    const dim3 grdc((L + 63) / 64, (M + 3) / 4, (N + kKC - 1) / kKC);
+    const int BY = c->tune.conv_by, KC = c->tune.conv_kc;
+    const dim3 blkc(64, BY), grdc((L + 63) / 64, (M + BY - 1) / BY, (N + KC - 1) / KC);
     const int npart = grdc.x * grdc.y * grdc.z;
     {
         EvScope ev(c, "conv_rhs");
-        hipLaunchKernelGGL(k_conv_rhs<W>, grdc, blk, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, kKC);
+        switch (BY) {
+#define CHS_CCASE(by) case by: hipLaunchKernelGGL((k_conv_rhs<W, by>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC); break;
+            CHS_CCASE(1) CHS_CCASE(2) CHS_CCASE(4) CHS_CCASE(8) CHS_CCASE(16)
+#undef CHS_CCASE
+        }
     }
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, npart, 1.0 / c->volm, c->scal);
     CnTab &t = c->cn[ns];
     {
-        const int NLX = (1024 / L) < 1 ? 1 : 1024 / L;
+        const int NLX = c->tune.cnx_nlx > 0 ? c->tune.cnx_nlx : ((2048 / L) < 1 ? 1 : 2048 / L);
         EvScope ev(c, "cn_x");
         hipLaunchKernelGGL(k_cn_x<W>, dim3((M * N + NLX - 1) / NLX, 3), dim3(256), sizeof(W) * NLX * L, c->stream,
                            g, (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, NLX);
     }
     // z then y CN stencils: marching kernels for band half-widths up to 8 (the configs need
     // 3..7), the direct-load kernels otherwise
-    const int KZ = 16, KY = 32;
+    const int KZ = c->tune.cnz_kz, KY = c->tune.cny_ky;
     const bool zmarch = !t.fullz && t.Dz <= 8;
     if (zmarch) {
         EvScope ev(c, "cn_z");
@@ -1245,12 +1348,18 @@ int poisson_device_impl(chs_ctx *c, const double *rhs, double *phi, int nref, do
     const int npart = grd.x * grd.y * grd.z;
     const double *acph = c->cf_acph;
     poisson_launch<T>(c, 1.0, rhs);
-    hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 0);
+    hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 0, (const double *)nullptr);
     for (int r = 0; r < nref; ++r) {
         hipLaunchKernelGGL(k_poisson_resid, grd, blk, 0, c->stream, g, c->d_amph, acph, c->d_apph, rhs, phi,
                            c->div, c->partial);
         poisson_launch<T>(c, 1.0, c->div);
-        hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 1);
+        // FP32 transforms: restore the exact gauge in FP64 on the last sweep.
+        const double *shift = nullptr;
+        if (sizeof(T) == 4 && r == nref - 1) {
+            hipLaunchKernelGGL(k_gauge_mean, dim3(1), dim3(256), 0, c->stream, g, phi, c->phi, c->scal + 4);
+            shift = c->scal + 4;
+        }
+        hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 1, shift);
     }
     if (resmax) {
         hipLaunchKernelGGL(k_poisson_resid, grd, blk, 0, c->stream, g, c->d_amph, acph, c->d_apph, rhs, phi,
@@ -1411,6 +1520,7 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
         return fail(nullptr, CHS_E_ARG, "L, N must be even, <= 2048, with factors 2,3,5 only; M >= 4");
     chs_ctx *c = new chs_ctx();
     c->L = L; c->M = M; c->N = N; c->H = L / 2 + 1; c->mode = mode;
+    if (mode == CHS_MIXED) { c->tune.cnz_kz = 8; c->tune.cny_ky = 32; }   // FP32 stencils
     if (mode == CHS_MIXED) c->nref_projection = 2;
     c->px = px; c->pz = pz;
     c->dxi = cf->dxi; c->dzi = cf->dzi; c->dxqi = cf->dxqi; c->dzqi = cf->dzqi; c->volm = cf->volm;
@@ -1481,7 +1591,8 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
     CK(hipMalloc(&c->rhs2, ws * 3 * cs));
     CK(hipMalloc(&c->spec, 2 * ws * (size_t)c->H * M * N));
     CK(hipMalloc(&c->pp, sizeof(double) * (size_t)c->H * M * N));
-    c->npartial = ((L + 63) / 64) * ((M + 3) / 4) * N;   // >= conv_rhs and k_cfl_div grids
+    // >= every k_conv_rhs grid chs_set_tuning allows (conv_by >= 1, conv_kc >= 1) and k_cfl_div's
+    c->npartial = ((L + 63) / 64) * M * N;
     CK(hipMalloc(&c->partial, sizeof(double) * c->npartial));
     CK(hipMalloc(&c->scal, sizeof(double) * 16));
     CK(hipMemsetAsync(c->q, 0, sizeof(double) * 3 * hs, c->stream));
@@ -1667,6 +1778,42 @@ int chs_memcpy(void *dst, const void *src, size_t bytes, int to_device)
     return 0;
 }
 
+int chs_set_tuning(chs_ctx *c, const char *key, int value)
+{
+    if (!c || !key) return fail(c, CHS_E_ARG, "null argument");
+    const std::string k(key);
+    const int L = c->L, M = c->M, N = c->N;
+    auto bad = [&](const char *why) { return fail(c, CHS_E_ARG, "chs_set_tuning " + k + ": " + why); };
+    if (k == "conv_by") {
+        if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return bad("must be 1, 2, 4, 8 or 16");
+        c->tune.conv_by = value;
+    } else if (k == "conv_kc") {
+        if (value < 1 || value > N) return bad("must be 1..N");
+        c->tune.conv_kc = value;
+    } else if (k == "cnx_nlx") {
+        if (value < 0 || value > M * N || (size_t)value * L * sizeof(double) > 65536) return bad("lines must fit 64 KiB of LDS");
+        c->tune.cnx_nlx = value;
+    } else if (k == "cnz_kz") {
+        if (value < 1 || value > N) return bad("must be 1..N");
+        c->tune.cnz_kz = value;
+    } else if (k == "cny_ky") {
+        if (value < 1 || value > M) return bad("must be 1..M");
+        c->tune.cny_ky = value;
+    } else if (k == "fftx_tk") {
+        if (value != 0 && (value < 2 || (value & 1) || N % value || (value / 2) * L > chs::kMaxPts * 256))
+            return bad("must be even, divide N, and (value/2)*L <= 2048");
+        c->tune.fftx_tk = value;
+    } else if (k == "fftz_nl") {
+        if (value < 0 || value * N > chs::kMaxPts * 256) return bad("value*N must be <= 2048");
+        c->tune.fftz_nl = value;
+    } else {
+        return bad("unknown key");
+    }
+    ++c->tune_gen;
+    c->gdt = 0.0;   // force re-capture of a step graph
+    return 0;
+}
+
 int chs_set_graph(chs_ctx *c, int enable)
 {
     if (!c) return fail(c, CHS_E_ARG, "null ctx");
@@ -1693,7 +1840,8 @@ int chs_cfl_tg(chs_ctx *c, double dt, double *cflmm)
     const Geo &g = c->geo;
     dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
     hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 0);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, c->npartial, c->scal + 1);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, (int)(grd.x * grd.y * grd.z),
+                       c->scal + 1);
     double v = 0.0;
     HIPCHK(c, hipMemcpyAsync(&v, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1707,7 +1855,8 @@ int chs_max_div(chs_ctx *c, double *maxdiv)
     const Geo &g = c->geo;
     dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
     hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 1);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, c->npartial, c->scal + 2);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, (int)(grd.x * grd.y * grd.z),
+                       c->scal + 2);
     double v = 0.0;
     HIPCHK(c, hipMemcpyAsync(&v, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1766,6 +1915,7 @@ struct chs_ens {
     hipStream_t stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<double> gdt, gcv;
+    std::vector<int> gtune;
     std::string err;
 };
 
@@ -1814,7 +1964,8 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
     if (!e || !dt || !cvisc) return fail(nullptr, CHS_E_ARG, "chs_ens_step: bad argument");
     const int n = (int)e->ch.size();
     bool same = e->gexec && (int)e->gdt.size() == n;
-    for (int i = 0; same && i < n; ++i) same = e->gdt[i] == dt[i] && e->gcv[i] == cvisc[i];
+    for (int i = 0; same && i < n; ++i)
+        same = e->gdt[i] == dt[i] && e->gcv[i] == cvisc[i] && e->gtune[i] == e->ch[i]->tune_gen;
     if (!same) {
         for (int i = 0; i < n; ++i) {
             chs_ctx *c = e->ch[i];
@@ -1849,6 +2000,8 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
         if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
         e->gdt.assign(dt, dt + n);
         e->gcv.assign(cvisc, cvisc + n);
+        e->gtune.resize(n);
+        for (int i = 0; i < n; ++i) e->gtune[i] = e->ch[i]->tune_gen;
     }
     hipError_t he = hipGraphLaunch(e->gexec, e->stream);
     if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));

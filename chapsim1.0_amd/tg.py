@@ -42,6 +42,7 @@ SIGNATURES = {
     "chs_step_tg": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "chs_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_set_refinement": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "chs_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "chs_dev_malloc": (C.c_void_p, [C.c_size_t]),
     "chs_dev_free": (C.c_int, [C.c_void_p]),
     "chs_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
@@ -189,9 +190,15 @@ class ChannelTG:
     def set_graph(self, enable: bool):
         """Replay rk3_step from a hipGraph captured per (dt, cvisc)."""
         _check(lib().chs_set_graph(self._h, 1 if enable else 0), self._h)
+        self.graph = bool(enable)
 
     def set_refinement(self, nref_step: int, nref_projection: int):
         _check(lib().chs_set_refinement(self._h, int(nref_step), int(nref_projection)), self._h)
+
+    def set_tuning(self, **shapes):
+        """Launch shapes, e.g. set_tuning(conv_by=8, conv_kc=16) (keys of chs_set_tuning)."""
+        for k, v in shapes.items():
+            _check(lib().chs_set_tuning(self._h, k.encode(), int(v)), self._h)
 
     def poisson(self, rhs):
         L, M, N = self.L, self.M, self.N

@@ -1,0 +1,69 @@
+"""Launch-shape search for one channel (the keys of chs_set_tuning).
+
+Each key is swept over its candidates with the others fixed, timing only the kernels that the
+key shapes (HIP events of chs_profile_*), and the fastest value is kept.  Fields are saved
+before and restored after, so the channel's state is unchanged; the step graph is re-made
+on the next step.  Shapes change no result except the summation order of the mass-flux
+partials (conv_by, conv_kc).
+"""
+from __future__ import annotations
+
+KERNELS = {
+    "conv_by": ("conv_rhs",), "conv_kc": ("conv_rhs",), "cnx_nlx": ("cn_x",),
+    "cnz_kz": ("cn_z",), "cny_ky": ("cn_y",), "fftx_tk": ("divg_fftx", "fftx_inv"),
+    "fftz_nl": ("fftz_fwd", "fftz_inv"),
+}
+
+
+def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
+    """Values of each key that satisfy its limits on an L x M x N mesh."""
+    c = {
+        "conv_by": [2, 4, 8, 16],
+        "conv_kc": [k for k in (4, 8, 16, 32) if k <= N],
+        "cnx_nlx": [n for n in (1, 2, 4, 8, 16) if n * L * 8 <= 65536 and n <= M * N],
+        "cnz_kz": [k for k in (8, 16, 32, 64) if k <= N],
+        "cny_ky": [k for k in (16, 24, 32, 48, 64, 96) if k <= M],
+        "fftx_tk": [t for t in (2, 4, 8, 16, 32) if N % t == 0 and (t // 2) * L <= 2048],
+        "fftz_nl": [n for n in (1, 2, 4, 8, 16) if n * N <= 2048],
+    }
+    return {k: v for k, v in c.items() if v}
+
+
+def _time(ch, names, dt, cvisc, steps):
+    ch.profile_reset(None)
+    for _ in range(steps):
+        ch.rk3_step(dt, cvisc)
+    tot = 0.0
+    for n in names:
+        avg, cnt = ch.profile_get(n)
+        tot += avg * cnt
+    return tot / steps
+
+
+def autotune(ch, dt: float, cvisc: float, steps: int = 3, keys=None, verbose=False) -> dict:
+    """Sweeps the launch shapes of channel ``ch`` (chapsim_amd.ChannelTG with fields loaded)
+    at time step dt and viscosity cvisc; applies and returns the best value per key and the
+    timings {key: {value: ms per RK3 step of the key's kernels}}."""
+    saved = ch.download(with_history=True)
+    graph = getattr(ch, "graph", False)
+    ch.set_graph(False)
+    best, table = {}, {}
+    try:
+        ch.rk3_step(dt, cvisc)                       # CN tables, warm caches
+        for key, vals in candidates(ch.L, ch.M, ch.N).items():
+            if keys is not None and key not in keys:
+                continue
+            table[key] = {}
+            for v in vals:
+                ch.set_tuning(**{key: v})
+                table[key][v] = _time(ch, KERNELS[key], dt, cvisc, steps)
+            best[key] = min(table[key], key=table[key].get)
+            ch.set_tuning(**{key: best[key]})
+            if verbose:
+                print(f"tune {key}: " + ", ".join(f"{v}={t:.4f}" for v, t in table[key].items())
+                      + f" -> {best[key]}", flush=True)
+    finally:
+        ch.profile_reset("none")
+        ch.upload(saved[0], saved[1], saved[2])
+        ch.set_graph(graph)
+    return {"best": best, "ms": table}

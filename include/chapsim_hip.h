@@ -114,6 +114,18 @@ int chs_set_graph(chs_ctx *ctx, int enable);
  * 0 per substep and 2 for the NS = 0 projection in the mixed form, 0 / 0 in plain FP64. */
 int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
 
+/* Launch shapes of the substep kernels (no counterpart in the reference, whose loops have no
+ * tiling).  Keys and limits: "conv_by" rows per k_conv_rhs block (1, 2, 4, 8, 16; default 4),
+ * "conv_kc" z-planes marched per k_conv_rhs thread (1..N; 16), "cnx_nlx" x-lines per CN-x
+ * block (0 = 2048/L), "cnz_kz" / "cny_ky" march lengths of the z / y CN stencils (FP64 16 / 48,
+ * mixed 8 / 32),
+ * "fftx_tk" z-planes per x-transform block (0 = derived; even, divides N, TK/2*L <= 2048),
+ * "fftz_nl" z-lines per z-transform block (0 = 2048/N; NL*N <= 2048).  Results do not
+ * depend on the shapes except through the summation order of the mass-flux partials
+ * (conv_by, conv_kc).  A captured step graph is re-made on the next step.  CHS_E_ARG for an
+ * unknown key or a value out of range. */
+int chs_set_tuning(chs_ctx *ctx, const char *key, int value);
+
 /* Device memory helpers for hosts without their own allocator (e.g. the standalone Poisson
  * workload): plain hipMalloc / hipFree / synchronous hipMemcpy (to_device 1: host->device). */
 void *chs_dev_malloc(size_t bytes);
@@ -143,8 +155,10 @@ int chs_sync(chs_ctx *ctx);
 int chs_device_fields(chs_ctx *ctx, double **q, double **pr);
 
 /* Timing aid: HIP events recorded on the context's stream around each launch of the kernel
- * named `only` (NULL: every kernel) from the next substep on; chs_profile_get returns the
- * average device time (ms) of one named kernel's launches and their count. */
+ * named `only` (NULL: every kernel; a name of no kernel, e.g. "none": recording off) from the
+ * next substep on; chs_profile_get returns the average device time (ms) of one named kernel's
+ * launches and their count.  Kernel names: conv_rhs, cn_x, cn_z, cn_y, divg_fftx, fftz_fwd,
+ * ytdma, fftz_inv, fftx_inv, update, refine_update. */
 int chs_profile_reset(chs_ctx *ctx, const char *only);
 int chs_profile_get(chs_ctx *ctx, const char *kernel, double *avg_ms, int *count);
 
