@@ -115,7 +115,7 @@ def load_pkg():
     return __graft_entry__.load()
 
 
-def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False):
+def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False, shapes=None):
     c = c or CFG
     coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
@@ -129,6 +129,9 @@ def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotu
     if form == "mixed":
         ch.set_refinement(0, 2)
     tuning = None
+    if shapes:
+        ch.set_tuning(**shapes)
+        tuning = dict(shapes)
     if autotune:
         from chapsim_amd import tune
         tuning = tune.autotune(ch, dt, cv, steps=3, verbose=True)["best"]
@@ -337,6 +340,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each RK3 step from a hipGraph")
     ap.add_argument("--autotune", action="store_true",
                     help="search the kernels' launch shapes (chapsim_amd.tune) before timing")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch shape for chs_set_tuning (repeatable), e.g. --set conv_bx=4")
     ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble"],
                     default="channel")
     ap.add_argument("--nref", type=int, default=1, help="refinement sweeps (poisson, mixed)")
@@ -362,7 +367,8 @@ def main():
     if args.workload == "ensemble":
         return bench_ensemble(pkg, args)
     c = CFG64 if args.workload == "channel64" else CFG
-    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune)
+    shapes = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set}
+    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune, shapes)
     F = c["L"] * c["M"] * c["N"]
     value = F / (r["ms_per_step"] * 1e-3)
     dom = r["dominant"]

@@ -3,7 +3,9 @@
 // complex line) and complex z-transforms.  Twiddles come from a table w[m] = exp(-2 pi i
 // m / n) held in LDS; the inverse uses its conjugate.  The transforms are unnormalised.
 #pragma once
+#ifndef CHS_HOST_EMU
 #include <hip/hip_runtime.h>
+#endif
 
 namespace chs {
 
@@ -69,6 +71,7 @@ struct FftPlan {
     int radix[16];
 };
 
+#ifndef CHS_HOST_EMU
 // Global -> LDS staging of n items by a block: ld(t) reads item t, st(t, v) stores it.  The U
 // reads of one thread are all issued before the first store, so U memory latencies overlap
 // instead of following one another.
@@ -92,10 +95,13 @@ __device__ __forceinline__ void stage_items(int n, Ld ld, St st)
     }
 }
 
+#endif
+
 // Largest number of line points per thread of a block (nlines * n <= kMaxPts * nthr); a
 // radix-R pass then holds ceil(kMaxPts / R) butterflies of one thread in registers.
 constexpr int kMaxPts = 8;
 
+#ifndef CHS_HOST_EMU
 template <typename T, int R>
 __device__ __forceinline__ void fft_pass_inplace(cpx<T> *a, int ls, int nlines, int n, int Ns,
                                                  const cpx<T> *tw, int sign, int tid, int nthr)
@@ -163,5 +169,7 @@ __device__ void fft_lds_inplace(cpx<T> *a, int ls, int nlines, const FftPlan &pl
         Ns *= R;
     }
 }
+
+#endif
 
 }  // namespace chs

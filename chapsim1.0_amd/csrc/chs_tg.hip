@@ -31,6 +31,7 @@
 
 #include "../../include/chapsim_hip.h"
 #include "chs_fft.h"
+#include "chs_fftr.h"
 
 using chs::cpx;
 
@@ -58,6 +59,21 @@ __device__ __forceinline__ size_t cix(const Geo &g, int i, int J, int k)
     return ((size_t)k * g.M + (J - 1)) * g.L + i;
 }
 
+// XCD-aware tile order.  The dispatcher hands consecutive workgroup ids to the XCDs in turn
+// (id % nxcd), and each XCD has its own L2; tiles that share halo rows should therefore run
+// on the same XCD.  Logical tile l = the id's position within its XCD's share, so XCD x gets
+// the contiguous tiles [start_x, start_x + count_x) (x fastest, then y, then z).  A bijection
+// for any nxcd >= 1 (1 = identity).
+__device__ __forceinline__ int xcd_tile(int nxcd)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nb = gx * gy * gridDim.z;
+    const int p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (nxcd <= 1) return p;
+    const int xcd = p % nxcd, slot = p / nxcd, per = nb / nxcd, rem = nb % nxcd;
+    return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
 // RK scalars of one substep.
 struct Sub {
     double tgam, troh, talp, dt, cvisc;
@@ -68,27 +84,32 @@ struct Sub {
 // zeta H0 + alpha nu Lap q_c) dt - alpha dt grad_c p (RHS_CvLpGpS_tg.f90:95-160) and H0[c] = H;
 // v only for J >= 2 (NII = 2 on the bottom rank).  Also block partial sums of rhs_u / DYFI
 // (RHS_CvLpGpS_tg.f90:171-178) in FP64.  Values a column needs again on the next plane (its
-// own u, v, w, p and v(J+1) of plane k, and the plane k+1 values read for the z-differences)
-// are carried in registers, so each plane costs 15 loads of plane k, one of plane k-1, 5 of
-// plane k+1 and the 3 history values.
-template <typename W, int BY>
-__global__ __launch_bounds__(64 * BY) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
+// own u, v, w and p of plane k, and the plane k+1 values read for the z-differences) are
+// carried in registers, so each plane costs 15 loads of plane k, two of plane k-1, 5 of plane
+// k+1 and the 3 history values.
+template <typename W, int BY, int BXW, bool FENCE>
+__global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
                                                       const double *__restrict__ pr,
                                                       W *__restrict__ h0, W *__restrict__ rhs,
-                                                      double *__restrict__ partial, int KC)
+                                                      double *__restrict__ partial, int KC, int nxcd)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    // blockDim = (64, BY): a wavefront is one row J, so J-indexed coefficients are uniform
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * BY + threadIdx.y + 1);
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int tx = tile % gridDim.x, ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
+    const int i = tx * (64 * BXW) + threadIdx.x;
+    // blockDim = (64 BXW, BY): a wavefront is one row J, so J-indexed coefficients are uniform
+    const int J = __builtin_amdgcn_readfirstlane(ty * BY + threadIdx.y + 1);
     double mf = 0.0;
-    const int kbeg = blockIdx.z * KC, kend = min(g.N, kbeg + KC);
+    const int kbeg = tz * KC, kend = min(g.N, kbeg + KC);
     if (i < g.L && J <= g.M) {
         const int L = g.L, N = g.N;
         const size_t hs = (size_t)N * (g.M + 2) * L;   // halo field size
         const size_t cs = (size_t)N * g.M * L;         // cell field size
-        const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+        // Row pointers are wave-uniform (J, k uniform): loads take a scalar base and one of
+        // three 32-bit lane offsets (i, i+1, i-1), so no per-load 64-bit address.
+        const unsigned ic = i, ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
         const double *u = q, *v = q + hs, *w = q + 2 * hs;
-        auto at = [&](const double *f, int ii, int jj, int kk) { return f[hix(g, ii, jj, kk)]; };
+        auto row = [&](const double *f, int jj, int kk) { return f + ((size_t)kk * (g.M + 2) + jj) * L; };
+        auto crow = [&](int kk) { return ((size_t)kk * g.M + (J - 1)) * L; };   // cell row offset
         // State is FP64.  Sums and differences of state values (and the viscous and
         // pressure-gradient stencils, whose rows cancel) are formed in FP64; the convection
         // products run in the work precision W.
@@ -104,22 +125,37 @@ __global__ __launch_bounds__(64 * BY) void k_conv_rhs(Geo g, Sub s, const double
         const W cv1 = (W)(g.dxi * 0.5), cv2 = (W)(dyc * 0.25), cv3 = (W)(g.dzi * 0.5);
         // carried: plane k-1 (_m) and plane k (_c)
         const int km0 = (kbeg == 0) ? N - 1 : kbeg - 1;
-        double u_m = at(u, i, J, km0), v_m = at(v, i, J, km0), w_m = at(w, i, J, km0);
-        double p_m = at(pr, i, J, km0), vjp_m = at(v, i, J + 1, km0);
-        double u_c = at(u, i, J, kbeg), v_c = at(v, i, J, kbeg), w_c = at(w, i, J, kbeg);
-        double wim_c = at(w, im, J, kbeg), wjm_c = at(w, i, J - 1, kbeg);
+        double u_m = row(u, J, km0)[ic], v_m = row(v, J, km0)[ic], w_m = row(w, J, km0)[ic];
+        double p_m = row(pr, J, km0)[ic];
+        double u_c = row(u, J, kbeg)[ic], v_c = row(v, J, kbeg)[ic], w_c = row(w, J, kbeg)[ic];
+        double wim_c = row(w, J, kbeg)[im], wjm_c = row(w, J - 1, kbeg)[ic];
+        double rsum = 0.0;   // sum of rhs_u over the column's planes (row J: one DYFI)
         for (int k = kbeg; k < kend; ++k) {
             const int kp = (k == N - 1) ? 0 : k + 1, km = (k == 0) ? N - 1 : k - 1;
+            // Every load of the plane is issued here, before any store, so a wavefront waits
+            // for memory once per plane (v's ghost-row reads at J = 1 are in bounds, unused).
+            const double *uk = row(u, J, k), *vk = row(v, J, k), *wk = row(w, J, k), *pk = row(pr, J, k);
             // plane k+1
-            const double u_p = at(u, i, J, kp), v_p = at(v, i, J, kp), w_p = at(w, i, J, kp);
-            const double wim_p = at(w, im, J, kp), wjm_p = at(w, i, J - 1, kp);
+            const double *ukp = row(u, J, kp), *vkp = row(v, J, kp), *wkp = row(w, J, kp);
+            const double u_p = ukp[ic], v_p = vkp[ic], w_p = wkp[ic];
+            const double wim_p = wkp[im], wjm_p = row(w, J - 1, kp)[ic];
             // plane k
-            const double uip = at(u, ip, J, k), uim = at(u, im, J, k), ujp = at(u, i, J + 1, k);
-            const double ujm = at(u, i, J - 1, k);
-            const double vjp = at(v, i, J + 1, k), vimjp = at(v, im, J + 1, k), vim = at(v, im, J, k);
-            const double p0 = at(pr, i, J, k), pim = at(pr, im, J, k);
-            const double wip = at(w, ip, J, k), wjp = at(w, i, J + 1, k);
-            const size_t c0 = cix(g, i, J, k);
+            const double uip = uk[ip], uim = uk[im], ujp = row(u, J + 1, k)[ic];
+            const double ujm = row(u, J - 1, k)[ic];
+            const double *vjk = row(v, J + 1, k);
+            const double vjp = vjk[ic], vimjp = vjk[im], vim = vk[im];
+            const double p0 = pk[ic], pim = pk[im];
+            const double wip = wk[ip], wjp = row(w, J + 1, k)[ic];
+            const double vip = vk[ip], vjm = row(v, J - 1, k)[ic];
+            const double uipjm = row(u, J - 1, k)[ip], pjm = row(pr, J - 1, k)[ic];
+            // plane k-1 (not carried: register budget)
+            const double uip_m = row(u, J, km)[ip], vjp_m = row(v, J + 1, km)[ic];
+            const size_t c0 = crow(k);
+            const W *hk = h0 + c0;
+            W *rk = rhs + c0, *hkw = h0 + c0;
+            const W hu = hk[ic], hv = hk[cs + ic], hw = hk[2 * cs + ic];
+            // FENCE: keep the compiler from sinking any of these loads toward its use
+            if constexpr (FENCE) asm volatile("" ::: "memory");
             // ---- u (CONVECTION_X.f90:31-59) ----
             {
                 const W a1 = (W)(uip + u_c), a2 = (W)(uim + u_c);
@@ -128,19 +164,18 @@ __global__ __launch_bounds__(64 * BY) void k_conv_rhs(Geo g, Sub s, const double
                                (W)(v_c + vim) * (W)(wffJ * u_c + wfbJ * ujm)) * cu2;
                 const W H13 = ((W)(w_p + wim_p) * (W)(u_p + u_c) - (W)(w_c + wim_c) * (W)(u_c + u_m)) * cu3;
                 const W conv = -(H11 + H12 + H13);
-                const W rhsc = tgam * conv + troh * h0[c0];
+                const W rhsc = tgam * conv + troh * hu;
                 const double lap = (uip - 2.0 * u_c + uim) * g.dxqi + (u_p - 2.0 * u_c + u_m) * g.dzqi +
                                    (ujp * apu + u_c * acu + ujm * amu);
-                h0[c0] = conv;
+                hkw[ic] = conv;
                 const W pgm = (W)((p0 - pim) * g.dxi * coep);
                 const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
-                rhs[c0] = r;
-                mf += (double)r / dyf;
+                rk[ic] = r;
+                rsum += (double)r;
             }
-            // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 ----
-            if (J >= 2) {
-                const double vip = at(v, ip, J, k), vjm = at(v, i, J - 1, k);
-                const double uipjm = at(u, ip, J - 1, k), pjm = at(pr, i, J - 1, k);
+            // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 (computed on every row so that its
+            // loads stay with the others; stored for J >= 2) ----
+            {
                 const W H21 = ((W)(wffJ * uip + wfbJ * uipjm) * (W)(vip + v_c) -
                                (W)(wffJ * u_c + wfbJ * ujm) * (W)(v_c + vim)) * cv1;
                 const W b1 = (W)(vjp + v_c), b2 = (W)(v_c + vjm);
@@ -148,44 +183,45 @@ __global__ __launch_bounds__(64 * BY) void k_conv_rhs(Geo g, Sub s, const double
                 const W H23 = ((W)(wffJ * w_p + wfbJ * wjm_p) * (W)(v_p + v_c) -
                                (W)(wffJ * w_c + wfbJ * wjm_c) * (W)(v_c + v_m)) * cv3;
                 const W conv = -(H21 + H22 + H23);
-                const size_t cc = cs + c0;
-                const W rhsc = tgam * conv + troh * h0[cc];
+                const W rhsc = tgam * conv + troh * hv;
                 const double lap = (vip - 2.0 * v_c + vim) * g.dxqi + (v_p - 2.0 * v_c + v_m) * g.dzqi +
                                    (vjp * apv + v_c * acv + vjm * amv);
-                h0[cc] = conv;
                 const W pgm = (W)((p0 - pjm) * dyc * coep);
-                rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
+                if (J >= 2) {
+                    hkw[cs + ic] = conv;
+                    rk[cs + ic] = (rhsc + (W)(coev * lap)) * dt - pgm;
+                }
             }
             // ---- w (CONVECTION_Z.f90:33-69) ----
             {
-                const W H31 = ((W)(uip + at(u, ip, J, km)) * (W)(wip + w_c) - (W)(u_c + u_m) * (W)(w_c + wim_c)) * cu1;
+                const W H31 = ((W)(uip + uip_m) * (W)(wip + w_c) - (W)(u_c + u_m) * (W)(w_c + wim_c)) * cu1;
                 const W H32 = ((W)(vjp + vjp_m) * (W)(wffP * wjp + wfbP * w_c) -
                                (W)(v_c + v_m) * (W)(wffJ * w_c + wfbJ * wjm_c)) * cu2;
                 const W c1 = (W)(w_p + w_c), c2 = (W)(w_c + w_m);
                 const W H33 = (c1 * c1 - c2 * c2) * cu3;
                 const W conv = -(H31 + H32 + H33);
-                const size_t cc = 2 * cs + c0;
-                const W rhsc = tgam * conv + troh * h0[cc];
+                const W rhsc = tgam * conv + troh * hw;
                 const double lap = (wip - 2.0 * w_c + wim_c) * g.dxqi + (w_p - 2.0 * w_c + w_m) * g.dzqi +
                                    (wjp * apw + w_c * acw + wjm_c * amw);
-                h0[cc] = conv;
+                hkw[2 * cs + ic] = conv;
                 const W pgm = (W)((p0 - p_m) * g.dzi * coep);
-                rhs[cc] = (rhsc + (W)(coev * lap)) * dt - pgm;
+                rk[2 * cs + ic] = (rhsc + (W)(coev * lap)) * dt - pgm;
             }
-            u_m = u_c; v_m = v_c; w_m = w_c; p_m = p0; vjp_m = vjp;
+            u_m = u_c; v_m = v_c; w_m = w_c; p_m = p0;
             u_c = u_p; v_c = v_p; w_c = w_p; wim_c = wim_p; wjm_c = wjm_p;
         }
+        mf = rsum / dyf;
     }
     // block reduction of the mass-flux partial (fixed order -> deterministic)
-    __shared__ double red[64 * BY];
-    const int t = threadIdx.y * 64 + threadIdx.x;
+    __shared__ double red[64 * BXW * BY];
+    const int t = threadIdx.y * (64 * BXW) + threadIdx.x;
     red[t] = mf;
     __syncthreads();
-    for (int o = 32 * BY; o > 0; o >>= 1) {
+    for (int o = 32 * BXW * BY; o > 0; o >>= 1) {
         if (t < o) red[t] += red[t + o];
         __syncthreads();
     }
-    if (t == 0) partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+    if (t == 0) partial[tile] = red[0];
 }
 
 // Sum of n partials in a fixed order by one block; out[0] = sum * scale.
@@ -224,7 +260,8 @@ __global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, i
     W *r = rhs + c * cs + (size_t)l0 * L;     // line l = k*M + J-1 starts at l*L
     const W sub = (c == 0) ? (W)dpg[0] : W(0);
     const int nel = min(NLX, nl - l0) * L;
-    for (int t = threadIdx.x; t < nel; t += blockDim.x) line[t] = r[t] - sub;
+    chs::stage_items<8, W>(
+        nel, [&](int t) { return r[t]; }, [&](int t, W x) { line[t] = x - sub; });
     __syncthreads();
     for (int t = threadIdx.x; t < nel; t += blockDim.x) {
         const int ln = t / L, i = t - ln * L;
@@ -739,6 +776,133 @@ __global__ __launch_bounds__(256, 4) void k_fftx_inv(Geo g, PoisArgs pa, const c
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Register-FFT forms of the three transform kernels, for line lengths with a plan in
+// chs_fftr.h (64 ... 1024).  One wavefront per block owns LPW = 64 / TL whole lines, so the
+// passes need only wavefront-level LDS ordering.  Same inputs, outputs, scaling and
+// separation as k_divg_fftx / k_fftz / k_fftx_inv.
+template <int N> struct RGeom {
+    static constexpr int E = chs::RPlan<N>::E, PS = chs::RPlan<N>::PS, TL = N / E, LPW = 64 / TL;
+    static constexpr int R0 = chs::rfirst<N>(), LS = chs::rline_stride<N, PS>();
+};
+
+// z: lines [J][kx][:] of length N, in place.
+template <typename T, int N>
+__global__ __launch_bounds__(64) void k_fftz_r(Geo g, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ spec, int sign)
+{
+    using G = RGeom<N>;
+    __shared__ cpx<T> lds[G::LPW * G::LS];
+    const int li = threadIdx.x / G::TL, t = threadIdx.x % G::TL;
+    const int nlines = g.M * g.H;
+    const int line = blockIdx.x * G::LPW + li;
+    const bool act = line < nlines;   // lanes past the last line transform a copy of it, unstored
+    cpx<T> *base = spec + (size_t)(act ? line : nlines - 1) * N;
+    cpx<T> v[G::E];
+#pragma unroll
+    for (int b = 0; b < G::E / G::R0; ++b)
+#pragma unroll
+        for (int r = 0; r < G::R0; ++r) v[b * G::R0 + r] = base[t + b * G::TL + r * (N / G::R0)];
+    chs::rfft_line<T, N>(v, lds + li * G::LS, t, tw, sign, [&](int idx, const cpx<T> &x) {
+        if (act) base[idx] = x;
+    });
+}
+
+// x forward: block = row J and TK = 2 LPW consecutive z-planes (plane pair per line).
+template <typename T, int L>
+__global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restrict__ tw, double coe0,
+                                                    const double *__restrict__ q, double *__restrict__ div,
+                                                    cpx<T> *__restrict__ spec, const double *__restrict__ src)
+{
+    using G = RGeom<L>;
+    constexpr int TK = 2 * G::LPW;
+    __shared__ cpx<T> lds[G::LPW * G::LS];
+    const int li = threadIdx.x / G::TL, t = threadIdx.x % G::TL;
+    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK, ka = k0 + 2 * li;
+    const size_t hs = (size_t)g.N * (g.M + 2) * L;
+    const double *u = q, *v = q + hs, *w = q + 2 * hs;
+    const double dyf = g.dyfi[J - 1];
+    cpx<T> x[G::E];
+#pragma unroll
+    for (int b = 0; b < G::E / G::R0; ++b)
+#pragma unroll
+        for (int r = 0; r < G::R0; ++r) {
+            const int i = t + b * G::TL + r * (L / G::R0);
+            double d[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = ka + h;
+                if (src) {
+                    d[h] = src[cix(g, i, J, k)];
+                } else {
+                    const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
+                    const double dd = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
+                                      (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * dyf +
+                                      (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
+                    d[h] = dd * coe0;
+                    div[cix(g, i, J, k)] = d[h];
+                }
+            }
+            x[b * G::R0 + r] = {(T)d[0], (T)d[1]};
+        }
+    cpx<T> *line = lds + li * G::LS;
+    chs::rfft_line<T, L>(x, line, t, tw, -1, [&](int idx, const cpx<T> &z) { line[chs::rpad<G::PS>(idx)] = z; });
+    chs::wave_lds_sync();
+    // separate the two real transforms, scale by 1/(L N), write spec[J][kx][k0 .. k0+TK-1]
+    const T scale = T(1.0 / ((double)L * (double)g.N));
+    const int ln = threadIdx.x % TK, p = ln >> 1;
+    const cpx<T> *zl = lds + p * G::LS;
+    for (int kx = threadIdx.x / TK; kx < g.H; kx += 64 / TK) {
+        const cpx<T> Z = zl[chs::rpad<G::PS>(kx)];
+        const cpx<T> Zc = zl[chs::rpad<G::PS>((L - kx) % L)];
+        cpx<T> o;
+        if ((ln & 1) == 0) o = {(Z.x + Zc.x) * T(0.5) * scale, (Z.y - Zc.y) * T(0.5) * scale};
+        else o = {(Z.y + Zc.y) * T(0.5) * scale, (Zc.x - Z.x) * T(0.5) * scale};
+        spec[six(g, J, kx, k0 + ln)] = o;
+    }
+}
+
+// x inverse: Hermitian extension of the plane pair of each line, complex inverse transform,
+// real parts -> phi with the Neumann ghost rows (BC_WALL.f90:369-386).
+template <typename T, int L>
+__global__ __launch_bounds__(64) void k_fftx_inv_r(Geo g, const cpx<T> *__restrict__ tw,
+                                                   const cpx<T> *__restrict__ spec, double *__restrict__ phi)
+{
+    using G = RGeom<L>;
+    constexpr int TK = 2 * G::LPW;
+    __shared__ cpx<T> lds[G::LPW * G::LS];
+    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK, M = g.M;
+    {
+        const int p = threadIdx.x % G::LPW;
+        cpx<T> *zl = lds + p * G::LS;
+        for (int kx = threadIdx.x / G::LPW; kx < g.H; kx += 64 / G::LPW) {
+            cpx<T> A = spec[six(g, J, kx, k0 + 2 * p)], B = spec[six(g, J, kx, k0 + 2 * p + 1)];
+            if (kx == 0 || kx == L / 2) { A.y = T(0); B.y = T(0); }
+            zl[chs::rpad<G::PS>(kx)] = {A.x - B.y, A.y + B.x};
+            if (kx > 0 && kx < L / 2) zl[chs::rpad<G::PS>(L - kx)] = {A.x + B.y, B.x - A.y};
+        }
+    }
+    chs::wave_lds_sync();
+    const int li = threadIdx.x / G::TL, t = threadIdx.x % G::TL;
+    cpx<T> *line = lds + li * G::LS;
+    cpx<T> x[G::E];
+#pragma unroll
+    for (int b = 0; b < G::E / G::R0; ++b)
+#pragma unroll
+        for (int r = 0; r < G::R0; ++r) x[b * G::R0 + r] = line[chs::rpad<G::PS>(t + b * G::TL + r * (L / G::R0))];
+    chs::wave_lds_sync();
+    const int ka = k0 + 2 * li;
+    chs::rfft_line<T, L>(x, line, t, tw, +1, [&](int i, const cpx<T> &z) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = ka + h;
+            const double val = h == 0 ? (double)z.x : (double)z.y;
+            phi[hix(g, i, J, k)] = val;
+            if (J == 1) phi[hix(g, i, 0, k)] = val;
+            if (J == M) phi[hix(g, i, M + 1, k)] = val;
+        }
+    });
+}
+
 // VELOUPDT_tg (VELO_CORRECTION.f90:31-71) + PRCALC_tg (PRESSURE_CORRECTION.f90:26-36) +
 // BC_WALL_Q_tg / BC_WALL_PR_TG ghost rows.
 __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__restrict__ phi,
@@ -952,6 +1116,7 @@ struct chs_ctx {
     float *pp32 = nullptr;   // FP32 copy of the Poisson pivots (mixed form, high modes)
     int npartial = 0;
     void *twx = nullptr, *twz = nullptr;
+    void *rtwx = nullptr, *rtwz = nullptr;   // register-FFT pass twiddles (chs_fftr.h), if planned
     chs::FftPlan px, pz;
     CnTab cn[4];
     // launch shapes (chs_set_tuning); 0 = derived from the mesh
@@ -959,9 +1124,13 @@ struct chs_ctx {
     // 2048 / L lines)
     struct {
         int conv_by = 4, conv_kc = 16;  // k_conv_rhs rows per block, z-planes per thread
+        int conv_bx = 1;                // k_conv_rhs wavefronts along x per block (1 or 4)
+        int conv_fence = 0;             // 1: all of a plane's loads stay ahead of its arithmetic
         int cnx_nlx = 0;                // k_cn_x lines per block
         int cnz_kz = 16, cny_ky = 48;   // march lengths of the z and y CN stencils (FP64)
         int fftx_tk = 0, fftz_nl = 0;   // z-planes per x-transform block, z-lines per block
+        int nxcd = 1;                   // XCDs the stencil tiles are grouped for (1 = off)
+        int fft_reg = 1;                // register FFTs where L / N have a plan (0 = LDS Stockham)
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
     bool have_fields = false;
@@ -1175,6 +1344,16 @@ int upload_twiddles(chs_ctx *c, int n, void **dst)
     return 0;
 }
 
+template <typename T>
+int upload_rtw(chs_ctx *c, int n, void **dst)
+{
+    const std::vector<cpx<T>> h = chs::rplan_twiddles<T>(n);
+    HIPCHK(c, hipMalloc(dst, sizeof(cpx<T>) * h.size()));
+    HIPCHK(c, hipMemcpyAsync(*dst, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // z-planes per x-transform block: TK x-lines = TK/2 complex lines, (TK/2)*L <= 2048 complex
 // values (the in-place FFT's register budget), TK a power of two dividing N, at most 16
 // (16 complex values = 256 contiguous bytes per kx on the spectral write).
@@ -1208,11 +1387,44 @@ struct EvScope {
     }
 };
 
+#define CHS_RCASES(X) X(64) X(128) X(256) X(512) X(1024)
+template <typename T>
+void fftz_r(int N, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, cpx<T> *spec, int sign)
+{
+    switch (N) {
+#define X(n) case n: hipLaunchKernelGGL((k_fftz_r<T, n>), grid, dim3(64), 0, st, g, tw, spec, sign); break;
+        CHS_RCASES(X)
+#undef X
+    }
+}
+template <typename T>
+void divg_fftx_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, double coe0, const double *q,
+                 double *div, cpx<T> *spec, const double *src)
+{
+    switch (L) {
+#define X(n) case n: hipLaunchKernelGGL((k_divg_fftx_r<T, n>), grid, dim3(64), 0, st, g, tw, coe0, q, div, spec, src); break;
+        CHS_RCASES(X)
+#undef X
+    }
+}
+template <typename T>
+void fftx_inv_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, const cpx<T> *spec, double *phi)
+{
+    switch (L) {
+#define X(n) case n: hipLaunchKernelGGL((k_fftx_inv_r<T, n>), grid, dim3(64), 0, st, g, tw, spec, phi); break;
+        CHS_RCASES(X)
+#undef X
+    }
+}
+#undef CHS_RCASES
+
 template <typename T>
 void poisson_launch(chs_ctx *c, double coe0, const double *src)
 {
     const Geo &g = c->geo;
     const int L = c->L, M = c->M, N = c->N, H = c->H;
+    const bool rx = c->tune.fft_reg && c->rtwx, rz = c->tune.fft_reg && c->rtwz;
+    const int TKR = 2 * (rx ? chs::rplan_lpw(L) : 1), LPZ = rz ? chs::rplan_lpw(N) : 1;
     PoisArgs pa{c->px, c->pz, c->twx, c->twz};
     const int TK = c->tune.fftx_tk > 0 ? c->tune.fftx_tk : tk_for(L, N);
     const int NL = c->tune.fftz_nl > 0 ? c->tune.fftz_nl : nlz_for(N);
@@ -1221,12 +1433,18 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src)
     cpx<T> *spec = (cpx<T> *)c->spec;
     {
         EvScope ev(c, "divg_fftx");
-        hipLaunchKernelGGL(k_divg_fftx<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa, coe0, c->q,
-                           c->div, spec, TK, src);
+        if (rx)
+            divg_fftx_r<T>(L, dim3(M, N / TKR), c->stream, g, (const cpx<T> *)c->rtwx, coe0, c->q, c->div, spec, src);
+        else
+            hipLaunchKernelGGL(k_divg_fftx<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa, coe0, c->q,
+                               c->div, spec, TK, src);
     }
     {
         EvScope ev(c, "fftz_fwd");
-        hipLaunchKernelGGL(k_fftz<T>, dim3((M * H + NL - 1) / NL), dim3(256), shz, c->stream, g, pa, spec, NL, -1);
+        if (rz)
+            fftz_r<T>(N, dim3((M * H + LPZ - 1) / LPZ), c->stream, g, (const cpx<T> *)c->rtwz, spec, -1);
+        else
+            hipLaunchKernelGGL(k_fftz<T>, dim3((M * H + NL - 1) / NL), dim3(256), shz, c->stream, g, pa, spec, NL, -1);
     }
     {
         EvScope ev(c, "ytdma");
@@ -1235,12 +1453,18 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src)
     }
     {
         EvScope ev(c, "fftz_inv");
-        hipLaunchKernelGGL(k_fftz<T>, dim3((M * H + NL - 1) / NL), dim3(256), shz, c->stream, g, pa, spec, NL, +1);
+        if (rz)
+            fftz_r<T>(N, dim3((M * H + LPZ - 1) / LPZ), c->stream, g, (const cpx<T> *)c->rtwz, spec, +1);
+        else
+            hipLaunchKernelGGL(k_fftz<T>, dim3((M * H + NL - 1) / NL), dim3(256), shz, c->stream, g, pa, spec, NL, +1);
     }
     {
         EvScope ev(c, "fftx_inv");
-        hipLaunchKernelGGL(k_fftx_inv<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa,
-                           (const cpx<T> *)spec, c->phi, TK);
+        if (rx)
+            fftx_inv_r<T>(L, dim3(M, N / TKR), c->stream, g, (const cpx<T> *)c->rtwx, (const cpx<T> *)spec, c->phi);
+        else
+            hipLaunchKernelGGL(k_fftx_inv<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa,
+                               (const cpx<T> *)spec, c->phi, TK);
     }
 }
 
@@ -1277,14 +1501,17 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     const size_t cs = (size_t)N * M * L;
     W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
     const dim3 blk(64, 4), grd((L + 63) / 64, (M + 3) / 4, N);
-    const int BY = c->tune.conv_by, KC = c->tune.conv_kc;
-    const dim3 blkc(64, BY), grdc((L + 63) / 64, (M + BY - 1) / BY, (N + KC - 1) / KC);
+    const int BY = c->tune.conv_by, KC = c->tune.conv_kc, BX = c->tune.conv_bx;
+    const dim3 blkc(64 * BX, BY), grdc((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
     const int npart = grdc.x * grdc.y * grdc.z;
     {
         EvScope ev(c, "conv_rhs");
-        switch (BY) {
-#define CHS_CCASE(by) case by: hipLaunchKernelGGL((k_conv_rhs<W, by>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC); break;
-            CHS_CCASE(1) CHS_CCASE(2) CHS_CCASE(4) CHS_CCASE(8) CHS_CCASE(16)
+        switch (c->tune.conv_fence * 1000 + BX * 100 + BY) {
+#define CHS_CCASE(f, bx, by) case f * 1000 + bx * 100 + by: hipLaunchKernelGGL((k_conv_rhs<W, by, bx, f>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd); break;
+            CHS_CCASE(0, 1, 1) CHS_CCASE(0, 1, 2) CHS_CCASE(0, 1, 4) CHS_CCASE(0, 1, 8) CHS_CCASE(0, 1, 16)
+            CHS_CCASE(0, 4, 1) CHS_CCASE(0, 4, 2) CHS_CCASE(0, 4, 4)
+            CHS_CCASE(1, 1, 1) CHS_CCASE(1, 1, 2) CHS_CCASE(1, 1, 4) CHS_CCASE(1, 1, 8) CHS_CCASE(1, 1, 16)
+            CHS_CCASE(1, 4, 1) CHS_CCASE(1, 4, 2) CHS_CCASE(1, 4, 4)
 #undef CHS_CCASE
         }
     }
@@ -1630,6 +1857,12 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
     }
     int e = (mode == CHS_MIXED) ? upload_twiddles<float>(c, L, &c->twx) : upload_twiddles<double>(c, L, &c->twx);
     if (!e) e = (mode == CHS_MIXED) ? upload_twiddles<float>(c, N, &c->twz) : upload_twiddles<double>(c, N, &c->twz);
+    // register FFTs: x needs a plan for L and N divisible by the plane pairs of one wavefront
+    int rad[8];
+    if (!e && chs::rplan_radices(L, rad) && N % (2 * chs::rplan_lpw(L)) == 0)
+        e = (mode == CHS_MIXED) ? upload_rtw<float>(c, L, &c->rtwx) : upload_rtw<double>(c, L, &c->rtwx);
+    if (!e && chs::rplan_radices(N, rad))
+        e = (mode == CHS_MIXED) ? upload_rtw<float>(c, N, &c->rtwz) : upload_rtw<double>(c, N, &c->rtwz);
     if (e) return bad(e);
 #undef CK
     *out = c;
@@ -1641,7 +1874,7 @@ void chs_destroy(chs_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cf, c->q, c->pr, c->phi, c->div, c->h0, c->rhs, c->rhs2, c->spec, c->pp,
-                    c->pp32, c->partial, c->scal, c->twx, c->twz};
+                    c->pp32, c->partial, c->scal, c->twx, c->twz, c->rtwx, c->rtwz};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->cn) {
@@ -1786,7 +2019,15 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     auto bad = [&](const char *why) { return fail(c, CHS_E_ARG, "chs_set_tuning " + k + ": " + why); };
     if (k == "conv_by") {
         if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return bad("must be 1, 2, 4, 8 or 16");
+        if (c->tune.conv_bx * value > 16) return bad("conv_bx * conv_by must be <= 16");
         c->tune.conv_by = value;
+    } else if (k == "conv_fence") {
+        if (value != 0 && value != 1) return bad("must be 0 or 1");
+        c->tune.conv_fence = value;
+    } else if (k == "conv_bx") {
+        if (value != 1 && value != 4) return bad("must be 1 or 4");
+        if (value * c->tune.conv_by > 16) return bad("conv_bx * conv_by must be <= 16");
+        c->tune.conv_bx = value;
     } else if (k == "conv_kc") {
         if (value < 1 || value > N) return bad("must be 1..N");
         c->tune.conv_kc = value;
@@ -1803,6 +2044,12 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
         if (value != 0 && (value < 2 || (value & 1) || N % value || (value / 2) * L > chs::kMaxPts * 256))
             return bad("must be even, divide N, and (value/2)*L <= 2048");
         c->tune.fftx_tk = value;
+    } else if (k == "fft_reg") {
+        if (value != 0 && value != 1) return bad("must be 0 or 1");
+        c->tune.fft_reg = value;
+    } else if (k == "nxcd") {
+        if (value < 1 || value > 64) return bad("must be 1..64");
+        c->tune.nxcd = value;
     } else if (k == "fftz_nl") {
         if (value < 0 || value * N > chs::kMaxPts * 256) return bad("value*N must be <= 2048");
         c->tune.fftz_nl = value;

@@ -8,10 +8,14 @@ partials (conv_by, conv_kc).
 """
 from __future__ import annotations
 
+from .tg import ChsError
+
 KERNELS = {
-    "conv_by": ("conv_rhs",), "conv_kc": ("conv_rhs",), "cnx_nlx": ("cn_x",),
+    "conv_by": ("conv_rhs",), "conv_kc": ("conv_rhs",), "conv_bx": ("conv_rhs",), "conv_fence": ("conv_rhs",),
+    "cnx_nlx": ("cn_x",),
     "cnz_kz": ("cn_z",), "cny_ky": ("cn_y",), "fftx_tk": ("divg_fftx", "fftx_inv"),
-    "fftz_nl": ("fftz_fwd", "fftz_inv"),
+    "fftz_nl": ("fftz_fwd", "fftz_inv"), "nxcd": ("conv_rhs",),
+    "fft_reg": ("divg_fftx", "fftz_fwd", "fftz_inv", "fftx_inv"),
 }
 
 
@@ -20,11 +24,15 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
     c = {
         "conv_by": [2, 4, 8, 16],
         "conv_kc": [k for k in (4, 8, 16, 32) if k <= N],
+        "conv_bx": [1, 4] if L % 256 == 0 else [1],
+        "conv_fence": [0, 1],
         "cnx_nlx": [n for n in (1, 2, 4, 8, 16) if n * L * 8 <= 65536 and n <= M * N],
         "cnz_kz": [k for k in (8, 16, 32, 64) if k <= N],
         "cny_ky": [k for k in (16, 24, 32, 48, 64, 96) if k <= M],
         "fftx_tk": [t for t in (2, 4, 8, 16, 32) if N % t == 0 and (t // 2) * L <= 2048],
         "fftz_nl": [n for n in (1, 2, 4, 8, 16) if n * N <= 2048],
+        "fft_reg": [0, 1],
+        "nxcd": [1, 8],
     }
     return {k: v for k, v in c.items() if v}
 
@@ -55,7 +63,10 @@ def autotune(ch, dt: float, cvisc: float, steps: int = 3, keys=None, verbose=Fal
                 continue
             table[key] = {}
             for v in vals:
-                ch.set_tuning(**{key: v})
+                try:
+                    ch.set_tuning(**{key: v})
+                except ChsError:              # excluded by the current value of another key
+                    continue
                 table[key][v] = _time(ch, KERNELS[key], dt, cvisc, steps)
             best[key] = min(table[key], key=table[key].get)
             ch.set_tuning(**{key: best[key]})

@@ -14,6 +14,7 @@ MESHES = [  # L, M, N, STR2, REN, dt
     (20, 18, 30, 2.2, 6875.0, 5.0e-4),
     (64, 64, 64, 1.8, 2800.0, 1.0e-3),
     (48, 36, 40, 2.0, 4000.0, 8.0e-4),
+    (256, 24, 256, 2.0, 6875.0, 5.0e-4),
 ]
 
 
@@ -78,8 +79,16 @@ def test_matches_reference_golden(pkg, golden, name):
     ch.close()
 
 
-@pytest.mark.parametrize("L,M,N", [(16, 16, 16), (30, 20, 24), (128, 96, 128)])
-def test_poisson_matches_oracle(pkg, L, M, N):
+# line lengths 64 ... 1024 run the register FFTs (chs_fftr.h), the others the LDS Stockham
+# transforms; fft_reg=0 forces the latter on planned lengths too
+POISSON_MESHES = [(16, 16, 16, 1), (30, 20, 24, 1), (128, 96, 128, 1), (128, 96, 128, 0),
+                  (64, 20, 96, 1), (256, 12, 256, 1), (512, 8, 512, 1), (1024, 6, 64, 1),
+                  (96, 10, 1024, 1)]
+
+
+@pytest.mark.parametrize("L,M,N,reg", POISSON_MESHES)
+def test_poisson_matches_oracle(pkg, L, M, N, reg):
+    print(f"poisson {L}x{M}x{N} fft_reg={reg}", flush=True)
     mesh = setup.Mesh(L, M, N, STR2=2.2)
     rng = np.random.default_rng(5)
     r = rng.standard_normal((N, M, L))
@@ -87,10 +96,12 @@ def test_poisson_matches_oracle(pkg, L, M, N):
     r -= np.sum(r * w) / (np.sum(w) * L * N)
     o = oracle.OracleTG(mesh, 2800.0)
     ref = o.poisson(r)
-    ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=2.2), pkg.PLAIN_FP64)
-    phi = ch.poisson(r)
-    assert rel(phi, ref) < 1e-10
-    ch.close()
+    for mode, tol in ((pkg.PLAIN_FP64, 1e-10), (pkg.MIXED, 1e-5)):
+        ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=2.2), mode)
+        ch.set_tuning(fft_reg=reg)
+        phi = ch.poisson(r)
+        assert rel(phi, ref) < tol, (mode, rel(phi, ref))
+        ch.close()
 
 
 def test_mass_flux_and_walls(pkg):
@@ -150,8 +161,9 @@ def test_poisson_device_refinement(pkg, L, M, N):
 
 
 SHAPES = [
-    dict(conv_by=16, conv_kc=3, cnx_nlx=3, cnz_kz=5, cny_ky=7, fftx_tk=2, fftz_nl=1),
+    dict(conv_by=16, conv_kc=3, cnx_nlx=3, cnz_kz=5, cny_ky=7, fftx_tk=2, fftz_nl=1, nxcd=3),
     dict(conv_by=1, conv_kc=1, cnx_nlx=1, cnz_kz=40, cny_ky=36, fftx_tk=8, fftz_nl=5),
+    dict(conv_by=4, conv_bx=4, conv_kc=7, nxcd=8),
 ]
 
 
@@ -179,7 +191,7 @@ def test_launch_shapes_keep_results(pkg, shape):
 
 def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
-    for bad in (dict(conv_by=3), dict(conv_kc=0), dict(conv_kc=41), dict(fftx_tk=3),
+    for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(conv_kc=41), dict(fftx_tk=3),
                 dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6),
                 dict(no_such_key=1)):
         with pytest.raises(pkg.ChsError):
