@@ -463,16 +463,34 @@ __global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__
         kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
         win[d + D] = r[(size_t)kk * ks];
     }
-    for (int k = k0; k < k1; ++k) {
-        int kk = k + D;
-        if (kk >= N) kk -= N;
-        win[2 * D] = r[(size_t)kk * ks];
+    // planes in batches of B: the batch's B loads are in flight together
+    constexpr int B = 8;
+    auto step = [&](int k, W nx) {
+        win[2 * D] = nx;
         W acc = coef[0] * win[D];
 #pragma unroll
         for (int d = 1; d <= D; ++d) acc += coef[d] * (win[D + d] + win[D - d]);
         o[(size_t)k * ks] = acc;
 #pragma unroll
         for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+    };
+    int k = k0;
+    for (; k + B <= k1; k += B) {
+        W nx[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            int kk = k + D + b;
+            kk = kk >= N ? kk - N : kk;
+            kk = kk >= N ? kk - N : kk;
+            nx[b] = r[(size_t)kk * ks];
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) step(k + b, nx[b]);
+    }
+    for (; k < k1; ++k) {
+        int kk = k + D;
+        if (kk >= N) kk -= N;
+        step(k, r[(size_t)kk * ks]);
     }
 }
 
@@ -501,17 +519,35 @@ __global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__
         const int jj = J0 + d;
         win[d + D] = (jj >= 1 && jj <= M) ? r[(size_t)(jj - 1) * L] : W(0);
     }
-    for (int J = J0; J <= J1; ++J) {
-        const int jj = J + D;
-        win[2 * D] = jj <= M ? r[(size_t)(jj - 1) * L] : W(0);
+    // rows in batches of B: the batch's input rows and Q values are loaded together
+    constexpr int B = 8;
+    auto step = [&](int J, W nx, double qv) {
+        win[2 * D] = nx;
         const W *bj = by + (size_t)(J - 1) * w;
         W acc = W(0);
 #pragma unroll
         for (int d = 0; d < w; ++d) acc += bj[d] * win[d];   // out-of-range rows hold 0 and 0 coefficients
-        qc[hix(g, i, J, k)] += (double)acc;
+        qc[hix(g, i, J, k)] = qv + (double)acc;
         if (c == 0 && J == 1) rin[cs + cix(g, i, 1, k)] = acc;
 #pragma unroll
         for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+    };
+    int J = J0;
+    for (; J + B - 1 <= J1; J += B) {
+        W nx[B];
+        double qv[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int jj = J + b + D;
+            nx[b] = jj <= M ? r[(size_t)(jj - 1) * L] : W(0);
+            qv[b] = qc[hix(g, i, J + b, k)];
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) step(J + b, nx[b], qv[b]);
+    }
+    for (; J <= J1; ++J) {
+        const int jj = J + D;
+        step(J, jj <= M ? r[(size_t)(jj - 1) * L] : W(0), qc[hix(g, i, J, k)]);
     }
     if (jc == 0) {
         qc[hix(g, i, 0, k)] = 0.0;
