@@ -88,6 +88,8 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
         "cn_zy": 3 * F * wbytes + 6 * F * 8,
         # z stencils, one launch for the three components: R + W RHS x3
         "cn_z": 6 * F * wbytes,
+        # x and z together, three components: R + W RHS x3
+        "cn_xz": 6 * F * wbytes,
         # y stencil + Q update, one launch per component: R RHS, R + W q
         "cn_y": F * wbytes + 2 * F * 8,
         # R u,v,w; W div (FP64) + spectrum
@@ -106,7 +108,7 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
     return table[kernel]
 
 
-KERNELS = ["conv_rhs", "cn_x", "cn_zy", "cn_z", "cn_y", "divg_fftx", "fftz_fwd", "ytdma",
+KERNELS = ["conv_rhs", "cn_x", "cn_zy", "cn_z", "cn_xz", "cn_y", "divg_fftx", "fftz_fwd", "ytdma",
            "fftz_inv", "fftx_inv", "update", "refine_update"]
 
 

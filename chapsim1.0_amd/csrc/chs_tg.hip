@@ -289,6 +289,90 @@ __global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, i
     }
 }
 
+// x and z together (MOMFA1_X_tg then MOMFA2_Z_tg): a block holds whole x-lines, one row J per
+// 64 LX threads (LX >= L), and marches the z-planes of one chunk of KZ planes in batches of B.
+// Each plane's RHS row goes through LDS, where the circulant x-inverse gx is applied; the
+// x-solved value enters the thread's window of 2D + 1 planes for the banded z-inverse gz.  The
+// chunk's D halo planes on each side are x-solved again (as the separate z-kernel reloads
+// them).  u's rows get the mass-flux correction -DPGRNS first; v's row J = 1 is not solved.
+// One barrier per batch; LDS holds two batches (double buffer).
+template <typename W, int D>
+__global__ __launch_bounds__(1024) void k_cn_xz(Geo g, const W *__restrict__ gx, int Dx, int fullx,
+                                                const W *__restrict__ gz, const W *__restrict__ rin,
+                                                W *__restrict__ rout, const double *__restrict__ dpg, int KZ)
+{
+    constexpr int B = 8;
+    extern __shared__ unsigned char smem_raw[];
+    W *buf = reinterpret_cast<W *>(smem_raw);   // [2][B][BY][LX]
+    const int L = g.L, N = g.N, LX = blockDim.x, BY = blockDim.y;
+    const int i = threadIdx.x, jr = threadIdx.y;
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.x * BY + jr + 1);
+    const int c = blockIdx.y % 3, kc = blockIdx.y / 3;
+    const bool inrow = J <= g.M && !(c == 1 && J == 1);
+    const bool act = inrow && i < L;
+    const size_t cs = (size_t)N * g.M * L, ks = (size_t)g.M * L;
+    const W *r = rin + c * cs + (inrow ? cix(g, 0, J, 0) : 0);
+    W *o = rout + c * cs + (inrow ? cix(g, 0, J, 0) : 0);
+    const W sub = (c == 0) ? (W)dpg[0] : W(0);
+    W cz[D + 1];
+#pragma unroll
+    for (int d = 0; d <= D; ++d) cz[d] = gz[d];
+    const int k0 = kc * KZ, k1 = min(N, k0 + KZ);
+    // planes k0 - D .. k1 + D - 1 (cyclic), x-solved in batches; the first 2D fill the window
+    const int np = (k1 - k0) + 2 * D;
+    W win[2 * D + 1];
+#pragma unroll
+    for (int d = 0; d <= 2 * D; ++d) win[d] = W(0);
+    for (int p0 = 0, it = 0; p0 < np; p0 += B, ++it) {
+        W *bb = buf + (size_t)(it & 1) * B * BY * LX;
+        W val[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            int kk = k0 - D + p0 + b;
+            kk = kk < 0 ? kk + N : (kk >= N ? kk - N : kk);
+            val[b] = (act && p0 + b < np) ? r[(size_t)kk * ks + i] - sub : W(0);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) bb[((size_t)b * BY + jr) * LX + i] = val[b];
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int p = p0 + b;   // position in the plane sequence (uniform)
+            if (p >= np) continue;
+            const W *ln = bb + ((size_t)b * BY + jr) * LX;
+            W xs = W(0);
+            if (act) {
+                if (fullx) {
+                    for (int l = 0; l < L; ++l) {
+                        int d = i - l;
+                        d = d < 0 ? d + L : d;
+                        xs += gx[d] * ln[l];
+                    }
+                } else {
+                    xs = gx[0] * ln[i];
+                    for (int d = 1; d <= Dx; ++d) {
+                        int a = i + d, e = i - d;
+                        a = a >= L ? a - L : a;
+                        e = e < 0 ? e + L : e;
+                        xs += gx[d] * (ln[a] + ln[e]);
+                    }
+                }
+            }
+            // the window is a shift register; once 2D planes are in, its centre is plane
+            // k0 + p - 2D
+            win[2 * D] = xs;
+            if (p >= 2 * D) {
+                W acc = cz[0] * win[D];
+#pragma unroll
+                for (int d = 1; d <= D; ++d) acc += cz[d] * (win[D + d] + win[D - d]);
+                if (act) o[(size_t)(k0 + p - 2 * D) * ks + i] = acc;
+            }
+#pragma unroll
+            for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+        }
+    }
+}
+
 // z: same circulant inverse along k (stride (M)*L in the cell layout), one thread per (i, J).
 template <typename W>
 __global__ __launch_bounds__(256) void k_cn_z(Geo g, const W *__restrict__ gz, int D, int full,
@@ -1167,6 +1251,7 @@ struct chs_ctx {
         int fftx_tk = 0, fftz_nl = 0;   // z-planes per x-transform block, z-lines per block
         int nxcd = 1;                   // XCDs the stencil tiles are grouped for (1 = off)
         int fft_reg = 1;                // register FFTs where L / N have a plan (0 = LDS Stockham)
+        int cn_xz = 0, cnxz_by = 1;     // fused x+z CN kernel (1; slower on MI355X), rows per block
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
     bool have_fields = false;
@@ -1516,6 +1601,18 @@ void cn_z_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, con
 }
 
 template <typename W>
+void cn_xz_dispatch(int D, dim3 grid, dim3 blk, size_t sh, hipStream_t st, const Geo &g, const W *gx, int Dx,
+                    int fullx, const W *gz, const W *rin, W *rout, const double *dpg, int KZ)
+{
+    switch (D) {
+#define CHS_XZCASE(d) case d: hipLaunchKernelGGL((k_cn_xz<W, d>), grid, blk, sh, st, g, gx, Dx, fullx, gz, rin, rout, dpg, KZ); break;
+        CHS_XZCASE(0) CHS_XZCASE(1) CHS_XZCASE(2) CHS_XZCASE(3) CHS_XZCASE(4) CHS_XZCASE(5) CHS_XZCASE(6)
+        CHS_XZCASE(7) CHS_XZCASE(8)
+#undef CHS_XZCASE
+    }
+}
+
+template <typename W>
 void cn_y_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *by, W *rin, double *q, int comp,
                    int KY)
 {
@@ -1553,17 +1650,28 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     }
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, npart, 1.0 / c->volm, c->scal);
     CnTab &t = c->cn[ns];
-    {
+    // z then y CN stencils: marching kernels for band half-widths up to 8 (the configs need
+    // 3..7), the direct-load kernels otherwise; x and z fused where a block holds a whole
+    // x-line (L <= 1024) and its two batches of planes fit 64 KiB of LDS
+    const int KZ = c->tune.cnz_kz, KY = c->tune.cny_ky;
+    const bool zmarch = !t.fullz && t.Dz <= 8;
+    const int LX = 64 * ((L + 63) / 64), BYX = c->tune.cnxz_by;
+    const size_t shxz = 2 * 8 * (size_t)BYX * LX * sizeof(W);
+    const bool xz = c->tune.cn_xz && zmarch && LX * BYX <= 1024 && shxz <= 65536;
+    if (xz) {
+        EvScope ev(c, "cn_xz");
+        const dim3 gxz((M + BYX - 1) / BYX, 3 * ((N + KZ - 1) / KZ));
+        cn_xz_dispatch<W>(t.Dz, gxz, dim3(LX, BYX), shxz, c->stream, g, (const W *)t.gx, t.Dx, t.fullx,
+                          (const W *)t.gz, rhs, rhs2, c->scal, KZ);
+    } else {
         const int NLX = c->tune.cnx_nlx > 0 ? c->tune.cnx_nlx : ((2048 / L) < 1 ? 1 : 2048 / L);
         EvScope ev(c, "cn_x");
         hipLaunchKernelGGL(k_cn_x<W>, dim3((M * N + NLX - 1) / NLX, 3), dim3(256), sizeof(W) * NLX * L, c->stream,
                            g, (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, NLX);
     }
-    // z then y CN stencils: marching kernels for band half-widths up to 8 (the configs need
-    // 3..7), the direct-load kernels otherwise
-    const int KZ = c->tune.cnz_kz, KY = c->tune.cny_ky;
-    const bool zmarch = !t.fullz && t.Dz <= 8;
-    if (zmarch) {
+    if (xz) {
+        // rhs2 already holds the x- and z-solved increments
+    } else if (zmarch) {
         EvScope ev(c, "cn_z");
         const dim3 gz((L + 63) / 64, (M + 3) / 4, 3 * ((N + KZ - 1) / KZ));
         cn_z_dispatch<W>(t.Dz, gz, blk, c->stream, g, (const W *)t.gz, rhs, rhs2, KZ);
@@ -2080,6 +2188,12 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
         if (value != 0 && (value < 2 || (value & 1) || N % value || (value / 2) * L > chs::kMaxPts * 256))
             return bad("must be even, divide N, and (value/2)*L <= 2048");
         c->tune.fftx_tk = value;
+    } else if (k == "cn_xz") {
+        if (value != 0 && value != 1) return bad("must be 0 or 1");
+        c->tune.cn_xz = value;
+    } else if (k == "cnxz_by") {
+        if (value != 1 && value != 2 && value != 4) return bad("must be 1, 2 or 4");
+        c->tune.cnxz_by = value;
     } else if (k == "fft_reg") {
         if (value != 0 && value != 1) return bad("must be 0 or 1");
         c->tune.fft_reg = value;

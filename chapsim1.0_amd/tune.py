@@ -12,8 +12,8 @@ from .tg import ChsError
 
 KERNELS = {
     "conv_by": ("conv_rhs",), "conv_kc": ("conv_rhs",), "conv_bx": ("conv_rhs",), "conv_fence": ("conv_rhs",),
-    "cnx_nlx": ("cn_x",),
-    "cnz_kz": ("cn_z",), "cny_ky": ("cn_y",), "fftx_tk": ("divg_fftx", "fftx_inv"),
+    "cnx_nlx": ("cn_x",), "cn_xz": ("cn_x", "cn_z", "cn_xz"), "cnxz_by": ("cn_xz",),
+    "cnz_kz": ("cn_z", "cn_xz"), "cny_ky": ("cn_y",), "fftx_tk": ("divg_fftx", "fftx_inv"),
     "fftz_nl": ("fftz_fwd", "fftz_inv"), "nxcd": ("conv_rhs",),
     "fft_reg": ("divg_fftx", "fftz_fwd", "fftz_inv", "fftx_inv"),
 }
@@ -26,6 +26,8 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "conv_kc": [k for k in (4, 8, 16, 32) if k <= N],
         "conv_bx": [1, 4] if L % 256 == 0 else [1],
         "conv_fence": [0, 1],
+        "cn_xz": [0, 1],
+        "cnxz_by": [1, 2],
         "cnx_nlx": [n for n in (1, 2, 4, 8, 16) if n * L * 8 <= 65536 and n <= M * N],
         "cnz_kz": [k for k in (8, 16, 32, 64) if k <= N],
         "cny_ky": [k for k in (16, 24, 32, 48, 64, 96) if k <= M],

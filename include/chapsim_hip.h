@@ -125,7 +125,9 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * "fftz_nl" z-lines per z-transform block (0 = 2048/N; NL*N <= 2048), "nxcd" number of L2
  * domains (XCDs) the k_conv_rhs tiles are grouped for (1..64; default 1 = dispatch order),
  * "fft_reg" 1 (default) to run the register-resident FFTs for line lengths 64 ... 1024, 0 for
- * the LDS Stockham transforms.  Results do not
+ * the LDS Stockham transforms, "cn_xz" 1 to run the x and z Crank-Nicolson solves in one kernel
+ * where a block can hold whole x-lines (default 0: two kernels, faster on MI355X at 256^2 x
+ * 192), "cnxz_by" x-lines per block of that kernel (1, 2, 4; 1).  Results do not
  * depend on the shapes except through the summation order of the mass-flux partials
  * (conv_by, conv_kc).  A captured step graph is re-made on the next step.  CHS_E_ARG for an
  * unknown key or a value out of range. */

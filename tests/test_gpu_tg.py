@@ -161,9 +161,10 @@ def test_poisson_device_refinement(pkg, L, M, N):
 
 
 SHAPES = [
-    dict(conv_by=16, conv_kc=3, cnx_nlx=3, cnz_kz=5, cny_ky=7, fftx_tk=2, fftz_nl=1, nxcd=3),
-    dict(conv_by=1, conv_kc=1, cnx_nlx=1, cnz_kz=40, cny_ky=36, fftx_tk=8, fftz_nl=5),
-    dict(conv_by=4, conv_bx=4, conv_kc=7, nxcd=8),
+    dict(conv_by=16, conv_kc=3, cnx_nlx=3, cnz_kz=5, cny_ky=7, fftx_tk=2, fftz_nl=1, nxcd=3, cn_xz=1,
+         cnxz_by=2),
+    dict(conv_by=1, conv_kc=1, cnx_nlx=1, cnz_kz=40, cny_ky=36, fftx_tk=8, fftz_nl=5, cn_xz=1),
+    dict(conv_by=4, conv_bx=4, conv_kc=7, nxcd=8, cnxz_by=4, cnz_kz=9, cn_xz=1),
 ]
 
 
@@ -191,7 +192,7 @@ def test_launch_shapes_keep_results(pkg, shape):
 
 def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
-    for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(conv_kc=41), dict(fftx_tk=3),
+    for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(cn_xz=2), dict(cnxz_by=3), dict(conv_kc=41), dict(fftx_tk=3),
                 dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6),
                 dict(no_such_key=1)):
         with pytest.raises(pkg.ChsError):
