@@ -6,5 +6,5 @@ to ``libchapsim_hip.so``), ``fortran/`` (ISO_C_BINDING module and drop-in subste
 ``tg.py`` (Python host mirror of the reference's substep interface).
 """
 from .tg import (MIXED, PLAIN_FP64, ChannelTG, ChsError, DeviceArray, Ensemble,  # noqa: F401
-                 channel_setup, lib)
+                 channel_setup, lib, tdma_lines)
 from .init import poiseuille_perturbed  # noqa: F401

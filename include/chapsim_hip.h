@@ -133,6 +133,22 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * unknown key or a value out of range. */
 int chs_set_tuning(chs_ctx *ctx, const char *key, int value);
 
+/* ---- Batched tridiagonal solvers ------------------------------------------------------ */
+/* The reference's four Thomas solvers on device arrays in its Fortran 2-D layout
+ * A, B, C, R(IS:IS+isz-1, JS:JS+jsz-1) (I fastest, no padding), R overwritten by the solution,
+ * bit-identical to the reference:
+ *   kind 0  TDMAIJI_CYC(A,B,C,R, IS,ISZ, JS,JSZ)          TDMAIJI_CYC.f90:67    cyclic along I
+ *   kind 1  TDMAIJJ_nonCYC(A,B,C,R, BCJ, JS,JSZ, IS,ISZ)  TDMAIJJ_nonCYC.f90:67 along J, bc = BCJ(isz, 2)
+ *   kind 2  TDMAIJI_nonCYC(A,B,C,R, BCI, IS,ISZ, JS,JSZ)  TDMAIJI_nonCYC.f90:67 along I, bc = BCI(jsz, 2)
+ *   kind 3  TDMAIJJ_CYC(A,B,C,R, JS,JSZ, IS,ISZ)          TDMAIJJ_CYC.f90:67    cyclic along J
+ * bc: the values before the first and after the last point of each line, (nlines, 2)
+ * column-major; ignored (may be NULL) for the cyclic kinds.  Runs on the default stream and
+ * returns when done.  CHS_E_ARG for a bad kind or size (a line needs 3 points if cyclic, 2
+ * otherwise); chs_tdma_last_error() explains. */
+int chs_tdma_lines(int kind, int isz, int jsz, const double *a, const double *b, const double *c, double *r,
+                   const double *bc);
+const char *chs_tdma_last_error(void);
+
 /* Device memory helpers for hosts without their own allocator (e.g. the standalone Poisson
  * workload): plain hipMalloc / hipFree / synchronous hipMemcpy (to_device 1: host->device). */
 void *chs_dev_malloc(size_t bytes);

@@ -55,6 +55,7 @@ def lib():
                            ("or_tdma_cyc", [C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                                             C.c_long, _dp]),
                            ("or_tdma_noncyc", [C.c_int, _dp, _dp, _dp, _dp, C.c_long, _dp]),
+                           ("or_tdma_lines", [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
                            ("or_dft", [C.c_int, C.c_int, _dp, _dp, _dp, _dp])):
             fn = getattr(_lib, name)
             fn.argtypes = args
@@ -132,6 +133,22 @@ def tdma_cyc(a, b, c, r):
     work = np.zeros(3 * len(r))
     lib().or_tdma_cyc(len(r), a, b, c, _p(r), 1, _p(work))
     return r
+
+
+TDMA_KINDS = {"IJI_CYC": 0, "IJJ_nonCYC": 1, "IJI_nonCYC": 2, "IJJ_CYC": 3}
+
+
+def tdma_lines(kind, A, B, C_, R, bc=None):
+    """or_tdma_lines on arrays given as C-order [jsz, isz] views of the Fortran (isz, jsz)
+    arrays; bc as [2, nlines].  Returns the solution (same shape as R)."""
+    k = TDMA_KINDS[kind] if isinstance(kind, str) else int(kind)
+    A, B, C_ = (np.ascontiguousarray(x, dtype=np.float64) for x in (A, B, C_))
+    out = np.array(R, dtype=np.float64, order="C")
+    jsz, isz = out.shape
+    bcv = np.ascontiguousarray(bc if bc is not None else np.zeros((2, isz if k == 1 else jsz)),
+                               dtype=np.float64)
+    lib().or_tdma_lines(k, isz, jsz, _p(A), _p(B), _p(C_), _p(out), _p(bcv))
+    return out
 
 
 def tdma_noncyc(a, b, c, r):

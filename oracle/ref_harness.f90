@@ -24,6 +24,11 @@
 !   dumptable FILE          rank 0: nondimensional table and spline coefficients
 !   proplookup FIN FOUT     rank 0: THERM_PROP_UPDATE_FROM_DH for every DH in FIN (int32 n,
 !                           then n float64); FOUT = H, T, D, M, K, Cp, B (7 x n float64)
+!   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
+!                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
+!                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
+!                           TDMAIJI_CYC, 1 TDMAIJJ_nonCYC, 2 TDMAIJI_nonCYC, 3 TDMAIJJ_CYC;
+!                           FOUT = R after the solve
 ! Must run on 2 or more MPI ranks (the reference deadlocks on one, INTFC_VARS.f90:57-79).
 !-----------------------------------------------------------------------------------------------
 PROGRAM ref_harness
@@ -125,6 +130,9 @@ PROGRAM ref_harness
         CASE ('proplookup')
             READ(line, *) cmd, fname2, fname
             IF (MYID == 0) CALL H_PROPLOOKUP(TRIM(fname2), TRIM(fname))
+        CASE ('tdma')
+            READ(line, *) cmd, idr, fname2, fname
+            IF (MYID == 0) CALL H_TDMA(idr, TRIM(fname2), TRIM(fname))
         CASE DEFAULT
             IF (MYID == 0) WRITE(*, *) 'ref_harness: unknown command ', TRIM(cmd)
             CALL MPI_ABORT(ICOMM, 2, IERROR)
@@ -273,6 +281,38 @@ CONTAINS
         WRITE(80) o
         CLOSE(80)
         DEALLOCATE(dh, o)
+    END SUBROUTINE
+
+    SUBROUTINE H_TDMA(kind, fin, fout)
+        INTEGER(4), INTENT(IN) :: kind
+        CHARACTER(*), INTENT(IN) :: fin, fout
+        INTEGER(4) :: isz, jsz, nl
+        REAL(WP), ALLOCATABLE :: A(:, :), B(:, :), C(:, :), R(:, :), BC(:, :)
+        OPEN(79, FILE = fin, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', ACTION = 'read')
+        READ(79) isz, jsz
+        ALLOCATE(A(isz, jsz), B(isz, jsz), C(isz, jsz), R(isz, jsz))
+        READ(79) A, B, C, R
+        nl = isz
+        IF (kind == 2) nl = jsz
+        ALLOCATE(BC(nl, 2))
+        BC = 0.0_WP
+        IF (kind == 1 .OR. kind == 2) READ(79) BC
+        CLOSE(79)
+        SELECT CASE (kind)
+        CASE (0)
+            CALL TDMAIJI_CYC(A, B, C, R, 1, isz, 1, jsz)
+        CASE (1)
+            CALL TDMAIJJ_nonCYC(A, B, C, R, BC, 1, jsz, 1, isz)
+        CASE (2)
+            CALL TDMAIJI_nonCYC(A, B, C, R, BC, 1, isz, 1, jsz)
+        CASE (3)
+            CALL TDMAIJJ_CYC(A, B, C, R, 1, jsz, 1, isz)
+        END SELECT
+        OPEN(80, FILE = fout, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'replace', &
+             ACTION = 'write')
+        WRITE(80) R
+        CLOSE(80)
+        DEALLOCATE(A, B, C, R, BC)
     END SUBROUTINE
 
 END PROGRAM ref_harness

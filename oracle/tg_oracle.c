@@ -240,6 +240,77 @@ void or_tdma_noncyc(int n, const double *a, const double *b, const double *c, do
     for (int j = n - 2; j >= 0; --j) r[j * rs] = G[j] - H[j] * r[(j + 1) * rs];
 }
 
+/* The reference's four batched Thomas solvers with full coefficient arrays, on Fortran 2-D
+ * arrays A, B, C, R(IS:IS+isz-1, JS:JS+jsz-1) (I fastest); R is overwritten by the solution.
+ *   kind 0  TDMAIJI_CYC    (TDMAIJI_CYC.f90:67-157)    cyclic, lines along I (one per J)
+ *   kind 1  TDMAIJJ_nonCYC (TDMAIJJ_nonCYC.f90:67-123) lines along J, BCJ(isz, 2)
+ *   kind 2  TDMAIJI_nonCYC (TDMAIJI_nonCYC.f90:67-123) lines along I, BCI(jsz, 2)
+ *   kind 3  TDMAIJJ_CYC    (TDMAIJJ_CYC.f90:67-157)    cyclic, lines along J (one per I)
+ * bc (kinds 1, 2): the boundary values BC(line, 1) before the first and BC(line, 2) after the
+ * last element, column-major.  Each line follows the reference's recurrences in its order. */
+void or_tdma_lines(int kind, int isz, int jsz, const double *A, const double *B, const double *C, double *R,
+                   const double *bc)
+{
+    const int alongI = (kind == 0 || kind == 2), cyc = (kind == 0 || kind == 3);
+    const int n = alongI ? isz : jsz, nl = alongI ? jsz : isz;
+    const long es = alongI ? 1 : isz, lstr = alongI ? isz : 1;
+    double *H = (double *)malloc(sizeof(double) * 3 * n);
+    double *G1 = H + n, *G2 = H + 2 * n;
+    for (int l = 0; l < nl; ++l) {
+        const double *a = A + l * lstr, *b = B + l * lstr, *c = C + l * lstr;
+        double *r = R + l * lstr;
+#define E(p, i) p[(long)(i) * es]
+        if (cyc) {
+            H[0] = E(c, 0) / E(b, 0);
+            G1[0] = E(r, 0) / E(b, 0);
+            G2[0] = -E(a, 0) / E(b, 0);
+            for (int i = 1; i <= n - 3; ++i) {
+                const double pp = 1.0 / (E(b, i) - H[i - 1] * E(a, i));
+                const double qq1 = G1[i - 1] * E(a, i), qq2 = G2[i - 1] * E(a, i);
+                H[i] = E(c, i) * pp;
+                G1[i] = (E(r, i) - qq1) * pp;
+                G2[i] = -qq2 * pp;
+            }
+            {
+                const int i = n - 2;
+                const double pp = 1.0 / (E(b, i) - H[i - 1] * E(a, i));
+                const double qq1 = G1[i - 1] * E(a, i), qq2 = G2[i - 1] * E(a, i);
+                G1[i] = (E(r, i) - qq1) * pp;
+                G2[i] = (-E(c, i) - qq2) * pp;
+            }
+            for (int i = n - 3; i >= 0; --i) {
+                G1[i] = -H[i] * G1[i + 1] + G1[i];
+                G2[i] = -H[i] * G2[i + 1] + G2[i];
+            }
+            {
+                const int i = n - 1;
+                const double pp = E(r, i) - E(c, i) * G1[0] - E(a, i) * G1[i - 1];
+                const double qq1 = E(b, i) + E(c, i) * G2[0] + E(a, i) * G2[i - 1];
+                E(r, i) = pp / qq1;
+            }
+            for (int i = n - 2; i >= 0; --i) E(r, i) = G1[i] + G2[i] * E(r, n - 1);
+        } else {
+            const double bc1 = bc[l], bc2 = bc[nl + l];
+            H[0] = E(c, 0) / E(b, 0);
+            G1[0] = (E(r, 0) - E(a, 0) * bc1) / E(b, 0);
+            for (int i = 1; i <= n - 2; ++i) {
+                const double pp = 1.0 / (E(b, i) - H[i - 1] * E(a, i));
+                H[i] = E(c, i) * pp;
+                G1[i] = (E(r, i) - E(a, i) * G1[i - 1]) * pp;
+            }
+            {
+                const int i = n - 1;
+                const double pp = 1.0 / (E(b, i) - H[i - 1] * E(a, i));
+                G1[i] = (E(r, i) - E(c, i) * bc2 - E(a, i) * G1[i - 1]) * pp;
+                E(r, i) = G1[i];
+            }
+            for (int i = n - 2; i >= 0; --i) E(r, i) = G1[i] - H[i] * E(r, i + 1);
+        }
+#undef E
+    }
+    free(H);
+}
+
 /* MOMFA_tg (MOMFA_tg.f90:15-47): x (MOMFA1_X_tg :51-106) and z (MOMFA2_Z_tg :110-169) cyclic
  * solves over rows J >= N2I, the y solve (MOMFA3_Y_tg :172-239) over all rows J = 1..M, then
  * Q += RHS for rows J >= N2I. */

@@ -10,7 +10,10 @@ Each ``tg_<L>x<M>x<N>.npz`` holds, for one channel mesh:
   a reference Poisson solve of a seeded compatible RHS (``poisson_rhs``, ``poisson_phi``).
 Arrays are C-order views of the reference's Fortran layout (z, y, x).
 
-Run: python tests/golden/make_golden.py   (needs oracle/_ref built; about a minute)
+Also ``props_2048.npz`` (property tables, make_props) and ``tdma.npz`` (the reference's
+batched Thomas solvers, make_tdma).
+
+Run: python tests/golden/make_golden.py [tg] [props] [tdma]   (needs oracle/_ref built)
 """
 from __future__ import annotations
 
@@ -98,7 +101,54 @@ def make_props(T0=665.0, seed=21):
     print("wrote", path, os.path.getsize(path))
 
 
+TDMA_SIZES = [(7, 5), (16, 9), (3, 4), (4, 3), (33, 12)]
+
+
+def make_tdma(seed=31):
+    """Batched Thomas fixture: for each kind of the reference's TDMAIJI_CYC, TDMAIJJ_nonCYC,
+    TDMAIJI_nonCYC, TDMAIJJ_CYC and each (isz, jsz) of TDMA_SIZES, seeded diagonally dominant
+    coefficients, RHS and boundary values, and R after the reference's solve."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    out = {}
+    h = refrun.Harness(8, 8, 8)
+    try:
+        cmds, cases = [], []
+        for kind in range(4):
+            for isz, jsz in TDMA_SIZES:
+                A = rng.uniform(-1, 1, (jsz, isz))
+                C_ = rng.uniform(-1, 1, (jsz, isz))
+                B = 2.5 + rng.uniform(0, 1, (jsz, isz))
+                R = rng.standard_normal((jsz, isz))
+                nl = jsz if kind == 2 else isz
+                bc = rng.standard_normal((2, nl)) if kind in (1, 2) else np.zeros((2, nl))
+                tag = f"k{kind}_{isz}x{jsz}"
+                with open(h.path(tag + ".in"), "wb") as fh:
+                    fh.write(np.array([isz, jsz], dtype=np.int32).tobytes())
+                    for x in (A, B, C_, R):
+                        fh.write(np.ascontiguousarray(x).tobytes())   # C [jsz, isz] = Fortran (isz, jsz)
+                    if kind in (1, 2):
+                        fh.write(np.ascontiguousarray(bc).tobytes())  # C [2, nl] = Fortran (nl, 2)
+                cmds.append(f"tdma {kind} {tag}.in {tag}.out")
+                cases.append((tag, A, B, C_, R, bc))
+        h.run(cmds)
+        for tag, A, B, C_, R, bc in cases:
+            isz = A.shape[1]
+            out.update({f"{tag}_A": A, f"{tag}_B": B, f"{tag}_C": C_, f"{tag}_R": R, f"{tag}_bc": bc,
+                        f"{tag}_X": np.fromfile(h.path(tag + ".out"), dtype=np.float64).reshape(-1, isz)})
+    finally:
+        h.close()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tdma.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
-    for case in CASES:
-        make_case(*case)
-    make_props()
+    import sys as _sys
+    which = _sys.argv[1:] or ["tg", "props", "tdma"]
+    if "tg" in which:
+        for case in CASES:
+            make_case(*case)
+    if "props" in which:
+        make_props()
+    if "tdma" in which:
+        make_tdma()
