@@ -756,8 +756,11 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
 template <typename T, typename P>
 __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict__ amph,
                                            const double *__restrict__ apph, const P *__restrict__ p,
-                                           cpx<T> *__restrict__ s, const size_t st, const bool zero)
+                                           const size_t pst, cpx<T> *__restrict__ s, const size_t st,
+                                           const bool zero, const bool lastpin)
 {
+    // zero: the (0,0) mode with FFT99's gauge (row 1 decoupled, RHS(1) = 0); lastpin: the
+    // (0,0) mode with TRID0's (x_M = 0, possion3d_fishpack.f90:322-332), pivots unmodified
     // Rows go in batches of B; the loads of the next batch are issued before the current
     // batch's recurrence (double buffer in registers), and pivots stay in their storage type
     // until used, so no load waits on a conversion.
@@ -779,7 +782,7 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             rc[u] = s[(size_t)u * st];
-            pc[u] = p[(size_t)u * st];
+            pc[u] = p[(size_t)u * pst];
         }
     }
     for (int b = 0; b < nf; ++b) {
@@ -788,7 +791,7 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             rn[u] = s[(size_t)(Jn + u - 1) * st];
-            pn[u] = p[(size_t)(Jn + u - 1) * st];
+            pn[u] = p[(size_t)(Jn + u - 1) * pst];
         }
 #pragma unroll
         for (int u = 0; u < B; ++u) fwd(J0 + u, rc[u], (double)pc[u]);
@@ -798,9 +801,14 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
             pc[u] = pn[u];
         }
     }
-    for (int J = nf * B + 1; J <= M; ++J) fwd(J, s[(size_t)(J - 1) * st], (double)p[(size_t)(J - 1) * st]);
+    for (int J = nf * B + 1; J <= M; ++J) fwd(J, s[(size_t)(J - 1) * st], (double)p[(size_t)(J - 1) * pst]);
     // back substitution x_J = g_J - c_J p_J x_{J+1}, rows M-1 down to 1 (x_M = g_M)
     double xr = gr, xi = gi;
+    if (lastpin) {
+        xr = 0.0;
+        xi = 0.0;
+        s[(size_t)(M - 1) * st] = {T(0), T(0)};
+    }
     auto bwd = [&](int J, cpx<T> gg, double pj) {
         const double c = (zero && J == 1) ? 0.0 : apph[J - 1];
         const double h = c * pj;
@@ -813,7 +821,7 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             rc[u] = s[(size_t)(M - 2 - u) * st];
-            pc[u] = p[(size_t)(M - 2 - u) * st];
+            pc[u] = p[(size_t)(M - 2 - u) * pst];
         }
     }
     for (int b = 0; b < nb; ++b) {
@@ -822,7 +830,7 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             rn[u] = s[(size_t)(Jn - u - 1) * st];
-            pn[u] = p[(size_t)(Jn - u - 1) * st];
+            pn[u] = p[(size_t)(Jn - u - 1) * pst];
         }
 #pragma unroll
         for (int u = 0; u < B; ++u) bwd(J0 - u, rc[u], (double)pc[u]);
@@ -832,13 +840,14 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
             pc[u] = pn[u];
         }
     }
-    for (int J = M - 1 - nb * B; J >= 1; --J) bwd(J, s[(size_t)(J - 1) * st], (double)p[(size_t)(J - 1) * st]);
+    for (int J = M - 1 - nb * B; J >= 1; --J) bwd(J, s[(size_t)(J - 1) * st], (double)p[(size_t)(J - 1) * pst]);
 }
 
+// pp00: FISHPACK gauge only, the (0,0) mode's unmodified pivots (M values).
 template <typename T>
 __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__ amph, const double *__restrict__ apph,
                                                const double *__restrict__ pp, const float *__restrict__ pp32,
-                                               cpx<T> *__restrict__ spec)
+                                               const double *__restrict__ pp00, cpx<T> *__restrict__ spec)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int H = g.H, N = g.N, M = g.M;
@@ -846,11 +855,15 @@ __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__
     const int kx = t / N, kz = t - kx * N;
     const bool zero = (kx == 0 && kz == 0);
     const size_t st = (size_t)H * N, off = (size_t)kx * N + kz;
+    if (zero && pp00) {
+        ytdma_line<T, double>(M, amph, apph, pp00, 1, spec, st, false, true);
+        return;
+    }
     // FP32 pivots for all but the lowest modes (kx <= 1, |kz| <= 1), whose Neumann/near-
     // Neumann y-systems are ill-conditioned on the stretched mesh
     const bool low = !pp32 || (kx <= 1 && (kz <= 1 || kz == N - 1));
-    if (low) ytdma_line<T, double>(M, amph, apph, pp + off, spec + off, st, zero);
-    else ytdma_line<T, float>(M, amph, apph, pp32 + off, spec + off, st, zero);
+    if (low) ytdma_line<T, double>(M, amph, apph, pp + off, st, spec + off, st, zero, false);
+    else ytdma_line<T, float>(M, amph, apph, pp32 + off, st, spec + off, st, zero, false);
 }
 
 // Inverse real x-transform of TK z-planes at row J -> phi (halo field) and the Neumann
@@ -1234,6 +1247,7 @@ struct chs_ctx {
     void *h0 = nullptr, *rhs = nullptr, *rhs2 = nullptr, *spec = nullptr;
     double *pp = nullptr, *partial = nullptr, *scal = nullptr;
     float *pp32 = nullptr;   // FP32 copy of the Poisson pivots (mixed form, high modes)
+    double *pp00 = nullptr;  // (0,0)-mode pivots without FFT99's decoupling (FISHPACK gauge)
     int npartial = 0;
     void *twx = nullptr, *twz = nullptr;
     void *rtwx = nullptr, *rtwz = nullptr;   // register-FFT pass twiddles (chs_fftr.h), if planned
@@ -1539,8 +1553,9 @@ void fftx_inv_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw
 }
 #undef CHS_RCASES
 
+// gauge 0: FFT99's (0,0) rule (row 1 decoupled), 1: TRID0's (last row pinned, FISHPACK)
 template <typename T>
-void poisson_launch(chs_ctx *c, double coe0, const double *src)
+void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
 {
     const Geo &g = c->geo;
     const int L = c->L, M = c->M, N = c->N, H = c->H;
@@ -1570,7 +1585,8 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src)
     {
         EvScope ev(c, "ytdma");
         hipLaunchKernelGGL(k_ytdma<T>, dim3((H * N + 255) / 256), dim3(256), 0, c->stream, g, c->d_amph,
-                           c->d_apph, c->pp, c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr, spec);
+                           c->d_apph, c->pp, c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr,
+                           gauge ? (const double *)c->pp00 : (const double *)nullptr, spec);
     }
     {
         EvScope ev(c, "fftz_inv");
@@ -1744,7 +1760,7 @@ int poisson_device_impl(chs_ctx *c, const double *rhs, double *phi, int nref, do
 }
 
 template <typename T>
-int poisson_impl(chs_ctx *c, const double *rhs, double *phi)
+int poisson_impl(chs_ctx *c, const double *rhs, double *phi, int gauge)
 {
     const int L = c->L, M = c->M, N = c->N;
     const size_t cs = (size_t)N * M * L, hs = (size_t)N * (M + 2) * L;
@@ -1753,7 +1769,7 @@ int poisson_impl(chs_ctx *c, const double *rhs, double *phi)
     HIPCHK(c, hipMalloc(&src, sizeof(double) * cs));
     hipError_t e = hipMemcpyAsync(src, rhs, sizeof(double) * cs, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        poisson_launch<T>(c, 1.0, src);
+        poisson_launch<T>(c, 1.0, src, gauge);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1992,6 +2008,19 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
             }
         CK(hipMemcpyAsync(c->pp, pp.data(), sizeof(double) * pp.size(), hipMemcpyHostToDevice, c->stream));
         CK(hipStreamSynchronize(c->stream));
+        // (0,0) mode as TRID0 eliminates it (possion3d_fishpack.f90:310-340): unmodified
+        // coefficients, rows 1..M-1; row M is the singular one whose unknown is pinned to 0
+        std::vector<double> p00(M, 0.0);
+        double hprev = 0.0;
+        for (int J = 1; J < M; ++J) {
+            const double a = c->amph[J - 1], cc = c->apph[J - 1], b = c->acph[J - 1];
+            const double p = (J == 1) ? 1.0 / b : 1.0 / (b - hprev * a);
+            hprev = cc * p;
+            p00[J - 1] = p;
+        }
+        CK(hipMalloc(&c->pp00, sizeof(double) * M));
+        CK(hipMemcpyAsync(c->pp00, p00.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->stream));
+        CK(hipStreamSynchronize(c->stream));
         if (mode == CHS_MIXED) {
             std::vector<float> p32(pp.begin(), pp.end());
             CK(hipMalloc(&c->pp32, sizeof(float) * p32.size()));
@@ -2018,7 +2047,7 @@ void chs_destroy(chs_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cf, c->q, c->pr, c->phi, c->div, c->h0, c->rhs, c->rhs2, c->spec, c->pp,
-                    c->pp32, c->partial, c->scal, c->twx, c->twz, c->rtwx, c->rtwz};
+                    c->pp32, c->pp00, c->partial, c->scal, c->twx, c->twz, c->rtwx, c->rtwz};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->cn) {
@@ -2221,7 +2250,13 @@ int chs_set_graph(chs_ctx *c, int enable)
 int chs_poisson(chs_ctx *c, const double *rhs, double *phi)
 {
     if (!c || !rhs || !phi) return fail(c, CHS_E_ARG, "null argument");
-    return c->mode == CHS_MIXED ? poisson_impl<float>(c, rhs, phi) : poisson_impl<double>(c, rhs, phi);
+    return c->mode == CHS_MIXED ? poisson_impl<float>(c, rhs, phi, 0) : poisson_impl<double>(c, rhs, phi, 0);
+}
+
+int chs_poisson_fishpack(chs_ctx *c, const double *rhs, double *phi)
+{
+    if (!c || !rhs || !phi) return fail(c, CHS_E_ARG, "null argument");
+    return c->mode == CHS_MIXED ? poisson_impl<float>(c, rhs, phi, 1) : poisson_impl<double>(c, rhs, phi, 1);
 }
 
 int chs_poisson_device(chs_ctx *c, const double *rhs, double *phi, int nref, double *resmax)

@@ -50,6 +50,7 @@ SIGNATURES = {
     "chs_dev_free": (C.c_int, [C.c_void_p]),
     "chs_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "chs_poisson": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "chs_poisson_fishpack": (C.c_int, [C.c_void_p, _dp, _dp]),
     "chs_poisson_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp]),
     "chs_cfl_tg": (C.c_int, [C.c_void_p, C.c_double, _dp]),
     "chs_max_div": (C.c_int, [C.c_void_p, _dp]),
@@ -210,6 +211,17 @@ class ChannelTG:
             raise ChsError("rhs shape must be [N,M,L]")
         phi = np.zeros((N, M, L))
         _check(lib().chs_poisson(self._h, _p(rhs), _p(phi)), self._h)
+        return phi
+
+    def FISHPACK_POIS3D_SIMPLE(self, rhs):
+        """Main-domain pressure solve with TRID0's gauge (chs_poisson_fishpack); rhs and the
+        result are [N,M,L] cell arrays."""
+        L, M, N = self.L, self.M, self.N
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        if rhs.shape != (N, M, L):
+            raise ChsError("rhs shape must be [N,M,L]")
+        phi = np.zeros((N, M, L))
+        _check(lib().chs_poisson_fishpack(self._h, _p(rhs), _p(phi)), self._h)
         return phi
 
     def poisson_device(self, rhs_ptr: int, phi_ptr: int, nref: int = 0, want_residual=True):

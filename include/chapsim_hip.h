@@ -160,6 +160,15 @@ int chs_memcpy(void *dst, const void *src, size_t bytes, int to_device);
  * (interior rows).  Host arrays; synchronous. */
 int chs_poisson(chs_ctx *ctx, const double *rhs, double *phi);
 
+/* The main-domain pressure solve FISHPACK_POIS3D_SIMPLE(RHS, VAR) (possion3d_fishpack.f90:342-453:
+ * FFTPACK real transforms in x and z, per-mode y-Thomas TRID0, scale 1/(L N)) on host cell
+ * arrays [N][M][L].  The same discrete operator as chs_poisson, with TRID0's gauge for the
+ * (0,0) mode: its last row is pinned, phi_hat_00(J=M) = 0 (possion3d_fishpack.f90:329-331;
+ * the reference pins it only when the singular pivot rounds below 1e-14, otherwise its
+ * constant is set by rounding).  VAR is the array of the routine's declared shape
+ * (NCL1_io, NCL2, NCL3), without y-halo rows. */
+int chs_poisson_fishpack(chs_ctx *ctx, const double *rhs, double *phi);
+
 /* Same solve on device arrays (cells, N x M x L as RHSLLPHI) on the context's stream, with
  * nref (0..8) refinement sweeps: r = rhs - L phi in FP64, phi += solve(r).  resmax, if not
  * NULL, receives max |rhs - L phi| after the last sweep (then the call waits).  This is

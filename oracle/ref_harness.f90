@@ -24,6 +24,11 @@
 !   dumptable FILE          rank 0: nondimensional table and spline coefficients
 !   proplookup FIN FOUT     rank 0: THERM_PROP_UPDATE_FROM_DH for every DH in FIN (int32 n,
 !                           then n float64); FOUT = H, T, D, M, K, Cp, B (7 x n float64)
+!   poissonio FIN FOUT      (main/IO domain) RHSLLPHI_io from a global L x M x N file,
+!                           FISHPACK_POIS3D_SIMPLE into an array of its declared shape,
+!                           dumps the solution rows 1..N2DO
+!   poissonio_call FIN FOUT same, with DPH_io as the output argument as the IO substep
+!                           passes it; dumps DPH_io rows 1..N2DO
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -37,11 +42,13 @@ PROGRAM ref_harness
     USE flow_info
     USE thermal_info
     USE FFT99_info1
+    USE FISHPACK_POIS3D
     IMPLICIT NONE
 
     CHARACTER(LEN=512) :: line, cmd, fname, fname2
     INTEGER(4) :: ios, ns, idr
     REAL(WP) :: dtin
+    REAL(WP), ALLOCATABLE :: VIO(:, :, :)
 
     CALL MKDIR
     CALL Start_MPI
@@ -67,6 +74,7 @@ PROGRAM ref_harness
     IF (MYID == 0) CALL TDMA_COEF
     CALL BCAST_TDMA_COEF
     IF (TgFlowFlg) CALL FFT99_POIS3D_INIT(ITG)
+    IF (IoFlowFlg) CALL FISHPACK_POIS3D_INIT   ! DNS_THERMAL.f90:128 (is_FFT_FISHPACK)
     CALL MPI_BARRIER(ICOMM, IERROR)
 
     IF (TgFlowFlg) CONVH0_tg = 0.0_WP
@@ -130,6 +138,26 @@ PROGRAM ref_harness
         CASE ('proplookup')
             READ(line, *) cmd, fname2, fname
             IF (MYID == 0) CALL H_PROPLOOKUP(TRIM(fname2), TRIM(fname))
+        CASE ('poissonio')
+            ! the solver's output argument is VAR(NCL1_io, N2DO(0), NCL3); SOLVERRK3_MOM_IO.f90:90-93
+            ! passes DPH_io, which has y-halo rows (MEMALLOCT.f90:157), so the harness gives it an
+            ! array of the declared shape to pin the solver itself
+            READ(line, *) cmd, fname2, fname
+            CALL H_LOADRHSIO(TRIM(fname2))
+            ALLOCATE(VIO(NCL1_io, N2DO(0), NCL3))
+            CALL FISHPACK_POIS3D_SIMPLE(RHSLLPHI_io, VIO)
+            CALL H_OPENRANK(TRIM(fname), 80)
+            WRITE(80) VIO(:, 1:N2DO(MYID), :)
+            CLOSE(80)
+            DEALLOCATE(VIO)
+        CASE ('poissonio_call')
+            ! as SOLVERRK3_MOM_IO.f90:93 calls it (DPH_io as the output argument)
+            READ(line, *) cmd, fname2, fname
+            CALL H_LOADRHSIO(TRIM(fname2))
+            CALL FISHPACK_POIS3D_SIMPLE(RHSLLPHI_io, DPH_io)
+            CALL H_OPENRANK(TRIM(fname), 80)
+            WRITE(80) DPH_io(1:NCL1_io, 1:N2DO(MYID), :)
+            CLOSE(80)
         CASE ('tdma')
             READ(line, *) cmd, idr, fname2, fname
             IF (MYID == 0) CALL H_TDMA(idr, TRIM(fname2), TRIM(fname))
@@ -153,7 +181,7 @@ CONTAINS
         OPEN(unit, FILE = fn // '.' // TRIM(rk), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
              STATUS = 'replace', ACTION = 'write')
         WRITE(unit) INT(MYID, 4), INT(JDSWT(MYID), 4), INT(N2DO(MYID), 4), &
-                    INT(NCL1_tg, 4), INT(NCL2, 4), INT(NCL3, 4)
+                    INT(NCL1_tg + NCL1_io, 4), INT(NCL2, 4), INT(NCL3, 4)
     END SUBROUTINE
 
     SUBROUTINE H_DUMPSETUP(fn)
@@ -238,6 +266,21 @@ CONTAINS
         CLOSE(80)
     END SUBROUTINE
 
+    SUBROUTINE H_LOADRHSIO(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: G(:, :, :)
+        INTEGER(4) :: J, JG
+        ALLOCATE(G(NCL1_io, NCL2, NCL3))
+        OPEN(79, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', &
+             ACTION = 'read')
+        READ(79) G
+        CLOSE(79)
+        DO J = 1, N2DO(MYID)
+            JG = JDSWT(MYID) - 1 + J
+            RHSLLPHI_io(:, J, :) = G(:, JG, :)
+        END DO
+        DEALLOCATE(G)
+    END SUBROUTINE
     SUBROUTINE H_DUMP3H(fn, A)
         CHARACTER(*), INTENT(IN) :: fn
         REAL(WP), INTENT(IN) :: A(NCL1_tg, 0:N2DO(0) + 1, NCL3)

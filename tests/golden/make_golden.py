@@ -10,10 +10,11 @@ Each ``tg_<L>x<M>x<N>.npz`` holds, for one channel mesh:
   a reference Poisson solve of a seeded compatible RHS (``poisson_rhs``, ``poisson_phi``).
 Arrays are C-order views of the reference's Fortran layout (z, y, x).
 
-Also ``props_2048.npz`` (property tables, make_props) and ``tdma.npz`` (the reference's
-batched Thomas solvers, make_tdma).
+Also ``props_2048.npz`` (property tables, make_props), ``tdma.npz`` (the reference's
+batched Thomas solvers, make_tdma) and ``fishpack.npz`` (the main-domain FISHPACK Poisson
+solve, make_fishpack).
 
-Run: python tests/golden/make_golden.py [tg] [props] [tdma]   (needs oracle/_ref built)
+Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack]   (needs oracle/_ref)
 """
 from __future__ import annotations
 
@@ -142,9 +143,34 @@ def make_tdma(seed=31):
     print("wrote", path, os.path.getsize(path))
 
 
+FISHPACK_MESHES = [(16, 12, 24, 2.2), (32, 20, 16, 2.2), (64, 8, 64, 2.0)]
+
+
+def make_fishpack(seed=41):
+    """Main-domain Poisson fixture: seeded compatible RHS (Delta-y-weighted mean removed) and
+    the reference FISHPACK_POIS3D_SIMPLE solution (harness command poissonio) per mesh."""
+    out = {}
+    for L, M, N, STR2 in FISHPACK_MESHES:
+        mesh = setup.Mesh(L, M, N, STR2=STR2)
+        rhs = compatible_rhs(mesh, seed + L)
+        h = refrun.Harness(L, M, N, STR2=STR2, iDomain=2)
+        try:
+            h.write_cells("prhs.bin", rhs)
+            h.run(["poissonio prhs.bin phi"])
+            phi = h.read_cells("phi")
+        finally:
+            h.close()
+        tag = f"m{L}x{M}x{N}"
+        out.update({f"{tag}_rhs": rhs, f"{tag}_phi": phi,
+                    f"{tag}_params": np.array([L, M, N, STR2], dtype=np.float64)})
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fishpack.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
     import sys as _sys
-    which = _sys.argv[1:] or ["tg", "props", "tdma"]
+    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack"]
     if "tg" in which:
         for case in CASES:
             make_case(*case)
@@ -152,3 +178,5 @@ if __name__ == "__main__":
         make_props()
     if "tdma" in which:
         make_tdma()
+    if "fishpack" in which:
+        make_fishpack()
