@@ -45,8 +45,10 @@ def test_gpu_fishpack_matches_reference(pkg, golden, mode):
         ref = g[t + "_phi"]
         tol = 1e-10 if mode == "plain" else 1e-5
         assert _rel_nogauge(phi, ref) < tol, (t, _rel_nogauge(phi, ref))
-        # TRID0's gauge: the (0,0) mode vanishes on the last row
-        assert abs(phi[:, M - 1, :].mean()) < 1e-12 * np.abs(ref).max()
+        # TRID0's gauge: the (0,0) mode vanishes on the last row (to the rounding of the
+        # inverse transforms: FP32 in the mixed form)
+        gtol = 1e-12 if mode == "plain" else 1e-8
+        assert abs(phi[:, M - 1, :].mean()) < gtol * np.abs(ref).max()
         if abs(ref[:, M - 1, :].mean()) < 1e-15 * np.abs(ref).max():   # the reference pinned it
             assert np.abs(phi - ref).max() / np.abs(ref).max() < tol
         ch.close()
