@@ -139,6 +139,147 @@ int first_min(const std::vector<double> &a)
     return k + 1;
 }
 
+// ---------------------------------------------------------------------------------------
+// Energy equation of the main domain: SOLVERRK3_ENG_io (SOLVERRK3_ENG_IO.f90:11-82).  State as
+// halo fields [N][M+2][L] (the reference's (L, 0:M+1, N)); one y-domain; channel (RCCI1 =
+// RNDI1 = RCCI2 = 1, COORDJR.f90:39-42).  Every expression keeps the reference's operation
+// order, contraction off, so results follow the reference's rounding.
+struct EngDev {
+    int L, M, N;
+    double dxi, dzi, dxqi, dzqi;
+    const double *dyfi, *dyci, *wff, *wfb;   // indexed by the reference's J (dyfi 1..M, others 1..M+1)
+    double *dh, *h, *t, *d, *m, *k, *cp;     // halo fields
+    const double *g1, *g2, *g3;              // G_io components, halo fields
+    double *rhs0;                            // cells [N][M][L]
+    const double *wallflux;                  // [2][N][L]
+    int wall_type0, wall_type1, top_face;
+    double cthecd, dhmin, dhmax;
+    double ws[2][5];                         // fixed-temperature wall states T, H, D, M, K
+};
+
+__device__ __forceinline__ size_t ehx(const EngDev &e, int i, int J, int k)
+{
+    return ((size_t)k * (e.M + 2) + J) * e.L + i;
+}
+
+// RHS_ENERGY_EXPLICIT (RHS_ENERGY_EXPLICIT.f90:43-150) and the RK update with the table
+// limiter (SOLVERRK3_ENG_IO.f90:35-47): DH += ca R + cb R0, clamp, R0 = R.
+__global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, double cb)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    if (i >= e.L || J > e.M) return;
+    const int L = e.L, N = e.N, M = e.M;
+    const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+    const int kp = (k == N - 1) ? 0 : k + 1, km = (k == 0) ? N - 1 : k - 1;
+    const double one = 1.0, xnd2cl = 0.5, znd2cl = 0.5;
+    const double coe1 = e.dxi * xnd2cl * 0.5;
+    const double coe11 = e.dxqi * xnd2cl * e.cthecd;
+    const double coe2 = e.dyfi[J] * one * 0.5;
+    const double coe3 = e.dzi * znd2cl * one * 0.5;
+    const double coe33 = e.dzqi * znd2cl * one * e.cthecd;
+    const double coe22 = e.dyfi[J] * one * e.cthecd;
+    double coe221, coe222;
+    if (J == 1) {
+        coe221 = e.dyci[J + 1] / one;
+        coe222 = e.wall_type0 == 1 ? 1.0 / one : e.dyci[J] / one;
+    } else if (J == M) {
+        coe222 = e.dyci[J] / one;
+        coe221 = e.wall_type1 == 1 ? 1.0 / one : e.dyci[J + 1] / one;
+    } else {
+        coe221 = e.dyci[J + 1] / one;
+        coe222 = e.dyci[J] / one;
+    }
+    const double wffp = e.wff[J + 1], wfbp = e.wfb[J + 1], wff = e.wff[J], wfb = e.wfb[J];
+    const double *H = e.h, *T = e.t, *K = e.k;
+    auto X = [&](const double *f, int ii, int jj, int kk) { return f[ehx(e, ii, jj, kk)]; };
+    double dx_dhu = (X(H, ip, J, k) + X(H, i, J, k)) * (X(e.g1, ip, J, k) + X(e.g1, ip, J, k)) -
+                    (X(H, i, J, k) + X(H, im, J, k)) * (X(e.g1, i, J, k) + X(e.g1, i, J, k));
+    dx_dhu = dx_dhu * coe1;
+    double dy_dhv = (wffp * X(H, i, J + 1, k) + wfbp * X(H, i, J, k)) * (X(e.g2, i, J + 1, k) + X(e.g2, i, J + 1, k)) -
+                    (wff * X(H, i, J, k) + wfb * X(H, i, J - 1, k)) * (X(e.g2, i, J, k) + X(e.g2, i, J, k));
+    dy_dhv = dy_dhv * coe2;
+    double dz_dhw = (X(H, i, J, kp) + X(H, i, J, k)) * (X(e.g3, i, J, kp) + X(e.g3, i, J, kp)) -
+                    (X(H, i, J, k) + X(H, i, J, km)) * (X(e.g3, i, J, k) + X(e.g3, i, J, k));
+    dz_dhw = dz_dhw * coe3;
+    double dx_kdt = (X(K, ip, J, k) + X(K, i, J, k)) * (X(T, ip, J, k) - X(T, i, J, k)) -
+                    (X(K, i, J, k) + X(K, im, J, k)) * (X(T, i, J, k) - X(T, im, J, k));
+    dx_kdt = dx_kdt * coe11;
+    double kdty1 = (wffp * X(K, i, J + 1, k) + wfbp * X(K, i, J, k)) * (X(T, i, J + 1, k) - X(T, i, J, k)) * coe221;
+    double kdty2 = (wff * X(K, i, J, k) + wfb * X(K, i, J - 1, k)) * (X(T, i, J, k) - X(T, i, J - 1, k)) * coe222;
+    if (J == 1 && e.wall_type0 == 1)
+        kdty2 = -e.wallflux[(size_t)k * L + i] * coe222;
+    else if (J == M && e.wall_type1 == 1)
+        kdty1 = -e.wallflux[((size_t)N + k) * L + i] * coe221;
+    const double dy_kdt = (kdty1 - kdty2) * coe22;
+    double dz_kdt = (X(K, i, J, kp) + X(K, i, J, k)) * (X(T, i, J, kp) - X(T, i, J, k)) -
+                    (X(K, i, J, km) + X(K, i, J, k)) * (X(T, i, J, k) - X(T, i, J, km));
+    dz_kdt = dz_kdt * coe33;
+    const double r = -dx_dhu - dy_dhv - dz_dhw + dx_kdt + dy_kdt + dz_kdt;
+    const size_t c0 = ((size_t)k * M + (J - 1)) * L + i;
+    const size_t h0 = ehx(e, i, J, k);
+    double dh = e.dh[h0] + ca * r + cb * e.rhs0[c0];
+    dh = fmin(e.dhmax - 1.0e-14, dh);   // REALMIN (modules.f90:56)
+    dh = fmax(e.dhmin + 1.0e-14, dh);
+    e.dh[h0] = dh;
+    e.rhs0[c0] = r;
+}
+
+__device__ void eng_cell_props(const EngDev &e, const PropDev &p, size_t h0, int *err)
+{
+    const double hv = spl_eval(p.dhh, p.n, e.dh[h0], err, 1);
+    e.h[h0] = hv;
+    e.t[h0] = spl_eval(p.ht, p.n, hv, err, 2);
+    e.d[h0] = spl_eval(p.hm, p.n, hv, err, 4);    // D from H->M, as the reference
+    e.m[h0] = spl_eval(p.hk, p.n, hv, err, 8);    // M from H->K
+    e.k[h0] = spl_eval(p.hd, p.n, hv, err, 16);   // K from H->D
+    e.cp[h0] = spl_eval(p.hcp, p.n, hv, err, 32);
+}
+
+// THERM_PROP_UPDATE (THERM_PROP_UPDATE.f90:124-168) over rows j0..j1.
+__global__ __launch_bounds__(256) void k_eng_props(EngDev e, PropDev p, int j0, int j1, int *err)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = j0 + blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= e.L || J > j1) return;
+    eng_cell_props(e, p, ehx(e, i, J, k), err);
+}
+
+// BC_WALL_THERMAL(IMAIND) (BC_WALL.f90:622-643): iso-flux walls extrapolate DH to the ghost
+// row and look up H, T, D, M, K, Cp there (BC_WALL_Isoflux :575-615); iso-thermal walls take
+// the wall state and DH = H D (BC_WALL_IsoTHERMAL :484-538).  One thread per (i, k).
+__global__ __launch_bounds__(256) void k_eng_walls(EngDev e, PropDev p, int *err)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= e.L) return;
+    const int M = e.M;
+    for (int w = 0; w < 2; ++w) {
+        const int J = w == 0 ? 0 : M + 1;
+        const size_t h0 = ehx(e, i, J, k);
+        if ((w == 0 ? e.wall_type0 : e.wall_type1) == 1) {
+            if (w == 0)
+                e.dh[h0] = 2.0 * e.dh[ehx(e, i, 1, k)] - e.wff[2] * e.dh[ehx(e, i, 2, k)] -
+                           e.wfb[2] * e.dh[ehx(e, i, 1, k)];
+            else
+                e.dh[h0] = 2.0 * e.dh[ehx(e, i, M, k)] - e.wff[e.top_face] * e.dh[ehx(e, i, M, k)] -
+                           e.wfb[e.top_face] * e.dh[ehx(e, i, M - 1, k)];
+            eng_cell_props(e, p, h0, err);
+        } else {
+            e.t[h0] = e.ws[w][0];
+            e.h[h0] = e.ws[w][1];
+            e.d[h0] = e.ws[w][2];
+            e.m[h0] = e.ws[w][3];
+            e.k[h0] = e.ws[w][4];
+            e.dh[h0] = e.h[h0] * e.d[h0];
+        }
+    }
+}
+
 }  // namespace
 
 struct chs_prop {
@@ -150,7 +291,186 @@ struct chs_prop {
     std::string errmsg;
 };
 
+struct chs_eng {
+    int L = 0, M = 0, N = 0;
+    hipStream_t stream = nullptr;
+    chs_prop *prop = nullptr;
+    double *buf = nullptr;     // device: fields, history, coefficients
+    int *err = nullptr;
+    EngDev e{};
+    double tgam[4], troh[4];
+    size_t hs = 0;             // halo field size
+    double *d0 = nullptr, *m0 = nullptr, *dh0 = nullptr;
+    std::string errmsg;
+};
+
+namespace {
+
+int eng_fail(chs_eng *e, int code, const std::string &msg)
+{
+    if (e) e->errmsg = msg;
+    g_prop_err = msg;
+    return code;
+}
+
+// Reads and clears the table-range flags of the last launches (the reference STOPs there).
+int eng_check(chs_eng *e, const char *what)
+{
+    int herr = 0;
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess) he = hipMemcpyAsync(&herr, e->err, sizeof(int), hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string(what) + ": " + hipGetErrorString(he));
+    if (herr) {
+        (void)hipMemsetAsync(e->err, 0, sizeof(int), e->stream);
+        return eng_fail(e, CHS_E_RANGE, std::string(what) + ": DH outside the property table (flags " +
+                                           std::to_string(herr) + ")");
+    }
+    return CHS_OK;
+}
+
+void eng_props_and_walls(chs_eng *e, int j0, int j1)
+{
+    const dim3 blk(64, 4), grd((e->L + 63) / 64, (j1 - j0 + 4) / 4, e->N);
+    hipLaunchKernelGGL(k_eng_props, grd, blk, 0, e->stream, e->e, e->prop->p, j0, j1, e->err);
+    hipLaunchKernelGGL(k_eng_walls, dim3((e->L + 255) / 256, e->N), dim3(256), 0, e->stream, e->e, e->prop->p,
+                       e->err);
+}
+
+}  // namespace
+
 extern "C" {
+
+int chs_eng_create(const chs_coeffs *c, const chs_prop_table *table, const chs_eng_params *p, chs_eng **out)
+{
+    if (!c || !table || !p || !out || c->L < 1 || c->M < 3 || c->N < 1 || !p->wallflux ||
+        (p->wall_type[0] != 1 && p->wall_type[0] != 2) || (p->wall_type[1] != 1 && p->wall_type[1] != 2) ||
+        p->top_face < 0 || p->top_face > c->M)
+        return eng_fail(nullptr, CHS_E_ARG, "chs_eng_create: bad argument (wall types 1 or 2, top_face 0..M)");
+    *out = nullptr;
+    chs_eng *e = new chs_eng();
+    const int L = c->L, M = c->M, N = c->N;
+    e->L = L; e->M = M; e->N = N;
+    e->hs = (size_t)N * (M + 2) * L;
+    const size_t cs = (size_t)N * M * L;
+    int rc = chs_prop_create(table, &e->prop);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    // layout: 10 halo fields (dh h t d m k cp d0 m0 dh0), 3 G, rhs0, wallflux [2][N][L],
+    // dyfi/dyci/wff/wfb (M + 2 each, reference index J)
+    const size_t nb = 13 * e->hs + cs + 2 * (size_t)N * L + 4 * (size_t)(M + 2);
+    hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipMalloc(&e->buf, sizeof(double) * nb);
+    if (he == hipSuccess) he = hipMalloc(&e->err, sizeof(int));
+    if (he == hipSuccess) he = hipMemsetAsync(e->buf, 0, sizeof(double) * nb, e->stream);
+    if (he == hipSuccess) he = hipMemsetAsync(e->err, 0, sizeof(int), e->stream);
+    std::vector<double> geo(4 * (size_t)(M + 2), 0.0);
+    for (int J = 1; J <= M; ++J) geo[J] = c->dyfi[J - 1];
+    for (int J = 1; J <= M + 1; ++J) {
+        geo[(M + 2) + J] = c->dyci[J - 1];
+        geo[2 * (M + 2) + J] = c->wff[J - 1];
+        geo[3 * (M + 2) + J] = c->wfb[J - 1];
+    }
+    double *f = e->buf;
+    double *wfd = f + 13 * e->hs + cs, *gd = wfd + 2 * (size_t)N * L;
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(wfd, p->wallflux, sizeof(double) * 2 * N * L, hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(gd, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) {
+        chs_eng_destroy(e);
+        return eng_fail(nullptr, CHS_E_HIP, std::string("chs_eng_create: ") + hipGetErrorString(he));
+    }
+    EngDev &d = e->e;
+    d.L = L; d.M = M; d.N = N;
+    d.dxi = c->dxi; d.dzi = c->dzi; d.dxqi = c->dxqi; d.dzqi = c->dzqi;
+    d.dyfi = gd; d.dyci = gd + (M + 2); d.wff = gd + 2 * (M + 2); d.wfb = gd + 3 * (M + 2);
+    double **fl[7] = {&d.dh, &d.h, &d.t, &d.d, &d.m, &d.k, &d.cp};
+    for (int a = 0; a < 7; ++a) *fl[a] = f + a * e->hs;
+    e->d0 = f + 7 * e->hs; e->m0 = f + 8 * e->hs; e->dh0 = f + 9 * e->hs;
+    d.g1 = f + 10 * e->hs; d.g2 = f + 11 * e->hs; d.g3 = f + 12 * e->hs;
+    d.rhs0 = f + 13 * e->hs;
+    d.wallflux = wfd;
+    d.wall_type0 = p->wall_type[0]; d.wall_type1 = p->wall_type[1];
+    d.top_face = p->top_face > 0 ? p->top_face : M;
+    d.cthecd = p->cthecd; d.dhmin = p->dh_min; d.dhmax = p->dh_max;
+    for (int w = 0; w < 2; ++w)
+        for (int j = 0; j < 5; ++j) d.ws[w][j] = p->wall_state[w][j];
+    for (int s = 0; s < 4; ++s) { e->tgam[s] = c->tgam[s]; e->troh[s] = c->troh[s]; }
+    *out = e;
+    return CHS_OK;
+}
+
+void chs_eng_destroy(chs_eng *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->buf) (void)hipFree(e->buf);
+    if (e->err) (void)hipFree(e->err);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->prop) chs_prop_destroy(e->prop);
+    delete e;
+}
+
+const char *chs_eng_last_error(const chs_eng *e) { return e ? e->errmsg.c_str() : g_prop_err.c_str(); }
+
+int chs_eng_upload(chs_eng *e, const double *dh, const double *g, const double *rhs0, int init_props)
+{
+    if (!e || !dh || !g) return eng_fail(e, CHS_E_ARG, "chs_eng_upload: null argument");
+    const size_t hsb = sizeof(double) * e->hs, cs = (size_t)e->N * e->M * e->L;
+    hipError_t he = hipMemcpyAsync(e->e.dh, dh, hsb, hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync((double *)e->e.g1, g, 3 * hsb, hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess)
+        he = rhs0 ? hipMemcpyAsync(e->e.rhs0, rhs0, sizeof(double) * cs, hipMemcpyHostToDevice, e->stream)
+                  : hipMemsetAsync(e->e.rhs0, 0, sizeof(double) * cs, e->stream);
+    if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string("chs_eng_upload: ") + hipGetErrorString(he));
+    if (init_props) eng_props_and_walls(e, 0, e->M + 1);
+    return eng_check(e, "chs_eng_upload");
+}
+
+int chs_eng_set_mass_flux(chs_eng *e, const double *g_dev)
+{
+    if (!e || !g_dev) return eng_fail(e, CHS_E_ARG, "chs_eng_set_mass_flux: null argument");
+    hipError_t he = hipMemcpyAsync((double *)e->e.g1, g_dev, 3 * sizeof(double) * e->hs, hipMemcpyDeviceToDevice,
+                                   e->stream);
+    if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string("chs_eng_set_mass_flux: ") + hipGetErrorString(he));
+    return CHS_OK;
+}
+
+int chs_substep_eng(chs_eng *e, int ns, double dt)
+{
+    if (!e || ns < 1 || ns > 3 || !(dt > 0)) return eng_fail(e, CHS_E_ARG, "chs_substep_eng: bad argument");
+    const size_t hsb = sizeof(double) * e->hs;
+    // Viscousity0, DENSITY0, DH0 (SOLVERRK3_ENG_IO.f90:23-27), for the momentum substep
+    hipError_t he = hipMemcpyAsync(e->m0, e->e.m, hsb, hipMemcpyDeviceToDevice, e->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(e->d0, e->e.d, hsb, hipMemcpyDeviceToDevice, e->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(e->dh0, e->e.dh, hsb, hipMemcpyDeviceToDevice, e->stream);
+    if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string("chs_substep_eng: ") + hipGetErrorString(he));
+    const dim3 blk(64, 4), grd((e->L + 63) / 64, (e->M + 3) / 4, e->N);
+    hipLaunchKernelGGL(k_eng_rhs_update, grd, blk, 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt);
+    eng_props_and_walls(e, 1, e->M);
+    return eng_check(e, "chs_substep_eng");
+}
+
+int chs_eng_download(chs_eng *e, double *dh, double *h, double *t, double *d, double *m, double *k, double *cp,
+                     double *rhs0)
+{
+    if (!e) return eng_fail(e, CHS_E_ARG, "chs_eng_download: null engine");
+    const size_t hsb = sizeof(double) * e->hs, cs = (size_t)e->N * e->M * e->L;
+    double *dst[7] = {dh, h, t, d, m, k, cp};
+    const double *src[7] = {e->e.dh, e->e.h, e->e.t, e->e.d, e->e.m, e->e.k, e->e.cp};
+    hipError_t he = hipSuccess;
+    for (int a = 0; a < 7 && he == hipSuccess; ++a)
+        if (dst[a]) he = hipMemcpyAsync(dst[a], src[a], hsb, hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess && rhs0)
+        he = hipMemcpyAsync(rhs0, e->e.rhs0, sizeof(double) * cs, hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string("chs_eng_download: ") + hipGetErrorString(he));
+    return CHS_OK;
+}
 
 int chs_prop_table_setup(int n, const double *rows, double T0, double *list, double *spl, int *idx,
                          double *ref)

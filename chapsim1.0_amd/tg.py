@@ -68,6 +68,13 @@ SIGNATURES = {
     "chs_prop_destroy": (None, [C.c_void_p]),
     "chs_prop_last_error": (C.c_char_p, [C.c_void_p]),
     "chs_prop_update_from_dh": (C.c_int, [C.c_void_p, C.c_long, _dp, _dp]),
+    "chs_eng_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "chs_eng_destroy": (None, [C.c_void_p]),
+    "chs_eng_last_error": (C.c_char_p, [C.c_void_p]),
+    "chs_eng_upload": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int]),
+    "chs_eng_set_mass_flux": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "chs_substep_eng": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "chs_eng_download": (C.c_int, [C.c_void_p] + [_dp] * 8),
 }
 
 

@@ -4,7 +4,7 @@
 format (Property_Table_Nondimensionalization, thermo_properties_table_searching.f90:1247-1264:
 a header token, the row count, then rows ``P H T D M K CP BETA``, SI units), and
 ``PropTable`` wraps the GPU lookup of THERM_PROP_UPDATE_FROM_DH (chs_prop_* in
-include/chapsim_hip.h).
+include/chapsim_hip.h) and ``EnergyTG`` the energy substep SOLVERRK3_ENG_io (chs_eng_*).
 """
 from __future__ import annotations
 
@@ -13,7 +13,7 @@ import math
 
 import numpy as np
 
-from .tg import _check, _dp, _p, lib, ChsError
+from .tg import _check, _coeff_struct, _dp, _p, lib, ChsError
 
 
 def synthetic_table(n=2048, T_lo=600.0, T_hi=720.0, T_pc=660.0, cp_base=5.0e3, peak=8.0,
@@ -130,6 +130,101 @@ class PropTable:
         if getattr(self, "_h", None):
             lib().chs_prop_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _EngParams(C.Structure):
+    _fields_ = [("wall_type", C.c_int * 2), ("cthecd", C.c_double), ("dh_min", C.c_double),
+                ("dh_max", C.c_double), ("wallflux", _dp), ("wall_state", (C.c_double * 5) * 2),
+                ("top_face", C.c_int)]
+
+
+ENG_FIELDS = ("DH", "H", "T", "D", "M", "K", "CP")
+
+
+class EnergyTG:
+    """Thermal state of one heated periodic channel on the GPU; ``SOLVERRK3_ENG_io(ns, dt)`` is
+    the reference's energy substep (SOLVERRK3_ENG_IO.f90:11-82) with table properties.
+
+    coeffs: ``channel_setup`` dict; t: nondimensional table (``table_setup``); wall_type
+    (bottom, top): 1 fixed heat flux, 2 fixed temperature (modules.f90:526-527); cthecd =
+    1/(REN PRT0); dhmin/dhmax: DHmin/DHmax; wallflux: (bottom, top) scalars or a [2, N, L]
+    array; wall_state: [2][T, H, D, M, K] of fixed-temperature walls; top_face: reference J
+    of the top wall's ghost-row weights (M on one rank, the last rank's N2DO on several).
+    Fields are C-order [N][M+2][L]."""
+
+    def __init__(self, coeffs: dict, t: dict, wall_type, cthecd, dhmin, dhmax, wallflux,
+                 wall_state=None, top_face=None):
+        L, M, N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
+        self.L, self.M, self.N = L, M, N
+        self._keep = []
+        self._c = _coeff_struct(coeffs, self._keep)
+        self._tab = PropTable(t)
+        wf = np.asarray(wallflux, dtype=np.float64)
+        if wf.shape == (2,):
+            wf = np.stack([np.full((N, L), wf[0]), np.full((N, L), wf[1])])
+        wf = np.ascontiguousarray(wf.reshape(2, N, L))
+        self._keep.append(wf)
+        ws = ((C.c_double * 5) * 2)()
+        if wall_state is not None:
+            for w in range(2):
+                for j in range(5):
+                    ws[w][j] = float(wall_state[w][j])
+        self._par = _EngParams((C.c_int * 2)(*[int(x) for x in wall_type]), float(cthecd), float(dhmin),
+                               float(dhmax), _p(wf), ws, 0 if top_face is None else int(top_face))
+        h = C.c_void_p()
+        rc = lib().chs_eng_create(C.byref(self._c), C.byref(self._tab._t), C.byref(self._par), C.byref(h))
+        if rc != 0:
+            raise ChsError(f"chs_eng_create failed ({rc}): {lib().chs_eng_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise ChsError(f"{what} failed ({rc}): {lib().chs_eng_last_error(self._h).decode()}")
+
+    def upload(self, dh, g, rhs0=None, init_props=True):
+        """DH [N, M+2, L], G [3, N, M+2, L], RHS_ENERGY0 [N, M, L] (None: zero); init_props runs
+        THERM_PROP_UPDATE on every row and BC_WALL_THERMAL, as the start-up does."""
+        hs = (self.N, self.M + 2, self.L)
+        dh = np.ascontiguousarray(dh, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        if dh.shape != hs or g.shape != (3,) + hs:
+            raise ChsError(f"upload: DH must be {hs} and G (3,) + {hs}")
+        r0 = None
+        if rhs0 is not None:
+            r0 = np.ascontiguousarray(rhs0, dtype=np.float64)
+            if r0.shape != (self.N, self.M, self.L):
+                raise ChsError("upload: RHS_ENERGY0 must be (N, M, L)")
+        self._check(lib().chs_eng_upload(self._h, _p(dh), _p(g), None if r0 is None else _p(r0),
+                                         1 if init_props else 0), "chs_eng_upload")
+
+    def set_mass_flux(self, g_dev_ptr: int):
+        self._check(lib().chs_eng_set_mass_flux(self._h, C.c_void_p(g_dev_ptr)), "chs_eng_set_mass_flux")
+
+    def SOLVERRK3_ENG_io(self, ns, dt):
+        self._check(lib().chs_substep_eng(self._h, int(ns), float(dt)), "chs_substep_eng")
+
+    def download(self):
+        """dict of DH, H, T, D, M, K, CP ([N, M+2, L]) and RHS0 ([N, M, L])."""
+        hs = (self.N, self.M + 2, self.L)
+        out = {k: np.zeros(hs) for k in ENG_FIELDS}
+        out["RHS0"] = np.zeros((self.N, self.M, self.L))
+        self._check(lib().chs_eng_download(self._h, *[_p(out[k]) for k in ENG_FIELDS + ("RHS0",)]),
+                    "chs_eng_download")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chs_eng_destroy(self._h)
+            self._h = None
+        if getattr(self, "_tab", None) is not None:
+            self._tab.close()
+            self._tab = None
 
     def __del__(self):
         try:

@@ -244,6 +244,57 @@ const char *chs_prop_last_error(const chs_prop *p);
  * if a value lies outside the table by more than 1e-8.  Host arrays; synchronous. */
 int chs_prop_update_from_dh(chs_prop *p, long n, const double *dh, double *out);
 
+/* ---- Energy equation of the heated channel (table properties) ------------------------- */
+
+/* Thermal wall set-up (BC_WALL.f90:484-643; constants from the reference's set-up):
+ *   wall_type[w]    1 = fixed heat flux, 2 = fixed temperature (modules.f90:526-527);
+ *                   w = 0 bottom (J = 0), 1 top (J = M+1)
+ *   cthecd          CTHECD = 1 / (REN PRT0)
+ *   dh_min, dh_max  DHmin, DHmax of the table (the update clamps DH 1e-14 inside them)
+ *   wallflux        WALLFLUX(1:NCL1, 1:NCL3, 1:2) nondimensional wall heat flux, [2][N][L]
+ *                   (I fastest; index 0 bottom, 1 top); read on iso-flux walls
+ *   wall_state[w]   fixed-temperature wall values T, H, D, M, K (T_WAL_GV, H_WAL_GV, D_WAL_GV,
+ *                   M_WAL_GV, K_WAL_GV)
+ *   top_face        reference J of the YCL2ND_WFF / WFB entry the top iso-flux wall reads:
+ *                   BC_WALL.f90:598-601 index them with the rank-local N2DO, so a one-rank
+ *                   run reads M and a P-rank run the last rank's N2DO; 0 means M */
+typedef struct {
+    int wall_type[2];
+    double cthecd, dh_min, dh_max;
+    const double *wallflux;
+    double wall_state[2][5];
+    int top_face;
+} chs_eng_params;
+
+typedef struct chs_eng chs_eng;
+
+/* Thermal state of one channel on the current GPU: DH, H, T, D, M, K, CP (halo layout
+ * (N, M+2, L)), the mass flux G (3 x halo), RHS_ENERGY0 (N, M, L) and the DENSITY0,
+ * Viscousity0, DH0 copies the momentum substep reads. */
+int chs_eng_create(const chs_coeffs *mesh, const chs_prop_table *table, const chs_eng_params *p, chs_eng **out);
+void chs_eng_destroy(chs_eng *e);
+const char *chs_eng_last_error(const chs_eng *e);
+
+/* Host arrays: dh (N, M+2, L), g[3] x (N, M+2, L), rhs0 (N, M, L) or NULL for zero.
+ * init_props != 0 then evaluates THERM_PROP_UPDATE on rows 0..M+1 and BC_WALL_THERMAL, as
+ * the reference's start-up does before the first substep. */
+int chs_eng_upload(chs_eng *e, const double *dh, const double *g, const double *rhs0, int init_props);
+
+/* Replace G by a device array of 3 halo fields (the momentum context's mass flux). */
+int chs_eng_set_mass_flux(chs_eng *e, const double *g_dev);
+
+/* SOLVERRK3_ENG_io(NS) (SOLVERRK3_ENG_IO.f90:11-82) for the periodic channel with table
+ * properties: DENSITY0, Viscousity0, DH0 <- current; RHS_ENERGY_EXPLICIT
+ * (RHS_ENERGY_EXPLICIT.f90:11-155); DH += TGAM(NS) dt RHS + TROH(NS) dt RHS0 with the table
+ * limiter (lines 34-48); THERM_PROP_UPDATE on J = 1..M; BC_WALL_THERMAL.  FMA contraction
+ * off, reference operation order: matches the reference to rounding (1e-12 relative).
+ * CHS_E_RANGE if DH leaves the table (the reference STOPs).  Synchronous. */
+int chs_substep_eng(chs_eng *e, int ns, double dt);
+
+/* Copy fields back (NULL skips one): halo arrays (N, M+2, L), rhs0 (N, M, L). */
+int chs_eng_download(chs_eng *e, double *dh, double *h, double *t, double *d, double *m, double *k, double *cp,
+                     double *rhs0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,14 @@ def lib():
             fn = getattr(_lib, name)
             fn.argtypes = args
             fn.restype = None
+        for name, args in (("or_prop_from_dh", [C.c_void_p, C.c_double, _dp]),
+                           ("or_eng_init", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                           ("or_eng_walls", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+                           ("or_eng_substep", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                               C.c_double])):
+            fn = getattr(_lib, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
     return _lib
 
 
@@ -183,3 +191,97 @@ def initial_field(mesh: _setup.Mesh, seed: int, amp: float = 0.1):
     q[2, :, 1:M + 1, :] = pert[2]
     pr = np.zeros((N, M + 2, L))
     return q, pr
+
+
+# ---- Energy equation of the main domain (or_eng_* in tg_oracle.c) -----------------------
+
+class _Table(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("n", "imin_dh", "imax_dh", "imin_h", "imax_h")] + \
+               [(k, _dp) for k in ("dh", "h", "t", "d", "m", "k", "cp", "b", "spl")]
+
+
+class _EngPar(C.Structure):
+    _fields_ = [("wall_type", C.c_int * 2), ("cthecd", C.c_double), ("dhmin", C.c_double),
+                ("dhmax", C.c_double), ("wallflux", _dp), ("wall_state", (C.c_double * 5) * 2),
+                ("top_face", C.c_int)]
+
+
+class _Eng(C.Structure):
+    _fields_ = [(k, _dp) for k in ("dh", "h", "t", "d", "m", "k", "cp", "g", "rhs0", "d0", "m0", "dh0")]
+
+
+SPLINES = ("DHH", "HT", "HD", "HM", "HK", "HCp", "HB")
+
+
+def eng_initial_field(mesh, t, seed):
+    """Seeded main-domain state: T/T0 = 1 + 0.01 (smooth + noise) mapped to DH through the
+    nondimensional table (monotone in T), Poiseuille-like G1 and small G2, G3 (G2 = 0 on the
+    walls).  DH [N, M+2, L], G [3, N, M+2, L]."""
+    L, M, N = mesh.L, mesh.M, mesh.N
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    y = np.concatenate([[-1.0], np.asarray(mesh.YCC), [1.0]])
+    x = np.arange(L) / L
+    z = np.arange(N) / N
+    T = 1.0 + 0.01 * (np.sin(2 * np.pi * x)[None, None, :] * np.cos(2 * np.pi * z)[:, None, None]
+                      + 0.5 * y[None, :, None]) + 0.002 * rng.standard_normal((N, M + 2, L))
+    dh = np.interp(T, t["T"], t["DH"])
+    g = np.zeros((3, N, M + 2, L))
+    g[0] = 1.5 * (1.0 - y[None, :, None] ** 2) * (1.0 + 0.05 * rng.standard_normal((N, M + 2, L)))
+    g[1] = 0.02 * rng.standard_normal((N, M + 2, L))
+    g[1][:, 1, :] = 0.0
+    g[1][:, M + 1, :] = 0.0
+    g[2] = 0.03 * rng.standard_normal((N, M + 2, L))
+    return dh, g
+
+
+class OracleEng:
+    """SOLVERRK3_ENG_io restated (or_eng_substep): table t (dict of the nondimensional table as
+    chapsim_amd.thermo.table_setup returns it), wall types (bottom, top; 1 flux, 2 fixed T),
+    CTHECD, DHmin/DHmax, wall fluxes (bottom, top), fixed-T wall states [2][T, H, D, M, K] and
+    the face index of the top ghost-row weights.  Fields are C-order [N][M+2][L] halo arrays."""
+
+    def __init__(self, mesh: _setup.Mesh, t: dict, wall_type, cthecd, dhmin, dhmax, wallflux,
+                 wall_state=None, top_face=None):
+        L, M, N = mesh.L, mesh.M, mesh.N
+        self.mesh = mesh
+        self._keep = []
+
+        def arr(x):
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            self._keep.append(a)
+            return _p(a)
+
+        self._om = OracleTG(mesh, 1.0)._m   # the mesh struct (RK coefficients, weights)
+        self._keep.append(self._om)
+        n = int(t["N"])
+        spl = np.stack([t[f"{s}_{c}"] for s in SPLINES for c in "BCD"])
+        self._t = _Table(n, int(t["IMIN_DH"]), int(t["IMAX_DH"]), int(t["IMIN_H"]), int(t["IMAX_H"]),
+                         *[arr(t[k]) for k in ("DH", "H", "T", "D", "M", "K", "CP", "B")], arr(spl))
+        wf = np.stack([np.full((N, L), float(wallflux[0])), np.full((N, L), float(wallflux[1]))])
+        ws = ((C.c_double * 5) * 2)()
+        if wall_state is not None:
+            for w in range(2):
+                for j in range(5):
+                    ws[w][j] = float(wall_state[w][j])
+        self._p = _EngPar((C.c_int * 2)(*wall_type), cthecd, dhmin, dhmax, arr(wf), ws,
+                          M if top_face is None else int(top_face))
+        hs = (N, M + 2, L)
+        self.f = {k: np.zeros(hs) for k in ("dh", "h", "t", "d", "m", "k", "cp", "d0", "m0", "dh0")}
+        self.f["g"] = np.zeros((3,) + hs)
+        self.f["rhs0"] = np.zeros((N, M, L))
+        self._e = _Eng(*[_p(self.f[k]) for k in ("dh", "h", "t", "d", "m", "k", "cp", "g", "rhs0", "d0",
+                                                  "m0", "dh0")])
+
+    def init(self, dh, g):
+        self.f["dh"][...] = dh
+        self.f["g"][...] = g
+        return lib().or_eng_init(C.byref(self._om), C.byref(self._t), C.byref(self._p), C.byref(self._e))
+
+    def substep(self, ns, dt):
+        return lib().or_eng_substep(C.byref(self._om), C.byref(self._t), C.byref(self._p), C.byref(self._e),
+                                    int(ns), float(dt))
+
+    def prop_from_dh(self, dh):
+        out = np.zeros(7)
+        err = lib().or_prop_from_dh(C.byref(self._t), float(dh), _p(out))
+        return out, err

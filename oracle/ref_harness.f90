@@ -21,6 +21,9 @@
 !                           does), dumps Q_tg rows 0..N2DO+1 and RHS_tg rows 1..N2DO
 !   thermoinit              rank 0: thermal_init (table read, sort, nondimensionalisation,
 !                           splines; Flow_State.f90:16-95), needs iThermoDynamics = 1
+!   thermoinitall           thermal_init on rank 0 and BCAST_THERMAL_INIT (all ranks)
+!   dumpengpar FILE         rank 0: CTHECD, DHmin, DHmax, WALLFLUX(1, bottom/top, 1), the wall
+!                           states T, H, D, M, K (bottom, top) and N2DO of the last rank
 !   dumptable FILE          rank 0: nondimensional table and spline coefficients
 !   proplookup FIN FOUT     rank 0: THERM_PROP_UPDATE_FROM_DH for every DH in FIN (int32 n,
 !                           then n float64); FOUT = H, T, D, M, K, Cp, B (7 x n float64)
@@ -29,6 +32,11 @@
 !                           dumps the solution rows 1..N2DO
 !   poissonio_call FIN FOUT same, with DPH_io as the output argument as the IO substep
 !                           passes it; dumps DPH_io rows 1..N2DO
+!   loadeng FILE            (main domain, iThermoDynamics = 1) DH and G_io from FILE, see
+!                           H_LOADENG
+!   engsubstep NS DT        DT = given, CALL SOLVERRK3_ENG_io(NS)
+!   dumpeng FILE            each rank writes FILE.<rank>: DH, H, T, D, M, K, Cp (x 1..L, rows
+!                           0..N2DO+1), then RHS_ENERGY0 rows 1..N2DO
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -132,6 +140,29 @@ PROGRAM ref_harness
             CALL H_DUMP3(TRIM(fname) // '.rhs', RHS_tg, 1)
         CASE ('thermoinit')
             IF (MYID == 0) CALL thermal_init
+        CASE ('thermoinitall')
+            IF (MYID == 0) CALL thermal_init
+            CALL BCAST_THERMAL_INIT
+        CASE ('dumpengpar')
+            READ(line, *) cmd, fname
+            IF (MYID == 0) THEN
+                OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+                     STATUS = 'replace', ACTION = 'write')
+                WRITE(80) CTHECD, DHmin, DHmax
+                IF (ALLOCATED(WALLFLUX)) THEN
+                    WRITE(80) WALLFLUX(1, iBotWall, 1), WALLFLUX(1, iTopWall, 1)
+                ELSE
+                    WRITE(80) 0.0_WP, 0.0_WP
+                END IF
+                IF (ALLOCATED(T_WAL_GV)) THEN
+                    WRITE(80) T_WAL_GV(1, 1:2), H_WAL_GV(1, 1:2), D_WAL_GV(1, 1:2), M_WAL_GV(1, 1:2), &
+                              K_WAL_GV(1, 1:2)
+                ELSE
+                    WRITE(80) (0.0_WP, idr = 1, 10)
+                END IF
+                WRITE(80) REAL(N2DO(NPSLV), WP)
+                CLOSE(80)
+            END IF
         CASE ('dumptable')
             READ(line, *) cmd, fname
             IF (MYID == 0) CALL H_DUMPTABLE(TRIM(fname))
@@ -157,6 +188,21 @@ PROGRAM ref_harness
             CALL FISHPACK_POIS3D_SIMPLE(RHSLLPHI_io, DPH_io)
             CALL H_OPENRANK(TRIM(fname), 80)
             WRITE(80) DPH_io(1:NCL1_io, 1:N2DO(MYID), :)
+            CLOSE(80)
+        CASE ('loadeng')
+            READ(line, *) cmd, fname
+            CALL H_LOADENG(TRIM(fname))
+        CASE ('engsubstep')
+            READ(line, *) cmd, ns, dtin
+            DT = dtin
+            CALL SOLVERRK3_ENG_io(ns)
+        CASE ('dumpeng')
+            READ(line, *) cmd, fname
+            CALL H_OPENRANK(TRIM(fname), 80)
+            WRITE(80) DH(1:NCL1_io, :, :), ENTHALPY(1:NCL1_io, :, :), TEMPERATURE(1:NCL1_io, :, :), &
+                      DENSITY(1:NCL1_io, :, :), Viscousity(1:NCL1_io, :, :), THERMCONDT(1:NCL1_io, :, :), &
+                      HEATCAP(1:NCL1_io, :, :)
+            WRITE(80) RHS_ENERGY0(:, 1:N2DO(MYID), :)
             CLOSE(80)
         CASE ('tdma')
             READ(line, *) cmd, idr, fname2, fname
@@ -279,6 +325,37 @@ CONTAINS
             JG = JDSWT(MYID) - 1 + J
             RHSLLPHI_io(:, J, :) = G(:, JG, :)
         END DO
+        DEALLOCATE(G)
+    END SUBROUTINE
+    ! Energy-equation state from one file: global DH (L x (M+2) x N, y rows 0..M+1) then G_io
+    ! (same, three components).  Every rank keeps its y-slab with halo rows; properties of rows
+    ! 0..N2DO+1 follow from DH (THERM_PROP_UPDATE_FROM_DH), then the walls are set as the
+    ! substep sets them (BC_WALL_THERMAL(IMAIND)); RHS_ENERGY0 = 0.
+    SUBROUTINE H_LOADENG(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: G(:, :, :, :)
+        INTEGER(4) :: I, J, K, JG
+        REAL(WP) :: B_tmp
+        ALLOCATE(G(NCL1_io, 0:NCL2 + 1, NCL3, 4))
+        OPEN(79, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', &
+             ACTION = 'read')
+        READ(79) G
+        CLOSE(79)
+        DO J = 0, N2DO(MYID) + 1
+            JG = JDSWT(MYID) - 1 + J
+            DH(1:NCL1_io, J, :) = G(:, JG, :, 1)
+            G_io(1:NCL1_io, J, :, 1) = G(:, JG, :, 2)
+            G_io(1:NCL1_io, J, :, 2) = G(:, JG, :, 3)
+            G_io(1:NCL1_io, J, :, 3) = G(:, JG, :, 4)
+            DO K = 1, NCL3
+                DO I = 1, NCL1_io
+                    CALL THERM_PROP_UPDATE_FROM_DH(DH(I, J, K), ENTHALPY(I, J, K), TEMPERATURE(I, J, K), &
+                         DENSITY(I, J, K), Viscousity(I, J, K), THERMCONDT(I, J, K), HEATCAP(I, J, K), B_tmp)
+                END DO
+            END DO
+        END DO
+        CALL BC_WALL_THERMAL(IMAIND)
+        RHS_ENERGY0 = 0.0_WP
         DEALLOCATE(G)
     END SUBROUTINE
     SUBROUTINE H_DUMP3H(fn, A)

@@ -186,6 +186,37 @@ class Harness:
         a = _gather(self.path(name), self.nranks, 4, halo=True)
         return a[0], a[1], a[2], a[3]
 
+    def read_eng(self, name):
+        """Energy state written by ``dumpeng``: dict of DH, H, T, D, M, K, CP as [N, M+2, L]
+        halo fields and RHS0 as [N, M, L] cells."""
+        parts = []
+        for r in range(self.nranks):
+            with open(self.path(f"{name}.{r}"), "rb") as fh:
+                hdr = np.frombuffer(fh.read(24), dtype=np.int32)
+                data = np.frombuffer(fh.read(), dtype=np.float64)
+            myid, j0, n2, L, M, N = (int(v) for v in hdr)
+            nh = N * (n2 + 2) * L
+            parts.append((j0, n2, data[:7 * nh].reshape(7, N, n2 + 2, L), data[7 * nh:].reshape(N, n2, L)))
+        M = sum(p[1] for p in parts)
+        N, L = parts[0][2].shape[1], parts[0][2].shape[3]
+        h = np.zeros((7, N, M + 2, L))
+        c = np.zeros((N, M, L))
+        for j0, n2, hv, cv in parts:
+            h[:, :, j0:j0 + n2, :] = hv[:, :, 1:n2 + 1, :]
+            if j0 == 1:
+                h[:, :, 0, :] = hv[:, :, 0, :]
+            if j0 + n2 - 1 == M:
+                h[:, :, M + 1, :] = hv[:, :, n2 + 1, :]
+            c[:, j0 - 1:j0 - 1 + n2, :] = cv
+        out = {k: h[i] for i, k in enumerate(("DH", "H", "T", "D", "M", "K", "CP"))}
+        out["RHS0"] = c
+        return out
+
+    def write_eng(self, name, dh, g):
+        """DH [N, M+2, L] and G [3, N, M+2, L] in the layout ``loadeng`` reads."""
+        np.concatenate([np.asarray(dh, dtype=np.float64)[None], np.asarray(g, dtype=np.float64)]).tofile(
+            self.path(name))
+
     def read_table(self, name):
         """Nondimensional property table + spline coefficients written by ``dumptable``."""
         with open(self.path(name), "rb") as fh:

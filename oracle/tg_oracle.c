@@ -540,3 +540,204 @@ void or_substep(const or_mesh *m, or_state *s, int ns, double dt, double cvisc)
     or_prcalc(m, s, ns, dt, cvisc);
     or_bc_copy(m, s->pr);
 }
+
+/* ======================================================================================
+ * Energy equation of the main domain (table mode): SOLVERRK3_ENG_io (SOLVERRK3_ENG_IO.f90:11-82)
+ * with RHS_ENERGY_EXPLICIT (RHS_ENERGY_EXPLICIT.f90:11-155), THERM_PROP_UPDATE(IMAIND)
+ * (THERM_PROP_UPDATE.f90:124-168) and BC_WALL_THERMAL(IMAIND) (BC_WALL.f90:484-643), channel
+ * (RCCI1 = RNDI1 = RCCI2 = 1, COORDJR.f90:39-42), periodic x and z, one y-domain.
+ * ====================================================================================== */
+
+typedef struct {
+    int n, imin_dh, imax_dh, imin_h, imax_h;   /* 1-based IMIN_/IMAX_ of the reference */
+    const double *dh, *h, *t, *d, *m, *k, *cp, *b;
+    const double *spl;   /* [7][3][n]: DHH, HT, HD, HM, HK, HCp, HB x (B, C, D) */
+} or_table;
+
+/* spline_interpolation_* (thermo_properties_table_searching.f90:678-724 and siblings): range
+ * check with tolerance 1e-8 (the reference STOPs; here bit 1 of the result), bisection
+ * I = 1, J = N + 1, Horner Y(I) + dx (B(I) + dx (C(I) + dx D(I))). */
+static double or_spline(int n, const double *x, const double *y, const double *sb, const double *sc,
+                        const double *sd, int imin, int imax, double xv, int *err)
+{
+    if ((xv - x[imin - 1]) < -1.0e-8 || (xv - x[imax - 1]) > 1.0e-8) *err = 1;
+    int I = 1, J = n + 1;
+    while (J > I + 1) {
+        const int K = (I + J) / 2;
+        if (xv < x[K - 1]) J = K;
+        else I = K;
+    }
+    const double dx = xv - x[I - 1];
+    return y[I - 1] + dx * (sb[I - 1] + dx * (sc[I - 1] + dx * sd[I - 1]));
+}
+
+/* THERM_PROP_UPDATE_FROM_DH, table branch (THERM_PROP_UPDATE.f90:45-52): out = H, T, D, M, K,
+ * Cp, B with D from the H->M spline, M from H->K and K from H->D (lines 49-51).  Returns 1 if
+ * a lookup left the table range. */
+int or_prop_from_dh(const or_table *t, double dh, double *out)
+{
+    int err = 0;
+    const int n = t->n;
+#define SP(s, c) (t->spl + ((size_t)(s) * 3 + (c)) * n)
+    const double h = or_spline(n, t->dh, t->h, SP(0, 0), SP(0, 1), SP(0, 2), t->imin_dh, t->imax_dh, dh, &err);
+    out[0] = h;
+    out[1] = or_spline(n, t->h, t->t, SP(1, 0), SP(1, 1), SP(1, 2), t->imin_h, t->imax_h, h, &err);
+    out[2] = or_spline(n, t->h, t->m, SP(3, 0), SP(3, 1), SP(3, 2), t->imin_h, t->imax_h, h, &err);
+    out[3] = or_spline(n, t->h, t->k, SP(4, 0), SP(4, 1), SP(4, 2), t->imin_h, t->imax_h, h, &err);
+    out[4] = or_spline(n, t->h, t->d, SP(2, 0), SP(2, 1), SP(2, 2), t->imin_h, t->imax_h, h, &err);
+    out[5] = or_spline(n, t->h, t->cp, SP(5, 0), SP(5, 1), SP(5, 2), t->imin_h, t->imax_h, h, &err);
+    out[6] = or_spline(n, t->h, t->b, SP(6, 0), SP(6, 1), SP(6, 2), t->imin_h, t->imax_h, h, &err);
+#undef SP
+    return err;
+}
+
+typedef struct {
+    int wall_type[2];        /* bottom, top: 1 fixed heat flux, 2 fixed temperature */
+    double cthecd;           /* CTHECD = 1 / (REN PRT0) (Flow_State.f90:79) */
+    double dhmin, dhmax;     /* DHmin, DHmax of the table (clamp, SOLVERRK3_ENG_IO.f90:39-42) */
+    const double *wallflux;  /* [2][N][L] WALLFLUX(I, iBotWall | iTopWall, K) */
+    double wall_state[2][5]; /* fixed-temperature walls: T, H, D, M, K (T_WAL_GV, ...) */
+    int top_face;            /* face index of the top ghost-row weights (the reference uses the
+                                local row count of the last rank; M on one y-domain) */
+} or_eng_par;
+
+typedef struct {
+    double *dh, *h, *t, *d, *m, *k, *cp;   /* [N][M+2][L] */
+    double *g;                             /* [3][N][M+2][L] G_io */
+    double *rhs0;                          /* [N][M][L] RHS_ENERGY0 */
+    double *d0, *m0, *dh0;                 /* [N][M+2][L] DENSITY0, Viscousity0, DH0 */
+} or_eng;
+
+static int or_eng_cell_props(const or_mesh *m, const or_table *tb, or_eng *e, int i, int J, int k)
+{
+    double o[7];
+    const int err = or_prop_from_dh(tb, HX(e->dh, i, J, k), o);
+    HX(e->h, i, J, k) = o[0];
+    HX(e->t, i, J, k) = o[1];
+    HX(e->d, i, J, k) = o[2];
+    HX(e->m, i, J, k) = o[3];
+    HX(e->k, i, J, k) = o[4];
+    HX(e->cp, i, J, k) = o[5];
+    return err;
+}
+
+/* BC_WALL_THERMAL(IMAIND): iso-flux walls extrapolate DH to the ghost row and look the
+ * properties up there (BC_WALL_Isoflux, BC_WALL.f90:566-643; not BETA); iso-thermal walls take
+ * the wall state (BC_WALL_IsoTHERMAL, :519-562). */
+int or_eng_walls(const or_mesh *m, const or_table *tb, const or_eng_par *p, or_eng *e)
+{
+    int err = 0;
+    for (int w = 0; w < 2; ++w) {
+        const int J = w == 0 ? 0 : MM + 1;
+        for (int k = 0; k < NN; ++k)
+            for (int i = 0; i < LL; ++i) {
+                if (p->wall_type[w] == 1) {
+                    if (w == 0)
+                        HX(e->dh, i, 0, k) = 2.0 * HX(e->dh, i, 1, k) - m->wff[2] * HX(e->dh, i, 2, k) -
+                                             m->wfb[2] * HX(e->dh, i, 1, k);
+                    else
+                        HX(e->dh, i, J, k) = 2.0 * HX(e->dh, i, MM, k) - m->wff[p->top_face] * HX(e->dh, i, MM, k) -
+                                             m->wfb[p->top_face] * HX(e->dh, i, MM - 1, k);
+                    err |= or_eng_cell_props(m, tb, e, i, J, k);
+                } else {
+                    const double *ws = p->wall_state[w];
+                    HX(e->t, i, J, k) = ws[0];
+                    HX(e->h, i, J, k) = ws[1];
+                    HX(e->d, i, J, k) = ws[2];
+                    HX(e->m, i, J, k) = ws[3];
+                    HX(e->k, i, J, k) = ws[4];
+                    HX(e->dh, i, J, k) = HX(e->h, i, J, k) * HX(e->d, i, J, k);
+                }
+            }
+    }
+    return err;
+}
+
+/* State from DH as the harness loads it: properties of every row 0..M+1 from DH, then the
+ * walls, RHS_ENERGY0 = 0. */
+int or_eng_init(const or_mesh *m, const or_table *tb, const or_eng_par *p, or_eng *e)
+{
+    int err = 0;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 0; J <= MM + 1; ++J)
+            for (int i = 0; i < LL; ++i) err |= or_eng_cell_props(m, tb, e, i, J, k);
+    err |= or_eng_walls(m, tb, p, e);
+    memset(e->rhs0, 0, sizeof(double) * (size_t)LL * MM * NN);
+    return err;
+}
+
+int or_eng_substep(const or_mesh *m, const or_table *tb, const or_eng_par *p, or_eng *e, int ns, double dt)
+{
+    const size_t hs = (size_t)NN * (MM + 2) * LL;
+    memcpy(e->m0, e->m, sizeof(double) * hs);
+    memcpy(e->d0, e->d, sizeof(double) * hs);
+    memcpy(e->dh0, e->dh, sizeof(double) * hs);
+    const double *g1 = e->g, *g2 = e->g + hs, *g3 = e->g + 2 * hs;
+    const double xnd2cl = 0.5, znd2cl = 0.5, one = 1.0;
+    const double coe1 = m->dxi * xnd2cl * 0.5;
+    const double coe11 = m->dxqi * xnd2cl * p->cthecd;
+    const double ca = m->tgam[ns] * dt, cb = m->troh[ns] * dt;
+    const double realmin = 1.0e-14;
+    for (int J = 1; J <= MM; ++J) {
+        const double coe2 = m->dyfi[J - 1] * one * 0.5;
+        const double coe3 = m->dzi * znd2cl * one * 0.5;
+        const double coe33 = m->dzqi * znd2cl * one * p->cthecd;
+        const double coe22 = m->dyfi[J - 1] * one * p->cthecd;
+        double coe221, coe222;
+        if (J == 1) {
+            coe221 = m->dyci[J + 1] / one;
+            coe222 = p->wall_type[0] == 1 ? 1.0 / one : m->dyci[J] / one;
+        } else if (J == MM) {
+            coe222 = m->dyci[J] / one;
+            coe221 = p->wall_type[1] == 1 ? 1.0 / one : m->dyci[J + 1] / one;
+        } else {
+            coe221 = m->dyci[J + 1] / one;
+            coe222 = m->dyci[J] / one;
+        }
+        const double wffp = m->wff[J + 1], wfbp = m->wfb[J + 1], wff = m->wff[J], wfb = m->wfb[J];
+        for (int i = 0; i < LL; ++i) {
+            const int ip = ip_(m, i), im = im_(m, i);
+            for (int k = 0; k < NN; ++k) {
+                const int kp = kp_(m, k), km = km_(m, k);
+                const double *H = e->h, *T = e->t, *K = e->k;
+                double dx_dhu = (HX(H, ip, J, k) + HX(H, i, J, k)) * (HX(g1, ip, J, k) + HX(g1, ip, J, k)) -
+                                (HX(H, i, J, k) + HX(H, im, J, k)) * (HX(g1, i, J, k) + HX(g1, i, J, k));
+                dx_dhu = dx_dhu * coe1;
+                double dy_dhv = (wffp * HX(H, i, J + 1, k) + wfbp * HX(H, i, J, k)) *
+                                    (HX(g2, i, J + 1, k) + HX(g2, i, J + 1, k)) -
+                                (wff * HX(H, i, J, k) + wfb * HX(H, i, J - 1, k)) * (HX(g2, i, J, k) + HX(g2, i, J, k));
+                dy_dhv = dy_dhv * coe2;
+                double dz_dhw = (HX(H, i, J, kp) + HX(H, i, J, k)) * (HX(g3, i, J, kp) + HX(g3, i, J, kp)) -
+                                (HX(H, i, J, k) + HX(H, i, J, km)) * (HX(g3, i, J, k) + HX(g3, i, J, k));
+                dz_dhw = dz_dhw * coe3;
+                double dx_kdt = (HX(K, ip, J, k) + HX(K, i, J, k)) * (HX(T, ip, J, k) - HX(T, i, J, k)) -
+                                (HX(K, i, J, k) + HX(K, im, J, k)) * (HX(T, i, J, k) - HX(T, im, J, k));
+                dx_kdt = dx_kdt * coe11;
+                double kdty1 = (wffp * HX(K, i, J + 1, k) + wfbp * HX(K, i, J, k)) *
+                               (HX(T, i, J + 1, k) - HX(T, i, J, k)) * coe221;
+                double kdty2 = (wff * HX(K, i, J, k) + wfb * HX(K, i, J - 1, k)) *
+                               (HX(T, i, J, k) - HX(T, i, J - 1, k)) * coe222;
+                if (J == 1 && p->wall_type[0] == 1)
+                    kdty2 = -p->wallflux[(size_t)k * LL + i] * coe222;
+                else if (J == MM && p->wall_type[1] == 1)
+                    kdty1 = -p->wallflux[((size_t)NN + k) * LL + i] * coe221;
+                const double dy_kdt = (kdty1 - kdty2) * coe22;
+                double dz_kdt = (HX(K, i, J, kp) + HX(K, i, J, k)) * (HX(T, i, J, kp) - HX(T, i, J, k)) -
+                                (HX(K, i, J, km) + HX(K, i, J, k)) * (HX(T, i, J, k) - HX(T, i, J, km));
+                dz_kdt = dz_kdt * coe33;
+                const double r = -dx_dhu - dy_dhv - dz_dhw + dx_kdt + dy_kdt + dz_kdt;
+                double dh = HX(e->dh, i, J, k) + ca * r + cb * CX(e->rhs0, i, J, k);
+                dh = fmin(p->dhmax - realmin, dh);
+                dh = fmax(p->dhmin + realmin, dh);
+                HX(e->dh, i, J, k) = dh;
+                CX(e->rhs0, i, J, k) = r;
+            }
+        }
+    }
+    int err = 0;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) err |= or_eng_cell_props(m, tb, e, i, J, k);
+    err |= or_eng_walls(m, tb, p, e);
+    return err;
+}

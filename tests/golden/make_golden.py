@@ -12,9 +12,9 @@ Arrays are C-order views of the reference's Fortran layout (z, y, x).
 
 Also ``props_2048.npz`` (property tables, make_props), ``tdma.npz`` (the reference's
 batched Thomas solvers, make_tdma) and ``fishpack.npz`` (the main-domain FISHPACK Poisson
-solve, make_fishpack).
+solve, make_fishpack) and ``eng.npz`` (the energy substep, make_eng).
 
-Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack]   (needs oracle/_ref)
+Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng]   (needs oracle/_ref)
 """
 from __future__ import annotations
 
@@ -168,9 +168,54 @@ def make_fishpack(seed=41):
     print("wrote", path, os.path.getsize(path))
 
 
+ENG_CASES = [  # L, M, N, STR2, REN, dt, wall types, thermalWallBC_Dim, seed
+    (16, 12, 8, 1.8, 2800.0, 1.0e-3, (1, 1), (2.0e4, 2.0e4), 51),
+    (12, 10, 16, 2.0, 2800.0, 5.0e-4, (2, 2), (655.0, 675.0), 52),
+]
+
+
+def make_eng(T0=665.0):
+    """Energy-equation fixture (SOLVERRK3_ENG_io on the main domain, table mode): per case the
+    seeded DH and G, the reference's constants (dumpengpar) and the state after loading
+    (loadeng: properties from DH, walls) and after each of the substeps NS = 1, 2, 3."""
+    sys.path.insert(0, os.path.join(ROOT))
+    import __graft_entry__
+    __graft_entry__.load()
+    from chapsim_amd import thermo
+    rows = thermo.synthetic_table()
+    t = thermo.table_setup(rows, T0)
+    out = {}
+    for L, M, N, STR2, REN, dt, wt, wbc, seed in ENG_CASES:
+        mesh = setup.Mesh(L, M, N, STR2=STR2)
+        dh, g = oracle.eng_initial_field(mesh, t, seed)
+        h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt, iDomain=2, iThermoDynamics=1, T0=T0, Ti=T0,
+                           iThermalWallType=wt, thermalWallBC=wbc)
+        try:
+            thermo.write_table_file(h.path("NIST_WATER_23.5MP.DAT"), rows)
+            h.write_eng("eng.bin", dh, g)
+            h.run(["thermoinitall", "dumpengpar par.bin", "loadeng eng.bin", "dumpeng s0",
+                   f"engsubstep 1 {dt!r}", "dumpeng s1", f"engsubstep 2 {dt!r}", "dumpeng s2",
+                   f"engsubstep 3 {dt!r}", "dumpeng s3"])
+            par = np.fromfile(h.path("par.bin"), dtype=np.float64)
+            states = {s: h.read_eng(s) for s in ("s0", "s1", "s2", "s3")}
+        finally:
+            h.close()
+        tag = f"e{L}x{M}x{N}"
+        out[f"{tag}_params"] = np.array([L, M, N, STR2, REN, dt, wt[0], wt[1], wbc[0], wbc[1], T0, seed])
+        out[f"{tag}_par"] = par
+        out[f"{tag}_dh"] = dh
+        out[f"{tag}_g"] = g
+        for s, st in states.items():
+            for k, v in st.items():
+                out[f"{tag}_{s}_{k}"] = v
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "eng.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
     import sys as _sys
-    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack"]
+    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng"]
     if "tg" in which:
         for case in CASES:
             make_case(*case)
@@ -180,3 +225,5 @@ if __name__ == "__main__":
         make_tdma()
     if "fishpack" in which:
         make_fishpack()
+    if "eng" in which:
+        make_eng()

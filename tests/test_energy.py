@@ -1,0 +1,128 @@
+"""Energy equation of the heated channel (SOLVERRK3_ENG_io, table properties): the oracle
+restatement and the GPU substep against the reference's own output (tests/golden/eng.npz,
+written by the reference's SOLVERRK3_ENG_io through oracle/ref_harness.f90 on two ranks).
+Tolerance 1e-12 relative (max-norm) per field: both follow the reference's operation order
+with FMA contraction off, so what remains is the rounding of the compilers' own choices."""
+import numpy as np
+import pytest
+
+from oracle import oracle, setup
+
+FIELDS = (("dh", "DH"), ("h", "H"), ("t", "T"), ("d", "D"), ("m", "M"), ("k", "K"), ("cp", "CP"))
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def cases(golden):
+    g = golden("eng.npz")
+    tags = sorted({k.split("_")[0] for k in g})
+    out = []
+    for tag in tags:
+        L, M, N, STR2, REN, dt, w0, w1, b0, b1, T0, seed = g[tag + "_params"]
+        par = g[tag + "_par"]
+        out.append(dict(tag=tag, L=int(L), M=int(M), N=int(N), STR2=float(STR2), dt=float(dt),
+                        wt=(int(w0), int(w1)), T0=float(T0), cthecd=par[0], dhmin=par[1], dhmax=par[2],
+                        wf=(par[3], par[4]), ws=par[5:15].reshape(5, 2).T, top_face=int(par[15]), g=g))
+    return out
+
+
+@pytest.fixture(scope="module")
+def table(pkg):
+    from importlib import import_module
+    thermo = import_module(pkg.__name__ + ".thermo")
+    return thermo, thermo.table_setup(thermo.synthetic_table(), 665.0)
+
+
+def test_fixture_constants_follow_from_table(golden, table):
+    """The reference's CTHECD = 1/(REN PRT0), PRT0 = M0 CP0 / K0, DHmin = DH(IMIN_DH),
+    DHmax = DH(IMAX_DH) equal the host set-up's values bit for bit."""
+    _, t = table
+    for c in cases(golden):
+        REN = float(c["g"][c["tag"] + "_params"][4])
+        assert 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"]) == c["cthecd"]
+        assert t["DH"][t["IMIN_DH"] - 1] == c["dhmin"] and t["DH"][t["IMAX_DH"] - 1] == c["dhmax"]
+        # two ranks: the top wall's face index is the last rank's row count
+        assert c["top_face"] == c["M"] // 2
+
+
+def test_oracle_matches_reference(golden, table):
+    _, t = table
+    for c in cases(golden):
+        g = c["g"]
+        mesh = setup.Mesh(c["L"], c["M"], c["N"], STR2=c["STR2"])
+        o = oracle.OracleEng(mesh, t, c["wt"], c["cthecd"], c["dhmin"], c["dhmax"], c["wf"],
+                             wall_state=c["ws"], top_face=c["top_face"])
+        assert o.init(g[c["tag"] + "_dh"], g[c["tag"] + "_g"]) == 0
+        for ns in (0, 1, 2, 3):
+            if ns:
+                assert o.substep(ns, c["dt"]) == 0
+            s = f"{c['tag']}_s{ns}"
+            for k, kk in FIELDS:
+                assert rel(o.f[k], g[f"{s}_{kk}"]) < TOL, (s, k, rel(o.f[k], g[f"{s}_{kk}"]))
+            assert rel(o.f["rhs0"], g[f"{s}_RHS0"]) < TOL if ns else np.all(o.f["rhs0"] == 0)
+
+
+def test_oracle_wall_face_index_matters(golden, table):
+    """Planted mistake: with the one-rank face index (M) the iso-flux case's top ghost row
+    departs from the two-rank reference output, so the fixture pins the index."""
+    _, t = table
+    c = [c for c in cases(golden) if c["wt"] == (1, 1)][0]
+    g = c["g"]
+    mesh = setup.Mesh(c["L"], c["M"], c["N"], STR2=c["STR2"])
+    o = oracle.OracleEng(mesh, t, c["wt"], c["cthecd"], c["dhmin"], c["dhmax"], c["wf"], top_face=c["M"])
+    o.init(g[c["tag"] + "_dh"], g[c["tag"] + "_g"])
+    assert rel(o.f["dh"][:, -1, :], g[c["tag"] + "_s0_DH"][:, -1, :]) > 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_substeps_match_reference(pkg, golden, table):
+    thermo, t = table
+    for c in cases(golden):
+        print(f"energy {c['tag']} walls {c['wt']}", flush=True)
+        g = c["g"]
+        co = pkg.channel_setup(c["L"], c["M"], c["N"], STR2=c["STR2"])
+        e = thermo.EnergyTG(co, t, c["wt"], c["cthecd"], c["dhmin"], c["dhmax"], c["wf"],
+                            wall_state=c["ws"], top_face=c["top_face"])
+        e.upload(g[c["tag"] + "_dh"], g[c["tag"] + "_g"])
+        for ns in (0, 1, 2, 3):
+            if ns:
+                e.SOLVERRK3_ENG_io(ns, c["dt"])
+            out = e.download()
+            s = f"{c['tag']}_s{ns}"
+            for _, kk in FIELDS:
+                assert rel(out[kk], g[f"{s}_{kk}"]) < TOL, (s, kk, rel(out[kk], g[f"{s}_{kk}"]))
+            if ns:
+                assert rel(out["RHS0"], g[f"{s}_RHS0"]) < TOL
+        e.close()
+
+
+@pytest.mark.gpu
+def test_gpu_energy_matches_oracle_large(pkg, table):
+    """A larger seeded state (64 x 48 x 32, iso-flux bottom, fixed-T top, one-rank face index)
+    against the oracle over a full RK3 step."""
+    thermo, t = table
+    L, M, N, STR2, dt = 64, 48, 32, 1.9, 5e-4
+    mesh = setup.Mesh(L, M, N, STR2=STR2)
+    dh, gg = oracle.eng_initial_field(mesh, t, 77)
+    REN = 2800.0
+    cthecd = 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"])
+    dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+    Tw = 675.0 / t["T0"]
+    hw = float(np.interp(Tw, t["T"], t["H"]))
+    ws = [[0] * 5, [Tw, hw, float(np.interp(hw, t["H"], t["D"])), float(np.interp(hw, t["H"], t["M"])),
+                   float(np.interp(hw, t["H"], t["K"]))]]
+    wf = (0.1, 0.0)
+    o = oracle.OracleEng(mesh, t, (1, 2), cthecd, dhmin, dhmax, wf, wall_state=ws)
+    assert o.init(dh, gg) == 0
+    e = thermo.EnergyTG(pkg.channel_setup(L, M, N, STR2=STR2), t, (1, 2), cthecd, dhmin, dhmax, wf, wall_state=ws)
+    e.upload(dh, gg)
+    for ns in (1, 2, 3):
+        assert o.substep(ns, dt) == 0
+        e.SOLVERRK3_ENG_io(ns, dt)
+    out = e.download()
+    for k, kk in FIELDS:
+        assert rel(out[kk], o.f[k]) < TOL, (kk, rel(out[kk], o.f[k]))
+    e.close()
