@@ -117,7 +117,8 @@ def load_pkg():
     return __graft_entry__.load()
 
 
-def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False, shapes=None):
+def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False, shapes=None,
+            roundtrip=0):
     c = c or CFG
     coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
@@ -169,11 +170,32 @@ def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotu
         dom_avg_ms, dom_n = pre_avg, pre_n
     cfl = ch.cfl(dt)
     div = ch.max_div()
+    rt = host_roundtrip(pkg, ch, dt, cv, roundtrip) if roundtrip else None
     ch.close()
     ms = (t1 - t0) * 1e3 / steps
     return dict(ms_per_step=ms, per_kernel_ms=per, launches=launches, dominant=dom,
                 dom_avg_ms=dom_avg_ms, dom_count=dom_n, cfl=cfl, max_div=div,
-                wbytes=4 if form == "mixed" else 8, start=(q0, p0), tuning=tuning)
+                wbytes=4 if form == "mixed" else 8, start=(q0, p0), tuning=tuning, roundtrip_ms=rt)
+
+
+def host_roundtrip(pkg, ch, dt, cv, steps):
+    """ms per RK3 step of the per-substep drop-in (SOLVERRK3_MOM_tg with host arrays): each
+    substep copies Q, PR, CONVH0 host -> HBM, runs, and copies them back (pageable host
+    buffers allocated once, as the Fortran glue holds them)."""
+    from chapsim_amd.tg import _p
+    q, p, h = ch.download(with_history=True)
+    lib = pkg.lib()
+    ch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for ns in (1, 2, 3):
+            if lib.chs_upload_fields(ch._h, _p(q), _p(p), _p(h)) != 0:
+                raise RuntimeError("chs_upload_fields failed")
+            ch.SOLVERRK3_MOM_tg(ns, dt, cv)
+            if lib.chs_download_fields(ch._h, _p(q), _p(p), _p(h)) != 0:
+                raise RuntimeError("chs_download_fields failed")
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps
 
 
 def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = None, c=None):
@@ -347,6 +369,8 @@ def main():
     ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble"],
                     default="channel")
     ap.add_argument("--nref", type=int, default=1, help="refinement sweeps (poisson, mixed)")
+    ap.add_argument("--host-roundtrip", type=int, default=0, metavar="STEPS",
+                    help="also time STEPS steps of the per-substep host-array drop-in (PCIe included)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -370,7 +394,8 @@ def main():
         return bench_ensemble(pkg, args)
     c = CFG64 if args.workload == "channel64" else CFG
     shapes = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set}
-    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune, shapes)
+    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune, shapes,
+                args.host_roundtrip)
     F = c["L"] * c["M"] * c["N"]
     value = F / (r["ms_per_step"] * 1e-3)
     dom = r["dominant"]
@@ -422,6 +447,7 @@ def main():
                           "frac": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "per_kernel_ms_per_step": r["per_kernel_ms"],
         "checks": {"cfl_dt": r["cfl"], "max_div": r["max_div"]},
+        "host_roundtrip_ms_per_step": r["roundtrip_ms"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -27,6 +27,24 @@ MODULE chapsim_hip
         TYPE(C_PTR) :: tgam, troh, talp
     END TYPE chs_coeffs
 
+    ! mirror of chs_prop_table (pointers to the thermal_info arrays LIST_* and SplineCoeff_*)
+    TYPE, BIND(C) :: chs_prop_table
+        INTEGER(C_INT) :: n, imin_dh, imax_dh, imin_h, imax_h
+        TYPE(C_PTR) :: dh, h, t, d, m, k, cp, b
+        TYPE(C_PTR) :: dhh_b, dhh_c, dhh_d, ht_b, ht_c, ht_d, hd_b, hd_c, hd_d
+        TYPE(C_PTR) :: hm_b, hm_c, hm_d, hk_b, hk_c, hk_d, hcp_b, hcp_c, hcp_d
+        TYPE(C_PTR) :: hb_b, hb_c, hb_d
+    END TYPE chs_prop_table
+
+    ! mirror of chs_eng_params (wall_state(1:5, w) = T, H, D, M, K of wall w = 1 bottom, 2 top)
+    TYPE, BIND(C) :: chs_eng_params
+        INTEGER(C_INT) :: wall_type(2)
+        REAL(C_DOUBLE) :: cthecd, dh_min, dh_max
+        TYPE(C_PTR) :: wallflux
+        REAL(C_DOUBLE) :: wall_state(5, 2)
+        INTEGER(C_INT) :: top_face
+    END TYPE chs_eng_params
+
     INTERFACE
         FUNCTION chs_version() BIND(C, NAME='chs_version') RESULT(p)
             IMPORT :: C_PTR
@@ -123,6 +141,98 @@ MODULE chapsim_hip
         FUNCTION chs_sync(ctx) BIND(C, NAME='chs_sync') RESULT(rc)
             IMPORT :: C_INT, C_PTR
             TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_set_refinement(ctx, nref_step, nref_projection) BIND(C, NAME='chs_set_refinement') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT), VALUE :: nref_step, nref_projection
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        ! key: a C string, e.g. 'conv_by' // C_NULL_CHAR
+        FUNCTION chs_set_tuning(ctx, key, val) BIND(C, NAME='chs_set_tuning') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_CHAR
+            TYPE(C_PTR), VALUE :: ctx
+            CHARACTER(KIND=C_CHAR), INTENT(IN) :: key(*)
+            INTEGER(C_INT), VALUE :: val
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        ! FISHPACK_POIS3D_SIMPLE (possion3d_fishpack.f90:342-453): host rhs(NCL1, NCL2, NCL3) ->
+        ! phi(NCL1, NCL2, NCL3)
+        FUNCTION chs_poisson_fishpack(ctx, rhs, phi) BIND(C, NAME='chs_poisson_fishpack') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: ctx, rhs, phi
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        ! TDMAIJI_CYC (kind 0), TDMAIJJ_nonCYC (1), TDMAIJI_nonCYC (2), TDMAIJJ_CYC (3) on device
+        ! arrays A, B, C, R(isz, jsz); bc(lines, 2) device boundary values of the non-cyclic kinds
+        FUNCTION chs_tdma_lines(kind, isz, jsz, a, b, c, r, bc) BIND(C, NAME='chs_tdma_lines') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            INTEGER(C_INT), VALUE :: kind, isz, jsz
+            TYPE(C_PTR), VALUE :: a, b, c, r, bc
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_prop_create(t, prop) BIND(C, NAME='chs_prop_create') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, chs_prop_table
+            TYPE(chs_prop_table), INTENT(IN) :: t
+            TYPE(C_PTR), INTENT(OUT) :: prop
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        SUBROUTINE chs_prop_destroy(prop) BIND(C, NAME='chs_prop_destroy')
+            IMPORT :: C_PTR
+            TYPE(C_PTR), VALUE :: prop
+        END SUBROUTINE
+
+        ! THERM_PROP_UPDATE_FROM_DH for n values: out(n, 7) = H, T, D, M, K, Cp, B
+        FUNCTION chs_prop_update_from_dh(prop, n, dh, out) BIND(C, NAME='chs_prop_update_from_dh') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_LONG, C_DOUBLE
+            TYPE(C_PTR), VALUE :: prop
+            INTEGER(C_LONG), VALUE :: n
+            REAL(C_DOUBLE), INTENT(IN) :: dh(n)
+            REAL(C_DOUBLE), INTENT(OUT) :: out(n, 7)
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        ! Energy substep SOLVERRK3_ENG_io (SOLVERRK3_ENG_IO.f90:11-82), table properties
+        FUNCTION chs_eng_create(c, t, p, eng) BIND(C, NAME='chs_eng_create') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, chs_coeffs, chs_prop_table, chs_eng_params
+            TYPE(chs_coeffs), INTENT(IN) :: c
+            TYPE(chs_prop_table), INTENT(IN) :: t
+            TYPE(chs_eng_params), INTENT(IN) :: p
+            TYPE(C_PTR), INTENT(OUT) :: eng
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        SUBROUTINE chs_eng_destroy(eng) BIND(C, NAME='chs_eng_destroy')
+            IMPORT :: C_PTR
+            TYPE(C_PTR), VALUE :: eng
+        END SUBROUTINE
+
+        ! dh = C_LOC(DH(1,0,1)), g = C_LOC(G_io(1,0,1,1)), rhs0 = C_LOC(RHS_ENERGY0) or C_NULL_PTR
+        FUNCTION chs_eng_upload(eng, dh, g, rhs0, init_props) BIND(C, NAME='chs_eng_upload') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: eng, dh, g, rhs0
+            INTEGER(C_INT), VALUE :: init_props
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_substep_eng(eng, ns, dt) BIND(C, NAME='chs_substep_eng') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: eng
+            INTEGER(C_INT), VALUE :: ns
+            REAL(C_DOUBLE), VALUE :: dt
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_eng_download(eng, dh, h, t, d, m, k, cp, rhs0) BIND(C, NAME='chs_eng_download') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: eng, dh, h, t, d, m, k, cp, rhs0
             INTEGER(C_INT) :: rc
         END FUNCTION
 
