@@ -56,7 +56,13 @@ def lib():
                                             C.c_long, _dp]),
                            ("or_tdma_noncyc", [C.c_int, _dp, _dp, _dp, _dp, C.c_long, _dp]),
                            ("or_tdma_lines", [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
-                           ("or_dft", [C.c_int, C.c_int, _dp, _dp, _dp, _dp])):
+                           ("or_dft", [C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+                           ("or_io_props", [C.c_void_p, C.c_void_p, C.c_int]),
+                           ("or_io_velocity", [C.c_void_p, C.c_void_p]),
+                           ("or_io_convection", [C.c_void_p, C.c_void_p]),
+                           ("or_io_divu", [C.c_void_p, C.c_void_p]),
+                           ("or_io_viscous", [C.c_void_p, C.c_void_p, C.c_void_p]),
+                           ("or_io_substep", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double])):
             fn = getattr(_lib, name)
             fn.argtypes = args
             fn.restype = None
@@ -285,3 +291,82 @@ class OracleEng:
         out = np.zeros(7)
         err = lib().or_prop_from_dh(C.byref(self._t), float(dh), _p(out))
         return out, err
+
+
+# ---- Main-domain momentum substep, explicit viscous scheme (or_io_* in tg_oracle.c) ---------
+
+class _Io(C.Structure):
+    _fields_ = [(k, _dp) for k in ("g", "g0", "q", "pr", "dph", "d0", "m0", "dstg", "mstg", "explt0",
+                                   "drho", "divu")] + [("h", _dp * 3), ("rhsp", _dp)]
+
+
+class _IoPar(C.Structure):
+    _fields_ = [("cvisc", C.c_double), ("gravity", C.c_int), ("f_a", C.c_double),
+                ("flow_driven", C.c_int), ("cf_given", C.c_double)]
+
+
+def io_initial_field(mesh, seed, rho_amp=0.3, mu_amp=0.3):
+    """Seeded main-domain state for the variable-property momentum substep: G = rho u with a
+    Poiseuille-like u and 5 % noise, G0 = G plus 1 % noise, a small random pressure, DENSITY0 and
+    Viscousity0 around 1 (smooth in y plus noise, amplitudes rho_amp, mu_amp) and a random
+    EXPLT0.  Walls: G ghost rows and v on J = 1 zero, pressure ghost rows copy the wall cells.
+    Returns dict G, G0 [3, N, M+2, L], PR, D0, M0 [N, M+2, L], EXPLT0 [3, N, M, L]."""
+    L, M, N = mesh.L, mesh.M, mesh.N
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    y = np.concatenate([[-1.0], np.asarray(mesh.YCC), [1.0]])
+    hs = (N, M + 2, L)
+    d0 = 1.0 + rho_amp * (0.5 * y[None, :, None] + 0.2 * rng.standard_normal(hs))
+    m0 = 1.0 + mu_amp * (-0.4 * y[None, :, None] + 0.2 * rng.standard_normal(hs))
+    g = np.zeros((3,) + hs)
+    g[0] = 1.5 * (1.0 - y[None, :, None] ** 2) * (1.0 + 0.05 * rng.standard_normal(hs))
+    g[1] = 0.05 * rng.standard_normal(hs)
+    g[2] = 0.05 * rng.standard_normal(hs)
+    g0 = g + 0.01 * rng.standard_normal((3,) + hs)
+    for a in (g, g0):
+        a[:, :, 0, :] = 0.0
+        a[:, :, M + 1, :] = 0.0
+        a[1, :, 1, :] = 0.0
+    pr = 0.01 * rng.standard_normal(hs)
+    pr[:, 0, :] = pr[:, 1, :]
+    pr[:, M + 1, :] = pr[:, M, :]
+    explt0 = 0.1 * rng.standard_normal((3, N, M, L))
+    return dict(G=g, G0=g0, PR=pr, D0=d0, M0=m0, EXPLT0=explt0)
+
+
+class OracleIO:
+    """SOLVERRK3_MOM_io restated (or_io_substep) with the staggered properties of SOLVE
+    (or_io_props).  Fields are C-order; ``f`` holds G, G0, Q, DSTG, MSTG [3, N, M+2, L], PR,
+    DPH, D0, M0 [N, M+2, L], EXPLT0, H [3, N, M, L], DRHO, DIVU, RHSP [N, M, L]."""
+
+    def __init__(self, mesh: _setup.Mesh, REN: float, gravity=0, f_a=0.0, flow_driven=1, cf_given=0.0):
+        L, M, N = mesh.L, mesh.M, mesh.N
+        self.mesh = mesh
+        self._om = OracleTG(mesh, REN)._m
+        hs = (N, M + 2, L)
+        self.f = {k: np.zeros((3,) + hs) for k in ("G", "G0", "Q", "DSTG", "MSTG")}
+        self.f.update({k: np.zeros(hs) for k in ("PR", "DPH", "D0", "M0")})
+        self.f.update({k: np.zeros((N, M, L)) for k in ("DRHO", "DIVU", "RHSP")})
+        self.f["EXPLT0"] = np.zeros((3, N, M, L))
+        self.f["H"] = np.zeros((3, N, M, L))
+        f = self.f
+        self._s = _Io(_p(f["G"]), _p(f["G0"]), _p(f["Q"]), _p(f["PR"]), _p(f["DPH"]), _p(f["D0"]),
+                      _p(f["M0"]), _p(f["DSTG"]), _p(f["MSTG"]), _p(f["EXPLT0"]), _p(f["DRHO"]),
+                      _p(f["DIVU"]), (_dp * 3)(*[_p(f["H"][c]) for c in range(3)]), _p(f["RHSP"]))
+        self._par = _IoPar(1.0 / REN, int(gravity), float(f_a), int(flow_driven), float(cf_given))
+
+    def set_state(self, st: dict):
+        for k in ("G", "G0", "PR", "D0", "M0", "EXPLT0"):
+            self.f[k][...] = st[k]
+
+    def props(self, ns):
+        lib().or_io_props(C.byref(self._om), C.byref(self._s), int(ns))
+
+    def terms(self):
+        """VELOCITY_CALC_io, convection, DIVG_U_io and the viscous terms (no state change but Q)."""
+        lib().or_io_velocity(C.byref(self._om), C.byref(self._s))
+        lib().or_io_convection(C.byref(self._om), C.byref(self._s))
+        lib().or_io_divu(C.byref(self._om), C.byref(self._s))
+        lib().or_io_viscous(C.byref(self._om), C.byref(self._par), C.byref(self._s))
+
+    def substep(self, ns, dt):
+        lib().or_io_substep(C.byref(self._om), C.byref(self._par), C.byref(self._s), int(ns), float(dt))

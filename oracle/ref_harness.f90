@@ -37,6 +37,19 @@
 !   engsubstep NS DT        DT = given, CALL SOLVERRK3_ENG_io(NS)
 !   dumpeng FILE            each rank writes FILE.<rank>: DH, H, T, D, M, K, Cp (x 1..L, rows
 !                           0..N2DO+1), then RHS_ENERGY0 rows 1..N2DO
+!   loadio FILE             (main domain, iThermoDynamics = 1) G_io, G0_io, PR_io, DENSITY0,
+!                           Viscousity0 and EXPLT0_io from FILE, see H_LOADIO
+!   ioprops NS              DENSITY_TIME_DERIVATION(NS), DENSITY_Staggered, MU_Staggered
+!                           (SOLVE.f90:122-124)
+!   ioterms FILE            VELOCITY_CALC_io, CONVECTION_X/Y/Z_io, DIVG_U_io and
+!                           VISCOUS_ALL_EXPLT_X/Y/Z_io; each rank writes FILE.<rank>: QTMP_io,
+!                           DPH_io, RHSLLPHI_io, DivU_io rows 1..N2DO
+!   iosubstep NS DT         DT = given; the body of SOLVERRK3_MOM_io(NS) (explicit viscous
+!                           scheme) with the FFT99 Poisson solve, see H_IOSUBSTEP
+!   dumpio FILE             each rank writes FILE.<rank>: G_io, Q_io, PR_io, D_STG, MU_STG,
+!                           DPH_io rows 0..N2DO+1, then EXPLT0_io, DrhoDtP, RHSLLPHI_io rows
+!                           1..N2DO
+!   dumpiopar FILE          rank 0: F_A, VOLM_io, CVISC
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -83,6 +96,8 @@ PROGRAM ref_harness
     CALL BCAST_TDMA_COEF
     IF (TgFlowFlg) CALL FFT99_POIS3D_INIT(ITG)
     IF (IoFlowFlg) CALL FISHPACK_POIS3D_INIT   ! DNS_THERMAL.f90:128 (is_FFT_FISHPACK)
+    ! the main domain's other Poisson method (DNS_THERMAL.f90:127), used by iosubstep
+    IF (IoFlowFlg .AND. .NOT. TgFlowFlg) CALL FFT99_POIS3D_INIT(IIO)
     CALL MPI_BARRIER(ICOMM, IERROR)
 
     IF (TgFlowFlg) CONVH0_tg = 0.0_WP
@@ -204,6 +219,49 @@ PROGRAM ref_harness
                       HEATCAP(1:NCL1_io, :, :)
             WRITE(80) RHS_ENERGY0(:, 1:N2DO(MYID), :)
             CLOSE(80)
+        CASE ('loadio')
+            READ(line, *) cmd, fname
+            CALL H_LOADIO(TRIM(fname))
+        CASE ('ioprops')
+            READ(line, *) cmd, ns
+            CALL DENSITY_TIME_DERIVATION(ns)
+            CALL DENSITY_Staggered
+            CALL MU_Staggered
+        CASE ('ioterms')
+            READ(line, *) cmd, fname
+            CALL VELOCITY_CALC_io
+            CALL CONVECTION_X_io
+            CALL CONVECTION_Y_io
+            CALL CONVECTION_Z_io
+            CALL DIVG_U_io
+            CALL VISCOUS_ALL_EXPLT_X_io
+            CALL VISCOUS_ALL_EXPLT_Y_io
+            CALL VISCOUS_ALL_EXPLT_Z_io
+            CALL H_OPENRANK(TRIM(fname), 80)
+            WRITE(80) QTMP_io(1:NCL1_io, 1:N2DO(MYID), :), DPH_io(1:NCL1_io, 1:N2DO(MYID), :), &
+                      RHSLLPHI_io(1:NCL1_io, 1:N2DO(MYID), :), DivU_io(1:NCL1_io, 1:N2DO(MYID), :)
+            CLOSE(80)
+        CASE ('iosubstep')
+            READ(line, *) cmd, ns, dtin
+            DT = dtin
+            CALL H_IOSUBSTEP(ns)
+        CASE ('dumpio')
+            READ(line, *) cmd, fname
+            CALL H_OPENRANK(TRIM(fname), 80)
+            WRITE(80) G_io(1:NCL1_io, 0:N2DO(MYID) + 1, :, :), Q_io(1:NCL1_io, 0:N2DO(MYID) + 1, :, :), &
+                      PR_io(1:NCL1_io, 0:N2DO(MYID) + 1, :), D_STG(1:NCL1_io, 0:N2DO(MYID) + 1, :, :), &
+                      MU_STG(1:NCL1_io, 0:N2DO(MYID) + 1, :, :), DPH_io(1:NCL1_io, 0:N2DO(MYID) + 1, :)
+            WRITE(80) EXPLT0_io(:, 1:N2DO(MYID), :, :), DrhoDtP(1:NCL1_io, 1:N2DO(MYID), :), &
+                      RHSLLPHI_io(1:NCL1_io, 1:N2DO(MYID), :)
+            CLOSE(80)
+        CASE ('dumpiopar')
+            READ(line, *) cmd, fname
+            IF (MYID == 0) THEN
+                OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+                     STATUS = 'replace', ACTION = 'write')
+                WRITE(80) F_A, VOLM_io, CVISC
+                CLOSE(80)
+            END IF
         CASE ('tdma')
             READ(line, *) cmd, idr, fname2, fname
             IF (MYID == 0) CALL H_TDMA(idr, TRIM(fname2), TRIM(fname))
@@ -358,6 +416,75 @@ CONTAINS
         RHS_ENERGY0 = 0.0_WP
         DEALLOCATE(G)
     END SUBROUTINE
+    ! Main-domain momentum state from one file: 12 global arrays, each L x (M+2) x N (y rows
+    ! 0..M+1, Fortran order): G_io (3 components), G0_io (3), PR_io, DENSITY0, Viscousity0,
+    ! EXPLT0_io (3, rows 1..M used).  Every rank keeps its y-slab; DENSITY and Viscousity are
+    ! set equal to DENSITY0 and Viscousity0.
+    SUBROUTINE H_LOADIO(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: A(:, :, :, :)
+        INTEGER(4) :: J, JG, C
+        ALLOCATE(A(NCL1_io, 0:NCL2 + 1, NCL3, 12))
+        OPEN(79, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', &
+             ACTION = 'read')
+        READ(79) A
+        CLOSE(79)
+        DO J = 0, N2DO(MYID) + 1
+            JG = JDSWT(MYID) - 1 + J
+            DO C = 1, 3
+                G_io(1:NCL1_io, J, :, C) = A(:, JG, :, C)
+                G0_io(1:NCL1_io, J, :, C) = A(:, JG, :, 3 + C)
+            END DO
+            PR_io(1:NCL1_io, J, :) = A(:, JG, :, 7)
+            DENSITY0(1:NCL1_io, J, :) = A(:, JG, :, 8)
+            DENSITY(1:NCL1_io, J, :) = A(:, JG, :, 8)
+            Viscousity0(1:NCL1_io, J, :) = A(:, JG, :, 9)
+            Viscousity(1:NCL1_io, J, :) = A(:, JG, :, 9)
+        END DO
+        DO J = 1, N2DO(MYID)
+            JG = JDSWT(MYID) - 1 + J
+            DO C = 1, 3
+                EXPLT0_io(:, J, :, C) = A(:, JG, :, 9 + C)
+            END DO
+        END DO
+        DEALLOCATE(A)
+    END SUBROUTINE
+
+    ! SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:24-117) for the periodic main domain with the
+    ! explicit viscous scheme, calling the reference's routines in the same order; the Poisson
+    ! solve is FFT99_POIS3D_periodicxz(IIO) (:92) instead of the compiled default
+    ! FISHPACK_POIS3D_SIMPLE(RHSLLPHI_io, DPH_io) (:93), whose output dummy has no y-halo rows
+    ! while DPH_io has them, so the default stores its solution shifted in memory.
+    SUBROUTINE H_IOSUBSTEP(NS)
+        INTEGER(4), INTENT(IN) :: NS
+        INTEGER(4) :: IDR
+        G0_io(:, :, :, :) = G_io(:, :, :, :)
+        CALL VELOCITY_CALC_io
+        CALL CONVECTION_X_io
+        CALL CONVECTION_Y_io
+        CALL CONVECTION_Z_io
+        CALL DIVG_U_io
+        IDR = 1
+        CALL VISCOUS_ALL_EXPLT_X_io
+        CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+        IDR = 2
+        CALL VISCOUS_ALL_EXPLT_Y_io
+        CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+        IDR = 3
+        CALL VISCOUS_ALL_EXPLT_Z_io
+        CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+        CALL MASSFLUX_CALC_io
+        CALL INTFC_VARS3(1, NCL1_io, NCL1S, NCL1E, G_io)
+        CALL BC_WALL_G_io
+        CALL DIVG_io(NS)
+        CALL FFT99_POIS3D_periodicxz(IIO)
+        CALL INTFC_VARS1(1, NCL1_io, NCL1S, NCL1E, DPH_io)
+        CALL BC_WALL_DPH_io
+        CALL PRCALC_io(NS)
+        CALL MASSFLUX_UPDATE_io(NS)
+        CALL VELOCITY_CALC_io
+    END SUBROUTINE
+
     SUBROUTINE H_DUMP3H(fn, A)
         CHARACTER(*), INTENT(IN) :: fn
         REAL(WP), INTENT(IN) :: A(NCL1_tg, 0:N2DO(0) + 1, NCL3)

@@ -29,7 +29,8 @@ def ini_text(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, DT=1e-3,
              NTSTF=1, iIniField_tg=0, CFLGV=1.0e6, TSTOP=1.0e6, dtSave1=1.0e6,
              iterMonitor=1, iRandomType=2, VPER=0.6, SVPER=0.3, TimeReStart_tg=0.0,
              iDomain=1, iThermoDynamics=0, T0=680.15, Ti=680.15, L0=1.5e-3,
-             iThermalWallType=(2, 2), thermalWallBC=(640.15, 660.15), iVisScheme=1) -> str:
+             iThermalWallType=(2, 2), thermalWallBC=(640.15, 660.15), iVisScheme=1,
+             iGravity=0, iFlowDriven=1) -> str:
     """Channel as key lines: iDomain=1 puts the mesh on the turbulence-generator domain
     (NCL1_tg, HX_tg), iDomain=2 on the main (IO) domain (NCL1_io, HX_io).
 
@@ -46,9 +47,9 @@ def ini_text(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, DT=1e-3,
         "[mesh]", f"NCL1_tg= {Ltg}", f"NCL1_io= {Lio}", f"NCL3= {N}", f"NCL2= {M}",
         f"STR2= {f(STR2)}",
         "[boundary]", "BCX12_tg= 3, 3", "BCX12_io= 3, 3", "BCZ12= 3, 3", "BCY12= 1, 1",
-        "[fluid]", f"REN= {f(REN)}", f"ReIni= {f(REN)}", "TLgRe= 0.0", "iFlowDriven= 1",
+        "[fluid]", f"REN= {f(REN)}", f"ReIni= {f(REN)}", "TLgRe= 0.0", f"iFlowDriven= {iFlowDriven}",
         "Cf_Given= 8.18E-3",
-        "[thermohydraulics]", "iFluidMedia= 1", "iGravity= 0", f"L0= {f(L0)}", f"T0= {f(T0)}",
+        "[thermohydraulics]", "iFluidMedia= 1", f"iGravity= {iGravity}", f"L0= {f(L0)}", f"T0= {f(T0)}",
         f"Ti= {f(Ti)}", f"iThermalWallType= {iThermalWallType[0]}, {iThermalWallType[1]}",
         f"thermalWallBC_Dim= {f(thermalWallBC[0])}, {f(thermalWallBC[1])}",
         "[conjugateHeatTransfer]", "HY_Solid= 0.1, 0.1", "HXs_Solid= 1.0, 1.0",
@@ -246,3 +247,65 @@ class Harness:
     def read_cells(self, name, nvars=1):
         a = _gather(self.path(name), self.nranks, nvars, halo=False)
         return a if nvars > 1 else a[0]
+
+
+def _split_dump(path, nranks, groups):
+    """Per-rank dumps holding consecutive groups of arrays; groups = [(nvars, halo), ...].
+    Returns one [nvars, N, rows, L] array per group, reassembled over the ranks."""
+    ranks = []
+    for r in range(nranks):
+        with open(f"{path}.{r}", "rb") as fh:
+            hdr = np.frombuffer(fh.read(24), dtype=np.int32)
+            data = np.frombuffer(fh.read(), dtype=np.float64)
+        myid, j0, n2, L, M, N = (int(v) for v in hdr)
+        off, arrs = 0, []
+        for nv, halo in groups:
+            rows = n2 + 2 if halo else n2
+            n = nv * N * rows * L
+            arrs.append(data[off:off + n].reshape(nv, N, rows, L))
+            off += n
+        ranks.append((j0, n2, arrs))
+    Mt = sum(r[1] for r in ranks)
+    N, L = ranks[0][2][0].shape[1], ranks[0][2][0].shape[3]
+    out = []
+    for g, (nv, halo) in enumerate(groups):
+        a = np.zeros((nv, N, Mt + 2 if halo else Mt, L))
+        for j0, n2, arrs in ranks:
+            x = arrs[g]
+            if halo:
+                a[:, :, j0:j0 + n2, :] = x[:, :, 1:n2 + 1, :]
+                if j0 == 1:
+                    a[:, :, 0, :] = x[:, :, 0, :]
+                if j0 + n2 - 1 == Mt:
+                    a[:, :, Mt + 1, :] = x[:, :, n2 + 1, :]
+            else:
+                a[:, :, j0 - 1:j0 - 1 + n2, :] = x
+        out.append(a)
+    return out
+
+
+def write_io(h: "Harness", name, g, g0, pr, d0, m0, explt0):
+    """Main-domain momentum state in the layout ``loadio`` reads: G, G0 [3, N, M+2, L], PR, D0,
+    M0 [N, M+2, L], EXPLT0 [3, N, M, L] (written with zero halo rows)."""
+    g = np.asarray(g, dtype=np.float64)
+    N, Mh, L = g.shape[1:]
+    ex = np.zeros((3, N, Mh, L))
+    ex[:, :, 1:Mh - 1, :] = explt0
+    np.concatenate([g, np.asarray(g0, dtype=np.float64), np.asarray(pr, dtype=np.float64)[None],
+                    np.asarray(d0, dtype=np.float64)[None], np.asarray(m0, dtype=np.float64)[None],
+                    ex]).tofile(h.path(name))
+
+
+def read_io(h: "Harness", name):
+    """State written by ``dumpio``: dict G, Q, DSTG, MSTG [3, N, M+2, L]; PR, DPH [N, M+2, L];
+    EXPLT0 [3, N, M, L]; DRHO, RHSP [N, M, L]."""
+    g, q, pr, ds, ms, dph, ex, dr, rp = _split_dump(
+        h.path(name), h.nranks, [(3, True), (3, True), (1, True), (3, True), (3, True), (1, True),
+                                 (3, False), (1, False), (1, False)])
+    return dict(G=g, Q=q, PR=pr[0], DSTG=ds, MSTG=ms, DPH=dph[0], EXPLT0=ex, DRHO=dr[0], RHSP=rp[0])
+
+
+def read_ioterms(h: "Harness", name):
+    """``ioterms`` dump: H [3, N, M, L] (x, y, z momentum terms) and DIVU [N, M, L]."""
+    a = h.read_cells(name, nvars=4)
+    return dict(H=a[:3], DIVU=a[3])

@@ -741,3 +741,440 @@ int or_eng_substep(const or_mesh *m, const or_table *tb, const or_eng_par *p, or
     err |= or_eng_walls(m, tb, p, e);
     return err;
 }
+
+/* ======================================================================================
+ * Main-domain momentum substep with variable density and viscosity, explicit viscous scheme
+ * (iVisScheme = 0, the setting of every main-domain template): SOLVERRK3_MOM_io
+ * (SOLVERRK3_MOM_IO.f90:11-120) and the staggered properties SOLVE computes before it in a
+ * heated run (DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered; SOLVE.f90:122-124).
+ * Channel (RCCI1 = RCCI2 = RNDI1 = 1), periodic x and z, no inlet (NXI = 1), one y-domain,
+ * iWeightedPre = 0.  The Poisson solve is FFT99_POIS3D_periodicxz(IIO), the reference's
+ * is_FFT_FFT99 option (SOLVERRK3_MOM_IO.f90:92), with the same gauge as the TG domain.
+ * ====================================================================================== */
+typedef struct {
+    double *g, *g0, *q;      /* [3][N][M+2][L] G_io, G0_io, Q_io                           */
+    double *pr, *dph;        /* [N][M+2][L] PR_io, DPH_io (pressure correction)            */
+    double *d0, *m0;         /* [N][M+2][L] DENSITY0, Viscousity0                          */
+    double *dstg, *mstg;     /* [3][N][M+2][L] D_STG, MU_STG                               */
+    double *explt0;          /* [3][N][M][L] EXPLT0_io                                     */
+    double *drho;            /* [N][M][L] DrhoDtP                                          */
+    double *divu;            /* [N][M][L] DivU_io                                          */
+    double *h[3];            /* [N][M][L] QTMP_io, DPH_io, RHSLLPHI_io as momentum terms   */
+    double *rhsp;            /* [N][M][L] RHSLLPHI_io as the Poisson right-hand side       */
+} or_io;
+
+typedef struct {
+    double cvisc;            /* CVISC = 1/REN                                              */
+    int gravity;             /* |iGravity|: 0 none, else the direction of the buoyancy term */
+    double f_a;              /* F_A (Flow_State.f90:104-123)                               */
+    int flow_driven;         /* iFlowDriven: 1 constant mass flux, 2 constant Cf           */
+    double cf_given;         /* Cf_Given (iFlowDriven = 2)                                 */
+} or_io_par;
+
+#define V3(a, c, i, J, k) (a)[(((size_t)(c) * NN + (k)) * (MM + 2) + (J)) * LL + (i)]
+#define C3(a, c, i, J, k) (a)[(((size_t)(c) * NN + (k)) * MM + (J) - 1) * LL + (i)]
+
+/* BC_WALL_G / BC_WALL_Q on the main domain (BC_WALL.f90:212-290, :435-481): ghost rows 0 and
+ * M+1 zero, v zero on the bottom wall row J = 1. */
+static void or_io_bc_zero(const or_mesh *m, double *a)
+{
+    for (int k = 0; k < NN; ++k)
+        for (int i = 0; i < LL; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                V3(a, c, i, 0, k) = 0.0;
+                V3(a, c, i, MM + 1, k) = 0.0;
+            }
+            V3(a, 1, i, 1, k) = 0.0;
+        }
+}
+
+/* DENSITY_TIME_DERIVATION (DIVG.f90:249-317). */
+void or_io_drhodt(const or_mesh *m, or_io *s, int ns)
+{
+    for (int J = 1; J <= MM; ++J) {
+        const double coe2 = m->dyfi[J - 1], coe3 = m->dzi;
+        for (int i = 0; i < LL; ++i) {
+            const int ip = ip_(m, i);
+            for (int k = 0; k < NN; ++k) {
+                const int kp = kp_(m, k);
+                const double divx = (V3(s->g, 0, ip, J, k) - V3(s->g, 0, i, J, k)) * m->dxi;
+                const double divx0 = (V3(s->g0, 0, ip, J, k) - V3(s->g0, 0, i, J, k)) * m->dxi;
+                const double divy = (V3(s->g, 1, i, J + 1, k) - V3(s->g, 1, i, J, k)) * coe2;
+                const double divy0 = (V3(s->g0, 1, i, J + 1, k) - V3(s->g0, 1, i, J, k)) * coe2;
+                const double divz = (V3(s->g, 2, i, J, kp) - V3(s->g, 2, i, J, k)) * coe3;
+                const double divz0 = (V3(s->g0, 2, i, J, kp) - V3(s->g0, 2, i, J, k)) * coe3;
+                CX(s->drho, i, J, k) = -m->tgam[ns] * (divx + divy + divz) - m->troh[ns] * (divx0 + divy0 + divz0);
+            }
+        }
+    }
+}
+
+/* DENSITY_Staggered (MASSFLUX_CALC_IO.f90:262-323): rho at the u, v, w points from DENSITY0;
+ * v's wall rows take the ghost-row density. */
+void or_io_dstg(const or_mesh *m, or_io *s)
+{
+    const double h = 0.5;   /* XND2CL = ZND2CL (COORDJR.f90:461-463) */
+    for (int k = 0; k < NN; ++k) {
+        const int km = km_(m, k);
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) {
+                V3(s->dstg, 0, i, J, k) = (HX(s->d0, i, J, k) + HX(s->d0, im_(m, i), J, k)) * h;
+                V3(s->dstg, 2, i, J, k) = (HX(s->d0, i, J, k) + HX(s->d0, i, J, km)) * h;
+            }
+        for (int J = 2; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i)
+                V3(s->dstg, 1, i, J, k) = m->wff[J] * HX(s->d0, i, J, k) + m->wfb[J] * HX(s->d0, i, J - 1, k);
+        for (int i = 0; i < LL; ++i) {
+            V3(s->dstg, 1, i, 1, k) = HX(s->d0, i, 0, k);
+            V3(s->dstg, 1, i, MM + 1, k) = HX(s->d0, i, MM + 1, k);
+        }
+    }
+}
+
+/* MU_Staggered (MASSFLUX_CALC_IO.f90:175-259): mu at the (x,y), (x,z) and (y,z) edges from
+ * Viscousity0; the xy and yz edges on the walls take the ghost-row viscosity. */
+void or_io_mustg(const or_mesh *m, or_io *s)
+{
+    const double h = 0.5;
+    for (int k = 0; k < NN; ++k) {
+        const int km = km_(m, k);
+        for (int J = 2; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) {
+                const int im = im_(m, i);
+                V3(s->mstg, 0, i, J, k) = ((HX(s->m0, i, J, k) + HX(s->m0, im, J, k)) * m->wff[J] +
+                                           (HX(s->m0, i, J - 1, k) + HX(s->m0, im, J - 1, k)) * m->wfb[J]) * h;
+                V3(s->mstg, 2, i, J, k) = ((HX(s->m0, i, J, km) + HX(s->m0, i, J, k)) * m->wff[J] +
+                                           (HX(s->m0, i, J - 1, km) + HX(s->m0, i, J - 1, k)) * m->wfb[J]) * h;
+            }
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) {
+                const int im = im_(m, i);
+                V3(s->mstg, 1, i, J, k) = (HX(s->m0, i, J, k) + HX(s->m0, im, J, k) + HX(s->m0, i, J, km) +
+                                           HX(s->m0, im, J, km)) * h * h;
+            }
+        for (int i = 0; i < LL; ++i) {
+            V3(s->mstg, 0, i, 1, k) = HX(s->m0, i, 0, k);
+            V3(s->mstg, 2, i, 1, k) = HX(s->m0, i, 0, k);
+            V3(s->mstg, 0, i, MM + 1, k) = HX(s->m0, i, MM + 1, k);
+            V3(s->mstg, 2, i, MM + 1, k) = HX(s->m0, i, MM + 1, k);
+        }
+    }
+}
+
+/* VELOCITY_CALC_io (MASSFLUX_CALC_IO.f90:55-107): u = G / rho at the staggered points, then
+ * the walls (BC_WALL_Q_io). */
+void or_io_velocity(const or_mesh *m, or_io *s)
+{
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < NN; ++k)
+            for (int J = (c == 1 ? 2 : 1); J <= MM; ++J)
+                for (int i = 0; i < LL; ++i)
+                    V3(s->q, c, i, J, k) = V3(s->g, c, i, J, k) / V3(s->dstg, c, i, J, k);
+    or_io_bc_zero(m, s->q);
+}
+
+/* CONVECTION_X_io / _Y_io / _Z_io (CONVECTION_X.f90:67-156, CONVECTION_Y.f90:117-272,
+ * CONVECTION_Z.f90:106-303): the divergence-form terms with G as the advecting flux. */
+void or_io_convection(const or_mesh *m, or_io *s)
+{
+    const double h = 0.5;
+    const double cx1 = m->dxi * h * h, cx3 = m->dzi * h * h;
+    const double cy1 = m->dxi * h, cy30 = m->dzi * h;
+    const double cz1 = m->dxi * h * h, cz30 = m->dzi * h * h;
+    const double *G = s->g, *Q = s->q;
+    for (int J = 1; J <= MM; ++J) {
+        const double wffp = m->wff[J + 1], wfbp = m->wfb[J + 1], wff = m->wff[J], wfb = m->wfb[J];
+        const double cx2 = m->dyfi[J - 1] * h, cy2 = m->dyci[J] * h * h, cz2 = m->dyfi[J - 1] * h;
+        for (int k = 0; k < NN; ++k) {
+            const int kp = kp_(m, k), km = km_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                /* x-momentum at (i', J, K) */
+                double f = (V3(G, 0, ip, J, k) + V3(G, 0, i, J, k)) * (V3(Q, 0, ip, J, k) + V3(Q, 0, i, J, k));
+                double b = (V3(G, 0, i, J, k) + V3(G, 0, im, J, k)) * (V3(Q, 0, i, J, k) + V3(Q, 0, im, J, k));
+                const double h11 = (f - b) * cx1;
+                f = (V3(G, 1, i, J + 1, k) + V3(G, 1, im, J + 1, k)) *
+                    (wffp * V3(Q, 0, i, J + 1, k) + wfbp * V3(Q, 0, i, J, k));
+                b = (V3(G, 1, i, J, k) + V3(G, 1, im, J, k)) * (wff * V3(Q, 0, i, J, k) + wfb * V3(Q, 0, i, J - 1, k));
+                const double h12 = (f - b) * cx2;
+                f = (V3(G, 2, i, J, kp) + V3(G, 2, im, J, kp)) * (V3(Q, 0, i, J, kp) + V3(Q, 0, i, J, k));
+                b = (V3(G, 2, i, J, k) + V3(G, 2, im, J, k)) * (V3(Q, 0, i, J, k) + V3(Q, 0, i, J, km));
+                const double h13 = (f - b) * cx3;
+                CX(s->h[0], i, J, k) = -(h11 + h12 + h13);
+                /* y-momentum at (i, j', K), j' >= 2 */
+                if (J >= 2) {
+                    f = (wff * V3(G, 0, ip, J, k) + wfb * V3(G, 0, ip, J - 1, k)) * (V3(Q, 1, ip, J, k) + V3(Q, 1, i, J, k));
+                    b = (wff * V3(G, 0, i, J, k) + wfb * V3(G, 0, i, J - 1, k)) * (V3(Q, 1, i, J, k) + V3(Q, 1, im, J, k));
+                    const double h21 = (f - b) * cy1;
+                    f = (V3(G, 1, i, J + 1, k) + V3(G, 1, i, J, k)) * (V3(Q, 1, i, J + 1, k) + V3(Q, 1, i, J, k));
+                    b = (V3(G, 1, i, J, k) + V3(G, 1, i, J - 1, k)) * (V3(Q, 1, i, J, k) + V3(Q, 1, i, J - 1, k));
+                    const double h22 = (f - b) * cy2;
+                    f = (wff * V3(G, 2, i, J, kp) + wfb * V3(G, 2, i, J - 1, kp)) * (V3(Q, 1, i, J, kp) + V3(Q, 1, i, J, k));
+                    b = (wff * V3(G, 2, i, J, k) + wfb * V3(G, 2, i, J - 1, k)) * (V3(Q, 1, i, J, k) + V3(Q, 1, i, J, km));
+                    const double h23 = (f - b) * cy30;
+                    CX(s->h[1], i, J, k) = -(h21 + h22 + h23);
+                } else {
+                    CX(s->h[1], i, J, k) = 0.0;
+                }
+                /* z-momentum at (i, J, k') */
+                f = (V3(G, 0, ip, J, k) + V3(G, 0, ip, J, km)) * (V3(Q, 2, ip, J, k) + V3(Q, 2, i, J, k));
+                b = (V3(G, 0, i, J, k) + V3(G, 0, i, J, km)) * (V3(Q, 2, i, J, k) + V3(Q, 2, im, J, k));
+                const double h31 = (f - b) * cz1;
+                f = (V3(G, 1, i, J + 1, k) + V3(G, 1, i, J + 1, km)) *
+                    (wffp * V3(Q, 2, i, J + 1, k) + wfbp * V3(Q, 2, i, J, k));
+                b = (V3(G, 1, i, J, k) + V3(G, 1, i, J, km)) * (wff * V3(Q, 2, i, J, k) + wfb * V3(Q, 2, i, J - 1, k));
+                const double h32 = (f - b) * cz2;
+                f = (V3(G, 2, i, J, kp) + V3(G, 2, i, J, k)) * (V3(Q, 2, i, J, kp) + V3(Q, 2, i, J, k));
+                b = (V3(G, 2, i, J, k) + V3(G, 2, i, J, km)) * (V3(Q, 2, i, J, k) + V3(Q, 2, i, J, km));
+                const double h33 = (f - b) * cz30;
+                CX(s->h[2], i, J, k) = -(h31 + h32 + h33);
+            }
+        }
+    }
+}
+
+/* DIVG_U_io (DIVG.f90:140-246): div(u) / 3 at the cell centres. */
+void or_io_divu(const or_mesh *m, or_io *s)
+{
+    for (int J = 1; J <= MM; ++J) {
+        const double coe2 = m->dyfi[J - 1], coe3 = m->dzi;
+        for (int k = 0; k < NN; ++k) {
+            const int kp = kp_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const double dx = (V3(s->q, 0, ip_(m, i), J, k) - V3(s->q, 0, i, J, k)) * m->dxi;
+                const double dy = (V3(s->q, 1, i, J + 1, k) - V3(s->q, 1, i, J, k)) * coe2;
+                const double dz = (V3(s->q, 2, i, J, kp) - V3(s->q, 2, i, J, k)) * coe3;
+                CX(s->divu, i, J, k) = (dx + dy + dz) / 3.0;
+            }
+        }
+    }
+}
+
+/* VISCOUS_ALL_EXPLT_X_io / _Y_io / _Z_io, channel branch (VISCOUS_X_EXPLICIT_io.f90:13-128,
+ * VISCOUS_Y_EXPLICIT_io.f90:14-120, VISCOUS_Z_EXPLICIT_io.f90:13-115): the divergence of the
+ * full variable-viscosity stress, added to the convection terms. */
+void or_io_viscous(const or_mesh *m, const or_io_par *p, or_io *s)
+{
+    const double cv = p->cvisc, dxi = m->dxi, dzi = m->dzi;
+    const double *Q = s->q, *MU = s->mstg, *V = s->m0;
+    for (int J = 1; J <= MM; ++J) {
+        for (int k = 0; k < NN; ++k) {
+            const int kp = kp_(m, k), km = km_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                /* x: VISCOUS_X_EXPLICIT_io.f90:44-120 */
+                {
+                    const double coe1 = 2.0 * dxi * cv, coe2 = m->dyfi[J - 1] * cv, coe3 = dzi * cv;
+                    double dudx = (V3(Q, 0, ip, J, k) - V3(Q, 0, i, J, k)) * dxi;
+                    const double t11f = 0.5 * (HX(V, i, J, k) + HX(V, i, J, k)) * (dudx - CX(s->divu, i, J, k));
+                    dudx = (V3(Q, 0, i, J, k) - V3(Q, 0, im, J, k)) * dxi;
+                    const double t11b = 0.5 * (HX(V, im, J, k) + HX(V, im, J, k)) * (dudx - CX(s->divu, im, J, k));
+                    const double d11 = (t11f - t11b) * coe1;
+                    double dudy = (V3(Q, 0, i, J + 1, k) - V3(Q, 0, i, J, k)) * m->dyci[J + 1];
+                    double dvdx = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, im, J + 1, k)) * dxi;
+                    const double t12f = V3(MU, 0, i, J + 1, k) * (dudy + dvdx);
+                    dudy = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J - 1, k)) * m->dyci[J];
+                    dvdx = (V3(Q, 1, i, J, k) - V3(Q, 1, im, J, k)) * dxi;
+                    const double t12b = V3(MU, 0, i, J, k) * (dudy + dvdx);
+                    const double d12 = (t12f - t12b) * coe2;
+                    double dudz = (V3(Q, 0, i, J, kp) - V3(Q, 0, i, J, k)) * dzi;
+                    double dwdx = (V3(Q, 2, i, J, kp) - V3(Q, 2, im, J, kp)) * dxi;
+                    const double t13f = V3(MU, 1, i, J, kp) * (dudz + dwdx);
+                    dudz = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J, km)) * dzi;
+                    dwdx = (V3(Q, 2, i, J, k) - V3(Q, 2, im, J, k)) * dxi;
+                    const double t13b = V3(MU, 1, i, J, k) * (dudz + dwdx);
+                    const double d13 = (t13f - t13b) * coe3;
+                    CX(s->h[0], i, J, k) = CX(s->h[0], i, J, k) + d11 + d12 + d13;
+                }
+                /* y (rows 2..M): VISCOUS_Y_EXPLICIT_io.f90:47-113 */
+                if (J >= 2) {
+                    const double coe1 = dxi * cv, coe2 = 2.0 * m->dyci[J] * cv, coe3 = dzi * cv;
+                    double dvdy = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, i, J, k)) * m->dyfi[J - 1];
+                    const double t22f = 0.5 * (HX(V, i, J, k) + HX(V, i, J, k)) * (dvdy - CX(s->divu, i, J, k));
+                    dvdy = (V3(Q, 1, i, J, k) - V3(Q, 1, i, J - 1, k)) * m->dyfi[J - 2];
+                    const double t22b = 0.5 * (HX(V, i, J - 1, k) + HX(V, i, J - 1, k)) * (dvdy - CX(s->divu, i, J - 1, k));
+                    const double d22 = (t22f - t22b) * coe2;
+                    double dvdx = (V3(Q, 1, ip, J, k) - V3(Q, 1, i, J, k)) * dxi;
+                    double dudy = (V3(Q, 0, ip, J, k) - V3(Q, 0, ip, J - 1, k)) * m->dyci[J];
+                    const double t21f = V3(MU, 0, ip, J, k) * (dvdx + dudy);
+                    dvdx = (V3(Q, 1, i, J, k) - V3(Q, 1, im, J, k)) * dxi;
+                    dudy = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J - 1, k)) * m->dyci[J];
+                    const double t21b = V3(MU, 0, i, J, k) * (dvdx + dudy);
+                    const double d21 = (t21f - t21b) * coe1;
+                    double dvdz = (V3(Q, 1, i, J, kp) - V3(Q, 1, i, J, k)) * dzi;
+                    double dwdy = (V3(Q, 2, i, J, kp) - V3(Q, 2, i, J - 1, kp)) * m->dyci[J];
+                    const double t23f = V3(MU, 2, i, J, kp) * (dvdz + dwdy);
+                    dvdz = (V3(Q, 1, i, J, k) - V3(Q, 1, i, J, km)) * dzi;
+                    dwdy = (V3(Q, 2, i, J, k) - V3(Q, 2, i, J - 1, k)) * m->dyci[J];
+                    const double t23b = V3(MU, 2, i, J, k) * (dvdz + dwdy);
+                    const double d23 = (t23f - t23b) * coe3;
+                    CX(s->h[1], i, J, k) = CX(s->h[1], i, J, k) + d21 + d22 + d23;
+                }
+                /* z: VISCOUS_Z_EXPLICIT_io.f90:44-107 */
+                {
+                    const double coe1 = dxi * cv, coe2 = m->dyfi[J - 1] * cv, coe3 = 2.0 * dzi * cv;
+                    double dwdz = (V3(Q, 2, i, J, kp) - V3(Q, 2, i, J, k)) * dzi;
+                    const double t33f = 0.5 * (HX(V, i, J, k) + HX(V, i, J, k)) * (dwdz - CX(s->divu, i, J, k));
+                    dwdz = (V3(Q, 2, i, J, k) - V3(Q, 2, i, J, km)) * dzi;
+                    const double t33b = 0.5 * (HX(V, i, J, km) + HX(V, i, J, km)) * (dwdz - CX(s->divu, i, J, km));
+                    const double d33 = (t33f - t33b) * coe3;
+                    double dwdx = (V3(Q, 2, ip, J, k) - V3(Q, 2, i, J, k)) * dxi;
+                    double dudz = (V3(Q, 0, ip, J, k) - V3(Q, 0, ip, J, km)) * dzi;
+                    const double t31f = V3(MU, 1, ip, J, k) * (dwdx + dudz);
+                    dwdx = (V3(Q, 2, i, J, k) - V3(Q, 2, im, J, k)) * dxi;
+                    dudz = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J, km)) * dzi;
+                    const double t31b = V3(MU, 1, i, J, k) * (dwdx + dudz);
+                    const double d31 = (t31f - t31b) * coe1;
+                    double dwdy = (V3(Q, 2, i, J + 1, k) - V3(Q, 2, i, J, k)) * m->dyci[J + 1];
+                    double dvdz = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, i, J + 1, km)) * dzi;
+                    const double t32f = V3(MU, 2, i, J + 1, k) * (dwdy + dvdz);
+                    dwdy = (V3(Q, 2, i, J, k) - V3(Q, 2, i, J - 1, k)) * m->dyci[J];
+                    dvdz = (V3(Q, 1, i, J, k) - V3(Q, 1, i, J, km)) * dzi;
+                    const double t32b = V3(MU, 2, i, J, k) * (dwdy + dvdz);
+                    const double d32 = (t32f - t32b) * coe2;
+                    CX(s->h[2], i, J, k) = CX(s->h[2], i, J, k) + d31 + d32 + d33;
+                }
+            }
+        }
+    }
+}
+
+/* RHS_MOM_EXPLICIT_io(NS, IDR), explicit branch (RHS_MOM_EXPLICIT_io.f90:11-304): RK
+ * combination with the previous substep's terms, pressure gradient, buoyancy, and for the
+ * streamwise component the mass-flux (or given-Cf) correction; the result replaces the
+ * momentum term (the increment of G). */
+void or_io_rhs(const or_mesh *m, const or_io_par *p, or_io *s, int ns, int idr, double dt)
+{
+    const int c = idr - 1, j0 = idr == 2 ? 2 : 1;
+    double *H = s->h[c];
+    for (int k = 0; k < NN; ++k)
+        for (int J = j0; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) {
+                const double rc = CX(H, i, J, k), rl = C3(s->explt0, c, i, J, k);
+                CX(H, i, J, k) = (m->tgam[ns] * rc + m->troh[ns] * rl) * dt;
+                C3(s->explt0, c, i, J, k) = rc;
+            }
+    for (int k = 0; k < NN; ++k)
+        for (int J = j0; J <= MM; ++J) {
+            double coe2;
+            if (idr == 1) coe2 = m->talp[ns] * dt * m->dxi;
+            else if (idr == 2) coe2 = m->talp[ns] * dt * m->dyci[J];
+            else coe2 = m->talp[ns] * dt * m->dzi;
+            for (int i = 0; i < LL; ++i) {
+                double pgm;
+                if (idr == 1) pgm = (HX(s->pr, i, J, k) - HX(s->pr, im_(m, i), J, k)) * coe2;
+                else if (idr == 2) pgm = (HX(s->pr, i, J, k) - HX(s->pr, i, J - 1, k)) * coe2;
+                else pgm = (HX(s->pr, i, J, k) - HX(s->pr, i, J, km_(m, k))) * coe2;
+                CX(H, i, J, k) = CX(H, i, J, k) - pgm;
+            }
+        }
+    if (p->gravity == idr) {
+        const double coe2 = m->talp[ns] * dt * p->f_a;
+        for (int k = 0; k < NN; ++k)
+            for (int J = j0; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) CX(H, i, J, k) = CX(H, i, J, k) + V3(s->dstg, c, i, J, k) * coe2;
+    }
+    if (idr == 1) {   /* NFLOW = 1 (modules.f90:45) */
+        double dpgrns = 0.0;
+        if (p->flow_driven == 1) {
+            double sum = 0.0;
+            for (int J = 1; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i)
+                    for (int k = 0; k < NN; ++k) sum = sum + CX(H, i, J, k) / m->dyfi[J - 1];
+            dpgrns = sum / m->volm;
+        } else if (p->flow_driven == 2) {
+            dpgrns = -0.5 * p->cf_given * (m->talp[ns] * dt);
+        }
+        for (int k = 0; k < NN; ++k)
+            for (int J = 1; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) CX(H, i, J, k) = CX(H, i, J, k) - dpgrns;
+    }
+}
+
+/* MASSFLUX_CALC_io (MASSFLUX_CALC_IO.f90:15-52): G += increments; then the walls. */
+void or_io_massflux_calc(const or_mesh *m, or_io *s)
+{
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < NN; ++k)
+            for (int J = (c == 1 ? 2 : 1); J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) V3(s->g, c, i, J, k) = V3(s->g, c, i, J, k) + CX(s->h[c], i, J, k);
+    or_io_bc_zero(m, s->g);
+}
+
+/* DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: (div G + d rho/dt) / (alpha dt). */
+void or_io_divg(const or_mesh *m, or_io *s, int ns, double dt)
+{
+    const double coe0 = 1.0 / dt / m->talp[ns];
+    const double coe4 = 1.0 / dt;
+    (void)coe4;
+    for (int J = 1; J <= MM; ++J) {
+        const double coe1 = m->dxi, coe2 = m->dyfi[J - 1];
+        for (int k = 0; k < NN; ++k) {
+            const int kp = kp_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const double divx = (V3(s->g, 0, ip_(m, i), J, k) - V3(s->g, 0, i, J, k)) * coe1;
+                const double divy = (V3(s->g, 1, i, J + 1, k) - V3(s->g, 1, i, J, k)) * coe2;
+                const double divz = (V3(s->g, 2, i, J, kp) - V3(s->g, 2, i, J, k)) * m->dzi;
+                const double dq = divx + divy + divz + CX(s->drho, i, J, k);
+                CX(s->rhsp, i, J, k) = dq * coe0;
+            }
+        }
+    }
+}
+
+/* PRCALC_io explicit branch (PRESSURE_CORRECTION.f90:102-170) and MASSFLUX_UPDATE_io
+ * (VELO_CORRECTION.f90:103-179): p += phi; G -= alpha dt grad(phi); walls. */
+void or_io_correct(const or_mesh *m, or_io *s, int ns, double dt)
+{
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) HX(s->pr, i, J, k) = HX(s->pr, i, J, k) + HX(s->dph, i, J, k);
+    or_bc_copy(m, s->pr);
+    double coe1 = dt * m->talp[ns] * m->dxi;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i)
+                V3(s->g, 0, i, J, k) = V3(s->g, 0, i, J, k) - (HX(s->dph, i, J, k) - HX(s->dph, im_(m, i), J, k)) * coe1;
+    coe1 = dt * m->talp[ns] * m->dzi;
+    for (int k = 0; k < NN; ++k)
+        for (int J = 1; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i)
+                V3(s->g, 2, i, J, k) = V3(s->g, 2, i, J, k) - (HX(s->dph, i, J, k) - HX(s->dph, i, J, km_(m, k))) * coe1;
+    for (int J = 2; J <= MM; ++J) {
+        coe1 = dt * m->talp[ns] * m->dyci[J];
+        for (int k = 0; k < NN; ++k)
+            for (int i = 0; i < LL; ++i)
+                V3(s->g, 1, i, J, k) = V3(s->g, 1, i, J, k) - (HX(s->dph, i, J, k) - HX(s->dph, i, J - 1, k)) * coe1;
+    }
+    or_io_bc_zero(m, s->g);
+}
+
+/* What SOLVE does between the energy and the momentum substep of stage NS in a heated run
+ * (SOLVE.f90:122-124). */
+void or_io_props(const or_mesh *m, or_io *s, int ns)
+{
+    or_io_drhodt(m, s, ns);
+    or_io_dstg(m, s);
+    or_io_mustg(m, s);
+}
+
+/* SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:11-120), explicit viscous scheme. */
+void or_io_substep(const or_mesh *m, const or_io_par *p, or_io *s, int ns, double dt)
+{
+    const size_t hs = (size_t)NN * (MM + 2) * LL;
+    memcpy(s->g0, s->g, sizeof(double) * 3 * hs);
+    or_io_velocity(m, s);
+    or_io_convection(m, s);
+    or_io_divu(m, s);
+    or_io_viscous(m, p, s);
+    for (int idr = 1; idr <= 3; ++idr) or_io_rhs(m, p, s, ns, idr, dt);
+    or_io_massflux_calc(m, s);
+    or_io_divg(m, s, ns, dt);
+    memset(s->dph, 0, sizeof(double) * hs);
+    or_poisson(m, s->rhsp, s->dph);
+    or_bc_copy(m, s->dph);
+    or_io_correct(m, s, ns, dt);
+    or_io_velocity(m, s);
+}
+#undef V3
+#undef C3

@@ -12,9 +12,10 @@ Arrays are C-order views of the reference's Fortran layout (z, y, x).
 
 Also ``props_2048.npz`` (property tables, make_props), ``tdma.npz`` (the reference's
 batched Thomas solvers, make_tdma) and ``fishpack.npz`` (the main-domain FISHPACK Poisson
-solve, make_fishpack) and ``eng.npz`` (the energy substep, make_eng).
+solve, make_fishpack), ``eng.npz`` (the energy substep, make_eng) and ``io.npz`` (the
+main-domain momentum substep, make_io).
 
-Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng]   (needs oracle/_ref)
+Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng] [io]   (needs oracle/_ref)
 """
 from __future__ import annotations
 
@@ -213,9 +214,60 @@ def make_eng(T0=665.0):
     print("wrote", path, os.path.getsize(path))
 
 
+IO_CASES = [  # L, M, N, STR2, REN, dt, iGravity, iFlowDriven, seed
+    (16, 12, 8, 1.8, 2800.0, 1.0e-3, 0, 1, 61),
+    (12, 10, 16, 2.0, 4000.0, 5.0e-4, 1, 2, 62),
+]
+
+
+def make_io(T0=665.0):
+    """Main-domain momentum fixture (SOLVERRK3_MOM_io, explicit viscous scheme, FFT99 Poisson
+    solve): per case the seeded state, the reference's F_A / VOLM_io / CVISC, the staggered
+    properties and the convection + viscous terms of stage 1, and the state after each of the
+    substeps NS = 1, 2, 3 (each preceded by the staggered-property update SOLVE does)."""
+    sys.path.insert(0, os.path.join(ROOT))
+    import __graft_entry__
+    __graft_entry__.load()
+    from chapsim_amd import thermo
+    rows = thermo.synthetic_table()
+    out = {}
+    for L, M, N, STR2, REN, dt, grav, fd, seed in IO_CASES:
+        mesh = setup.Mesh(L, M, N, STR2=STR2)
+        st = oracle.io_initial_field(mesh, seed)
+        h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt, iDomain=2, iThermoDynamics=1, T0=T0, Ti=T0,
+                           iVisScheme=0, iGravity=grav, iFlowDriven=fd)
+        try:
+            thermo.write_table_file(h.path("NIST_WATER_23.5MP.DAT"), rows)
+            refrun.write_io(h, "io.bin", st["G"], st["G0"], st["PR"], st["D0"], st["M0"], st["EXPLT0"])
+            h.run(["thermoinitall", "dumpiopar par.bin", "loadio io.bin", "ioprops 1", "ioterms t1", "dumpio p1",
+                   f"iosubstep 1 {dt!r}", "dumpio s1", "ioprops 2", f"iosubstep 2 {dt!r}", "dumpio s2",
+                   "ioprops 3", f"iosubstep 3 {dt!r}", "dumpio s3"])
+            par = np.fromfile(h.path("par.bin"), dtype=np.float64)
+            p1 = refrun.read_io(h, "p1")
+            t1 = refrun.read_ioterms(h, "t1")
+            ss = {s: refrun.read_io(h, s) for s in ("s1", "s2", "s3")}
+        finally:
+            h.close()
+        tag = f"io{L}x{M}x{N}"
+        out[f"{tag}_params"] = np.array([L, M, N, STR2, REN, dt, grav, fd, seed, T0])
+        out[f"{tag}_par"] = par
+        for k, v in st.items():
+            out[f"{tag}_init_{k}"] = v
+        for k in ("DSTG", "MSTG", "DRHO"):
+            out[f"{tag}_p1_{k}"] = p1[k]
+        out[f"{tag}_t1_H"] = t1["H"]
+        out[f"{tag}_t1_DIVU"] = t1["DIVU"]
+        for s, r in ss.items():
+            for k in ("G", "Q", "PR", "DPH", "DRHO", "RHSP", "EXPLT0"):
+                out[f"{tag}_{s}_{k}"] = r[k]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "io.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
     import sys as _sys
-    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng"]
+    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io"]
     if "tg" in which:
         for case in CASES:
             make_case(*case)
@@ -227,3 +279,5 @@ if __name__ == "__main__":
         make_fishpack()
     if "eng" in which:
         make_eng()
+    if "io" in which:
+        make_io()
