@@ -1,0 +1,603 @@
+// Main-domain momentum substep with variable density and viscosity (SOLVERRK3_MOM_io,
+// SOLVERRK3_MOM_IO.f90:11-120, explicit viscous scheme) and the staggered properties SOLVE
+// computes before it in a heated run (DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered;
+// SOLVE.f90:122-124), for the periodic plane channel (RCCI1 = RCCI2 = RNDI1 = 1, no inlet,
+// iWeightedPre = 0).  The pressure Poisson solve is the FFT99 one of a chs_ctx on the same
+// mesh (FFT99_POIS3D_periodicxz(IIO), the reference's is_FFT_FFT99 option).
+//
+// Every expression keeps the reference's operation order with FMA contraction off, so the
+// results follow the reference's rounding.  State stays in HBM; fields use the reference's
+// layout: halo fields [N][M+2][L] (x fastest, y rows 0..M+1), cell fields [N][M][L].
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/chapsim_hip.h"
+
+hipStream_t chs_internal_stream(chs_ctx *c);
+void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **g);
+
+namespace {
+
+thread_local std::string g_io_err;
+
+struct IoDev {
+    int L, M, N;
+    double dxi, dzi, cvisc, f_a;
+    int gravity;
+    const double *dyfi, *dyci, *wff, *wfb;    // indexed by the reference's J (dyfi 1..M, others 1..M+1)
+    double *g, *g0, *q, *dstg, *mstg;         // 3 halo fields each
+    double *pr, *d0, *m0;                     // halo fields
+    double *explt0, *h;                       // 3 cell fields each
+    double *drho, *divu, *rhsp, *dph;         // cell fields
+    double *partial, *scal;
+};
+
+__device__ __forceinline__ size_t hx(const IoDev &d, int i, int J, int k) { return ((size_t)k * (d.M + 2) + J) * d.L + i; }
+__device__ __forceinline__ size_t cx(const IoDev &d, int i, int J, int k) { return ((size_t)k * d.M + (J - 1)) * d.L + i; }
+__device__ __forceinline__ size_t hsz(const IoDev &d) { return (size_t)d.N * (d.M + 2) * d.L; }
+__device__ __forceinline__ size_t csz(const IoDev &d) { return (size_t)d.N * d.M * d.L; }
+
+// Thread (i, J = j0 + row, k) of a 64 x 4 block on a (ceil(L/64), ceil(rows/4), N) grid.
+#define IO_THREAD(j0)                                        \
+    const int i = blockIdx.x * 64 + threadIdx.x;             \
+    const int J = (j0) + (int)(blockIdx.y * 4 + threadIdx.y); \
+    const int k = blockIdx.z;                                \
+    const int L = d.L, M = d.M, N = d.N;                     \
+    (void)N;
+
+// DENSITY_TIME_DERIVATION (DIVG.f90:249-317), DENSITY_Staggered (MASSFLUX_CALC_IO.f90:262-323)
+// and MU_Staggered (:175-259), rows J = 1..M+1.
+__global__ __launch_bounds__(256) void k_io_props(IoDev d, double tg, double tr)
+{
+#pragma clang fp contract(off)
+    IO_THREAD(1)
+    if (i >= L || J > M + 1) return;
+    const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+    const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
+    const size_t hs = hsz(d);
+    const double h = 0.5;
+    const double *D = d.d0, *V = d.m0;
+    const size_t c0 = hx(d, i, J, k);
+    if (J <= M) {
+        const double *G = d.g, *G0 = d.g0;
+        const double coe2 = d.dyfi[J], coe3 = d.dzi;
+        const double divx = (G[hx(d, ip, J, k)] - G[c0]) * d.dxi;
+        const double divx0 = (G0[hx(d, ip, J, k)] - G0[c0]) * d.dxi;
+        const double divy = (G[hs + hx(d, i, J + 1, k)] - G[hs + c0]) * coe2;
+        const double divy0 = (G0[hs + hx(d, i, J + 1, k)] - G0[hs + c0]) * coe2;
+        const double divz = (G[2 * hs + hx(d, i, J, kp)] - G[2 * hs + c0]) * coe3;
+        const double divz0 = (G0[2 * hs + hx(d, i, J, kp)] - G0[2 * hs + c0]) * coe3;
+        d.drho[cx(d, i, J, k)] = -tg * (divx + divy + divz) - tr * (divx0 + divy0 + divz0);
+        d.dstg[c0] = (D[c0] + D[hx(d, im, J, k)]) * h;
+        d.dstg[2 * hs + c0] = (D[c0] + D[hx(d, i, J, km)]) * h;
+        d.mstg[hs + c0] = (V[c0] + V[hx(d, im, J, k)] + V[hx(d, i, J, km)] + V[hx(d, im, J, km)]) * h * h;
+    }
+    if (J == 1) {
+        d.dstg[hs + c0] = D[hx(d, i, 0, k)];
+        d.mstg[c0] = V[hx(d, i, 0, k)];
+        d.mstg[2 * hs + c0] = V[hx(d, i, 0, k)];
+    } else if (J == M + 1) {
+        d.dstg[hs + c0] = D[c0];
+        d.mstg[c0] = V[c0];
+        d.mstg[2 * hs + c0] = V[c0];
+    } else {
+        const size_t b0 = hx(d, i, J - 1, k);
+        d.dstg[hs + c0] = d.wff[J] * D[c0] + d.wfb[J] * D[b0];
+        d.mstg[c0] = ((V[c0] + V[hx(d, im, J, k)]) * d.wff[J] + (V[b0] + V[hx(d, im, J - 1, k)]) * d.wfb[J]) * h;
+        d.mstg[2 * hs + c0] = ((V[hx(d, i, J, km)] + V[c0]) * d.wff[J] + (V[hx(d, i, J - 1, km)] + V[b0]) * d.wfb[J]) * h;
+    }
+}
+
+// VELOCITY_CALC_io (MASSFLUX_CALC_IO.f90:55-107) with BC_WALL_Q_io: rows J = 0..M+1.
+__global__ __launch_bounds__(256) void k_io_velocity(IoDev d)
+{
+#pragma clang fp contract(off)
+    IO_THREAD(0)
+    if (i >= L || J > M + 1) return;
+    const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = c * hs + c0;
+        d.q[o] = (J == 0 || J == M + 1 || (c == 1 && J == 1)) ? 0.0 : d.g[o] / d.dstg[o];
+    }
+}
+
+// DIVG_U_io (DIVG.f90:140-246): rows 1..M.
+__global__ __launch_bounds__(256) void k_io_divu(IoDev d)
+{
+#pragma clang fp contract(off)
+    IO_THREAD(1)
+    if (i >= L || J > M) return;
+    const int ip = i == L - 1 ? 0 : i + 1, kp = k == N - 1 ? 0 : k + 1;
+    const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+    const double *Q = d.q;
+    const double dx = (Q[hx(d, ip, J, k)] - Q[c0]) * d.dxi;
+    const double dy = (Q[hs + hx(d, i, J + 1, k)] - Q[hs + c0]) * d.dyfi[J];
+    const double dz = (Q[2 * hs + hx(d, i, J, kp)] - Q[2 * hs + c0]) * d.dzi;
+    d.divu[cx(d, i, J, k)] = (dx + dy + dz) / 3.0;
+}
+
+// Block sum (256 threads) in a fixed order; thread 0 gets the result.
+__device__ double block_sum256(double v, double *sh)
+{
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// One momentum component C (0 x, 1 y, 2 z): convection (CONVECTION_X/Y/Z_io), viscous stress
+// divergence (VISCOUS_ALL_EXPLT_X/Y/Z_io, channel branch) and RHS_MOM_EXPLICIT_io(NS, C+1),
+// explicit branch: RK combination, EXPLT0 update, pressure gradient, buoyancy.  Rows 1..M
+// (2..M for v).  For C = 0 each block also writes its partial of sum(RHS / DYFI) for the
+// mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265).
+template <int C>
+__global__ __launch_bounds__(256) void k_io_mom(IoDev d, double tg, double tr, double ta, double dt)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[256];
+    IO_THREAD(1)
+    const bool on = i < L && J <= M && !(C == 1 && J == 1);
+    double contrib = 0.0;
+    if (on) {
+        const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+        const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
+        const size_t hs = hsz(d);
+        const double *G0 = d.g, *G1 = d.g + hs, *G2 = d.g + 2 * hs;
+        const double *Q0 = d.q, *Q1 = d.q + hs, *Q2 = d.q + 2 * hs;
+        const double *V = d.m0, *DU = d.divu;
+        const double hf = 0.5, cv = d.cvisc, dxi = d.dxi, dzi = d.dzi;
+        const double wffp = d.wff[J + 1], wfbp = d.wfb[J + 1], wff = d.wff[J], wfb = d.wfb[J];
+        auto H = [&](int ii, int jj, int kk) { return hx(d, ii, jj, kk); };
+        auto CI = [&](int ii, int jj, int kk) { return cx(d, ii, jj, kk); };
+        double acc, pgm;
+        if constexpr (C == 0) {
+            const double c1 = dxi * hf * hf, c2 = d.dyfi[J] * hf, c3 = dzi * hf * hf;
+            double f = (G0[H(ip, J, k)] + G0[H(i, J, k)]) * (Q0[H(ip, J, k)] + Q0[H(i, J, k)]);
+            double b = (G0[H(i, J, k)] + G0[H(im, J, k)]) * (Q0[H(i, J, k)] + Q0[H(im, J, k)]);
+            const double h11 = (f - b) * c1;
+            f = (G1[H(i, J + 1, k)] + G1[H(im, J + 1, k)]) * (wffp * Q0[H(i, J + 1, k)] + wfbp * Q0[H(i, J, k)]);
+            b = (G1[H(i, J, k)] + G1[H(im, J, k)]) * (wff * Q0[H(i, J, k)] + wfb * Q0[H(i, J - 1, k)]);
+            const double h12 = (f - b) * c2;
+            f = (G2[H(i, J, kp)] + G2[H(im, J, kp)]) * (Q0[H(i, J, kp)] + Q0[H(i, J, k)]);
+            b = (G2[H(i, J, k)] + G2[H(im, J, k)]) * (Q0[H(i, J, k)] + Q0[H(i, J, km)]);
+            const double h13 = (f - b) * c3;
+            acc = -(h11 + h12 + h13);
+            const double v1 = 2.0 * dxi * cv, v2 = d.dyfi[J] * cv, v3 = dzi * cv;
+            double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
+            const double t11f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dudx - DU[CI(i, J, k)]);
+            dudx = (Q0[H(i, J, k)] - Q0[H(im, J, k)]) * dxi;
+            const double t11b = 0.5 * (V[H(im, J, k)] + V[H(im, J, k)]) * (dudx - DU[CI(im, J, k)]);
+            const double d11 = (t11f - t11b) * v1;
+            double dudy = (Q0[H(i, J + 1, k)] - Q0[H(i, J, k)]) * d.dyci[J + 1];
+            double dvdx = (Q1[H(i, J + 1, k)] - Q1[H(im, J + 1, k)]) * dxi;
+            const double t12f = d.mstg[H(i, J + 1, k)] * (dudy + dvdx);
+            dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
+            dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
+            const double t12b = d.mstg[H(i, J, k)] * (dudy + dvdx);
+            const double d12 = (t12f - t12b) * v2;
+            double dudz = (Q0[H(i, J, kp)] - Q0[H(i, J, k)]) * dzi;
+            double dwdx = (Q2[H(i, J, kp)] - Q2[H(im, J, kp)]) * dxi;
+            const double t13f = d.mstg[hs + H(i, J, kp)] * (dudz + dwdx);
+            dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
+            dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
+            const double t13b = d.mstg[hs + H(i, J, k)] * (dudz + dwdx);
+            const double d13 = (t13f - t13b) * v3;
+            acc = acc + d11 + d12 + d13;
+            pgm = (d.pr[H(i, J, k)] - d.pr[H(im, J, k)]) * (ta * dt * dxi);
+        } else if constexpr (C == 1) {
+            const double c1 = dxi * hf, c2 = d.dyci[J] * hf * hf, c3 = dzi * hf;
+            double f = (wff * G0[H(ip, J, k)] + wfb * G0[H(ip, J - 1, k)]) * (Q1[H(ip, J, k)] + Q1[H(i, J, k)]);
+            double b = (wff * G0[H(i, J, k)] + wfb * G0[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(im, J, k)]);
+            const double h21 = (f - b) * c1;
+            f = (G1[H(i, J + 1, k)] + G1[H(i, J, k)]) * (Q1[H(i, J + 1, k)] + Q1[H(i, J, k)]);
+            b = (G1[H(i, J, k)] + G1[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(i, J - 1, k)]);
+            const double h22 = (f - b) * c2;
+            f = (wff * G2[H(i, J, kp)] + wfb * G2[H(i, J - 1, kp)]) * (Q1[H(i, J, kp)] + Q1[H(i, J, k)]);
+            b = (wff * G2[H(i, J, k)] + wfb * G2[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(i, J, km)]);
+            const double h23 = (f - b) * c3;
+            acc = -(h21 + h22 + h23);
+            const double v1 = dxi * cv, v2 = 2.0 * d.dyci[J] * cv, v3 = dzi * cv;
+            double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
+            const double t22f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dvdy - DU[CI(i, J, k)]);
+            dvdy = (Q1[H(i, J, k)] - Q1[H(i, J - 1, k)]) * d.dyfi[J - 1];
+            const double t22b = 0.5 * (V[H(i, J - 1, k)] + V[H(i, J - 1, k)]) * (dvdy - DU[CI(i, J - 1, k)]);
+            const double d22 = (t22f - t22b) * v2;
+            double dvdx = (Q1[H(ip, J, k)] - Q1[H(i, J, k)]) * dxi;
+            double dudy = (Q0[H(ip, J, k)] - Q0[H(ip, J - 1, k)]) * d.dyci[J];
+            const double t21f = d.mstg[H(ip, J, k)] * (dvdx + dudy);
+            dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
+            dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
+            const double t21b = d.mstg[H(i, J, k)] * (dvdx + dudy);
+            const double d21 = (t21f - t21b) * v1;
+            double dvdz = (Q1[H(i, J, kp)] - Q1[H(i, J, k)]) * dzi;
+            double dwdy = (Q2[H(i, J, kp)] - Q2[H(i, J - 1, kp)]) * d.dyci[J];
+            const double t23f = d.mstg[2 * hs + H(i, J, kp)] * (dvdz + dwdy);
+            dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
+            dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
+            const double t23b = d.mstg[2 * hs + H(i, J, k)] * (dvdz + dwdy);
+            const double d23 = (t23f - t23b) * v3;
+            acc = acc + d21 + d22 + d23;
+            pgm = (d.pr[H(i, J, k)] - d.pr[H(i, J - 1, k)]) * (ta * dt * d.dyci[J]);
+        } else {
+            const double c1 = dxi * hf * hf, c2 = d.dyfi[J] * hf, c3 = dzi * hf * hf;
+            double f = (G0[H(ip, J, k)] + G0[H(ip, J, km)]) * (Q2[H(ip, J, k)] + Q2[H(i, J, k)]);
+            double b = (G0[H(i, J, k)] + G0[H(i, J, km)]) * (Q2[H(i, J, k)] + Q2[H(im, J, k)]);
+            const double h31 = (f - b) * c1;
+            f = (G1[H(i, J + 1, k)] + G1[H(i, J + 1, km)]) * (wffp * Q2[H(i, J + 1, k)] + wfbp * Q2[H(i, J, k)]);
+            b = (G1[H(i, J, k)] + G1[H(i, J, km)]) * (wff * Q2[H(i, J, k)] + wfb * Q2[H(i, J - 1, k)]);
+            const double h32 = (f - b) * c2;
+            f = (G2[H(i, J, kp)] + G2[H(i, J, k)]) * (Q2[H(i, J, kp)] + Q2[H(i, J, k)]);
+            b = (G2[H(i, J, k)] + G2[H(i, J, km)]) * (Q2[H(i, J, k)] + Q2[H(i, J, km)]);
+            const double h33 = (f - b) * c3;
+            acc = -(h31 + h32 + h33);
+            const double v1 = dxi * cv, v2 = d.dyfi[J] * cv, v3 = 2.0 * dzi * cv;
+            double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
+            const double t33f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dwdz - DU[CI(i, J, k)]);
+            dwdz = (Q2[H(i, J, k)] - Q2[H(i, J, km)]) * dzi;
+            const double t33b = 0.5 * (V[H(i, J, km)] + V[H(i, J, km)]) * (dwdz - DU[CI(i, J, km)]);
+            const double d33 = (t33f - t33b) * v3;
+            double dwdx = (Q2[H(ip, J, k)] - Q2[H(i, J, k)]) * dxi;
+            double dudz = (Q0[H(ip, J, k)] - Q0[H(ip, J, km)]) * dzi;
+            const double t31f = d.mstg[hs + H(ip, J, k)] * (dwdx + dudz);
+            dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
+            dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
+            const double t31b = d.mstg[hs + H(i, J, k)] * (dwdx + dudz);
+            const double d31 = (t31f - t31b) * v1;
+            double dwdy = (Q2[H(i, J + 1, k)] - Q2[H(i, J, k)]) * d.dyci[J + 1];
+            double dvdz = (Q1[H(i, J + 1, k)] - Q1[H(i, J + 1, km)]) * dzi;
+            const double t32f = d.mstg[2 * hs + H(i, J + 1, k)] * (dwdy + dvdz);
+            dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
+            dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
+            const double t32b = d.mstg[2 * hs + H(i, J, k)] * (dwdy + dvdz);
+            const double d32 = (t32f - t32b) * v2;
+            acc = acc + d31 + d32 + d33;
+            pgm = (d.pr[H(i, J, k)] - d.pr[H(i, J, km)]) * (ta * dt * dzi);
+        }
+        const size_t o = (size_t)C * csz(d) + CI(i, J, k);
+        double r = (tg * acc + tr * d.explt0[o]) * dt;
+        d.explt0[o] = acc;
+        r = r - pgm;
+        if (d.gravity == C + 1) r = r + d.dstg[C * hs + H(i, J, k)] * (ta * dt * d.f_a);
+        d.h[o] = r;
+        if (C == 0) contrib = r / d.dyfi[J];
+    }
+    if constexpr (C == 0) {
+        const double s = block_sum256(contrib, sh);
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// DPGRNS = sum of the partials / VOLM_io, one block, fixed order.
+__global__ __launch_bounds__(256) void k_io_reduce(const double *__restrict__ part, int n, double volm,
+                                                   double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) v = v + part[j];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] / volm;
+}
+
+// The streamwise correction (RHS_MOM_EXPLICIT_io.f90:259-265) and MASSFLUX_CALC_io
+// (MASSFLUX_CALC_IO.f90:15-52) with BC_WALL_G_io: rows 0..M+1.
+__global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__restrict__ dpg_dev, double dpg)
+{
+#pragma clang fp contract(off)
+    IO_THREAD(0)
+    if (i >= L || J > M + 1) return;
+    const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+    if (J == 0 || J == M + 1) {
+        for (int c = 0; c < 3; ++c) d.g[c * hs + c0] = 0.0;
+        return;
+    }
+    const double dp = dpg_dev ? dpg_dev[0] : dpg;
+    const size_t ci = cx(d, i, J, k), cs = csz(d);
+    d.g[c0] = d.g[c0] + (d.h[ci] - dp);
+    d.g[2 * hs + c0] = d.g[2 * hs + c0] + d.h[2 * cs + ci];
+    d.g[hs + c0] = J == 1 ? 0.0 : d.g[hs + c0] + d.h[cs + ci];
+}
+
+// DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
+__global__ __launch_bounds__(256) void k_io_divg(IoDev d, double coe0)
+{
+#pragma clang fp contract(off)
+    IO_THREAD(1)
+    if (i >= L || J > M) return;
+    const int ip = i == L - 1 ? 0 : i + 1, kp = k == N - 1 ? 0 : k + 1;
+    const size_t hs = hsz(d), c0 = hx(d, i, J, k), ci = cx(d, i, J, k);
+    const double divx = (d.g[hx(d, ip, J, k)] - d.g[c0]) * d.dxi;
+    const double divy = (d.g[hs + hx(d, i, J + 1, k)] - d.g[hs + c0]) * d.dyfi[J];
+    const double divz = (d.g[2 * hs + hx(d, i, J, kp)] - d.g[2 * hs + c0]) * d.dzi;
+    const double dq = divx + divy + divz + d.drho[ci];
+    d.rhsp[ci] = dq * coe0;
+}
+
+// PRCALC_io explicit branch (PRESSURE_CORRECTION.f90:102-170) with BC_WALL_PR_io,
+// MASSFLUX_UPDATE_io (VELO_CORRECTION.f90:103-179) with BC_WALL_G_io, and VELOCITY_CALC_io:
+// rows 0..M+1.
+__global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt)
+{
+#pragma clang fp contract(off)
+    IO_THREAD(0)
+    if (i >= L || J > M + 1) return;
+    const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+    if (J >= 1 && J <= M) {   // the wall cells' threads also write the ghost-row copies
+        const double p = d.pr[c0] + d.dph[cx(d, i, J, k)];
+        d.pr[c0] = p;
+        if (J == 1) d.pr[hx(d, i, 0, k)] = p;
+        if (J == M) d.pr[hx(d, i, M + 1, k)] = p;
+    }
+    if (J == 0 || J == M + 1) {
+        for (int c = 0; c < 3; ++c) {
+            d.g[c * hs + c0] = 0.0;
+            d.q[c * hs + c0] = 0.0;
+        }
+        return;
+    }
+    const int im = i == 0 ? L - 1 : i - 1, km = k == 0 ? N - 1 : k - 1;
+    const double ph = d.dph[cx(d, i, J, k)];
+    const double gx = d.g[c0] - (ph - d.dph[cx(d, im, J, k)]) * (tadt * d.dxi);
+    const double gz = d.g[2 * hs + c0] - (ph - d.dph[cx(d, i, J, km)]) * (tadt * d.dzi);
+    d.g[c0] = gx;
+    d.g[2 * hs + c0] = gz;
+    d.q[c0] = gx / d.dstg[c0];
+    d.q[2 * hs + c0] = gz / d.dstg[2 * hs + c0];
+    if (J == 1) {
+        d.g[hs + c0] = 0.0;
+        d.q[hs + c0] = 0.0;
+    } else {
+        const double gy = d.g[hs + c0] - (ph - d.dph[cx(d, i, J - 1, k)]) * (tadt * d.dyci[J]);
+        d.g[hs + c0] = gy;
+        d.q[hs + c0] = gy / d.dstg[hs + c0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_io_fill(double *__restrict__ p, size_t n, double v)
+{
+    const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (j < n) p[j] = v;
+}
+
+#undef IO_THREAD
+
+}  // namespace
+
+struct chs_io {
+    int L = 0, M = 0, N = 0;
+    chs_ctx *pois = nullptr;      // Poisson solver on the same mesh (its stream is ours)
+    hipStream_t stream = nullptr;
+    IoDev d{};
+    double *buf = nullptr;
+    int npart = 0, flow_driven = 1;
+    double cf_given = 0.0, volm = 0.0;
+    double tgam[4], troh[4], talp[4];
+    std::string err;
+};
+
+namespace {
+
+int io_fail(chs_io *m, int code, const std::string &msg)
+{
+    if (m) m->err = msg;
+    g_io_err = msg;
+    return code;
+}
+
+dim3 io_grid(const chs_io *m, int rows) { return dim3((m->L + 63) / 64, (rows + 3) / 4, m->N); }
+
+int io_hip(chs_io *m, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) e = hipGetLastError();
+    return e == hipSuccess ? CHS_OK : io_fail(m, CHS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" {
+
+int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
+{
+    if (!c || !p || !out || c->L < 2 || c->M < 3 || c->N < 2 || (p->flow_driven != 1 && p->flow_driven != 2) ||
+        p->gravity < 0 || p->gravity > 3 || !(p->cvisc > 0))
+        return io_fail(nullptr, CHS_E_ARG, "chs_io_create: bad argument (flow_driven 1 or 2, gravity 0..3, cvisc > 0)");
+    *out = nullptr;
+    chs_io *m = new chs_io();
+    const int L = c->L, M = c->M, N = c->N;
+    m->L = L; m->M = M; m->N = N;
+    int rc = chs_create(c, CHS_PLAIN_FP64, &m->pois);
+    if (rc) {
+        std::string e = std::string("chs_io_create: Poisson context: ") + chs_last_error(nullptr);
+        delete m;
+        return io_fail(nullptr, rc, e);
+    }
+    m->stream = chs_internal_stream(m->pois);
+    const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L;
+    m->npart = (int)(((L + 63) / 64) * ((M + 3) / 4) * N);
+    const size_t ng = 4 * (size_t)(M + 2);
+    const size_t nb = 18 * hs + 10 * cs + ng + m->npart + 4;
+    hipError_t he = hipMalloc(&m->buf, sizeof(double) * nb);
+    if (he == hipSuccess) he = hipMemsetAsync(m->buf, 0, sizeof(double) * nb, m->stream);
+    IoDev &d = m->d;
+    double *f = m->buf;
+    d.g = f; d.g0 = f + 3 * hs; d.q = f + 6 * hs; d.dstg = f + 9 * hs; d.mstg = f + 12 * hs;
+    d.pr = f + 15 * hs; d.d0 = f + 16 * hs; d.m0 = f + 17 * hs;
+    f += 18 * hs;
+    d.explt0 = f; d.h = f + 3 * cs; d.drho = f + 6 * cs; d.divu = f + 7 * cs; d.rhsp = f + 8 * cs; d.dph = f + 9 * cs;
+    f += 10 * cs;
+    double *geo = f;
+    d.partial = f + ng;
+    d.scal = d.partial + m->npart;
+    std::vector<double> hg(ng, 0.0);
+    for (int J = 1; J <= M; ++J) hg[J] = c->dyfi[J - 1];
+    for (int J = 1; J <= M + 1; ++J) {
+        hg[(M + 2) + J] = c->dyci[J - 1];
+        hg[2 * (M + 2) + J] = c->wff[J - 1];
+        hg[3 * (M + 2) + J] = c->wfb[J - 1];
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(geo, hg.data(), sizeof(double) * ng, hipMemcpyHostToDevice, m->stream);
+    // D_STG = MU_STG = 1 until the first chs_io_props (MEMALLOCT.f90:200-201)
+    if (he == hipSuccess) {
+        const size_t n = 6 * hs;
+        hipLaunchKernelGGL(k_io_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, d.dstg, n, 1.0);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    if (he != hipSuccess) {
+        chs_io_destroy(m);
+        return io_fail(nullptr, CHS_E_HIP, std::string("chs_io_create: ") + hipGetErrorString(he));
+    }
+    d.L = L; d.M = M; d.N = N;
+    d.dxi = c->dxi; d.dzi = c->dzi; d.cvisc = p->cvisc; d.f_a = p->f_a; d.gravity = p->gravity;
+    d.dyfi = geo; d.dyci = geo + (M + 2); d.wff = geo + 2 * (M + 2); d.wfb = geo + 3 * (M + 2);
+    m->flow_driven = p->flow_driven; m->cf_given = p->cf_given; m->volm = c->volm;
+    for (int s = 0; s < 4; ++s) { m->tgam[s] = c->tgam[s]; m->troh[s] = c->troh[s]; m->talp[s] = c->talp[s]; }
+    *out = m;
+    return CHS_OK;
+}
+
+void chs_io_destroy(chs_io *m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->buf) (void)hipFree(m->buf);
+    if (m->pois) chs_destroy(m->pois);
+    delete m;
+}
+
+const char *chs_io_last_error(const chs_io *m) { return m ? m->err.c_str() : g_io_err.c_str(); }
+
+int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr, const double *d0,
+                  const double *m0, const double *explt0)
+{
+    if (!m || !g || !pr || !d0 || !m0) return io_fail(m, CHS_E_ARG, "chs_io_upload: null argument");
+    const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L, cb = sizeof(double) * (size_t)m->N * m->M * m->L;
+    const IoDev &d = m->d;
+    hipError_t he = hipMemcpyAsync(d.g, g, 3 * hb, hipMemcpyHostToDevice, m->stream);
+    if (he == hipSuccess)
+        he = g0 ? hipMemcpyAsync(d.g0, g0, 3 * hb, hipMemcpyHostToDevice, m->stream)
+                : hipMemcpyAsync(d.g0, d.g, 3 * hb, hipMemcpyDeviceToDevice, m->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(d.pr, pr, hb, hipMemcpyHostToDevice, m->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(d.d0, d0, hb, hipMemcpyHostToDevice, m->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(d.m0, m0, hb, hipMemcpyHostToDevice, m->stream);
+    if (he == hipSuccess)
+        he = explt0 ? hipMemcpyAsync(d.explt0, explt0, 3 * cb, hipMemcpyHostToDevice, m->stream)
+                    : hipMemsetAsync(d.explt0, 0, 3 * cb, m->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    return io_hip(m, he, "chs_io_upload");
+}
+
+int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev)
+{
+    if (!m || !d0_dev || !m0_dev) return io_fail(m, CHS_E_ARG, "chs_io_set_properties: null argument");
+    const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L;
+    hipError_t he = hipMemcpyAsync(m->d.d0, d0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(m->d.m0, m0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
+    return io_hip(m, he, "chs_io_set_properties");
+}
+
+int chs_io_props(chs_io *m, int ns)
+{
+    if (!m || ns < 1 || ns > 3) return io_fail(m, CHS_E_ARG, "chs_io_props: bad argument (NS 1..3)");
+    hipLaunchKernelGGL(k_io_props, io_grid(m, m->M + 1), dim3(64, 4), 0, m->stream, m->d, m->tgam[ns], m->troh[ns]);
+    return io_hip(m, hipSuccess, "chs_io_props");
+}
+
+int chs_substep_io(chs_io *m, int ns, double dt)
+{
+    if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io: bad argument");
+    const IoDev &d = m->d;
+    const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L;
+    const double tg = m->tgam[ns], tr = m->troh[ns], ta = m->talp[ns];
+    const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
+    hipError_t he = hipMemcpyAsync(d.g0, d.g, 3 * hb, hipMemcpyDeviceToDevice, m->stream);
+    if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
+    hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d);
+    hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
+    hipLaunchKernelGGL(k_io_mom<0>, gin, blk, 0, m->stream, d, tg, tr, ta, dt);
+    hipLaunchKernelGGL(k_io_mom<1>, gin, blk, 0, m->stream, d, tg, tr, ta, dt);
+    hipLaunchKernelGGL(k_io_mom<2>, gin, blk, 0, m->stream, d, tg, tr, ta, dt);
+    const double *dpg_dev = nullptr;
+    double dpg = 0.0;
+    if (m->flow_driven == 1) {
+        hipLaunchKernelGGL(k_io_reduce, dim3(1), dim3(256), 0, m->stream, d.partial, m->npart, m->volm, d.scal);
+        dpg_dev = d.scal;
+    } else {
+        dpg = -0.5 * m->cf_given * (ta * dt);
+    }
+    hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, dpg_dev, dpg);
+    hipLaunchKernelGGL(k_io_divg, gin, blk, 0, m->stream, d, 1.0 / dt / ta);
+    he = hipGetLastError();
+    if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
+    int rc = chs_poisson_device(m->pois, d.rhsp, d.dph, 0, nullptr);
+    if (rc) return io_fail(m, rc, std::string("chs_substep_io: Poisson: ") + chs_last_error(m->pois));
+    hipLaunchKernelGGL(k_io_correct, gall, blk, 0, m->stream, d, dt * ta);
+    return io_hip(m, hipSuccess, "chs_substep_io");
+}
+
+int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, double *explt0, double *drho,
+                    double *rhsp, double *dstg, double *mstg)
+{
+    if (!m) return io_fail(m, CHS_E_ARG, "chs_io_download: null context");
+    const int L = m->L, M = m->M, N = m->N;
+    const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L, hb = sizeof(double) * hs, cb = sizeof(double) * cs;
+    const IoDev &d = m->d;
+    std::vector<double> ph;
+    hipError_t he = hipSuccess;
+    struct { double *dst; const double *src; size_t bytes; } cp[] = {
+        {g, d.g, 3 * hb}, {q, d.q, 3 * hb}, {pr, d.pr, hb}, {explt0, d.explt0, 3 * cb},
+        {drho, d.drho, cb}, {rhsp, d.rhsp, cb}, {dstg, d.dstg, 3 * hb}, {mstg, d.mstg, 3 * hb}};
+    for (auto &x : cp)
+        if (x.dst && he == hipSuccess) he = hipMemcpyAsync(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost, m->stream);
+    if (dph && he == hipSuccess) {
+        ph.resize(cs);
+        he = hipMemcpyAsync(ph.data(), d.dph, cb, hipMemcpyDeviceToHost, m->stream);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    if (he != hipSuccess) return io_hip(m, he, "chs_io_download");
+    if (dph) {   // halo layout, ghost rows copy the wall cells (BC_WALL_DPH_io)
+        for (int k = 0; k < N; ++k)
+            for (int J = 0; J <= M + 1; ++J) {
+                const int Js = J == 0 ? 1 : (J == M + 1 ? M : J);
+                for (int i = 0; i < L; ++i)
+                    dph[((size_t)k * (M + 2) + J) * L + i] = ph[((size_t)k * M + Js - 1) * L + i];
+            }
+    }
+    return CHS_OK;
+}
+
+int chs_io_sync(chs_io *m)
+{
+    if (!m) return io_fail(m, CHS_E_ARG, "chs_io_sync: null context");
+    return io_hip(m, hipStreamSynchronize(m->stream), "chs_io_sync");
+}
+
+int chs_heated_substep(chs_eng *e, chs_io *m, int ns, double dt)
+{
+    if (!e || !m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_heated_substep: bad argument");
+    hipStream_t es;
+    double *ed0, *em0, *eg;
+    chs_internal_eng_fields(e, &es, &ed0, &em0, &eg);
+    hipError_t he = hipStreamSynchronize(m->stream);   // G of the last momentum substep is final
+    if (he != hipSuccess) return io_hip(m, he, "chs_heated_substep");
+    int rc = chs_eng_set_mass_flux(e, m->d.g);
+    if (!rc) rc = chs_substep_eng(e, ns, dt);          // synchronous
+    if (rc) return io_fail(m, rc, std::string("chs_heated_substep: energy: ") + chs_eng_last_error(e));
+    rc = chs_io_set_properties(m, ed0, em0);
+    if (!rc) rc = chs_io_props(m, ns);
+    if (!rc) rc = chs_substep_io(m, ns, dt);
+    return rc;
+}
+
+}  // extern "C"

@@ -2449,3 +2449,7 @@ int chs_ens_sync(chs_ens *e)
 }
 
 }  // extern "C"
+
+// Stream of a context, for the library's other components that enqueue work behind its
+// solves (the main-domain momentum substep uses a context's Poisson solver).
+__attribute__((visibility("hidden"))) hipStream_t chs_internal_stream(chs_ctx *c) { return c ? c->stream : nullptr; }

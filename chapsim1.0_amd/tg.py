@@ -75,6 +75,16 @@ SIGNATURES = {
     "chs_eng_set_mass_flux": (C.c_int, [C.c_void_p, C.c_void_p]),
     "chs_substep_eng": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_eng_download": (C.c_int, [C.c_void_p] + [_dp] * 8),
+    "chs_io_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "chs_io_destroy": (None, [C.c_void_p]),
+    "chs_io_last_error": (C.c_char_p, [C.c_void_p]),
+    "chs_io_upload": (C.c_int, [C.c_void_p] + [_dp] * 6),
+    "chs_io_set_properties": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "chs_io_props": (C.c_int, [C.c_void_p, C.c_int]),
+    "chs_substep_io": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "chs_io_download": (C.c_int, [C.c_void_p] + [_dp] * 9),
+    "chs_io_sync": (C.c_int, [C.c_void_p]),
+    "chs_heated_substep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double]),
 }
 
 
@@ -358,3 +368,93 @@ def tdma_lines(kind, A, B, C_, R, bc=None):
     finally:
         for d in dev + ([dbc] if dbc else []):
             d.free()
+
+
+class _IoParams(C.Structure):
+    _fields_ = [("cvisc", C.c_double), ("gravity", C.c_int), ("f_a", C.c_double),
+                ("flow_driven", C.c_int), ("cf_given", C.c_double)]
+
+
+class ChannelIO:
+    """The main-domain (IO) flow of one periodic heated channel on the GPU, variable density
+    and viscosity, explicit viscous scheme: ``ioprops(ns)`` is what SOLVE runs before the
+    momentum substep (SOLVE.f90:122-124), ``SOLVERRK3_MOM_io(ns, dt)`` the substep
+    (SOLVERRK3_MOM_IO.f90:11-120).  gravity = |iGravity|, f_a = F_A, flow_driven = iFlowDriven.
+    Fields are C-order: G, G0 [3, N, M+2, L], PR, D0, M0 [N, M+2, L], EXPLT0 [3, N, M, L]."""
+
+    def __init__(self, coeffs: dict, REN: float, gravity=0, f_a=0.0, flow_driven=1, cf_given=0.0):
+        self.L, self.M, self.N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
+        self._keep = []
+        self._c = _coeff_struct(coeffs, self._keep)
+        self._par = _IoParams(1.0 / float(REN), int(gravity), float(f_a), int(flow_driven), float(cf_given))
+        h = C.c_void_p()
+        rc = lib().chs_io_create(C.byref(self._c), C.byref(self._par), C.byref(h))
+        if rc != 0:
+            raise ChsError(f"chs_io_create failed ({rc}): {lib().chs_io_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise ChsError(f"{what} failed ({rc}): {lib().chs_io_last_error(self._h).decode()}")
+
+    def upload(self, G, PR, D0, M0, G0=None, EXPLT0=None):
+        L, M, N = self.L, self.M, self.N
+        hs = (N, M + 2, L)
+        arrs = {}
+        for k, a, shp in (("G", G, (3,) + hs), ("G0", G0, (3,) + hs), ("PR", PR, hs), ("D0", D0, hs),
+                          ("M0", M0, hs), ("EXPLT0", EXPLT0, (3, N, M, L))):
+            if a is None:
+                arrs[k] = None
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != shp:
+                raise ChsError(f"upload: {k} must have shape {shp}")
+            arrs[k] = a
+        P = lambda a: None if a is None else _p(a)
+        self._check(lib().chs_io_upload(self._h, P(arrs["G"]), P(arrs["G0"]), P(arrs["PR"]), P(arrs["D0"]),
+                                        P(arrs["M0"]), P(arrs["EXPLT0"])), "chs_io_upload")
+
+    def set_properties(self, d0_dev: int, m0_dev: int):
+        self._check(lib().chs_io_set_properties(self._h, C.c_void_p(d0_dev), C.c_void_p(m0_dev)),
+                    "chs_io_set_properties")
+
+    def ioprops(self, ns):
+        self._check(lib().chs_io_props(self._h, int(ns)), "chs_io_props")
+
+    def SOLVERRK3_MOM_io(self, ns, dt):
+        self._check(lib().chs_substep_io(self._h, int(ns), float(dt)), "chs_substep_io")
+
+    def sync(self):
+        self._check(lib().chs_io_sync(self._h), "chs_io_sync")
+
+    def download(self):
+        """dict of G, Q, DSTG, MSTG [3, N, M+2, L]; PR, DPH [N, M+2, L]; EXPLT0 [3, N, M, L];
+        DRHO, RHSP [N, M, L]."""
+        L, M, N = self.L, self.M, self.N
+        hs = (N, M + 2, L)
+        o = dict(G=np.zeros((3,) + hs), Q=np.zeros((3,) + hs), PR=np.zeros(hs), DPH=np.zeros(hs),
+                 EXPLT0=np.zeros((3, N, M, L)), DRHO=np.zeros((N, M, L)), RHSP=np.zeros((N, M, L)),
+                 DSTG=np.zeros((3,) + hs), MSTG=np.zeros((3,) + hs))
+        self._check(lib().chs_io_download(self._h, *[_p(o[k]) for k in ("G", "Q", "PR", "DPH", "EXPLT0", "DRHO",
+                                                                         "RHSP", "DSTG", "MSTG")]),
+                    "chs_io_download")
+        return o
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chs_io_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def heated_substep(eng, io: ChannelIO, ns: int, dt: float):
+    """One stage of the heated channel as SOLVE runs it (SOLVE.f90:119-133): energy substep,
+    staggered properties, SOLVERRK3_MOM_io (chs_heated_substep)."""
+    rc = lib().chs_heated_substep(eng._h, io._h, int(ns), float(dt))
+    if rc != 0:
+        raise ChsError(f"chs_heated_substep failed ({rc}): {lib().chs_io_last_error(io._h).decode()}")

@@ -295,6 +295,69 @@ int chs_substep_eng(chs_eng *e, int ns, double dt);
 int chs_eng_download(chs_eng *e, double *dh, double *h, double *t, double *d, double *m, double *k, double *cp,
                      double *rhs0);
 
+/* ---- Main-domain momentum with variable density and viscosity ------------------------ */
+
+/* Physical constants of the main-domain momentum substep (READINI / Flow_State set-up):
+ *   cvisc        CVISC = 1/REN
+ *   gravity      |iGravity|: 0 none, else the momentum component (1 x, 2 y, 3 z) that gets the
+ *                buoyancy term D_STG * F_A (RHS_MOM_EXPLICIT_io.f90:162-226, channel)
+ *   f_a          F_A (Flow_State.f90:96-126)
+ *   flow_driven  iFlowDriven: 1 constant mass flux (the x-increment's volume mean is removed),
+ *                2 constant skin friction Cf_Given (RHS_MOM_EXPLICIT_io.f90:229-265)
+ *   cf_given     Cf_Given */
+typedef struct {
+    double cvisc;
+    int gravity;
+    double f_a;
+    int flow_driven;
+    double cf_given;
+} chs_io_params;
+
+typedef struct chs_io chs_io;
+
+/* Main-domain flow state of one periodic channel on the current GPU: G_io, G0_io, Q_io, D_STG,
+ * MU_STG (3 halo fields each, [c][N][M+2][L]), PR_io, DENSITY0, Viscousity0 (halo), EXPLT0_io
+ * ([3][N][M][L]), DrhoDtP and the Poisson right-hand side (cells), with a Poisson solver on
+ * the same mesh (a chs_ctx, FFT99_POIS3D_periodicxz(IIO): the reference's is_FFT_FFT99 method,
+ * SOLVERRK3_MOM_IO.f90:92).  Replaces the device side of MEM_ALLOCAT for the main domain
+ * (MEMALLOCT.f90:150-201). */
+int chs_io_create(const chs_coeffs *mesh, const chs_io_params *p, chs_io **out);
+void chs_io_destroy(chs_io *m);
+const char *chs_io_last_error(const chs_io *m);
+
+/* Host arrays: g, g0 [3][N][M+2][L] (g0 NULL: = g), pr, d0 (DENSITY0), m0 (Viscousity0)
+ * [N][M+2][L], explt0 [3][N][M][L] or NULL for zero.  Synchronous. */
+int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr, const double *d0,
+                  const double *m0, const double *explt0);
+
+/* DENSITY0 and Viscousity0 from device halo arrays (e.g. an energy context's). */
+int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev);
+
+/* What SOLVE runs between the energy and the momentum substep of stage NS in a heated run
+ * (SOLVE.f90:122-124): DENSITY_TIME_DERIVATION(NS) (DIVG.f90:249-317), DENSITY_Staggered
+ * (MASSFLUX_CALC_IO.f90:262-323), MU_Staggered (:175-259).  Enqueued; returns at once. */
+int chs_io_props(chs_io *m, int ns);
+
+/* SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:11-120) with the explicit viscous scheme
+ * (iVisScheme = 0, every main-domain template; iWeightedPre = 0): VELOCITY_CALC_io,
+ * CONVECTION_X/Y/Z_io, DIVG_U_io, VISCOUS_ALL_EXPLT_X/Y/Z_io + RHS_MOM_EXPLICIT_io,
+ * MASSFLUX_CALC_io, DIVG_io, the Poisson solve, PRCALC_io, MASSFLUX_UPDATE_io, VELOCITY_CALC_io,
+ * with the walls (BC_WALL_G/Q/PR/DPH_io).  FMA contraction off, reference operation order:
+ * matches the reference to 1e-12 (G, Q) / 1e-10 (pressure).  Enqueued; returns at once. */
+int chs_substep_io(chs_io *m, int ns, double dt);
+
+/* Copy fields back (NULL skips one): g, q, dstg, mstg [3][N][M+2][L]; pr, dph [N][M+2][L]
+ * (dph's ghost rows copy the wall cells, BC_WALL_DPH_io); explt0 [3][N][M][L]; drho, rhsp
+ * [N][M][L].  Synchronous. */
+int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, double *explt0, double *drho,
+                    double *rhsp, double *dstg, double *mstg);
+int chs_io_sync(chs_io *m);
+
+/* One stage of the heated channel as SOLVE runs it (SOLVE.f90:119-133): the energy substep
+ * with the momentum context's current G, then its DENSITY0 / Viscousity0 into the momentum
+ * context, the staggered properties and SOLVERRK3_MOM_io(NS). */
+int chs_heated_substep(chs_eng *e, chs_io *m, int ns, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,3 +130,44 @@ def test_gpu_io_matches_oracle(pkg, L, M, N):
             assert rel(f[k][c], o.f[k][c]) < 1e-11, (k, c, rel(f[k][c], o.f[k][c]))
     assert rel(f["PR"], o.f["PR"]) < 1e-9
     ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_heated_stage_matches_oracle(pkg):
+    """chs_heated_substep (energy substep, staggered properties, momentum substep, in SOLVE's
+    order) against the same composition of the two pinned oracles, one RK3 step."""
+    from importlib import import_module
+    thermo = import_module(pkg.__name__ + ".thermo")
+    t = thermo.table_setup(thermo.synthetic_table(), 665.0)
+    L, M, N, STR2, REN, dt = 32, 24, 16, 1.9, 2800.0, 5e-4
+    mesh = setup.Mesh(L, M, N, STR2=STR2)
+    dh, _ = oracle.eng_initial_field(mesh, t, 81)
+    st = oracle.io_initial_field(mesh, 82)
+    cthecd = 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"])
+    dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+    wf = (0.05, -0.05)
+    oe = oracle.OracleEng(mesh, t, (1, 1), cthecd, dhmin, dhmax, wf)
+    assert oe.init(dh, st["G"]) == 0
+    oi = oracle.OracleIO(mesh, REN, flow_driven=1)
+    oi.set_state(dict(st, D0=oe.f["d"], M0=oe.f["m"]))
+    co = pkg.channel_setup(L, M, N, STR2=STR2)
+    ge = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, wf)
+    ge.upload(dh, st["G"])
+    gi = pkg.ChannelIO(co, REN, flow_driven=1)
+    gi.upload(G=st["G"], G0=st["G0"], PR=st["PR"], D0=oe.f["d"], M0=oe.f["m"], EXPLT0=st["EXPLT0"])
+    for ns in (1, 2, 3):
+        oe.f["g"][...] = oi.f["G"]
+        assert oe.substep(ns, dt) == 0
+        oi.f["D0"][...] = oe.f["d0"]
+        oi.f["M0"][...] = oe.f["m0"]
+        oi.props(ns)
+        oi.substep(ns, dt)
+        pkg.heated_substep(ge, gi, ns, dt)
+    f, fe = gi.download(), ge.download()
+    for c in range(3):
+        assert rel(f["G"][c], oi.f["G"][c]) < 1e-11, (c, rel(f["G"][c], oi.f["G"][c]))
+    assert rel(f["PR"], oi.f["PR"]) < 1e-9
+    for k, kk in (("dh", "DH"), ("t", "T"), ("d", "D")):
+        assert rel(fe[kk], oe.f[k]) < 1e-11, (kk, rel(fe[kk], oe.f[k]))
+    ge.close()
+    gi.close()
