@@ -354,6 +354,64 @@ def bench_ensemble(pkg, args):
                                              "Re_tau U[180,395], one hipGraph per RK3 step"}}))
 
 
+CFG_HEATED = dict(L=256, M=128, N=256, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, dt=1.0e-3,
+                  T0=665.0, wallflux=(0.115, -0.115))
+
+
+def bench_heated(pkg, args):
+    """Quasi-incompressible heated channel on the main domain (config 5): per RK3 stage the
+    energy substep (table properties, iso-flux walls), the staggered properties and the
+    variable-property momentum substep with the explicit viscous scheme, in SOLVE's order
+    (chs_heated_substep).  Plain FP64.  value = grid-point RK3-steps/s."""
+    from chapsim_amd import init, thermo
+    c = CFG_HEATED
+    L, M, N = c["L"], c["M"], c["N"]
+    t = thermo.table_setup(thermo.synthetic_table(), c["T0"])
+    co = pkg.channel_setup(L, M, N, c["HX"], c["HZ"], c["STR2"])
+    dh, g, pr = init.heated_channel_initial(co, t, seed=180)
+    cthecd = 1.0 / c["REN"] / (t["M0"] * t["CP0"] / t["K0"])
+    dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+    eng = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, c["wallflux"])
+    eng.upload(dh, g)
+    f0 = eng.download()
+    io = pkg.ChannelIO(co, c["REN"], flow_driven=1)
+    io.upload(G=g, PR=pr, D0=f0["D"], M0=f0["M"])
+    dt = c["dt"]
+
+    def step():
+        for ns in (1, 2, 3):
+            pkg.heated_substep(eng, io, ns, dt)
+
+    for _ in range(args.warmup):
+        step()
+    io.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    io.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    f = io.download()
+    fe = eng.download()
+    # per-step algorithmic bytes: 148 full-field FP64 reads/writes per stage (kernel list in
+    # the chs_io.hip / chs_thermo.hip launch sequence), three stages
+    alg = 3 * 148 * 8.0 * L * M * N
+    eng.close()
+    io.close()
+    print(json.dumps({
+        "metric": "grid-point RK3-steps/s, heated channel (energy + variable-property momentum)",
+        "value": L * M * N / (ms * 1e-3), "unit": "grid-point RK3-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic: laminar G + 5% seeded noise, T/T0 = 1 + 1% seeded; "
+                               "synthetic 2048-point table",
+        "config": {"workload": f"heated channel REN={c['REN']:g}, {L}x{M}x{N}, 4pi x 2 x 2pi, tanh "
+                               f"beta={c['STR2']:g}, dt={dt:g}, iso-flux walls, explicit viscous, RK3"},
+        "step_roofline": {"alg_bytes_per_step": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
+                          "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "checks": {"T_range": [float(fe["T"][:, 1:M + 1].min()), float(fe["T"][:, 1:M + 1].max())],
+                   "bulk_G": float(np.mean(f["G"][0][:, 1:M + 1] / np.asarray(co["dyfi"])[None, :, None]))},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,7 +424,7 @@ def main():
                     help="search the kernels' launch shapes (chapsim_amd.tune) before timing")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="launch shape for chs_set_tuning (repeatable), e.g. --set conv_bx=4")
-    ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble"],
+    ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble", "heated"],
                     default="channel")
     ap.add_argument("--nref", type=int, default=1, help="refinement sweeps (poisson, mixed)")
     ap.add_argument("--host-roundtrip", type=int, default=0, metavar="STEPS",
@@ -392,6 +450,8 @@ def main():
         return bench_poisson(pkg, args)
     if args.workload == "ensemble":
         return bench_ensemble(pkg, args)
+    if args.workload == "heated":
+        return bench_heated(pkg, args)
     c = CFG64 if args.workload == "channel64" else CFG
     shapes = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set}
     r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune, shapes,

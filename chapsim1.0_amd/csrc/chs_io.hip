@@ -11,12 +11,14 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/chapsim_hip.h"
 
 hipStream_t chs_internal_stream(chs_ctx *c);
 void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **g);
+void chs_internal_eng_set_g(chs_eng *e, const double *g);
 
 namespace {
 
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(256) void k_io_props(IoDev d, double tg, double tr)
 }
 
 // VELOCITY_CALC_io (MASSFLUX_CALC_IO.f90:55-107) with BC_WALL_Q_io: rows J = 0..M+1.
-__global__ __launch_bounds__(256) void k_io_velocity(IoDev d)
+__global__ __launch_bounds__(256) void k_io_velocity(IoDev d, const double *__restrict__ gs)
 {
 #pragma clang fp contract(off)
     IO_THREAD(0)
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(256) void k_io_velocity(IoDev d)
     const size_t hs = hsz(d), c0 = hx(d, i, J, k);
     for (int c = 0; c < 3; ++c) {
         const size_t o = c * hs + c0;
-        d.q[o] = (J == 0 || J == M + 1 || (c == 1 && J == 1)) ? 0.0 : d.g[o] / d.dstg[o];
+        d.q[o] = (J == 0 || J == M + 1 || (c == 1 && J == 1)) ? 0.0 : gs[o] / d.dstg[o];
     }
 }
 
@@ -137,7 +139,8 @@ __device__ double block_sum256(double v, double *sh)
 // (2..M for v).  For C = 0 each block also writes its partial of sum(RHS / DYFI) for the
 // mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265).
 template <int C>
-__global__ __launch_bounds__(256) void k_io_mom(IoDev d, double tg, double tr, double ta, double dt)
+__global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restrict__ gs, double tg, double tr,
+                                                double ta, double dt)
 {
 #pragma clang fp contract(off)
     __shared__ double sh[256];
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(256) void k_io_mom(IoDev d, double tg, double tr, d
         const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
         const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
         const size_t hs = hsz(d);
-        const double *G0 = d.g, *G1 = d.g + hs, *G2 = d.g + 2 * hs;
+        const double *G0 = gs, *G1 = gs + hs, *G2 = gs + 2 * hs;
         const double *Q0 = d.q, *Q1 = d.q + hs, *Q2 = d.q + 2 * hs;
         const double *V = d.m0, *DU = d.divu;
         const double hf = 0.5, cv = d.cvisc, dxi = d.dxi, dzi = d.dzi;
@@ -292,7 +295,9 @@ __global__ __launch_bounds__(256) void k_io_reduce(const double *__restrict__ pa
 
 // The streamwise correction (RHS_MOM_EXPLICIT_io.f90:259-265) and MASSFLUX_CALC_io
 // (MASSFLUX_CALC_IO.f90:15-52) with BC_WALL_G_io: rows 0..M+1.
-__global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__restrict__ dpg_dev, double dpg)
+// gs: G at the start of the substep (G0_io); writes every row of the new G.
+__global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__restrict__ gs,
+                                                     const double *__restrict__ dpg_dev, double dpg)
 {
 #pragma clang fp contract(off)
     IO_THREAD(0)
@@ -304,9 +309,9 @@ __global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__re
     }
     const double dp = dpg_dev ? dpg_dev[0] : dpg;
     const size_t ci = cx(d, i, J, k), cs = csz(d);
-    d.g[c0] = d.g[c0] + (d.h[ci] - dp);
-    d.g[2 * hs + c0] = d.g[2 * hs + c0] + d.h[2 * cs + ci];
-    d.g[hs + c0] = J == 1 ? 0.0 : d.g[hs + c0] + d.h[cs + ci];
+    d.g[c0] = gs[c0] + (d.h[ci] - dp);
+    d.g[2 * hs + c0] = gs[2 * hs + c0] + d.h[2 * cs + ci];
+    d.g[hs + c0] = J == 1 ? 0.0 : gs[hs + c0] + d.h[cs + ci];
 }
 
 // DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
@@ -381,6 +386,7 @@ struct chs_io {
     IoDev d{};
     double *buf = nullptr;
     int npart = 0, flow_driven = 1;
+    double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
     double cf_given = 0.0, volm = 0.0;
     double tgam[4], troh[4], talp[4];
     std::string err;
@@ -433,6 +439,7 @@ int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
     double *f = m->buf;
     d.g = f; d.g0 = f + 3 * hs; d.q = f + 6 * hs; d.dstg = f + 9 * hs; d.mstg = f + 12 * hs;
     d.pr = f + 15 * hs; d.d0 = f + 16 * hs; d.m0 = f + 17 * hs;
+    m->own_d0 = d.d0; m->own_m0 = d.m0;
     f += 18 * hs;
     d.explt0 = f; d.h = f + 3 * cs; d.drho = f + 6 * cs; d.divu = f + 7 * cs; d.rhsp = f + 8 * cs; d.dph = f + 9 * cs;
     f += 10 * cs;
@@ -483,6 +490,8 @@ int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr
 {
     if (!m || !g || !pr || !d0 || !m0) return io_fail(m, CHS_E_ARG, "chs_io_upload: null argument");
     const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L, cb = sizeof(double) * (size_t)m->N * m->M * m->L;
+    m->d.d0 = m->own_d0;
+    m->d.m0 = m->own_m0;
     const IoDev &d = m->d;
     hipError_t he = hipMemcpyAsync(d.g, g, 3 * hb, hipMemcpyHostToDevice, m->stream);
     if (he == hipSuccess)
@@ -502,6 +511,8 @@ int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev)
 {
     if (!m || !d0_dev || !m0_dev) return io_fail(m, CHS_E_ARG, "chs_io_set_properties: null argument");
     const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L;
+    m->d.d0 = m->own_d0;
+    m->d.m0 = m->own_m0;
     hipError_t he = hipMemcpyAsync(m->d.d0, d0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
     if (he == hipSuccess) he = hipMemcpyAsync(m->d.m0, m0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
     return io_hip(m, he, "chs_io_set_properties");
@@ -517,17 +528,19 @@ int chs_io_props(chs_io *m, int ns)
 int chs_substep_io(chs_io *m, int ns, double dt)
 {
     if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io: bad argument");
+    // G0_io <- G_io (SOLVERRK3_MOM_IO.f90:25) by exchanging the two arrays: the terms read the
+    // old G from g0 and k_io_update_g writes every row of the new G.
+    std::swap(m->d.g, m->d.g0);
     const IoDev &d = m->d;
-    const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L;
+    const double *gs = d.g0;
     const double tg = m->tgam[ns], tr = m->troh[ns], ta = m->talp[ns];
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
-    hipError_t he = hipMemcpyAsync(d.g0, d.g, 3 * hb, hipMemcpyDeviceToDevice, m->stream);
-    if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
-    hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d);
+    hipError_t he = hipSuccess;
+    hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
     hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
-    hipLaunchKernelGGL(k_io_mom<0>, gin, blk, 0, m->stream, d, tg, tr, ta, dt);
-    hipLaunchKernelGGL(k_io_mom<1>, gin, blk, 0, m->stream, d, tg, tr, ta, dt);
-    hipLaunchKernelGGL(k_io_mom<2>, gin, blk, 0, m->stream, d, tg, tr, ta, dt);
+    hipLaunchKernelGGL(k_io_mom<0>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+    hipLaunchKernelGGL(k_io_mom<1>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+    hipLaunchKernelGGL(k_io_mom<2>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
     const double *dpg_dev = nullptr;
     double dpg = 0.0;
     if (m->flow_driven == 1) {
@@ -536,7 +549,7 @@ int chs_substep_io(chs_io *m, int ns, double dt)
     } else {
         dpg = -0.5 * m->cf_given * (ta * dt);
     }
-    hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, dpg_dev, dpg);
+    hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
     hipLaunchKernelGGL(k_io_divg, gin, blk, 0, m->stream, d, 1.0 / dt / ta);
     he = hipGetLastError();
     if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
@@ -591,12 +604,21 @@ int chs_heated_substep(chs_eng *e, chs_io *m, int ns, double dt)
     chs_internal_eng_fields(e, &es, &ed0, &em0, &eg);
     hipError_t he = hipStreamSynchronize(m->stream);   // G of the last momentum substep is final
     if (he != hipSuccess) return io_hip(m, he, "chs_heated_substep");
-    int rc = chs_eng_set_mass_flux(e, m->d.g);
-    if (!rc) rc = chs_substep_eng(e, ns, dt);          // synchronous
-    if (rc) return io_fail(m, rc, std::string("chs_heated_substep: energy: ") + chs_eng_last_error(e));
-    rc = chs_io_set_properties(m, ed0, em0);
-    if (!rc) rc = chs_io_props(m, ns);
+    chs_internal_eng_set_g(e, m->d.g);                 // the energy reads the current G in place
+    int rc = chs_substep_eng(e, ns, dt);               // synchronous
+    if (rc) {
+        chs_internal_eng_set_g(e, nullptr);
+        return io_fail(m, rc, std::string("chs_heated_substep: energy: ") + chs_eng_last_error(e));
+    }
+    chs_internal_eng_fields(e, &es, &ed0, &em0, &eg);  // DENSITY0 / Viscousity0 of this stage
+    m->d.d0 = ed0;                                     // read in place by the momentum kernels
+    m->d.m0 = em0;
+    rc = chs_io_props(m, ns);
     if (!rc) rc = chs_substep_io(m, ns, dt);
+    // the launches above hold the aliases by value; later calls see each context's own arrays
+    chs_internal_eng_set_g(e, nullptr);
+    m->d.d0 = m->own_d0;
+    m->d.m0 = m->own_m0;
     return rc;
 }
 

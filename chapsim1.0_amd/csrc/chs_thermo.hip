@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/chapsim_hip.h"
@@ -227,25 +228,67 @@ __global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, dou
     e.rhs0[c0] = r;
 }
 
-__device__ void eng_cell_props(const EngDev &e, const PropDev &p, size_t h0, int *err)
+// The bisection of spline_interpolation_* (I = 1, J = N+1, thermo_properties_table_searching.f90:
+// 700-712) on the table x (global or an LDS copy: same values, same index).
+__device__ __forceinline__ int spl_index(const double *x, int n, double xv)
 {
-    const double hv = spl_eval(p.dhh, p.n, e.dh[h0], err, 1);
-    e.h[h0] = hv;
-    e.t[h0] = spl_eval(p.ht, p.n, hv, err, 2);
-    e.d[h0] = spl_eval(p.hm, p.n, hv, err, 4);    // D from H->M, as the reference
-    e.m[h0] = spl_eval(p.hk, p.n, hv, err, 8);    // M from H->K
-    e.k[h0] = spl_eval(p.hd, p.n, hv, err, 16);   // K from H->D
-    e.cp[h0] = spl_eval(p.hcp, p.n, hv, err, 32);
+    int I = 1, J = n + 1;
+    while (J > I + 1) {
+        const int K = (I + J) / 2;
+        if (xv < x[K - 1]) J = K;
+        else I = K;
+    }
+    return I;
 }
 
-// THERM_PROP_UPDATE (THERM_PROP_UPDATE.f90:124-168) over rows j0..j1.
-__global__ __launch_bounds__(256) void k_eng_props(EngDev e, PropDev p, int j0, int j1, int *err)
+__device__ __forceinline__ double spl_at(const SplineDev &s, int I, double xv)
 {
+#pragma clang fp contract(off)
+    const double dx = xv - s.x[I - 1];
+    return s.y[I - 1] + dx * (s.b[I - 1] + dx * (s.c[I - 1] + dx * s.d[I - 1]));
+}
+
+// THERM_PROP_UPDATE_FROM_DH for one cell.  The five H-based splines share their table
+// (LIST_H) and range (IMIN_H, IMAX_H), so one bisection serves all of them; each result equals
+// spl_eval's.  xdh / xh: LIST_DH / LIST_H (LDS copies or the global tables).
+__device__ void eng_cell_props(const EngDev &e, const PropDev &p, size_t h0, int *err, const double *xdh,
+                               const double *xh)
+{
+    const double dhv = e.dh[h0];
+    if ((dhv - p.dhh.x[p.dhh.imin - 1]) < -1.0e-8) atomicOr(err, 1);
+    if ((dhv - p.dhh.x[p.dhh.imax - 1]) > 1.0e-8) atomicOr(err, 1 << 8);
+    const double hv = spl_at(p.dhh, spl_index(xdh, p.n, dhv), dhv);
+    if ((hv - p.ht.x[p.ht.imin - 1]) < -1.0e-8) atomicOr(err, 2 | 4 | 8 | 16 | 32);
+    if ((hv - p.ht.x[p.ht.imax - 1]) > 1.0e-8) atomicOr(err, (2 | 4 | 8 | 16 | 32) << 8);
+    const int I = spl_index(xh, p.n, hv);
+    e.h[h0] = hv;
+    e.t[h0] = spl_at(p.ht, I, hv);
+    e.d[h0] = spl_at(p.hm, I, hv);    // D from H->M, as the reference
+    e.m[h0] = spl_at(p.hk, I, hv);    // M from H->K
+    e.k[h0] = spl_at(p.hd, I, hv);    // K from H->D
+    e.cp[h0] = spl_at(p.hcp, I, hv);
+}
+
+// THERM_PROP_UPDATE (THERM_PROP_UPDATE.f90:124-168) over rows j0..j1.  With lds != 0 the two
+// searched tables (2 n doubles of dynamic LDS) are staged per block first.
+__global__ __launch_bounds__(256) void k_eng_props(EngDev e, PropDev p, int j0, int j1, int *err, int lds)
+{
+    extern __shared__ double tab[];
+    const double *xdh = p.dhh.x, *xh = p.ht.x;
+    if (lds) {
+        for (int t = threadIdx.x + 64 * threadIdx.y; t < p.n; t += 256) {
+            tab[t] = p.dhh.x[t];
+            tab[p.n + t] = p.ht.x[t];
+        }
+        __syncthreads();
+        xdh = tab;
+        xh = tab + p.n;
+    }
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = j0 + blockIdx.y * 4 + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= e.L || J > j1) return;
-    eng_cell_props(e, p, ehx(e, i, J, k), err);
+    eng_cell_props(e, p, ehx(e, i, J, k), err, xdh, xh);
 }
 
 // BC_WALL_THERMAL(IMAIND) (BC_WALL.f90:622-643): iso-flux walls extrapolate DH to the ghost
@@ -268,7 +311,7 @@ __global__ __launch_bounds__(256) void k_eng_walls(EngDev e, PropDev p, int *err
             else
                 e.dh[h0] = 2.0 * e.dh[ehx(e, i, M, k)] - e.wff[e.top_face] * e.dh[ehx(e, i, M, k)] -
                            e.wfb[e.top_face] * e.dh[ehx(e, i, M - 1, k)];
-            eng_cell_props(e, p, h0, err);
+            eng_cell_props(e, p, h0, err, p.dhh.x, p.ht.x);
         } else {
             e.t[h0] = e.ws[w][0];
             e.h[h0] = e.ws[w][1];
@@ -300,7 +343,8 @@ struct chs_eng {
     EngDev e{};
     double tgam[4], troh[4];
     size_t hs = 0;             // halo field size
-    double *d0 = nullptr, *m0 = nullptr, *dh0 = nullptr;
+    double *d0 = nullptr, *m0 = nullptr;   // DENSITY0, Viscousity0 (swapped with e.d, e.m each substep)
+    double *gown = nullptr;                 // own G (e.g1.. may point at a momentum context's G)
     std::string errmsg;
 };
 
@@ -329,10 +373,19 @@ int eng_check(chs_eng *e, const char *what)
     return CHS_OK;
 }
 
+void eng_own_g(chs_eng *e)
+{
+    e->e.g1 = e->gown;
+    e->e.g2 = e->gown + e->hs;
+    e->e.g3 = e->gown + 2 * e->hs;
+}
+
 void eng_props_and_walls(chs_eng *e, int j0, int j1)
 {
     const dim3 blk(64, 4), grd((e->L + 63) / 64, (j1 - j0 + 4) / 4, e->N);
-    hipLaunchKernelGGL(k_eng_props, grd, blk, 0, e->stream, e->e, e->prop->p, j0, j1, e->err);
+    const size_t sh = 2 * sizeof(double) * (size_t)e->prop->p.n;
+    const int lds = sh <= 64 * 1024;
+    hipLaunchKernelGGL(k_eng_props, grd, blk, lds ? sh : 0, e->stream, e->e, e->prop->p, j0, j1, e->err, lds);
     hipLaunchKernelGGL(k_eng_walls, dim3((e->L + 255) / 256, e->N), dim3(256), 0, e->stream, e->e, e->prop->p,
                        e->err);
 }
@@ -358,9 +411,9 @@ int chs_eng_create(const chs_coeffs *c, const chs_prop_table *table, const chs_e
         delete e;
         return rc;
     }
-    // layout: 10 halo fields (dh h t d m k cp d0 m0 dh0), 3 G, rhs0, wallflux [2][N][L],
+    // layout: 9 halo fields (dh h t d m k cp d0 m0), 3 G, rhs0, wallflux [2][N][L],
     // dyfi/dyci/wff/wfb (M + 2 each, reference index J)
-    const size_t nb = 13 * e->hs + cs + 2 * (size_t)N * L + 4 * (size_t)(M + 2);
+    const size_t nb = 12 * e->hs + cs + 2 * (size_t)N * L + 4 * (size_t)(M + 2);
     hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he == hipSuccess) he = hipMalloc(&e->buf, sizeof(double) * nb);
     if (he == hipSuccess) he = hipMalloc(&e->err, sizeof(int));
@@ -374,7 +427,7 @@ int chs_eng_create(const chs_coeffs *c, const chs_prop_table *table, const chs_e
         geo[3 * (M + 2) + J] = c->wfb[J - 1];
     }
     double *f = e->buf;
-    double *wfd = f + 13 * e->hs + cs, *gd = wfd + 2 * (size_t)N * L;
+    double *wfd = f + 12 * e->hs + cs, *gd = wfd + 2 * (size_t)N * L;
     if (he == hipSuccess)
         he = hipMemcpyAsync(wfd, p->wallflux, sizeof(double) * 2 * N * L, hipMemcpyHostToDevice, e->stream);
     if (he == hipSuccess)
@@ -390,9 +443,10 @@ int chs_eng_create(const chs_coeffs *c, const chs_prop_table *table, const chs_e
     d.dyfi = gd; d.dyci = gd + (M + 2); d.wff = gd + 2 * (M + 2); d.wfb = gd + 3 * (M + 2);
     double **fl[7] = {&d.dh, &d.h, &d.t, &d.d, &d.m, &d.k, &d.cp};
     for (int a = 0; a < 7; ++a) *fl[a] = f + a * e->hs;
-    e->d0 = f + 7 * e->hs; e->m0 = f + 8 * e->hs; e->dh0 = f + 9 * e->hs;
-    d.g1 = f + 10 * e->hs; d.g2 = f + 11 * e->hs; d.g3 = f + 12 * e->hs;
-    d.rhs0 = f + 13 * e->hs;
+    e->d0 = f + 7 * e->hs; e->m0 = f + 8 * e->hs;
+    e->gown = f + 9 * e->hs;
+    d.g1 = e->gown; d.g2 = e->gown + e->hs; d.g3 = e->gown + 2 * e->hs;
+    d.rhs0 = f + 12 * e->hs;
     d.wallflux = wfd;
     d.wall_type0 = p->wall_type[0]; d.wall_type1 = p->wall_type[1];
     d.top_face = p->top_face > 0 ? p->top_face : M;
@@ -421,6 +475,7 @@ int chs_eng_upload(chs_eng *e, const double *dh, const double *g, const double *
 {
     if (!e || !dh || !g) return eng_fail(e, CHS_E_ARG, "chs_eng_upload: null argument");
     const size_t hsb = sizeof(double) * e->hs, cs = (size_t)e->N * e->M * e->L;
+    eng_own_g(e);
     hipError_t he = hipMemcpyAsync(e->e.dh, dh, hsb, hipMemcpyHostToDevice, e->stream);
     if (he == hipSuccess) he = hipMemcpyAsync((double *)e->e.g1, g, 3 * hsb, hipMemcpyHostToDevice, e->stream);
     if (he == hipSuccess)
@@ -434,6 +489,7 @@ int chs_eng_upload(chs_eng *e, const double *dh, const double *g, const double *
 int chs_eng_set_mass_flux(chs_eng *e, const double *g_dev)
 {
     if (!e || !g_dev) return eng_fail(e, CHS_E_ARG, "chs_eng_set_mass_flux: null argument");
+    eng_own_g(e);
     hipError_t he = hipMemcpyAsync((double *)e->e.g1, g_dev, 3 * sizeof(double) * e->hs, hipMemcpyDeviceToDevice,
                                    e->stream);
     if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string("chs_eng_set_mass_flux: ") + hipGetErrorString(he));
@@ -443,12 +499,12 @@ int chs_eng_set_mass_flux(chs_eng *e, const double *g_dev)
 int chs_substep_eng(chs_eng *e, int ns, double dt)
 {
     if (!e || ns < 1 || ns > 3 || !(dt > 0)) return eng_fail(e, CHS_E_ARG, "chs_substep_eng: bad argument");
-    const size_t hsb = sizeof(double) * e->hs;
-    // Viscousity0, DENSITY0, DH0 (SOLVERRK3_ENG_IO.f90:23-27), for the momentum substep
-    hipError_t he = hipMemcpyAsync(e->m0, e->e.m, hsb, hipMemcpyDeviceToDevice, e->stream);
-    if (he == hipSuccess) he = hipMemcpyAsync(e->d0, e->e.d, hsb, hipMemcpyDeviceToDevice, e->stream);
-    if (he == hipSuccess) he = hipMemcpyAsync(e->dh0, e->e.dh, hsb, hipMemcpyDeviceToDevice, e->stream);
-    if (he != hipSuccess) return eng_fail(e, CHS_E_HIP, std::string("chs_substep_eng: ") + hipGetErrorString(he));
+    // Viscousity0 and DENSITY0 (SOLVERRK3_ENG_IO.f90:23-27), for the momentum substep: the
+    // current arrays become the saved ones and the property update below rewrites every row of
+    // D and M (rows 1..M here, the ghost rows in the wall update), so no copy is needed.  DH0
+    // (:27) is never read by the reference and is not kept.
+    std::swap(e->e.d, e->d0);
+    std::swap(e->e.m, e->m0);
     const dim3 blk(64, 4), grd((e->L + 63) / 64, (e->M + 3) / 4, e->N);
     hipLaunchKernelGGL(k_eng_rhs_update, grd, blk, 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt);
     eng_props_and_walls(e, 1, e->M);
@@ -642,4 +698,15 @@ __attribute__((visibility("hidden"))) void chs_internal_eng_fields(chs_eng *e, h
     *d0 = e->d0;
     *m0 = e->m0;
     *g = const_cast<double *>(e->e.g1);
+}
+
+// Point an energy context's G at a momentum context's G array (3 halo fields) instead of
+// copying it (NULL: back to the context's own array, as chs_eng_upload and
+// chs_eng_set_mass_flux also do).
+__attribute__((visibility("hidden"))) void chs_internal_eng_set_g(chs_eng *e, const double *g)
+{
+    if (!g) g = e->gown;
+    e->e.g1 = g;
+    e->e.g2 = g + e->hs;
+    e->e.g3 = g + 2 * e->hs;
 }

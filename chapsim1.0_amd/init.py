@@ -26,3 +26,29 @@ def poiseuille_perturbed(coeffs: dict, seed: int, amp: float = 0.1):
     q[1, :, 2:M + 1, :] = pert[1, :, 1:, :]
     q[2, :, 1:M + 1, :] = pert[2]
     return q, np.zeros((N, M + 2, L))
+
+
+def heated_channel_initial(coeffs: dict, table: dict, seed: int, amp: float = 0.05, t_amp: float = 0.01):
+    """Seeded start of the heated main-domain channel: T / T0 = 1 + t_amp (smooth + noise)
+    mapped to rho h through the nondimensional table (monotone in T; the ghost rows take the
+    first / last cell's value), G = rho u with the laminar profile plus amp noise (v and w
+    noise only, zero on the walls) and zero pressure.  Returns DH, PR [N, M+2, L] and
+    G [3, N, M+2, L]; densities follow from DH in the energy context."""
+    L, M, N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
+    dyfi = np.asarray(coeffs["dyfi"])
+    ynd = np.concatenate([[-1.0], -1.0 + np.cumsum(1.0 / dyfi)])
+    ycc = 0.5 * (ynd[:-1] + ynd[1:])
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    x = np.arange(L) / L
+    z = np.arange(N) / N
+    T = 1.0 + t_amp * (np.sin(2 * np.pi * x)[None, None, :] * np.cos(2 * np.pi * z)[:, None, None]
+                       + 0.2 * rng.standard_normal((N, M, L)))
+    dh = np.zeros((N, M + 2, L))
+    dh[:, 1:M + 1, :] = np.interp(T, table["T"], table["DH"])
+    dh[:, 0, :] = dh[:, 1, :]
+    dh[:, M + 1, :] = dh[:, M, :]
+    g = np.zeros((3, N, M + 2, L))
+    g[0, :, 1:M + 1, :] = 1.5 * (1.0 - ycc ** 2)[None, :, None] + rng.uniform(-amp, amp, (N, M, L))
+    g[1, :, 2:M + 1, :] = rng.uniform(-amp, amp, (N, M - 1, L))
+    g[2, :, 1:M + 1, :] = rng.uniform(-amp, amp, (N, M, L))
+    return dh, g, np.zeros((N, M + 2, L))

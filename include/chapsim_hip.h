@@ -269,8 +269,8 @@ typedef struct {
 typedef struct chs_eng chs_eng;
 
 /* Thermal state of one channel on the current GPU: DH, H, T, D, M, K, CP (halo layout
- * (N, M+2, L)), the mass flux G (3 x halo), RHS_ENERGY0 (N, M, L) and the DENSITY0,
- * Viscousity0, DH0 copies the momentum substep reads. */
+ * (N, M+2, L)), the mass flux G (3 x halo), RHS_ENERGY0 (N, M, L) and DENSITY0, Viscousity0
+ * for the momentum substep (DH0, which the reference saves but never reads, is not kept). */
 int chs_eng_create(const chs_coeffs *mesh, const chs_prop_table *table, const chs_eng_params *p, chs_eng **out);
 void chs_eng_destroy(chs_eng *e);
 const char *chs_eng_last_error(const chs_eng *e);
@@ -284,7 +284,7 @@ int chs_eng_upload(chs_eng *e, const double *dh, const double *g, const double *
 int chs_eng_set_mass_flux(chs_eng *e, const double *g_dev);
 
 /* SOLVERRK3_ENG_io(NS) (SOLVERRK3_ENG_IO.f90:11-82) for the periodic channel with table
- * properties: DENSITY0, Viscousity0, DH0 <- current; RHS_ENERGY_EXPLICIT
+ * properties: DENSITY0, Viscousity0 <- current; RHS_ENERGY_EXPLICIT
  * (RHS_ENERGY_EXPLICIT.f90:11-155); DH += TGAM(NS) dt RHS + TROH(NS) dt RHS0 with the table
  * limiter (lines 34-48); THERM_PROP_UPDATE on J = 1..M; BC_WALL_THERMAL.  FMA contraction
  * off, reference operation order: matches the reference to rounding (1e-12 relative).
