@@ -22,6 +22,7 @@
 //   k_update        VELOUPDT_tg + PRCALC_tg + wall BCs
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1197,6 +1198,112 @@ __global__ void k_reduce_max(const double *__restrict__ part, int n, double *__r
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// x-z mean statistics, PP_MEAN_ZX_FLOW_TG (processing/PP_MEAN_ZX.f90:14-307), channel.  Per
+// row J the 122 statistics are (s): 0..3 U1 (u, v, w, p), 4..6 UP, 7..12 U2 (the reference's
+// upper-triangle storage index), 13..22 U3, 23..31 DVDL1 (3 x 3, row major), 32..40 DVDLP,
+// 41..121 DVDL2 (9 x 9, row major).  Thread (i, J) sums one group of them over k:
+// GRP 0 s = 0..22, GRP 1 s = 23..40, GRP 2..4 the DVDL2 rows l1 = 3 (GRP-2) .. 3 (GRP-2)+2.
+template <int GRP>
+__global__ __launch_bounds__(64) void k_mean_zx(Geo g, const double *__restrict__ q, const double *__restrict__ pr,
+                                                double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    constexpr int NS = GRP == 0 ? 23 : (GRP == 1 ? 18 : 27);
+    constexpr int S0 = GRP == 0 ? 0 : (GRP == 1 ? 23 : 41 + 27 * (GRP - 2));
+    const int L = g.L, N = g.N;
+    const int i = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y + 1;
+    if (i >= L) return;
+    const size_t hs = (size_t)N * (g.M + 2) * L;
+    const double *U = q, *V = q + hs, *W = q + 2 * hs;
+    const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+    const double dycp = g.dyci[J + 1], dyc = g.dyci[J], dyf = g.dyfi[J - 1];
+    double acc[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) acc[t] = 0.0;
+    for (int k = 0; k < N; ++k) {
+        const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
+        auto X = [&](const double *f, int ii, int jj, int kk) { return f[hix(g, ii, jj, kk)]; };
+        double u[3];
+        u[0] = X(U, i, J, k) + X(U, ip, J, k);
+        u[1] = X(V, i, J, k) + X(V, i, J + 1, k);
+        u[2] = X(W, i, J, k) + X(W, i, J, kp);
+        const double p = X(pr, i, J, k);
+        if constexpr (GRP == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                acc[a] = acc[a] + u[a];
+                acc[4 + a] = acc[4 + a] + u[a] * p;
+            }
+            acc[3] = acc[3] + p;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = a; b < 3; ++b) {
+                    const int l = ((a + 1) * (6 - a)) / 2 + (b + 1) - 3;
+                    acc[7 + l - 1] = acc[7 + l - 1] + u[a] * u[b];
+                }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = a; b < 3; ++b)
+#pragma unroll
+                    for (int c = b; c < 3; ++c) {
+                        const int l = (a + 1) * (5 - a) + ((b + 1) * (6 - b)) / 2 + (c + 1) - 8;
+                        acc[13 + l - 1] = acc[13 + l - 1] + u[a] * u[b] * u[c];
+                    }
+        } else {
+            const double ujp0 = X(U, i, J + 1, k) + X(U, ip, J + 1, k), ujm0 = X(U, i, J - 1, k) + X(U, ip, J - 1, k);
+            const double ukp0 = X(U, i, J, kp) + X(U, ip, J, kp), ukm0 = X(U, i, J, km) + X(U, ip, J, km);
+            const double uip2 = X(W, ip, J, k) + X(W, ip, J, kp), uim2 = X(W, im, J, k) + X(W, im, J, kp);
+            const double ujp2 = X(W, i, J + 1, k) + X(W, i, J + 1, kp), ujm2 = X(W, i, J - 1, k) + X(W, i, J - 1, kp);
+            const double uip1 = X(V, ip, J, k) + X(V, ip, J + 1, k), uim1 = X(V, im, J, k) + X(V, im, J + 1, k);
+            const double ukp1 = X(V, i, J, kp) + X(V, i, J + 1, kp), ukm1 = X(V, i, J, km) + X(V, i, J + 1, km);
+            double gr[3][3];
+            gr[0][0] = X(U, ip, J, k) - X(U, i, J, k);
+            gr[0][1] = (ujp0 - u[0]) * dycp + (u[0] - ujm0) * dyc;
+            gr[0][2] = ukp0 - ukm0;
+            gr[1][0] = uip1 - uim1;
+            gr[1][1] = (X(V, i, J + 1, k) - X(V, i, J, k)) * dyf;
+            gr[1][2] = ukp1 - ukm1;
+            gr[2][0] = uip2 - uim2;
+            gr[2][1] = (ujp2 - u[2]) * dycp + (u[2] - ujm2) * dyc;
+            gr[2][2] = X(W, i, J, kp) - X(W, i, J, k);
+            if constexpr (GRP == 1) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        acc[3 * a + b] = acc[3 * a + b] + gr[a][b];
+                        acc[9 + 3 * a + b] = acc[9 + 3 * a + b] + gr[a][b] * p;
+                    }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const int l1 = 3 * (GRP - 2) + r;
+                    const double gl = gr[l1 / 3][l1 % 3];
+#pragma unroll
+                    for (int l2 = 0; l2 < 9; ++l2) acc[9 * r + l2] = acc[9 * r + l2] + gl * gr[l2 / 3][l2 % 3];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NS; ++t) part[((size_t)(J - 1) * 122 + S0 + t) * L + i] = acc[t];
+}
+
+// Row sums of the partials in index order, times the statistic's coefficient.
+__global__ __launch_bounds__(256) void k_mean_zx_sum(int L, int n, const double *__restrict__ part,
+                                                     const double *__restrict__ coef, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const double *p = part + (size_t)t * L;
+    double s = 0.0;
+    for (int i = 0; i < L; ++i) s = s + p[i];
+    out[t] = s * coef[t % 122];
 }
 
 }  // namespace
@@ -2453,3 +2560,58 @@ int chs_ens_sync(chs_ens *e)
 // Stream of a context, for the library's other components that enqueue work behind its
 // solves (the main-domain momentum substep uses a context's Poisson solver).
 __attribute__((visibility("hidden"))) hipStream_t chs_internal_stream(chs_ctx *c) { return c ? c->stream : nullptr; }
+
+extern "C" int chs_mean_zx_tg(chs_ctx *c, double *stats, double *scal)
+{
+    if (!c || (!stats && !scal)) return fail(c, CHS_E_ARG, "chs_mean_zx_tg: null argument");
+    const int L = c->L, M = c->M, N = c->N;
+    // coefficients (PP_MEAN_ZX.f90:34-58, :261-292): VL1313_tg = 1 / (NCL1 NCL3)
+    const double vl = 1.0 / (double)(L * N);
+    const double cog1[3][3] = {{vl * c->dxi, vl * 0.25, vl * 0.25 * c->dzi},
+                               {vl * 0.25 * c->dxi, vl, vl * 0.25 * c->dzi},
+                               {vl * 0.25 * c->dxi, vl * 0.25, vl * c->dzi}};
+    std::vector<double> coef(122);
+    for (int a = 0; a < 3; ++a) coef[a] = vl * 0.5;
+    coef[3] = vl;
+    for (int a = 0; a < 3; ++a) coef[4 + a] = vl * 0.5;
+    for (int t = 0; t < 6; ++t) coef[7 + t] = vl * (0.5 * 0.5);
+    for (int t = 0; t < 10; ++t) coef[13 + t] = vl * (0.5 * 0.5 * 0.5);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) coef[23 + 3 * a + b] = coef[32 + 3 * a + b] = cog1[a][b];
+    for (int l1 = 0; l1 < 9; ++l1)
+        for (int l2 = 0; l2 < 9; ++l2) coef[41 + 9 * l1 + l2] = cog1[l1 / 3][l1 % 3] * cog1[l2 / 3][l2 % 3];
+    const size_t n = (size_t)M * 122;
+    double *buf = nullptr;
+    HIPCHK(c, hipMalloc(&buf, sizeof(double) * (n * L + 122 + n)));
+    double *part = buf, *dcoef = buf + n * L, *dout = dcoef + 122;
+    std::vector<double> hout(n);
+    hipError_t e = hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * 122, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const dim3 grd((L + 63) / 64, M);
+        hipLaunchKernelGGL(k_mean_zx<0>, grd, dim3(64), 0, c->stream, c->geo, c->q, c->pr, part);
+        hipLaunchKernelGGL(k_mean_zx<1>, grd, dim3(64), 0, c->stream, c->geo, c->q, c->pr, part);
+        hipLaunchKernelGGL(k_mean_zx<2>, grd, dim3(64), 0, c->stream, c->geo, c->q, c->pr, part);
+        hipLaunchKernelGGL(k_mean_zx<3>, grd, dim3(64), 0, c->stream, c->geo, c->q, c->pr, part);
+        hipLaunchKernelGGL(k_mean_zx<4>, grd, dim3(64), 0, c->stream, c->geo, c->q, c->pr, part);
+        hipLaunchKernelGGL(k_mean_zx_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, L, (int)n,
+                           part, dcoef, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hout.data(), dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(c, CHS_E_HIP, std::string("chs_mean_zx_tg: ") + hipGetErrorString(e));
+    if (stats) std::copy(hout.begin(), hout.end(), stats);
+    if (scal) {   // U1mean_WORK_tg, U1maxx_WORK_tg (PP_MEAN_ZX.f90:294-305), Area_inlet (COORDJR.f90:368-373)
+        double um = 0.0, ux = 0.0, area = 0.0;
+        for (int J = 1; J <= M; ++J) {
+            um = um + hout[(size_t)(J - 1) * 122] / c->dyfi[J - 1];
+            ux = std::max(ux, std::fabs(hout[(size_t)(J - 1) * 122]));
+        }
+        for (int J = 1; J <= M; ++J)
+            for (int k = 0; k < N; ++k) area = area + 1.0 / c->dyfi[J - 1] / c->dzi;
+        scal[0] = um / c->dzi * (double)N / area;
+        scal[1] = ux;
+    }
+    return 0;
+}

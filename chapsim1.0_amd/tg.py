@@ -75,6 +75,7 @@ SIGNATURES = {
     "chs_eng_set_mass_flux": (C.c_int, [C.c_void_p, C.c_void_p]),
     "chs_substep_eng": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_eng_download": (C.c_int, [C.c_void_p] + [_dp] * 8),
+    "chs_mean_zx_tg": (C.c_int, [C.c_void_p, _dp, _dp]),
     "chs_io_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "chs_io_destroy": (None, [C.c_void_p]),
     "chs_io_last_error": (C.c_char_p, [C.c_void_p]),
@@ -156,6 +157,7 @@ class ChannelTG:
     def __init__(self, coeffs: dict, mode: int = PLAIN_FP64, handle=None, owner=None):
         self.L, self.M, self.N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
         self.mode = mode
+        self._dyfi = np.array(coeffs["dyfi"], dtype=np.float64)
         self._keep = []
         self._owner = owner
         if handle is not None:          # a channel of an Ensemble (destroyed by it)
@@ -200,6 +202,29 @@ class ChannelTG:
         _check(lib().chs_download_fields(self._h, _p(q), _p(pr), _p(h0) if with_history else None),
                self._h)
         return (q, pr, h0) if with_history else (q, pr)
+
+    def PP_MEAN_ZX_FLOW_TG(self):
+        """x-z mean statistics (chs_mean_zx_tg): stats [M, 122] and [U1mean, U1maxx]."""
+        st = np.zeros((self.M, 122))
+        sc = np.zeros(2)
+        _check(lib().chs_mean_zx_tg(self._h, _p(st), _p(sc)), self._h)
+        return st, sc
+
+    def monitor(self, REN):
+        """Instantaneous monitor quantities of PP_MONITOR_TG (processing/PP_SCN_MONITOR.f90:
+        147-253) from the x-z means: Cf at the bottom and top walls, mean u, v, w and the
+        normal stresses uu, uv, vv, ww at the centre row NCL2 / 2, U1mean, U1maxx."""
+        st, sc = self.PP_MEAN_ZX_FLOW_TG()
+        M = self.M
+        dyfi = self._dyfi
+        cf_b = (st[0, 0] - 0.0) * dyfi[0] * 2.0 * 2.0 / REN
+        cf_t = (st[M - 1, 0] - 0.0) * dyfi[M - 1] * 2.0 * 2.0 / REN
+        j = M // 2 - 1
+        u1 = st[j, 0:3]
+        uu = [st[j, 7] - u1[0] * u1[0], st[j, 8] - u1[0] * u1[1], st[j, 10] - u1[1] * u1[1],
+              st[j, 12] - u1[2] * u1[2]]
+        return dict(cf_bottom=cf_b, cf_top=cf_t, u_centre=u1.tolist(), uu_uv_vv_ww=uu, u1mean=sc[0],
+                    u1maxx=sc[1])
 
     # -- the reference's call structure ----------------------------------------------
     def SOLVERRK3_MOM_tg(self, ns: int, dt: float, cvisc: float):

@@ -183,6 +183,17 @@ int chs_max_div(chs_ctx *ctx, double *maxdiv);
 /* Wait for all enqueued work on the context's stream. */
 int chs_sync(chs_ctx *ctx);
 
+/* x-z mean statistics of the current field, PP_MEAN_ZX_FLOW_TG (processing/PP_MEAN_ZX.f90:
+ * 14-307), computed on the device.  stats (host, [M][122], NULL to skip): per row J the
+ * statistics s = 0..3 U1xzL_tg (u, v, w, p), 4..6 UPxzL_tg, 7..12 U2xzL_tg, 13..22 U3xzL_tg
+ * (the reference's storage index of the upper-triangle products), 23..31 DVDL1xzL_tg (3 x 3,
+ * row major: s = 23 + 3 (i-1) + (j-1) for dU_i/dx_j), 32..40 DVDLPxzL_tg, 41..121
+ * DVDL2xzL_tg (9 x 9, row major); scal (host, 2, NULL to skip): U1mean_WORK_tg and
+ * U1maxx_WORK_tg (:294-305), the bulk velocity and the largest mean u, which the monitor
+ * (processing/PP_SCN_MONITOR.f90:147-253) prints with the wall skin friction.  Synchronous;
+ * sums in a fixed order (matches the reference to rounding). */
+int chs_mean_zx_tg(chs_ctx *ctx, double *stats, double *scal);
+
 /* Device pointers of the resident state (for zero-copy hosts): q (3 halo fields), pr. */
 int chs_device_fields(chs_ctx *ctx, double **q, double **pr);
 

@@ -58,6 +58,7 @@ def lib():
                            ("or_tdma_lines", [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
                            ("or_dft", [C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
                            ("or_io_props", [C.c_void_p, C.c_void_p, C.c_int]),
+                           ("or_mean_zx", [C.c_void_p, C.c_void_p, _dp, _dp]),
                            ("or_io_velocity", [C.c_void_p, C.c_void_p]),
                            ("or_io_convection", [C.c_void_p, C.c_void_p]),
                            ("or_io_divu", [C.c_void_p, C.c_void_p]),
@@ -140,6 +141,13 @@ class OracleTG:
     @property
     def dpgrns(self):
         return self._s.dpgrns
+
+    def mean_zx(self):
+        """PP_MEAN_ZX_FLOW_TG (or_mean_zx): stats [M, 122] and [U1mean, U1maxx]."""
+        out = np.zeros((self.mesh.M, 122))
+        scal = np.zeros(2)
+        lib().or_mean_zx(C.byref(self._m), C.byref(self._s), _p(out), _p(scal))
+        return out, scal
 
 
 def tdma_cyc(a, b, c, r):

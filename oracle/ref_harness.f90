@@ -50,6 +50,10 @@
 !                           DPH_io rows 0..N2DO+1, then EXPLT0_io, DrhoDtP, RHSLLPHI_io rows
 !                           1..N2DO
 !   dumpiopar FILE          rank 0: F_A, VOLM_io, CVISC
+!   meanzx FILE             PP_MEAN_ZX_FLOW_TG on the loaded TG field; each rank writes
+!                           FILE.<rank>: U1xzL_tg, UPxzL_tg, U2xzL_tg, U3xzL_tg, DVDL1xzL_tg,
+!                           DVDLPxzL_tg, DVDL2xzL_tg (rows 1..N2DO, Fortran order), then
+!                           U1mean_WORK_tg, U1maxx_WORK_tg
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -58,6 +62,7 @@
 ! Must run on 2 or more MPI ranks (the reference deadlocks on one, INTFC_VARS.f90:57-79).
 !-----------------------------------------------------------------------------------------------
 PROGRAM ref_harness
+    USE postprocess_info
     USE init_info
     USE mesh_info
     USE flow_info
@@ -262,6 +267,15 @@ PROGRAM ref_harness
                 WRITE(80) F_A, VOLM_io, CVISC
                 CLOSE(80)
             END IF
+        CASE ('meanzx')
+            READ(line, *) cmd, fname
+            CALL PP_MEAN_ZX_FLOW_TG
+            CALL H_OPENRANK(TRIM(fname), 80)
+            WRITE(80) U1xzL_tg(1:N2DO(MYID), :), UPxzL_tg(1:N2DO(MYID), :), U2xzL_tg(1:N2DO(MYID), :), &
+                      U3xzL_tg(1:N2DO(MYID), :), DVDL1xzL_tg(1:N2DO(MYID), :, :), &
+                      DVDLPxzL_tg(1:N2DO(MYID), :, :), DVDL2xzL_tg(1:N2DO(MYID), :, :)
+            WRITE(80) U1mean_WORK_tg, U1maxx_WORK_tg
+            CLOSE(80)
         CASE ('tdma')
             READ(line, *) cmd, idr, fname2, fname
             IF (MYID == 0) CALL H_TDMA(idr, TRIM(fname2), TRIM(fname))

@@ -309,3 +309,34 @@ def read_ioterms(h: "Harness", name):
     """``ioterms`` dump: H [3, N, M, L] (x, y, z momentum terms) and DIVU [N, M, L]."""
     a = h.read_cells(name, nvars=4)
     return dict(H=a[:3], DIVU=a[3])
+
+
+def read_meanzx(h: "Harness", name):
+    """``meanzx`` dump -> (stats [M, 122] in or_mean_zx's order, [U1mean, U1maxx])."""
+    rows, tail = [], None
+    for r in range(h.nranks):
+        with open(h.path(f"{name}.{r}"), "rb") as fh:
+            hdr = np.frombuffer(fh.read(24), dtype=np.int32)
+            data = np.frombuffer(fh.read(), dtype=np.float64)
+        myid, j0, n2, L, M, N = (int(v) for v in hdr)
+        off = 0
+
+        def take(n):
+            nonlocal off
+            a = data[off:off + n]
+            off += n
+            return a
+
+        u1 = take(4 * n2).reshape(4, n2)
+        up = take(3 * n2).reshape(3, n2)
+        u2 = take(6 * n2).reshape(6, n2)
+        u3 = take(10 * n2).reshape(10, n2)
+        d1 = take(9 * n2).reshape(3, 3, n2)      # [N][M][J]
+        dp = take(9 * n2).reshape(3, 3, n2)
+        d2 = take(81 * n2).reshape(9, 9, n2)     # [L2][L1][J]
+        tail = take(2)
+        blk = np.concatenate([u1, up, u2, u3, d1.transpose(1, 0, 2).reshape(9, n2),
+                              dp.transpose(1, 0, 2).reshape(9, n2), d2.transpose(1, 0, 2).reshape(81, n2)])
+        rows.append((j0, blk.T))
+    rows.sort(key=lambda x: x[0])
+    return np.concatenate([b for _, b in rows]), np.array(tail)

@@ -1178,3 +1178,110 @@ void or_io_substep(const or_mesh *m, const or_io_par *p, or_io *s, int ns, doubl
 }
 #undef V3
 #undef C3
+
+/* ======================================================================================
+ * x-z mean statistics of the TG field: PP_MEAN_ZX_FLOW_TG (processing/PP_MEAN_ZX.f90:14-307),
+ * channel.  Per row J (out[(J-1)*122 + s]): s = 0..3 U1 (u, v, w, p), 4..6 UP, 7..12 U2
+ * (upper triangle, the reference's storage index), 13..22 U3, 23..31 DVDL1 (3 x 3, row
+ * major), 32..40 DVDLP, 41..121 DVDL2 (9 x 9, row major).  scal[0] = U1mean_WORK_tg
+ * (its / DZI * NCL3 / Area_inlet form), scal[1] = U1maxx_WORK_tg.  Summation order I, then K,
+ * as the reference.
+ * ====================================================================================== */
+void or_mean_zx(const or_mesh *m, const or_state *s, double *out, double *scal)
+{
+    const double vl = 1.0 / (double)(LL * NN);
+    const double coe0 = vl, coe1 = vl * 0.5, coe2 = vl * (0.5 * 0.5), coe3 = vl * (0.5 * 0.5 * 0.5);
+    double cog1[3][3] = {{vl * m->dxi, vl * 0.25, vl * 0.25 * m->dzi},
+                         {vl * 0.25 * m->dxi, vl, vl * 0.25 * m->dzi},
+                         {vl * 0.25 * m->dxi, vl * 0.25, vl * m->dzi}};
+    double u1mean = 0.0, u1max = 0.0;
+    for (int J = 1; J <= MM; ++J) {
+        double *o = out + (size_t)(J - 1) * 122;
+        for (int t = 0; t < 122; ++t) o[t] = 0.0;
+        for (int i = 0; i < LL; ++i) {
+            const int ip = ip_(m, i), im = im_(m, i);
+            for (int k = 0; k < NN; ++k) {
+                const int kp = kp_(m, k), km = km_(m, k);
+                double u[3], ujp[3], ujm[3], ukp[3], ukm[3], uip[3], uim[3];
+                u[0] = QC(0, i, J, k) + QC(0, ip, J, k);
+                ujp[0] = QC(0, i, J + 1, k) + QC(0, ip, J + 1, k);
+                ujm[0] = QC(0, i, J - 1, k) + QC(0, ip, J - 1, k);
+                ukp[0] = QC(0, i, J, kp) + QC(0, ip, J, kp);
+                ukm[0] = QC(0, i, J, km) + QC(0, ip, J, km);
+                u[2] = QC(2, i, J, k) + QC(2, i, J, kp);
+                uip[2] = QC(2, ip, J, k) + QC(2, ip, J, kp);
+                uim[2] = QC(2, im, J, k) + QC(2, im, J, kp);
+                ujp[2] = QC(2, i, J + 1, k) + QC(2, i, J + 1, kp);
+                ujm[2] = QC(2, i, J - 1, k) + QC(2, i, J - 1, kp);
+                u[1] = QC(1, i, J, k) + QC(1, i, J + 1, k);
+                uip[1] = QC(1, ip, J, k) + QC(1, ip, J + 1, k);
+                uim[1] = QC(1, im, J, k) + QC(1, im, J + 1, k);
+                ukp[1] = QC(1, i, J, kp) + QC(1, i, J + 1, kp);
+                ukm[1] = QC(1, i, J, km) + QC(1, i, J + 1, km);
+                const double p = HX(s->pr, i, J, k);
+                for (int a = 0; a < 3; ++a) {
+                    o[a] = o[a] + u[a];
+                    o[4 + a] = o[4 + a] + u[a] * p;
+                }
+                o[3] = o[3] + p;
+                for (int a = 0; a < 3; ++a)
+                    for (int b = a; b < 3; ++b) {
+                        const int l = ((a + 1) * (6 - a)) / 2 + (b + 1) - 3;   /* 1-based */
+                        o[7 + l - 1] = o[7 + l - 1] + u[a] * u[b];
+                    }
+                for (int a = 0; a < 3; ++a)
+                    for (int b = a; b < 3; ++b)
+                        for (int c = b; c < 3; ++c) {
+                            const int l = (a + 1) * (5 - a) + ((b + 1) * (6 - b)) / 2 + (c + 1) - 8;
+                            o[13 + l - 1] = o[13 + l - 1] + u[a] * u[b] * u[c];
+                        }
+                double g[3][3];
+                g[0][0] = QC(0, ip, J, k) - QC(0, i, J, k);
+                g[0][1] = (ujp[0] - u[0]) * m->dyci[J + 1] + (u[0] - ujm[0]) * m->dyci[J];
+                g[0][2] = ukp[0] - ukm[0];
+                g[1][0] = uip[1] - uim[1];
+                g[1][1] = (QC(1, i, J + 1, k) - QC(1, i, J, k)) * m->dyfi[J - 1];
+                g[1][2] = ukp[1] - ukm[1];
+                g[2][0] = uip[2] - uim[2];
+                g[2][1] = (ujp[2] - u[2]) * m->dyci[J + 1] + (u[2] - ujm[2]) * m->dyci[J];
+                g[2][2] = QC(2, i, J, kp) - QC(2, i, J, k);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) {
+                        o[23 + 3 * a + b] = o[23 + 3 * a + b] + g[a][b];
+                        o[32 + 3 * a + b] = o[32 + 3 * a + b] + g[a][b] * p;
+                    }
+                for (int a = 0; a < 3; ++a)
+                    for (int h = 0; h < 3; ++h)
+                        for (int b = 0; b < 3; ++b)
+                            for (int c = 0; c < 3; ++c) {
+                                const int l1 = 3 * a + h, l2 = 3 * b + c;
+                                o[41 + 9 * l1 + l2] = o[41 + 9 * l1 + l2] + g[a][h] * g[b][c];
+                            }
+            }
+        }
+        o[3] = o[3] * coe0;
+        for (int a = 0; a < 3; ++a) o[a] = o[a] * coe1;
+        for (int a = 0; a < 3; ++a) o[4 + a] = o[4 + a] * coe1;
+        for (int t = 0; t < 6; ++t) o[7 + t] = o[7 + t] * coe2;
+        for (int t = 0; t < 10; ++t) o[13 + t] = o[13 + t] * coe3;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                o[23 + 3 * a + b] = o[23 + 3 * a + b] * cog1[a][b];
+                o[32 + 3 * a + b] = o[32 + 3 * a + b] * cog1[a][b];
+            }
+        for (int a = 0; a < 3; ++a)
+            for (int h = 0; h < 3; ++h)
+                for (int b = 0; b < 3; ++b)
+                    for (int c = 0; c < 3; ++c) {
+                        const int l1 = 3 * a + h, l2 = 3 * b + c;
+                        o[41 + 9 * l1 + l2] = o[41 + 9 * l1 + l2] * (cog1[a][h] * cog1[b][c]);
+                    }
+        u1mean = u1mean + o[0] / m->dyfi[J - 1];
+        u1max = fmax(u1max, fabs(o[0]));
+    }
+    double area = 0.0;   /* Area_inlet (COORDJR.f90:368-373) */
+    for (int J = 1; J <= MM; ++J)
+        for (int k = 0; k < NN; ++k) area = area + 1.0 / m->dyfi[J - 1] / m->dzi;
+    scal[0] = u1mean / m->dzi * (double)NN / area;
+    scal[1] = u1max;
+}

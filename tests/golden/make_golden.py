@@ -13,9 +13,10 @@ Arrays are C-order views of the reference's Fortran layout (z, y, x).
 Also ``props_2048.npz`` (property tables, make_props), ``tdma.npz`` (the reference's
 batched Thomas solvers, make_tdma) and ``fishpack.npz`` (the main-domain FISHPACK Poisson
 solve, make_fishpack), ``eng.npz`` (the energy substep, make_eng) and ``io.npz`` (the
-main-domain momentum substep, make_io).
+main-domain momentum substep, make_io) and ``meanzx.npz`` (x-z mean statistics,
+make_meanzx).
 
-Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng] [io]   (needs oracle/_ref)
+Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng] [io] [meanzx]   (needs oracle/_ref)
 """
 from __future__ import annotations
 
@@ -265,9 +266,33 @@ def make_io(T0=665.0):
     print("wrote", path, os.path.getsize(path))
 
 
+def make_meanzx():
+    """x-z mean statistics fixture (PP_MEAN_ZX_FLOW_TG) of the fields after one RK3 step held
+    in the tg_*.npz fixtures: stats [M, 122] (or_mean_zx's order) and U1mean / U1maxx."""
+    out = {}
+    for name in ("tg_16x16x16.npz", "tg_20x18x30.npz"):
+        g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), name))
+        L, M, N, STR2, REN, dt = (g["params"][i] for i in range(6))
+        L, M, N = int(L), int(M), int(N)
+        h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt)
+        try:
+            q, p = g["q_step1"], g["pr_step1"]
+            h.write_fields("f.bin", q[0], q[1], q[2], p)
+            h.run(["load f.bin", "meanzx mz"])
+            st, sc = refrun.read_meanzx(h, "mz")
+        finally:
+            h.close()
+        tag = name[:-4]
+        out[f"{tag}_stats"] = st
+        out[f"{tag}_scal"] = sc
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "meanzx.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 if __name__ == "__main__":
     import sys as _sys
-    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io"]
+    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io", "meanzx"]
     if "tg" in which:
         for case in CASES:
             make_case(*case)
@@ -281,3 +306,5 @@ if __name__ == "__main__":
         make_eng()
     if "io" in which:
         make_io()
+    if "meanzx" in which:
+        make_meanzx()
