@@ -76,6 +76,11 @@ SIGNATURES = {
     "chs_substep_eng": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_eng_download": (C.c_int, [C.c_void_p] + [_dp] * 8),
     "chs_mean_zx_tg": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "chs_restart_write_tg": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, _dp, _dp]),
+    "chs_restart_read_tg": (C.c_int, [C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                      _dp, _dp, _dp]),
+    "chs_restart_last_error": (C.c_char_p, []),
     "chs_io_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "chs_io_destroy": (None, [C.c_void_p]),
     "chs_io_last_error": (C.c_char_p, [C.c_void_p]),
@@ -483,3 +488,26 @@ def heated_substep(eng, io: ChannelIO, ns: int, dt: float):
     rc = lib().chs_heated_substep(eng._h, io._h, int(ns), float(dt))
     if rc != 0:
         raise ChsError(f"chs_heated_substep failed ({rc}): {lib().chs_io_last_error(io._h).decode()}")
+
+
+def restart_write_tg(directory, q, pr, iterg, phytime, REN, dt):
+    """WRT_INSTANT_VARS_TG file set (chs_restart_write_tg) from halo arrays q [3, N, M+2, L], pr."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    _, N, Mh, L = q.shape
+    rc = lib().chs_restart_write_tg(str(directory).encode(), L, Mh - 2, N, int(iterg), float(phytime), float(REN),
+                                    float(dt), _p(q), _p(pr))
+    if rc != 0:
+        raise ChsError(lib().chs_restart_last_error().decode())
+
+
+def restart_read_tg(directory, tfm, L, M, N):
+    """ReStart_INSTANT_VARS_TG(TFM) (chs_restart_read_tg): returns q, pr, ITERG, PhyTIME."""
+    q = np.zeros((3, N, M + 2, L))
+    pr = np.zeros((N, M + 2, L))
+    it = C.c_int(0)
+    t = np.zeros(1)
+    rc = lib().chs_restart_read_tg(str(directory).encode(), float(tfm), L, M, N, C.byref(it), _p(t), _p(q), _p(pr))
+    if rc != 0:
+        raise ChsError(lib().chs_restart_last_error().decode())
+    return q, pr, it.value, float(t[0])

@@ -306,6 +306,23 @@ int chs_substep_eng(chs_eng *e, int ns, double dt);
 int chs_eng_download(chs_eng *e, double *dh, double *h, double *t, double *d, double *m, double *k, double *cp,
                      double *rhs0);
 
+/* ---- Restart files (instantaneous TG field, the reference's format) ------------------- */
+
+/* WRT_INSTANT_VARS_TG (processing/WRT_VARS_TG.f90:12-104): four files
+ * <dir>/DNS_perixz_INSTANT_T<phytime as ES15.9>_{U,V,W,P}.D, each int32 NCL1, NCL2, NCL3,
+ * ITERG, float64 PhyTIME, REN, DT, then the L x M x N interior values in Fortran order.
+ * q [3][N][M+2][L], pr [N][M+2][L] host arrays (e.g. from chs_download_fields). */
+int chs_restart_write_tg(const char *dir, int L, int M, int N, int iterg, double phytime, double ren, double dt,
+                         const double *q, const double *pr);
+
+/* ReStart_INSTANT_VARS_TG(TFM) (processing/RESTART_tg.f90:11-56, READ_3D_VARS
+ * processing/READ_INTP_3D_VARS.f90:20-103) into q, pr (interior rows), then the wall ghost
+ * rows FLOWSTART sets after a restart (FLOWSTART.f90:37-40).  iterg, phytime (may be NULL)
+ * receive ITERG and PhyTIME.  CHS_E_ARG if a file is missing, short or of another mesh. */
+int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *iterg, double *phytime, double *q,
+                        double *pr);
+const char *chs_restart_last_error(void);
+
 /* ---- Main-domain momentum with variable density and viscosity ------------------------ */
 
 /* Physical constants of the main-domain momentum substep (READINI / Flow_State set-up):

@@ -54,6 +54,8 @@
 !                           FILE.<rank>: U1xzL_tg, UPxzL_tg, U2xzL_tg, U3xzL_tg, DVDL1xzL_tg,
 !                           DVDLPxzL_tg, DVDL2xzL_tg (rows 1..N2DO, Fortran order), then
 !                           U1mean_WORK_tg, U1maxx_WORK_tg
+!   rstwrite T IT           PhyTIME = T, ITERG = IT, WRT_INSTANT_VARS_TG (into FilePath1)
+!   rstread T               ReStart_INSTANT_VARS_TG(T) and the walls as FLOWSTART.f90:37-40
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -63,6 +65,7 @@
 !-----------------------------------------------------------------------------------------------
 PROGRAM ref_harness
     USE postprocess_info
+    USE wrt_info
     USE init_info
     USE mesh_info
     USE flow_info
@@ -276,6 +279,18 @@ PROGRAM ref_harness
                       DVDLPxzL_tg(1:N2DO(MYID), :, :), DVDL2xzL_tg(1:N2DO(MYID), :, :)
             WRITE(80) U1mean_WORK_tg, U1maxx_WORK_tg
             CLOSE(80)
+        CASE ('rstwrite')
+            READ(line, *) cmd, dtin, idr
+            PhyTIME = dtin
+            ITERG = idr
+            CALL WRT_INSTANT_VARS_TG
+        CASE ('rstread')
+            READ(line, *) cmd, dtin
+            CALL ReStart_INSTANT_VARS_TG(dtin)
+            CALL INTFC_VARS1(1, NCL1_tg, 1, NCL1_tg, PR_tg)
+            CALL INTFC_VARS3(1, NCL1_tg, 1, NCL1_tg, Q_tg)
+            CALL BC_WALL_Q_tg
+            CALL BC_WALL_PR_TG
         CASE ('tdma')
             READ(line, *) cmd, idr, fname2, fname
             IF (MYID == 0) CALL H_TDMA(idr, TRIM(fname2), TRIM(fname))
