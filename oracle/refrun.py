@@ -30,7 +30,7 @@ def ini_text(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, DT=1e-3,
              iterMonitor=1, iRandomType=2, VPER=0.6, SVPER=0.3, TimeReStart_tg=0.0,
              iDomain=1, iThermoDynamics=0, T0=680.15, Ti=680.15, L0=1.5e-3,
              iThermalWallType=(2, 2), thermalWallBC=(640.15, 660.15), iVisScheme=1,
-             iGravity=0, iFlowDriven=1) -> str:
+             iGravity=0, iFlowDriven=1, timeThermoStart=1.0e6) -> str:
     """Channel as key lines: iDomain=1 puts the mesh on the turbulence-generator domain
     (NCL1_tg, HX_tg), iDomain=2 on the main (IO) domain (NCL1_io, HX_io).
 
@@ -60,7 +60,7 @@ def ini_text(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, DT=1e-3,
         f"TimeReStart_tg= {f(TimeReStart_tg)}", "TimeReStart_io= 0.0", "iIniFieldType= 0",
         "iIniFieldTime= 0",
         "[numerics]", f"NTSTF= {NTSTF}", f"TSTOP= {f(TSTOP)}", "timeFlowStart= 0.0",
-        "timeThermoStart= 1.0E6", f"DT= {f(DT)}", "DtMin= 1.0E-9", f"CFLGV= {f(CFLGV)}",
+        f"timeThermoStart= {f(timeThermoStart)}", f"DT= {f(DT)}", "DtMin= 1.0E-9", f"CFLGV= {f(CFLGV)}",
         "[methods]", f"iVisScheme= {iVisScheme}", "iWeightedPre= 0",
         "[statistics]", f"dtSave1= {f(dtSave1)}", f"tRunAve= {f(dtSave1)}",
         "tRunAve_Reset= 0.0", f"dtAveView= {f(dtSave1)}", f"dtRawView= {f(dtSave1)}",
