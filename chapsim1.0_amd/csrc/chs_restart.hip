@@ -5,6 +5,7 @@
 // interior values in Fortran order (MPI native representation, little-endian here);
 // ReStart_INSTANT_VARS_TG (processing/RESTART_tg.f90:11-56) with READ_3D_VARS
 // (processing/READ_INTP_3D_VARS.f90:20-103) reads them back.
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -109,3 +110,140 @@ int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *i
 const char *chs_restart_last_error(void) { return g_rst_err.c_str(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Interpolated start from a coarser field: INITIAL_INTERP_tg (processing/INTERPOLATION_INITIAL.f90:
+// 189-300), INITIAL_INTERP_MESH (:51-186), INTP_3D_VARS (processing/READ_INTP_3D_VARS.f90:106-165).
+// ---------------------------------------------------------------------------------------
+namespace {
+
+// SPLINE with both end slopes given as 0 (the callers pass DP1 = DPN = 0,
+// READ_INTP_3D_VARS.f90:341-386): second derivatives y2 of the interpolating cubic spline.
+void spline0(int n, const double *x, const double *y, double *y2, double *u)
+{
+    y2[0] = -0.5;
+    u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - 0.0);
+    for (int i = 1; i < n - 1; ++i) {
+        const double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        const double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) /
+                    (x[i + 1] - x[i - 1]) -
+                sig * u[i - 1]) /
+               p;
+    }
+    const double qn = 0.5, un = (3.0 / (x[n - 1] - x[n - 2])) * (0.0 - (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]));
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+// SPLINT (READ_INTP_3D_VARS.f90:389-432): bisection, then the cubic of the bracketing interval
+// (extrapolated beyond the end points).
+double splint(int n, const double *xa, const double *ya, const double *y2a, double x)
+{
+    int klo = 1, khi = n;
+    while (khi - klo > 1) {
+        const int k = (khi + klo) / 2;
+        if (xa[k - 1] > x) khi = k;
+        else klo = k;
+    }
+    const double h = xa[khi - 1] - xa[klo - 1];
+    const double a = (xa[khi - 1] - x) / h, b = (x - xa[klo - 1]) / h;
+    return a * ya[klo - 1] + b * ya[khi - 1] +
+           ((a * a * a - a) * y2a[klo - 1] + (b * b * b - b) * y2a[khi - 1]) * (h * h) / 6.0;
+}
+
+// One 1-D interpolation along a line of n old values with stride so, onto nn new points with
+// stride sn.
+void interp_line(int n, const double *xo, const double *fo, long so, int nn, const double *xn, double *fn, long sn,
+                 std::vector<double> &buf)
+{
+    buf.resize(4 * (size_t)n);
+    double *f = buf.data(), *y2 = f + n, *u = y2 + n;
+    for (int i = 0; i < n; ++i) f[i] = fo[(long)i * so];
+    spline0(n, xo, f, y2, u);
+    for (int i = 0; i < nn; ++i) fn[(long)i * sn] = splint(n, xo, f, y2, xn[i]);
+}
+
+}  // namespace
+
+extern "C" int chs_interp_initial_tg(int Loo, int Mo, int Noo, double HXoo, double HZoo, const double *yndo,
+                                     const double *uo, const double *vo, const double *wo, const double *po, int L,
+                                     int M, int N, double HX, double HZ, double STR2, double *q, double *pr)
+{
+    if (Loo < 2 || Mo < 2 || Noo < 2 || L < 2 || M < 2 || N < 2 || !yndo || !uo || !vo || !wo || !po || !q || !pr ||
+        !(HXoo > 0) || !(HZoo > 0) || !(HX > 0) || !(HZ > 0))
+        return rst_fail(CHS_E_ARG, "chs_interp_initial_tg: bad argument");
+    // INITIAL_INTERP_MESH: the given box repeated to cover the target box (:113-114)
+    const int Lo = (int)std::ceil(HX / (HXoo / (double)Loo)), No = (int)std::ceil(HZ / (HZoo / (double)Noo));
+    std::vector<double> xndo(Lo + 1), xcco(Lo), zndo(No + 1), zcco(No), ycco(Mo);
+    for (int i = 0; i <= Lo; ++i) xndo[i] = (double)i * HX / (double)Lo;
+    for (int k = 0; k <= No; ++k) zndo[k] = (double)k * HZ / (double)No;
+    for (int i = 0; i < Lo; ++i) xcco[i] = 0.5 * (xndo[i] + xndo[i + 1]);
+    for (int k = 0; k < No; ++k) zcco[k] = 0.5 * (zndo[k] + zndo[k + 1]);
+    for (int j = 0; j < Mo; ++j) ycco[j] = 0.5 * (yndo[j] + yndo[j + 1]);
+    // target mesh (CONSPARA: DX = HX / NCL1, DZ = HZ / NCL3; COORDJR.f90:44-55, :166-194)
+    const double dx = HX / (double)L, dz = HZ / (double)N;
+    std::vector<double> xnd(L + 1), xcc(L), znd(N + 1), zcc(N), ynd(M + 1), ycc(M), dyfi(M);
+    for (int i = 0; i <= L; ++i) xnd[i] = (double)i * dx;
+    for (int i = 0; i < L; ++i) xcc[i] = 0.5 * (xnd[i] + xnd[i + 1]);
+    for (int k = 0; k <= N; ++k) znd[k] = (double)k * dz;
+    for (int k = 0; k < N; ++k) zcc[k] = 0.5 * (znd[k] + znd[k + 1]);
+    for (int j = 0; j <= M; ++j) {
+        const double x2 = (double)j / (double)M;
+        ynd[j] = std::tanh(STR2 * (x2 - 0.5)) / std::tanh(STR2 * 0.5);
+    }
+    for (int j = 0; j < M; ++j) {
+        ycc[j] = (ynd[j] + ynd[j + 1]) * 0.5;
+        dyfi[j] = 1.0 / (ynd[j + 1] - ynd[j]);
+    }
+    const size_t hs = (size_t)N * (M + 2) * L;
+    const double *src[4] = {uo, vo, wo, po};
+    std::vector<double> qo((size_t)Lo * Mo * No), qx((size_t)L * No), qxz((size_t)L * Mo * N), buf;
+    for (int v = 0; v < 4; ++v) {
+        // QO(I, J, K) = DUMMY(MOD(I, NCLOO1), J, MOD(K, NCLOO3)) (:221-231), layout [k][j][i]
+        for (int k = 0; k < No; ++k)
+            for (int j = 0; j < Mo; ++j)
+                for (int i = 0; i < Lo; ++i)
+                    qo[((size_t)k * Mo + j) * Lo + i] = src[v][((size_t)(k % Noo) * Mo + j) * Loo + (i % Loo)];
+        const double *xo = v == 0 ? xndo.data() : xcco.data(), *xn = v == 0 ? xnd.data() : xcc.data();
+        const double *zo = v == 2 ? zndo.data() : zcco.data(), *zn = v == 2 ? znd.data() : zcc.data();
+        const double *yo = v == 1 ? yndo : ycco.data(), *yn = v == 1 ? ynd.data() : ycc.data();
+        for (int j = 0; j < Mo; ++j) {
+            for (int k = 0; k < No; ++k)   // x: INTRPRI (:172-206)
+                interp_line(Lo, xo, &qo[((size_t)k * Mo + j) * Lo], 1, L, xn, &qx[(size_t)k * L], 1, buf);
+            for (int i = 0; i < L; ++i)    // z: INTRPRK (:215-247)
+                interp_line(No, zo, &qx[i], L, N, zn, &qxz[(size_t)j * L + i], (long)Mo * L, buf);
+        }
+        double *dst = v < 3 ? q + v * hs : pr;
+        for (int k = 0; k < N; ++k)        // y: INTRPRJG (:290-337), into rows 1..M
+            for (int i = 0; i < L; ++i)
+                interp_line(Mo, yo, &qxz[(size_t)k * Mo * L + i], L, M, yn, &dst[((size_t)k * (M + 2) + 1) * L + i],
+                            L, buf);
+    }
+    // u / bulk velocity (BULK_VELOCITY_TG, INITIAL_FL_FLD_TG.f90:142-166; Area_inlet
+    // COORDJR.f90:368-373), then the walls of FLOWSTART.f90:37-40
+    double um = 0.0;
+    for (int j = 1; j <= M; ++j)
+        for (int i = 0; i < L; ++i)
+            for (int k = 0; k < N; ++k) um = um + q[((size_t)k * (M + 2) + j) * L + i] / dyfi[j - 1];
+    um = um / (1.0 / dz);
+    double area = 0.0;
+    for (int j = 0; j < M; ++j)
+        for (int k = 0; k < N; ++k) area = area + 1.0 / dyfi[j] / (1.0 / dz);
+    um = um / (area * (double)L);
+    for (int k = 0; k < N; ++k)
+        for (int j = 1; j <= M; ++j)
+            for (int i = 0; i < L; ++i) q[((size_t)k * (M + 2) + j) * L + i] = q[((size_t)k * (M + 2) + j) * L + i] / um;
+    for (int k = 0; k < N; ++k)
+        for (int i = 0; i < L; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                q[c * hs + ((size_t)k * (M + 2)) * L + i] = 0.0;
+                q[c * hs + ((size_t)k * (M + 2) + M + 1) * L + i] = 0.0;
+            }
+            q[hs + ((size_t)k * (M + 2) + 1) * L + i] = 0.0;
+            pr[((size_t)k * (M + 2)) * L + i] = pr[((size_t)k * (M + 2) + 1) * L + i];
+            pr[((size_t)k * (M + 2) + M + 1) * L + i] = pr[((size_t)k * (M + 2) + M) * L + i];
+        }
+    return CHS_OK;
+}

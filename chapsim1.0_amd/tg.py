@@ -81,6 +81,8 @@ SIGNATURES = {
     "chs_restart_read_tg": (C.c_int, [C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                       _dp, _dp, _dp]),
     "chs_restart_last_error": (C.c_char_p, []),
+    "chs_interp_initial_tg": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp,
+                                        C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "chs_io_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "chs_io_destroy": (None, [C.c_void_p]),
     "chs_io_last_error": (C.c_char_p, [C.c_void_p]),
@@ -511,3 +513,21 @@ def restart_read_tg(directory, tfm, L, M, N):
     if rc != 0:
         raise ChsError(lib().chs_restart_last_error().decode())
     return q, pr, it.value, float(t[0])
+
+
+def interp_initial_tg(qo, po, HXoo, HZoo, yndo, L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8):
+    """INITIAL_INTERP_tg (chs_interp_initial_tg) from a coarse field given as halo arrays
+    qo [3, No, Mo+2, Lo], po [No, Mo+2, Lo] and its y nodes yndo [Mo+1]; returns q, pr on the
+    channel mesh (L, M, N, HX, HZ, STR2) with u scaled to unit bulk velocity."""
+    qo = np.asarray(qo, dtype=np.float64)
+    _, No, Mh, Lo = qo.shape
+    Mo = Mh - 2
+    parts = [np.ascontiguousarray(a[:, 1:Mo + 1, :]) for a in (qo[0], qo[1], qo[2], np.asarray(po, dtype=np.float64))]
+    yndo = np.ascontiguousarray(yndo, dtype=np.float64)
+    q = np.zeros((3, N, M + 2, L))
+    pr = np.zeros((N, M + 2, L))
+    rc = lib().chs_interp_initial_tg(Lo, Mo, No, float(HXoo), float(HZoo), _p(yndo), *[_p(a) for a in parts],
+                                     L, M, N, float(HX), float(HZ), float(STR2), _p(q), _p(pr))
+    if rc != 0:
+        raise ChsError(lib().chs_restart_last_error().decode())
+    return q, pr

@@ -323,6 +323,20 @@ int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *i
                         double *pr);
 const char *chs_restart_last_error(void);
 
+/* Interpolated start from a coarser field, INITIAL_INTERP_tg (processing/INTERPOLATION_INITIAL.f90:
+ * 189-300) with INITIAL_INTERP_MESH (:51-186) and INTP_3D_VARS (processing/READ_INTP_3D_VARS.f90:
+ * 106-165): the given field (uo, vo, wo, po: Loo x Mo x Noo interior values, Fortran order, as
+ * the restart files hold them; box HXoo x HZoo; y nodes yndo[Mo+1], INTP_COORD_YND.ini) is
+ * repeated to cover the target box and interpolated along x, z, then y by cubic splines with
+ * zero end slopes (SPLINE / SPLINT, :341-432) onto the channel mesh (L, M, N, HX, HZ, STR2),
+ * u and w at their faces, v at the y nodes, p at the cell centres; u is then divided by the
+ * bulk velocity (BULK_VELOCITY_TG, INITIAL_FL_FLD_TG.f90:142-166) and the walls set as
+ * FLOWSTART does (FLOWSTART.f90:37-40).  Output halo arrays q [3][N][M+2][L], pr [N][M+2][L].
+ * Host only. */
+int chs_interp_initial_tg(int Loo, int Mo, int Noo, double HXoo, double HZoo, const double *yndo, const double *uo,
+                          const double *vo, const double *wo, const double *po, int L, int M, int N, double HX,
+                          double HZ, double STR2, double *q, double *pr);
+
 /* ---- Main-domain momentum with variable density and viscosity ------------------------ */
 
 /* Physical constants of the main-domain momentum substep (READINI / Flow_State set-up):

@@ -56,6 +56,9 @@
 !                           U1mean_WORK_tg, U1maxx_WORK_tg
 !   rstwrite T IT           PhyTIME = T, ITERG = IT, WRT_INSTANT_VARS_TG (into FilePath1)
 !   rstread T               ReStart_INSTANT_VARS_TG(T) and the walls as FLOWSTART.f90:37-40
+!   interp                  INITIAL_INTERP_tg (needs iIniField_tg = 1, the coarse files in
+!                           FilePath3 and INTP_COORD_YND.ini), then the walls as
+!                           FLOWSTART.f90:37-40
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -287,6 +290,12 @@ PROGRAM ref_harness
         CASE ('rstread')
             READ(line, *) cmd, dtin
             CALL ReStart_INSTANT_VARS_TG(dtin)
+            CALL INTFC_VARS1(1, NCL1_tg, 1, NCL1_tg, PR_tg)
+            CALL INTFC_VARS3(1, NCL1_tg, 1, NCL1_tg, Q_tg)
+            CALL BC_WALL_Q_tg
+            CALL BC_WALL_PR_TG
+        CASE ('interp')
+            CALL INITIAL_INTERP_tg
             CALL INTFC_VARS1(1, NCL1_tg, 1, NCL1_tg, PR_tg)
             CALL INTFC_VARS3(1, NCL1_tg, 1, NCL1_tg, Q_tg)
             CALL BC_WALL_Q_tg
