@@ -117,13 +117,37 @@ def load_pkg():
     return __graft_entry__.load()
 
 
+def start_field(pkg, c):
+    """The workload's start: the Re_tau 180 field (64^3, beta 1.8, REN 2800: 100 RK3 steps from
+    the laminar profile with 10 % seeded noise, projected) interpolated onto the workload's mesh
+    (INITIAL_INTERP_tg restated, chs_interp_initial_tg) and re-perturbed by 5 % (seeded); the
+    NS = 0 substep then projects it.  For the 64^3 workload itself: laminar + 5 %."""
+    from chapsim_amd import init
+    c1 = CFG64
+    if (c["L"], c["M"], c["N"]) == (c1["L"], c1["M"], c1["N"]):
+        return pkg.poiseuille_perturbed(pkg.channel_setup(c1["L"], c1["M"], c1["N"], c1["HX"], c1["HZ"], c1["STR2"]),
+                                        seed=395, amp=0.05)
+    co1 = pkg.channel_setup(c1["L"], c1["M"], c1["N"], c1["HX"], c1["HZ"], c1["STR2"])
+    ch1 = pkg.ChannelTG(co1, pkg.PLAIN_FP64)
+    q1, p1 = pkg.poiseuille_perturbed(co1, seed=180, amp=0.1)
+    ch1.upload(q1, p1)
+    ch1.SOLVERRK3_MOM_tg(0, c1["dt"], 1.0 / c1["REN"])
+    for _ in range(100):
+        ch1.rk3_step(c1["dt"], 1.0 / c1["REN"])
+    q1, p1 = ch1.download()
+    ch1.close()
+    q, pr = pkg.interp_initial_tg(q1, p1, c1["HX"], c1["HZ"], init.channel_ynd(c1["M"], c1["STR2"]),
+                                  c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
+    return init.add_perturbation(q, seed=395, amp=0.05), pr
+
+
 def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False, shapes=None,
             roundtrip=0):
     c = c or CFG
     coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
     ch = pkg.ChannelTG(coeffs, mode)
-    q, pr = pkg.poiseuille_perturbed(coeffs, seed=395, amp=0.05)
+    q, pr = start_field(pkg, c)
     ch.upload(q, pr)
     cv = 1.0 / c["REN"]
     dt = c["dt"]
@@ -544,7 +568,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.form == "plain" else "f32+f64",
-        "data": "synthetic: Poiseuille + 5% seeded (Philox) perturbation, projected by NS=0",
+        "data": "synthetic: Re_tau 180 field (64^3, 100 steps from laminar + 10% seeded noise) "
+                "interpolated to the mesh, + 5% seeded (Philox) perturbation, projected by NS=0",
         "config": {"workload": (f"periodic channel REN={c['REN']:g}, {c['L']}x{c['M']}x{c['N']}, "
                                 f"4pi x 2 x 2pi, tanh beta={c['STR2']:g}, dt={c['dt']:g}, "
                                 "RK3+CN, flow only"),

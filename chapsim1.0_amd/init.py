@@ -52,3 +52,21 @@ def heated_channel_initial(coeffs: dict, table: dict, seed: int, amp: float = 0.
     g[1, :, 2:M + 1, :] = rng.uniform(-amp, amp, (N, M - 1, L))
     g[2, :, 1:M + 1, :] = rng.uniform(-amp, amp, (N, M, L))
     return dh, g, np.zeros((N, M + 2, L))
+
+
+def channel_ynd(M: int, STR2: float):
+    """y nodes YND(1..M+1) of the tanh-stretched channel (COORDJR.f90:47-51)."""
+    x2 = np.arange(M + 1) / M
+    return np.tanh(STR2 * (x2 - 0.5)) / np.tanh(STR2 * 0.5)
+
+
+def add_perturbation(q, seed: int, amp: float):
+    """Seeded uniform perturbation U(-amp, amp) of u, w (rows 1..M) and v (rows 2..M), in place."""
+    _, N, Mh, L = q.shape
+    M = Mh - 2
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    pert = rng.uniform(-amp, amp, size=(3, N, M, L))
+    q[0, :, 1:M + 1, :] += pert[0]
+    q[1, :, 2:M + 1, :] += pert[1, :, 1:, :]
+    q[2, :, 1:M + 1, :] += pert[2]
+    return q
