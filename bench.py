@@ -1,9 +1,9 @@
 """Benchmark of the RK3/Crank-Nicolson channel time step on one MI355X.
 
-Workload: single periodic channel, flow only, Re_tau = 395 (REN = 6875),
-grid 256 x 192 x 256 (x, y, z), box 4 pi x 2 x 2 pi, y tanh-stretched with beta = 2.0,
-dt = 5e-4.  Synthetic start: Poiseuille + 5 % seeded perturbation, projected by the NS = 0
-substep.  A "step" is one full RK3 step (three substeps SOLVERRK3_MOM_tg(1..3)) of the whole
+Workload: single periodic channel, flow only, Re_tau = 395 (REN = 6875), grid 256 x 192 x 256
+(x, y, z), box 4 pi x 2 x 2 pi, y tanh-stretched with beta = 2.0, dt = 5e-4.  Synthetic start:
+the Re_tau 180 field (64^3, 100 steps from laminar + 10 % seeded noise) interpolated onto the
+mesh (chs_interp_initial_tg), re-perturbed by 5 % (seeded) and projected by the NS = 0 substep.  A "step" is one full RK3 step (three substeps SOLVERRK3_MOM_tg(1..3)) of the whole
 grid with all fields resident in HBM.
 
 Prints one JSON line.  ``value`` is grid-point RK3-steps/s of the form named in
@@ -17,9 +17,10 @@ reference build is absent.
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--form plain|mixed]
                        [--no-cpu-baseline] [--graph]
 Other workloads (each prints its own JSON line; not the default run):
-  --workload channel64   64^3 channel, Re_tau 180, dt 1e-3 (BASELINE config 1)
-  --workload poisson     standalone pressure Poisson 512x384x512, beta 2.2 (config 3)
-  --workload ensemble    32 channels of seeded uneven sizes, one hipGraph per step (config 4)
+  --workload channel64   64^3 channel, Re_tau 180, dt 1e-3
+  --workload poisson     standalone pressure Poisson 512x384x512, beta 2.2
+  --workload ensemble    32 channels of seeded uneven sizes, one hipGraph per step
+  --workload heated      heated channel 256x128x256 (energy + variable-property momentum)
 """
 from __future__ import annotations
 
