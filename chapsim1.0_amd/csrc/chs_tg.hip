@@ -2452,6 +2452,10 @@ int chs_profile_get(chs_ctx *c, const char *kernel, double *avg_ms, int *count)
 struct chs_ens {
     std::vector<chs_ctx *> ch;
     hipStream_t stream = nullptr;
+    // Capture lanes: channel i's launches go to lane i % lanes.size(), so the step graph holds
+    // independent chains that the GPU runs side by side (small channels alone fill little of it).
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> ev;   // ev[0] fork, ev[1 + s] join of lane s
     hipGraphExec_t gexec = nullptr;
     std::vector<double> gdt, gcv;
     std::vector<int> gtune;
@@ -2477,6 +2481,18 @@ int chs_ens_create(int n, const chs_coeffs *const *coeffs, int mode, chs_ens **o
         }
         e->ch.push_back(c);
     }
+    const int nl = std::min(n, 8);
+    e->ev.assign(nl + 1, nullptr);
+    for (int s = 0; s < nl && he == hipSuccess; ++s) {
+        hipStream_t st = nullptr;
+        he = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (he == hipSuccess) e->lanes.push_back(st);
+    }
+    for (int s = 0; s <= nl && he == hipSuccess; ++s) he = hipEventCreateWithFlags(&e->ev[s], hipEventDisableTiming);
+    if (he != hipSuccess) {
+        chs_ens_destroy(e);
+        return fail(nullptr, CHS_E_HIP, std::string("chs_ens_create: ") + hipGetErrorString(he));
+    }
     *out = e;
     return 0;
 }
@@ -2487,6 +2503,9 @@ void chs_ens_destroy(chs_ens *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto *c : e->ch) chs_destroy(c);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    for (auto st : e->lanes) (void)hipStreamDestroy(st);
+    for (auto ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -2521,13 +2540,26 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
         hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
         if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(he));
         int rc = 0;
-        // RK stages outermost so the channels' kernels of one stage are adjacent in the graph
+        // fork: every lane joins the capture behind the origin stream
+        const int nl = (int)e->lanes.size();
+        hipError_t ce = hipEventRecord(e->ev[0], e->stream);
+        for (int s = 0; s < nl && ce == hipSuccess; ++s) ce = hipStreamWaitEvent(e->lanes[s], e->ev[0], 0);
+        if (ce != hipSuccess) rc = fail(nullptr, CHS_E_HIP, std::string("chs_ens_step fork: ") + hipGetErrorString(ce));
+        // RK stages outermost; channel i on lane i % nl (its stages stay in order on that lane)
         for (int ns = 1; ns <= 3 && !rc; ++ns)
             for (int i = 0; i < n && !rc; ++i) {
                 chs_ctx *c = e->ch[i];
+                c->stream = e->lanes[i % nl];
                 rc = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt[i], cvisc[i])
                                           : substep_impl<double>(c, ns, dt[i], cvisc[i]);
             }
+        for (auto *c : e->ch) c->stream = e->stream;
+        // join: the origin stream waits for every lane before the capture ends
+        for (int s = 0; s < nl && ce == hipSuccess; ++s) {
+            ce = hipEventRecord(e->ev[1 + s], e->lanes[s]);
+            if (ce == hipSuccess) ce = hipStreamWaitEvent(e->stream, e->ev[1 + s], 0);
+        }
+        if (ce != hipSuccess && !rc) rc = fail(nullptr, CHS_E_HIP, std::string("chs_ens_step join: ") + hipGetErrorString(ce));
         he = hipStreamEndCapture(e->stream, &graph);
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
