@@ -50,8 +50,10 @@ __device__ __forceinline__ size_t csz(const IoDev &d) { return (size_t)d.N * d.M
     (void)N;
 
 // DENSITY_TIME_DERIVATION (DIVG.f90:249-317), DENSITY_Staggered (MASSFLUX_CALC_IO.f90:262-323)
-// and MU_Staggered (:175-259), rows J = 1..M+1.
-__global__ __launch_bounds__(256) void k_io_props(IoDev d, double tg, double tr)
+// and MU_Staggered (:175-259), rows J = 1..M+1; with vel != 0 also the velocity the next
+// momentum substep starts from (VELOCITY_CALC_io with BC_WALL_Q_io, as k_io_velocity), which
+// needs only this point's G and staggered density.
+__global__ __launch_bounds__(256) void k_io_props(IoDev d, double tg, double tr, int vel)
 {
 #pragma clang fp contract(off)
     IO_THREAD(1)
@@ -89,6 +91,13 @@ __global__ __launch_bounds__(256) void k_io_props(IoDev d, double tg, double tr)
         d.dstg[hs + c0] = d.wff[J] * D[c0] + d.wfb[J] * D[b0];
         d.mstg[c0] = ((V[c0] + V[hx(d, im, J, k)]) * d.wff[J] + (V[b0] + V[hx(d, im, J - 1, k)]) * d.wfb[J]) * h;
         d.mstg[2 * hs + c0] = ((V[hx(d, i, J, km)] + V[c0]) * d.wff[J] + (V[hx(d, i, J - 1, km)] + V[b0]) * d.wfb[J]) * h;
+    }
+    if (vel) {
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = c * hs + c0;
+            d.q[o] = (J == M + 1 || (c == 1 && J == 1)) ? 0.0 : d.g[o] / d.dstg[o];
+            if (J == 1) d.q[c * hs + hx(d, i, 0, k)] = 0.0;
+        }
     }
 }
 
@@ -133,21 +142,18 @@ __device__ double block_sum256(double v, double *sh)
     return sh[0];
 }
 
-// One momentum component C (0 x, 1 y, 2 z): convection (CONVECTION_X/Y/Z_io), viscous stress
-// divergence (VISCOUS_ALL_EXPLT_X/Y/Z_io, channel branch) and RHS_MOM_EXPLICIT_io(NS, C+1),
-// explicit branch: RK combination, EXPLT0 update, pressure gradient, buoyancy.  Rows 1..M
-// (2..M for v).  For C = 0 each block also writes its partial of sum(RHS / DYFI) for the
-// mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265).
+// One momentum component C (0 x, 1 y, 2 z) at a point: convection (CONVECTION_X/Y/Z_io),
+// viscous stress divergence (VISCOUS_ALL_EXPLT_X/Y/Z_io, channel branch) and
+// RHS_MOM_EXPLICIT_io(NS, C+1), explicit branch: RK combination, EXPLT0 update, pressure
+// gradient, buoyancy.  Rows 1..M (2..M for v).  Returns RHS / DYFI for C = 0 (the mass-flux
+// partial), 0 otherwise.
 template <int C>
-__global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restrict__ gs, double tg, double tr,
-                                                double ta, double dt)
+__device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__restrict__ gs, int i, int J, int k,
+                                               double tg, double tr, double ta, double dt)
 {
 #pragma clang fp contract(off)
-    __shared__ double sh[256];
-    IO_THREAD(1)
-    const bool on = i < L && J <= M && !(C == 1 && J == 1);
-    double contrib = 0.0;
-    if (on) {
+    {
+        const int L = d.L, N = d.N;
         const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
         const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
         const size_t hs = hsz(d);
@@ -268,13 +274,27 @@ __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restric
         r = r - pgm;
         if (d.gravity == C + 1) r = r + d.dstg[C * hs + H(i, J, k)] * (ta * dt * d.f_a);
         d.h[o] = r;
-        if (C == 0) contrib = r / d.dyfi[J];
+        return C == 0 ? r / d.dyfi[J] : 0.0;
     }
-    if constexpr (C == 0) {
-        const double s = block_sum256(contrib, sh);
-        if (threadIdx.x == 0 && threadIdx.y == 0)
-            d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// All three momentum components of a point in one pass, so G, Q, the viscosities and the
+// pressure are read once for the three; the block's partial of sum(RHS_u / DYFI) for the
+// mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265) is written as well.
+__global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restrict__ gs, double tg, double tr, double ta,
+                                                double dt)
+{
+    __shared__ double sh[256];
+    IO_THREAD(1)
+    double contrib = 0.0;
+    if (i < L && J <= M) {
+        contrib = io_mom_point<0>(d, gs, i, J, k, tg, tr, ta, dt);
+        if (J >= 2) io_mom_point<1>(d, gs, i, J, k, tg, tr, ta, dt);
+        io_mom_point<2>(d, gs, i, J, k, tg, tr, ta, dt);
     }
+    const double s = block_sum256(contrib, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
 }
 
 // DPGRNS = sum of the partials / VOLM_io, one block, fixed order.
@@ -387,6 +407,7 @@ struct chs_io {
     double *buf = nullptr;
     int npart = 0, flow_driven = 1;
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
+    bool q_fresh = false;   // Q = G / D_STG of the current G and D_STG (written by chs_io_props)
     double cf_given = 0.0, volm = 0.0;
     double tgam[4], troh[4], talp[4];
     std::string err;
@@ -492,6 +513,7 @@ int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr
     const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L, cb = sizeof(double) * (size_t)m->N * m->M * m->L;
     m->d.d0 = m->own_d0;
     m->d.m0 = m->own_m0;
+    m->q_fresh = false;
     const IoDev &d = m->d;
     hipError_t he = hipMemcpyAsync(d.g, g, 3 * hb, hipMemcpyHostToDevice, m->stream);
     if (he == hipSuccess)
@@ -513,6 +535,7 @@ int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev)
     const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L;
     m->d.d0 = m->own_d0;
     m->d.m0 = m->own_m0;
+    m->q_fresh = false;
     hipError_t he = hipMemcpyAsync(m->d.d0, d0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
     if (he == hipSuccess) he = hipMemcpyAsync(m->d.m0, m0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
     return io_hip(m, he, "chs_io_set_properties");
@@ -521,7 +544,8 @@ int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev)
 int chs_io_props(chs_io *m, int ns)
 {
     if (!m || ns < 1 || ns > 3) return io_fail(m, CHS_E_ARG, "chs_io_props: bad argument (NS 1..3)");
-    hipLaunchKernelGGL(k_io_props, io_grid(m, m->M + 1), dim3(64, 4), 0, m->stream, m->d, m->tgam[ns], m->troh[ns]);
+    hipLaunchKernelGGL(k_io_props, io_grid(m, m->M + 1), dim3(64, 4), 0, m->stream, m->d, m->tgam[ns], m->troh[ns], 1);
+    m->q_fresh = true;
     return io_hip(m, hipSuccess, "chs_io_props");
 }
 
@@ -536,11 +560,10 @@ int chs_substep_io(chs_io *m, int ns, double dt)
     const double tg = m->tgam[ns], tr = m->troh[ns], ta = m->talp[ns];
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
     hipError_t he = hipSuccess;
-    hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
+    if (!m->q_fresh) hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
+    m->q_fresh = false;
     hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
-    hipLaunchKernelGGL(k_io_mom<0>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
-    hipLaunchKernelGGL(k_io_mom<1>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
-    hipLaunchKernelGGL(k_io_mom<2>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+    hipLaunchKernelGGL(k_io_mom, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
     const double *dpg_dev = nullptr;
     double dpg = 0.0;
     if (m->flow_driven == 1) {
