@@ -1,5 +1,6 @@
 // Main-domain momentum substep with variable density and viscosity (SOLVERRK3_MOM_io,
-// SOLVERRK3_MOM_IO.f90:11-120, explicit viscous scheme) and the staggered properties SOLVE
+// SOLVERRK3_MOM_IO.f90:11-120, explicit or Crank-Nicolson viscous scheme) and the staggered
+// properties SOLVE
 // computes before it in a heated run (DENSITY_TIME_DERIVATION, DENSITY_Staggered, MU_Staggered;
 // SOLVE.f90:122-124), for the periodic plane channel (RCCI1 = RCCI2 = RNDI1 = 1, no inlet,
 // iWeightedPre = 0).  The pressure Poisson solve is the FFT99 one of a chs_ctx on the same
@@ -17,7 +18,7 @@
 #include "../../include/chapsim_hip.h"
 
 hipStream_t chs_internal_stream(chs_ctx *c);
-void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **g);
+void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **m, double **g);
 void chs_internal_eng_set_g(chs_eng *e, const double *g);
 
 namespace {
@@ -34,6 +35,11 @@ struct IoDev {
     double *explt0, *h;                       // 3 cell fields each
     double *drho, *divu, *rhsp, *dph;         // cell fields
     double *partial, *scal;
+    // Crank-Nicolson scheme (iVisScheme = 1)
+    double *m1;                               // Viscousity, the current viscosity (halo field)
+    double *work;                             // line-solve recurrences: 3 x 3 cell fields
+    const double *amvr, *apvr;                // [3][M+2] AMVR / APVR(J, IDR) at [IDR-1][J]
+    double amvr1, apvrn, dxqi, dzqi;
 };
 
 __device__ __forceinline__ size_t hx(const IoDev &d, int i, int J, int k) { return ((size_t)k * (d.M + 2) + J) * d.L + i; }
@@ -143,11 +149,17 @@ __device__ double block_sum256(double v, double *sh)
 }
 
 // One momentum component C (0 x, 1 y, 2 z) at a point: convection (CONVECTION_X/Y/Z_io),
-// viscous stress divergence (VISCOUS_ALL_EXPLT_X/Y/Z_io, channel branch) and
-// RHS_MOM_EXPLICIT_io(NS, C+1), explicit branch: RK combination, EXPLT0 update, pressure
-// gradient, buoyancy.  Rows 1..M (2..M for v).  Returns RHS / DYFI for C = 0 (the mass-flux
-// partial), 0 otherwise.
-template <int C>
+// the viscous terms and RHS_MOM_EXPLICIT_io(NS, C+1): RK combination, EXPLT0 update, pressure
+// gradient, buoyancy.  Explicit scheme (CN false): the full variable-mu stress divergence
+// (VISCOUS_ALL_EXPLT_X/Y/Z_io, channel branch) joins the convection term.  Crank-Nicolson
+// (CN true): the split of VISCOUS_PAR_EXPLT_X/Y/Z_io (VISCOUS_X_EXPLICIT_io.f90:131-293,
+// VISCOUS_Y_EXPLICIT_io.f90:278-434, VISCOUS_Z_EXPLICIT_io.f90:300-462) with the current
+// viscosity: its explicit part joins the convection term (EX + conv), its implicit part enters
+// the increment with alpha dt (RHS_MOM_EXPLICIT_io.f90:87-99).  Their density-correction parts
+// multiply DRHOI_STG, which DENSITY_IMPLICIT_add evaluates as 1/D_STG minus the reciprocal of
+// D_STG's own expression of the same DENSITY0 (MASSFLUX_CALC_IO.f90:332-416): exactly zero.
+// Rows 1..M (2..M for v).  Returns RHS / DYFI for C = 0 (the mass-flux partial), 0 otherwise.
+template <int C, bool CN>
 __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__restrict__ gs, int i, int J, int k,
                                                double tg, double tr, double ta, double dt)
 {
@@ -164,7 +176,8 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
         const double wffp = d.wff[J + 1], wfbp = d.wfb[J + 1], wff = d.wff[J], wfb = d.wfb[J];
         auto H = [&](int ii, int jj, int kk) { return hx(d, ii, jj, kk); };
         auto CI = [&](int ii, int jj, int kk) { return cx(d, ii, jj, kk); };
-        double acc, pgm;
+        double acc, pgm, imp = 0.0;
+        (void)imp;
         if constexpr (C == 0) {
             const double c1 = dxi * hf * hf, c2 = d.dyfi[J] * hf, c3 = dzi * hf * hf;
             double f = (G0[H(ip, J, k)] + G0[H(i, J, k)]) * (Q0[H(ip, J, k)] + Q0[H(i, J, k)]);
@@ -177,6 +190,39 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
             b = (G2[H(i, J, k)] + G2[H(im, J, k)]) * (Q0[H(i, J, k)] + Q0[H(i, J, km)]);
             const double h13 = (f - b) * c3;
             acc = -(h11 + h12 + h13);
+            if constexpr (CN) {
+                const double *V1 = d.m1;
+                const double e1 = -2.0 / 3.0 * dxi * cv, e2 = d.dyfi[J] * cv, e3 = dzi * cv;
+                double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
+                double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
+                double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
+                const double t11fe = V1[H(i, J, k)] * (dvdy + dwdz), t11fi = V1[H(i, J, k)] * dudx;
+                dudx = (Q0[H(i, J, k)] - Q0[H(im, J, k)]) * dxi;
+                dvdy = (Q1[H(im, J + 1, k)] - Q1[H(im, J, k)]) * d.dyfi[J];
+                dwdz = (Q2[H(im, J, kp)] - Q2[H(im, J, k)]) * dzi;
+                const double t11be = V1[H(im, J, k)] * (dvdy + dwdz), t11bi = V1[H(im, J, k)] * dudx;
+                const double d11e = (t11fe - t11be) * e1, d11i = (t11fi - t11bi) * e1 * (-2.0);
+                double vis = d.mstg[H(i, J + 1, k)];
+                double dvdx = (Q1[H(i, J + 1, k)] - Q1[H(im, J + 1, k)]) * dxi;
+                double dudy = (Q0[H(i, J + 1, k)] - Q0[H(i, J, k)]) * d.dyci[J + 1];
+                const double t12fe = vis * dvdx, t12fi = vis * dudy;
+                vis = d.mstg[H(i, J, k)];
+                dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
+                dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
+                const double t12be = vis * dvdx, t12bi = vis * dudy;
+                const double d12e = (t12fe - t12be) * e2, d12i = (t12fi - t12bi) * e2;
+                vis = d.mstg[hs + H(i, J, kp)];
+                double dudz = (Q0[H(i, J, kp)] - Q0[H(i, J, k)]) * dzi;
+                double dwdx = (Q2[H(i, J, kp)] - Q2[H(im, J, kp)]) * dxi;
+                const double t13fe = vis * dwdx, t13fi = vis * dudz;
+                vis = d.mstg[hs + H(i, J, k)];
+                dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
+                dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
+                const double t13be = vis * dwdx, t13bi = vis * dudz;
+                const double d13e = (t13fe - t13be) * e3, d13i = (t13fi - t13bi) * e3;
+                acc = d11e + d12e + d13e + acc;
+                imp = d11i + d12i + d13i;
+            } else {
             const double v1 = 2.0 * dxi * cv, v2 = d.dyfi[J] * cv, v3 = dzi * cv;
             double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
             const double t11f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dudx - DU[CI(i, J, k)]);
@@ -198,6 +244,7 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
             const double t13b = d.mstg[hs + H(i, J, k)] * (dudz + dwdx);
             const double d13 = (t13f - t13b) * v3;
             acc = acc + d11 + d12 + d13;
+            }
             pgm = (d.pr[H(i, J, k)] - d.pr[H(im, J, k)]) * (ta * dt * dxi);
         } else if constexpr (C == 1) {
             const double c1 = dxi * hf, c2 = d.dyci[J] * hf * hf, c3 = dzi * hf;
@@ -211,6 +258,39 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
             b = (wff * G2[H(i, J, k)] + wfb * G2[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(i, J, km)]);
             const double h23 = (f - b) * c3;
             acc = -(h21 + h22 + h23);
+            if constexpr (CN) {
+                const double *V1 = d.m1;
+                const double e1 = dxi * cv, e3 = dzi * cv, e2 = -2.0 / 3.0 * d.dyci[J] * cv;
+                double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
+                double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
+                double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
+                const double t22fe = V1[H(i, J, k)] * (dudx + dwdz), t22fi = V1[H(i, J, k)] * dvdy;
+                dudx = (Q0[H(ip, J - 1, k)] - Q0[H(i, J - 1, k)]) * dxi;
+                dvdy = (Q1[H(i, J, k)] - Q1[H(i, J - 1, k)]) * d.dyfi[J - 1];
+                dwdz = (Q2[H(i, J - 1, kp)] - Q2[H(i, J - 1, k)]) * dzi;
+                const double t22be = V1[H(i, J - 1, k)] * (dudx + dwdz), t22bi = V1[H(i, J - 1, k)] * dvdy;
+                const double d22e = (t22fe - t22be) * e2, d22i = (t22fi - t22bi) * e2 * (-2.0);
+                double vis = d.mstg[H(ip, J, k)];
+                double dvdx = (Q1[H(ip, J, k)] - Q1[H(i, J, k)]) * dxi;
+                double dudy = (Q0[H(ip, J, k)] - Q0[H(ip, J - 1, k)]) * d.dyci[J];
+                const double t21fe = vis * dudy, t21fi = vis * dvdx;
+                vis = d.mstg[H(i, J, k)];
+                dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
+                dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
+                const double t21be = vis * dudy, t21bi = vis * dvdx;
+                const double d21e = (t21fe - t21be) * e1, d21i = (t21fi - t21bi) * e1;
+                vis = d.mstg[2 * hs + H(i, J, kp)];
+                double dvdz = (Q1[H(i, J, kp)] - Q1[H(i, J, k)]) * dzi;
+                double dwdy = (Q2[H(i, J, kp)] - Q2[H(i, J - 1, kp)]) * d.dyci[J];
+                const double t23fe = vis * dwdy, t23fi = vis * dvdz;
+                vis = d.mstg[2 * hs + H(i, J, k)];
+                dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
+                dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
+                const double t23be = vis * dwdy, t23bi = vis * dvdz;
+                const double d23e = (t23fe - t23be) * e3, d23i = (t23fi - t23bi) * e3;
+                acc = d21e + d22e + d23e + acc;
+                imp = d21i + d22i + d23i;
+            } else {
             const double v1 = dxi * cv, v2 = 2.0 * d.dyci[J] * cv, v3 = dzi * cv;
             double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
             const double t22f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dvdy - DU[CI(i, J, k)]);
@@ -232,6 +312,7 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
             const double t23b = d.mstg[2 * hs + H(i, J, k)] * (dvdz + dwdy);
             const double d23 = (t23f - t23b) * v3;
             acc = acc + d21 + d22 + d23;
+            }
             pgm = (d.pr[H(i, J, k)] - d.pr[H(i, J - 1, k)]) * (ta * dt * d.dyci[J]);
         } else {
             const double c1 = dxi * hf * hf, c2 = d.dyfi[J] * hf, c3 = dzi * hf * hf;
@@ -245,6 +326,39 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
             b = (G2[H(i, J, k)] + G2[H(i, J, km)]) * (Q2[H(i, J, k)] + Q2[H(i, J, km)]);
             const double h33 = (f - b) * c3;
             acc = -(h31 + h32 + h33);
+            if constexpr (CN) {
+                const double *V1 = d.m1;
+                const double e1 = dxi * cv, e3 = -2.0 / 3.0 * dzi * cv, e2 = d.dyfi[J] * cv;
+                double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
+                double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
+                double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
+                const double t33fe = V1[H(i, J, k)] * (dudx + dvdy), t33fi = V1[H(i, J, k)] * dwdz;
+                dudx = (Q0[H(ip, J, km)] - Q0[H(i, J, km)]) * dxi;
+                dvdy = (Q1[H(i, J + 1, km)] - Q1[H(i, J, km)]) * d.dyfi[J];
+                dwdz = (Q2[H(i, J, k)] - Q2[H(i, J, km)]) * dzi;
+                const double t33be = V1[H(i, J, km)] * (dudx + dvdy), t33bi = V1[H(i, J, km)] * dwdz;
+                const double d33e = (t33fe - t33be) * e3, d33i = (t33fi - t33bi) * e3 * (-2.0);
+                double vis = d.mstg[hs + H(ip, J, k)];
+                double dwdx = (Q2[H(ip, J, k)] - Q2[H(i, J, k)]) * dxi;
+                double dudz = (Q0[H(ip, J, k)] - Q0[H(ip, J, km)]) * dzi;
+                const double t31fe = vis * dudz, t31fi = vis * dwdx;
+                vis = d.mstg[hs + H(i, J, k)];
+                dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
+                dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
+                const double t31be = vis * dudz, t31bi = vis * dwdx;
+                const double d31e = (t31fe - t31be) * e1, d31i = (t31fi - t31bi) * e1;
+                vis = d.mstg[2 * hs + H(i, J + 1, k)];
+                double dwdy = (Q2[H(i, J + 1, k)] - Q2[H(i, J, k)]) * d.dyci[J + 1];
+                double dvdz = (Q1[H(i, J + 1, k)] - Q1[H(i, J + 1, km)]) * dzi;
+                const double t32fe = vis * dvdz, t32fi = vis * dwdy;
+                vis = d.mstg[2 * hs + H(i, J, k)];
+                dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
+                dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
+                const double t32be = vis * dvdz, t32bi = vis * dwdy;
+                const double d32e = (t32fe - t32be) * e2, d32i = (t32fi - t32bi) * e2;
+                acc = d31e + d32e + d33e + acc;
+                imp = d31i + d32i + d33i;
+            } else {
             const double v1 = dxi * cv, v2 = d.dyfi[J] * cv, v3 = 2.0 * dzi * cv;
             double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
             const double t33f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dwdz - DU[CI(i, J, k)]);
@@ -266,10 +380,13 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
             const double t32b = d.mstg[2 * hs + H(i, J, k)] * (dwdy + dvdz);
             const double d32 = (t32f - t32b) * v2;
             acc = acc + d31 + d32 + d33;
+            }
             pgm = (d.pr[H(i, J, k)] - d.pr[H(i, J, km)]) * (ta * dt * dzi);
         }
         const size_t o = (size_t)C * csz(d) + CI(i, J, k);
-        double r = (tg * acc + tr * d.explt0[o]) * dt;
+        double r;
+        if constexpr (CN) r = (tg * acc + tr * d.explt0[o]) * dt + (ta * dt) * imp;
+        else r = (tg * acc + tr * d.explt0[o]) * dt;
         d.explt0[o] = acc;
         r = r - pgm;
         if (d.gravity == C + 1) r = r + d.dstg[C * hs + H(i, J, k)] * (ta * dt * d.f_a);
@@ -281,6 +398,7 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
 // All three momentum components of a point in one pass, so G, Q, the viscosities and the
 // pressure are read once for the three; the block's partial of sum(RHS_u / DYFI) for the
 // mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265) is written as well.
+template <bool CN>
 __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restrict__ gs, double tg, double tr, double ta,
                                                 double dt)
 {
@@ -288,9 +406,9 @@ __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restric
     IO_THREAD(1)
     double contrib = 0.0;
     if (i < L && J <= M) {
-        contrib = io_mom_point<0>(d, gs, i, J, k, tg, tr, ta, dt);
-        if (J >= 2) io_mom_point<1>(d, gs, i, J, k, tg, tr, ta, dt);
-        io_mom_point<2>(d, gs, i, J, k, tg, tr, ta, dt);
+        contrib = io_mom_point<0, CN>(d, gs, i, J, k, tg, tr, ta, dt);
+        if (J >= 2) io_mom_point<1, CN>(d, gs, i, J, k, tg, tr, ta, dt);
+        io_mom_point<2, CN>(d, gs, i, J, k, tg, tr, ta, dt);
     }
     const double s = block_sum256(contrib, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0)
@@ -334,6 +452,257 @@ __global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__re
     d.g[hs + c0] = J == 1 ? 0.0 : gs[hs + c0] + d.h[cs + ci];
 }
 
+// ---- Crank-Nicolson line solves (MOMFA_io, MOMFA_io.f90:18-406) --------------------------
+// One thread per line, each following the reference's Thomas recurrences in their order
+// (TDMAIJI_CYC.f90:88-157, TDMAIJJ_nonCYC.f90:88-123) with the coefficients built per element
+// as MOMFA1_X_io / MOMFA2_Z_io / MOMFA3_Y_io build them.  The recurrences' intermediate rows
+// go to d.work element-major ([element][line]), so a wave's accesses are contiguous; the three
+// momentum components run side by side (grid.y), each on its own third of d.work.
+
+// Cyclic recurrence of TDMAIJI_CYC on one line of n >= 3 elements; coef(i, a, b, c) gives
+// row i, rhs(i) its right-hand side, out(i, x) takes the solution.  H, G1, G2: element i at
+// [i * es].
+template <class Coef, class Rhs, class Out>
+__device__ __forceinline__ void thomas_cyc(int n, const Coef &coef, const Rhs &rhs, const Out &out,
+                                           double *__restrict__ H, double *__restrict__ G1,
+                                           double *__restrict__ G2, size_t es)
+{
+#pragma clang fp contract(off)
+    double a, b, c;
+    coef(0, a, b, c);
+    double h = c / b, g1 = rhs(0) / b, g2 = -a / b;
+    H[0] = h;
+    G1[0] = g1;
+    G2[0] = g2;
+    for (int i = 1; i <= n - 3; ++i) {
+        coef(i, a, b, c);
+        const double pp = 1.0 / (b - h * a);
+        const double qq1 = g1 * a, qq2 = g2 * a;
+        h = c * pp;
+        g1 = (rhs(i) - qq1) * pp;
+        g2 = -qq2 * pp;
+        H[i * es] = h;
+        G1[i * es] = g1;
+        G2[i * es] = g2;
+    }
+    {
+        const int i = n - 2;
+        coef(i, a, b, c);
+        const double pp = 1.0 / (b - h * a);
+        const double qq1 = g1 * a, qq2 = g2 * a;
+        g1 = (rhs(i) - qq1) * pp;
+        g2 = (-c - qq2) * pp;
+        G1[i * es] = g1;
+        G2[i * es] = g2;
+    }
+    const double ge1 = g1, ge2 = g2;   // rows n-2, final
+    for (int i = n - 3; i >= 0; --i) {
+        const double hi = H[i * es];
+        g1 = -hi * g1 + G1[i * es];
+        g2 = -hi * g2 + G2[i * es];
+        G1[i * es] = g1;
+        G2[i * es] = g2;
+    }
+    const int e = n - 1;
+    coef(e, a, b, c);
+    const double pp = rhs(e) - c * g1 - a * ge1;
+    const double qq1 = b + c * g2 + a * ge2;
+    const double xe = pp / qq1;
+    out(e, xe);
+    out(n - 2, ge1 + ge2 * xe);
+    for (int i = n - 3; i >= 0; --i) out(i, G1[i * es] + G2[i * es] * xe);
+}
+
+// Recurrence of TDMAIJJ_nonCYC with zero boundary values on one line of n >= 2 elements.
+template <class Coef, class Rhs, class Out>
+__device__ __forceinline__ void thomas_line(int n, const Coef &coef, const Rhs &rhs, const Out &out,
+                                            double *__restrict__ H, double *__restrict__ G, size_t es)
+{
+#pragma clang fp contract(off)
+    const double bc = 0.0;
+    double a, b, c;
+    coef(0, a, b, c);
+    double h = c / b, g = (rhs(0) - a * bc) / b;
+    H[0] = h;
+    G[0] = g;
+    for (int j = 1; j <= n - 2; ++j) {
+        coef(j, a, b, c);
+        const double pp = 1.0 / (b - h * a);
+        h = c * pp;
+        g = (rhs(j) - a * g) * pp;
+        H[j * es] = h;
+        G[j * es] = g;
+    }
+    coef(n - 1, a, b, c);
+    const double pp = 1.0 / (b - h * a);
+    double x = (rhs(n - 1) - c * bc - a * g) * pp;
+    out(n - 1, x);
+    for (int j = n - 2; j >= 0; --j) {
+        x = G[j * es] - H[j * es] * x;
+        out(j, x);
+    }
+}
+
+// MOMFA1_X_io (MOMFA_io.f90:77-176): cyclic x-lines of component C over rows J0..M (2..M for v)
+// with FACOE = alpha dt / (2 Re); the mass-flux correction of the streamwise increment
+// (RHS_MOM_EXPLICIT_io.f90:259-265) is applied as the line is read.  In place on d.h[C].
+template <int C>
+__device__ __forceinline__ void io_cn_x_line(const IoDev &d, int l, double dp, double facoe)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M, j0 = C == 1 ? 2 : 1, nj = M - j0 + 1, nl = nj * d.N;
+    if (l >= nl) return;
+    const int J = j0 + l % nj, k = l / nj;
+    const size_t hs = hsz(d), cs = csz(d);
+    const double *DS = d.dstg + C * hs;
+    double *R = d.h + C * cs;
+    double *W = d.work + (size_t)C * 3 * cs + l;
+    const double coe1 = facoe * 4.0 / 3.0 * d.dxqi, coe2 = facoe * d.dxqi;
+    auto coef = [&](int i, double &a, double &b, double &c) {
+        const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+        double amu, apu;
+        if constexpr (C == 0) {
+            amu = -coe1 * d.m1[hx(d, im, J, k)];
+            apu = -coe1 * d.m1[hx(d, i, J, k)];
+        } else if constexpr (C == 1) {
+            amu = -coe2 * d.mstg[hx(d, i, J, k)];
+            apu = -coe2 * d.mstg[hx(d, ip, J, k)];
+        } else {
+            amu = -coe2 * d.mstg[hs + hx(d, i, J, k)];
+            apu = -coe2 * d.mstg[hs + hx(d, ip, J, k)];
+        }
+        a = amu / DS[hx(d, im, J, k)];
+        c = apu / DS[hx(d, ip, J, k)];
+        b = 1.0 - (amu + apu) / DS[hx(d, i, J, k)];
+    };
+    auto rhs = [&](int i) {
+        if constexpr (C == 0) return R[cx(d, i, J, k)] - dp;
+        else return R[cx(d, i, J, k)];
+    };
+    auto out = [&](int i, double x) { R[cx(d, i, J, k)] = x; };
+    thomas_cyc(L, coef, rhs, out, W, W + cs, W + 2 * cs, (size_t)nl);
+}
+
+__global__ __launch_bounds__(128) void k_io_cn_x(IoDev d, const double *__restrict__ dpg_dev, double dpg, double facoe)
+{
+    const int l = blockIdx.x * 128 + threadIdx.x;
+    if (blockIdx.y == 0) io_cn_x_line<0>(d, l, dpg_dev ? dpg_dev[0] : dpg, facoe);
+    else if (blockIdx.y == 1) io_cn_x_line<1>(d, l, 0.0, facoe);
+    else io_cn_x_line<2>(d, l, 0.0, facoe);
+}
+
+// MOMFA2_Z_io (MOMFA_io.f90:179-265): cyclic z-lines (RCCI2 = RNDI2 = 1 on the channel).
+template <int C>
+__device__ __forceinline__ void io_cn_z_line(const IoDev &d, int l, double facoe)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M, N = d.N, j0 = C == 1 ? 2 : 1, nl = (M - j0 + 1) * L;
+    if (l >= nl) return;
+    const int i = l % L, J = j0 + l / L;
+    const size_t hs = hsz(d), cs = csz(d);
+    const double *DS = d.dstg + C * hs;
+    double *R = d.h + C * cs;
+    double *W = d.work + (size_t)C * 3 * cs + l;
+    const double coe1 = facoe * 4.0 / 3.0 * d.dzqi, coe2 = facoe * d.dzqi;
+    auto coef = [&](int k, double &a, double &b, double &c) {
+        const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
+        double amu, apu;
+        if constexpr (C == 0) {
+            amu = -coe2 * d.mstg[hs + hx(d, i, J, k)];
+            apu = -coe2 * d.mstg[hs + hx(d, i, J, kp)];
+        } else if constexpr (C == 1) {
+            amu = -coe2 * d.mstg[2 * hs + hx(d, i, J, k)];
+            apu = -coe2 * d.mstg[2 * hs + hx(d, i, J, kp)];
+        } else {
+            amu = -coe1 * d.m1[hx(d, i, J, km)];
+            apu = -coe1 * d.m1[hx(d, i, J, k)];
+        }
+        a = amu / DS[hx(d, i, J, km)];
+        c = apu / DS[hx(d, i, J, kp)];
+        b = 1.0 - (amu + apu) / DS[hx(d, i, J, k)];
+    };
+    auto rhs = [&](int k) { return R[cx(d, i, J, k)]; };
+    auto out = [&](int k, double x) { R[cx(d, i, J, k)] = x; };
+    thomas_cyc(N, coef, rhs, out, W, W + cs, W + 2 * cs, (size_t)nl);
+}
+
+__global__ __launch_bounds__(128) void k_io_cn_z(IoDev d, double facoe)
+{
+    const int l = blockIdx.x * 128 + threadIdx.x;
+    if (blockIdx.y == 0) io_cn_z_line<0>(d, l, facoe);
+    else if (blockIdx.y == 1) io_cn_z_line<1>(d, l, facoe);
+    else io_cn_z_line<2>(d, l, facoe);
+}
+
+// MOMFA3_Y_io (MOMFA_io.f90:268-406): y-lines over rows 1..M with zero boundary values, then
+// G = increment + G0 for rows N2I..M (MOMFA_io.f90:38-44) with BC_WALL_G_io: every row of the
+// line's new G.  u, w: the wall rows' viscosity is the mean of the wall cell and its ghost in
+// the line's own z-plane (the one-rank form of MOMFA_io.f90:342-352).  v: row 1 is the wall,
+// AMVR = APVR = 0 there, RHS 0.
+template <int C>
+__device__ __forceinline__ void io_cn_y_line(const IoDev &d, const double *__restrict__ gs, int l, double facoe)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M, nl = L * d.N;
+    if (l >= nl) return;
+    const int i = l % L, k = l / L;
+    const size_t hs = hsz(d), cs = csz(d);
+    const double *DS = d.dstg + C * hs, *FM = C == 1 ? d.m1 : d.mstg + C * hs;
+    const double *am = d.amvr + C * (M + 2), *ap = d.apvr + C * (M + 2);
+    const double *R = d.h + C * cs;
+    double *W = d.work + (size_t)C * 3 * cs + l;
+    const double coe1 = facoe * 4.0 / 3.0, coe2 = facoe;
+    auto coef = [&](int jj, double &a, double &b, double &c) {
+        const int J = jj + 1;
+        double amu, apu;
+        if constexpr (C != 1) {
+            if (J == 1) {
+                const double muqt = (d.m1[hx(d, i, 1, k)] + d.m1[hx(d, i, 0, k)]) * 0.5;
+                amu = -coe2 * d.amvr1 * muqt;
+                apu = -coe2 * ap[J] * FM[hx(d, i, J + 1, k)];
+                a = 0.0;
+                c = apu / DS[hx(d, i, J + 1, k)];
+            } else if (J == M) {
+                const double muqt = (d.m1[hx(d, i, M, k)] + d.m1[hx(d, i, M + 1, k)]) * 0.5;
+                amu = -coe2 * am[J] * FM[hx(d, i, J, k)];
+                apu = -coe2 * d.apvrn * muqt;
+                a = amu / DS[hx(d, i, J - 1, k)];
+                c = 0.0;
+            } else {
+                amu = -coe2 * am[J] * FM[hx(d, i, J, k)];
+                apu = -coe2 * ap[J] * FM[hx(d, i, J + 1, k)];
+                a = amu / DS[hx(d, i, J - 1, k)];
+                c = apu / DS[hx(d, i, J + 1, k)];
+            }
+        } else {
+            amu = -coe1 * am[J] * FM[hx(d, i, J - 1, k)];
+            apu = -coe1 * ap[J] * FM[hx(d, i, J, k)];
+            a = J == 1 ? 0.0 : amu / DS[hx(d, i, J - 1, k)];
+            c = apu / DS[hx(d, i, J == 1 ? J + 1 : J, k)];
+        }
+        b = 1.0 - (amu + apu) / DS[hx(d, i, J, k)];
+    };
+    auto rhs = [&](int jj) { return (C == 1 && jj == 0) ? 0.0 : R[cx(d, i, jj + 1, k)]; };
+    double *G = d.g + C * hs;
+    const double *G0 = gs + C * hs;
+    auto out = [&](int jj, double x) {
+        const int J = jj + 1;
+        const size_t o = hx(d, i, J, k);
+        G[o] = (C == 1 && J == 1) ? 0.0 : x + G0[o];
+    };
+    G[hx(d, i, 0, k)] = 0.0;
+    G[hx(d, i, M + 1, k)] = 0.0;
+    thomas_line(M, coef, rhs, out, W, W + cs, (size_t)nl);
+}
+
+__global__ __launch_bounds__(128) void k_io_cn_y(IoDev d, const double *__restrict__ gs, double facoe)
+{
+    const int l = blockIdx.x * 128 + threadIdx.x;
+    if (blockIdx.y == 0) io_cn_y_line<0>(d, gs, l, facoe);
+    else if (blockIdx.y == 1) io_cn_y_line<1>(d, gs, l, facoe);
+    else io_cn_y_line<2>(d, gs, l, facoe);
+}
+
 // DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
 __global__ __launch_bounds__(256) void k_io_divg(IoDev d, double coe0)
 {
@@ -349,17 +718,19 @@ __global__ __launch_bounds__(256) void k_io_divg(IoDev d, double coe0)
     d.rhsp[ci] = dq * coe0;
 }
 
-// PRCALC_io explicit branch (PRESSURE_CORRECTION.f90:102-170) with BC_WALL_PR_io,
+// PRCALC_io (PRESSURE_CORRECTION.f90:102-170; cn: the Crank-Nicolson branch's viscous term
+// -BE * RHSLLPHI_io, BE = alpha dt / (2 Re), the Poisson right-hand side) with BC_WALL_PR_io,
 // MASSFLUX_UPDATE_io (VELO_CORRECTION.f90:103-179) with BC_WALL_G_io, and VELOCITY_CALC_io:
 // rows 0..M+1.
-__global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt)
+__global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt, int cn, double be)
 {
 #pragma clang fp contract(off)
     IO_THREAD(0)
     if (i >= L || J > M + 1) return;
     const size_t hs = hsz(d), c0 = hx(d, i, J, k);
     if (J >= 1 && J <= M) {   // the wall cells' threads also write the ghost-row copies
-        const double p = d.pr[c0] + d.dph[cx(d, i, J, k)];
+        double p = d.pr[c0] + d.dph[cx(d, i, J, k)];
+        if (cn) p = p - be * d.rhsp[cx(d, i, J, k)];
         d.pr[c0] = p;
         if (J == 1) d.pr[hx(d, i, 0, k)] = p;
         if (J == M) d.pr[hx(d, i, M + 1, k)] = p;
@@ -407,6 +778,9 @@ struct chs_io {
     double *buf = nullptr;
     int npart = 0, flow_driven = 1;
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
+    double *own_m1 = nullptr;                       // d.m1 likewise (Crank-Nicolson only)
+    double *cnbuf = nullptr;                        // m1, work, amvr, apvr
+    int cn = 0;                                     // iVisScheme
     bool q_fresh = false;   // Q = G / D_STG of the current G and D_STG (written by chs_io_props)
     double cf_given = 0.0, volm = 0.0;
     double tgam[4], troh[4], talp[4];
@@ -437,8 +811,11 @@ extern "C" {
 int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
 {
     if (!c || !p || !out || c->L < 2 || c->M < 3 || c->N < 2 || (p->flow_driven != 1 && p->flow_driven != 2) ||
-        p->gravity < 0 || p->gravity > 3 || !(p->cvisc > 0))
-        return io_fail(nullptr, CHS_E_ARG, "chs_io_create: bad argument (flow_driven 1 or 2, gravity 0..3, cvisc > 0)");
+        p->gravity < 0 || p->gravity > 3 || !(p->cvisc > 0) || (p->vis_scheme != 0 && p->vis_scheme != 1))
+        return io_fail(nullptr, CHS_E_ARG,
+                       "chs_io_create: bad argument (flow_driven 1 or 2, gravity 0..3, cvisc > 0, vis_scheme 0 or 1)");
+    if (p->vis_scheme == 1 && (c->L < 3 || c->N < 3 || !c->amvr || !c->apvr))
+        return io_fail(nullptr, CHS_E_ARG, "chs_io_create: the Crank-Nicolson scheme needs L, N >= 3 and AMVR/APVR");
     *out = nullptr;
     chs_io *m = new chs_io();
     const int L = c->L, M = c->M, N = c->N;
@@ -486,6 +863,35 @@ int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
         chs_io_destroy(m);
         return io_fail(nullptr, CHS_E_HIP, std::string("chs_io_create: ") + hipGetErrorString(he));
     }
+    if (p->vis_scheme == 1) {
+        // m1 (halo), work (9 cell fields), AMVR / APVR [3][M+2]
+        m->cn = 1;
+        const size_t na = 3 * (size_t)(M + 2), nc = hs + 9 * cs + 2 * na;
+        he = hipMalloc(&m->cnbuf, sizeof(double) * nc);
+        std::vector<double> ha(2 * na, 0.0);
+        for (int r = 0; r < 3; ++r)
+            for (int J = 1; J <= M + 1; ++J) {
+                ha[r * (M + 2) + J] = c->amvr[r * (M + 1) + J - 1];
+                ha[na + r * (M + 2) + J] = c->apvr[r * (M + 1) + J - 1];
+            }
+        if (he == hipSuccess) he = hipMemcpyAsync(m->cnbuf + hs + 9 * cs, ha.data(), sizeof(double) * 2 * na,
+                                                  hipMemcpyHostToDevice, m->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+        if (he != hipSuccess) {
+            chs_io_destroy(m);
+            return io_fail(nullptr, CHS_E_HIP, std::string("chs_io_create: ") + hipGetErrorString(he));
+        }
+        d.m1 = m->own_m1 = m->cnbuf;
+        d.work = m->cnbuf + hs;
+        d.amvr = m->cnbuf + hs + 9 * cs;
+        d.apvr = d.amvr + na;
+        // the wall coefficients of the u, w y-stencils (TDMA_COEF.f90:76-87)
+        double ucaj = 4.0 / (2.0 / c->dyci[1] + 1.0 / c->dyfi[0]);
+        d.amvr1 = ucaj * (2.0 * c->dyfi[0]);
+        ucaj = 4.0 / (2.0 / c->dyci[M - 1] + 1.0 / c->dyfi[M - 1]);
+        d.apvrn = ucaj * (2.0 * c->dyfi[M - 1]);
+    }
+    d.dxqi = c->dxqi; d.dzqi = c->dzqi;
     d.L = L; d.M = M; d.N = N;
     d.dxi = c->dxi; d.dzi = c->dzi; d.cvisc = p->cvisc; d.f_a = p->f_a; d.gravity = p->gravity;
     d.dyfi = geo; d.dyci = geo + (M + 2); d.wff = geo + 2 * (M + 2); d.wfb = geo + 3 * (M + 2);
@@ -500,6 +906,7 @@ void chs_io_destroy(chs_io *m)
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->buf) (void)hipFree(m->buf);
+    if (m->cnbuf) (void)hipFree(m->cnbuf);
     if (m->pois) chs_destroy(m->pois);
     delete m;
 }
@@ -522,6 +929,8 @@ int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr
     if (he == hipSuccess) he = hipMemcpyAsync(d.pr, pr, hb, hipMemcpyHostToDevice, m->stream);
     if (he == hipSuccess) he = hipMemcpyAsync(d.d0, d0, hb, hipMemcpyHostToDevice, m->stream);
     if (he == hipSuccess) he = hipMemcpyAsync(d.m0, m0, hb, hipMemcpyHostToDevice, m->stream);
+    m->d.m1 = m->own_m1;
+    if (he == hipSuccess && m->cn) he = hipMemcpyAsync(m->own_m1, d.m0, hb, hipMemcpyDeviceToDevice, m->stream);
     if (he == hipSuccess)
         he = explt0 ? hipMemcpyAsync(d.explt0, explt0, 3 * cb, hipMemcpyHostToDevice, m->stream)
                     : hipMemsetAsync(d.explt0, 0, 3 * cb, m->stream);
@@ -538,7 +947,20 @@ int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev)
     m->q_fresh = false;
     hipError_t he = hipMemcpyAsync(m->d.d0, d0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
     if (he == hipSuccess) he = hipMemcpyAsync(m->d.m0, m0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
+    m->d.m1 = m->own_m1;
+    if (he == hipSuccess && m->cn) he = hipMemcpyAsync(m->own_m1, m0_dev, hb, hipMemcpyDeviceToDevice, m->stream);
     return io_hip(m, he, "chs_io_set_properties");
+}
+
+int chs_io_upload_viscosity(chs_io *m, const double *visc)
+{
+    if (!m || !visc) return io_fail(m, CHS_E_ARG, "chs_io_upload_viscosity: null argument");
+    if (!m->cn) return io_fail(m, CHS_E_ARG, "chs_io_upload_viscosity: the context uses the explicit scheme");
+    const size_t hb = sizeof(double) * (size_t)m->N * (m->M + 2) * m->L;
+    m->d.m1 = m->own_m1;
+    hipError_t he = hipMemcpyAsync(m->own_m1, visc, hb, hipMemcpyHostToDevice, m->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    return io_hip(m, he, "chs_io_upload_viscosity");
 }
 
 int chs_io_props(chs_io *m, int ns)
@@ -549,21 +971,24 @@ int chs_io_props(chs_io *m, int ns)
     return io_hip(m, hipSuccess, "chs_io_props");
 }
 
-int chs_substep_io(chs_io *m, int ns, double dt)
+int chs_substep_io_momentum(chs_io *m, int ns, double dt)
 {
-    if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io: bad argument");
+    if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io_momentum: bad argument");
     // G0_io <- G_io (SOLVERRK3_MOM_IO.f90:25) by exchanging the two arrays: the terms read the
-    // old G from g0 and k_io_update_g writes every row of the new G.
+    // old G from g0 and k_io_update_g / k_io_cn_y write every row of the new G.
     std::swap(m->d.g, m->d.g0);
     const IoDev &d = m->d;
     const double *gs = d.g0;
     const double tg = m->tgam[ns], tr = m->troh[ns], ta = m->talp[ns];
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
-    hipError_t he = hipSuccess;
     if (!m->q_fresh) hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
     m->q_fresh = false;
-    hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
-    hipLaunchKernelGGL(k_io_mom, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+    if (m->cn) {
+        hipLaunchKernelGGL(k_io_mom<true>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+    } else {
+        hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
+        hipLaunchKernelGGL(k_io_mom<false>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+    }
     const double *dpg_dev = nullptr;
     double dpg = 0.0;
     if (m->flow_driven == 1) {
@@ -572,13 +997,33 @@ int chs_substep_io(chs_io *m, int ns, double dt)
     } else {
         dpg = -0.5 * m->cf_given * (ta * dt);
     }
-    hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
+    if (m->cn) {
+        // MOMFA_io per component (SOLVERRK3_MOM_IO.f90:61-74); the components are independent
+        // here (each reads Q, the properties and its own increment), so they run side by side.
+        const double facoe = 0.5 * ta * dt * d.cvisc;
+        const int L = m->L, M = m->M, N = m->N;
+        hipLaunchKernelGGL(k_io_cn_x, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg, facoe);
+        hipLaunchKernelGGL(k_io_cn_z, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
+        hipLaunchKernelGGL(k_io_cn_y, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
+    } else {
+        hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
+    }
+    return io_hip(m, hipSuccess, "chs_substep_io_momentum");
+}
+
+int chs_substep_io(chs_io *m, int ns, double dt)
+{
+    int rc = chs_substep_io_momentum(m, ns, dt);
+    if (rc) return rc;
+    const IoDev &d = m->d;
+    const double ta = m->talp[ns];
+    const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
     hipLaunchKernelGGL(k_io_divg, gin, blk, 0, m->stream, d, 1.0 / dt / ta);
-    he = hipGetLastError();
+    hipError_t he = hipGetLastError();
     if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
-    int rc = chs_poisson_device(m->pois, d.rhsp, d.dph, 0, nullptr);
+    rc = chs_poisson_device(m->pois, d.rhsp, d.dph, 0, nullptr);
     if (rc) return io_fail(m, rc, std::string("chs_substep_io: Poisson: ") + chs_last_error(m->pois));
-    hipLaunchKernelGGL(k_io_correct, gall, blk, 0, m->stream, d, dt * ta);
+    hipLaunchKernelGGL(k_io_correct, gall, blk, 0, m->stream, d, dt * ta, m->cn, 0.5 * ta * dt * d.cvisc);
     return io_hip(m, hipSuccess, "chs_substep_io");
 }
 
@@ -623,8 +1068,8 @@ int chs_heated_substep(chs_eng *e, chs_io *m, int ns, double dt)
 {
     if (!e || !m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_heated_substep: bad argument");
     hipStream_t es;
-    double *ed0, *em0, *eg;
-    chs_internal_eng_fields(e, &es, &ed0, &em0, &eg);
+    double *ed0, *em0, *em, *eg;
+    chs_internal_eng_fields(e, &es, &ed0, &em0, &em, &eg);
     hipError_t he = hipStreamSynchronize(m->stream);   // G of the last momentum substep is final
     if (he != hipSuccess) return io_hip(m, he, "chs_heated_substep");
     chs_internal_eng_set_g(e, m->d.g);                 // the energy reads the current G in place
@@ -633,15 +1078,17 @@ int chs_heated_substep(chs_eng *e, chs_io *m, int ns, double dt)
         chs_internal_eng_set_g(e, nullptr);
         return io_fail(m, rc, std::string("chs_heated_substep: energy: ") + chs_eng_last_error(e));
     }
-    chs_internal_eng_fields(e, &es, &ed0, &em0, &eg);  // DENSITY0 / Viscousity0 of this stage
+    chs_internal_eng_fields(e, &es, &ed0, &em0, &em, &eg);  // DENSITY0 / Viscousity0 / Viscousity
     m->d.d0 = ed0;                                     // read in place by the momentum kernels
     m->d.m0 = em0;
+    if (m->cn) m->d.m1 = em;
     rc = chs_io_props(m, ns);
     if (!rc) rc = chs_substep_io(m, ns, dt);
     // the launches above hold the aliases by value; later calls see each context's own arrays
     chs_internal_eng_set_g(e, nullptr);
     m->d.d0 = m->own_d0;
     m->d.m0 = m->own_m0;
+    m->d.m1 = m->own_m1;
     return rc;
 }
 

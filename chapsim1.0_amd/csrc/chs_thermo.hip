@@ -690,13 +690,15 @@ int chs_prop_update_from_dh(chs_prop *p, long n, const double *dh, double *out)
 }  // extern "C"
 
 // Device fields of an energy context that the coupled heated-channel substep hands to the
-// momentum substep: its stream, DENSITY0, Viscousity0 and the mass-flux array G.
+// momentum substep: its stream, DENSITY0, Viscousity0, Viscousity (the current viscosity,
+// read by the Crank-Nicolson momentum terms) and the mass-flux array G.
 __attribute__((visibility("hidden"))) void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0,
-                                                                   double **m0, double **g)
+                                                                   double **m0, double **m, double **g)
 {
     *s = e->stream;
     *d0 = e->d0;
     *m0 = e->m0;
+    *m = e->e.m;
     *g = const_cast<double *>(e->e.g1);
 }
 

@@ -45,6 +45,16 @@ MODULE chapsim_hip
         INTEGER(C_INT) :: top_face
     END TYPE chs_eng_params
 
+    ! mirror of chs_io_params (gravity = |iGravity|, vis_scheme = iVisScheme)
+    TYPE, BIND(C) :: chs_io_params
+        REAL(C_DOUBLE) :: cvisc
+        INTEGER(C_INT) :: gravity
+        REAL(C_DOUBLE) :: f_a
+        INTEGER(C_INT) :: flow_driven
+        REAL(C_DOUBLE) :: cf_given
+        INTEGER(C_INT) :: vis_scheme
+    END TYPE chs_io_params
+
     INTERFACE
         FUNCTION chs_version() BIND(C, NAME='chs_version') RESULT(p)
             IMPORT :: C_PTR
@@ -233,6 +243,63 @@ MODULE chapsim_hip
         FUNCTION chs_eng_download(eng, dh, h, t, d, m, k, cp, rhs0) BIND(C, NAME='chs_eng_download') RESULT(rc)
             IMPORT :: C_INT, C_PTR
             TYPE(C_PTR), VALUE :: eng, dh, h, t, d, m, k, cp, rhs0
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        ! Main-domain momentum SOLVERRK3_MOM_io (SOLVERRK3_MOM_IO.f90:11-120) and what SOLVE runs
+        ! before it (SOLVE.f90:122-127); pointers: C_LOC(G_io(1,0,1,1)), C_LOC(PR_io(1,0,1)), ...
+        FUNCTION chs_io_create(c, p, io) BIND(C, NAME='chs_io_create') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, chs_coeffs, chs_io_params
+            TYPE(chs_coeffs), INTENT(IN) :: c
+            TYPE(chs_io_params), INTENT(IN) :: p
+            TYPE(C_PTR), INTENT(OUT) :: io
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        SUBROUTINE chs_io_destroy(io) BIND(C, NAME='chs_io_destroy')
+            IMPORT :: C_PTR
+            TYPE(C_PTR), VALUE :: io
+        END SUBROUTINE
+
+        FUNCTION chs_io_upload(io, g, g0, pr, d0, m0, explt0) BIND(C, NAME='chs_io_upload') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: io, g, g0, pr, d0, m0, explt0
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_io_upload_viscosity(io, visc) BIND(C, NAME='chs_io_upload_viscosity') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: io, visc
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_io_props(io, ns) BIND(C, NAME='chs_io_props') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: io
+            INTEGER(C_INT), VALUE :: ns
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_substep_io(io, ns, dt) BIND(C, NAME='chs_substep_io') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: io
+            INTEGER(C_INT), VALUE :: ns
+            REAL(C_DOUBLE), VALUE :: dt
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_io_download(io, g, q, pr, dph, explt0, drho, rhsp, dstg, mstg) &
+            BIND(C, NAME='chs_io_download') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: io, g, q, pr, dph, explt0, drho, rhsp, dstg, mstg
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_heated_substep(eng, io, ns, dt) BIND(C, NAME='chs_heated_substep') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: eng, io
+            INTEGER(C_INT), VALUE :: ns
+            REAL(C_DOUBLE), VALUE :: dt
             INTEGER(C_INT) :: rc
         END FUNCTION
 

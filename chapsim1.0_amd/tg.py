@@ -90,6 +90,8 @@ SIGNATURES = {
     "chs_io_set_properties": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "chs_io_props": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_substep_io": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "chs_substep_io_momentum": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "chs_io_upload_viscosity": (C.c_int, [C.c_void_p, _dp]),
     "chs_io_download": (C.c_int, [C.c_void_p] + [_dp] * 9),
     "chs_io_sync": (C.c_int, [C.c_void_p]),
     "chs_heated_substep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double]),
@@ -404,21 +406,23 @@ def tdma_lines(kind, A, B, C_, R, bc=None):
 
 class _IoParams(C.Structure):
     _fields_ = [("cvisc", C.c_double), ("gravity", C.c_int), ("f_a", C.c_double),
-                ("flow_driven", C.c_int), ("cf_given", C.c_double)]
+                ("flow_driven", C.c_int), ("cf_given", C.c_double), ("vis_scheme", C.c_int)]
 
 
 class ChannelIO:
     """The main-domain (IO) flow of one periodic heated channel on the GPU, variable density
-    and viscosity, explicit viscous scheme: ``ioprops(ns)`` is what SOLVE runs before the
-    momentum substep (SOLVE.f90:122-124), ``SOLVERRK3_MOM_io(ns, dt)`` the substep
-    (SOLVERRK3_MOM_IO.f90:11-120).  gravity = |iGravity|, f_a = F_A, flow_driven = iFlowDriven.
-    Fields are C-order: G, G0 [3, N, M+2, L], PR, D0, M0 [N, M+2, L], EXPLT0 [3, N, M, L]."""
+    and viscosity: ``ioprops(ns)`` is what SOLVE runs before the momentum substep
+    (SOLVE.f90:122-127), ``SOLVERRK3_MOM_io(ns, dt)`` the substep (SOLVERRK3_MOM_IO.f90:11-120).
+    gravity = |iGravity|, f_a = F_A, flow_driven = iFlowDriven, vis_scheme = iVisScheme (0
+    explicit, 1 Crank-Nicolson).  Fields are C-order: G, G0 [3, N, M+2, L], PR, D0, M0, M1
+    [N, M+2, L], EXPLT0 [3, N, M, L]."""
 
-    def __init__(self, coeffs: dict, REN: float, gravity=0, f_a=0.0, flow_driven=1, cf_given=0.0):
+    def __init__(self, coeffs: dict, REN: float, gravity=0, f_a=0.0, flow_driven=1, cf_given=0.0, vis_scheme=0):
         self.L, self.M, self.N = int(coeffs["L"]), int(coeffs["M"]), int(coeffs["N"])
         self._keep = []
         self._c = _coeff_struct(coeffs, self._keep)
-        self._par = _IoParams(1.0 / float(REN), int(gravity), float(f_a), int(flow_driven), float(cf_given))
+        self._par = _IoParams(1.0 / float(REN), int(gravity), float(f_a), int(flow_driven), float(cf_given),
+                              int(vis_scheme))
         h = C.c_void_p()
         rc = lib().chs_io_create(C.byref(self._c), C.byref(self._par), C.byref(h))
         if rc != 0:
@@ -429,7 +433,8 @@ class ChannelIO:
         if rc != 0:
             raise ChsError(f"{what} failed ({rc}): {lib().chs_io_last_error(self._h).decode()}")
 
-    def upload(self, G, PR, D0, M0, G0=None, EXPLT0=None):
+    def upload(self, G, PR, D0, M0, G0=None, EXPLT0=None, M1=None):
+        """Host state; M1 (Crank-Nicolson only) is the current viscosity, default M0."""
         L, M, N = self.L, self.M, self.N
         hs = (N, M + 2, L)
         arrs = {}
@@ -445,6 +450,11 @@ class ChannelIO:
         P = lambda a: None if a is None else _p(a)
         self._check(lib().chs_io_upload(self._h, P(arrs["G"]), P(arrs["G0"]), P(arrs["PR"]), P(arrs["D0"]),
                                         P(arrs["M0"]), P(arrs["EXPLT0"])), "chs_io_upload")
+        if M1 is not None:
+            m1 = np.ascontiguousarray(M1, dtype=np.float64)
+            if m1.shape != hs:
+                raise ChsError(f"upload: M1 must have shape {hs}")
+            self._check(lib().chs_io_upload_viscosity(self._h, _p(m1)), "chs_io_upload_viscosity")
 
     def set_properties(self, d0_dev: int, m0_dev: int):
         self._check(lib().chs_io_set_properties(self._h, C.c_void_p(d0_dev), C.c_void_p(m0_dev)),
@@ -455,6 +465,10 @@ class ChannelIO:
 
     def SOLVERRK3_MOM_io(self, ns, dt):
         self._check(lib().chs_substep_io(self._h, int(ns), float(dt)), "chs_substep_io")
+
+    def momentum(self, ns, dt):
+        """The momentum part of the substep alone: G before the projection."""
+        self._check(lib().chs_substep_io_momentum(self._h, int(ns), float(dt)), "chs_substep_io_momentum")
 
     def sync(self):
         self._check(lib().chs_io_sync(self._h), "chs_io_sync")

@@ -346,13 +346,16 @@ int chs_interp_initial_tg(int Loo, int Mo, int Noo, double HXoo, double HZoo, co
  *   f_a          F_A (Flow_State.f90:96-126)
  *   flow_driven  iFlowDriven: 1 constant mass flux (the x-increment's volume mean is removed),
  *                2 constant skin friction Cf_Given (RHS_MOM_EXPLICIT_io.f90:229-265)
- *   cf_given     Cf_Given */
+ *   cf_given     Cf_Given
+ *   vis_scheme   iVisScheme: 0 explicit viscous terms (VISCOUS_ALL_EXPLT_*_io), 1 Crank-Nicolson
+ *                (VISCOUS_PAR_EXPLT_*_io + MOMFA_io line solves; SOLVERRK3_MOM_IO.f90:43-78) */
 typedef struct {
     double cvisc;
     int gravity;
     double f_a;
     int flow_driven;
     double cf_given;
+    int vis_scheme;
 } chs_io_params;
 
 typedef struct chs_io chs_io;
@@ -372,21 +375,36 @@ const char *chs_io_last_error(const chs_io *m);
 int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr, const double *d0,
                   const double *m0, const double *explt0);
 
-/* DENSITY0 and Viscousity0 from device halo arrays (e.g. an energy context's). */
+/* DENSITY0 and Viscousity0 from device halo arrays (e.g. an energy context's); with the
+ * Crank-Nicolson scheme the current viscosity is set to Viscousity0 as well (as chs_io_upload
+ * does). */
 int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev);
+
+/* Crank-Nicolson contexts: the current viscosity Viscousity (host halo array [N][M+2][L]),
+ * which VISCOUS_PAR_EXPLT_*_io and MOMFA_io read (the explicit terms read Viscousity0).  In a
+ * heated run SOLVERRK3_ENG_io updates it (SOLVERRK3_ENG_IO.f90:23, THERM_PROP_UPDATE);
+ * chs_heated_substep takes it from the energy context.  Synchronous. */
+int chs_io_upload_viscosity(chs_io *m, const double *visc);
 
 /* What SOLVE runs between the energy and the momentum substep of stage NS in a heated run
  * (SOLVE.f90:122-124): DENSITY_TIME_DERIVATION(NS) (DIVG.f90:249-317), DENSITY_Staggered
  * (MASSFLUX_CALC_IO.f90:262-323), MU_Staggered (:175-259).  Enqueued; returns at once. */
 int chs_io_props(chs_io *m, int ns);
 
-/* SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:11-120) with the explicit viscous scheme
- * (iVisScheme = 0, every main-domain template; iWeightedPre = 0): VELOCITY_CALC_io,
- * CONVECTION_X/Y/Z_io, DIVG_U_io, VISCOUS_ALL_EXPLT_X/Y/Z_io + RHS_MOM_EXPLICIT_io,
- * MASSFLUX_CALC_io, DIVG_io, the Poisson solve, PRCALC_io, MASSFLUX_UPDATE_io, VELOCITY_CALC_io,
- * with the walls (BC_WALL_G/Q/PR/DPH_io).  FMA contraction off, reference operation order:
- * matches the reference to 1e-12 (G, Q) / 1e-10 (pressure).  Enqueued; returns at once. */
+/* SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:11-120), iWeightedPre = 0: VELOCITY_CALC_io,
+ * CONVECTION_X/Y/Z_io; explicit scheme: DIVG_U_io, VISCOUS_ALL_EXPLT_X/Y/Z_io +
+ * RHS_MOM_EXPLICIT_io, MASSFLUX_CALC_io; Crank-Nicolson: per component
+ * VISCOUS_PAR_EXPLT_X/Y/Z_io + RHS_MOM_EXPLICIT_io + MOMFA_io (MOMFA_io.f90:18-406; the y-solve
+ * in its one-rank form, wall viscosities from the line's own z-plane); then DIVG_io, the
+ * Poisson solve, PRCALC_io, MASSFLUX_UPDATE_io, VELOCITY_CALC_io, with the walls
+ * (BC_WALL_G/Q/PR/DPH_io).  FMA contraction off, reference operation order: matches the
+ * reference to 1e-12 (G, Q) / 1e-10 (pressure).  Enqueued; returns at once. */
 int chs_substep_io(chs_io *m, int ns, double dt);
+
+/* The momentum part of chs_substep_io alone (SOLVERRK3_MOM_IO.f90:25-80): leaves the new G
+ * before the projection (DIVG_io onwards not run), for hosts that split the substep and for
+ * checking the momentum solves.  Enqueued. */
+int chs_substep_io_momentum(chs_io *m, int ns, double dt);
 
 /* Copy fields back (NULL skips one): g, q, dstg, mstg [3][N][M+2][L]; pr, dph [N][M+2][L]
  * (dph's ghost rows copy the wall cells, BC_WALL_DPH_io); explt0 [3][N][M][L]; drho, rhsp
@@ -396,8 +414,9 @@ int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, do
 int chs_io_sync(chs_io *m);
 
 /* One stage of the heated channel as SOLVE runs it (SOLVE.f90:119-133): the energy substep
- * with the momentum context's current G, then its DENSITY0 / Viscousity0 into the momentum
- * context, the staggered properties and SOLVERRK3_MOM_io(NS). */
+ * with the momentum context's current G, then its DENSITY0 / Viscousity0 (and, for the
+ * Crank-Nicolson scheme, Viscousity) into the momentum context, the staggered properties
+ * (DRHOI_STG of DENSITY_IMPLICIT_add is identically zero) and SOLVERRK3_MOM_io(NS). */
 int chs_heated_substep(chs_eng *e, chs_io *m, int ns, double dt);
 
 #ifdef __cplusplus

@@ -63,7 +63,8 @@ def lib():
                            ("or_io_convection", [C.c_void_p, C.c_void_p]),
                            ("or_io_divu", [C.c_void_p, C.c_void_p]),
                            ("or_io_viscous", [C.c_void_p, C.c_void_p, C.c_void_p]),
-                           ("or_io_substep", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double])):
+                           ("or_io_substep", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double]),
+                           ("or_io_momentum_cn", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double])):
             fn = getattr(_lib, name)
             fn.argtypes = args
             fn.restype = None
@@ -305,12 +306,13 @@ class OracleEng:
 
 class _Io(C.Structure):
     _fields_ = [(k, _dp) for k in ("g", "g0", "q", "pr", "dph", "d0", "m0", "dstg", "mstg", "explt0",
-                                   "drho", "divu")] + [("h", _dp * 3), ("rhsp", _dp)]
+                                   "drho", "divu")] + [("h", _dp * 3), ("rhsp", _dp), ("m1", _dp), ("rim", _dp),
+                                                          ("rkeep", _dp)]
 
 
 class _IoPar(C.Structure):
     _fields_ = [("cvisc", C.c_double), ("gravity", C.c_int), ("f_a", C.c_double),
-                ("flow_driven", C.c_int), ("cf_given", C.c_double)]
+                ("flow_driven", C.c_int), ("cf_given", C.c_double), ("vis_scheme", C.c_int)]
 
 
 def io_initial_field(mesh, seed, rho_amp=0.3, mu_amp=0.3):
@@ -344,27 +346,41 @@ def io_initial_field(mesh, seed, rho_amp=0.3, mu_amp=0.3):
 class OracleIO:
     """SOLVERRK3_MOM_io restated (or_io_substep) with the staggered properties of SOLVE
     (or_io_props).  Fields are C-order; ``f`` holds G, G0, Q, DSTG, MSTG [3, N, M+2, L], PR,
-    DPH, D0, M0 [N, M+2, L], EXPLT0, H [3, N, M, L], DRHO, DIVU, RHSP [N, M, L]."""
+    DPH, D0, M0, M1 [N, M+2, L], EXPLT0, H [3, N, M, L], DRHO, DIVU, RHSP, RIM [N, M, L].
+    vis_scheme 0: explicit viscous terms (reads Viscousity0 = M0); 1: Crank-Nicolson (reads the
+    current viscosity M1 and MU_STG)."""
 
-    def __init__(self, mesh: _setup.Mesh, REN: float, gravity=0, f_a=0.0, flow_driven=1, cf_given=0.0):
+    def __init__(self, mesh: _setup.Mesh, REN: float, gravity=0, f_a=0.0, flow_driven=1, cf_given=0.0,
+                 vis_scheme=0):
         L, M, N = mesh.L, mesh.M, mesh.N
         self.mesh = mesh
         self._om = OracleTG(mesh, REN)._m
         hs = (N, M + 2, L)
         self.f = {k: np.zeros((3,) + hs) for k in ("G", "G0", "Q", "DSTG", "MSTG")}
-        self.f.update({k: np.zeros(hs) for k in ("PR", "DPH", "D0", "M0")})
-        self.f.update({k: np.zeros((N, M, L)) for k in ("DRHO", "DIVU", "RHSP")})
+        self.f.update({k: np.zeros(hs) for k in ("PR", "DPH", "D0", "M0", "M1")})
+        self.f.update({k: np.zeros((N, M, L)) for k in ("DRHO", "DIVU", "RHSP", "RIM")})
+        self.f["RKEEP"] = np.zeros((3, N, M, L))
         self.f["EXPLT0"] = np.zeros((3, N, M, L))
         self.f["H"] = np.zeros((3, N, M, L))
         f = self.f
         self._s = _Io(_p(f["G"]), _p(f["G0"]), _p(f["Q"]), _p(f["PR"]), _p(f["DPH"]), _p(f["D0"]),
                       _p(f["M0"]), _p(f["DSTG"]), _p(f["MSTG"]), _p(f["EXPLT0"]), _p(f["DRHO"]),
-                      _p(f["DIVU"]), (_dp * 3)(*[_p(f["H"][c]) for c in range(3)]), _p(f["RHSP"]))
-        self._par = _IoPar(1.0 / REN, int(gravity), float(f_a), int(flow_driven), float(cf_given))
+                      _p(f["DIVU"]), (_dp * 3)(*[_p(f["H"][c]) for c in range(3)]), _p(f["RHSP"]),
+                      _p(f["M1"]), _p(f["RIM"]), _p(f["RKEEP"]))
+        self._par = _IoPar(1.0 / REN, int(gravity), float(f_a), int(flow_driven), float(cf_given),
+                           int(vis_scheme))
 
     def set_state(self, st: dict):
+        """G, G0, PR, D0, M0, EXPLT0 and optionally M1 (the current viscosity; default M0)."""
         for k in ("G", "G0", "PR", "D0", "M0", "EXPLT0"):
             self.f[k][...] = st[k]
+        self.f["M1"][...] = st.get("M1", st["M0"])
+
+    def momentum_cn(self, ns, dt):
+        """The Crank-Nicolson momentum part of the substep, up to the Poisson step: H holds
+        convection + explicit viscous part (= the new EXPLT0), RKEEP each component's RHS_io as
+        MOMFA_io receives it, G the momentum after the three MOMFA_io solves with its walls."""
+        lib().or_io_momentum_cn(C.byref(self._om), C.byref(self._par), C.byref(self._s), int(ns), float(dt))
 
     def props(self, ns):
         lib().or_io_props(C.byref(self._om), C.byref(self._s), int(ns))

@@ -40,11 +40,17 @@
 !   loadio FILE             (main domain, iThermoDynamics = 1) G_io, G0_io, PR_io, DENSITY0,
 !                           Viscousity0 and EXPLT0_io from FILE, see H_LOADIO
 !   ioprops NS              DENSITY_TIME_DERIVATION(NS), DENSITY_Staggered, MU_Staggered
-!                           (SOLVE.f90:122-124)
+!                           and, with iVisScheme = 1, DENSITY_IMPLICIT_add (SOLVE.f90:122-127)
+!   loadvisc FILE           Viscousity (the current viscosity) from a global L x (M+2) x N file
+!   iocn NS DT FILE         DT = given; the momentum part of SOLVERRK3_MOM_io(NS) with the
+!                           Crank-Nicolson viscous scheme (SOLVERRK3_MOM_IO.f90:25-41, :61-74):
+!                           each rank writes FILE.<rank>: G_io and DRHOI_STG rows 0..N2DO+1
+!                           after the three MOMFA_io calls, then RHS_io of each component as
+!                           MOMFA_io receives it and EXPLT0_io, rows 1..N2DO
 !   ioterms FILE            VELOCITY_CALC_io, CONVECTION_X/Y/Z_io, DIVG_U_io and
 !                           VISCOUS_ALL_EXPLT_X/Y/Z_io; each rank writes FILE.<rank>: QTMP_io,
 !                           DPH_io, RHSLLPHI_io, DivU_io rows 1..N2DO
-!   iosubstep NS DT         DT = given; the body of SOLVERRK3_MOM_io(NS) (explicit viscous
+!   iosubstep NS DT         DT = given; the body of SOLVERRK3_MOM_io(NS) (either viscous
 !                           scheme) with the FFT99 Poisson solve, see H_IOSUBSTEP
 !   dumpio FILE             each rank writes FILE.<rank>: G_io, Q_io, PR_io, D_STG, MU_STG,
 !                           DPH_io rows 0..N2DO+1, then EXPLT0_io, DrhoDtP, RHSLLPHI_io rows
@@ -238,6 +244,14 @@ PROGRAM ref_harness
             CALL DENSITY_TIME_DERIVATION(ns)
             CALL DENSITY_Staggered
             CALL MU_Staggered
+            IF (iVisScheme == VisImplicit) CALL DENSITY_IMPLICIT_add
+        CASE ('loadvisc')
+            READ(line, *) cmd, fname
+            CALL H_LOADVISC(TRIM(fname))
+        CASE ('iocn')
+            READ(line, *) cmd, ns, dtin, fname
+            DT = dtin
+            CALL H_IOCN(ns, TRIM(fname))
         CASE ('ioterms')
             READ(line, *) cmd, fname
             CALL VELOCITY_CALC_io
@@ -488,8 +502,8 @@ CONTAINS
         DEALLOCATE(A)
     END SUBROUTINE
 
-    ! SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:24-117) for the periodic main domain with the
-    ! explicit viscous scheme, calling the reference's routines in the same order; the Poisson
+    ! SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:24-117) for the periodic main domain, either
+    ! viscous scheme, calling the reference's routines in the same order; the Poisson
     ! solve is FFT99_POIS3D_periodicxz(IIO) (:92) instead of the compiled default
     ! FISHPACK_POIS3D_SIMPLE(RHSLLPHI_io, DPH_io) (:93), whose output dummy has no y-halo rows
     ! while DPH_io has them, so the default stores its solution shifted in memory.
@@ -501,17 +515,32 @@ CONTAINS
         CALL CONVECTION_X_io
         CALL CONVECTION_Y_io
         CALL CONVECTION_Z_io
-        CALL DIVG_U_io
-        IDR = 1
-        CALL VISCOUS_ALL_EXPLT_X_io
-        CALL RHS_MOM_EXPLICIT_io(NS, IDR)
-        IDR = 2
-        CALL VISCOUS_ALL_EXPLT_Y_io
-        CALL RHS_MOM_EXPLICIT_io(NS, IDR)
-        IDR = 3
-        CALL VISCOUS_ALL_EXPLT_Z_io
-        CALL RHS_MOM_EXPLICIT_io(NS, IDR)
-        CALL MASSFLUX_CALC_io
+        IF (iVisScheme == VisImplicit) THEN
+            IDR = 1
+            CALL VISCOUS_PAR_EXPLT_X_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            CALL MOMFA_io(NS, IDR)
+            IDR = 2
+            CALL VISCOUS_PAR_EXPLT_Y_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            CALL MOMFA_io(NS, IDR)
+            IDR = 3
+            CALL VISCOUS_PAR_EXPLT_Z_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            CALL MOMFA_io(NS, IDR)
+        ELSE
+            CALL DIVG_U_io
+            IDR = 1
+            CALL VISCOUS_ALL_EXPLT_X_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            IDR = 2
+            CALL VISCOUS_ALL_EXPLT_Y_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            IDR = 3
+            CALL VISCOUS_ALL_EXPLT_Z_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            CALL MASSFLUX_CALC_io
+        END IF
         CALL INTFC_VARS3(1, NCL1_io, NCL1S, NCL1E, G_io)
         CALL BC_WALL_G_io
         CALL DIVG_io(NS)
@@ -521,6 +550,53 @@ CONTAINS
         CALL PRCALC_io(NS)
         CALL MASSFLUX_UPDATE_io(NS)
         CALL VELOCITY_CALC_io
+    END SUBROUTINE
+
+    ! Viscousity (current viscosity; the Crank-Nicolson terms read it, the explicit ones read
+    ! Viscousity0) from one global L x (M+2) x N file; every rank keeps its y-slab.
+    SUBROUTINE H_LOADVISC(fn)
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: A(:, :, :)
+        INTEGER(4) :: J, JG
+        ALLOCATE(A(NCL1_io, 0:NCL2 + 1, NCL3))
+        OPEN(79, FILE = fn, ACCESS = 'STREAM', FORM = 'UNFORMATTED', STATUS = 'old', &
+             ACTION = 'read')
+        READ(79) A
+        CLOSE(79)
+        DO J = 0, N2DO(MYID) + 1
+            JG = JDSWT(MYID) - 1 + J
+            Viscousity(1:NCL1_io, J, :) = A(:, JG, :)
+        END DO
+        DEALLOCATE(A)
+    END SUBROUTINE
+
+    ! The momentum part of SOLVERRK3_MOM_io(NS) with iVisScheme = 1 (SOLVERRK3_MOM_IO.f90:25,
+    ! :27, :38-40, :61-74), stopping before the Poisson step; RHS_io is kept as each MOMFA_io
+    ! receives it.
+    SUBROUTINE H_IOCN(NS, fn)
+        INTEGER(4), INTENT(IN) :: NS
+        CHARACTER(*), INTENT(IN) :: fn
+        REAL(WP), ALLOCATABLE :: R(:, :, :, :)
+        INTEGER(4) :: IDR
+        ALLOCATE(R(NCL1_io, N2DO(MYID), NCL3, 3))
+        G0_io(:, :, :, :) = G_io(:, :, :, :)
+        CALL VELOCITY_CALC_io
+        CALL CONVECTION_X_io
+        CALL CONVECTION_Y_io
+        CALL CONVECTION_Z_io
+        DO IDR = 1, 3
+            IF (IDR == 1) CALL VISCOUS_PAR_EXPLT_X_io
+            IF (IDR == 2) CALL VISCOUS_PAR_EXPLT_Y_io
+            IF (IDR == 3) CALL VISCOUS_PAR_EXPLT_Z_io
+            CALL RHS_MOM_EXPLICIT_io(NS, IDR)
+            R(:, :, :, IDR) = RHS_io(:, 1:N2DO(MYID), :)
+            CALL MOMFA_io(NS, IDR)
+        END DO
+        CALL H_OPENRANK(fn, 80)
+        WRITE(80) G_io(1:NCL1_io, 0:N2DO(MYID) + 1, :, :), DRHOI_STG(1:NCL1_io, 0:N2DO(MYID) + 1, :, :)
+        WRITE(80) R, EXPLT0_io(:, 1:N2DO(MYID), :, :)
+        CLOSE(80)
+        DEALLOCATE(R)
     END SUBROUTINE
 
     SUBROUTINE H_DUMP3H(fn, A)

@@ -305,6 +305,13 @@ def read_io(h: "Harness", name):
     return dict(G=g, Q=q, PR=pr[0], DSTG=ds, MSTG=ms, DPH=dph[0], EXPLT0=ex, DRHO=dr[0], RHSP=rp[0])
 
 
+def read_iocn(h: "Harness", name):
+    """``iocn`` dump: G, DRHOI [3, N, M+2, L] after the three MOMFA_io calls; RHS [3, N, M, L]
+    as each MOMFA_io receives it; EXPLT0 [3, N, M, L]."""
+    g, dr, r, ex = _split_dump(h.path(name), h.nranks, [(3, True), (3, True), (3, False), (3, False)])
+    return dict(G=g, DRHOI=dr, RHS=r, EXPLT0=ex)
+
+
 def read_ioterms(h: "Harness", name):
     """``ioterms`` dump: H [3, N, M, L] (x, y, z momentum terms) and DIVU [N, M, L]."""
     a = h.read_cells(name, nvars=4)

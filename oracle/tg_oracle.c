@@ -761,6 +761,9 @@ typedef struct {
     double *divu;            /* [N][M][L] DivU_io                                          */
     double *h[3];            /* [N][M][L] QTMP_io, DPH_io, RHSLLPHI_io as momentum terms   */
     double *rhsp;            /* [N][M][L] RHSLLPHI_io as the Poisson right-hand side       */
+    double *m1;              /* [N][M+2][L] Viscousity (current; Crank-Nicolson terms)      */
+    double *rim;             /* [N][M][L] RHS_io: implicit viscous part, then the increment */
+    double *rkeep;           /* NULL or [3][N][M][L]: RHS_io of each component as MOMFA_io gets it */
 } or_io;
 
 typedef struct {
@@ -769,6 +772,7 @@ typedef struct {
     double f_a;              /* F_A (Flow_State.f90:104-123)                               */
     int flow_driven;         /* iFlowDriven: 1 constant mass flux, 2 constant Cf           */
     double cf_given;         /* Cf_Given (iFlowDriven = 2)                                 */
+    int vis_scheme;          /* iVisScheme: 0 explicit, 1 Crank-Nicolson                    */
 } or_io_par;
 
 #define V3(a, c, i, J, k) (a)[(((size_t)(c) * NN + (k)) * (MM + 2) + (J)) * LL + (i)]
@@ -1046,12 +1050,17 @@ void or_io_viscous(const or_mesh *m, const or_io_par *p, or_io *s)
 void or_io_rhs(const or_mesh *m, const or_io_par *p, or_io *s, int ns, int idr, double dt)
 {
     const int c = idr - 1, j0 = idr == 2 ? 2 : 1;
-    double *H = s->h[c];
+    const int cn = p->vis_scheme == 1;
+    const double coe2i = m->talp[ns] * dt;
+    double *H = cn ? s->rim : s->h[c];
     for (int k = 0; k < NN; ++k)
         for (int J = j0; J <= MM; ++J)
             for (int i = 0; i < LL; ++i) {
-                const double rc = CX(H, i, J, k), rl = C3(s->explt0, c, i, J, k);
-                CX(H, i, J, k) = (m->tgam[ns] * rc + m->troh[ns] * rl) * dt;
+                const double rc = CX(s->h[c], i, J, k), rl = C3(s->explt0, c, i, J, k);
+                if (cn)   /* RHS_MOM_EXPLICIT_io.f90:87-99; the DIVU_io term is zero (or_io_viscous_cn) */
+                    CX(H, i, J, k) = (m->tgam[ns] * rc + m->troh[ns] * rl) * dt + coe2i * CX(s->rim, i, J, k);
+                else
+                    CX(H, i, J, k) = (m->tgam[ns] * rc + m->troh[ns] * rl) * dt;
                 C3(s->explt0, c, i, J, k) = rc;
             }
     for (int k = 0; k < NN; ++k)
@@ -1122,13 +1131,20 @@ void or_io_divg(const or_mesh *m, or_io *s, int ns, double dt)
     }
 }
 
-/* PRCALC_io explicit branch (PRESSURE_CORRECTION.f90:102-170) and MASSFLUX_UPDATE_io
+/* PRCALC_io (PRESSURE_CORRECTION.f90:102-170; with iVisScheme = 1 the viscous term of the
+ * pressure, RHSLLPHI_io being the Poisson right-hand side) and MASSFLUX_UPDATE_io
  * (VELO_CORRECTION.f90:103-179): p += phi; G -= alpha dt grad(phi); walls. */
-void or_io_correct(const or_mesh *m, or_io *s, int ns, double dt)
+void or_io_correct(const or_mesh *m, const or_io_par *p, or_io *s, int ns, double dt)
 {
+    const double be = 0.5 * m->talp[ns] * dt * p->cvisc;   /* PRESSURE_CORRECTION.f90:115-122 */
     for (int k = 0; k < NN; ++k)
         for (int J = 1; J <= MM; ++J)
-            for (int i = 0; i < LL; ++i) HX(s->pr, i, J, k) = HX(s->pr, i, J, k) + HX(s->dph, i, J, k);
+            for (int i = 0; i < LL; ++i) {
+                if (p->vis_scheme == 1)
+                    HX(s->pr, i, J, k) = HX(s->pr, i, J, k) + HX(s->dph, i, J, k) - be * CX(s->rhsp, i, J, k);
+                else
+                    HX(s->pr, i, J, k) = HX(s->pr, i, J, k) + HX(s->dph, i, J, k);
+            }
     or_bc_copy(m, s->pr);
     double coe1 = dt * m->talp[ns] * m->dxi;
     for (int k = 0; k < NN; ++k)
@@ -1149,6 +1165,260 @@ void or_io_correct(const or_mesh *m, or_io *s, int ns, double dt)
     or_io_bc_zero(m, s->g);
 }
 
+/* VISCOUS_PAR_EXPLT_X_io (VISCOUS_X_EXPLICIT_io.f90:131-293), _Y_io (VISCOUS_Y_EXPLICIT_io.f90:
+ * 278-434, channel branch), _Z_io (VISCOUS_Z_EXPLICIT_io.f90:300-462, channel branch) for
+ * component idr: the viscous stress split after Nicoud (2000).  The part left explicit is
+ * added in front of the convection term (QTMP_io / DPH_io / RHSLLPHI_io = EX + conv); the part
+ * the line solves treat implicitly goes to RHS_io (rim).  Both read the current viscosity
+ * (Viscousity) and MU_STG.  The density-correction parts (the *_AD terms into DIVU_io) are
+ * G * DRHOI_STG products, and DRHOI_STG = 1/D_STG - 1/DEN_STG where DENSITY_IMPLICIT_add
+ * (MASSFLUX_CALC_IO.f90:332-416) evaluates DEN_STG by D_STG's own expression of the same
+ * DENSITY0 (:276-309), right after DENSITY_Staggered (SOLVE.f90:123-126): it is exactly
+ * zero, so those parts are +-0 and add nothing. */
+void or_io_viscous_cn(const or_mesh *m, const or_io_par *p, or_io *s, int idr)
+{
+    const double cv = p->cvisc, dxi = m->dxi, dzi = m->dzi;
+    const double *Q = s->q, *MU = s->mstg, *V = s->m1;
+    const int c = idr - 1;
+    memset(s->rim, 0, sizeof(double) * (size_t)NN * MM * LL);   /* RHS_io = 0 */
+    for (int J = (idr == 2 ? 2 : 1); J <= MM; ++J) {
+        for (int k = 0; k < NN; ++k) {
+            const int kp = kp_(m, k), km = km_(m, k);
+            for (int i = 0; i < LL; ++i) {
+                const int ip = ip_(m, i), im = im_(m, i);
+                double ex, imp;
+                if (idr == 1) {
+                    const double coe1 = -2.0 / 3.0 * dxi * cv, coe2 = m->dyfi[J - 1] * cv, coe3 = dzi * cv;
+                    double dudx = (V3(Q, 0, ip, J, k) - V3(Q, 0, i, J, k)) * dxi;
+                    double dvdy = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, i, J, k)) * m->dyfi[J - 1];
+                    double dwdz = (V3(Q, 2, i, J, kp) - V3(Q, 2, i, J, k)) * dzi;
+                    const double t11fe = HX(V, i, J, k) * (dvdy + dwdz), t11fi = HX(V, i, J, k) * dudx;
+                    dudx = (V3(Q, 0, i, J, k) - V3(Q, 0, im, J, k)) * dxi;
+                    dvdy = (V3(Q, 1, im, J + 1, k) - V3(Q, 1, im, J, k)) * m->dyfi[J - 1];
+                    dwdz = (V3(Q, 2, im, J, kp) - V3(Q, 2, im, J, k)) * dzi;
+                    const double t11be = HX(V, im, J, k) * (dvdy + dwdz), t11bi = HX(V, im, J, k) * dudx;
+                    const double d11e = (t11fe - t11be) * coe1, d11i = (t11fi - t11bi) * coe1 * (-2.0);
+                    double vis = V3(MU, 0, i, J + 1, k);
+                    double dvdx = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, im, J + 1, k)) * dxi;
+                    double dudy = (V3(Q, 0, i, J + 1, k) - V3(Q, 0, i, J, k)) * m->dyci[J + 1];
+                    const double t12fe = vis * dvdx, t12fi = vis * dudy;
+                    vis = V3(MU, 0, i, J, k);
+                    dudy = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J - 1, k)) * m->dyci[J];
+                    dvdx = (V3(Q, 1, i, J, k) - V3(Q, 1, im, J, k)) * dxi;
+                    const double t12be = vis * dvdx, t12bi = vis * dudy;
+                    const double d12e = (t12fe - t12be) * coe2, d12i = (t12fi - t12bi) * coe2;
+                    vis = V3(MU, 1, i, J, kp);
+                    double dudz = (V3(Q, 0, i, J, kp) - V3(Q, 0, i, J, k)) * dzi;
+                    double dwdx = (V3(Q, 2, i, J, kp) - V3(Q, 2, im, J, kp)) * dxi;
+                    const double t13fe = vis * dwdx, t13fi = vis * dudz;
+                    vis = V3(MU, 1, i, J, k);
+                    dudz = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J, km)) * dzi;
+                    dwdx = (V3(Q, 2, i, J, k) - V3(Q, 2, im, J, k)) * dxi;
+                    const double t13be = vis * dwdx, t13bi = vis * dudz;
+                    const double d13e = (t13fe - t13be) * coe3, d13i = (t13fi - t13bi) * coe3;
+                    ex = d11e + d12e + d13e;
+                    imp = d11i + d12i + d13i;
+                } else if (idr == 2) {
+                    const double coe1 = dxi * cv, coe3 = dzi * cv, coe2 = -2.0 / 3.0 * m->dyci[J] * cv;
+                    double dudx = (V3(Q, 0, ip, J, k) - V3(Q, 0, i, J, k)) * dxi;
+                    double dvdy = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, i, J, k)) * m->dyfi[J - 1];
+                    double dwdz = (V3(Q, 2, i, J, kp) - V3(Q, 2, i, J, k)) * dzi;
+                    const double t22fe = HX(V, i, J, k) * (dudx + dwdz), t22fi = HX(V, i, J, k) * dvdy;
+                    dudx = (V3(Q, 0, ip, J - 1, k) - V3(Q, 0, i, J - 1, k)) * dxi;
+                    dvdy = (V3(Q, 1, i, J, k) - V3(Q, 1, i, J - 1, k)) * m->dyfi[J - 2];
+                    dwdz = (V3(Q, 2, i, J - 1, kp) - V3(Q, 2, i, J - 1, k)) * dzi;
+                    const double t22be = HX(V, i, J - 1, k) * (dudx + dwdz), t22bi = HX(V, i, J - 1, k) * dvdy;
+                    const double d22e = (t22fe - t22be) * coe2, d22i = (t22fi - t22bi) * coe2 * (-2.0);
+                    double vis = V3(MU, 0, ip, J, k);
+                    double dvdx = (V3(Q, 1, ip, J, k) - V3(Q, 1, i, J, k)) * dxi;
+                    double dudy = (V3(Q, 0, ip, J, k) - V3(Q, 0, ip, J - 1, k)) * m->dyci[J];
+                    const double t21fe = vis * dudy, t21fi = vis * dvdx;
+                    vis = V3(MU, 0, i, J, k);
+                    dvdx = (V3(Q, 1, i, J, k) - V3(Q, 1, im, J, k)) * dxi;
+                    dudy = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J - 1, k)) * m->dyci[J];
+                    const double t21be = vis * dudy, t21bi = vis * dvdx;
+                    const double d21e = (t21fe - t21be) * coe1, d21i = (t21fi - t21bi) * coe1;
+                    vis = V3(MU, 2, i, J, kp);
+                    double dvdz = (V3(Q, 1, i, J, kp) - V3(Q, 1, i, J, k)) * dzi;
+                    double dwdy = (V3(Q, 2, i, J, kp) - V3(Q, 2, i, J - 1, kp)) * m->dyci[J];
+                    const double t23fe = vis * dwdy, t23fi = vis * dvdz;
+                    vis = V3(MU, 2, i, J, k);
+                    dvdz = (V3(Q, 1, i, J, k) - V3(Q, 1, i, J, km)) * dzi;
+                    dwdy = (V3(Q, 2, i, J, k) - V3(Q, 2, i, J - 1, k)) * m->dyci[J];
+                    const double t23be = vis * dwdy, t23bi = vis * dvdz;
+                    const double d23e = (t23fe - t23be) * coe3, d23i = (t23fi - t23bi) * coe3;
+                    ex = d21e + d22e + d23e;
+                    imp = d21i + d22i + d23i;
+                } else {
+                    const double coe1 = dxi * cv, coe3 = -2.0 / 3.0 * dzi * cv, coe2 = m->dyfi[J - 1] * cv;
+                    double dudx = (V3(Q, 0, ip, J, k) - V3(Q, 0, i, J, k)) * dxi;
+                    double dvdy = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, i, J, k)) * m->dyfi[J - 1];
+                    double dwdz = (V3(Q, 2, i, J, kp) - V3(Q, 2, i, J, k)) * dzi;
+                    const double t33fe = HX(V, i, J, k) * (dudx + dvdy), t33fi = HX(V, i, J, k) * dwdz;
+                    dudx = (V3(Q, 0, ip, J, km) - V3(Q, 0, i, J, km)) * dxi;
+                    dvdy = (V3(Q, 1, i, J + 1, km) - V3(Q, 1, i, J, km)) * m->dyfi[J - 1];
+                    dwdz = (V3(Q, 2, i, J, k) - V3(Q, 2, i, J, km)) * dzi;
+                    const double t33be = HX(V, i, J, km) * (dudx + dvdy), t33bi = HX(V, i, J, km) * dwdz;
+                    const double d33e = (t33fe - t33be) * coe3, d33i = (t33fi - t33bi) * coe3 * (-2.0);
+                    double vis = V3(MU, 1, ip, J, k);
+                    double dwdx = (V3(Q, 2, ip, J, k) - V3(Q, 2, i, J, k)) * dxi;
+                    double dudz = (V3(Q, 0, ip, J, k) - V3(Q, 0, ip, J, km)) * dzi;
+                    const double t31fe = vis * dudz, t31fi = vis * dwdx;
+                    vis = V3(MU, 1, i, J, k);
+                    dwdx = (V3(Q, 2, i, J, k) - V3(Q, 2, im, J, k)) * dxi;
+                    dudz = (V3(Q, 0, i, J, k) - V3(Q, 0, i, J, km)) * dzi;
+                    const double t31be = vis * dudz, t31bi = vis * dwdx;
+                    const double d31e = (t31fe - t31be) * coe1, d31i = (t31fi - t31bi) * coe1;
+                    vis = V3(MU, 2, i, J + 1, k);
+                    double dwdy = (V3(Q, 2, i, J + 1, k) - V3(Q, 2, i, J, k)) * m->dyci[J + 1];
+                    double dvdz = (V3(Q, 1, i, J + 1, k) - V3(Q, 1, i, J + 1, km)) * dzi;
+                    const double t32fe = vis * dvdz, t32fi = vis * dwdy;
+                    vis = V3(MU, 2, i, J, k);
+                    dwdy = (V3(Q, 2, i, J, k) - V3(Q, 2, i, J - 1, k)) * m->dyci[J];
+                    dvdz = (V3(Q, 1, i, J, k) - V3(Q, 1, i, J, km)) * dzi;
+                    const double t32be = vis * dvdz, t32bi = vis * dwdy;
+                    const double d32e = (t32fe - t32be) * coe2, d32i = (t32fi - t32bi) * coe2;
+                    ex = d31e + d32e + d33e;
+                    imp = d31i + d32i + d33i;
+                }
+                CX(s->h[c], i, J, k) = ex + CX(s->h[c], i, J, k);
+                CX(s->rim, i, J, k) = imp;
+            }
+        }
+    }
+}
+
+/* The Crank-Nicolson wall coefficients of TDMA_COEF (TDMA_COEF.f90:76-87): AMVR1 at the
+ * bottom wall, APVRN at the top wall of the u and w y-stencils. */
+static void or_cn_wall_coefs(const or_mesh *m, double *amvr1, double *apvrn)
+{
+    double ucaj = 4.0 / (2.0 / m->dyci[2] + 1.0 / m->dyfi[0]);
+    *amvr1 = ucaj * (2.0 * m->dyfi[0]);
+    ucaj = 4.0 / (2.0 / m->dyci[MM] + 1.0 / m->dyfi[MM - 1]);
+    *apvrn = ucaj * (2.0 * m->dyfi[MM - 1]);
+}
+
+/* MOMFA_io(NS, IDR) (MOMFA_io.f90:18-46): the variable-coefficient ADI factors
+ * (1 - Ax)(1 - Az)(1 - Ay) dG = RHS_io with FACOE = alpha dt / (2 Re) and coefficients
+ * mu / rho_stg per point, MOMFA1_X_io (:77-176, cyclic TDMAIJI_CYC per z-plane over rows
+ * N2I..M), MOMFA2_Z_io (:179-265, cyclic along z per x-plane), MOMFA3_Y_io (:268-406,
+ * TDMAIJJ_nonCYC along y per z-plane, zero boundary values), then G += RHS_io for rows
+ * N2I..M.  The y-solve is the one-rank form: the wall viscosities of the u, w lines come from
+ * the line's own z-plane (MOMFA_io.f90:342-352). */
+void or_io_momfa(const or_mesh *m, const or_io_par *p, or_io *s, int ns, int idr, double dt)
+{
+    const int c = idr - 1, n2i = idr == 2 ? 2 : 1, nj = MM - n2i + 1;
+    const double facoe = 0.5 * m->talp[ns] * dt * p->cvisc;
+    const double *V = s->m1, *MU = s->mstg, *DS = s->dstg;
+    double *R = s->rim;
+    const size_t nmax = (size_t)(LL > NN ? LL : NN) * (MM > nj ? MM : nj);
+    double *A = (double *)malloc(sizeof(double) * 4 * nmax + sizeof(double) * 2 * LL);
+    double *B = A + nmax, *Cc = A + 2 * nmax, *F = A + 3 * nmax, *bcz = A + 4 * nmax;
+    /* x */
+    {
+        const double coe1 = facoe * 4.0 / 3.0 * m->dxqi, coe2 = facoe * m->dxqi;
+        for (int k = 0; k < NN; ++k) {
+            for (int J = n2i; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) {
+                    const int im = im_(m, i), ip = ip_(m, i);
+                    double amu, apu;
+                    if (idr == 1) { amu = -coe1 * HX(V, im, J, k); apu = -coe1 * HX(V, i, J, k); }
+                    else if (idr == 2) { amu = -coe2 * V3(MU, 0, i, J, k); apu = -coe2 * V3(MU, 0, ip, J, k); }
+                    else { amu = -coe2 * V3(MU, 1, i, J, k); apu = -coe2 * V3(MU, 1, ip, J, k); }
+                    const size_t o = (size_t)(J - n2i) * LL + i;
+                    A[o] = amu / V3(DS, c, im, J, k);
+                    Cc[o] = apu / V3(DS, c, ip, J, k);
+                    B[o] = 1.0 - (amu + apu) / V3(DS, c, i, J, k);
+                    F[o] = CX(R, i, J, k);
+                }
+            or_tdma_lines(0, LL, nj, A, B, Cc, F, NULL);
+            for (int J = n2i; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) CX(R, i, J, k) = F[(size_t)(J - n2i) * LL + i];
+        }
+    }
+    /* z (RCCI2 = RNDI2 = 1 on the channel) */
+    {
+        const double coe1 = facoe * 4.0 / 3.0 * m->dzqi, coe2 = facoe * m->dzqi;
+        for (int i = 0; i < LL; ++i) {
+            for (int J = n2i; J <= MM; ++J)
+                for (int k = 0; k < NN; ++k) {
+                    const int km = km_(m, k), kp = kp_(m, k);
+                    double amu, apu;
+                    if (idr == 1) { amu = -coe2 * V3(MU, 1, i, J, k); apu = -coe2 * V3(MU, 1, i, J, kp); }
+                    else if (idr == 2) { amu = -coe2 * V3(MU, 2, i, J, k); apu = -coe2 * V3(MU, 2, i, J, kp); }
+                    else { amu = -coe1 * HX(V, i, J, km); apu = -coe1 * HX(V, i, J, k); }
+                    const size_t o = (size_t)(J - n2i) * NN + k;
+                    A[o] = amu / V3(DS, c, i, J, km);
+                    Cc[o] = apu / V3(DS, c, i, J, kp);
+                    B[o] = 1.0 - (amu + apu) / V3(DS, c, i, J, k);
+                    F[o] = CX(R, i, J, k);
+                }
+            or_tdma_lines(0, NN, nj, A, B, Cc, F, NULL);
+            for (int J = n2i; J <= MM; ++J)
+                for (int k = 0; k < NN; ++k) CX(R, i, J, k) = F[(size_t)(J - n2i) * NN + k];
+        }
+    }
+    /* y, rows 1..M of every line */
+    {
+        double amvr1, apvrn;
+        or_cn_wall_coefs(m, &amvr1, &apvrn);
+        const double coe1 = facoe * 4.0 / 3.0, coe2 = facoe;
+        const double *am = m->amvr + (size_t)c * (MM + 2), *ap = m->apvr + (size_t)c * (MM + 2);
+        memset(bcz, 0, sizeof(double) * 2 * LL);
+        for (int k = 0; k < NN; ++k) {
+            for (int J = 1; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) {
+                    const size_t o = (size_t)(J - 1) * LL + i;
+                    double amu, apu, a, cc, b;
+                    if (idr != 2) {
+                        const double *FM = MU + (size_t)c * NN * (MM + 2) * LL;
+                        if (J == 1) {
+                            const double muqt = (HX(V, i, 1, k) + HX(V, i, 0, k)) * 0.5;
+                            amu = -coe2 * amvr1 * muqt;
+                            apu = -coe2 * ap[J] * HX(FM, i, J + 1, k);
+                            a = 0.0;
+                            cc = apu / V3(DS, c, i, J + 1, k);
+                        } else if (J == MM) {
+                            const double muqt = (HX(V, i, MM, k) + HX(V, i, MM + 1, k)) * 0.5;
+                            amu = -coe2 * am[J] * HX(FM, i, J, k);
+                            apu = -coe2 * apvrn * muqt;
+                            a = amu / V3(DS, c, i, J - 1, k);
+                            cc = 0.0;
+                        } else {
+                            amu = -coe2 * am[J] * HX(FM, i, J, k);
+                            apu = -coe2 * ap[J] * HX(FM, i, J + 1, k);
+                            a = amu / V3(DS, c, i, J - 1, k);
+                            cc = apu / V3(DS, c, i, J + 1, k);
+                        }
+                        b = 1.0 - (amu + apu) / V3(DS, c, i, J, k);
+                    } else {
+                        amu = -coe1 * am[J] * HX(V, i, J - 1, k);
+                        apu = -coe1 * ap[J] * HX(V, i, J, k);
+                        if (J == 1) {   /* AMVR(1, 2) = 0: AMU = 0, whatever D_STG holds below the wall */
+                            a = 0.0;
+                            cc = apu / V3(DS, 1, i, J + 1, k);
+                        } else {
+                            a = amu / V3(DS, 1, i, J - 1, k);
+                            cc = apu / V3(DS, 1, i, J, k);
+                        }
+                        b = 1.0 - (amu + apu) / V3(DS, 1, i, J, k);
+                    }
+                    A[o] = a;
+                    Cc[o] = cc;
+                    B[o] = b;
+                    F[o] = CX(R, i, J, k);
+                }
+            or_tdma_lines(1, LL, MM, A, B, Cc, F, bcz);
+            for (int J = 1; J <= MM; ++J)
+                for (int i = 0; i < LL; ++i) CX(R, i, J, k) = F[(size_t)(J - 1) * LL + i];
+        }
+    }
+    free(A);
+    for (int k = 0; k < NN; ++k)
+        for (int J = n2i; J <= MM; ++J)
+            for (int i = 0; i < LL; ++i) V3(s->g, c, i, J, k) = CX(R, i, J, k) + V3(s->g, c, i, J, k);
+}
+
 /* What SOLVE does between the energy and the momentum substep of stage NS in a heated run
  * (SOLVE.f90:122-124). */
 void or_io_props(const or_mesh *m, or_io *s, int ns)
@@ -1158,22 +1428,45 @@ void or_io_props(const or_mesh *m, or_io *s, int ns)
     or_io_mustg(m, s);
 }
 
-/* SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:11-120), explicit viscous scheme. */
-void or_io_substep(const or_mesh *m, const or_io_par *p, or_io *s, int ns, double dt)
+/* The momentum part of SOLVERRK3_MOM_io(NS) with iVisScheme = 1 (SOLVERRK3_MOM_IO.f90:25-40,
+ * :61-74): per component the split viscous terms, RHS_MOM_EXPLICIT_io and MOMFA_io; then the
+ * walls of G (BC_WALL_G_io). */
+void or_io_momentum_cn(const or_mesh *m, const or_io_par *p, or_io *s, int ns, double dt)
 {
     const size_t hs = (size_t)NN * (MM + 2) * LL;
     memcpy(s->g0, s->g, sizeof(double) * 3 * hs);
     or_io_velocity(m, s);
     or_io_convection(m, s);
-    or_io_divu(m, s);
-    or_io_viscous(m, p, s);
-    for (int idr = 1; idr <= 3; ++idr) or_io_rhs(m, p, s, ns, idr, dt);
-    or_io_massflux_calc(m, s);
+    for (int idr = 1; idr <= 3; ++idr) {
+        or_io_viscous_cn(m, p, s, idr);
+        or_io_rhs(m, p, s, ns, idr, dt);
+        if (s->rkeep)
+            memcpy(s->rkeep + (size_t)(idr - 1) * NN * MM * LL, s->rim, sizeof(double) * (size_t)NN * MM * LL);
+        or_io_momfa(m, p, s, ns, idr, dt);
+    }
+    or_io_bc_zero(m, s->g);
+}
+
+/* SOLVERRK3_MOM_io(NS) (SOLVERRK3_MOM_IO.f90:11-120), either viscous scheme. */
+void or_io_substep(const or_mesh *m, const or_io_par *p, or_io *s, int ns, double dt)
+{
+    const size_t hs = (size_t)NN * (MM + 2) * LL;
+    if (p->vis_scheme == 1) {
+        or_io_momentum_cn(m, p, s, ns, dt);
+    } else {
+        memcpy(s->g0, s->g, sizeof(double) * 3 * hs);
+        or_io_velocity(m, s);
+        or_io_convection(m, s);
+        or_io_divu(m, s);
+        or_io_viscous(m, p, s);
+        for (int idr = 1; idr <= 3; ++idr) or_io_rhs(m, p, s, ns, idr, dt);
+        or_io_massflux_calc(m, s);
+    }
     or_io_divg(m, s, ns, dt);
     memset(s->dph, 0, sizeof(double) * hs);
     or_poisson(m, s->rhsp, s->dph);
     or_bc_copy(m, s->dph);
-    or_io_correct(m, s, ns, dt);
+    or_io_correct(m, p, s, ns, dt);
     or_io_velocity(m, s);
 }
 #undef V3

@@ -266,6 +266,68 @@ def make_io(T0=665.0):
     print("wrote", path, os.path.getsize(path))
 
 
+def cn_viscosity(st, seed, walls_zero):
+    """Current viscosity for the Crank-Nicolson fixtures: Viscousity0 with 10 % seeded noise;
+    walls_zero: the ghost rows are the negatives of the wall cells, so the wall-row means the
+    u, w y-solves use are zero."""
+    rng = np.random.Generator(np.random.Philox(key=seed + 1000))
+    m1 = st["M0"] * (1.0 + 0.1 * rng.standard_normal(st["M0"].shape))
+    if walls_zero:
+        M = m1.shape[1] - 2
+        m1[:, 0] = -m1[:, 1]
+        m1[:, M + 1] = -m1[:, M]
+    return m1
+
+
+def make_io_cn(T0=665.0):
+    """Main-domain momentum fixture, Crank-Nicolson viscous scheme (iVisScheme = 1), from the
+    IO_CASES states (io.npz holds them):
+      cn_*   ``iocn`` of stage 1 with a noisy current viscosity: G after the three MOMFA_io
+             calls, RHS_io as each receives it, EXPLT0, DRHOI_STG;
+      s1..3  three substeps (``iosubstep``, FFT99 Poisson) with the wall-row viscosity means
+             zero: G, PR, EXPLT0 after each, RHSP and DPH of the first.
+    The reference runs on two ranks; the first inputs are compared on the first half of z
+    only, the second anywhere (the tests say why)."""
+    sys.path.insert(0, os.path.join(ROOT))
+    import __graft_entry__
+    __graft_entry__.load()
+    from chapsim_amd import thermo
+    rows = thermo.synthetic_table()
+    out = {}
+    for L, M, N, STR2, REN, dt, grav, fd, seed in IO_CASES:
+        mesh = setup.Mesh(L, M, N, STR2=STR2)
+        st = oracle.io_initial_field(mesh, seed)
+        m1 = cn_viscosity(st, seed, False)
+        m1w = cn_viscosity(st, seed, True)
+        tag = f"io{L}x{M}x{N}"
+        for run, visc, cmds in (
+                ("cn", m1, ["ioprops 1", f"iocn 1 {dt!r} c1"]),
+                ("sub", m1w, ["ioprops 1", f"iosubstep 1 {dt!r}", "dumpio s1", "ioprops 2", f"iosubstep 2 {dt!r}",
+                              "dumpio s2", "ioprops 3", f"iosubstep 3 {dt!r}", "dumpio s3"])):
+            h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt, iDomain=2, iThermoDynamics=1, T0=T0, Ti=T0,
+                               iVisScheme=1, iGravity=grav, iFlowDriven=fd)
+            try:
+                thermo.write_table_file(h.path("NIST_WATER_23.5MP.DAT"), rows)
+                refrun.write_io(h, "io.bin", st["G"], st["G0"], st["PR"], st["D0"], st["M0"], st["EXPLT0"])
+                visc.tofile(h.path("m1.bin"))
+                h.run(["thermoinitall", "loadio io.bin", "loadvisc m1.bin"] + cmds)
+                if run == "cn":
+                    r = refrun.read_iocn(h, "c1")
+                    for k in ("G", "RHS", "EXPLT0", "DRHOI"):
+                        out[f"{tag}_cn_{k}"] = r[k]
+                else:
+                    for sname in ("s1", "s2", "s3"):
+                        r = refrun.read_io(h, sname)
+                        for k in ("G", "PR", "EXPLT0") + (("RHSP", "DPH") if sname == "s1" else ()):
+                            out[f"{tag}_{sname}_{k}"] = r[k]
+            finally:
+                h.close()
+        out[f"{tag}_params"] = np.array([L, M, N, STR2, REN, dt, grav, fd, seed, T0])
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "io_cn.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
 def make_meanzx():
     """x-z mean statistics fixture (PP_MEAN_ZX_FLOW_TG) of the fields after one RK3 step held
     in the tg_*.npz fixtures: stats [M, 122] (or_mean_zx's order) and U1mean / U1maxx."""
@@ -292,7 +354,7 @@ def make_meanzx():
 
 if __name__ == "__main__":
     import sys as _sys
-    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io", "meanzx"]
+    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io", "io_cn", "meanzx"]
     if "tg" in which:
         for case in CASES:
             make_case(*case)
@@ -306,5 +368,7 @@ if __name__ == "__main__":
         make_eng()
     if "io" in which:
         make_io()
+    if "io_cn" in which:
+        make_io_cn()
     if "meanzx" in which:
         make_meanzx()

@@ -1,0 +1,239 @@
+"""Main-domain momentum substep with the Crank-Nicolson viscous scheme (SOLVERRK3_MOM_io with
+iVisScheme = 1: VISCOUS_PAR_EXPLT_X/Y/Z_io, RHS_MOM_EXPLICIT_io, MOMFA_io, PRCALC_io's viscous
+term): the oracle restatement and the GPU substep against the reference's own output
+(tests/golden/io_cn.npz, written by the reference's routines through oracle/ref_harness.f90 on
+two ranks, from the io.npz states).
+
+The reference's y-solve (MOMFA3_Y_io) depends on the rank count in two ways, so the fixtures
+are read where a two-rank run computes the one-rank formula the oracle and the GPU follow:
+  * after TRASP23_Y2Z a rank holds z-planes KDSWT(MYID).., but the wall viscosities of the u, w
+    lines are read from the y-slab arrays Viscousity / VISN with the slab-local z index
+    (MOMFA_io.f90:342-352), and AMVR1 / APVRN, the wall coefficients, are set by TDMA_COEF on
+    rank 0 only and never broadcast (TDMA_COEF.f90:81, :87, :338-358): rank 1 solves its
+    z-planes with no wall term.  Hence the ``cn`` fixture (noisy current viscosity) is compared
+    on rank 0's z-planes k < N/2 only, where all of it is the one-rank formula, and the test
+    shows that rank 1's planes differ.
+  * the ``s1..s3`` fixtures use a viscosity whose ghost rows are the negatives of the wall
+    cells: the wall-row means are zero, both ranks' wall terms vanish alike, and every z-plane
+    of three whole substeps compares.
+DENSITY_IMPLICIT_add's DRHOI_STG is identically zero (it subtracts the reciprocal of D_STG's own
+expression of the same DENSITY0), checked on the reference's dump; the GPU and the oracle omit
+the terms it multiplies.  Tolerances (max-norm relative per field) as the explicit scheme's:
+1e-12 for G, Q and the momentum terms, 1e-10 for pressure and its correction."""
+import numpy as np
+import pytest
+
+from oracle import oracle, setup
+
+TOL, PTOL = 1e-12, 1e-10
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def cn_viscosity(st, seed, walls_zero):
+    """The current viscosity the fixtures were made with (tests/golden/make_golden.py)."""
+    rng = np.random.Generator(np.random.Philox(key=seed + 1000))
+    m1 = st["M0"] * (1.0 + 0.1 * rng.standard_normal(st["M0"].shape))
+    if walls_zero:
+        M = m1.shape[1] - 2
+        m1[:, 0] = -m1[:, 1]
+        m1[:, M + 1] = -m1[:, M]
+    return m1
+
+
+def cases(golden):
+    g, gi = golden("io_cn.npz"), golden("io.npz")
+    out = []
+    for tag in sorted({k.split("_")[0] for k in g}):
+        L, M, N, STR2, REN, dt, grav, fd, seed, T0 = g[tag + "_params"]
+        f_a, volm, cvisc = gi[tag + "_par"]
+        init = {k: gi[f"{tag}_init_{k}"] for k in ("G", "G0", "PR", "D0", "M0", "EXPLT0")}
+        out.append(dict(tag=tag, L=int(L), M=int(M), N=int(N), STR2=float(STR2), REN=float(REN), dt=float(dt),
+                        grav=int(grav), fd=int(fd), f_a=float(f_a), seed=int(seed), init=init,
+                        m1=cn_viscosity(init, int(seed), False), m1w=cn_viscosity(init, int(seed), True), g=g))
+    return out
+
+
+def make_oracle(c, vis_scheme=1):
+    mesh = setup.Mesh(c["L"], c["M"], c["N"], STR2=c["STR2"])
+    return oracle.OracleIO(mesh, c["REN"], gravity=c["grav"], f_a=c["f_a"], flow_driven=c["fd"],
+                           cf_given=8.18e-3, vis_scheme=vis_scheme)
+
+
+def check_state(f, g, s, M, first=False):
+    for c in range(3):
+        assert rel(f["G"][c], g[f"{s}_G"][c]) < TOL, (s, c, rel(f["G"][c], g[f"{s}_G"][c]))
+        j0 = 1 if c == 1 else 0        # v's row J = 1 is not an unknown
+        assert rel(f["EXPLT0"][c][:, j0:], g[f"{s}_EXPLT0"][c][:, j0:]) < TOL, (s, c)
+    assert rel(f["PR"], g[f"{s}_PR"]) < PTOL, (s, rel(f["PR"], g[f"{s}_PR"]))
+    if first:
+        assert rel(f["RHSP"], g[f"{s}_RHSP"]) < TOL
+        assert rel(f["DPH"], g[f"{s}_DPH"]) < PTOL
+
+
+def test_fixture_inputs(golden):
+    """The viscosities are reproducible from the seeds, DRHOI_STG of the reference is zero
+    everywhere, and the substep inputs have zero wall-row means."""
+    for c in cases(golden):
+        assert np.all(c["g"][c["tag"] + "_cn_DRHOI"] == 0.0)
+        M = c["M"]
+        assert np.all(c["m1w"][:, 0] + c["m1w"][:, 1] == 0.0)
+        assert np.all(c["m1w"][:, M] + c["m1w"][:, M + 1] == 0.0)
+        assert np.all(c["m1"] != c["init"]["M0"])
+
+
+def test_oracle_cn_momentum_matches_reference(golden):
+    for c in cases(golden):
+        g, tag, M, N = c["g"], c["tag"], c["M"], c["N"]
+        o = make_oracle(c)
+        o.set_state(dict(c["init"], M1=c["m1"]))
+        o.props(1)
+        o.momentum_cn(1, c["dt"])
+        # RHS_io as each MOMFA_io receives it and the new EXPLT0: every z-plane
+        for comp in range(3):
+            j0 = 1 if comp == 1 else 0
+            assert rel(o.f["RKEEP"][comp][:, j0:], g[f"{tag}_cn_RHS"][comp][:, j0:]) < TOL, comp
+            assert rel(o.f["EXPLT0"][comp][:, j0:], g[f"{tag}_cn_EXPLT0"][comp][:, j0:]) < TOL, comp
+        # G after the three line solves: rank 0's z-planes
+        h = N // 2
+        for comp in range(3):
+            gr = g[f"{tag}_cn_G"][comp]
+            assert rel(o.f["G"][comp][:h, 1:M + 1], gr[:h, 1:M + 1]) < TOL, comp
+        # ... and rank 1's planes of u and w carry the reference's rank-1 wall terms (none)
+        d = np.abs(o.f["G"][0][h:] - g[f"{tag}_cn_G"][0][h:]).max()
+        assert d > 1e-12 * np.abs(g[f"{tag}_cn_G"][0]).max()
+
+
+def test_oracle_cn_substeps_match_reference(golden):
+    for c in cases(golden):
+        g, tag, M = c["g"], c["tag"], c["M"]
+        o = make_oracle(c)
+        o.set_state(dict(c["init"], M1=c["m1w"]))
+        for ns in (1, 2, 3):
+            o.props(ns)
+            o.substep(ns, c["dt"])
+            check_state(o.f, g, f"{tag}_s{ns}", M, first=ns == 1)
+
+
+def test_oracle_cn_planted_mistakes_fail(golden):
+    """The fixtures tell the schemes apart and pin that the Crank-Nicolson terms read the
+    current viscosity: the explicit scheme, or Viscousity0 in place of Viscousity, moves G."""
+    c = cases(golden)[0]
+    g, tag = c["g"], c["tag"]
+    for vis, st in ((0, dict(c["init"], M1=c["m1w"])), (1, dict(c["init"]))):
+        o = make_oracle(c, vis_scheme=vis)
+        o.set_state(st)
+        o.props(1)
+        o.substep(1, c["dt"])
+        assert rel(o.f["G"][0], g[f"{tag}_s1_G"][0]) > 1e-9, vis
+
+
+@pytest.mark.gpu
+def test_gpu_cn_momentum_matches_reference(pkg, golden):
+    for c in cases(golden):
+        g, tag, M, N = c["g"], c["tag"], c["M"], c["N"]
+        print(f"io cn momentum {tag}", flush=True)
+        co = pkg.channel_setup(c["L"], M, N, STR2=c["STR2"])
+        ch = pkg.ChannelIO(co, c["REN"], gravity=c["grav"], f_a=c["f_a"], flow_driven=c["fd"], cf_given=8.18e-3,
+                           vis_scheme=1)
+        ch.upload(**c["init"], M1=c["m1"])
+        ch.ioprops(1)
+        ch.momentum(1, c["dt"])
+        f = ch.download()
+        o = make_oracle(c)
+        o.set_state(dict(c["init"], M1=c["m1"]))
+        o.props(1)
+        o.momentum_cn(1, c["dt"])
+        h = N // 2
+        for comp in range(3):
+            assert rel(f["G"][comp][:h, 1:M + 1], g[f"{tag}_cn_G"][comp][:h, 1:M + 1]) < TOL, comp
+            assert rel(f["G"][comp], o.f["G"][comp]) < TOL, comp
+            j0 = 1 if comp == 1 else 0
+            assert rel(f["EXPLT0"][comp][:, j0:], g[f"{tag}_cn_EXPLT0"][comp][:, j0:]) < TOL, comp
+        ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_cn_substeps_match_reference(pkg, golden):
+    for c in cases(golden):
+        g, tag, M = c["g"], c["tag"], c["M"]
+        print(f"io cn substeps {tag} gravity {c['grav']} driven {c['fd']}", flush=True)
+        co = pkg.channel_setup(c["L"], M, c["N"], STR2=c["STR2"])
+        ch = pkg.ChannelIO(co, c["REN"], gravity=c["grav"], f_a=c["f_a"], flow_driven=c["fd"], cf_given=8.18e-3,
+                           vis_scheme=1)
+        ch.upload(**c["init"], M1=c["m1w"])
+        for ns in (1, 2, 3):
+            ch.ioprops(ns)
+            ch.SOLVERRK3_MOM_io(ns, c["dt"])
+            check_state(ch.download(), g, f"{tag}_s{ns}", M, first=ns == 1)
+        ch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,M,N,steps", [(64, 48, 32, 3), (48, 36, 40, 3), (128, 64, 96, 1)])
+def test_gpu_cn_matches_oracle(pkg, L, M, N, steps):
+    """Larger seeded states with a noisy current viscosity against the oracle (gravity and
+    constant mass flux on)."""
+    print(f"io cn oracle {L}x{M}x{N}", flush=True)
+    mesh = setup.Mesh(L, M, N, STR2=2.0)
+    st = oracle.io_initial_field(mesh, 90 + L)
+    m1 = cn_viscosity(st, 90 + L, False)
+    REN, dt = 4000.0, 5e-4
+    o = oracle.OracleIO(mesh, REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
+    o.set_state(dict(st, M1=m1))
+    ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
+    ch.upload(**st, M1=m1)
+    for _ in range(steps):
+        for ns in (1, 2, 3):
+            o.props(ns)
+            o.substep(ns, dt)
+            ch.ioprops(ns)
+            ch.SOLVERRK3_MOM_io(ns, dt)
+    f = ch.download()
+    for k in ("G", "Q"):
+        for c in range(3):
+            assert rel(f[k][c], o.f[k][c]) < 1e-11, (k, c, rel(f[k][c], o.f[k][c]))
+    assert rel(f["PR"], o.f["PR"]) < 1e-9
+    ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_heated_stage_cn_matches_oracle(pkg):
+    """chs_heated_substep with the Crank-Nicolson momentum scheme: the momentum terms read the
+    energy context's current viscosity in place."""
+    from importlib import import_module
+    thermo = import_module(pkg.__name__ + ".thermo")
+    t = thermo.table_setup(thermo.synthetic_table(), 665.0)
+    L, M, N, STR2, REN, dt = 32, 24, 16, 1.9, 2800.0, 5e-4
+    mesh = setup.Mesh(L, M, N, STR2=STR2)
+    dh, _ = oracle.eng_initial_field(mesh, t, 83)
+    st = oracle.io_initial_field(mesh, 84)
+    cthecd = 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"])
+    dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+    wf = (0.05, -0.05)
+    oe = oracle.OracleEng(mesh, t, (1, 1), cthecd, dhmin, dhmax, wf)
+    assert oe.init(dh, st["G"]) == 0
+    oi = oracle.OracleIO(mesh, REN, flow_driven=1, vis_scheme=1)
+    oi.set_state(dict(st, D0=oe.f["d"], M0=oe.f["m"]))
+    co = pkg.channel_setup(L, M, N, STR2=STR2)
+    ge = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, wf)
+    ge.upload(dh, st["G"])
+    gi = pkg.ChannelIO(co, REN, flow_driven=1, vis_scheme=1)
+    gi.upload(G=st["G"], G0=st["G0"], PR=st["PR"], D0=oe.f["d"], M0=oe.f["m"], EXPLT0=st["EXPLT0"])
+    for ns in (1, 2, 3):
+        oe.f["g"][...] = oi.f["G"]
+        assert oe.substep(ns, dt) == 0
+        oi.f["D0"][...] = oe.f["d0"]
+        oi.f["M0"][...] = oe.f["m0"]
+        oi.f["M1"][...] = oe.f["m"]
+        oi.props(ns)
+        oi.substep(ns, dt)
+        pkg.heated_substep(ge, gi, ns, dt)
+    f = gi.download()
+    for c in range(3):
+        assert rel(f["G"][c], oi.f["G"][c]) < 1e-11, (c, rel(f["G"][c], oi.f["G"][c]))
+    assert rel(f["PR"], oi.f["PR"]) < 1e-9
+    ge.close()
+    gi.close()
