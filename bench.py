@@ -20,7 +20,8 @@ Other workloads (each prints its own JSON line; not the default run):
   --workload channel64   64^3 channel, Re_tau 180, dt 1e-3
   --workload poisson     standalone pressure Poisson 512x384x512, beta 2.2
   --workload ensemble    32 channels of seeded uneven sizes, one hipGraph per step
-  --workload heated      heated channel 256x128x256 (energy + variable-property momentum)
+  --workload heated      heated channel 256x128x256 (energy + variable-property momentum,
+                         Crank-Nicolson viscous scheme; --vis explicit for iVisScheme 0)
 """
 from __future__ import annotations
 
@@ -248,10 +249,11 @@ def _cpu_model():
     return "host"
 
 
-def cpu_baseline_heated(sample_steps: int = 3, cores: int | None = None):
+def cpu_baseline_heated(sample_steps: int = 3, cores: int | None = None, vis_scheme: int = 1):
     """Time the reference program on the heated main-domain workload: its own random start
-    (INITIAL_FL_THEML_FLD_io), energy from the first step (timeThermoStart = 0), explicit viscous
-    scheme, the same synthetic table; median per-step CpuT of the main-domain monitor.  The
+    (INITIAL_FL_THEML_FLD_io), energy from the first step (timeThermoStart = 0), the given
+    viscous scheme (iVisScheme), the same synthetic table; median per-step CpuT of the
+    main-domain monitor.  The
     reference's compiled Poisson call for this domain stores its solution with a shifted
     layout (SOLVERRK3_MOM_IO.f90:93), so its field diverges within a few steps; the per-step
     cost of the work it does is still what is timed."""
@@ -267,7 +269,7 @@ def cpu_baseline_heated(sample_steps: int = 3, cores: int | None = None):
     try:
         ini = refrun.ini_text(L, M, N, HX=c["HX"], HZ=c["HZ"], STR2=c["STR2"], REN=c["REN"], DT=c["dt"],
                               NTSTF=sample_steps, iDomain=2, iThermoDynamics=1, T0=c["T0"], Ti=c["T0"],
-                              iThermalWallType=(1, 1), thermalWallBC=(2.0e4, 2.0e4), iVisScheme=0,
+                              iThermalWallType=(1, 1), thermalWallBC=(2.0e4, 2.0e4), iVisScheme=vis_scheme,
                               timeThermoStart=0.0)
         with open(os.path.join(work, "readdata.ini"), "w") as fh:
             fh.write(ini)
@@ -282,7 +284,8 @@ def cpu_baseline_heated(sample_steps: int = 3, cores: int | None = None):
         s = float(np.median(cput))
         return dict(value=L * M * N / s, unit="grid-point RK3-steps/s", cores=P, kind="reference",
                     sample=f"reference CHAPSim (oracle/_ref/chapsim, amdflang -O2) mpirun -np {P} on "
-                           f"{_cpu_model()}; heated channel {L}x{M}x{N}, {len(cput)} RK3 steps from its own "
+                           f"{_cpu_model()}; heated channel {L}x{M}x{N}, iVisScheme={vis_scheme}, "
+                           f"{len(cput)} RK3 steps from its own "
                            f"random start; median per-step CpuT {s:.3f} s; run wall {wall:.1f} s")
     finally:
         shutil.rmtree(work, ignore_errors=True)
@@ -433,8 +436,9 @@ CFG_HEATED = dict(L=256, M=128, N=256, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN
 def bench_heated(pkg, args):
     """Quasi-incompressible heated channel on the main domain (config 5): per RK3 stage the
     energy substep (table properties, iso-flux walls), the staggered properties and the
-    variable-property momentum substep with the explicit viscous scheme, in SOLVE's order
-    (chs_heated_substep).  Plain FP64.  value = grid-point RK3-steps/s."""
+    variable-property momentum substep, Crank-Nicolson viscous scheme (iVisScheme = 1; --vis
+    explicit for iVisScheme = 0), in SOLVE's order (chs_heated_substep).  Plain FP64.
+    value = grid-point RK3-steps/s."""
     from chapsim_amd import init, thermo
     c = CFG_HEATED
     L, M, N = c["L"], c["M"], c["N"]
@@ -446,7 +450,8 @@ def bench_heated(pkg, args):
     eng = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, c["wallflux"])
     eng.upload(dh, g)
     f0 = eng.download()
-    io = pkg.ChannelIO(co, c["REN"], flow_driven=1)
+    vis = 1 if args.vis == "cn" else 0
+    io = pkg.ChannelIO(co, c["REN"], flow_driven=1, vis_scheme=vis)
     io.upload(G=g, PR=pr, D0=f0["D"], M0=f0["M"])
     dt = c["dt"]
 
@@ -464,15 +469,16 @@ def bench_heated(pkg, args):
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     f = io.download()
     fe = eng.download()
-    # per-step algorithmic bytes: 148 full-field FP64 reads/writes per stage (kernel list in
-    # the chs_io.hip / chs_thermo.hip launch sequence), three stages
-    alg = 3 * 148 * 8.0 * L * M * N
+    # per-step algorithmic bytes: full-field FP64 reads/writes per stage (kernel list in the
+    # chs_io.hip / chs_thermo.hip launch sequence): 148 explicit; Crank-Nicolson 174 (no
+    # DIVG_U_io pass, -4; line solves x 12, z 12, y + G update 15 in place of the G update 9)
+    alg = 3 * (174 if vis else 148) * 8.0 * L * M * N
     eng.close()
     io.close()
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline_heated()
+            cpu = cpu_baseline_heated(vis_scheme=vis)
         except Exception as e:  # pragma: no cover - host-dependent
             cpu = dict(error=str(e))
     print(json.dumps({
@@ -482,7 +488,8 @@ def bench_heated(pkg, args):
         "dtype": "f64", "data": "synthetic: laminar G + 5% seeded noise, T/T0 = 1 + 1% seeded; "
                                "synthetic 2048-point table",
         "config": {"workload": f"heated channel REN={c['REN']:g}, {L}x{M}x{N}, 4pi x 2 x 2pi, tanh "
-                               f"beta={c['STR2']:g}, dt={dt:g}, iso-flux walls, explicit viscous, RK3"},
+                               f"beta={c['STR2']:g}, dt={dt:g}, iso-flux walls, "
+                               f"{'Crank-Nicolson' if vis else 'explicit'} viscous, RK3"},
         "step_roofline": {"alg_bytes_per_step": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
                           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
@@ -498,6 +505,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--vis", choices=["cn", "explicit"], default="cn",
+                    help="heated workload's viscous scheme (iVisScheme 1 or 0)")
     ap.add_argument("--graph", action="store_true", help="replay each RK3 step from a hipGraph")
     ap.add_argument("--autotune", action="store_true",
                     help="search the kernels' launch shapes (chapsim_amd.tune) before timing")

@@ -11,6 +11,7 @@
 // layout: halo fields [N][M+2][L] (x fastest, y rows 0..M+1), cell fields [N][M][L].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <utility>
 #include <vector>
@@ -591,6 +592,149 @@ __global__ __launch_bounds__(128) void k_io_cn_x(IoDev d, const double *__restri
     else io_cn_x_line<2>(d, l, 0.0, facoe);
 }
 
+// Cyclic tridiagonal solve by parallel cyclic reduction in LDS: the threads of a block hold
+// one equation each (element e of a line of n, a power of two >= 4; slot(e) its LDS index).
+// Each step eliminates the neighbours at distance s (s = 1, 2, .., n/4); then every equation
+// couples x_e with x_(e + n/2) only and each thread solves its 2 x 2 pair.  Strictly diagonally
+// dominant rows (the Crank-Nicolson factors) keep that property through the steps.  Every
+// thread of the block must call it.  Returns x_e.
+template <class Slot>
+__device__ __forceinline__ double pcr_cyc(int n, int e, const Slot &slot, double a, double b, double c, double r,
+                                          double *sa, double *sb, double *sc, double *sr)
+{
+#pragma clang fp contract(off)
+    const int me = slot(e);
+    for (int s = 1; s < n / 2; s <<= 1) {
+        sa[me] = a;
+        sb[me] = 1.0 / b;
+        sc[me] = c;
+        sr[me] = r;
+        __syncthreads();
+        const int m = slot((e - s) & (n - 1)), p = slot((e + s) & (n - 1));
+        const double k1 = a * sb[m], k2 = c * sb[p];
+        const double am = sa[m], cm = sc[m], rm = sr[m], ap = sa[p], cp = sc[p], rp = sr[p];
+        __syncthreads();
+        a = -am * k1;
+        c = -cp * k2;
+        b = b - cm * k1 - ap * k2;
+        r = r - rm * k1 - rp * k2;
+    }
+    const double ei = a + c;
+    sa[me] = ei;
+    sb[me] = b;
+    sr[me] = r;
+    __syncthreads();
+    const int j = slot((e + n / 2) & (n - 1));
+    const double ej = sa[j], bj = sb[j], rj = sr[j];
+    return (r * bj - ei * rj) / (b * bj - ei * ej);
+}
+
+// MOMFA1_X_io for L a power of two (4 <= L <= 512): T = blockDim.x / L x-lines per block, one
+// thread per point, coalesced; the coefficients are built from the staged property rows as
+// io_cn_x_line builds them, then pcr_cyc.  Lines past the end solve a dummy identity row.
+template <int C>
+__device__ __forceinline__ void io_cn_x_pcr(const IoDev &d, double dp, double facoe, double *lds)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M, j0 = C == 1 ? 2 : 1, nj = M - j0 + 1, nl = nj * d.N;
+    const int T = blockDim.x / L, lt = threadIdx.x / L, e = threadIdx.x % L;
+    const int line = blockIdx.x * T + lt;
+    const bool valid = line < nl;
+    const int J = j0 + line % nj, k = line / nj;
+    const size_t hs = hsz(d), cs = csz(d);
+    const int S = blockDim.x;
+    double *s0 = lds, *s1 = lds + S, *s2 = lds + 2 * S, *s3 = lds + 3 * S;
+    auto slot = [&](int ee) { return lt * L + ee; };
+    double f0 = 0.0, f1 = 1.0, r = 0.0;
+    if (valid) {
+        const size_t o = hx(d, e, J, k);
+        f0 = C == 0 ? d.m1[o] : d.mstg[(C - 1) * hs + o];
+        f1 = d.dstg[C * hs + o];
+        r = d.h[C * cs + cx(d, e, J, k)];
+        if constexpr (C == 0) r = r - dp;
+    }
+    s0[threadIdx.x] = f0;
+    s1[threadIdx.x] = f1;
+    __syncthreads();
+    const int em = slot((e - 1) & (L - 1)), ep = slot((e + 1) & (L - 1));
+    double amu, apu;
+    if constexpr (C == 0) {
+        const double coe1 = facoe * 4.0 / 3.0 * d.dxqi;
+        amu = -coe1 * s0[em];
+        apu = -coe1 * f0;
+    } else {
+        const double coe2 = facoe * d.dxqi;
+        amu = -coe2 * f0;
+        apu = -coe2 * s0[ep];
+    }
+    double a = amu / s1[em], c = apu / s1[ep], b = 1.0 - (amu + apu) / f1;
+    if (!valid) a = c = 0.0;
+    __syncthreads();
+    const double x = pcr_cyc(L, e, slot, a, b, c, r, s0, s1, s2, s3);
+    if (valid) d.h[C * cs + cx(d, e, J, k)] = x;
+}
+
+__global__ __launch_bounds__(512) void k_io_cn_x_pcr(IoDev d, const double *__restrict__ dpg_dev, double dpg,
+                                                     double facoe)
+{
+    extern __shared__ double lds[];
+    if (blockIdx.y == 0) io_cn_x_pcr<0>(d, dpg_dev ? dpg_dev[0] : dpg, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_x_pcr<1>(d, 0.0, facoe, lds);
+    else io_cn_x_pcr<2>(d, 0.0, facoe, lds);
+}
+
+// MOMFA2_Z_io for N a power of two (4 <= N <= 1024): a block holds T = blockDim.x / N
+// consecutive x-columns of one row J, thread (lt, k) at lt + T k so a wave reads runs of
+// consecutive x; coefficients from the staged property columns, then pcr_cyc.
+template <int C>
+__device__ __forceinline__ void io_cn_z_pcr(const IoDev &d, double facoe, double *lds)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M, N = d.N, j0 = C == 1 ? 2 : 1;
+    const int T = blockDim.x / N, lt = threadIdx.x % T, e = threadIdx.x / T;
+    const int ng = (L + T - 1) / T;
+    const int i = (blockIdx.x % ng) * T + lt, J = j0 + blockIdx.x / ng;
+    const bool valid = i < L && J <= M;
+    const size_t hs = hsz(d), cs = csz(d);
+    const int S = blockDim.x;
+    double *s0 = lds, *s1 = lds + S, *s2 = lds + 2 * S, *s3 = lds + 3 * S;
+    auto slot = [&](int ee) { return ee * T + lt; };
+    double f0 = 0.0, f1 = 1.0, r = 0.0;
+    if (valid) {
+        const size_t o = hx(d, i, J, e);
+        f0 = C == 2 ? d.m1[o] : d.mstg[(C + 1) * hs + o];
+        f1 = d.dstg[C * hs + o];
+        r = d.h[C * cs + cx(d, i, J, e)];
+    }
+    s0[threadIdx.x] = f0;
+    s1[threadIdx.x] = f1;
+    __syncthreads();
+    const int km = slot((e - 1) & (N - 1)), kp = slot((e + 1) & (N - 1));
+    double amu, apu;
+    if constexpr (C == 2) {
+        const double coe1 = facoe * 4.0 / 3.0 * d.dzqi;
+        amu = -coe1 * s0[km];
+        apu = -coe1 * f0;
+    } else {
+        const double coe2 = facoe * d.dzqi;
+        amu = -coe2 * f0;
+        apu = -coe2 * s0[kp];
+    }
+    double a = amu / s1[km], c = apu / s1[kp], b = 1.0 - (amu + apu) / f1;
+    if (!valid) a = c = 0.0;
+    __syncthreads();
+    const double x = pcr_cyc(N, e, slot, a, b, c, r, s0, s1, s2, s3);
+    if (valid) d.h[C * cs + cx(d, i, J, e)] = x;
+}
+
+__global__ __launch_bounds__(1024) void k_io_cn_z_pcr(IoDev d, double facoe)
+{
+    extern __shared__ double lds[];
+    if (blockIdx.y == 0) io_cn_z_pcr<0>(d, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_z_pcr<1>(d, facoe, lds);
+    else io_cn_z_pcr<2>(d, facoe, lds);
+}
+
 // MOMFA2_Z_io (MOMFA_io.f90:179-265): cyclic z-lines (RCCI2 = RNDI2 = 1 on the channel).
 template <int C>
 __device__ __forceinline__ void io_cn_z_line(const IoDev &d, int l, double facoe)
@@ -1002,8 +1146,22 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
         // here (each reads Q, the properties and its own increment), so they run side by side.
         const double facoe = 0.5 * ta * dt * d.cvisc;
         const int L = m->L, M = m->M, N = m->N;
-        hipLaunchKernelGGL(k_io_cn_x, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg, facoe);
-        hipLaunchKernelGGL(k_io_cn_z, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
+        if ((L & (L - 1)) == 0 && L >= 4 && L <= 512) {
+            const int T = L >= 256 ? 1 : 256 / L;   // x-lines per block
+            hipLaunchKernelGGL(k_io_cn_x_pcr, dim3((M * N + T - 1) / T, 3), dim3(T * L), 4 * sizeof(double) * T * L,
+                               m->stream, d, dpg_dev, dpg, facoe);
+        } else {
+            hipLaunchKernelGGL(k_io_cn_x, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg,
+                               facoe);
+        }
+        if ((N & (N - 1)) == 0 && N >= 4 && N <= 1024) {
+            const int T = std::max(1, 1024 / N);   // x-columns per block
+            const int ng = (L + T - 1) / T;
+            hipLaunchKernelGGL(k_io_cn_z_pcr, dim3(ng * M, 3), dim3(T * N), 4 * sizeof(double) * T * N, m->stream, d,
+                               facoe);
+        } else {
+            hipLaunchKernelGGL(k_io_cn_z, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
+        }
         hipLaunchKernelGGL(k_io_cn_y, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
     } else {
         hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
