@@ -21,6 +21,7 @@
 hipStream_t chs_internal_stream(chs_ctx *c);
 void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **m, double **g);
 void chs_internal_eng_set_g(chs_eng *e, const double *g);
+double *chs_internal_eng_density(chs_eng *e);
 
 namespace {
 
@@ -904,6 +905,50 @@ __global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt, int cn
     }
 }
 
+// CFL_io (CFL.f90:59-107): per cell the larger of the convective rate
+// (|u_i + u_i+1| DXI + |v_J + v_J+1| DYFI + |w_k + w_k+1| DZI) / 2 and the viscous rate
+// 2 CVISC (DXQI + DZQI + DYFI^2) Viscousity / DENSITY; block maxima to d.partial.
+__global__ __launch_bounds__(256) void k_io_cfl(IoDev d, const double *__restrict__ dens,
+                                                const double *__restrict__ visc)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[256];
+    IO_THREAD(1)
+    double v = 0.0;
+    if (i < L && J <= M) {
+        const int ip = i == L - 1 ? 0 : i + 1, kp = k == N - 1 ? 0 : k + 1;
+        const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+        const double *Q = d.q;
+        const double qcf = (fabs((Q[c0] + Q[hx(d, ip, J, k)]) * d.dxi) +
+                            fabs((Q[hs + c0] + Q[hs + hx(d, i, J + 1, k)]) * d.dyfi[J]) +
+                            fabs((Q[2 * hs + c0] + Q[2 * hs + hx(d, i, J, kp)]) * d.dzi)) * 0.5;
+        const double vcf = 2.0 * d.cvisc * (d.dxqi + d.dzqi + d.dyfi[J] * d.dyfi[J]) * visc[c0] / dens[c0];
+        v = fmax(qcf, vcf);
+    }
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) sh[t] = fmax(sh[t], sh[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_io_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) v = fmax(v, part[j]);
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 __global__ __launch_bounds__(256) void k_io_fill(double *__restrict__ p, size_t n, double v)
 {
     const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -1214,6 +1259,42 @@ int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, do
             }
     }
     return CHS_OK;
+}
+
+namespace {
+
+int io_cfl(chs_io *m, const double *dens, const double *visc, double dt, double *cfl, const char *what)
+{
+    const dim3 blk(64, 4), gin = io_grid(m, m->M);
+    hipLaunchKernelGGL(k_io_cfl, gin, blk, 0, m->stream, m->d, dens, visc);
+    hipLaunchKernelGGL(k_io_reduce_max, dim3(1), dim3(256), 0, m->stream, m->d.partial, m->npart, m->d.scal + 1);
+    double v = 0.0;
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess) he = hipMemcpyAsync(&v, m->d.scal + 1, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    if (he != hipSuccess) return io_hip(m, he, what);
+    *cfl = v * dt;
+    return CHS_OK;
+}
+
+}  // namespace
+
+int chs_cfl_io(chs_io *m, double dt, double *cfl)
+{
+    if (!m || !cfl) return io_fail(m, CHS_E_ARG, "chs_cfl_io: null argument");
+    return io_cfl(m, m->own_d0, m->cn ? m->own_m1 : m->own_m0, dt, cfl, "chs_cfl_io");
+}
+
+int chs_heated_cfl(chs_eng *e, chs_io *m, double dt, double *cfl)
+{
+    if (!e || !m || !cfl) return io_fail(m, CHS_E_ARG, "chs_heated_cfl: null argument");
+    hipStream_t es;
+    double *ed0, *em0, *em, *eg;
+    chs_internal_eng_fields(e, &es, &ed0, &em0, &em, &eg);
+    double *ed = chs_internal_eng_density(e);
+    hipError_t he = hipStreamSynchronize(es);   // the energy context's properties are final
+    if (he != hipSuccess) return io_hip(m, he, "chs_heated_cfl");
+    return io_cfl(m, ed, em, dt, cfl, "chs_heated_cfl");
 }
 
 int chs_io_sync(chs_io *m)

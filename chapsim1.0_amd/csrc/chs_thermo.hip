@@ -702,6 +702,9 @@ __attribute__((visibility("hidden"))) void chs_internal_eng_fields(chs_eng *e, h
     *g = const_cast<double *>(e->e.g1);
 }
 
+// The current DENSITY of an energy context (CFL_io reads it).
+__attribute__((visibility("hidden"))) double *chs_internal_eng_density(chs_eng *e) { return e->e.d; }
+
 // Point an energy context's G at a momentum context's G array (3 halo fields) instead of
 // copying it (NULL: back to the context's own array, as chs_eng_upload and
 // chs_eng_set_mass_flux also do).

@@ -295,6 +295,23 @@ MODULE chapsim_hip
             INTEGER(C_INT) :: rc
         END FUNCTION
 
+        ! CFL_io (CFL.f90:59-107) x dt; dt = 1 gives CFLMM_io
+        FUNCTION chs_cfl_io(io, dt, cfl) BIND(C, NAME='chs_cfl_io') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: io
+            REAL(C_DOUBLE), VALUE :: dt
+            REAL(C_DOUBLE), INTENT(OUT) :: cfl
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        FUNCTION chs_heated_cfl(eng, io, dt, cfl) BIND(C, NAME='chs_heated_cfl') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: eng, io
+            REAL(C_DOUBLE), VALUE :: dt
+            REAL(C_DOUBLE), INTENT(OUT) :: cfl
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
         FUNCTION chs_heated_substep(eng, io, ns, dt) BIND(C, NAME='chs_heated_substep') RESULT(rc)
             IMPORT :: C_INT, C_PTR, C_DOUBLE
             TYPE(C_PTR), VALUE :: eng, io

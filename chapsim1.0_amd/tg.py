@@ -92,6 +92,8 @@ SIGNATURES = {
     "chs_substep_io": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_substep_io_momentum": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_io_upload_viscosity": (C.c_int, [C.c_void_p, _dp]),
+    "chs_cfl_io": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
+    "chs_heated_cfl": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
     "chs_io_download": (C.c_int, [C.c_void_p] + [_dp] * 9),
     "chs_io_sync": (C.c_int, [C.c_void_p]),
     "chs_heated_substep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double]),
@@ -473,6 +475,13 @@ class ChannelIO:
     def sync(self):
         self._check(lib().chs_io_sync(self._h), "chs_io_sync")
 
+    def cfl(self, dt: float) -> float:
+        """CFL_io (CFL.f90:59-107) x dt on the current velocity, with the context's DENSITY0 and
+        viscosity (a flow-only run's DENSITY, Viscousity); cfl(1.0) is CFLMM_io."""
+        v = C.c_double()
+        self._check(lib().chs_cfl_io(self._h, float(dt), C.byref(v)), "chs_cfl_io")
+        return v.value
+
     def download(self):
         """dict of G, Q, DSTG, MSTG [3, N, M+2, L]; PR, DPH [N, M+2, L]; EXPLT0 [3, N, M, L];
         DRHO, RHSP [N, M, L]."""
@@ -504,6 +513,23 @@ def heated_substep(eng, io: ChannelIO, ns: int, dt: float):
     rc = lib().chs_heated_substep(eng._h, io._h, int(ns), float(dt))
     if rc != 0:
         raise ChsError(f"chs_heated_substep failed ({rc}): {lib().chs_io_last_error(io._h).decode()}")
+
+
+def heated_cfl(eng, io: ChannelIO, dt: float) -> float:
+    """CFL_io x dt of a heated run: the momentum context's velocity with the energy context's
+    current DENSITY and Viscousity (chs_heated_cfl); heated_cfl(eng, io, 1.0) is CFLMM_io."""
+    v = C.c_double()
+    rc = lib().chs_heated_cfl(eng._h, io._h, float(dt), C.byref(v))
+    if rc != 0:
+        raise ChsError(f"chs_heated_cfl failed ({rc}): {lib().chs_io_last_error(io._h).decode()}")
+    return v.value
+
+
+def next_dt(dt: float, dt0: float, cflgv: float, cflmm: float, dtmin: float) -> float:
+    """The time-step rule of BCAST_COMM_STEP (SOLVE.f90:169-178): the given DT0, at most five
+    times the last step and CFLGV / CFLMM (CFLMM = max of CFLMM_tg and CFLMM_io, SOLVE.f90:79-85,
+    e.g. ChannelTG.cfl(1.0), ChannelIO.cfl(1.0)), not below DTMIN."""
+    return max(min(dt0, 5.0 * dt, cflgv / cflmm), dtmin)
 
 
 def restart_write_tg(directory, q, pr, iterg, phytime, REN, dt):

@@ -413,6 +413,16 @@ int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, do
                     double *rhsp, double *dstg, double *mstg);
 int chs_io_sync(chs_io *m);
 
+/* CFL_io (CFL.f90:59-107) on the current Q_io: max over cells of the convective rate and the
+ * viscous rate 2 CVISC (DXQI + DZQI + DYFI^2) Viscousity / DENSITY, times dt (CFLMM_io * dt;
+ * dt = 1 gives CFLMM_io itself, what SOLVE's dt rule takes, SOLVE.f90:79-85, :177-178).
+ * chs_cfl_io reads the context's DENSITY0 and viscosity (Viscousity for the Crank-Nicolson
+ * scheme, else Viscousity0), as a flow-only run holds them; chs_heated_cfl the energy
+ * context's current DENSITY and Viscousity, as a heated run calls it at the start of a step.
+ * Synchronous. */
+int chs_cfl_io(chs_io *m, double dt, double *cfl);
+int chs_heated_cfl(chs_eng *e, chs_io *m, double dt, double *cfl);
+
 /* One stage of the heated channel as SOLVE runs it (SOLVE.f90:119-133): the energy substep
  * with the momentum context's current G, then its DENSITY0 / Viscousity0 (and, for the
  * Crank-Nicolson scheme, Viscousity) into the momentum context, the staggered properties

@@ -343,6 +343,20 @@ def io_initial_field(mesh, seed, rho_amp=0.3, mu_amp=0.3):
     return dict(G=g, G0=g0, PR=pr, D0=d0, M0=m0, EXPLT0=explt0)
 
 
+def cfl_io(mesh, q, dens, visc, cvisc):
+    """CFL_io (CFL.f90:59-107) restated: max over cells of the convective rate
+    (|u_i + u_i+1| DXI + |v_J + v_J+1| DYFI + |w_k + w_k+1| DZI) / 2 and the viscous rate
+    2 CVISC (DXQI + DZQI + DYFI^2) Viscousity / DENSITY, in the reference's operation order.
+    q [3, N, M+2, L], dens, visc [N, M+2, L].  Returns CFLMM_io."""
+    M = mesh.M
+    dyfi = np.asarray(mesh.DYFI)[None, :, None]
+    u, v, w = q[0][:, 1:M + 1], q[1], q[2][:, 1:M + 1]
+    qcf = (np.abs((u + np.roll(u, -1, axis=2)) * mesh.DXI) + np.abs((v[:, 1:M + 1] + v[:, 2:M + 2]) * dyfi)
+           + np.abs((w + np.roll(w, -1, axis=0)) * mesh.DZI)) * 0.5
+    vcf = 2.0 * cvisc * (mesh.DXQI + mesh.DZQI + dyfi * dyfi) * visc[:, 1:M + 1] / dens[:, 1:M + 1]
+    return float(np.max(np.maximum(qcf, vcf)))
+
+
 class OracleIO:
     """SOLVERRK3_MOM_io restated (or_io_substep) with the staggered properties of SOLVE
     (or_io_props).  Fields are C-order; ``f`` holds G, G0, Q, DSTG, MSTG [3, N, M+2, L], PR,

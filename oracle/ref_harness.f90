@@ -42,6 +42,7 @@
 !   ioprops NS              DENSITY_TIME_DERIVATION(NS), DENSITY_Staggered, MU_Staggered
 !                           and, with iVisScheme = 1, DENSITY_IMPLICIT_add (SOLVE.f90:122-127)
 !   loadvisc FILE           Viscousity (the current viscosity) from a global L x (M+2) x N file
+!   cflio FILE              VELOCITY_CALC_io and CFL_io; rank 0 writes CFLMM_io to FILE
 !   iocn NS DT FILE         DT = given; the momentum part of SOLVERRK3_MOM_io(NS) with the
 !                           Crank-Nicolson viscous scheme (SOLVERRK3_MOM_IO.f90:25-41, :61-74):
 !                           each rank writes FILE.<rank>: G_io and DRHOI_STG rows 0..N2DO+1
@@ -248,6 +249,16 @@ PROGRAM ref_harness
         CASE ('loadvisc')
             READ(line, *) cmd, fname
             CALL H_LOADVISC(TRIM(fname))
+        CASE ('cflio')
+            READ(line, *) cmd, fname
+            CALL VELOCITY_CALC_io
+            CALL CFL_io
+            IF (MYID == 0) THEN
+                OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+                     STATUS = 'replace', ACTION = 'write')
+                WRITE(80) CFLMM_io
+                CLOSE(80)
+            END IF
         CASE ('iocn')
             READ(line, *) cmd, ns, dtin, fname
             DT = dtin

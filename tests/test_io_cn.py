@@ -237,3 +237,72 @@ def test_gpu_heated_stage_cn_matches_oracle(pkg):
     assert rel(f["PR"], oi.f["PR"]) < 1e-9
     ge.close()
     gi.close()
+
+
+def test_oracle_cfl_io_matches_reference(golden):
+    """CFL_io on the stage-1 starting velocity Q = G / D_STG, DENSITY = DENSITY0, Viscousity the
+    noisy current viscosity (the ``cflio`` fixture); both terms matter on these meshes."""
+    for c in cases(golden):
+        g, tag = c["g"], c["tag"]
+        mesh = setup.Mesh(c["L"], c["M"], c["N"], STR2=c["STR2"])
+        o = make_oracle(c)
+        o.set_state(dict(c["init"], M1=c["m1"]))
+        o.props(1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = o.f["G"] / o.f["DSTG"]
+        q[:, :, 0] = 0.0
+        q[:, :, c["M"] + 1] = 0.0
+        q[1, :, 1] = 0.0
+        v = oracle.cfl_io(mesh, q, c["init"]["D0"], c["m1"], 1.0 / c["REN"])
+        assert abs(v - g[f"{tag}_cn_CFLMM"][0]) <= 1e-15 * abs(v), (v, g[f"{tag}_cn_CFLMM"][0])
+
+
+def test_next_dt_rule(pkg):
+    """BCAST_COMM_STEP's rule (SOLVE.f90:177-178): each bound wins in its regime."""
+    assert pkg.next_dt(1e-3, 2e-3, 1.0, 10.0, 1e-6) == 2e-3            # DT0
+    assert pkg.next_dt(1e-4, 2e-3, 1.0, 10.0, 1e-6) == 5e-4            # five times the last step
+    assert pkg.next_dt(1e-3, 2e-3, 1.0, 2000.0, 1e-6) == 1.0 / 2000.0  # CFLGV / CFLMM
+    assert pkg.next_dt(1e-3, 2e-3, 1.0, 1e9, 1e-6) == 1e-6             # DTMIN
+
+
+@pytest.mark.gpu
+def test_gpu_cfl_io_matches_reference(pkg, golden):
+    for c in cases(golden):
+        g, tag = c["g"], c["tag"]
+        print(f"cfl_io {tag}", flush=True)
+        co = pkg.channel_setup(c["L"], c["M"], c["N"], STR2=c["STR2"])
+        ch = pkg.ChannelIO(co, c["REN"], gravity=c["grav"], f_a=c["f_a"], flow_driven=c["fd"], cf_given=8.18e-3,
+                           vis_scheme=1)
+        ch.upload(**c["init"], M1=c["m1"])
+        ch.ioprops(1)
+        ref = g[f"{tag}_cn_CFLMM"][0]
+        assert abs(ch.cfl(1.0) - ref) <= 1e-14 * ref
+        assert abs(ch.cfl(2.5e-4) - ref * 2.5e-4) <= 1e-14 * ref * 2.5e-4
+        ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_heated_cfl_matches_oracle(pkg):
+    """chs_heated_cfl reads the energy context's current DENSITY and Viscousity."""
+    from importlib import import_module
+    thermo = import_module(pkg.__name__ + ".thermo")
+    t = thermo.table_setup(thermo.synthetic_table(), 665.0)
+    L, M, N, STR2, REN, dt = 32, 24, 16, 1.9, 2800.0, 5e-4
+    mesh = setup.Mesh(L, M, N, STR2=STR2)
+    dh, _ = oracle.eng_initial_field(mesh, t, 85)
+    st = oracle.io_initial_field(mesh, 86)
+    cthecd = 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"])
+    dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+    co = pkg.channel_setup(L, M, N, STR2=STR2)
+    ge = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, (0.05, -0.05))
+    ge.upload(dh, st["G"])
+    fe0 = ge.download()
+    gi = pkg.ChannelIO(co, REN, flow_driven=1, vis_scheme=1)
+    gi.upload(G=st["G"], G0=st["G0"], PR=st["PR"], D0=fe0["D"], M0=fe0["M"], EXPLT0=st["EXPLT0"])
+    for ns in (1, 2, 3):
+        pkg.heated_substep(ge, gi, ns, dt)
+    f, fe = gi.download(), ge.download()
+    ref = oracle.cfl_io(mesh, f["Q"], fe["D"], fe["M"], 1.0 / REN)
+    assert abs(pkg.heated_cfl(ge, gi, 1.0) - ref) <= 1e-14 * ref
+    ge.close()
+    gi.close()
