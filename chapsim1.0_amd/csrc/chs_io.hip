@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <string>
 #include <utility>
 #include <vector>
@@ -593,48 +594,38 @@ __global__ __launch_bounds__(128) void k_io_cn_x(IoDev d, const double *__restri
     else io_cn_x_line<2>(d, l, 0.0, facoe);
 }
 
-// Cyclic tridiagonal solve by parallel cyclic reduction in LDS: the threads of a block hold
-// one equation each (element e of a line of n, a power of two >= 4; slot(e) its LDS index).
-// Each step eliminates the neighbours at distance s (s = 1, 2, .., n/4); then every equation
-// couples x_e with x_(e + n/2) only and each thread solves its 2 x 2 pair.  Strictly diagonally
-// dominant rows (the Crank-Nicolson factors) keep that property through the steps.  Every
-// thread of the block must call it.  Returns x_e.
-template <class Slot>
-__device__ __forceinline__ double pcr_cyc(int n, int e, const Slot &slot, double a, double b, double c, double r,
-                                          double *sa, double *sb, double *sc, double *sr)
+// Block maximum of a non-negative value: a wave maximum by shuffles, one 64-bit atomic per
+// wave on an LDS word (for non-negative doubles the bit order is the value order).  Every
+// thread of the block must call it; blockDim.x must be a multiple of 64.
+__device__ __forceinline__ double block_max_nonneg(double v, unsigned long long *word)
 {
-#pragma clang fp contract(off)
-    const int me = slot(e);
-    for (int s = 1; s < n / 2; s <<= 1) {
-        sa[me] = a;
-        sb[me] = 1.0 / b;
-        sc[me] = c;
-        sr[me] = r;
-        __syncthreads();
-        const int m = slot((e - s) & (n - 1)), p = slot((e + s) & (n - 1));
-        const double k1 = a * sb[m], k2 = c * sb[p];
-        const double am = sa[m], cm = sc[m], rm = sr[m], ap = sa[p], cp = sc[p], rp = sr[p];
-        __syncthreads();
-        a = -am * k1;
-        c = -cp * k2;
-        b = b - cm * k1 - ap * k2;
-        r = r - rm * k1 - rp * k2;
-    }
-    const double ei = a + c;
-    sa[me] = ei;
-    sb[me] = b;
-    sr[me] = r;
+    if (threadIdx.x == 0) *word = 0ull;
     __syncthreads();
-    const int j = slot((e + n / 2) & (n - 1));
-    const double ej = sa[j], bj = sb[j], rj = sr[j];
-    return (r * bj - ei * rj) / (b * bj - ei * ej);
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(word, (unsigned long long)__double_as_longlong(v));
+    __syncthreads();
+    return __longlong_as_double((long long)*word);
+}
+
+// The Crank-Nicolson factors are strictly diagonally dominant with a small ratio
+// rho = max (|a| + |c|) / |b| (about 1e-4 in x and z and 2e-3 at the walls in y on the
+// channel meshes).  Starting from x0 = r / b, K Jacobi sweeps x <- (r - a x_- - c x_+) / b leave
+// an error of at most rho^(K+1) max|x| (|x0 - x| <= rho |x|, each sweep contracts by rho); K is
+// the least count with rho^(K+1) <= 1e-17, e.g. 3 sweeps at rho = 1e-4.  The line kernels below
+// solve this way and hand a block whose rho exceeds 1/4 to the thread-per-line recurrences
+// (io_cn_x_line, io_cn_z_line, io_cn_y_line).
+__device__ __forceinline__ int jacobi_sweeps(double rho)
+{
+    int K = 0;
+    for (double q = rho; q > 1e-17 && K < 60; q *= rho) ++K;
+    return K;
 }
 
 // MOMFA1_X_io for L a power of two (4 <= L <= 512): T = blockDim.x / L x-lines per block, one
 // thread per point, coalesced; the coefficients are built from the staged property rows as
-// io_cn_x_line builds them, then pcr_cyc.  Lines past the end solve a dummy identity row.
+// io_cn_x_line builds them.  Lines past the end carry a dummy identity row.
 template <int C>
-__device__ __forceinline__ void io_cn_x_pcr(const IoDev &d, double dp, double facoe, double *lds)
+__device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double facoe, double *lds)
 {
 #pragma clang fp contract(off)
     const int L = d.L, M = d.M, j0 = C == 1 ? 2 : 1, nj = M - j0 + 1, nl = nj * d.N;
@@ -644,8 +635,9 @@ __device__ __forceinline__ void io_cn_x_pcr(const IoDev &d, double dp, double fa
     const int J = j0 + line % nj, k = line / nj;
     const size_t hs = hsz(d), cs = csz(d);
     const int S = blockDim.x;
-    double *s0 = lds, *s1 = lds + S, *s2 = lds + 2 * S, *s3 = lds + 3 * S;
-    auto slot = [&](int ee) { return lt * L + ee; };
+    double *s0 = lds, *s1 = lds + S;
+    unsigned long long *word = reinterpret_cast<unsigned long long *>(lds + 2 * S);
+    const int sm = lt * L + ((e - 1) & (L - 1)), sp = lt * L + ((e + 1) & (L - 1));
     double f0 = 0.0, f1 = 1.0, r = 0.0;
     if (valid) {
         const size_t o = hx(d, e, J, k);
@@ -657,83 +649,46 @@ __device__ __forceinline__ void io_cn_x_pcr(const IoDev &d, double dp, double fa
     s0[threadIdx.x] = f0;
     s1[threadIdx.x] = f1;
     __syncthreads();
-    const int em = slot((e - 1) & (L - 1)), ep = slot((e + 1) & (L - 1));
     double amu, apu;
     if constexpr (C == 0) {
         const double coe1 = facoe * 4.0 / 3.0 * d.dxqi;
-        amu = -coe1 * s0[em];
+        amu = -coe1 * s0[sm];
         apu = -coe1 * f0;
     } else {
         const double coe2 = facoe * d.dxqi;
         amu = -coe2 * f0;
-        apu = -coe2 * s0[ep];
+        apu = -coe2 * s0[sp];
     }
-    double a = amu / s1[em], c = apu / s1[ep], b = 1.0 - (amu + apu) / f1;
-    if (!valid) a = c = 0.0;
-    __syncthreads();
-    const double x = pcr_cyc(L, e, slot, a, b, c, r, s0, s1, s2, s3);
+    double a = amu / s1[sm], c = apu / s1[sp], b = 1.0 - (amu + apu) / f1;
+    if (!valid) {
+        a = c = 0.0;
+        b = 1.0;
+    }
+    const double ib = 1.0 / b;
+    const double rho = block_max_nonneg((fabs(a) + fabs(c)) * fabs(ib), word);
+    if (!(rho <= 0.25)) {   // the same branch for the whole block
+        if (valid && e == 0) io_cn_x_line<C>(d, line, dp, facoe);
+        return;
+    }
+    const int K = jacobi_sweeps(rho);
+    double x = r * ib;
+    for (int it = 0; it < K; ++it) {
+        s0[threadIdx.x] = x;
+        __syncthreads();
+        const double xm = s0[sm], xp = s0[sp];
+        __syncthreads();
+        x = (r - a * xm - c * xp) * ib;
+    }
     if (valid) d.h[C * cs + cx(d, e, J, k)] = x;
 }
 
-__global__ __launch_bounds__(512) void k_io_cn_x_pcr(IoDev d, const double *__restrict__ dpg_dev, double dpg,
+__global__ __launch_bounds__(512) void k_io_cn_x_jac(IoDev d, const double *__restrict__ dpg_dev, double dpg,
                                                      double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_x_pcr<0>(d, dpg_dev ? dpg_dev[0] : dpg, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_x_pcr<1>(d, 0.0, facoe, lds);
-    else io_cn_x_pcr<2>(d, 0.0, facoe, lds);
-}
-
-// MOMFA2_Z_io for N a power of two (4 <= N <= 1024): a block holds T = blockDim.x / N
-// consecutive x-columns of one row J, thread (lt, k) at lt + T k so a wave reads runs of
-// consecutive x; coefficients from the staged property columns, then pcr_cyc.
-template <int C>
-__device__ __forceinline__ void io_cn_z_pcr(const IoDev &d, double facoe, double *lds)
-{
-#pragma clang fp contract(off)
-    const int L = d.L, M = d.M, N = d.N, j0 = C == 1 ? 2 : 1;
-    const int T = blockDim.x / N, lt = threadIdx.x % T, e = threadIdx.x / T;
-    const int ng = (L + T - 1) / T;
-    const int i = (blockIdx.x % ng) * T + lt, J = j0 + blockIdx.x / ng;
-    const bool valid = i < L && J <= M;
-    const size_t hs = hsz(d), cs = csz(d);
-    const int S = blockDim.x;
-    double *s0 = lds, *s1 = lds + S, *s2 = lds + 2 * S, *s3 = lds + 3 * S;
-    auto slot = [&](int ee) { return ee * T + lt; };
-    double f0 = 0.0, f1 = 1.0, r = 0.0;
-    if (valid) {
-        const size_t o = hx(d, i, J, e);
-        f0 = C == 2 ? d.m1[o] : d.mstg[(C + 1) * hs + o];
-        f1 = d.dstg[C * hs + o];
-        r = d.h[C * cs + cx(d, i, J, e)];
-    }
-    s0[threadIdx.x] = f0;
-    s1[threadIdx.x] = f1;
-    __syncthreads();
-    const int km = slot((e - 1) & (N - 1)), kp = slot((e + 1) & (N - 1));
-    double amu, apu;
-    if constexpr (C == 2) {
-        const double coe1 = facoe * 4.0 / 3.0 * d.dzqi;
-        amu = -coe1 * s0[km];
-        apu = -coe1 * f0;
-    } else {
-        const double coe2 = facoe * d.dzqi;
-        amu = -coe2 * f0;
-        apu = -coe2 * s0[kp];
-    }
-    double a = amu / s1[km], c = apu / s1[kp], b = 1.0 - (amu + apu) / f1;
-    if (!valid) a = c = 0.0;
-    __syncthreads();
-    const double x = pcr_cyc(N, e, slot, a, b, c, r, s0, s1, s2, s3);
-    if (valid) d.h[C * cs + cx(d, i, J, e)] = x;
-}
-
-__global__ __launch_bounds__(1024) void k_io_cn_z_pcr(IoDev d, double facoe)
-{
-    extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_z_pcr<0>(d, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_z_pcr<1>(d, facoe, lds);
-    else io_cn_z_pcr<2>(d, facoe, lds);
+    if (blockIdx.y == 0) io_cn_x_jac<0>(d, dpg_dev ? dpg_dev[0] : dpg, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_x_jac<1>(d, 0.0, facoe, lds);
+    else io_cn_x_jac<2>(d, 0.0, facoe, lds);
 }
 
 // MOMFA2_Z_io (MOMFA_io.f90:179-265): cyclic z-lines (RCCI2 = RNDI2 = 1 on the channel).
@@ -846,6 +801,240 @@ __global__ __launch_bounds__(128) void k_io_cn_y(IoDev d, const double *__restri
     if (blockIdx.y == 0) io_cn_y_line<0>(d, gs, l, facoe);
     else if (blockIdx.y == 1) io_cn_y_line<1>(d, gs, l, facoe);
     else io_cn_y_line<2>(d, gs, l, facoe);
+}
+
+// MOMFA2_Z_io with P consecutive z-points per thread (N a multiple of P): a block holds T
+// x-columns of one row J, thread (lt, chunk) at lt + T chunk, so a wave reads runs of T
+// consecutive x; only the chunk edges go through LDS (coefficient neighbours, then the Jacobi
+// sweeps' neighbours).
+template <int C, int P>
+__device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double *lds)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M, N = d.N, j0 = C == 1 ? 2 : 1;
+    const int NT = N / P, T = blockDim.x / NT, lt = threadIdx.x % T, tt = threadIdx.x / T;
+    const int ng = (L + T - 1) / T;
+    const int i = (blockIdx.x % ng) * T + lt, J = j0 + blockIdx.x / ng;
+    const bool valid = i < L && J <= M;
+    const size_t hs = hsz(d), cs = csz(d);
+    const int S = blockDim.x;
+    double *e0 = lds, *e1 = lds + S, *e2 = lds + 2 * S, *e3 = lds + 3 * S;
+    unsigned long long *word = reinterpret_cast<unsigned long long *>(lds + 4 * S);
+    const int k0 = tt * P;
+    const int sl = ((tt + NT - 1) % NT) * T + lt, sr = ((tt + 1) % NT) * T + lt;   // neighbour chunks
+    double f0[P], f1[P], r[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        f0[j] = 0.0;
+        f1[j] = 1.0;
+        r[j] = 0.0;
+        if (valid) {
+            const size_t o = hx(d, i, J, k0 + j);
+            f0[j] = C == 2 ? d.m1[o] : d.mstg[(C + 1) * hs + o];
+            f1[j] = d.dstg[C * hs + o];
+            r[j] = d.h[C * cs + cx(d, i, J, k0 + j)];
+        }
+    }
+    e0[threadIdx.x] = f0[0];
+    e1[threadIdx.x] = f0[P - 1];
+    e2[threadIdx.x] = f1[0];
+    e3[threadIdx.x] = f1[P - 1];
+    __syncthreads();
+    const double f0m = e1[sl], f1m = e3[sl], f0p = e0[sr], f1p = e2[sr];
+    double a[P], c[P], ib[P], rho = 0.0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const double vm = j > 0 ? f0[j - 1] : f0m, vp = j < P - 1 ? f0[j + 1] : f0p;
+        const double dm = j > 0 ? f1[j - 1] : f1m, dq = j < P - 1 ? f1[j + 1] : f1p;
+        double amu, apu;
+        if constexpr (C == 2) {
+            const double coe1 = facoe * 4.0 / 3.0 * d.dzqi;
+            amu = -coe1 * vm;
+            apu = -coe1 * f0[j];
+        } else {
+            const double coe2 = facoe * d.dzqi;
+            amu = -coe2 * f0[j];
+            apu = -coe2 * vp;
+        }
+        a[j] = amu / dm;
+        c[j] = apu / dq;
+        double b = 1.0 - (amu + apu) / f1[j];
+        if (!valid) {
+            a[j] = c[j] = 0.0;
+            b = 1.0;
+        }
+        ib[j] = 1.0 / b;
+        rho = fmax(rho, (fabs(a[j]) + fabs(c[j])) * fabs(ib[j]));
+    }
+    rho = block_max_nonneg(rho, word);   // also orders the edge reads above before the sweeps
+    if (!(rho <= 0.25)) {
+        if (valid && tt == 0) io_cn_z_line<C>(d, (J - j0) * L + i, facoe);
+        return;
+    }
+    const int K = jacobi_sweeps(rho);
+    double x[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) x[j] = r[j] * ib[j];
+    for (int it = 0; it < K; ++it) {
+        e0[threadIdx.x] = x[0];
+        e1[threadIdx.x] = x[P - 1];
+        __syncthreads();
+        const double xl = e1[sl], xr = e0[sr];
+        __syncthreads();
+        double xn[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const double xm = j > 0 ? x[j - 1] : xl, xp = j < P - 1 ? x[j + 1] : xr;
+            xn[j] = (r[j] - a[j] * xm - c[j] * xp) * ib[j];
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) x[j] = xn[j];
+    }
+    if (valid) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) d.h[C * cs + cx(d, i, J, k0 + j)] = x[j];
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_io_cn_z_jac(IoDev d, double facoe)
+{
+    extern __shared__ double lds[];
+    if (blockIdx.y == 0) io_cn_z_jac<0, 4>(d, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_z_jac<1, 4>(d, facoe, lds);
+    else io_cn_z_jac<2, 4>(d, facoe, lds);
+}
+
+// MOMFA3_Y_io + the G update with P consecutive rows per thread (NT = ceil(M / P) chunks per
+// column, rows past M are identity rows): a block holds T x-columns of one z-plane, thread
+// (lt, chunk) at lt + T chunk; coefficients as io_cn_y_line builds them; writes every row of
+// the new G of its columns.
+template <int C, int P>
+__device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__restrict__ gs, double facoe, double *lds)
+{
+#pragma clang fp contract(off)
+    const int L = d.L, M = d.M;
+    const int NT = (M + P - 1) / P, T = blockDim.x / NT, lt = threadIdx.x % T, tt = threadIdx.x / T;
+    const int ng = (L + T - 1) / T;
+    const int i = (blockIdx.x % ng) * T + lt, k = blockIdx.x / ng;
+    const bool valid = i < L;
+    const size_t hs = hsz(d), cs = csz(d);
+    const int S = blockDim.x;
+    double *e0 = lds, *e1 = lds + S, *e2 = lds + 2 * S, *e3 = lds + 3 * S;
+    unsigned long long *word = reinterpret_cast<unsigned long long *>(lds + 4 * S);
+    const int J0 = tt * P + 1;
+    const int sl = tt > 0 ? (tt - 1) * T + lt : -1, sr = tt < NT - 1 ? (tt + 1) * T + lt : -1;
+    const double *DS = d.dstg + C * hs, *FM = C == 1 ? d.m1 : d.mstg + C * hs;
+    double f0[P], f1[P], r[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int J = J0 + j;
+        f0[j] = 0.0;
+        f1[j] = 1.0;
+        r[j] = 0.0;
+        if (valid && J <= M) {
+            const size_t o = hx(d, i, J, k);
+            f0[j] = FM[o];
+            f1[j] = DS[o];
+            r[j] = (C == 1 && J == 1) ? 0.0 : d.h[C * cs + cx(d, i, J, k)];
+        }
+    }
+    e0[threadIdx.x] = f0[0];
+    e1[threadIdx.x] = f0[P - 1];
+    e2[threadIdx.x] = f1[0];
+    e3[threadIdx.x] = f1[P - 1];
+    __syncthreads();
+    const double f0m = sl >= 0 ? e1[sl] : 0.0, f1m = sl >= 0 ? e3[sl] : 1.0;
+    const double f0p = sr >= 0 ? e0[sr] : 0.0, f1p = sr >= 0 ? e2[sr] : 1.0;
+    const double *am = d.amvr + C * (M + 2), *ap = d.apvr + C * (M + 2);
+    double a[P], c[P], ib[P], rho = 0.0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int J = J0 + j;
+        double aa = 0.0, cc = 0.0, b = 1.0;
+        if (valid && J <= M) {
+            const double fm = J > 1 ? (j > 0 ? f0[j - 1] : f0m) : 0.0;
+            const double fp = J < M ? (j < P - 1 ? f0[j + 1] : f0p) : 0.0;
+            const double dm = J > 1 ? (j > 0 ? f1[j - 1] : f1m) : 1.0;
+            const double dq = J < M ? (j < P - 1 ? f1[j + 1] : f1p) : 1.0;
+            double amu, apu;
+            if constexpr (C != 1) {
+                const double coe2 = facoe;
+                if (J == 1) {
+                    const double muqt = (d.m1[hx(d, i, 1, k)] + d.m1[hx(d, i, 0, k)]) * 0.5;
+                    amu = -coe2 * d.amvr1 * muqt;
+                    apu = -coe2 * ap[J] * fp;
+                    cc = apu / dq;
+                } else if (J == M) {
+                    const double muqt = (d.m1[hx(d, i, M, k)] + d.m1[hx(d, i, M + 1, k)]) * 0.5;
+                    amu = -coe2 * am[J] * f0[j];
+                    apu = -coe2 * d.apvrn * muqt;
+                    aa = amu / dm;
+                } else {
+                    amu = -coe2 * am[J] * f0[j];
+                    apu = -coe2 * ap[J] * fp;
+                    aa = amu / dm;
+                    cc = apu / dq;
+                }
+            } else {
+                const double coe1 = facoe * 4.0 / 3.0;
+                amu = -coe1 * am[J] * fm;
+                apu = -coe1 * ap[J] * f0[j];
+                aa = J == 1 ? 0.0 : amu / dm;
+                cc = apu / (J == 1 ? dq : f1[j]);
+            }
+            b = 1.0 - (amu + apu) / f1[j];
+        }
+        a[j] = aa;
+        c[j] = cc;
+        ib[j] = 1.0 / b;
+        rho = fmax(rho, (fabs(aa) + fabs(cc)) * fabs(ib[j]));
+    }
+    rho = block_max_nonneg(rho, word);
+    if (!(rho <= 0.25)) {
+        if (valid && tt == 0) io_cn_y_line<C>(d, gs, k * L + i, facoe);
+        return;
+    }
+    const int K = jacobi_sweeps(rho);
+    double x[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) x[j] = r[j] * ib[j];
+    for (int it = 0; it < K; ++it) {
+        e0[threadIdx.x] = x[0];
+        e1[threadIdx.x] = x[P - 1];
+        __syncthreads();
+        const double xl = sl >= 0 ? e1[sl] : 0.0, xr = sr >= 0 ? e0[sr] : 0.0;
+        __syncthreads();
+        double xn[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const double xm = j > 0 ? x[j - 1] : xl, xp = j < P - 1 ? x[j + 1] : xr;
+            xn[j] = (r[j] - a[j] * xm - c[j] * xp) * ib[j];
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) x[j] = xn[j];
+    }
+    if (valid) {
+        double *G = d.g + C * hs;
+        const double *G0 = gs + C * hs;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int J = J0 + j;
+            if (J <= M) {
+                const size_t o = hx(d, i, J, k);
+                G[o] = (C == 1 && J == 1) ? 0.0 : x[j] + G0[o];
+            }
+        }
+        if (tt == 0) G[hx(d, i, 0, k)] = 0.0;
+        if (tt == NT - 1) G[hx(d, i, M + 1, k)] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_io_cn_y_jac(IoDev d, const double *__restrict__ gs, double facoe)
+{
+    extern __shared__ double lds[];
+    if (blockIdx.y == 0) io_cn_y_jac<0, 4>(d, gs, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_y_jac<1, 4>(d, gs, facoe, lds);
+    else io_cn_y_jac<2, 4>(d, gs, facoe, lds);
 }
 
 // DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
@@ -1193,21 +1382,33 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
         const int L = m->L, M = m->M, N = m->N;
         if ((L & (L - 1)) == 0 && L >= 4 && L <= 512) {
             const int T = L >= 256 ? 1 : 256 / L;   // x-lines per block
-            hipLaunchKernelGGL(k_io_cn_x_pcr, dim3((M * N + T - 1) / T, 3), dim3(T * L), 4 * sizeof(double) * T * L,
-                               m->stream, d, dpg_dev, dpg, facoe);
+            hipLaunchKernelGGL(k_io_cn_x_jac, dim3((M * N + T - 1) / T, 3), dim3(T * L),
+                               sizeof(double) * (2 * T * L + 1), m->stream, d, dpg_dev, dpg, facoe);
         } else {
             hipLaunchKernelGGL(k_io_cn_x, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg,
                                facoe);
         }
-        if ((N & (N - 1)) == 0 && N >= 4 && N <= 1024) {
-            const int T = std::max(1, 1024 / N);   // x-columns per block
-            const int ng = (L + T - 1) / T;
-            hipLaunchKernelGGL(k_io_cn_z_pcr, dim3(ng * M, 3), dim3(T * N), 4 * sizeof(double) * T * N, m->stream, d,
-                               facoe);
+        constexpr int P = 4;   // line points per thread of the z and y kernels
+        const int NTz = N / P;   // z: block = Tz x-columns x NTz chunks, whole waves
+        int Tz = (N % P == 0 && NTz >= 2 && NTz <= 1024) ? std::min(64, 1024 / NTz) : 0;
+        if (Tz > 0) Tz -= Tz % (64 / std::gcd(NTz, 64));
+        if (Tz > 0) {
+            const int ng = (L + Tz - 1) / Tz;
+            hipLaunchKernelGGL(k_io_cn_z_jac, dim3(ng * M, 3), dim3(Tz * NTz), sizeof(double) * (4 * Tz * NTz + 1),
+                               m->stream, d, facoe);
         } else {
             hipLaunchKernelGGL(k_io_cn_z, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
         }
-        hipLaunchKernelGGL(k_io_cn_y, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
+        const int NTy = (M + P - 1) / P;   // y: block = Ty x-columns x NTy chunks, whole waves
+        int Ty = NTy <= 1024 ? std::min(64, 1024 / NTy) : 0;
+        if (Ty > 0) Ty -= Ty % (64 / std::gcd(NTy, 64));
+        if (Ty > 0) {
+            const int ng = (L + Ty - 1) / Ty;
+            hipLaunchKernelGGL(k_io_cn_y_jac, dim3(ng * N, 3), dim3(Ty * NTy), sizeof(double) * (4 * Ty * NTy + 1),
+                               m->stream, d, gs, facoe);
+        } else {
+            hipLaunchKernelGGL(k_io_cn_y, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
+        }
     } else {
         hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
     }
