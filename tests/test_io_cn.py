@@ -306,3 +306,29 @@ def test_gpu_heated_cfl_matches_oracle(pkg):
     assert abs(pkg.heated_cfl(ge, gi, 1.0) - ref) <= 1e-14 * ref
     ge.close()
     gi.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("REN,dt", [(1.0, 0.5), (50.0, 0.02)])
+def test_gpu_cn_line_solve_regimes(pkg, REN, dt):
+    """The line kernels solve by Jacobi sweeps while a block's diagonal-dominance ratio rho is
+    at most 1/4 and hand the block to the thread-per-line recurrences above it.  REN = 1,
+    dt = 0.5: rho > 1/4 in x, z and y (the recurrences); REN = 50, dt = 0.02: rho ~ 1e-3 in x and
+    z and ~ 4e-2 at the walls in y (a dozen sweeps).  Momentum part against the oracle."""
+    L, M, N = 32, 16, 32
+    print(f"io cn line regimes REN {REN} dt {dt}", flush=True)
+    mesh = setup.Mesh(L, M, N, STR2=2.0)
+    st = oracle.io_initial_field(mesh, 95)
+    m1 = cn_viscosity(st, 95, False)
+    o = oracle.OracleIO(mesh, REN, flow_driven=1, vis_scheme=1)
+    o.set_state(dict(st, M1=m1))
+    o.props(1)
+    o.momentum_cn(1, dt)
+    ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), REN, flow_driven=1, vis_scheme=1)
+    ch.upload(**st, M1=m1)
+    ch.ioprops(1)
+    ch.momentum(1, dt)
+    f = ch.download()
+    for c in range(3):
+        assert rel(f["G"][c], o.f["G"][c]) < TOL, (c, rel(f["G"][c], o.f["G"][c]))
+    ch.close()
