@@ -88,12 +88,21 @@ struct Sub {
 // own u, v, w and p of plane k, and the plane k+1 values read for the z-differences) are
 // carried in registers, so each plane costs 15 loads of plane k, two of plane k-1, 5 of plane
 // k+1 and the 3 history values.
-template <typename W, int BY, int BXW, bool FENCE>
+// FX: the block is one whole x-line row (BY = 1, L = 64 BXW) and each plane's three RHS rows
+// go through LDS, where the circulant x-inverse gx (half-width Dx) of MOMFA1_X_tg is applied
+// before the store, so no separate x-solve pass is needed.  The mass-flux correction -DPGRNS,
+// which MOMFA_tg receives already subtracted (RHS_CvLpGpS_tg.f90:179-201), is then subtracted
+// after the z-solve instead: the cyclic x and z factors map a constant to itself, so only the
+// rounding changes.  The partial sums use the unsolved u-row, as the reference's.
+template <typename W, int BY, int BXW, bool FENCE, bool FX = false>
 __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
                                                       const double *__restrict__ pr,
                                                       W *__restrict__ h0, W *__restrict__ rhs,
-                                                      double *__restrict__ partial, int KC, int nxcd)
+                                                      double *__restrict__ partial, int KC, int nxcd,
+                                                      const W *__restrict__ gx = nullptr, int Dx = 0)
 {
+    static_assert(!FX || BY == 1, "FX: one row per block");
+    __shared__ W lxs[FX ? 2 * 3 * 64 * BXW : 1];   // two planes' u, v, w rows (double buffer)
     const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
     const int tx = tile % gridDim.x, ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
     const int i = tx * (64 * BXW) + threadIdx.x;
@@ -171,7 +180,8 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                 hkw[ic] = conv;
                 const W pgm = (W)((p0 - pim) * g.dxi * coep);
                 const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
-                rk[ic] = r;
+                if constexpr (FX) lxs[(k & 1) * 3 * 64 * BXW + ic] = r;
+                else rk[ic] = r;
                 rsum += (double)r;
             }
             // ---- v (CONVECTION_Y.f90:37-84), rows J >= 2 (computed on every row so that its
@@ -190,7 +200,9 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                 const W pgm = (W)((p0 - pjm) * dyc * coep);
                 if (J >= 2) {
                     hkw[cs + ic] = conv;
-                    rk[cs + ic] = (rhsc + (W)(coev * lap)) * dt - pgm;
+                    const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
+                    if constexpr (FX) lxs[(k & 1) * 3 * 64 * BXW + 64 * BXW + ic] = r;
+                    else rk[cs + ic] = r;
                 }
             }
             // ---- w (CONVECTION_Z.f90:33-69) ----
@@ -206,7 +218,29 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                                    (wjp * apw + w_c * acw + wjm_c * amw);
                 hkw[2 * cs + ic] = conv;
                 const W pgm = (W)((p0 - p_m) * g.dzi * coep);
-                rk[2 * cs + ic] = (rhsc + (W)(coev * lap)) * dt - pgm;
+                const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
+                if constexpr (FX) lxs[(k & 1) * 3 * 64 * BXW + 2 * 64 * BXW + ic] = r;
+                else rk[2 * cs + ic] = r;
+            }
+            if constexpr (FX) {
+                // the whole block runs this loop (every thread holds a point of the row); the
+                // next plane writes the other buffer, so one barrier per plane suffices
+                __syncthreads();
+                const W *lb = lxs + (k & 1) * 3 * 64 * BXW;
+                constexpr int LX = 64 * BXW;
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    if (cc == 1 && J == 1) continue;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
+                    const W *ln = lb + cc * LX;
+                    W acc = gx[0] * ln[ic];
+                    for (int d = 1; d <= Dx; ++d) {
+                        int a = (int)ic + d, b = (int)ic - d;
+                        if (a >= LX) a -= LX;
+                        if (b < 0) b += LX;
+                        acc += gx[d] * (ln[a] + ln[b]);
+                    }
+                    rk[cc * cs + ic] = acc;
+                }
             }
             u_m = u_c; v_m = v_c; w_m = w_c; p_m = p0;
             u_c = u_p; v_c = v_p; w_c = w_p; wim_c = wim_p; wjm_c = wjm_p;
@@ -525,7 +559,7 @@ __global__ __launch_bounds__(256) void k_cn_zy(Geo g, CnArgs a, const W *__restr
 // z: one thread per (i, J) marches KZ consecutive planes k; circulant inverse gz (cyclic k).
 template <typename W, int D>
 __global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__ gz, const W *__restrict__ rin,
-                                                    W *__restrict__ rout, int KZ)
+                                                    W *__restrict__ rout, int KZ, const double *__restrict__ dpg)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
@@ -537,6 +571,8 @@ __global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__
     const size_t ks = (size_t)g.M * g.L;          // stride between z-planes
     const W *r = rin + c * cs + cix(g, i, J, 0);
     W *o = rout + c * cs + cix(g, i, J, 0);
+    // dpg: -DPGRNS of u deferred from an x-solve fused into k_conv_rhs (see there)
+    const W sub = (c == 0 && dpg) ? (W)dpg[0] : W(0);
     W coef[D + 1];
 #pragma unroll
     for (int d = 0; d <= D; ++d) coef[d] = gz[d];
@@ -555,7 +591,7 @@ __global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__
         W acc = coef[0] * win[D];
 #pragma unroll
         for (int d = 1; d <= D; ++d) acc += coef[d] * (win[D + d] + win[D - d]);
-        o[(size_t)k * ks] = acc;
+        o[(size_t)k * ks] = acc - sub;
 #pragma unroll
         for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
     };
@@ -1373,6 +1409,7 @@ struct chs_ctx {
         int nxcd = 1;                   // XCDs the stencil tiles are grouped for (1 = off)
         int fft_reg = 1;                // register FFTs where L / N have a plan (0 = LDS Stockham)
         int cn_xz = 0, cnxz_by = 1;     // fused x+z CN kernel (1; slower on MI355X), rows per block
+        int conv_fx = 1;                // x-solve inside k_conv_rhs where L = 64, 128, 192, 256
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
     bool have_fields = false;
@@ -1713,10 +1750,11 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
 }
 
 template <typename W>
-void cn_z_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *gz, const W *rin, W *rout, int KZ)
+void cn_z_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *gz, const W *rin, W *rout, int KZ,
+                   const double *dpg)
 {
     switch (D) {
-#define CHS_ZCASE(d) case d: hipLaunchKernelGGL((k_cn_z_march<W, d>), grid, blk, 0, st, g, gz, rin, rout, KZ); break;
+#define CHS_ZCASE(d) case d: hipLaunchKernelGGL((k_cn_z_march<W, d>), grid, blk, 0, st, g, gz, rin, rout, KZ, dpg); break;
         CHS_ZCASE(0) CHS_ZCASE(1) CHS_ZCASE(2) CHS_ZCASE(3) CHS_ZCASE(4) CHS_ZCASE(5) CHS_ZCASE(6)
         CHS_ZCASE(7) CHS_ZCASE(8)
 #undef CHS_ZCASE
@@ -1758,9 +1796,29 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
     const dim3 blk(64, 4), grd((L + 63) / 64, (M + 3) / 4, N);
     const int BY = c->tune.conv_by, KC = c->tune.conv_kc, BX = c->tune.conv_bx;
-    const dim3 blkc(64 * BX, BY), grdc((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
+    CnTab &t = c->cn[ns];
+    const int KZ = c->tune.cnz_kz, KY = c->tune.cny_ky;
+    const bool zmarch = !t.fullz && t.Dz <= 8;
+    const int LX = 64 * ((L + 63) / 64), BYX = c->tune.cnxz_by;
+    const size_t shxz = 2 * 8 * (size_t)BYX * LX * sizeof(W);
+    const bool xz = c->tune.cn_xz && zmarch && LX * BYX <= 1024 && shxz <= 65536;
+    // x-solve inside k_conv_rhs: whole x-lines of 1..4 wavefronts, a short x-inverse band, and
+    // the z-marching kernel to take the deferred mass-flux correction
+    const bool fx = c->tune.conv_fx && !xz && zmarch && L % 64 == 0 && L <= 256 && !t.fullx && t.Dx <= 16;
+    const dim3 blkc = fx ? dim3(L, 1) : dim3(64 * BX, BY);
+    const dim3 grdc = fx ? dim3(1, M, (N + KC - 1) / KC)
+                         : dim3((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
     const int npart = grdc.x * grdc.y * grdc.z;
-    {
+    if (fx) {
+        EvScope ev(c, "conv_rhs");
+        const int fence = c->tune.conv_fence;
+        switch (fence * 10 + L / 64) {
+#define CHS_FCASE(f, bx) case f * 10 + bx: hipLaunchKernelGGL((k_conv_rhs<W, 1, bx, f, true>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd, (const W *)t.gx, t.Dx); break;
+            CHS_FCASE(0, 1) CHS_FCASE(0, 2) CHS_FCASE(0, 3) CHS_FCASE(0, 4)
+            CHS_FCASE(1, 1) CHS_FCASE(1, 2) CHS_FCASE(1, 3) CHS_FCASE(1, 4)
+#undef CHS_FCASE
+        }
+    } else {
         EvScope ev(c, "conv_rhs");
         switch (c->tune.conv_fence * 1000 + BX * 100 + BY) {
 #define CHS_CCASE(f, bx, by) case f * 1000 + bx * 100 + by: hipLaunchKernelGGL((k_conv_rhs<W, by, bx, f>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd); break;
@@ -1772,16 +1830,12 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
         }
     }
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, npart, 1.0 / c->volm, c->scal);
-    CnTab &t = c->cn[ns];
     // z then y CN stencils: marching kernels for band half-widths up to 8 (the configs need
     // 3..7), the direct-load kernels otherwise; x and z fused where a block holds a whole
     // x-line (L <= 1024) and its two batches of planes fit 64 KiB of LDS
-    const int KZ = c->tune.cnz_kz, KY = c->tune.cny_ky;
-    const bool zmarch = !t.fullz && t.Dz <= 8;
-    const int LX = 64 * ((L + 63) / 64), BYX = c->tune.cnxz_by;
-    const size_t shxz = 2 * 8 * (size_t)BYX * LX * sizeof(W);
-    const bool xz = c->tune.cn_xz && zmarch && LX * BYX <= 1024 && shxz <= 65536;
-    if (xz) {
+    if (fx) {
+        // the x-solve ran inside k_conv_rhs
+    } else if (xz) {
         EvScope ev(c, "cn_xz");
         const dim3 gxz((M + BYX - 1) / BYX, 3 * ((N + KZ - 1) / KZ));
         cn_xz_dispatch<W>(t.Dz, gxz, dim3(LX, BYX), shxz, c->stream, g, (const W *)t.gx, t.Dx, t.fullx,
@@ -1797,7 +1851,7 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     } else if (zmarch) {
         EvScope ev(c, "cn_z");
         const dim3 gz((L + 63) / 64, (M + 3) / 4, 3 * ((N + KZ - 1) / KZ));
-        cn_z_dispatch<W>(t.Dz, gz, blk, c->stream, g, (const W *)t.gz, rhs, rhs2, KZ);
+        cn_z_dispatch<W>(t.Dz, gz, blk, c->stream, g, (const W *)t.gz, rhs, rhs2, KZ, fx ? c->scal : nullptr);
     } else {
         for (int comp = 0; comp < 3; ++comp) {
             EvScope ev(c, "cn_z");
@@ -2311,6 +2365,9 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "conv_kc") {
         if (value < 1 || value > N) return bad("must be 1..N");
         c->tune.conv_kc = value;
+    } else if (k == "conv_fx") {
+        if (value != 0 && value != 1) return bad("conv_fx must be 0 or 1");
+        c->tune.conv_fx = value;
     } else if (k == "cnx_nlx") {
         if (value < 0 || value > M * N || (size_t)value * L * sizeof(double) > 65536) return bad("lines must fit 64 KiB of LDS");
         c->tune.cnx_nlx = value;

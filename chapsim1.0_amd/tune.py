@@ -12,6 +12,7 @@ from .tg import ChsError
 
 KERNELS = {
     "conv_by": ("conv_rhs",), "conv_kc": ("conv_rhs",), "conv_bx": ("conv_rhs",), "conv_fence": ("conv_rhs",),
+    "conv_fx": ("conv_rhs", "cn_x"),
     "cnx_nlx": ("cn_x",), "cn_xz": ("cn_x", "cn_z", "cn_xz"), "cnxz_by": ("cn_xz",),
     "cnz_kz": ("cn_z", "cn_xz"), "cny_ky": ("cn_y",), "fftx_tk": ("divg_fftx", "fftx_inv"),
     "fftz_nl": ("fftz_fwd", "fftz_inv"), "nxcd": ("conv_rhs",),
@@ -26,6 +27,7 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "conv_kc": [k for k in (4, 8, 16, 32) if k <= N],
         "conv_bx": [1, 4] if L % 256 == 0 else [1],
         "conv_fence": [0, 1],
+        "conv_fx": [0, 1] if L % 64 == 0 and L <= 256 else [0],
         "cn_xz": [0, 1],
         "cnxz_by": [1, 2],
         "cnx_nlx": [n for n in (1, 2, 4, 8, 16) if n * L * 8 <= 65536 and n <= M * N],
