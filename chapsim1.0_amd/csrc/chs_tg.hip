@@ -88,7 +88,7 @@ struct Sub {
 // own u, v, w and p of plane k, and the plane k+1 values read for the z-differences) are
 // carried in registers, so each plane costs 15 loads of plane k, two of plane k-1, 5 of plane
 // k+1 and the 3 history values.
-// FX: the block is one whole x-line row (BY = 1, L = 64 BXW) and each plane's three RHS rows
+// FX: the block holds whole x-lines (L = 64 BXW) of BY rows and each plane's three RHS rows
 // go through LDS, where the circulant x-inverse gx (half-width Dx) of MOMFA1_X_tg is applied
 // before the store, so no separate x-solve pass is needed.  The mass-flux correction -DPGRNS,
 // which MOMFA_tg receives already subtracted (RHS_CvLpGpS_tg.f90:179-201), is then subtracted
@@ -101,8 +101,10 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                                                       double *__restrict__ partial, int KC, int nxcd,
                                                       const W *__restrict__ gx = nullptr, int Dx = 0)
 {
-    static_assert(!FX || BY == 1, "FX: one row per block");
-    __shared__ W lxs[FX ? 2 * 3 * 64 * BXW : 1];   // two planes' u, v, w rows (double buffer)
+    // FX: whole x-lines, BY rows; the block's rows are all inside the mesh (M % BY == 0)
+    constexpr int LXB = 64 * BXW * BY;               // one plane's row-block of one component
+    __shared__ W lxs[FX ? 2 * 3 * LXB : 1];           // two planes' u, v, w rows (double buffer)
+    const int lrow = threadIdx.y * 64 * BXW;
     const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
     const int tx = tile % gridDim.x, ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
     const int i = tx * (64 * BXW) + threadIdx.x;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                 hkw[ic] = conv;
                 const W pgm = (W)((p0 - pim) * g.dxi * coep);
                 const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
-                if constexpr (FX) lxs[(k & 1) * 3 * 64 * BXW + ic] = r;
+                if constexpr (FX) lxs[(k & 1) * 3 * LXB + lrow + ic] = r;
                 else rk[ic] = r;
                 rsum += (double)r;
             }
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                 if (J >= 2) {
                     hkw[cs + ic] = conv;
                     const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
-                    if constexpr (FX) lxs[(k & 1) * 3 * 64 * BXW + 64 * BXW + ic] = r;
+                    if constexpr (FX) lxs[(k & 1) * 3 * LXB + LXB + lrow + ic] = r;
                     else rk[cs + ic] = r;
                 }
             }
@@ -219,19 +221,19 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
                 hkw[2 * cs + ic] = conv;
                 const W pgm = (W)((p0 - p_m) * g.dzi * coep);
                 const W r = (rhsc + (W)(coev * lap)) * dt - pgm;
-                if constexpr (FX) lxs[(k & 1) * 3 * 64 * BXW + 2 * 64 * BXW + ic] = r;
+                if constexpr (FX) lxs[(k & 1) * 3 * LXB + 2 * LXB + lrow + ic] = r;
                 else rk[2 * cs + ic] = r;
             }
             if constexpr (FX) {
                 // the whole block runs this loop (every thread holds a point of the row); the
                 // next plane writes the other buffer, so one barrier per plane suffices
                 __syncthreads();
-                const W *lb = lxs + (k & 1) * 3 * 64 * BXW;
+                const W *lb = lxs + (k & 1) * 3 * LXB + lrow;
                 constexpr int LX = 64 * BXW;
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc) {
                     if (cc == 1 && J == 1) continue;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
-                    const W *ln = lb + cc * LX;
+                    const W *ln = lb + cc * LXB;
                     W acc = gx[0] * ln[ic];
                     for (int d = 1; d <= Dx; ++d) {
                         int a = (int)ic + d, b = (int)ic - d;
@@ -247,12 +249,19 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
         }
         mf = rsum / dyf;
     }
-    // block reduction of the mass-flux partial (fixed order -> deterministic)
-    __shared__ double red[64 * BXW * BY];
+    // block reduction of the mass-flux partial (fixed order -> deterministic); a block of
+    // 3 * 2^n threads folds its top third first
+    constexpr int NTB = 64 * BXW * BY, P2 = (NTB & (NTB - 1)) ? (NTB / 3) * 2 : NTB;
+    static_assert((P2 & (P2 - 1)) == 0, "block size 2^n or 3 * 2^n");
+    __shared__ double red[NTB];
     const int t = threadIdx.y * (64 * BXW) + threadIdx.x;
     red[t] = mf;
     __syncthreads();
-    for (int o = 32 * BXW * BY; o > 0; o >>= 1) {
+    if (P2 < NTB) {
+        if (t < NTB - P2) red[t] += red[t + P2];
+        __syncthreads();
+    }
+    for (int o = P2 / 2; o > 0; o >>= 1) {
         if (t < o) red[t] += red[t + o];
         __syncthreads();
     }
@@ -1409,7 +1418,7 @@ struct chs_ctx {
         int nxcd = 1;                   // XCDs the stencil tiles are grouped for (1 = off)
         int fft_reg = 1;                // register FFTs where L / N have a plan (0 = LDS Stockham)
         int cn_xz = 0, cnxz_by = 1;     // fused x+z CN kernel (1; slower on MI355X), rows per block
-        int conv_fx = 1;                // x-solve inside k_conv_rhs where L = 64, 128, 192, 256
+        int conv_fx = 1;                // x-solve inside k_conv_rhs (rows per block; 0 = off)
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
     bool have_fields = false;
@@ -1804,18 +1813,22 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     const bool xz = c->tune.cn_xz && zmarch && LX * BYX <= 1024 && shxz <= 65536;
     // x-solve inside k_conv_rhs: whole x-lines of 1..4 wavefronts, a short x-inverse band, and
     // the z-marching kernel to take the deferred mass-flux correction
-    const bool fx = c->tune.conv_fx && !xz && zmarch && L % 64 == 0 && L <= 256 && !t.fullx && t.Dx <= 16;
-    const dim3 blkc = fx ? dim3(L, 1) : dim3(64 * BX, BY);
-    const dim3 grdc = fx ? dim3(1, M, (N + KC - 1) / KC)
+    const int FY = c->tune.conv_fx;   // rows per block of the fused kernel (0: not fused)
+    const bool fx = FY > 0 && M % FY == 0 && !xz && zmarch && L % 64 == 0 && L * FY <= 512 && !t.fullx &&
+                    t.Dx <= 16;
+    const dim3 blkc = fx ? dim3(L, FY) : dim3(64 * BX, BY);
+    const dim3 grdc = fx ? dim3(1, M / FY, (N + KC - 1) / KC)
                          : dim3((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
     const int npart = grdc.x * grdc.y * grdc.z;
     if (fx) {
         EvScope ev(c, "conv_rhs");
         const int fence = c->tune.conv_fence;
-        switch (fence * 10 + L / 64) {
-#define CHS_FCASE(f, bx) case f * 10 + bx: hipLaunchKernelGGL((k_conv_rhs<W, 1, bx, f, true>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd, (const W *)t.gx, t.Dx); break;
-            CHS_FCASE(0, 1) CHS_FCASE(0, 2) CHS_FCASE(0, 3) CHS_FCASE(0, 4)
-            CHS_FCASE(1, 1) CHS_FCASE(1, 2) CHS_FCASE(1, 3) CHS_FCASE(1, 4)
+        switch (fence * 100 + FY * 10 + L / 64) {
+#define CHS_FCASE(f, by, bx) case f * 100 + by * 10 + bx: hipLaunchKernelGGL((k_conv_rhs<W, by, bx, f, true>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd, (const W *)t.gx, t.Dx); break;
+            CHS_FCASE(0, 1, 1) CHS_FCASE(0, 1, 2) CHS_FCASE(0, 1, 3) CHS_FCASE(0, 1, 4)
+            CHS_FCASE(0, 2, 1) CHS_FCASE(0, 2, 2) CHS_FCASE(0, 2, 3) CHS_FCASE(0, 2, 4)
+            CHS_FCASE(1, 1, 1) CHS_FCASE(1, 1, 2) CHS_FCASE(1, 1, 3) CHS_FCASE(1, 1, 4)
+            CHS_FCASE(1, 2, 1) CHS_FCASE(1, 2, 2) CHS_FCASE(1, 2, 3) CHS_FCASE(1, 2, 4)
 #undef CHS_FCASE
         }
     } else {
@@ -2366,7 +2379,7 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
         if (value < 1 || value > N) return bad("must be 1..N");
         c->tune.conv_kc = value;
     } else if (k == "conv_fx") {
-        if (value != 0 && value != 1) return bad("conv_fx must be 0 or 1");
+        if (value < 0 || value > 2) return bad("conv_fx must be 0, 1 or 2 (rows per block)");
         c->tune.conv_fx = value;
     } else if (k == "cnx_nlx") {
         if (value < 0 || value > M * N || (size_t)value * L * sizeof(double) > 65536) return bad("lines must fit 64 KiB of LDS");

@@ -27,7 +27,7 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "conv_kc": [k for k in (4, 8, 16, 32) if k <= N],
         "conv_bx": [1, 4] if L % 256 == 0 else [1],
         "conv_fence": [0, 1],
-        "conv_fx": [0, 1] if L % 64 == 0 and L <= 256 else [0],
+        "conv_fx": [f for f in (0, 1, 2) if f == 0 or (L % 64 == 0 and L * f <= 512 and M % f == 0)],
         "cn_xz": [0, 1],
         "cnxz_by": [1, 2],
         "cnx_nlx": [n for n in (1, 2, 4, 8, 16) if n * L * 8 <= 65536 and n <= M * N],

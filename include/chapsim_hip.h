@@ -127,10 +127,11 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * "fft_reg" 1 (default) to run the register-resident FFTs for line lengths 64 ... 1024, 0 for
  * the LDS Stockham transforms, "cn_xz" 1 to run the x and z Crank-Nicolson solves in one kernel
  * where a block can hold whole x-lines (default 0: two kernels, faster on MI355X at 256^2 x
- * 192), "cnxz_by" x-lines per block of that kernel (1, 2, 4; 1), "conv_fx" 1 (default) to
- * apply the x Crank-Nicolson inverse inside k_conv_rhs where L is 64, 128, 192 or 256 and its
- * band is short (one x-line row per block; conv_bx / conv_by then do not apply; the mass-flux
- * correction moves after the z-solve, which changes rounding only).  Results do not
+ * 192), "cnxz_by" x-lines per block of that kernel (1, 2, 4; 1), "conv_fx" rows per block
+ * (1 default, 2) of k_conv_rhs applying the x Crank-Nicolson inverse itself where L is 64,
+ * 128, 192 or 256 (L * conv_fx <= 512, M divisible by conv_fx) and the inverse's band is
+ * short, 0 for the separate x kernel (with conv_fx > 0, conv_bx / conv_by do not apply; the
+ * mass-flux correction moves after the z-solve, which changes rounding only).  Results do not
  * depend on the shapes except through the summation order of the mass-flux partials
  * (conv_by, conv_kc, conv_fx) and conv_fx's rounding.  A captured step graph is re-made on
  * the next step.  CHS_E_ARG for an unknown key or a value out of range. */

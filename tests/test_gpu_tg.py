@@ -190,10 +190,35 @@ def test_launch_shapes_keep_results(pkg, shape):
     ch.close()
 
 
+@pytest.mark.parametrize("L,M,N,fx", [(192, 24, 32, 1), (192, 24, 32, 2), (128, 32, 48, 2), (128, 32, 48, 0),
+                                       (64, 48, 32, 2)])
+def test_fused_x_solve_matches_oracle(pkg, L, M, N, fx):
+    """The x-solve inside k_conv_rhs (conv_fx rows per block; 0: the separate x kernel) on
+    whole-line tiles of one to three wavefronts (a 3-wave tile folds its mass-flux partials
+    first), with the mass-flux correction applied after the z-solve."""
+    STR2, REN, dt = 2.0, 4000.0, 8.0e-4
+    print(f"fused x {L}x{M}x{N} conv_fx {fx}", flush=True)
+    mesh, ch, o = make(pkg, L, M, N, STR2, REN)
+    ch.set_tuning(conv_fx=fx)
+    cv = 1.0 / REN
+    ch.SOLVERRK3_MOM_tg(0, dt, cv)
+    o.substep(0, dt)
+    for _ in range(2):
+        ch.rk3_step(dt, cv)
+        for ns in (1, 2, 3):
+            o.substep(ns, dt)
+    gq, gp = ch.download()
+    for c in range(3):
+        assert rel(gq[c], o.q[c]) < 1e-10, (c, rel(gq[c], o.q[c]))
+    assert rel(gp, o.pr) < 1e-9
+    assert ch.max_div() < 1e-10
+    ch.close()
+
+
 def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
     for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(cn_xz=2), dict(cnxz_by=3), dict(conv_kc=41), dict(fftx_tk=3),
-                dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6),
+                dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6), dict(conv_fx=3),
                 dict(no_such_key=1)):
         with pytest.raises(pkg.ChsError):
             ch.set_tuning(**bad)
