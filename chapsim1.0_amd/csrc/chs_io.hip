@@ -1124,6 +1124,32 @@ __global__ __launch_bounds__(256) void k_io_cfl(IoDev d, const double *__restric
     if (t == 0) d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
 }
 
+// DIVGCK_io / DIVGCK_Comm_io (DIVGCK.f90:60-175), periodic x: per cell
+// |dG1/dx + dG2/dy + dG3/dz + DrhoDtP|; block maxima to d.partial.
+__global__ __launch_bounds__(256) void k_io_divck(IoDev d)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[256];
+    IO_THREAD(1)
+    double v = 0.0;
+    if (i < L && J <= M) {
+        const int ip = i == L - 1 ? 0 : i + 1, kp = k == N - 1 ? 0 : k + 1;
+        const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+        const double divx = (d.g[hx(d, ip, J, k)] - d.g[c0]) * d.dxi;
+        const double divy = (d.g[hs + hx(d, i, J + 1, k)] - d.g[hs + c0]) * d.dyfi[J];
+        const double divz = (d.g[2 * hs + hx(d, i, J, kp)] - d.g[2 * hs + c0]) * d.dzi;
+        v = fabs(divx + divy + divz + d.drho[cx(d, i, J, k)]);
+    }
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) sh[t] = fmax(sh[t], sh[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
+}
+
 __global__ __launch_bounds__(256) void k_io_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
 {
     __shared__ double sh[256];
@@ -1479,6 +1505,20 @@ int io_cfl(chs_io *m, const double *dens, const double *visc, double dt, double 
 }
 
 }  // namespace
+
+int chs_max_div_io(chs_io *m, double *maxdiv)
+{
+    if (!m || !maxdiv) return io_fail(m, CHS_E_ARG, "chs_max_div_io: null argument");
+    hipLaunchKernelGGL(k_io_divck, io_grid(m, m->M), dim3(64, 4), 0, m->stream, m->d);
+    hipLaunchKernelGGL(k_io_reduce_max, dim3(1), dim3(256), 0, m->stream, m->d.partial, m->npart, m->d.scal + 2);
+    double v = 0.0;
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess) he = hipMemcpyAsync(&v, m->d.scal + 2, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    if (he != hipSuccess) return io_hip(m, he, "chs_max_div_io");
+    *maxdiv = v;
+    return CHS_OK;
+}
 
 int chs_cfl_io(chs_io *m, double dt, double *cfl)
 {

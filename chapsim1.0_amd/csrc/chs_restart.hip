@@ -1,5 +1,6 @@
-// Instantaneous-field files of the turbulence-generator domain in the reference's restart
-// format (host code): WRT_INSTANT_VARS_TG (processing/WRT_VARS_TG.f90:12-104) writes, per
+// Instantaneous-field files in the reference's restart format (host code), for the
+// turbulence-generator and the main domain: WRT_INSTANT_VARS_TG (processing/WRT_VARS_TG.f90:
+// 12-104) and WRT_INSTANT_VARS_IO (processing/WRT_VARS_IO.f90:16-227) write, per
 // variable U, V, W, P, the file <dir>/DNS_perixz_INSTANT_T<PhyTIME as ES15.9>_<var>.D holding
 // int32 NCL1, NCL2, NCL3, ITERG, then float64 PhyTIME, REN, DT, then the NCL1 x NCL2 x NCL3
 // interior values in Fortran order (MPI native representation, little-endian here);
@@ -34,66 +35,54 @@ std::string rst_name(const char *dir, double t, char var)
 
 const char kVars[4] = {'U', 'V', 'W', 'P'};
 
-}  // namespace
-
-extern "C" {
-
-int chs_restart_write_tg(const char *dir, int L, int M, int N, int iterg, double phytime, double ren, double dt,
-                         const double *q, const double *pr)
+// One variable file of the format above from the interior rows of a halo array f.
+int write_var(const char *who, const char *dir, char var, int L, int M, int N, int iterg, double phytime, double ren,
+              double dt, const double *f, std::vector<double> &buf)
 {
-    if (L < 1 || M < 1 || N < 1 || !q || !pr) return rst_fail(CHS_E_ARG, "chs_restart_write_tg: bad argument");
-    const size_t hs = (size_t)N * (M + 2) * L;
-    std::vector<double> buf((size_t)L * M * N);
-    for (int v = 0; v < 4; ++v) {
-        const double *f = v < 3 ? q + v * hs : pr;
-        // Fortran order (I fastest, then J = 1..M, then K): the halo array's interior rows
-        for (int k = 0; k < N; ++k)
-            for (int J = 1; J <= M; ++J)
-                for (int i = 0; i < L; ++i)
-                    buf[((size_t)k * M + (J - 1)) * L + i] = f[((size_t)k * (M + 2) + J) * L + i];
-        const std::string fn = rst_name(dir, phytime, kVars[v]);
-        FILE *fp = std::fopen(fn.c_str(), "wb");
-        if (!fp) return rst_fail(CHS_E_ARG, "chs_restart_write_tg: cannot open " + fn);
-        const int32_t hdr[4] = {L, M, N, iterg};
-        const double rl[3] = {phytime, ren, dt};
-        const bool ok = std::fwrite(hdr, sizeof hdr, 1, fp) == 1 && std::fwrite(rl, sizeof rl, 1, fp) == 1 &&
-                        std::fwrite(buf.data(), sizeof(double), buf.size(), fp) == buf.size();
-        if (std::fclose(fp) != 0 || !ok) return rst_fail(CHS_E_ARG, "chs_restart_write_tg: write failed: " + fn);
-    }
+    for (int k = 0; k < N; ++k)
+        for (int J = 1; J <= M; ++J)
+            for (int i = 0; i < L; ++i) buf[((size_t)k * M + (J - 1)) * L + i] = f[((size_t)k * (M + 2) + J) * L + i];
+    const std::string fn = rst_name(dir, phytime, var);
+    FILE *fp = std::fopen(fn.c_str(), "wb");
+    if (!fp) return rst_fail(CHS_E_ARG, std::string(who) + ": cannot open " + fn);
+    const int32_t hdr[4] = {L, M, N, iterg};
+    const double rl[3] = {phytime, ren, dt};
+    const bool ok = std::fwrite(hdr, sizeof hdr, 1, fp) == 1 && std::fwrite(rl, sizeof rl, 1, fp) == 1 &&
+                    std::fwrite(buf.data(), sizeof(double), buf.size(), fp) == buf.size();
+    if (std::fclose(fp) != 0 || !ok) return rst_fail(CHS_E_ARG, std::string(who) + ": write failed: " + fn);
     return CHS_OK;
 }
 
-int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *iterg, double *phytime, double *q,
-                        double *pr)
+// One variable file into the interior rows of a halo array f (READ_3D_VARS,
+// processing/READ_INTP_3D_VARS.f90:20-103); the header's ITERG, PhyTIME to iterg, phytime.
+int read_var(const char *who, const char *dir, double tfm, char var, int L, int M, int N, int *iterg, double *phytime,
+             double *f, std::vector<double> &buf)
 {
-    if (L < 1 || M < 1 || N < 1 || !q || !pr) return rst_fail(CHS_E_ARG, "chs_restart_read_tg: bad argument");
+    const std::string fn = rst_name(dir, tfm, var);
+    FILE *fp = std::fopen(fn.c_str(), "rb");
+    if (!fp) return rst_fail(CHS_E_ARG, std::string(who) + ": file " + fn + " does not exist");
+    int32_t hdr[4];
+    double rl[3];
+    const bool ok = std::fread(hdr, sizeof hdr, 1, fp) == 1 && std::fread(rl, sizeof rl, 1, fp) == 1 &&
+                    std::fread(buf.data(), sizeof(double), buf.size(), fp) == buf.size();
+    std::fclose(fp);
+    if (!ok) return rst_fail(CHS_E_ARG, std::string(who) + ": short file " + fn);
+    if (hdr[0] != L || hdr[1] != M || hdr[2] != N)
+        return rst_fail(CHS_E_ARG, std::string(who) + ": " + fn + " holds " + std::to_string(hdr[0]) + " x " +
+                                       std::to_string(hdr[1]) + " x " + std::to_string(hdr[2]));
+    if (iterg) *iterg = hdr[3];
+    if (phytime) *phytime = rl[0];
+    for (int k = 0; k < N; ++k)
+        for (int J = 1; J <= M; ++J)
+            for (int i = 0; i < L; ++i) f[((size_t)k * (M + 2) + J) * L + i] = buf[((size_t)k * M + (J - 1)) * L + i];
+    return CHS_OK;
+}
+
+// Wall rows after a restart read: velocity (or mass flux) ghost rows and v on J = 1 zero,
+// pressure ghosts copy the wall cells (BC_WALL_Q_tg / BC_WALL_G_io, BC_WALL_PR_TG / _io).
+void restart_walls(int L, int M, int N, double *q, double *pr)
+{
     const size_t hs = (size_t)N * (M + 2) * L;
-    std::vector<double> buf((size_t)L * M * N);
-    for (int v = 0; v < 4; ++v) {
-        const std::string fn = rst_name(dir, tfm, kVars[v]);
-        FILE *fp = std::fopen(fn.c_str(), "rb");
-        if (!fp) return rst_fail(CHS_E_ARG, "chs_restart_read_tg: file " + fn + " does not exist");
-        int32_t hdr[4];
-        double rl[3];
-        const bool ok = std::fread(hdr, sizeof hdr, 1, fp) == 1 && std::fread(rl, sizeof rl, 1, fp) == 1 &&
-                        std::fread(buf.data(), sizeof(double), buf.size(), fp) == buf.size();
-        std::fclose(fp);
-        if (!ok) return rst_fail(CHS_E_ARG, "chs_restart_read_tg: short file " + fn);
-        if (hdr[0] != L || hdr[1] != M || hdr[2] != N)
-            return rst_fail(CHS_E_ARG, "chs_restart_read_tg: " + fn + " holds " + std::to_string(hdr[0]) + " x " +
-                                           std::to_string(hdr[1]) + " x " + std::to_string(hdr[2]));
-        if (v == 0) {
-            if (iterg) *iterg = hdr[3];
-            if (phytime) *phytime = rl[0];
-        }
-        double *f = v < 3 ? q + v * hs : pr;
-        for (int k = 0; k < N; ++k)
-            for (int J = 1; J <= M; ++J)
-                for (int i = 0; i < L; ++i)
-                    f[((size_t)k * (M + 2) + J) * L + i] = buf[((size_t)k * M + (J - 1)) * L + i];
-    }
-    // walls as FLOWSTART applies them after the read (FLOWSTART.f90:37-40: BC_WALL_Q_tg,
-    // BC_WALL_PR_TG): velocity ghost rows and v on J = 1 zero, pressure ghosts copy
     for (int k = 0; k < N; ++k)
         for (int i = 0; i < L; ++i) {
             for (int c = 0; c < 3; ++c) {
@@ -104,6 +93,94 @@ int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *i
             pr[((size_t)k * (M + 2)) * L + i] = pr[((size_t)k * (M + 2) + 1) * L + i];
             pr[((size_t)k * (M + 2) + M + 1) * L + i] = pr[((size_t)k * (M + 2) + M) * L + i];
         }
+}
+
+}  // namespace
+
+extern "C" {
+
+int chs_restart_write_tg(const char *dir, int L, int M, int N, int iterg, double phytime, double ren, double dt,
+                         const double *q, const double *pr)
+{
+    if (L < 1 || M < 1 || N < 1 || !q || !pr) return rst_fail(CHS_E_ARG, "chs_restart_write_tg: bad argument");
+    const size_t hs = (size_t)N * (M + 2) * L;
+    std::vector<double> buf((size_t)L * M * N);
+    for (int v = 0; v < 4; ++v)
+        if (int rc = write_var("chs_restart_write_tg", dir, kVars[v], L, M, N, iterg, phytime, ren, dt,
+                               v < 3 ? q + v * hs : pr, buf))
+            return rc;
+    return CHS_OK;
+}
+
+int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *iterg, double *phytime, double *q,
+                        double *pr)
+{
+    if (L < 1 || M < 1 || N < 1 || !q || !pr) return rst_fail(CHS_E_ARG, "chs_restart_read_tg: bad argument");
+    const size_t hs = (size_t)N * (M + 2) * L;
+    std::vector<double> buf((size_t)L * M * N);
+    for (int v = 0; v < 4; ++v)
+        if (int rc = read_var("chs_restart_read_tg", dir, tfm, kVars[v], L, M, N, v == 0 ? iterg : nullptr,
+                              v == 0 ? phytime : nullptr, v < 3 ? q + v * hs : pr, buf))
+            return rc;
+    // walls as FLOWSTART applies them after the read (FLOWSTART.f90:37-40)
+    restart_walls(L, M, N, q, pr);
+    return CHS_OK;
+}
+
+int chs_restart_write_io(const char *dir, int L, int M, int N, int iterg, double phytime, double ren, double dt,
+                         const double *q, const double *pr, const double *t, const double *d, const double *h,
+                         const double *m, const double *k)
+{
+    const bool th = t || d || h || m || k;
+    if (L < 1 || M < 1 || N < 1 || !q || !pr || (th && !(t && d && h && m && k)))
+        return rst_fail(CHS_E_ARG, "chs_restart_write_io: bad argument (thermal fields: all five or none)");
+    const size_t hs = (size_t)N * (M + 2) * L;
+    std::vector<double> buf((size_t)L * M * N);
+    // WRT_INSTANT_VARS_FLOW_IO (processing/WRT_VARS_IO.f90:24-123): Q_io, PR_io
+    for (int v = 0; v < 4; ++v)
+        if (int rc = write_var("chs_restart_write_io", dir, kVars[v], L, M, N, iterg, phytime, ren, dt,
+                               v < 3 ? q + v * hs : pr, buf))
+            return rc;
+    if (!th) return CHS_OK;
+    // WRT_INSTANT_VARS_THERMAL_IO (:126-227): TEMPERATURE, DENSITY, ENTHALPY, Viscousity, THERMCONDT
+    const char tv[5] = {'T', 'D', 'H', 'M', 'K'};
+    const double *tf[5] = {t, d, h, m, k};
+    for (int v = 0; v < 5; ++v)
+        if (int rc = write_var("chs_restart_write_io", dir, tv[v], L, M, N, iterg, phytime, ren, dt, tf[v], buf))
+            return rc;
+    return CHS_OK;
+}
+
+int chs_restart_read_io(const char *dir, double tfm, int L, int M, int N, int ini_field_type, int *iterg,
+                        double *phytime, double *flow, double *pr, double *t, double *d, double *h, double *dh)
+{
+    const bool th = t || d || h || dh;
+    if (L < 1 || M < 1 || N < 1 || !flow || !pr || (ini_field_type != 0 && ini_field_type != 1) ||
+        (th && !(t && d && h && dh)) || (th && ini_field_type != 0))
+        return rst_fail(CHS_E_ARG, "chs_restart_read_io: bad argument (type 0 or 1; thermal fields: all four, "
+                                   "type 0 only)");
+    const size_t hs = (size_t)N * (M + 2) * L;
+    std::vector<double> buf((size_t)L * M * N);
+    // ReStart_INSTANT_VARS_io (processing/RESTART_io.f90:16-92): U, V, W, P into G_io (type 0)
+    // or Q_io (type 1), and PR_io
+    for (int v = 0; v < 4; ++v)
+        if (int rc = read_var("chs_restart_read_io", dir, tfm, kVars[v], L, M, N, v == 0 ? iterg : nullptr,
+                              v == 0 ? phytime : nullptr, v < 3 ? flow + v * hs : pr, buf))
+            return rc;
+    restart_walls(L, M, N, flow, pr);   // BC_WALL_G_io / BC_WALL_PR_io (:147-157)
+    if (!th) return CHS_OK;
+    // thermal field (:95-131): TEMPERATURE, DENSITY, ENTHALPY, then DH = DENSITY * ENTHALPY
+    const char tv[3] = {'T', 'D', 'H'};
+    double *tf[3] = {t, d, h};
+    for (int v = 0; v < 3; ++v)
+        if (int rc = read_var("chs_restart_read_io", dir, tfm, tv[v], L, M, N, nullptr, nullptr, tf[v], buf))
+            return rc;
+    for (int kk = 0; kk < N; ++kk)
+        for (int J = 1; J <= M; ++J)
+            for (int i = 0; i < L; ++i) {
+                const size_t o = ((size_t)kk * (M + 2) + J) * L + i;
+                dh[o] = d[o] * h[o];
+            }
     return CHS_OK;
 }
 

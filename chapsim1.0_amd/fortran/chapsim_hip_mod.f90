@@ -304,6 +304,14 @@ MODULE chapsim_hip
             INTEGER(C_INT) :: rc
         END FUNCTION
 
+        ! DIVGCK_io (DIVGCK.f90:60-175): MAXDIVGV_io
+        FUNCTION chs_max_div_io(io, maxdiv) BIND(C, NAME='chs_max_div_io') RESULT(rc)
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: io
+            REAL(C_DOUBLE), INTENT(OUT) :: maxdiv
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
         FUNCTION chs_heated_cfl(eng, io, dt, cfl) BIND(C, NAME='chs_heated_cfl') RESULT(rc)
             IMPORT :: C_INT, C_PTR, C_DOUBLE
             TYPE(C_PTR), VALUE :: eng, io

@@ -93,6 +93,11 @@ SIGNATURES = {
     "chs_substep_io_momentum": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_io_upload_viscosity": (C.c_int, [C.c_void_p, _dp]),
     "chs_cfl_io": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
+    "chs_max_div_io": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "chs_restart_write_io": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double] + [_dp] * 7),
+    "chs_restart_read_io": (C.c_int, [C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int), _dp] + [_dp] * 6),
     "chs_heated_cfl": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
     "chs_io_download": (C.c_int, [C.c_void_p] + [_dp] * 9),
     "chs_io_sync": (C.c_int, [C.c_void_p]),
@@ -475,6 +480,12 @@ class ChannelIO:
     def sync(self):
         self._check(lib().chs_io_sync(self._h), "chs_io_sync")
 
+    def max_div(self) -> float:
+        """DIVGCK_io (DIVGCK.f90:60-175): max |div G + DrhoDtP|."""
+        v = C.c_double()
+        self._check(lib().chs_max_div_io(self._h, C.byref(v)), "chs_max_div_io")
+        return v.value
+
     def cfl(self, dt: float) -> float:
         """CFL_io (CFL.f90:59-107) x dt on the current velocity, with the context's DENSITY0 and
         viscosity (a flow-only run's DENSITY, Viscousity); cfl(1.0) is CFLMM_io."""
@@ -553,6 +564,38 @@ def restart_read_tg(directory, tfm, L, M, N):
     if rc != 0:
         raise ChsError(lib().chs_restart_last_error().decode())
     return q, pr, it.value, float(t[0])
+
+
+def restart_write_io(directory, q, pr, iterg, phytime, REN, dt, thermal=None):
+    """WRT_INSTANT_VARS_IO file set (chs_restart_write_io) from halo arrays q [3, N, M+2, L]
+    (Q_io), pr and, for a thermal run, thermal = dict T, D, H, M, K [N, M+2, L]."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    _, N, Mh, L = q.shape
+    th = [None if thermal is None or thermal.get(k) is None else np.ascontiguousarray(thermal[k], dtype=np.float64)
+          for k in "TDHMK"]
+    rc = lib().chs_restart_write_io(str(directory).encode(), L, Mh - 2, N, int(iterg), float(phytime), float(REN),
+                                    float(dt), _p(q), _p(pr), *[None if a is None else _p(a) for a in th])
+    if rc != 0:
+        raise ChsError(lib().chs_restart_last_error().decode())
+
+
+def restart_read_io(directory, tfm, L, M, N, ini_field_type=0, thermal=True):
+    """ReStart_INSTANT_VARS_io(TFM), file part (chs_restart_read_io): dict FLOW [3, N, M+2, L]
+    (G_io for ini_field_type 0, Q_io for 1), PR, ITERG, PhyTIME and, with thermal, T, D, H, DH."""
+    hs = (N, M + 2, L)
+    o = dict(FLOW=np.zeros((3,) + hs), PR=np.zeros(hs))
+    if thermal:
+        o.update({k: np.zeros(hs) for k in ("T", "D", "H", "DH")})
+    it = C.c_int(0)
+    t = np.zeros(1)
+    th = [_p(o[k]) for k in ("T", "D", "H", "DH")] if thermal else [None] * 4
+    rc = lib().chs_restart_read_io(str(directory).encode(), float(tfm), L, M, N, int(ini_field_type), C.byref(it),
+                                   _p(t), _p(o["FLOW"]), _p(o["PR"]), *th)
+    if rc != 0:
+        raise ChsError(lib().chs_restart_last_error().decode())
+    o["ITERG"], o["PhyTIME"] = it.value, float(t[0])
+    return o
 
 
 def interp_initial_tg(qo, po, HXoo, HZoo, yndo, L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8):

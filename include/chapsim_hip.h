@@ -325,6 +325,24 @@ int chs_restart_write_tg(const char *dir, int L, int M, int N, int iterg, double
  * receive ITERG and PhyTIME.  CHS_E_ARG if a file is missing, short or of another mesh. */
 int chs_restart_read_tg(const char *dir, double tfm, int L, int M, int N, int *iterg, double *phytime, double *q,
                         double *pr);
+
+/* WRT_INSTANT_VARS_IO (processing/WRT_VARS_IO.f90:16-227), periodic main domain: the same
+ * format, files _U, _V, _W, _P from q (Q_io) and pr (PR_io) and, for a thermal run, _T, _D,
+ * _H, _M, _K from t, d, h, m, k (TEMPERATURE, DENSITY, ENTHALPY, Viscousity, THERMCONDT), all
+ * [N][M+2][L] host arrays (thermal pointers all NULL for a flow-only run). */
+int chs_restart_write_io(const char *dir, int L, int M, int N, int iterg, double phytime, double ren, double dt,
+                         const double *q, const double *pr, const double *t, const double *d, const double *h,
+                         const double *m, const double *k);
+
+/* ReStart_INSTANT_VARS_io(TFM) (processing/RESTART_io.f90:16-131), file part: _U, _V, _W, _P
+ * into flow [3][N][M+2][L] and pr, which the reference takes as G_io (iIniFieldType = 0;
+ * note that WRT_INSTANT_VARS_IO writes Q_io into those files) or as Q_io (iIniFieldType = 1),
+ * with the wall rows (BC_WALL_G_io, BC_WALL_PR_io); for iIniFieldType = 0 and a thermal run
+ * also _T, _D, _H into t, d, h and DH = DENSITY * ENTHALPY into dh (rows 1..M; thermal pointers
+ * all NULL otherwise).  The properties, their walls and the staggered fields the reference
+ * then rebuilds come from chs_eng_upload(dh, g, NULL, 1) and chs_io_upload / chs_io_props. */
+int chs_restart_read_io(const char *dir, double tfm, int L, int M, int N, int ini_field_type, int *iterg,
+                        double *phytime, double *flow, double *pr, double *t, double *d, double *h, double *dh);
 const char *chs_restart_last_error(void);
 
 /* Interpolated start from a coarser field, INITIAL_INTERP_tg (processing/INTERPOLATION_INITIAL.f90:
@@ -425,6 +443,10 @@ int chs_io_sync(chs_io *m);
  * context's current DENSITY and Viscousity, as a heated run calls it at the start of a step.
  * Synchronous. */
 int chs_cfl_io(chs_io *m, double dt, double *cfl);
+
+/* DIVGCK_io (DIVGCK.f90:60-175, periodic x): max over cells of |div G + DrhoDtP| for the
+ * current G and the DrhoDtP of the last chs_io_props (MAXDIVGV_io).  Synchronous. */
+int chs_max_div_io(chs_io *m, double *maxdiv);
 int chs_heated_cfl(chs_eng *e, chs_io *m, double dt, double *cfl);
 
 /* One stage of the heated channel as SOLVE runs it (SOLVE.f90:119-133): the energy substep

@@ -357,6 +357,18 @@ def cfl_io(mesh, q, dens, visc, cvisc):
     return float(np.max(np.maximum(qcf, vcf)))
 
 
+def max_div_io(mesh, g, drho):
+    """DIVGCK_io (DIVGCK.f90:60-175, periodic x) restated: max over cells of
+    |dG1/dx + dG2/dy + dG3/dz + DrhoDtP|.  g [3, N, M+2, L], drho [N, M, L]."""
+    M = mesh.M
+    dyfi = np.asarray(mesh.DYFI)[None, :, None]
+    g1, g2, g3 = g[0][:, 1:M + 1], g[1], g[2][:, 1:M + 1]
+    divx = (np.roll(g1, -1, axis=2) - g1) * mesh.DXI
+    divy = (g2[:, 2:M + 2] - g2[:, 1:M + 1]) * dyfi
+    divz = (np.roll(g3, -1, axis=0) - g3) * mesh.DZI
+    return float(np.max(np.abs(divx + divy + divz + drho)))
+
+
 class OracleIO:
     """SOLVERRK3_MOM_io restated (or_io_substep) with the staggered properties of SOLVE
     (or_io_props).  Fields are C-order; ``f`` holds G, G0, Q, DSTG, MSTG [3, N, M+2, L], PR,

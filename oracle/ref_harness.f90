@@ -43,6 +43,7 @@
 !                           and, with iVisScheme = 1, DENSITY_IMPLICIT_add (SOLVE.f90:122-127)
 !   loadvisc FILE           Viscousity (the current viscosity) from a global L x (M+2) x N file
 !   cflio FILE              VELOCITY_CALC_io and CFL_io; rank 0 writes CFLMM_io to FILE
+!   divckio FILE            DIVGCK_io; rank 0 writes MAXDIVGV_io(1:3) to FILE
 !   iocn NS DT FILE         DT = given; the momentum part of SOLVERRK3_MOM_io(NS) with the
 !                           Crank-Nicolson viscous scheme (SOLVERRK3_MOM_IO.f90:25-41, :61-74):
 !                           each rank writes FILE.<rank>: G_io and DRHOI_STG rows 0..N2DO+1
@@ -63,6 +64,8 @@
 !                           U1mean_WORK_tg, U1maxx_WORK_tg
 !   rstwrite T IT           PhyTIME = T, ITERG = IT, WRT_INSTANT_VARS_TG (into FilePath1)
 !   rstread T               ReStart_INSTANT_VARS_TG(T) and the walls as FLOWSTART.f90:37-40
+!   rstwriteio T IT         PhyTIME = T, ITERG = IT, VELOCITY_CALC_io, WRT_INSTANT_VARS_IO
+!   rstreadio T             ReStart_INSTANT_VARS_io(T) (main domain, iIniFieldType of the ini)
 !   interp                  INITIAL_INTERP_tg (needs iIniField_tg = 1, the coarse files in
 !                           FilePath3 and INTP_COORD_YND.ini), then the walls as
 !                           FLOWSTART.f90:37-40
@@ -259,6 +262,15 @@ PROGRAM ref_harness
                 WRITE(80) CFLMM_io
                 CLOSE(80)
             END IF
+        CASE ('divckio')
+            READ(line, *) cmd, fname
+            CALL DIVGCK_io
+            IF (MYID == 0) THEN
+                OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+                     STATUS = 'replace', ACTION = 'write')
+                WRITE(80) MAXDIVGV_io
+                CLOSE(80)
+            END IF
         CASE ('iocn')
             READ(line, *) cmd, ns, dtin, fname
             DT = dtin
@@ -312,6 +324,15 @@ PROGRAM ref_harness
             PhyTIME = dtin
             ITERG = idr
             CALL WRT_INSTANT_VARS_TG
+        CASE ('rstwriteio')
+            READ(line, *) cmd, dtin, idr
+            PhyTIME = dtin
+            ITERG = idr
+            CALL VELOCITY_CALC_io
+            CALL WRT_INSTANT_VARS_IO
+        CASE ('rstreadio')
+            READ(line, *) cmd, dtin
+            CALL ReStart_INSTANT_VARS_io(dtin)
         CASE ('rstread')
             READ(line, *) cmd, dtin
             CALL ReStart_INSTANT_VARS_TG(dtin)

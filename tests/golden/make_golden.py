@@ -283,8 +283,9 @@ def make_io_cn(T0=665.0):
     """Main-domain momentum fixture, Crank-Nicolson viscous scheme (iVisScheme = 1), from the
     IO_CASES states (io.npz holds them):
       cn_*   ``iocn`` of stage 1 with a noisy current viscosity: G after the three MOMFA_io
-             calls, RHS_io as each receives it, EXPLT0, DRHOI_STG; and CFLMM_io of CFL_io
-             on the stage's starting velocity (``cflio``);
+             calls, RHS_io as each receives it, EXPLT0, DRHOI_STG; CFLMM_io of CFL_io on
+             the stage's starting velocity (``cflio``) and MAXDIVGV_io of DIVGCK_io on the
+             starting G with the stage's DrhoDtP (``divckio``);
       s1..3  three substeps (``iosubstep``, FFT99 Poisson) with the wall-row viscosity means
              zero: G, PR, EXPLT0 after each, RHSP and DPH of the first.
     The reference runs on two ranks; the first inputs are compared on the first half of z
@@ -302,7 +303,7 @@ def make_io_cn(T0=665.0):
         m1w = cn_viscosity(st, seed, True)
         tag = f"io{L}x{M}x{N}"
         for run, visc, cmds in (
-                ("cn", m1, ["ioprops 1", "cflio cfl.bin", f"iocn 1 {dt!r} c1"]),
+                ("cn", m1, ["ioprops 1", "cflio cfl.bin", "divckio dck.bin", f"iocn 1 {dt!r} c1"]),
                 ("sub", m1w, ["ioprops 1", f"iosubstep 1 {dt!r}", "dumpio s1", "ioprops 2", f"iosubstep 2 {dt!r}",
                               "dumpio s2", "ioprops 3", f"iosubstep 3 {dt!r}", "dumpio s3"])):
             h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt, iDomain=2, iThermoDynamics=1, T0=T0, Ti=T0,
@@ -317,6 +318,7 @@ def make_io_cn(T0=665.0):
                     for k in ("G", "RHS", "EXPLT0", "DRHOI"):
                         out[f"{tag}_cn_{k}"] = r[k]
                     out[f"{tag}_cn_CFLMM"] = np.fromfile(h.path("cfl.bin"), dtype=np.float64)
+                    out[f"{tag}_cn_MAXDIV"] = np.fromfile(h.path("dck.bin"), dtype=np.float64)
                 else:
                     for sname in ("s1", "s2", "s3"):
                         r = refrun.read_io(h, sname)

@@ -332,3 +332,43 @@ def test_gpu_cn_line_solve_regimes(pkg, REN, dt):
     for c in range(3):
         assert rel(f["G"][c], o.f["G"][c]) < TOL, (c, rel(f["G"][c], o.f["G"][c]))
     ch.close()
+
+
+def test_oracle_max_div_io_matches_reference(golden):
+    """DIVGCK_io on the stage-1 starting G with the stage's DrhoDtP (the ``divckio`` fixture);
+    periodic x: the three maxima agree.  After a whole substep (projection) it is at solver
+    precision, for both viscous schemes."""
+    for c in cases(golden):
+        g, tag = c["g"], c["tag"]
+        mesh = setup.Mesh(c["L"], c["M"], c["N"], STR2=c["STR2"])
+        ref = g[f"{tag}_cn_MAXDIV"]
+        assert ref[0] == ref[1] == ref[2] and ref[0] > 0.1
+        o = make_oracle(c)
+        o.set_state(dict(c["init"], M1=c["m1"]))
+        o.props(1)
+        v = oracle.max_div_io(mesh, o.f["G"], o.f["DRHO"])
+        assert abs(v - ref[0]) <= 1e-14 * ref[0], (v, ref[0])
+        for vis in (0, 1):
+            o = make_oracle(c, vis_scheme=vis)
+            o.set_state(dict(c["init"], M1=c["m1w"]))
+            o.props(1)
+            o.substep(1, c["dt"])
+            assert oracle.max_div_io(mesh, o.f["G"], o.f["DRHO"]) < 1e-9 * ref[0]
+
+
+@pytest.mark.gpu
+def test_gpu_max_div_io(pkg, golden):
+    for c in cases(golden):
+        g, tag = c["g"], c["tag"]
+        print(f"max_div_io {tag}", flush=True)
+        co = pkg.channel_setup(c["L"], c["M"], c["N"], STR2=c["STR2"])
+        for vis in (0, 1):
+            ch = pkg.ChannelIO(co, c["REN"], gravity=c["grav"], f_a=c["f_a"], flow_driven=c["fd"],
+                               cf_given=8.18e-3, vis_scheme=vis)
+            ch.upload(**c["init"], **({"M1": c["m1"]} if vis else {}))
+            ch.ioprops(1)
+            ref = g[f"{tag}_cn_MAXDIV"][0]
+            assert abs(ch.max_div() - ref) <= 1e-14 * ref
+            ch.SOLVERRK3_MOM_io(1, c["dt"])
+            assert ch.max_div() < 1e-9 * ref
+            ch.close()
