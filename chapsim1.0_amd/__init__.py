@@ -8,6 +8,6 @@ turbulence-generator domain, ChannelIO for the main domain), ``thermo.py`` (prop
 and the energy substep).
 """
 from .tg import (MIXED, PLAIN_FP64, ChannelIO, ChannelTG, ChsError, DeviceArray, Ensemble,  # noqa: F401
-                 channel_setup, heated_cfl, heated_substep, interp_initial_tg, lib, next_dt, restart_read_io,
+                 channel_setup, heated_cfl, heated_substep, interp_initial_tg, lib, next_dt, random_field_tg, restart_read_io,
                  restart_read_tg, restart_write_io, restart_write_tg, tdma_lines)
 from .init import poiseuille_perturbed  # noqa: F401

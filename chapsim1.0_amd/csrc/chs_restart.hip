@@ -6,6 +6,7 @@
 // interior values in Fortran order (MPI native representation, little-endian here);
 // ReStart_INSTANT_VARS_TG (processing/RESTART_tg.f90:11-56) with READ_3D_VARS
 // (processing/READ_INTP_3D_VARS.f90:20-103) reads them back.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -322,5 +323,141 @@ extern "C" int chs_interp_initial_tg(int Loo, int Mo, int Noo, double HXoo, doub
             pr[((size_t)k * (M + 2)) * L + i] = pr[((size_t)k * (M + 2) + 1) * L + i];
             pr[((size_t)k * (M + 2) + M + 1) * L + i] = pr[((size_t)k * (M + 2) + M) * L + i];
         }
+    return CHS_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Random (or sine) start of the turbulence-generator channel: random_FL_FLD_TG
+// (INITIAL_FL_FLD_TG.f90:16-104) with IniField_FLOW_tg (:174-244), INITIAL_MEAN_TG
+// (processing/INITIAL_MEAN.f90:13-122), the laminar profile Vini (LAMPOISLPROF.f90:29-51) and
+// BULK_VELOCITY_TG (INITIAL_FL_FLD_TG.f90:142-166).
+// ---------------------------------------------------------------------------------------
+namespace {
+
+// splitmix64: a counter-based stand-in for the compiler's RANDOM_NUMBER, which no other
+// compiler reproduces; the value at a point is a pure function of its key.
+uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// rvec_random(-1, 1, 3, XRD) (processing/randomgen.f90): a + (b - a) r with r uniform in [0, 1)
+void rvec3(uint64_t key, double *x)
+{
+    for (int c = 0; c < 3; ++c) {
+        const double r = (double)(mix64(key * 3ull + (uint64_t)c) >> 11) * 0x1.0p-53;
+        x[c] = -1.0 + 2.0 * r;
+    }
+}
+
+}  // namespace
+
+extern "C" int chs_random_field_tg(int L, int M, int N, double HX, double HZ, double STR2, int random_type,
+                                   double vperg, double svperg, unsigned long long seed, double m_over_d, double *q,
+                                   double *pr)
+{
+    if (L < 2 || M < 2 || N < 2 || !q || !pr || !(HX > 0) || !(HZ > 0) || random_type < 1 || random_type > 3)
+        return rst_fail(CHS_E_ARG, "chs_random_field_tg: bad argument (random_type 1, 2 or 3)");
+    // mesh of the turbulence-generator domain alone (CONSPARA; COORDJR.f90:44-55, :166-194)
+    const double dx = HX / (double)L, dz = HZ / (double)N;
+    std::vector<double> xnd(L + 1), xcc(L), znd(N + 1), zcc(N), ynd(M + 1), ycc(M), dyfi(M), vini(M);
+    for (int i = 0; i <= L; ++i) xnd[i] = (double)i * dx;
+    for (int i = 0; i < L; ++i) xcc[i] = 0.5 * (xnd[i] + xnd[i + 1]);
+    for (int k = 0; k <= N; ++k) znd[k] = (double)k * dz;
+    for (int k = 0; k < N; ++k) zcc[k] = 0.5 * (znd[k] + znd[k + 1]);
+    for (int j = 0; j <= M; ++j) {
+        const double x2 = (double)j / (double)M;
+        ynd[j] = std::tanh(STR2 * (x2 - 0.5)) / std::tanh(STR2 * 0.5);
+    }
+    for (int j = 0; j < M; ++j) {
+        ycc[j] = (ynd[j] + ynd[j + 1]) * 0.5;
+        dyfi[j] = 1.0 / (ynd[j + 1] - ynd[j]);
+        // channel: a = 1, b = 0, PCOE = 3/2
+        vini[j] = (ycc[j] < 1.0 && ycc[j] > -1.0) ? (1.0 - (ycc[j] * ycc[j]) / 1.0 / 1.0) * 1.5 : 0.0;
+    }
+    const size_t hs = (size_t)N * (M + 2) * L;
+    auto at = [&](int c, int k, int j, int i) -> double & { return q[c * hs + ((size_t)k * (M + 2) + j) * L + i]; };
+    std::fill(q, q + 3 * hs, 0.0);
+    std::fill(pr, pr + hs, 0.0);
+    // IniField_FLOW_tg: random perturbations (types 1, 2) or the sine field (type 3)
+    if (random_type != 3) {
+        for (int j = 1; j <= M; ++j) {
+            const double vperq = (1.0 - std::fabs(ycc[j - 1])) < 0.25 ? svperg : vperg;
+            for (int i = 0; i < L; ++i)
+                for (int k = 0; k < N; ++k) {
+                    // type 2 reseeds every point with 1973 + I + K + JJ (1-based), so its field is a
+                    // function of I + K + JJ; type 1 draws every point independently
+                    const uint64_t rd = (uint64_t)(i + 1) + (uint64_t)(k + 1) + (uint64_t)j;
+                    const uint64_t key = random_type == 2 ? mix64(seed ^ 0x5bd1e995ull) + 1973ull + rd
+                                                          : mix64(seed) ^ ((((uint64_t)j * (uint64_t)N + (uint64_t)k)
+                                                                            * (uint64_t)L + (uint64_t)i) + 1ull);
+                    double x[3];
+                    rvec3(key, x);
+                    at(0, k, j, i) = vperq * x[0];
+                    at(1, k, j, i) = vperq * x[1];   // RNDI1 = 1 in the channel
+                    at(2, k, j, i) = vperq * x[2];   // RCCI1 = 1
+                }
+            if (j == 1)
+                for (int k = 0; k < N; ++k)
+                    for (int i = 0; i < L; ++i) at(1, k, j, i) = 0.0;
+        }
+    } else {
+        for (int i = 0; i < L; ++i)
+            for (int k = 0; k < N; ++k)
+                for (int j = 1; j <= M; ++j) {
+                    at(0, k, j, i) = std::sin(xnd[i]) * std::cos(ycc[j - 1]) * std::cos(zcc[k]);
+                    at(1, k, j, i) = -std::cos(xcc[i]) * std::sin(ynd[j - 1]) * std::cos(zcc[k]);
+                    at(2, k, j, i) = 0.0;
+                    pr[((size_t)k * (M + 2) + j) * L + i] = (std::cos(2.0 * xcc[i]) + std::cos(2.0 * ycc[j - 1])) *
+                                                            (std::cos(2.0 * zcc[k] + 2.0)) / 16.0;
+                }
+    }
+    // INITIAL_MEAN_TG, then Q = Q - <Q>_xz row by row: u and w averaged at the cell centre
+    // ((Q(IP) + Q(I)) / 2, (Q(KP) + Q(K)) / 2), v as stored
+    for (int c = 0; c < 3; ++c)
+        for (int j = 1; j <= M; ++j) {
+            double s = 0.0;
+            for (int i = 0; i < L; ++i)
+                for (int k = 0; k < N; ++k) {
+                    const double ve = c == 0   ? (at(0, k, j, (i + 1) % L) + at(0, k, j, i)) * 0.5
+                                      : c == 2 ? (at(2, (k + 1) % N, j, i) + at(2, k, j, i)) * 0.5
+                                               : at(1, k, j, i);
+                    s = ve + s;
+                }
+            const double mean = s / (double)(L * N);
+            for (int k = 0; k < N; ++k)
+                for (int i = 0; i < L; ++i) at(c, k, j, i) = at(c, k, j, i) - mean;
+        }
+    // u + Vini; v = 0 on the lower wall (row 1); u / bulk velocity; Q * M_inlet / D_inlet
+    for (int j = 1; j <= M; ++j)
+        for (int i = 0; i < L; ++i)
+            for (int k = 0; k < N; ++k) at(0, k, j, i) = at(0, k, j, i) + vini[j - 1];
+    for (int k = 0; k < N; ++k)
+        for (int i = 0; i < L; ++i) at(1, k, 1, i) = 0.0;
+    double um = 0.0;
+    for (int j = 1; j <= M; ++j)
+        for (int i = 0; i < L; ++i)
+            for (int k = 0; k < N; ++k) um = um + at(0, k, j, i) / dyfi[j - 1];
+    um = um / (1.0 / dz);
+    double area = 0.0;
+    for (int j = 0; j < M; ++j)
+        for (int k = 0; k < N; ++k) area = area + 1.0 / dyfi[j] / (1.0 / dz);
+    um = um / (area * (double)L);
+    for (int j = 1; j <= M; ++j)
+        for (int k = 0; k < N; ++k)
+            for (int i = 0; i < L; ++i) at(0, k, j, i) = at(0, k, j, i) / um;
+    for (size_t o = 0; o < 3 * hs; ++o) q[o] = q[o] * m_over_d;
+    // INTFC_VARS3 (:93) exchanges halo rows around the periodic rank ring, so when the NS = 0
+    // substep starts, the wall halo rows hold the opposite wall's first row: row 0 = row M,
+    // row M + 1 = row 1 (that substep's convection reads them before BC_WALL_Q_tg)
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < N; ++k)
+            for (int i = 0; i < L; ++i) {
+                at(c, k, 0, i) = at(c, k, M, i);
+                at(c, k, M + 1, i) = at(c, k, 1, i);
+            }
     return CHS_OK;
 }

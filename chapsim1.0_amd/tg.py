@@ -83,6 +83,8 @@ SIGNATURES = {
     "chs_restart_last_error": (C.c_char_p, []),
     "chs_interp_initial_tg": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp,
                                         C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+    "chs_random_field_tg": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_double, C.c_double, C.c_ulonglong, C.c_double, _dp, _dp]),
     "chs_io_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "chs_io_destroy": (None, [C.c_void_p]),
     "chs_io_last_error": (C.c_char_p, [C.c_void_p]),
@@ -611,6 +613,23 @@ def interp_initial_tg(qo, po, HXoo, HZoo, yndo, L, M, N, HX=4 * np.pi, HZ=2 * np
     pr = np.zeros((N, M + 2, L))
     rc = lib().chs_interp_initial_tg(Lo, Mo, No, float(HXoo), float(HZoo), _p(yndo), *[_p(a) for a in parts],
                                      L, M, N, float(HX), float(HZ), float(STR2), _p(q), _p(pr))
+    if rc != 0:
+        raise ChsError(lib().chs_restart_last_error().decode())
+    return q, pr
+
+
+def random_field_tg(L, M, N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, random_type=2, vper=0.6, svper=0.3, seed=0,
+                    m_over_d=1.0):
+    """random_FL_FLD_TG (chs_random_field_tg): the start field before the NS = 0 projection, as
+    halo arrays q [3, N, M+2, L], pr [N, M+2, L].  random_type 3 is the reference's sine field;
+    1 and 2 draw the perturbations from a seeded counter-based generator in place of the
+    compiler's RANDOM_NUMBER (type 2 keeps the reference's dependence on I + K + J only).  The
+    velocity halo rows hold the opposite wall's first row, as the reference's ring exchange
+    leaves them for the projection substep."""
+    q = np.zeros((3, N, M + 2, L))
+    pr = np.zeros((N, M + 2, L))
+    rc = lib().chs_random_field_tg(L, M, N, float(HX), float(HZ), float(STR2), int(random_type), float(vper),
+                                   float(svper), int(seed), float(m_over_d), _p(q), _p(pr))
     if rc != 0:
         raise ChsError(lib().chs_restart_last_error().decode())
     return q, pr

@@ -359,6 +359,24 @@ int chs_interp_initial_tg(int Loo, int Mo, int Noo, double HXoo, double HZoo, co
                           const double *vo, const double *wo, const double *po, int L, int M, int N, double HX,
                           double HZ, double STR2, double *q, double *pr);
 
+/* Random start of the turbulence-generator channel, random_FL_FLD_TG (INITIAL_FL_FLD_TG.f90:
+ * 16-104): IniField_FLOW_tg (:174-244) fills rows 1..M with perturbations of amplitude VPERG,
+ * SVPERG where 1 - |y| < 1/4 (random_type 1 iRandomType real, 2 fixed), or with the sine field
+ * of random_type 3 (flag_random_sine, incl. its pressure); the x-z mean of every row is removed
+ * (INITIAL_MEAN_TG, processing/INITIAL_MEAN.f90:13-122), u gets the laminar profile Vini
+ * (LAMPOISLPROF.f90:29-51), v = 0 on the lower wall, u is divided by the bulk velocity
+ * (BULK_VELOCITY_TG) and all of Q multiplied by m_over_d = M_inlet / D_inlet (1 for the
+ * isothermal generator).  The perturbations of types 1 and 2 come from a seeded counter-based
+ * generator (splitmix64 of the point's key) in place of the compiler's RANDOM_NUMBER; type 2
+ * keeps the reference's per-point reseeding with 1973 + I + K + J, so its field depends on
+ * I + K + J only.  The result is not divergence-free: CALC_INITIALIZATION_tg
+ * (INITIAL_FL_FLD_TG.f90:107-140) follows with the NS = 0 substep, whose convection reads the
+ * velocity halo rows as the reference leaves them after its ring exchange (INTFC_VARS3, :93):
+ * row 0 = row M, row M+1 = row 1; the pressure halo rows are 0.  Output halo arrays
+ * q [3][N][M+2][L], pr [N][M+2][L].  Host only. */
+int chs_random_field_tg(int L, int M, int N, double HX, double HZ, double STR2, int random_type, double vperg,
+                        double svperg, unsigned long long seed, double m_over_d, double *q, double *pr);
+
 /* ---- Main-domain momentum with variable density and viscosity ------------------------ */
 
 /* Physical constants of the main-domain momentum substep (READINI / Flow_State set-up):

@@ -69,6 +69,8 @@
 !   interp                  INITIAL_INTERP_tg (needs iIniField_tg = 1, the coarse files in
 !                           FilePath3 and INTP_COORD_YND.ini), then the walls as
 !                           FLOWSTART.f90:37-40
+!   randomtg                random_FL_FLD_TG (iRandomType of the ini; before the projection
+!                           substep, which is 'substep 0 DT')
 !   tdma KIND FIN FOUT      rank 0: one of the reference's batched Thomas solvers on FIN
 !                           (int32 isz, jsz; A, B, C, R as isz x jsz float64, Fortran order;
 !                           for the non-cyclic kinds BC as nlines x 2 float64); KIND 0
@@ -346,6 +348,8 @@ PROGRAM ref_harness
             CALL INTFC_VARS3(1, NCL1_tg, 1, NCL1_tg, Q_tg)
             CALL BC_WALL_Q_tg
             CALL BC_WALL_PR_TG
+        CASE ('randomtg')
+            CALL random_FL_FLD_TG
         CASE ('tdma')
             READ(line, *) cmd, idr, fname2, fname
             IF (MYID == 0) CALL H_TDMA(idr, TRIM(fname2), TRIM(fname))
