@@ -799,7 +799,7 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
 // h_J = c_J pp_J; the (kx,kz) = (0,0) mode has RHS(J=1) = 0 and row 1 decoupled.  P is the
 // pivot type of the line (FP64, or FP32 for the well-conditioned modes of the mixed form);
 // the arithmetic is FP64 either way.
-template <typename T, typename P>
+template <typename T, typename P, int B>
 __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict__ amph,
                                            const double *__restrict__ apph, const P *__restrict__ p,
                                            const size_t pst, cpx<T> *__restrict__ s, const size_t st,
@@ -810,7 +810,6 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
     // Rows go in batches of B; the loads of the next batch are issued before the current
     // batch's recurrence (double buffer in registers), and pivots stay in their storage type
     // until used, so no load waits on a conversion.
-    constexpr int B = 8;
     cpx<T> rc[B], rn[B];
     P pc[B], pn[B];
     double gr = 0.0, gi = 0.0;
@@ -890,26 +889,29 @@ __device__ __forceinline__ void ytdma_line(const int M, const double *__restrict
 }
 
 // pp00: FISHPACK gauge only, the (0,0) mode's unmodified pivots (M values).
-template <typename T>
+// One thread per (kx, kz) line: (L/2+1) N lines, rows in batches of B.  The lines are few
+// (33 K at 256 x 256: half a wavefront per SIMD), so small blocks spread them over every CU
+// and longer batches keep more loads in flight per thread.
+template <typename T, int B>
 __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__ amph, const double *__restrict__ apph,
                                                const double *__restrict__ pp, const float *__restrict__ pp32,
                                                const double *__restrict__ pp00, cpx<T> *__restrict__ spec)
 {
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = g.H, N = g.N, M = g.M;
     if (t >= H * N) return;
     const int kx = t / N, kz = t - kx * N;
     const bool zero = (kx == 0 && kz == 0);
     const size_t st = (size_t)H * N, off = (size_t)kx * N + kz;
     if (zero && pp00) {
-        ytdma_line<T, double>(M, amph, apph, pp00, 1, spec, st, false, true);
+        ytdma_line<T, double, B>(M, amph, apph, pp00, 1, spec, st, false, true);
         return;
     }
     // FP32 pivots for all but the lowest modes (kx <= 1, |kz| <= 1), whose Neumann/near-
     // Neumann y-systems are ill-conditioned on the stretched mesh
     const bool low = !pp32 || (kx <= 1 && (kz <= 1 || kz == N - 1));
-    if (low) ytdma_line<T, double>(M, amph, apph, pp + off, st, spec + off, st, zero, false);
-    else ytdma_line<T, float>(M, amph, apph, pp32 + off, st, spec + off, st, zero, false);
+    if (low) ytdma_line<T, double, B>(M, amph, apph, pp + off, st, spec + off, st, zero, false);
+    else ytdma_line<T, float, B>(M, amph, apph, pp32 + off, st, spec + off, st, zero, false);
 }
 
 // Inverse real x-transform of TK z-planes at row J -> phi (halo field) and the Neumann
@@ -1419,6 +1421,7 @@ struct chs_ctx {
         int fft_reg = 1;                // register FFTs where L / N have a plan (0 = LDS Stockham)
         int cn_xz = 0, cnxz_by = 1;     // fused x+z CN kernel (1; slower on MI355X), rows per block
         int conv_fx = 1;                // x-solve inside k_conv_rhs (rows per block; 0 = off)
+        int ytdma_bs = 64, ytdma_rb = 16;   // k_ytdma threads per block, rows per load batch
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
     bool have_fields = false;
@@ -1737,7 +1740,9 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
     }
     {
         EvScope ev(c, "ytdma");
-        hipLaunchKernelGGL(k_ytdma<T>, dim3((H * N + 255) / 256), dim3(256), 0, c->stream, g, c->d_amph,
+        const int bs = c->tune.ytdma_bs;
+        auto kern = c->tune.ytdma_rb == 8 ? k_ytdma<T, 8> : k_ytdma<T, 16>;
+        hipLaunchKernelGGL(kern, dim3((H * N + bs - 1) / bs), dim3(bs), 0, c->stream, g, c->d_amph,
                            c->d_apph, c->pp, c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr,
                            gauge ? (const double *)c->pp00 : (const double *)nullptr, spec);
     }
@@ -2406,6 +2411,12 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "nxcd") {
         if (value < 1 || value > 64) return bad("must be 1..64");
         c->tune.nxcd = value;
+    } else if (k == "ytdma_bs") {
+        if (value != 64 && value != 128 && value != 256) return bad("must be 64, 128 or 256");
+        c->tune.ytdma_bs = value;
+    } else if (k == "ytdma_rb") {
+        if (value != 8 && value != 16) return bad("must be 8 or 16");
+        c->tune.ytdma_rb = value;
     } else if (k == "fftz_nl") {
         if (value < 0 || value * N > chs::kMaxPts * 256) return bad("value*N must be <= 2048");
         c->tune.fftz_nl = value;

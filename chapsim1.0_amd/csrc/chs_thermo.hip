@@ -8,6 +8,7 @@
 // is restated in the same operation order for the same reason.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <utility>
@@ -27,28 +28,58 @@ struct SplineDev {
     int imin, imax;   // 1-based, as IMIN_*/IMAX_* of the reference
 };
 
-// spline_interpolation_* (thermo_properties_table_searching.f90:678-723 and siblings):
-// range check (tolerance 1e-8), bisection I = 1, J = N+1, then
-// Y(I) + DX*(B(I) + DX*(C(I) + DX*D(I))).
-__device__ double spl_eval(const SplineDev &s, int n, double xv, int *err, int bit)
-{
-#pragma clang fp contract(off)
-    if ((xv - s.x[s.imin - 1]) < -1.0e-8) atomicOr(err, bit);
-    if ((xv - s.x[s.imax - 1]) > 1.0e-8) atomicOr(err, bit << 8);
-    int I = 1, J = n + 1;
-    while (J > I + 1) {
-        const int K = (I + J) / 2;
-        if (xv < s.x[K - 1]) J = K;
-        else I = K;
-    }
-    const double dx = xv - s.x[I - 1];
-    return s.y[I - 1] + dx * (s.b[I - 1] + dx * (s.c[I - 1] + dx * s.d[I - 1]));
-}
+// Bucket index over a non-decreasing searched table x: bucket b = floor((xv - x0) inv),
+// clamped to 0..nb-1, holds an index range lohi[2b] .. lohi[2b+1] that contains the
+// bisection's result for every xv of the bucket (the result, the count of x <= xv clamped to
+// 1..n, is monotone in xv; the range has one index of margin either side for the rounding of
+// the bucket edges).  Bisection inside that range gives the same index in one or two steps
+// instead of log2(n).  nb = 0: no index (table not monotone), full bisection.
+struct BucketDev {
+    const int *lohi;
+    double x0, inv;
+    int nb;
+};
 
 struct PropDev {
     int n;
     SplineDev dhh, ht, hd, hm, hk, hcp, hb;
+    BucketDev bdh, bh;   // over LIST_DH (dhh.x) and LIST_H (ht.x and the other H splines)
 };
+
+// The bisection of spline_interpolation_* (I = 1, J = N+1, thermo_properties_table_searching.f90:
+// 700-712), started from the bucket's index range when there is one.
+__device__ __forceinline__ int spl_index_b(const BucketDev &bk, const double *x, int n, double xv)
+{
+    int I = 1, J = n + 1;
+    if (bk.nb > 0) {
+        const double f = (xv - bk.x0) * bk.inv;
+        if (f == f) {   // a NaN searches the whole table, as the plain bisection
+            const int b = f <= 0.0 ? 0 : (f >= (double)(bk.nb - 1) ? bk.nb - 1 : (int)f);
+            const int2 r = reinterpret_cast<const int2 *>(bk.lohi)[b];
+            I = r.x;
+            J = r.y + 1;
+        }
+    }
+    while (J > I + 1) {
+        const int K = (I + J) / 2;
+        if (xv < x[K - 1]) J = K;
+        else I = K;
+    }
+    return I;
+}
+
+// spline_interpolation_* (thermo_properties_table_searching.f90:678-723 and siblings):
+// range check (tolerance 1e-8), bisection I = 1, J = N+1, then
+// Y(I) + DX*(B(I) + DX*(C(I) + DX*D(I))).
+__device__ double spl_eval(const SplineDev &s, const BucketDev &bk, int n, double xv, int *err, int bit)
+{
+#pragma clang fp contract(off)
+    if ((xv - s.x[s.imin - 1]) < -1.0e-8) atomicOr(err, bit);
+    if ((xv - s.x[s.imax - 1]) > 1.0e-8) atomicOr(err, bit << 8);
+    const int I = spl_index_b(bk, s.x, n, xv);
+    const double dx = xv - s.x[I - 1];
+    return s.y[I - 1] + dx * (s.b[I - 1] + dx * (s.c[I - 1] + dx * s.d[I - 1]));
+}
 
 // THERM_PROP_UPDATE_FROM_DH, table branch (THERM_PROP_UPDATE.f90:45-52), one thread per
 // value.  D is taken from the H->M spline, M from H->K and K from H->D, as the reference
@@ -58,14 +89,14 @@ __global__ __launch_bounds__(256) void k_prop_from_dh(PropDev p, long n, const d
 {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
-    const double h = spl_eval(p.dhh, p.n, dh[t], err, 1);
+    const double h = spl_eval(p.dhh, p.bdh, p.n, dh[t], err, 1);
     out[t] = h;
-    out[n + t] = spl_eval(p.ht, p.n, h, err, 2);
-    out[2 * n + t] = spl_eval(p.hm, p.n, h, err, 4);
-    out[3 * n + t] = spl_eval(p.hk, p.n, h, err, 8);
-    out[4 * n + t] = spl_eval(p.hd, p.n, h, err, 16);
-    out[5 * n + t] = spl_eval(p.hcp, p.n, h, err, 32);
-    out[6 * n + t] = spl_eval(p.hb, p.n, h, err, 64);
+    out[n + t] = spl_eval(p.ht, p.bh, p.n, h, err, 2);
+    out[2 * n + t] = spl_eval(p.hm, p.bh, p.n, h, err, 4);
+    out[3 * n + t] = spl_eval(p.hk, p.bh, p.n, h, err, 8);
+    out[4 * n + t] = spl_eval(p.hd, p.bh, p.n, h, err, 16);
+    out[5 * n + t] = spl_eval(p.hcp, p.bh, p.n, h, err, 32);
+    out[6 * n + t] = spl_eval(p.hb, p.bh, p.n, h, err, 64);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -228,19 +259,6 @@ __global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, dou
     e.rhs0[c0] = r;
 }
 
-// The bisection of spline_interpolation_* (I = 1, J = N+1, thermo_properties_table_searching.f90:
-// 700-712) on the table x (global or an LDS copy: same values, same index).
-__device__ __forceinline__ int spl_index(const double *x, int n, double xv)
-{
-    int I = 1, J = n + 1;
-    while (J > I + 1) {
-        const int K = (I + J) / 2;
-        if (xv < x[K - 1]) J = K;
-        else I = K;
-    }
-    return I;
-}
-
 __device__ __forceinline__ double spl_at(const SplineDev &s, int I, double xv)
 {
 #pragma clang fp contract(off)
@@ -257,10 +275,10 @@ __device__ void eng_cell_props(const EngDev &e, const PropDev &p, size_t h0, int
     const double dhv = e.dh[h0];
     if ((dhv - p.dhh.x[p.dhh.imin - 1]) < -1.0e-8) atomicOr(err, 1);
     if ((dhv - p.dhh.x[p.dhh.imax - 1]) > 1.0e-8) atomicOr(err, 1 << 8);
-    const double hv = spl_at(p.dhh, spl_index(xdh, p.n, dhv), dhv);
+    const double hv = spl_at(p.dhh, spl_index_b(p.bdh, xdh, p.n, dhv), dhv);
     if ((hv - p.ht.x[p.ht.imin - 1]) < -1.0e-8) atomicOr(err, 2 | 4 | 8 | 16 | 32);
     if ((hv - p.ht.x[p.ht.imax - 1]) > 1.0e-8) atomicOr(err, (2 | 4 | 8 | 16 | 32) << 8);
-    const int I = spl_index(xh, p.n, hv);
+    const int I = spl_index_b(p.bh, xh, p.n, hv);
     e.h[h0] = hv;
     e.t[h0] = spl_at(p.ht, I, hv);
     e.d[h0] = spl_at(p.hm, I, hv);    // D from H->M, as the reference
@@ -328,6 +346,7 @@ __global__ __launch_bounds__(256) void k_eng_walls(EngDev e, PropDev p, int *err
 struct chs_prop {
     int n = 0;
     double *dev = nullptr;   // all table arrays, packed
+    int *buckets = nullptr;  // bucket indexes of LIST_DH and LIST_H
     int *err = nullptr;
     PropDev p;
     hipStream_t stream = nullptr;
@@ -383,8 +402,10 @@ void eng_own_g(chs_eng *e)
 void eng_props_and_walls(chs_eng *e, int j0, int j1)
 {
     const dim3 blk(64, 4), grd((e->L + 63) / 64, (j1 - j0 + 4) / 4, e->N);
+    // without bucket indexes the bisections run on LDS copies of the two searched tables;
+    // with them, a search touches two or three entries, which the caches serve
     const size_t sh = 2 * sizeof(double) * (size_t)e->prop->p.n;
-    const int lds = sh <= 64 * 1024;
+    const int lds = sh <= 64 * 1024 && (e->prop->p.bdh.nb == 0 || e->prop->p.bh.nb == 0);
     hipLaunchKernelGGL(k_eng_props, grd, blk, lds ? sh : 0, e->stream, e->e, e->prop->p, j0, j1, e->err, lds);
     hipLaunchKernelGGL(k_eng_walls, dim3((e->L + 255) / 256, e->N), dim3(256), 0, e->stream, e->e, e->prop->p,
                        e->err);
@@ -637,6 +658,45 @@ int chs_prop_create(const chs_prop_table *t, chs_prop **out)
     const int ys[6] = {2, 3, 4, 5, 6, 7};   // T, D, M, K, Cp, B
     SplineDev *dst[6] = {&p->p.ht, &p->p.hd, &p->p.hm, &p->p.hk, &p->p.hcp, &p->p.hb};
     for (int c = 0; c < 6; ++c) *dst[c] = {A(1), A(ys[c]), A(11 + 3 * c), A(12 + 3 * c), A(13 + 3 * c), t->imin_h, t->imax_h};
+    // bucket indexes (2 n buckets) for the two searched tables where they are non-decreasing
+    const int nb = 2 * n;
+    std::vector<int> bk((size_t)4 * nb);
+    BucketDev *bd[2] = {&p->p.bdh, &p->p.bh};
+    const double *xs[2] = {t->dh, t->h};
+    for (int a = 0; a < 2; ++a) {
+        *bd[a] = {nullptr, 0.0, 0.0, 0};
+        const double *x = xs[a];
+        bool mono = x[n - 1] > x[0];
+        for (int i = 1; i < n && mono; ++i) mono = x[i] >= x[i - 1];
+        if (!mono) continue;
+        auto bis = [&](double xv) {
+            int I = 1, J = n + 1;
+            while (J > I + 1) {
+                const int K = (I + J) / 2;
+                if (xv < x[K - 1]) J = K;
+                else I = K;
+            }
+            return I;
+        };
+        const double x0 = x[0], w = (x[n - 1] - x[0]) / (double)nb;
+        int *r = bk.data() + (size_t)a * 2 * nb;
+        for (int b = 0; b < nb; ++b) {
+            r[2 * b] = b == 0 ? 1 : std::max(1, bis(x0 + (double)b * w) - 1);
+            r[2 * b + 1] = b == nb - 1 ? n : std::min(n, bis(x0 + (double)(b + 1) * w) + 1);
+        }
+        *bd[a] = {nullptr, x0, (double)nb / (x[n - 1] - x[0]), nb};
+    }
+    e = hipMalloc(&p->buckets, sizeof(int) * bk.size());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->buckets, bk.data(), sizeof(int) * bk.size(), hipMemcpyHostToDevice, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) {
+        g_prop_err = std::string("chs_prop_create: ") + hipGetErrorString(e);
+        chs_prop_destroy(p);
+        return CHS_E_HIP;
+    }
+    p->p.bdh.lohi = p->buckets;
+    p->p.bh.lohi = p->buckets + 2 * (size_t)nb;
     *out = p;
     return CHS_OK;
 }
@@ -646,6 +706,7 @@ void chs_prop_destroy(chs_prop *p)
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->dev) (void)hipFree(p->dev);
+    if (p->buckets) (void)hipFree(p->buckets);
     if (p->err) (void)hipFree(p->err);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;

@@ -17,6 +17,7 @@ KERNELS = {
     "cnz_kz": ("cn_z", "cn_xz"), "cny_ky": ("cn_y",), "fftx_tk": ("divg_fftx", "fftx_inv"),
     "fftz_nl": ("fftz_fwd", "fftz_inv"), "nxcd": ("conv_rhs",),
     "fft_reg": ("divg_fftx", "fftz_fwd", "fftz_inv", "fftx_inv"),
+    "ytdma_bs": ("ytdma",), "ytdma_rb": ("ytdma",),
 }
 
 
@@ -37,6 +38,8 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "fftz_nl": [n for n in (1, 2, 4, 8, 16) if n * N <= 2048],
         "fft_reg": [0, 1],
         "nxcd": [1, 8],
+        "ytdma_bs": [64, 128, 256],
+        "ytdma_rb": [8, 16],
     }
     return {k: v for k, v in c.items() if v}
 

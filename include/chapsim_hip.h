@@ -131,7 +131,9 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * (1 default, 2) of k_conv_rhs applying the x Crank-Nicolson inverse itself where L is 64,
  * 128, 192 or 256 (L * conv_fx <= 512, M divisible by conv_fx) and the inverse's band is
  * short, 0 for the separate x kernel (with conv_fx > 0, conv_bx / conv_by do not apply; the
- * mass-flux correction moves after the z-solve, which changes rounding only).  Results do not
+ * mass-flux correction moves after the z-solve, which changes rounding only), "ytdma_bs"
+ * threads per block of the Poisson y-solve (64 default, 128, 256) and "ytdma_rb" its rows per
+ * load batch (8, 16 default).  Results do not
  * depend on the shapes except through the summation order of the mass-flux partials
  * (conv_by, conv_kc, conv_fx) and conv_fx's rounding.  A captured step graph is re-made on
  * the next step.  CHS_E_ARG for an unknown key or a value out of range. */
