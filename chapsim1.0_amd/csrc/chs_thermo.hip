@@ -266,25 +266,100 @@ __device__ __forceinline__ double spl_at(const SplineDev &s, int I, double xv)
     return s.y[I - 1] + dx * (s.b[I - 1] + dx * (s.c[I - 1] + dx * s.d[I - 1]));
 }
 
-// THERM_PROP_UPDATE_FROM_DH for one cell.  The five H-based splines share their table
-// (LIST_H) and range (IMIN_H, IMAX_H), so one bisection serves all of them; each result equals
-// spl_eval's.  xdh / xh: LIST_DH / LIST_H (LDS copies or the global tables).
+// spl_index_b for C values in lockstep: the bisection steps are branch-free (a finished search
+// re-reads its own entry and keeps its bounds), so the C searches' loads go out together.
+// Two lockstep steps cover a bucket's index range unless the table clusters; the rest, if
+// any, runs as the plain loop.  Each result equals spl_index_b's.
+template <int C>
+__device__ __forceinline__ void spl_index_lock(const BucketDev &bk, const double *x, int n, const double *xv,
+                                               int *I)
+{
+    int J[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        I[c] = 1;
+        J[c] = n + 1;
+        if (bk.nb > 0) {
+            const double f = (xv[c] - bk.x0) * bk.inv;
+            if (f == f) {
+                const int b = f <= 0.0 ? 0 : (f >= (double)(bk.nb - 1) ? bk.nb - 1 : (int)f);
+                const int2 r = reinterpret_cast<const int2 *>(bk.lohi)[b];
+                I[c] = r.x;
+                J[c] = r.y + 1;
+            }
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        double xk[C];
+        int K[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            K[c] = (I[c] + J[c]) / 2;   // = I when the search is done (J = I + 1): a valid entry
+            xk[c] = x[K[c] - 1];
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const bool go = J[c] > I[c] + 1, lt = xv[c] < xk[c];
+            J[c] = (go && lt) ? K[c] : J[c];
+            I[c] = (go && !lt) ? K[c] : I[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        while (J[c] > I[c] + 1) {
+            const int K = (I[c] + J[c]) / 2;
+            if (xv[c] < x[K - 1]) J[c] = K;
+            else I[c] = K;
+        }
+}
+
+// THERM_PROP_UPDATE_FROM_DH for C cells (h0[c], those with act[c]).  The five H-based splines
+// share their table (LIST_H) and range (IMIN_H, IMAX_H), so one search serves all of them;
+// each result equals spl_eval's.  xdh / xh: LIST_DH / LIST_H (LDS copies or the global
+// tables).  An inactive cell searches the table's first entry and stores nothing.
+template <int C>
+__device__ void eng_cells_props(const EngDev &e, const PropDev &p, const size_t *h0, const bool *act, int *err,
+                                const double *xdh, const double *xh)
+{
+    double dv[C], hv[C];
+    int I[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) dv[c] = act[c] ? e.dh[h0[c]] : p.dhh.x[0];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        if (!act[c]) continue;
+        if ((dv[c] - p.dhh.x[p.dhh.imin - 1]) < -1.0e-8) atomicOr(err, 1);
+        if ((dv[c] - p.dhh.x[p.dhh.imax - 1]) > 1.0e-8) atomicOr(err, 1 << 8);
+    }
+    spl_index_lock<C>(p.bdh, xdh, p.n, dv, I);
+#pragma unroll
+    for (int c = 0; c < C; ++c) hv[c] = act[c] ? spl_at(p.dhh, I[c], dv[c]) : p.ht.x[0];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        if (!act[c]) continue;
+        if ((hv[c] - p.ht.x[p.ht.imin - 1]) < -1.0e-8) atomicOr(err, 2 | 4 | 8 | 16 | 32);
+        if ((hv[c] - p.ht.x[p.ht.imax - 1]) > 1.0e-8) atomicOr(err, (2 | 4 | 8 | 16 | 32) << 8);
+    }
+    spl_index_lock<C>(p.bh, xh, p.n, hv, I);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        if (!act[c]) continue;
+        const size_t o = h0[c];
+        e.h[o] = hv[c];
+        e.t[o] = spl_at(p.ht, I[c], hv[c]);
+        e.d[o] = spl_at(p.hm, I[c], hv[c]);    // D from H->M, as the reference
+        e.m[o] = spl_at(p.hk, I[c], hv[c]);    // M from H->K
+        e.k[o] = spl_at(p.hd, I[c], hv[c]);    // K from H->D
+        e.cp[o] = spl_at(p.hcp, I[c], hv[c]);
+    }
+}
+
 __device__ void eng_cell_props(const EngDev &e, const PropDev &p, size_t h0, int *err, const double *xdh,
                                const double *xh)
 {
-    const double dhv = e.dh[h0];
-    if ((dhv - p.dhh.x[p.dhh.imin - 1]) < -1.0e-8) atomicOr(err, 1);
-    if ((dhv - p.dhh.x[p.dhh.imax - 1]) > 1.0e-8) atomicOr(err, 1 << 8);
-    const double hv = spl_at(p.dhh, spl_index_b(p.bdh, xdh, p.n, dhv), dhv);
-    if ((hv - p.ht.x[p.ht.imin - 1]) < -1.0e-8) atomicOr(err, 2 | 4 | 8 | 16 | 32);
-    if ((hv - p.ht.x[p.ht.imax - 1]) > 1.0e-8) atomicOr(err, (2 | 4 | 8 | 16 | 32) << 8);
-    const int I = spl_index_b(p.bh, xh, p.n, hv);
-    e.h[h0] = hv;
-    e.t[h0] = spl_at(p.ht, I, hv);
-    e.d[h0] = spl_at(p.hm, I, hv);    // D from H->M, as the reference
-    e.m[h0] = spl_at(p.hk, I, hv);    // M from H->K
-    e.k[h0] = spl_at(p.hd, I, hv);    // K from H->D
-    e.cp[h0] = spl_at(p.hcp, I, hv);
+    const bool act = true;
+    eng_cells_props<1>(e, p, &h0, &act, err, xdh, xh);
 }
 
 // THERM_PROP_UPDATE (THERM_PROP_UPDATE.f90:124-168) over rows j0..j1.  With lds != 0 the two
@@ -302,11 +377,14 @@ __global__ __launch_bounds__(256) void k_eng_props(EngDev e, PropDev p, int j0, 
         xdh = tab;
         xh = tab + p.n;
     }
+    // two cells per thread, z-planes k and k + gridDim.z, searched in lockstep
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = j0 + blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const int k = blockIdx.z, k2 = k + gridDim.z;
     if (i >= e.L || J > j1) return;
-    eng_cell_props(e, p, ehx(e, i, J, k), err, xdh, xh);
+    const size_t h0[2] = {ehx(e, i, J, k), ehx(e, i, J, k2 < e.N ? k2 : k)};
+    const bool act[2] = {true, k2 < e.N};
+    eng_cells_props<2>(e, p, h0, act, err, xdh, xh);
 }
 
 // BC_WALL_THERMAL(IMAIND) (BC_WALL.f90:622-643): iso-flux walls extrapolate DH to the ghost
@@ -401,7 +479,7 @@ void eng_own_g(chs_eng *e)
 
 void eng_props_and_walls(chs_eng *e, int j0, int j1)
 {
-    const dim3 blk(64, 4), grd((e->L + 63) / 64, (j1 - j0 + 4) / 4, e->N);
+    const dim3 blk(64, 4), grd((e->L + 63) / 64, (j1 - j0 + 4) / 4, (e->N + 1) / 2);
     // without bucket indexes the bisections run on LDS copies of the two searched tables;
     // with them, a search touches two or three entries, which the caches serve
     const size_t sh = 2 * sizeof(double) * (size_t)e->prop->p.n;

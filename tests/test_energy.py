@@ -126,3 +126,39 @@ def test_gpu_energy_matches_oracle_large(pkg, table):
     for k, kk in FIELDS:
         assert rel(out[kk], o.f[k]) < TOL, (kk, rel(out[kk], o.f[k]))
     e.close()
+
+
+@pytest.mark.gpu
+def test_gpu_cell_props_equal_lookup(pkg, table):
+    """The energy context's property update (two cells per thread, searches in lockstep over the
+    bucket index) gives bit for bit what the single-value lookup gives, for DH at the table's
+    knots, between them, and at and beside the bucket edges."""
+    thermo, t = table
+    L, M, N, STR2 = 64, 48, 32, 1.9
+    x = np.asarray(t["DH"])
+    lo, hi = t["IMIN_DH"] - 1, t["IMAX_DH"] - 1
+    nb = 2 * len(x)
+    edges = x[0] + np.arange(nb) * ((x[-1] - x[0]) / nb)
+    vals = np.concatenate([x[lo:hi + 1], 0.5 * (x[lo:hi] + x[lo + 1:hi + 1]), edges,
+                           np.nextafter(edges, -np.inf), np.nextafter(edges, np.inf)])
+    vals = vals[(vals >= x[lo]) & (vals <= x[hi])]
+    cells = np.resize(vals, N * M * L).reshape(N, M, L)
+    dh = np.zeros((N, M + 2, L))
+    dh[:, 1:M + 1] = cells
+    dh[:, 0], dh[:, M + 1] = cells[:, 0], cells[:, -1]
+    REN = 2800.0
+    cthecd = 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"])
+    Tw = 675.0 / t["T0"]
+    hw = float(np.interp(Tw, t["T"], t["H"]))
+    ws = [[Tw, hw, float(np.interp(hw, t["H"], t["D"])), float(np.interp(hw, t["H"], t["M"])),
+           float(np.interp(hw, t["H"], t["K"]))]] * 2
+    e = thermo.EnergyTG(pkg.channel_setup(L, M, N, STR2=STR2), t, (2, 2), cthecd, x[lo], x[hi], (0.0, 0.0),
+                        wall_state=ws)
+    e.upload(dh, np.zeros((3, N, M + 2, L)))
+    out = e.download()
+    e.close()
+    pt = thermo.PropTable(t)
+    ref = pt.from_dh(cells.ravel())
+    pt.close()
+    for i, kk in enumerate(("H", "T", "D", "M", "K", "CP")):
+        np.testing.assert_array_equal(out[kk][:, 1:M + 1].ravel(), ref[i], err_msg=kk)
