@@ -195,17 +195,18 @@ __device__ __forceinline__ size_t ehx(const EngDev &e, int i, int J, int k)
 }
 
 // RHS_ENERGY_EXPLICIT (RHS_ENERGY_EXPLICIT.f90:43-150) and the RK update with the table
-// limiter (SOLVERRK3_ENG_IO.f90:35-47): DH += ca R + cb R0, clamp, R0 = R.
-__global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, double cb)
+// limiter (SOLVERRK3_ENG_IO.f90:35-47): DH += ca R + cb R0, clamp, R0 = R.  Each thread
+// marches KC z-planes: H, K, T at (i, J) of plane k and the plane-(k+1) values of H, K, T,
+// G_z read for plane k's z-differences stay in registers for the next plane.
+__global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, double cb, int KC)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z;
     if (i >= e.L || J > e.M) return;
     const int L = e.L, N = e.N, M = e.M;
+    const int k0 = blockIdx.z * KC, k1 = min(N, k0 + KC);
     const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
-    const int kp = (k == N - 1) ? 0 : k + 1, km = (k == 0) ? N - 1 : k - 1;
     const double one = 1.0, xnd2cl = 0.5, znd2cl = 0.5;
     const double coe1 = e.dxi * xnd2cl * 0.5;
     const double coe11 = e.dxqi * xnd2cl * e.cthecd;
@@ -227,36 +228,42 @@ __global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, dou
     const double wffp = e.wff[J + 1], wfbp = e.wfb[J + 1], wff = e.wff[J], wfb = e.wfb[J];
     const double *H = e.h, *T = e.t, *K = e.k;
     auto X = [&](const double *f, int ii, int jj, int kk) { return f[ehx(e, ii, jj, kk)]; };
-    double dx_dhu = (X(H, ip, J, k) + X(H, i, J, k)) * (X(e.g1, ip, J, k) + X(e.g1, ip, J, k)) -
-                    (X(H, i, J, k) + X(H, im, J, k)) * (X(e.g1, i, J, k) + X(e.g1, i, J, k));
-    dx_dhu = dx_dhu * coe1;
-    double dy_dhv = (wffp * X(H, i, J + 1, k) + wfbp * X(H, i, J, k)) * (X(e.g2, i, J + 1, k) + X(e.g2, i, J + 1, k)) -
-                    (wff * X(H, i, J, k) + wfb * X(H, i, J - 1, k)) * (X(e.g2, i, J, k) + X(e.g2, i, J, k));
-    dy_dhv = dy_dhv * coe2;
-    double dz_dhw = (X(H, i, J, kp) + X(H, i, J, k)) * (X(e.g3, i, J, kp) + X(e.g3, i, J, kp)) -
-                    (X(H, i, J, k) + X(H, i, J, km)) * (X(e.g3, i, J, k) + X(e.g3, i, J, k));
-    dz_dhw = dz_dhw * coe3;
-    double dx_kdt = (X(K, ip, J, k) + X(K, i, J, k)) * (X(T, ip, J, k) - X(T, i, J, k)) -
-                    (X(K, i, J, k) + X(K, im, J, k)) * (X(T, i, J, k) - X(T, im, J, k));
-    dx_kdt = dx_kdt * coe11;
-    double kdty1 = (wffp * X(K, i, J + 1, k) + wfbp * X(K, i, J, k)) * (X(T, i, J + 1, k) - X(T, i, J, k)) * coe221;
-    double kdty2 = (wff * X(K, i, J, k) + wfb * X(K, i, J - 1, k)) * (X(T, i, J, k) - X(T, i, J - 1, k)) * coe222;
-    if (J == 1 && e.wall_type0 == 1)
-        kdty2 = -e.wallflux[(size_t)k * L + i] * coe222;
-    else if (J == M && e.wall_type1 == 1)
-        kdty1 = -e.wallflux[((size_t)N + k) * L + i] * coe221;
-    const double dy_kdt = (kdty1 - kdty2) * coe22;
-    double dz_kdt = (X(K, i, J, kp) + X(K, i, J, k)) * (X(T, i, J, kp) - X(T, i, J, k)) -
-                    (X(K, i, J, km) + X(K, i, J, k)) * (X(T, i, J, k) - X(T, i, J, km));
-    dz_kdt = dz_kdt * coe33;
-    const double r = -dx_dhu - dy_dhv - dz_dhw + dx_kdt + dy_kdt + dz_kdt;
-    const size_t c0 = ((size_t)k * M + (J - 1)) * L + i;
-    const size_t h0 = ehx(e, i, J, k);
-    double dh = e.dh[h0] + ca * r + cb * e.rhs0[c0];
-    dh = fmin(e.dhmax - 1.0e-14, dh);   // REALMIN (modules.f90:56)
-    dh = fmax(e.dhmin + 1.0e-14, dh);
-    e.dh[h0] = dh;
-    e.rhs0[c0] = r;
+    const int kb = k0 == 0 ? N - 1 : k0 - 1;
+    double Hm = X(H, i, J, kb), Km = X(K, i, J, kb), Tm = X(T, i, J, kb);
+    double Hc = X(H, i, J, k0), Kc = X(K, i, J, k0), Tc = X(T, i, J, k0), G3c = X(e.g3, i, J, k0);
+    for (int k = k0; k < k1; ++k) {
+        const int kp = (k == N - 1) ? 0 : k + 1;
+        const double Hp = X(H, i, J, kp), Kp = X(K, i, J, kp), Tp = X(T, i, J, kp), G3p = X(e.g3, i, J, kp);
+        double dx_dhu = (X(H, ip, J, k) + Hc) * (X(e.g1, ip, J, k) + X(e.g1, ip, J, k)) -
+                        (Hc + X(H, im, J, k)) * (X(e.g1, i, J, k) + X(e.g1, i, J, k));
+        dx_dhu = dx_dhu * coe1;
+        double dy_dhv = (wffp * X(H, i, J + 1, k) + wfbp * Hc) * (X(e.g2, i, J + 1, k) + X(e.g2, i, J + 1, k)) -
+                        (wff * Hc + wfb * X(H, i, J - 1, k)) * (X(e.g2, i, J, k) + X(e.g2, i, J, k));
+        dy_dhv = dy_dhv * coe2;
+        double dz_dhw = (Hp + Hc) * (G3p + G3p) - (Hc + Hm) * (G3c + G3c);
+        dz_dhw = dz_dhw * coe3;
+        double dx_kdt = (X(K, ip, J, k) + Kc) * (X(T, ip, J, k) - Tc) - (Kc + X(K, im, J, k)) * (Tc - X(T, im, J, k));
+        dx_kdt = dx_kdt * coe11;
+        double kdty1 = (wffp * X(K, i, J + 1, k) + wfbp * Kc) * (X(T, i, J + 1, k) - Tc) * coe221;
+        double kdty2 = (wff * Kc + wfb * X(K, i, J - 1, k)) * (Tc - X(T, i, J - 1, k)) * coe222;
+        if (J == 1 && e.wall_type0 == 1)
+            kdty2 = -e.wallflux[(size_t)k * L + i] * coe222;
+        else if (J == M && e.wall_type1 == 1)
+            kdty1 = -e.wallflux[((size_t)N + k) * L + i] * coe221;
+        const double dy_kdt = (kdty1 - kdty2) * coe22;
+        double dz_kdt = (Kp + Kc) * (Tp - Tc) - (Km + Kc) * (Tc - Tm);
+        dz_kdt = dz_kdt * coe33;
+        const double r = -dx_dhu - dy_dhv - dz_dhw + dx_kdt + dy_kdt + dz_kdt;
+        const size_t c0 = ((size_t)k * M + (J - 1)) * L + i;
+        const size_t h0 = ehx(e, i, J, k);
+        double dh = e.dh[h0] + ca * r + cb * e.rhs0[c0];
+        dh = fmin(e.dhmax - 1.0e-14, dh);   // REALMIN (modules.f90:56)
+        dh = fmax(e.dhmin + 1.0e-14, dh);
+        e.dh[h0] = dh;
+        e.rhs0[c0] = r;
+        Hm = Hc; Km = Kc; Tm = Tc;
+        Hc = Hp; Kc = Kp; Tc = Tp; G3c = G3p;
+    }
 }
 
 __device__ __forceinline__ double spl_at(const SplineDev &s, int I, double xv)
@@ -604,8 +611,9 @@ int chs_substep_eng(chs_eng *e, int ns, double dt)
     // (:27) is never read by the reference and is not kept.
     std::swap(e->e.d, e->d0);
     std::swap(e->e.m, e->m0);
-    const dim3 blk(64, 4), grd((e->L + 63) / 64, (e->M + 3) / 4, e->N);
-    hipLaunchKernelGGL(k_eng_rhs_update, grd, blk, 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt);
+    constexpr int KC = 8;
+    const dim3 blk(64, 4), grd((e->L + 63) / 64, (e->M + 3) / 4, (e->N + KC - 1) / KC);
+    hipLaunchKernelGGL(k_eng_rhs_update, grd, blk, 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt, KC);
     eng_props_and_walls(e, 1, e->M);
     return eng_check(e, "chs_substep_eng");
 }
