@@ -471,6 +471,9 @@ def bench_heated(pkg, args):
     f0 = eng.download()
     vis = 1 if args.vis == "cn" else 0
     io = pkg.ChannelIO(co, c["REN"], flow_driven=1, vis_scheme=vis)
+    io_shapes = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set}
+    if io_shapes:
+        io.set_tuning(**io_shapes)
     io.upload(G=g, PR=pr, D0=f0["D"], M0=f0["M"])
     dt = c["dt"]
 

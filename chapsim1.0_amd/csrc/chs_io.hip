@@ -430,14 +430,19 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__r
 // All three momentum components of a point in one pass, so G, Q, the viscosities and the
 // pressure are read once for the three; the block's partial of sum(RHS_u / DYFI) for the
 // mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265) is written as well.
+// A 64 x 4 block covers TY rows of TZ = 4 / TY z-planes (TY 4, 2 or 1), so the z-neighbour
+// values a point reads belong to the block's own other planes when TY < 4.
 template <bool CN>
 __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restrict__ gs, double tg, double tr, double ta,
-                                                double dt)
+                                                double dt, int TY)
 {
     __shared__ double sh[256];
-    IO_THREAD(1)
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = 1 + (int)(blockIdx.y * TY + threadIdx.y % TY);
+    const int k = (int)(blockIdx.z * (4 / TY) + threadIdx.y / TY);
+    const int L = d.L, M = d.M, N = d.N;
     double contrib = 0.0;
-    if (i < L && J <= M) {
+    if (i < L && J <= M && k < N) {
         contrib = io_mom_point<0, CN>(d, gs, i, J, k, tg, tr, ta, dt);
         if (J >= 2) io_mom_point<1, CN>(d, gs, i, J, k, tg, tr, ta, dt);
         io_mom_point<2, CN>(d, gs, i, J, k, tg, tr, ta, dt);
@@ -1210,6 +1215,7 @@ struct chs_io {
     IoDev d{};
     double *buf = nullptr;
     int npart = 0, flow_driven = 1;
+    int mom_rows = 4;   // rows per momentum block (4, 2 or 1; the rest of its 4 rows of threads are z-planes)
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
     double *own_m1 = nullptr;                       // d.m1 likewise (Crank-Nicolson only)
     double *cnbuf = nullptr;                        // m1, work, amvr, apvr
@@ -1261,9 +1267,12 @@ int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
     }
     m->stream = chs_internal_stream(m->pois);
     const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L;
-    m->npart = (int)(((L + 63) / 64) * ((M + 3) / 4) * N);
+    m->npart = (int)(((L + 63) / 64) * ((M + 3) / 4) * N);   // partials of the 64 x 4 row-block kernels
+    // room for the largest block count of the row-block kernels and of every momentum shape
+    const size_t npart_alloc = (size_t)((L + 63) / 64) *
+        std::max({(size_t)((M + 3) / 4) * N, (size_t)((M + 1) / 2) * ((N + 1) / 2), (size_t)M * ((N + 3) / 4)});
     const size_t ng = 4 * (size_t)(M + 2);
-    const size_t nb = 18 * hs + 10 * cs + ng + m->npart + 4;
+    const size_t nb = 18 * hs + 10 * cs + ng + npart_alloc + 4;
     hipError_t he = hipMalloc(&m->buf, sizeof(double) * nb);
     if (he == hipSuccess) he = hipMemsetAsync(m->buf, 0, sizeof(double) * nb, m->stream);
     IoDev &d = m->d;
@@ -1276,7 +1285,7 @@ int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
     f += 10 * cs;
     double *geo = f;
     d.partial = f + ng;
-    d.scal = d.partial + m->npart;
+    d.scal = d.partial + npart_alloc;
     std::vector<double> hg(ng, 0.0);
     for (int J = 1; J <= M; ++J) hg[J] = c->dyfi[J - 1];
     for (int J = 1; J <= M + 1; ++J) {
@@ -1409,6 +1418,18 @@ int chs_io_props(chs_io *m, int ns)
     return io_hip(m, hipSuccess, "chs_io_props");
 }
 
+int chs_io_set_tuning(chs_io *m, const char *key, int value)
+{
+    if (!m || !key) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: bad argument");
+    const std::string k(key);
+    if (k == "mom_rows") {
+        if (value != 1 && value != 2 && value != 4) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: mom_rows must be 1, 2 or 4");
+        m->mom_rows = value;
+        return CHS_OK;
+    }
+    return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: unknown key " + k);
+}
+
 int chs_substep_io_momentum(chs_io *m, int ns, double dt)
 {
     if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io_momentum: bad argument");
@@ -1421,16 +1442,19 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
     if (!m->q_fresh) hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
     m->q_fresh = false;
+    const int TY = m->mom_rows, TZ = 4 / TY;
+    const dim3 gmom((m->L + 63) / 64, (m->M + TY - 1) / TY, (m->N + TZ - 1) / TZ);
+    const int npart = (int)(gmom.x * gmom.y * gmom.z);
     if (m->cn) {
-        hipLaunchKernelGGL(k_io_mom<true>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+        hipLaunchKernelGGL(k_io_mom<true>, gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
     } else {
         hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
-        hipLaunchKernelGGL(k_io_mom<false>, gin, blk, 0, m->stream, d, gs, tg, tr, ta, dt);
+        hipLaunchKernelGGL(k_io_mom<false>, gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
     }
     const double *dpg_dev = nullptr;
     double dpg = 0.0;
     if (m->flow_driven == 1) {
-        hipLaunchKernelGGL(k_io_reduce, dim3(1), dim3(256), 0, m->stream, d.partial, m->npart, m->volm, d.scal);
+        hipLaunchKernelGGL(k_io_reduce, dim3(1), dim3(256), 0, m->stream, d.partial, npart, m->volm, d.scal);
         dpg_dev = d.scal;
     } else {
         dpg = -0.5 * m->cf_given * (ta * dt);

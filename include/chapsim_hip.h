@@ -448,6 +448,13 @@ int chs_substep_io(chs_io *m, int ns, double dt);
  * checking the momentum solves.  Enqueued. */
 int chs_substep_io_momentum(chs_io *m, int ns, double dt);
 
+/* Launch shape of the main-domain momentum kernel, as chs_set_tuning is for the turbulence
+ * generator; no counterpart in the reference.  Key "mom_rows": rows per 64 x 4 block (4
+ * default, 2, 1; the block's other rows of threads take further z-planes).  Results change
+ * only through the summation order of the mass-flux partials.  CHS_E_ARG for an unknown key
+ * or value. */
+int chs_io_set_tuning(chs_io *m, const char *key, int value);
+
 /* Copy fields back (NULL skips one): g, q, dstg, mstg [3][N][M+2][L]; pr, dph [N][M+2][L]
  * (dph's ghost rows copy the wall cells, BC_WALL_DPH_io); explt0 [3][N][M][L]; drho, rhsp
  * [N][M][L].  Synchronous. */

@@ -172,11 +172,12 @@ def test_gpu_cn_substeps_match_reference(pkg, golden):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("L,M,N,steps", [(64, 48, 32, 3), (48, 36, 40, 3), (128, 64, 96, 1)])
-def test_gpu_cn_matches_oracle(pkg, L, M, N, steps):
+@pytest.mark.parametrize("L,M,N,steps,rows", [(64, 48, 32, 3, 4), (48, 36, 40, 3, 4), (128, 64, 96, 1, 4),
+                                              (64, 48, 32, 1, 2), (48, 37, 30, 1, 2), (48, 37, 30, 1, 1)])
+def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows):
     """Larger seeded states with a noisy current viscosity against the oracle (gravity and
-    constant mass flux on)."""
-    print(f"io cn oracle {L}x{M}x{N}", flush=True)
+    constant mass flux on), with each momentum block shape (rows per block; ragged M, N)."""
+    print(f"io cn oracle {L}x{M}x{N} rows {rows}", flush=True)
     mesh = setup.Mesh(L, M, N, STR2=2.0)
     st = oracle.io_initial_field(mesh, 90 + L)
     m1 = cn_viscosity(st, 90 + L, False)
@@ -184,6 +185,7 @@ def test_gpu_cn_matches_oracle(pkg, L, M, N, steps):
     o = oracle.OracleIO(mesh, REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
     o.set_state(dict(st, M1=m1))
     ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
+    ch.set_tuning(mom_rows=rows)
     ch.upload(**st, M1=m1)
     for _ in range(steps):
         for ns in (1, 2, 3):
@@ -372,3 +374,33 @@ def test_gpu_max_div_io(pkg, golden):
             ch.SOLVERRK3_MOM_io(1, c["dt"])
             assert ch.max_div() < 1e-9 * ref
             ch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows", [2, 1])
+def test_gpu_explicit_momentum_block_shapes(pkg, rows):
+    """Explicit viscous scheme: every momentum block shape gives the default shape's substeps
+    (only the mass-flux partials' summation order differs)."""
+    L, M, N = 48, 37, 30
+    print(f"io explicit rows {rows}", flush=True)
+    mesh = setup.Mesh(L, M, N, STR2=2.0)
+    st = oracle.io_initial_field(mesh, 71)
+    out = []
+    for r in (4, rows):
+        ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=1, f_a=0.02, flow_driven=1,
+                           vis_scheme=0)
+        ch.set_tuning(mom_rows=r)
+        ch.upload(**st)
+        for ns in (1, 2, 3):
+            ch.ioprops(ns)
+            ch.SOLVERRK3_MOM_io(ns, 5e-4)
+        out.append(ch.download())
+        ch.close()
+    for k in ("G", "Q"):
+        for c in range(3):
+            assert rel(out[1][k][c], out[0][k][c]) < 1e-13, (k, c)
+    assert rel(out[1]["PR"], out[0]["PR"]) < 1e-11
+    ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0)
+    with pytest.raises(pkg.ChsError):
+        ch.set_tuning(mom_rows=3)
+    ch.close()
