@@ -62,6 +62,16 @@ def test_random_types_structure(pkg, rtype):
     np.testing.assert_array_equal(q4, 2.0 * q)                 # Q * M_inlet / D_inlet
 
 
+def test_bad_arguments_are_errors(pkg):
+    for rtype in (0, 4):
+        with pytest.raises(pkg.ChsError):
+            pkg.random_field_tg(L, M, N, random_type=rtype)
+    with pytest.raises(pkg.ChsError):
+        pkg.random_field_tg(1, M, N)
+    with pytest.raises(pkg.ChsError):
+        pkg.random_field_tg(L, M, N, HX=0.0)
+
+
 def test_sine_field_closed_form(pkg):
     """The sine field's x-z means vanish over the periodic box (HX = 4 pi, HZ = 2 pi), so after
     the mean removal u is sin(x) cos(y) cos(z) plus the laminar profile, over the bulk velocity."""
