@@ -2086,7 +2086,14 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
         return fail(nullptr, CHS_E_ARG, "L, N must be even, <= 2048, with factors 2,3,5 only; M >= 4");
     chs_ctx *c = new chs_ctx();
     c->L = L; c->M = M; c->N = N; c->H = L / 2 + 1; c->mode = mode;
-    if (mode == CHS_MIXED) { c->tune.cnz_kz = 8; c->tune.cny_ky = 32; }   // FP32 stencils
+    if (mode == CHS_MIXED) {   // FP32 stencils
+        c->tune.cnz_kz = 8;
+        c->tune.cny_ky = 32;
+        // k_conv_rhs: loads of a plane ahead of its arithmetic and XCD-grouped tiles, 1.2 %
+        // faster per step at 256 x 192 x 256 in this form (no change in the FP64 form)
+        c->tune.conv_fence = 1;
+        c->tune.nxcd = 8;
+    }
     if (mode == CHS_MIXED) c->nref_projection = 2;
     c->px = px; c->pz = pz;
     c->dxi = cf->dxi; c->dzi = cf->dzi; c->dxqi = cf->dxqi; c->dzqi = cf->dzqi; c->volm = cf->volm;

@@ -117,13 +117,14 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
 /* Launch shapes of the substep kernels (no counterpart in the reference, whose loops have no
  * tiling).  Keys and limits: "conv_by" rows per k_conv_rhs block (1, 2, 4, 8, 16; default 4),
  * "conv_bx" wavefronts along x per k_conv_rhs block (1 or 4, conv_bx * conv_by <= 16; 1),
- * "conv_fence" 1 to keep all of a z-plane's loads ahead of its arithmetic in k_conv_rhs (0),
+ * "conv_fence" 1 to keep all of a z-plane's loads ahead of its arithmetic in k_conv_rhs (FP64 0,
+ * mixed 1),
  * "conv_kc" z-planes marched per k_conv_rhs thread (1..N; 16), "cnx_nlx" x-lines per CN-x
  * block (0 = 2048/L), "cnz_kz" / "cny_ky" march lengths of the z / y CN stencils (FP64 16 / 48,
  * mixed 8 / 32),
  * "fftx_tk" z-planes per x-transform block (0 = derived; even, divides N, TK/2*L <= 2048),
  * "fftz_nl" z-lines per z-transform block (0 = 2048/N; NL*N <= 2048), "nxcd" number of L2
- * domains (XCDs) the k_conv_rhs tiles are grouped for (1..64; default 1 = dispatch order),
+ * domains (XCDs) the k_conv_rhs tiles are grouped for (1..64; FP64 1 = dispatch order, mixed 8),
  * "fft_reg" 1 (default) to run the register-resident FFTs for line lengths 64 ... 1024, 0 for
  * the LDS Stockham transforms, "cn_xz" 1 to run the x and z Crank-Nicolson solves in one kernel
  * where a block can hold whole x-lines (default 0: two kernels, faster on MI355X at 256^2 x
