@@ -451,7 +451,9 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt);
 
 /* Launch shape of the main-domain momentum kernel, as chs_set_tuning is for the turbulence
  * generator; no counterpart in the reference.  Key "mom_rows": rows per 64 x 4 block (4
- * default, 2, 1; the block's other rows of threads take further z-planes).  Results change
+ * default, 2, 1; the block's other rows of threads take further z-planes); "cn_cols": most
+ * x-columns per block of the Crank-Nicolson z and y Jacobi solves (1..64, default 16; rounded
+ * down to whole wavefronts, the sequential solves where that leaves none).  Results change
  * only through the summation order of the mass-flux partials.  CHS_E_ARG for an unknown key
  * or value. */
 int chs_io_set_tuning(chs_io *m, const char *key, int value);

@@ -172,12 +172,15 @@ def test_gpu_cn_substeps_match_reference(pkg, golden):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("L,M,N,steps,rows", [(64, 48, 32, 3, 4), (48, 36, 40, 3, 4), (128, 64, 96, 1, 4),
-                                              (64, 48, 32, 1, 2), (48, 37, 30, 1, 2), (48, 37, 30, 1, 1)])
-def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows):
+@pytest.mark.parametrize("L,M,N,steps,rows,cols", [(64, 48, 32, 3, 4, 16), (48, 36, 40, 3, 4, 16),
+                                                   (128, 64, 96, 1, 4, 16), (64, 48, 32, 1, 2, 64),
+                                                   (48, 37, 30, 1, 2, 8), (48, 37, 30, 1, 1, 64),
+                                                   (128, 64, 96, 1, 4, 64)])
+def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows, cols):
     """Larger seeded states with a noisy current viscosity against the oracle (gravity and
-    constant mass flux on), with each momentum block shape (rows per block; ragged M, N)."""
-    print(f"io cn oracle {L}x{M}x{N} rows {rows}", flush=True)
+    constant mass flux on), with each momentum block shape (rows per block; ragged M, N) and
+    several line-solve block widths (x-columns per block)."""
+    print(f"io cn oracle {L}x{M}x{N} rows {rows} cols {cols}", flush=True)
     mesh = setup.Mesh(L, M, N, STR2=2.0)
     st = oracle.io_initial_field(mesh, 90 + L)
     m1 = cn_viscosity(st, 90 + L, False)
@@ -185,7 +188,7 @@ def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows):
     o = oracle.OracleIO(mesh, REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
     o.set_state(dict(st, M1=m1))
     ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
-    ch.set_tuning(mom_rows=rows)
+    ch.set_tuning(mom_rows=rows, cn_cols=cols)
     ch.upload(**st, M1=m1)
     for _ in range(steps):
         for ns in (1, 2, 3):
