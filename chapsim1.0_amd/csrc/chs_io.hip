@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <string>
 #include <utility>
@@ -25,6 +26,10 @@ void chs_internal_eng_set_g(chs_eng *e, const double *g);
 double *chs_internal_eng_density(chs_eng *e);
 
 namespace {
+
+// Maximum that propagates NaN (fmax drops a NaN operand, so a diverged field would report a
+// finite CFL or divergence).
+__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
 
 thread_local std::string g_io_err;
 
@@ -635,7 +640,7 @@ __device__ __forceinline__ double block_max_nonneg(double v, unsigned long long 
 {
     if (threadIdx.x == 0) *word = 0ull;
     __syncthreads();
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o));
     if ((threadIdx.x & 63) == 0) atomicMax(word, (unsigned long long)__double_as_longlong(v));
     __syncthreads();
     return __longlong_as_double((long long)*word);
@@ -898,7 +903,7 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
             b = 1.0;
         }
         ib[j] = 1.0 / b;
-        rho = fmax(rho, (fabs(a[j]) + fabs(c[j])) * fabs(ib[j]));
+        rho = nanmax(rho, (fabs(a[j]) + fabs(c[j])) * fabs(ib[j]));
     }
     rho = block_max_nonneg(rho, word);   // also orders the edge reads above before the sweeps
     if (!(rho <= 0.25)) {
@@ -1021,7 +1026,7 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
         a[j] = aa;
         c[j] = cc;
         ib[j] = 1.0 / b;
-        rho = fmax(rho, (fabs(aa) + fabs(cc)) * fabs(ib[j]));
+        rho = nanmax(rho, (fabs(aa) + fabs(cc)) * fabs(ib[j]));
     }
     rho = block_max_nonneg(rho, word);
     if (!(rho <= 0.25)) {
@@ -1146,13 +1151,13 @@ __global__ __launch_bounds__(256) void k_io_cfl(IoDev d, const double *__restric
                             fabs((Q[hs + c0] + Q[hs + hx(d, i, J + 1, k)]) * d.dyfi[J]) +
                             fabs((Q[2 * hs + c0] + Q[2 * hs + hx(d, i, J, kp)]) * d.dzi)) * 0.5;
         const double vcf = 2.0 * d.cvisc * (d.dxqi + d.dzqi + d.dyfi[J] * d.dyfi[J]) * visc[c0] / dens[c0];
-        v = fmax(qcf, vcf);
+        v = nanmax(qcf, vcf);
     }
     const int t = threadIdx.x + 64 * threadIdx.y;
     sh[t] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) sh[t] = fmax(sh[t], sh[t + o]);
+        if (t < o) sh[t] = nanmax(sh[t], sh[t + o]);
         __syncthreads();
     }
     if (t == 0) d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
@@ -1178,7 +1183,7 @@ __global__ __launch_bounds__(256) void k_io_divck(IoDev d)
     sh[t] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) sh[t] = fmax(sh[t], sh[t + o]);
+        if (t < o) sh[t] = nanmax(sh[t], sh[t + o]);
         __syncthreads();
     }
     if (t == 0) d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
@@ -1188,11 +1193,11 @@ __global__ __launch_bounds__(256) void k_io_reduce_max(const double *__restrict_
 {
     __shared__ double sh[256];
     double v = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) v = fmax(v, part[j]);
+    for (int j = threadIdx.x; j < n; j += 256) v = nanmax(v, part[j]);
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = nanmax(sh[threadIdx.x], sh[threadIdx.x + o]);
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = sh[0];
@@ -1566,6 +1571,7 @@ int io_cfl(chs_io *m, const double *dens, const double *visc, double dt, double 
     if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
     if (he != hipSuccess) return io_hip(m, he, what);
     *cfl = v * dt;
+    if (!std::isfinite(v)) return io_fail(m, CHS_E_NONFINITE, std::string(what) + ": non-finite velocity or property");
     return CHS_OK;
 }
 
@@ -1582,6 +1588,7 @@ int chs_max_div_io(chs_io *m, double *maxdiv)
     if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
     if (he != hipSuccess) return io_hip(m, he, "chs_max_div_io");
     *maxdiv = v;
+    if (!std::isfinite(v)) return io_fail(m, CHS_E_NONFINITE, "chs_max_div_io: non-finite divergence");
     return CHS_OK;
 }
 

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/chapsim_hip.h"
@@ -39,6 +40,10 @@ using chs::cpx;
 #define CHS_VERSION "chapsim_hip 0.1 (gfx950)"
 
 namespace {
+
+// Maximum that propagates NaN (fmax drops a NaN operand, so a diverged field would report a
+// finite CFL or divergence).
+__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
 
 // ---------------------------------------------------------------------------------------
 // Device-side geometry passed by value to every kernel.
@@ -266,6 +271,214 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
         __syncthreads();
     }
     if (t == 0) partial[tile] = red[0];
+}
+
+// k_conv_rhs_lds: the k_conv_rhs FX form (whole x-lines, x-solve applied in LDS, same
+// arithmetic) with the state staged through LDS, so that the stencils' neighbours cost LDS
+// reads instead of global loads.  Block = L = 64 BXW threads along x by BY rows J0..J0+BY-1,
+// marching KC z-planes.  For each plane the rows J0-1 .. J0+BY of u, v, w (and p, which needs
+// no row J0+BY) are loaded once, as whole coalesced rows, and kept in LDS for two planes: while
+// plane k is computed, the buffers hold planes k and k+1 (the x-, y- and z+1 neighbours); the
+// plane k-1 values a point needs are its own from the previous plane, kept in registers.  Plane
+// k+2 is loaded into registers one plane ahead and written over plane k's buffer after it has
+// been read.  Thread row t loads its own row (whose plane k+1 values are its u_p, v_p, w_p) and
+// a share of the two halo rows.
+template <typename W, int BY, int BXW>
+__global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_lds(Geo g, Sub s, const double *__restrict__ q,
+                                                          const double *__restrict__ pr, W *__restrict__ h0,
+                                                          W *__restrict__ rhs, double *__restrict__ partial,
+                                                          int KC, int nxcd, const W *__restrict__ gx, int Dx)
+{
+    constexpr int L = 64 * BXW, R = BY + 2, NT = L * BY;
+    constexpr int NH = 7, HPT = (NH + BY - 1) / BY;   // halo row-loads (4 at J0-1, 3 at J0+BY)
+    constexpr int XH = 16, LP = L + 2 * XH;            // RHS rows carry XH wrapped points each side
+    __shared__ double st[2][4][R][L];                  // [plane buffer][u v w p][row][x]
+    __shared__ W xs[3 * BY * LP];                       // one plane's three RHS row-blocks
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
+    // blockDim.x = L is a multiple of 64, so the row t is wave-uniform: scalar branches and bases
+    const int i = threadIdx.x, t = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int J0 = ty * BY + 1;
+    const int J = __builtin_amdgcn_readfirstlane(J0 + t);
+    const int N = g.N, M = g.M;
+    const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L;
+    const int kbeg = tz * KC, kend = min(N, kbeg + KC);
+    const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+    auto wrapk = [&](int kk) { return kk >= N ? kk - N : (kk < 0 ? kk + N : kk); };
+    // row (jj, kk) of field f (u, v, w, p); f may vary between wavefronts (halo shares), so it
+    // selects a base rather than indexing an array of pointers
+    auto grow = [&](int f, int jj, int kk) {
+        const double *b = (f == 3) ? pr : q + (size_t)f * hs;
+        return b + ((size_t)kk * (M + 2) + jj) * L;
+    };
+    // halo item h (h % BY == t): rows J0-1 (u, v, w, p) and J0+BY (u, v, w)
+    auto hrow = [&](int h) { return h < 4 ? 0 : BY + 1; };
+    auto hfld = [&](int h) { return h < 4 ? h : h - 4; };
+    double own[4], hal[HPT];
+    auto load_plane = [&](int kk) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) own[f] = grow(f, J, kk)[i];
+#pragma unroll
+        for (int e = 0; e < HPT; ++e) {
+            const int h = e * BY + t;
+            if (h < NH) hal[e] = grow(hfld(h), J0 - 1 + hrow(h), kk)[i];
+        }
+    };
+    auto store_plane = [&](int b) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) st[b][f][t + 1][i] = own[f];
+#pragma unroll
+        for (int e = 0; e < HPT; ++e) {
+            const int h = e * BY + t;
+            if (h < NH) st[b][hfld(h)][hrow(h)][i] = hal[e];
+        }
+    };
+    // RHS value of component cc at x = i, with its copies in the wrapped halo points
+    auto put = [&](int cc, W v) {
+        W *row = xs + (cc * BY + t) * LP + XH;
+        row[i] = v;
+        if (i < XH) row[L + i] = v;
+        if (i >= L - XH) row[i - L] = v;
+    };
+    double mf = 0.0;
+    {
+        const double wffJ = g.wff[J], wfbJ = g.wfb[J], wffP = g.wff[J + 1], wfbP = g.wfb[J + 1];
+        const double dyf = g.dyfi[J - 1], dyc = g.dyci[J];
+        const int o1 = M + 2, o2 = 2 * (M + 2);
+        const double apu = g.apvr[J], acu = g.acvr[J], amu = g.amvr[J];
+        const double apv = g.apvr[o1 + J], acv = g.acvr[o1 + J], amv = g.amvr[o1 + J];
+        const double apw = g.apvr[o2 + J], acw = g.acvr[o2 + J], amw = g.amvr[o2 + J];
+        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)s.dt;
+        const double coev = s.talp * s.cvisc, coep = s.talp * s.dt;
+        const W cu1 = (W)(0.25 * g.dxi), cu2 = (W)(dyf * 0.5), cu3 = (W)(0.25 * g.dzi);
+        const W cv1 = (W)(g.dxi * 0.5), cv2 = (W)(dyc * 0.25), cv3 = (W)(g.dzi * 0.5);
+        // prologue: planes kbeg, kbeg+1 into LDS; plane kbeg-1 values of this point into
+        // registers; plane kbeg+2 in flight
+        const int km0 = wrapk(kbeg - 1);
+        double u_m = grow(0, J, km0)[i], v_m = grow(1, J, km0)[i], w_m = grow(2, J, km0)[i];
+        double p_m = grow(3, J, km0)[i], uip_m = grow(0, J, km0)[ip], vjp_m = grow(1, J + 1, km0)[i];
+        load_plane(kbeg);
+        store_plane(0);
+        load_plane(wrapk(kbeg + 1));
+        store_plane(1);
+        if (kbeg + 2 <= kend) load_plane(wrapk(kbeg + 2));
+        __syncthreads();
+        const int r = t + 1;
+        double rsum = 0.0;
+        // one plane; CB (the buffer holding plane k) is a compile-time constant so that every
+        // LDS address is a lane offset plus an immediate
+        auto plane = [&](int k, auto cbc) {
+            constexpr int cb = decltype(cbc)::value, nb = cb ^ 1;
+            const size_t c0 = ((size_t)k * M + (J - 1)) * L;
+            const W hu = h0[c0 + i], hv = h0[cs + c0 + i], hw = h0[2 * cs + c0 + i];
+            const double(*U)[L] = st[cb][0], (*V)[L] = st[cb][1], (*Wf)[L] = st[cb][2], (*Pf)[L] = st[cb][3];
+            const double(*Wn)[L] = st[nb][2];
+            // values every component reads; the others are read from LDS inside each component's
+            // block (the fences keep the compiler from hoisting them, which costs registers)
+            const double u_c = U[r][i], v_c = V[r][i], w_c = Wf[r][i], p0 = Pf[r][i];
+            const double uip = U[r][ip], vjp = V[r + 1][i], ujm = U[r - 1][i], wim_c = Wf[r][im];
+            const double wjm_c = Wf[r - 1][i], w_p = Wn[r][i];
+            W cu, cv, cw;
+            // ---- u (CONVECTION_X.f90:31-59) ----
+            {
+                const double uim = U[r][im], ujp = U[r + 1][i], vimjp = V[r + 1][im], vim = V[r][im];
+                const double u_p = st[nb][0][r][i], wim_p = Wn[r][im], pim = Pf[r][im];
+                const W a1 = (W)(uip + u_c), a2 = (W)(uim + u_c);
+                const W H11 = (a1 * a1 - a2 * a2) * cu1;
+                const W H12 = ((W)(vjp + vimjp) * (W)(wffP * ujp + wfbP * u_c) -
+                               (W)(v_c + vim) * (W)(wffJ * u_c + wfbJ * ujm)) * cu2;
+                const W H13 = ((W)(w_p + wim_p) * (W)(u_p + u_c) - (W)(w_c + wim_c) * (W)(u_c + u_m)) * cu3;
+                const W conv = -(H11 + H12 + H13);
+                const W rhsc = tgam * conv + troh * hu;
+                const double lap = (uip - 2.0 * u_c + uim) * g.dxqi + (u_p - 2.0 * u_c + u_m) * g.dzqi +
+                                   (ujp * apu + u_c * acu + ujm * amu);
+                cu = conv;
+                const W pgm = (W)((p0 - pim) * g.dxi * coep);
+                const W rr = (rhsc + (W)(coev * lap)) * dt - pgm;
+                put(0, rr);
+                rsum += (double)rr;
+            }
+            asm volatile("" ::: "memory");
+            // ---- v (CONVECTION_Y.f90:37-84), stored for rows J >= 2 ----
+            {
+                const double uipjm = U[r - 1][ip], vip = V[r][ip], vim = V[r][im], vjm = V[r - 1][i];
+                const double v_p = st[nb][1][r][i], wjm_p = Wn[r - 1][i], pjm = Pf[r - 1][i];
+                const W H21 = ((W)(wffJ * uip + wfbJ * uipjm) * (W)(vip + v_c) -
+                               (W)(wffJ * u_c + wfbJ * ujm) * (W)(v_c + vim)) * cv1;
+                const W b1 = (W)(vjp + v_c), b2 = (W)(v_c + vjm);
+                const W H22 = (b1 * b1 - b2 * b2) * cv2;
+                const W H23 = ((W)(wffJ * w_p + wfbJ * wjm_p) * (W)(v_p + v_c) -
+                               (W)(wffJ * w_c + wfbJ * wjm_c) * (W)(v_c + v_m)) * cv3;
+                const W conv = -(H21 + H22 + H23);
+                const W rhsc = tgam * conv + troh * hv;
+                const double lap = (vip - 2.0 * v_c + vim) * g.dxqi + (v_p - 2.0 * v_c + v_m) * g.dzqi +
+                                   (vjp * apv + v_c * acv + vjm * amv);
+                cv = conv;
+                const W pgm = (W)((p0 - pjm) * dyc * coep);
+                put(1, (rhsc + (W)(coev * lap)) * dt - pgm);
+            }
+            asm volatile("" ::: "memory");
+            // ---- w (CONVECTION_Z.f90:33-69) ----
+            {
+                const double wip = Wf[r][ip], wjp = Wf[r + 1][i];
+                const W H31 = ((W)(uip + uip_m) * (W)(wip + w_c) - (W)(u_c + u_m) * (W)(w_c + wim_c)) * cu1;
+                const W H32 = ((W)(vjp + vjp_m) * (W)(wffP * wjp + wfbP * w_c) -
+                               (W)(v_c + v_m) * (W)(wffJ * w_c + wfbJ * wjm_c)) * cu2;
+                const W c1 = (W)(w_p + w_c), c2 = (W)(w_c + w_m);
+                const W H33 = (c1 * c1 - c2 * c2) * cu3;
+                const W conv = -(H31 + H32 + H33);
+                const W rhsc = tgam * conv + troh * hw;
+                const double lap = (wip - 2.0 * w_c + wim_c) * g.dxqi + (w_p - 2.0 * w_c + w_m) * g.dzqi +
+                                   (wjp * apw + w_c * acw + wjm_c * amw);
+                cw = conv;
+                const W pgm = (W)((p0 - p_m) * g.dzi * coep);
+                put(2, (rhsc + (W)(coev * lap)) * dt - pgm);
+            }
+            h0[c0 + i] = cu;
+            if (J >= 2) h0[cs + c0 + i] = cv;
+            h0[2 * cs + c0 + i] = cw;
+            u_m = u_c; v_m = v_c; w_m = w_c; p_m = p0; uip_m = uip; vjp_m = vjp;
+            __syncthreads();   // RHS rows complete; plane k's buffer read by every thread
+            // x-solve (MOMFA1_X_tg): circulant inverse gx of half-width Dx along the line
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                if (cc == 1 && J == 1) continue;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
+                const W *ln = xs + (cc * BY + t) * LP + XH + i;
+                W acc = gx[0] * ln[0];
+                for (int d = 1; d <= Dx; ++d) acc += gx[d] * (ln[d] + ln[-d]);   // Dx <= XH
+                rhs[cc * cs + c0 + i] = acc;
+            }
+            if (k + 2 <= kend) {                             // plane kend is the last one read
+                store_plane(cb);                             // plane k+2 over plane k
+                if (k + 3 <= kend) load_plane(wrapk(k + 3));
+            }
+            __syncthreads();   // plane k+2 stored; the RHS rows read
+        };
+        int k = kbeg;
+        for (; k + 1 < kend; k += 2) {
+            plane(k, std::integral_constant<int, 0>{});
+            plane(k + 1, std::integral_constant<int, 1>{});
+        }
+        if (k < kend) plane(k, std::integral_constant<int, 0>{});
+        mf = rsum / dyf;
+    }
+    // block reduction of the mass-flux partial in a fixed order (xs reused: 3 BY LP W >= NT
+    // doubles); a block of 3 * 2^n threads folds its top third first
+    constexpr int P2 = (NT & (NT - 1)) ? (NT / 3) * 2 : NT;
+    static_assert((P2 & (P2 - 1)) == 0, "block size 2^n or 3 * 2^n");
+    double *red = reinterpret_cast<double *>(xs);
+    const int tid = t * L + i;
+    red[tid] = mf;
+    __syncthreads();
+    if (P2 < NT) {
+        if (tid < NT - P2) red[tid] += red[tid + P2];
+        __syncthreads();
+    }
+    for (int o = P2 / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) partial[tile] = red[0];
 }
 
 // Sum of n partials in a fixed order by one block; out[0] = sum * scale.
@@ -914,6 +1127,130 @@ __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__
     else ytdma_line<T, float, B>(M, amph, apph, pp32 + off, st, spec + off, st, zero, false);
 }
 
+// The same per-mode y-Thomas with each line split into P chunks of C rows solved side by side,
+// so that no intermediate leaves the chip and a line's rows are spread over P wavefronts.
+// Block = 64 consecutive lines t = kx N + kz (lanes; a row of the block is 64 consecutive
+// spectral values) x P chunks (wavefronts); chunk q holds rows q C + 1 .. min(M, q C + C) of
+// its line, and their pivots, in registers.  Both sweeps are linear in what they receive from
+// the neighbouring chunk:
+//   forward   g_J = gh_J + lam_J g_in,   gh from g_in = 0,  lam_J = (-a_J p_J) lam_{J-1}, lam = 1
+//             before the chunk;
+//   backward  x_J = xh_J + mu_J x_in,    xh from x_in = 0,  mu_J = (-c_J p_J) mu_{J+1}, mu = 1
+//             after the chunk;
+// so each chunk publishes its end values (gh, lam at its last row; xh, mu at its first) in LDS,
+// composes its neighbours' into g_in / x_in, and corrects its rows.  Chunk 0 receives g_in = 0
+// and the last chunk x_in = 0 (x_{M+1} = 0 gives x_M = g_M), so these are the recurrences of
+// ytdma_line with the sweeps split by linearity; only rounding differs.  Gauges as ytdma_line.
+template <typename T, int C>
+__global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__restrict__ amph,
+                                                      const double *__restrict__ apph, const double *__restrict__ pp,
+                                                      const float *__restrict__ pp32, const double *__restrict__ pp00,
+                                                      cpx<T> *__restrict__ spec)
+{
+    __shared__ double xr[16][64], xi[16][64], xl[16][64];
+    const int HN = g.H * g.N, M = g.M;
+    const int lane = threadIdx.x, q = threadIdx.y, P = blockDim.y;
+    const int t = blockIdx.x * 64 + lane;
+    const bool act = t < HN;
+    const int tt = act ? t : HN - 1;   // lanes past the last line solve a copy of it, unstored
+    const int kx = tt / g.N, kz = tt - kx * g.N;
+    const bool zero = kx == 0 && kz == 0;
+    const bool pin = zero && pp00;      // TRID0: x_M = 0, unmodified pivots pp00[J-1]
+    const bool z99 = zero && !pp00;     // FFT99: RHS(1) = 0, row 1 decoupled
+    // FP32 pivots (mixed form) for all but the lowest modes, as k_ytdma
+    const bool low = !pp32 || (kx <= 1 && (kz <= 1 || kz == g.N - 1));
+    const int J0 = q * C + 1, n = min(C, M - q * C);   // rows of this chunk (n >= 1)
+    const size_t st = (size_t)HN;
+    cpx<T> *s = spec + (size_t)(J0 - 1) * st + tt;
+    double rr[C], ri[C], pv[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u)
+        if (u < n) {
+            const cpx<T> v = s[(size_t)u * st];
+            rr[u] = (double)v.x;
+            ri[u] = (double)v.y;
+            const size_t o = (size_t)(J0 - 1 + u) * st + tt;
+            pv[u] = pin ? pp00[J0 - 1 + u] : (low ? pp[o] : (double)pp32[o]);
+        }
+    // forward sweep from g_in = 0
+    double gr = 0.0, gi = 0.0, lam = 1.0;
+#pragma unroll
+    for (int u = 0; u < C; ++u)
+        if (u < n) {
+            const int J = J0 + u;
+            const double a = (J == 1) ? 0.0 : amph[J - 1];
+            const bool r0 = z99 && J == 1;
+            gr = ((r0 ? 0.0 : rr[u]) - a * gr) * pv[u];
+            gi = ((r0 ? 0.0 : ri[u]) - a * gi) * pv[u];
+            lam = -(a * pv[u]) * lam;
+            rr[u] = gr;
+            ri[u] = gi;
+        }
+    xr[q][lane] = gr;
+    xi[q][lane] = gi;
+    xl[q][lane] = lam;
+    __syncthreads();
+    double inr = 0.0, ini = 0.0;
+    for (int qq = 0; qq < q; ++qq) {
+        inr = xr[qq][lane] + xl[qq][lane] * inr;
+        ini = xi[qq][lane] + xl[qq][lane] * ini;
+    }
+    if (q > 0) {
+        lam = 1.0;
+#pragma unroll
+        for (int u = 0; u < C; ++u)
+            if (u < n) {
+                lam = -(amph[J0 - 1 + u] * pv[u]) * lam;   // J >= 2 here
+                rr[u] += lam * inr;
+                ri[u] += lam * ini;
+            }
+    }
+    // backward sweep from x_in = 0
+    double xhr = 0.0, xhi = 0.0, mu = 1.0;
+#pragma unroll
+    for (int u = C - 1; u >= 0; --u)
+        if (u < n) {
+            const int J = J0 + u;
+            const double h = ((z99 && J == 1) ? 0.0 : apph[J - 1]) * pv[u];
+            if (pin && J == M) {
+                xhr = 0.0;
+                xhi = 0.0;
+            } else {
+                xhr = rr[u] - h * xhr;
+                xhi = ri[u] - h * xhi;
+            }
+            mu = -h * mu;
+            rr[u] = xhr;
+            ri[u] = xhi;
+        }
+    __syncthreads();   // every chunk has read the forward end values
+    xr[q][lane] = xhr;
+    xi[q][lane] = xhi;
+    xl[q][lane] = mu;
+    __syncthreads();
+    double xir = 0.0, xii = 0.0;
+    for (int qq = P - 1; qq > q; --qq) {
+        xir = xr[qq][lane] + xl[qq][lane] * xir;
+        xii = xi[qq][lane] + xl[qq][lane] * xii;
+    }
+    if (q < P - 1) {
+        mu = 1.0;
+#pragma unroll
+        for (int u = C - 1; u >= 0; --u)
+            if (u < n) {
+                const int J = J0 + u;
+                mu = -(((z99 && J == 1) ? 0.0 : apph[J - 1]) * pv[u]) * mu;
+                rr[u] += mu * xir;
+                ri[u] += mu * xii;
+            }
+    }
+    if (act) {
+#pragma unroll
+        for (int u = 0; u < C; ++u)
+            if (u < n) s[(size_t)u * st] = {(T)rr[u], (T)ri[u]};
+    }
+}
+
 // Inverse real x-transform of TK z-planes at row J -> phi (halo field) and the Neumann
 // ghost rows of BC_WALL_DPH_tg (BC_WALL.f90:369-386).  Z = A + i B with the Hermitian
 // extension; imaginary parts at kx = 0 and L/2 are not used (FFT99.f90:84-86).
@@ -1157,7 +1494,7 @@ __global__ __launch_bounds__(256) void k_poisson_resid(Geo g, const double *__re
     red[t] = val;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) red[t] = fmax(red[t], red[t + o]);
+        if (t < o) red[t] = nanmax(red[t], red[t + o]);
         __syncthreads();
     }
     if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
@@ -1227,7 +1564,7 @@ __global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict
     red[t] = val;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) red[t] = fmax(red[t], red[t + o]);
+        if (t < o) red[t] = nanmax(red[t], red[t + o]);
         __syncthreads();
     }
     if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
@@ -1237,11 +1574,11 @@ __global__ void k_reduce_max(const double *__restrict__ part, int n, double *__r
 {
     __shared__ double red[256];
     double a = 0.0;
-    for (int t = threadIdx.x; t < n; t += 256) a = fmax(a, part[t]);
+    for (int t = threadIdx.x; t < n; t += 256) a = nanmax(a, part[t]);
     red[threadIdx.x] = a;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        if (threadIdx.x < o) red[threadIdx.x] = nanmax(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
@@ -1411,7 +1748,7 @@ struct chs_ctx {
     // (defaults: the fastest shapes on MI355X for a 256 x 192 x 256 channel; cnx_nlx 0 means
     // 2048 / L lines)
     struct {
-        int conv_by = 4, conv_kc = 16;  // k_conv_rhs rows per block, z-planes per thread
+        int conv_by = 4, conv_kc = 32;  // k_conv_rhs rows per block, z-planes per thread
         int conv_bx = 1;                // k_conv_rhs wavefronts along x per block (1 or 4)
         int conv_fence = 0;             // 1: all of a plane's loads stay ahead of its arithmetic
         int cnx_nlx = 0;                // k_cn_x lines per block
@@ -1422,8 +1759,11 @@ struct chs_ctx {
         int cn_xz = 0, cnxz_by = 1;     // fused x+z CN kernel (1; slower on MI355X), rows per block
         int conv_fx = 1;                // x-solve inside k_conv_rhs (rows per block; 0 = off)
         int ytdma_bs = 64, ytdma_rb = 16;   // k_ytdma threads per block, rows per load batch
+        int ytdma_c = -1;                   // k_ytdma_chunk rows per chunk (-1 auto = 16, 0 = k_ytdma)
+        int conv_lds = 2;                   // k_conv_rhs_lds rows per block (1, 2, 4; 0 = k_conv_rhs)
     } tune;
-    int tune_gen = 0;   // bumped by every chs_set_tuning (captured graphs are re-made)
+    int tune_gen = 0;   // bumped by every chs_set_tuning / chs_set_refinement (graphs are re-made)
+    int cn_gen = 0;     // bumped by every rebuild of a Crank-Nicolson table (its memory may move)
     bool have_fields = false;
     // profiling of the dominant kernel
     bool prof = false;
@@ -1434,6 +1774,7 @@ struct chs_ctx {
     bool own_stream = true;   // false for channels of an ensemble (stream owned by chs_ens)
     hipGraphExec_t gexec = nullptr;
     double gdt = 0.0, gcv = 0.0;
+    int gtune = -1, gcn = -1;   // tune_gen and cn_gen the step graph was captured with
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 };
 
@@ -1536,8 +1877,10 @@ int build_cn(chs_ctx *c, int ns, double dt, double cvisc)
 {
     CnTab &t = c->cn[ns];
     if (t.valid && t.dt == dt && t.cvisc == cvisc) return 0;
-    // Tables of an earlier (dt, cvisc) may still be read by enqueued kernels.
+    // Tables of an earlier (dt, cvisc) may still be read by enqueued kernels; captured graphs
+    // that hold their addresses are re-made (cn_gen).
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    ++c->cn_gen;
     const bool mixed = c->mode == CHS_MIXED;
     const long double tol = mixed ? 1e-12L : 1e-19L;
     const long double beta = 0.5L * (long double)c->talp[ns] * (long double)dt * (long double)cvisc;
@@ -1709,6 +2052,16 @@ void fftx_inv_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw
 }
 #undef CHS_RCASES
 
+// Rows per chunk of k_ytdma_chunk (4, 8 or 16; at most 16 chunks, so M <= 256 — longer
+// chunks exceed the 128 registers a 1024-thread block allows): the tuning value if it fits,
+// else 16; 0, or a mesh with M > 256, selects the one-thread-per-line k_ytdma.
+int ytdma_chunk_rows(int want, int M)
+{
+    if (want == 0 || M > 16 * 16) return 0;
+    if ((want == 4 || want == 8) && (M + want - 1) / want <= 16) return want;
+    return 16;
+}
+
 // gauge 0: FFT99's (0,0) rule (row 1 decoupled), 1: TRID0's (last row pinned, FISHPACK)
 template <typename T>
 void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
@@ -1740,11 +2093,22 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
     }
     {
         EvScope ev(c, "ytdma");
-        const int bs = c->tune.ytdma_bs;
-        auto kern = c->tune.ytdma_rb == 8 ? k_ytdma<T, 8> : k_ytdma<T, 16>;
-        hipLaunchKernelGGL(kern, dim3((H * N + bs - 1) / bs), dim3(bs), 0, c->stream, g, c->d_amph,
-                           c->d_apph, c->pp, c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr,
-                           gauge ? (const double *)c->pp00 : (const double *)nullptr, spec);
+        const float *p32 = c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr;
+        const double *p00 = gauge ? (const double *)c->pp00 : (const double *)nullptr;
+        const int C = ytdma_chunk_rows(c->tune.ytdma_c, M);
+        if (C > 0) {
+            const dim3 grid((H * N + 63) / 64), blk(64, (M + C - 1) / C);
+            switch (C) {
+#define CHS_YTC(cc) case cc: hipLaunchKernelGGL((k_ytdma_chunk<T, cc>), grid, blk, 0, c->stream, g, c->d_amph, c->d_apph, c->pp, p32, p00, spec); break;
+                CHS_YTC(4) CHS_YTC(8) CHS_YTC(16)
+#undef CHS_YTC
+            }
+        } else {
+            const int bs = c->tune.ytdma_bs;
+            auto kern = c->tune.ytdma_rb == 8 ? k_ytdma<T, 8> : k_ytdma<T, 16>;
+            hipLaunchKernelGGL(kern, dim3((H * N + bs - 1) / bs), dim3(bs), 0, c->stream, g, c->d_amph,
+                               c->d_apph, c->pp, p32, p00, spec);
+        }
     }
     {
         EvScope ev(c, "fftz_inv");
@@ -1821,11 +2185,24 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
     const int FY = c->tune.conv_fx;   // rows per block of the fused kernel (0: not fused)
     const bool fx = FY > 0 && M % FY == 0 && !xz && zmarch && L % 64 == 0 && L * FY <= 512 && !t.fullx &&
                     t.Dx <= 16;
-    const dim3 blkc = fx ? dim3(L, FY) : dim3(64 * BX, BY);
-    const dim3 grdc = fx ? dim3(1, M / FY, (N + KC - 1) / KC)
-                         : dim3((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
+    // state staged through LDS (k_conv_rhs_lds): LY rows of whole x-lines per block
+    const int LY = c->tune.conv_lds;
+    const bool lds = fx && LY > 0 && M % LY == 0 && L <= 256 && L * LY <= 1024;
+    const dim3 blkc = lds ? dim3(L, LY) : fx ? dim3(L, FY) : dim3(64 * BX, BY);
+    const dim3 grdc = lds  ? dim3(1, M / LY, (N + KC - 1) / KC)
+                      : fx ? dim3(1, M / FY, (N + KC - 1) / KC)
+                           : dim3((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
     const int npart = grdc.x * grdc.y * grdc.z;
-    if (fx) {
+    if (lds) {
+        EvScope ev(c, "conv_rhs");
+        switch (LY * 10 + L / 64) {
+#define CHS_LCASE(by, bx) case by * 10 + bx: hipLaunchKernelGGL((k_conv_rhs_lds<W, by, bx>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd, (const W *)t.gx, t.Dx); break;
+            CHS_LCASE(1, 1) CHS_LCASE(1, 2) CHS_LCASE(1, 3) CHS_LCASE(1, 4)
+            CHS_LCASE(2, 1) CHS_LCASE(2, 2) CHS_LCASE(2, 3) CHS_LCASE(2, 4)
+            CHS_LCASE(4, 1) CHS_LCASE(4, 2) CHS_LCASE(4, 3) CHS_LCASE(4, 4)
+#undef CHS_LCASE
+        }
+    } else if (fx) {
         EvScope ev(c, "conv_rhs");
         const int fence = c->tune.conv_fence;
         switch (fence * 100 + FY * 10 + L / 64) {
@@ -2089,8 +2466,8 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
     if (mode == CHS_MIXED) {   // FP32 stencils
         c->tune.cnz_kz = 8;
         c->tune.cny_ky = 32;
-        // k_conv_rhs: loads of a plane ahead of its arithmetic and XCD-grouped tiles, 1.2 %
-        // faster per step at 256 x 192 x 256 in this form (no change in the FP64 form)
+        // k_conv_rhs (where k_conv_rhs_lds does not apply): loads of a plane ahead of its
+        // arithmetic and XCD-grouped tiles, 1.2 % faster per step at 256 x 192 x 256 in this form
         c->tune.conv_fence = 1;
         c->tune.nxcd = 8;
     }
@@ -2311,10 +2688,11 @@ int chs_step_tg(chs_ctx *c, double dt, double cvisc)
     }
     if (!(dt > 0) || !(cvisc > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
     if (!c->have_fields) return fail(c, CHS_E_STATE, "no fields uploaded");
-    if (!c->gexec || c->gdt != dt || c->gcv != cvisc) {
-        // tables first: building them synchronises the stream, which capture forbids
-        for (int ns = 1; ns <= 3; ++ns)
-            if (int e = build_cn(c, ns, dt, cvisc)) return e;
+    // tables first (a no-op when they are current): building them synchronises the stream,
+    // which capture forbids, and a rebuild moves them, which a captured graph cannot follow
+    for (int ns = 1; ns <= 3; ++ns)
+        if (int e = build_cn(c, ns, dt, cvisc)) return e;
+    if (!c->gexec || c->gdt != dt || c->gcv != cvisc || c->gtune != c->tune_gen || c->gcn != c->cn_gen) {
         if (c->gexec) {
             HIPCHK(c, hipGraphExecDestroy(c->gexec));
             c->gexec = nullptr;
@@ -2335,6 +2713,8 @@ int chs_step_tg(chs_ctx *c, double dt, double cvisc)
         if (ce != hipSuccess) return fail(c, CHS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ce));
         c->gdt = dt;
         c->gcv = cvisc;
+        c->gtune = c->tune_gen;
+        c->gcn = c->cn_gen;
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     return 0;
@@ -2346,7 +2726,7 @@ int chs_set_refinement(chs_ctx *c, int nref_step, int nref_projection)
         return fail(c, CHS_E_ARG, "refinement sweeps must be 0..4");
     c->nref_step = nref_step;
     c->nref_projection = nref_projection;
-    c->gdt = 0.0;   // force re-capture of a step graph
+    ++c->tune_gen;   // captured graphs hold the old sweep count
     return 0;
 }
 
@@ -2421,6 +2801,12 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "ytdma_bs") {
         if (value != 64 && value != 128 && value != 256) return bad("must be 64, 128 or 256");
         c->tune.ytdma_bs = value;
+    } else if (k == "conv_lds") {
+        if (value != 0 && value != 1 && value != 2 && value != 4) return bad("must be 0, 1, 2 or 4");
+        c->tune.conv_lds = value;
+    } else if (k == "ytdma_c") {
+        if (value != -1 && value != 0 && value != 4 && value != 8 && value != 16) return bad("must be -1, 0, 4, 8 or 16");
+        c->tune.ytdma_c = value;
     } else if (k == "ytdma_rb") {
         if (value != 8 && value != 16) return bad("must be 8 or 16");
         c->tune.ytdma_rb = value;
@@ -2473,6 +2859,7 @@ int chs_cfl_tg(chs_ctx *c, double dt, double *cflmm)
     HIPCHK(c, hipMemcpyAsync(&v, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *cflmm = v * dt;
+    if (!std::isfinite(v)) return fail(c, CHS_E_NONFINITE, "chs_cfl_tg: non-finite velocity");
     return 0;
 }
 
@@ -2488,6 +2875,7 @@ int chs_max_div(chs_ctx *c, double *maxdiv)
     HIPCHK(c, hipMemcpyAsync(&v, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *maxdiv = v;
+    if (!std::isfinite(v)) return fail(c, CHS_E_NONFINITE, "chs_max_div: non-finite divergence");
     return 0;
 }
 
@@ -2546,7 +2934,7 @@ struct chs_ens {
     std::vector<hipEvent_t> ev;   // ev[0] fork, ev[1 + s] join of lane s
     hipGraphExec_t gexec = nullptr;
     std::vector<double> gdt, gcv;
-    std::vector<int> gtune;
+    std::vector<int> gtune, gcn;
     std::string err;
 };
 
@@ -2609,17 +2997,19 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
 {
     if (!e || !dt || !cvisc) return fail(nullptr, CHS_E_ARG, "chs_ens_step: bad argument");
     const int n = (int)e->ch.size();
+    // tables first (no-ops when current); a rebuild moves them, so the graph is re-made
+    for (int i = 0; i < n; ++i) {
+        chs_ctx *c = e->ch[i];
+        if (!c->have_fields) return fail(c, CHS_E_STATE, "chs_ens_step: channel without fields");
+        if (!(dt[i] > 0) || !(cvisc[i] > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
+        for (int ns = 1; ns <= 3; ++ns)
+            if (int rc = build_cn(c, ns, dt[i], cvisc[i])) return rc;
+    }
     bool same = e->gexec && (int)e->gdt.size() == n;
     for (int i = 0; same && i < n; ++i)
-        same = e->gdt[i] == dt[i] && e->gcv[i] == cvisc[i] && e->gtune[i] == e->ch[i]->tune_gen;
+        same = e->gdt[i] == dt[i] && e->gcv[i] == cvisc[i] && e->gtune[i] == e->ch[i]->tune_gen &&
+               e->gcn[i] == e->ch[i]->cn_gen;
     if (!same) {
-        for (int i = 0; i < n; ++i) {
-            chs_ctx *c = e->ch[i];
-            if (!c->have_fields) return fail(c, CHS_E_STATE, "chs_ens_step: channel without fields");
-            if (!(dt[i] > 0) || !(cvisc[i] > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
-            for (int ns = 1; ns <= 3; ++ns)
-                if (int rc = build_cn(c, ns, dt[i], cvisc[i])) return rc;
-        }
         if (e->gexec) {
             (void)hipGraphExecDestroy(e->gexec);
             e->gexec = nullptr;
@@ -2660,7 +3050,11 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
         e->gdt.assign(dt, dt + n);
         e->gcv.assign(cvisc, cvisc + n);
         e->gtune.resize(n);
-        for (int i = 0; i < n; ++i) e->gtune[i] = e->ch[i]->tune_gen;
+        e->gcn.resize(n);
+        for (int i = 0; i < n; ++i) {
+            e->gtune[i] = e->ch[i]->tune_gen;
+            e->gcn[i] = e->ch[i]->cn_gen;
+        }
     }
     hipError_t he = hipGraphLaunch(e->gexec, e->stream);
     if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));

@@ -53,7 +53,7 @@ $(OBJDIR)/.stamp: | $(OBJDIR)
 	set -e; for f in $(SRC1); do $(FC) $(FFLAGS) -c $(REF)/src/$$f.f90 -o $(OBJDIR)/$$f.o; done
 	set -e; for f in $(SRC2); do $(FC) $(FFLAGS) -c $(REF)/src/processing/$$f.f90 -o $(OBJDIR)/$$f.o; done
 	set -e; for f in $(SRC3); do $(FC) $(FFLAGS) -c $(REF)/lib/fft99/$$f.f90 -o $(OBJDIR)/$$f.o; done
-	$(FC) -O2 -fdefault-real-8 -fdefault-double-8 -c $(REF)/lib/fishpack4.1/fftpack.f -o $(OBJDIR)/fftpack.o
+	$(FC) -O2 -fdefault-real-8 -fdefault-double-8 -module-dir $(OBJDIR) -c $(REF)/lib/fishpack4.1/fftpack.f -o $(OBJDIR)/fftpack.o
 	set -e; for f in $(SRC5); do $(FC) -O2 -c $(REF)/lib/eigen33/$$f.f -o $(OBJDIR)/$$f.o; done
 	$(FC) $(FFLAGS) -c $(REF)/src/DNS_THERMAL.f90 -o $(OBJDIR)/DNS_THERMAL.o
 	touch $@

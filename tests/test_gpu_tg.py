@@ -1,7 +1,6 @@
 """GPU parity of the plain FP64 substep against the oracle (and the reference's golden
-vectors).  Tolerances: velocity 1e-10 relative (max-norm), pressure 1e-9 relative
-(the pressure gauge mode is solved by an ill-conditioned Neumann Thomas sweep),
-divergence after projection < 1e-10."""
+vectors).  Tolerances: velocity and pressure 1e-10 relative (max-norm) per substep,
+divergence after projection < 1e-10.  Each test prints the largest errors it saw."""
 import numpy as np
 import pytest
 
@@ -40,20 +39,25 @@ def test_projection_and_steps_match_oracle(pkg, L, M, N, STR2, REN, dt):
     ch.SOLVERRK3_MOM_tg(0, dt, cv)
     o.substep(0, dt)
     gq, gp = ch.download()
-    for c in range(3):
-        assert rel(gq[c], o.q[c]) < 1e-10, ("ns0", c, rel(gq[c], o.q[c]))
-    assert rel(gp, o.pr) < 1e-9
+    eq = max(rel(gq[c], o.q[c]) for c in range(3))
+    ep = rel(gp, o.pr)
+    assert eq < 1e-10, ("ns0", eq)
+    assert ep < 1e-10, ("ns0", ep)
     assert ch.max_div() < 1e-10
     for step in range(3):
         for ns in (1, 2, 3):
             ch.SOLVERRK3_MOM_tg(ns, dt, cv)
             o.substep(ns, dt)
             gq, gp, gh = ch.download(with_history=True)
-            for c in range(3):
-                assert rel(gq[c], o.q[c]) < 1e-10, (step, ns, c, rel(gq[c], o.q[c]))
-            assert rel(gp, o.pr) < 1e-9, (step, ns, rel(gp, o.pr))
+            e = max(rel(gq[c], o.q[c]) for c in range(3))
+            e_p = rel(gp, o.pr)
+            eq, ep = max(eq, e), max(ep, e_p)
+            assert e < 1e-10, (step, ns, e)
+            assert e_p < 1e-10, (step, ns, e_p)
             assert rel(gh[0], o.convh0[0]) < 1e-10
-    assert ch.max_div() < 1e-10
+    div = ch.max_div()
+    print(f"  {L}x{M}x{N}: max rel error velocity {eq:.2e}, pressure {ep:.2e}; max div {div:.2e}")
+    assert div < 1e-10
     ch.close()
 
 
@@ -66,29 +70,33 @@ def test_matches_reference_golden(pkg, golden, name):
     ch.upload(g["q_init"], g["pr_init"])
     ch.SOLVERRK3_MOM_tg(0, dt, 1.0 / REN)
     gq, gp = ch.download()
-    for c in range(3):
-        assert rel(gq[c], g["q_ns0"][c]) < 1e-10
-    assert rel(gp, g["pr_ns0"]) < 1e-9
+    e0 = [max(rel(gq[c], g["q_ns0"][c]) for c in range(3)), rel(gp, g["pr_ns0"])]
     ch.rk3_step(dt, 1.0 / REN)
     gq, gp = ch.download()
-    for c in range(3):
-        assert rel(gq[c], g["q_step1"][c]) < 1e-10
-    assert rel(gp, g["pr_step1"]) < 1e-9
+    e1 = [max(rel(gq[c], g["q_step1"][c]) for c in range(3)), rel(gp, g["pr_step1"])]
     phi = ch.poisson(g["poisson_rhs"])
-    assert rel(phi, g["poisson_phi"]) < 1e-10
+    e2 = rel(phi, g["poisson_phi"])
+    print(f"  {name}: projection velocity {e0[0]:.2e} pressure {e0[1]:.2e}; RK3 step velocity "
+          f"{e1[0]:.2e} pressure {e1[1]:.2e}; Poisson {e2:.2e}")
+    assert e0[0] < 1e-10 and e1[0] < 1e-10
+    assert e0[1] < 1e-10 and e1[1] < 1e-10
+    assert e2 < 1e-10
     ch.close()
 
 
 # line lengths 64 ... 1024 run the register FFTs (chs_fftr.h), the others the LDS Stockham
 # transforms; fft_reg=0 forces the latter on planned lengths too
-POISSON_MESHES = [(16, 16, 16, 1), (30, 20, 24, 1), (128, 96, 128, 1), (128, 96, 128, 0),
-                  (64, 20, 96, 1), (256, 12, 256, 1), (512, 8, 512, 1), (1024, 6, 64, 1),
-                  (96, 10, 1024, 1)]
+# transforms; fft_reg=0 forces the latter on planned lengths too.  ytc: rows per chunk of the
+# chunked y-Thomas (-1 automatic, 0 the one-thread-per-line kernel; 4 and 8 give many chunks)
+POISSON_MESHES = [(16, 16, 16, 1, -1), (16, 16, 16, 1, 0), (30, 20, 24, 1, 4), (128, 96, 128, 1, -1),
+                  (128, 96, 128, 0, 8), (128, 96, 128, 1, 0), (64, 20, 96, 1, 4), (256, 12, 256, 1, -1),
+                  (512, 8, 512, 1, -1), (1024, 6, 64, 1, -1), (96, 10, 1024, 1, 4), (64, 256, 64, 1, -1),
+                  (64, 300, 32, 1, -1), (32, 61, 48, 1, 4)]
 
 
-@pytest.mark.parametrize("L,M,N,reg", POISSON_MESHES)
-def test_poisson_matches_oracle(pkg, L, M, N, reg):
-    print(f"poisson {L}x{M}x{N} fft_reg={reg}", flush=True)
+@pytest.mark.parametrize("L,M,N,reg,ytc", POISSON_MESHES)
+def test_poisson_matches_oracle(pkg, L, M, N, reg, ytc):
+    print(f"poisson {L}x{M}x{N} fft_reg={reg} ytdma_c={ytc}", flush=True)
     mesh = setup.Mesh(L, M, N, STR2=2.2)
     rng = np.random.default_rng(5)
     r = rng.standard_normal((N, M, L))
@@ -98,9 +106,11 @@ def test_poisson_matches_oracle(pkg, L, M, N, reg):
     ref = o.poisson(r)
     for mode, tol in ((pkg.PLAIN_FP64, 1e-10), (pkg.MIXED, 1e-5)):
         ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=2.2), mode)
-        ch.set_tuning(fft_reg=reg)
+        ch.set_tuning(fft_reg=reg, ytdma_c=ytc)
         phi = ch.poisson(r)
-        assert rel(phi, ref) < tol, (mode, rel(phi, ref))
+        e = rel(phi, ref)
+        print(f"  {'plain' if mode == pkg.PLAIN_FP64 else 'mixed'}: rel error {e:.2e}")
+        assert e < tol, (mode, e)
         ch.close()
 
 
@@ -184,9 +194,9 @@ def test_launch_shapes_keep_results(pkg, shape):
         for ns in (1, 2, 3):
             o.substep(ns, dt)
     gq, gp = ch.download()
-    for c in range(3):
-        assert rel(gq[c], o.q[c]) < 1e-10, (c, rel(gq[c], o.q[c]))
-    assert rel(gp, o.pr) < 1e-9
+    e, ep = max(rel(gq[c], o.q[c]) for c in range(3)), rel(gp, o.pr)
+    print(f"  velocity {e:.2e} pressure {ep:.2e}")
+    assert e < 1e-10 and ep < 1e-10
     ch.close()
 
 
@@ -208,9 +218,9 @@ def test_fused_x_solve_matches_oracle(pkg, L, M, N, fx):
         for ns in (1, 2, 3):
             o.substep(ns, dt)
     gq, gp = ch.download()
-    for c in range(3):
-        assert rel(gq[c], o.q[c]) < 1e-10, (c, rel(gq[c], o.q[c]))
-    assert rel(gp, o.pr) < 1e-9
+    e, ep = max(rel(gq[c], o.q[c]) for c in range(3)), rel(gp, o.pr)
+    print(f"  velocity {e:.2e} pressure {ep:.2e}")
+    assert e < 1e-10 and ep < 1e-10
     assert ch.max_div() < 1e-10
     ch.close()
 
@@ -219,7 +229,7 @@ def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
     for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(cn_xz=2), dict(cnxz_by=3), dict(conv_kc=41), dict(fftx_tk=3),
                 dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6), dict(conv_fx=3),
-                dict(no_such_key=1)):
+                dict(ytdma_c=12), dict(ytdma_c=-2), dict(no_such_key=1)):
         with pytest.raises(pkg.ChsError):
             ch.set_tuning(**bad)
     ch.close()

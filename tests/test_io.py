@@ -128,7 +128,7 @@ def test_gpu_io_matches_oracle(pkg, L, M, N):
     for k in ("G", "Q"):
         for c in range(3):
             assert rel(f[k][c], o.f[k][c]) < 1e-11, (k, c, rel(f[k][c], o.f[k][c]))
-    assert rel(f["PR"], o.f["PR"]) < 1e-9
+    assert rel(f["PR"], o.f["PR"]) < 1e-10, rel(f["PR"], o.f["PR"])
     ch.close()
 
 
@@ -166,7 +166,7 @@ def test_gpu_heated_stage_matches_oracle(pkg):
     f, fe = gi.download(), ge.download()
     for c in range(3):
         assert rel(f["G"][c], oi.f["G"][c]) < 1e-11, (c, rel(f["G"][c], oi.f["G"][c]))
-    assert rel(f["PR"], oi.f["PR"]) < 1e-9
+    assert rel(f["PR"], oi.f["PR"]) < 1e-10, rel(f["PR"], oi.f["PR"])
     for k, kk in (("dh", "DH"), ("t", "T"), ("d", "D")):
         assert rel(fe[kk], oe.f[k]) < 1e-11, (kk, rel(fe[kk], oe.f[k]))
     ge.close()

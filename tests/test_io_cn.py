@@ -200,7 +200,7 @@ def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows, cols):
     for k in ("G", "Q"):
         for c in range(3):
             assert rel(f[k][c], o.f[k][c]) < 1e-11, (k, c, rel(f[k][c], o.f[k][c]))
-    assert rel(f["PR"], o.f["PR"]) < 1e-9
+    assert rel(f["PR"], o.f["PR"]) < 1e-10, rel(f["PR"], o.f["PR"])
     ch.close()
 
 
@@ -239,7 +239,7 @@ def test_gpu_heated_stage_cn_matches_oracle(pkg):
     f = gi.download()
     for c in range(3):
         assert rel(f["G"][c], oi.f["G"][c]) < 1e-11, (c, rel(f["G"][c], oi.f["G"][c]))
-    assert rel(f["PR"], oi.f["PR"]) < 1e-9
+    assert rel(f["PR"], oi.f["PR"]) < 1e-10, rel(f["PR"], oi.f["PR"])
     ge.close()
     gi.close()
 

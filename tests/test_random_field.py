@@ -117,7 +117,7 @@ def test_sine_field_and_projection_match_reference(pkg):
     o.substep(0, dt)
     for c in range(3):
         assert rel(o.q[c], f1[c]) < 1e-12, c
-    assert rel(o.pr, f1[3]) < 1e-9
+    assert rel(o.pr, f1[3]) < 1e-10, rel(o.pr, f1[3])
 
 
 @pytest.mark.skipif(not refrun.available(), reason="compiled reference (oracle/_ref) absent")
@@ -151,4 +151,4 @@ def test_gpu_projection_of_random_field(pkg, rtype):
     o.substep(0, dt)
     for c in range(3):
         assert rel(gq[c], o.q[c]) < 1e-10, c
-    assert rel(gp, o.pr) < 1e-9
+    assert rel(gp, o.pr) < 1e-10, rel(gp, o.pr)
