@@ -86,8 +86,6 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
         "conv_rhs": 4 * F * 8 + 3 * F * wbytes + 6 * F * wbytes,
         # R RHS x3, W RHS x3 (v skips row 1); one launch for the three components
         "cn_x": 6 * F * wbytes,
-        # fused z+y stencils + Q update, three components: R RHS x3, R+W q x3
-        "cn_zy": 3 * F * wbytes + 6 * F * 8,
         # z stencils, one launch for the three components: R + W RHS x3
         "cn_z": 6 * F * wbytes,
         # x and z together, three components: R + W RHS x3
@@ -122,7 +120,7 @@ def canonical_step_bytes(L: int, M: int, N: int, wbytes: int) -> float:
     return 3 * (22 * 8 + 26 * wbytes) * F
 
 
-KERNELS = ["conv_rhs", "cn_x", "cn_zy", "cn_z", "cn_xz", "cn_y", "divg_fftx", "fftz_fwd", "ytdma",
+KERNELS = ["conv_rhs", "cn_x", "cn_z", "cn_xz", "cn_y", "divg_fftx", "fftz_fwd", "ytdma",
            "fftz_inv", "fftx_inv", "update", "refine_update"]
 
 

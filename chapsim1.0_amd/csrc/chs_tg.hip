@@ -80,9 +80,21 @@ __device__ __forceinline__ int xcd_tile(int nxcd)
     return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
-// RK scalars of one substep.
+// The cell divergence of DIVG_tg (DIVG.f90:31-46) before its 1 / (alpha DT) scaling, in one
+// fixed rounding (no contraction) wherever it is formed: the Poisson RHS and PRCALC_tg's
+// RHSLLPHI are the same numbers.
+__device__ __forceinline__ double divg_cell(double up, double u, double vp, double v, double wp, double w, double dxi,
+                                            double dyf, double dzi)
+{
+#pragma clang fp contract(off)
+    return (up - u) * dxi + (vp - v) * dyf + (wp - w) * dzi;
+}
+
+// RK scalars of one substep; the time step and viscosity are read from device memory
+// (dv[0] = DT, dv[1] = CVISC), so a step whose DT the device chose needs no host value.
 struct Sub {
-    double tgam, troh, talp, dt, cvisc;
+    double tgam, troh, talp;
+    const double *dv;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -136,8 +148,9 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
         const double apu = g.apvr[J], acu = g.acvr[J], amu = g.amvr[J];
         const double apv = g.apvr[o1 + J], acv = g.acvr[o1 + J], amv = g.amvr[o1 + J];
         const double apw = g.apvr[o2 + J], acw = g.acvr[o2 + J], amw = g.amvr[o2 + J];
-        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)s.dt;
-        const double coev = s.talp * s.cvisc, coep = s.talp * s.dt;
+        const double sdt = s.dv[0], scv = s.dv[1];
+        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)sdt;
+        const double coev = s.talp * scv, coep = s.talp * sdt;
         const W cu1 = (W)(0.25 * g.dxi), cu2 = (W)(dyf * 0.5), cu3 = (W)(0.25 * g.dzi);
         const W cv1 = (W)(g.dxi * 0.5), cv2 = (W)(dyc * 0.25), cv3 = (W)(g.dzi * 0.5);
         // carried: plane k-1 (_m) and plane k (_c)
@@ -348,8 +361,9 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_lds(Geo g, Sub s, co
         const double apu = g.apvr[J], acu = g.acvr[J], amu = g.amvr[J];
         const double apv = g.apvr[o1 + J], acv = g.acvr[o1 + J], amv = g.amvr[o1 + J];
         const double apw = g.apvr[o2 + J], acw = g.acvr[o2 + J], amw = g.amvr[o2 + J];
-        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)s.dt;
-        const double coev = s.talp * s.cvisc, coep = s.talp * s.dt;
+        const double sdt = s.dv[0], scv = s.dv[1];
+        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)sdt;
+        const double coev = s.talp * scv, coep = s.talp * sdt;
         const W cu1 = (W)(0.25 * g.dxi), cu2 = (W)(dyf * 0.5), cu3 = (W)(0.25 * g.dzi);
         const W cv1 = (W)(g.dxi * 0.5), cv2 = (W)(dyc * 0.25), cv3 = (W)(g.dzi * 0.5);
         // prologue: planes kbeg, kbeg+1 into LDS; plane kbeg-1 values of this point into
@@ -479,6 +493,117 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_lds(Geo g, Sub s, co
         __syncthreads();
     }
     if (tid == 0) partial[tile] = red[0];
+}
+
+// Crank-Nicolson tables of the RK stages ns (blockIdx.y) for the device's current DT and CVISC
+// (dv[0], dv[1]), beta = alpha_ns DT CVISC / 2 (MOMFA_tg.f90:29), computed in FP64 and stored
+// in the work precision.
+//  block x = 0, 1: first column of the inverse of the periodic x (z) operator 1 - beta dxx,
+//    a = c = -beta DXQI (DZQI), b = 1 - a - c (MOMFA_tg.f90:72-82, :129-146).  It is circulant:
+//    with r the root of a r^2 + b r + a = 0 inside the unit circle, the entry at cyclic
+//    distance d on a line of n points is (r^d + r^(n-d)) / ((1 - r^n) (b + 2 a r)).
+//  block x = 2 + comp: band (half-width D) of the inverse of the y operator of the component
+//    (MOMFA_tg.f90:206-216): rows J = 1..M, sub a_J = -beta AMVR, diagonal b_J = 1 - beta ACVR,
+//    super c_J = -beta APVR, a_1 = c_M = 0 (the zero boundary vectors of TDMAIJJ_nonCYC).  With
+//    the forward pivots rho_1 = b_1, rho_J = b_J - a_J c_{J-1} / rho_{J-1} and the backward ones
+//    psi_M = b_M, psi_J = b_J - c_J a_{J+1} / psi_{J+1}: inv[J][J] = 1 / (rho_J + psi_J - b_J),
+//    inv[I][J] = -(c_I / rho_I) inv[I+1][J] above the diagonal and -(a_I / psi_I) inv[I-1][J]
+//    below it (the ratios of Usmani's closed form).  band[J][d + D] = inv[J][J + d], zero
+//    outside the matrix.
+// stamp[2 ns .. 2 ns + 1] holds the (DT, CVISC) a stage's tables were built for; a stage whose
+// stamp equals dv returns at once, so a step graph launches this unconditionally (k_cn_stamp
+// records the new values after it).
+template <typename W> struct CnDev {
+    W *gx[4], *gz[4], *by[4][3];
+    int Dy[4][3];
+    double talp[4];
+};
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_tables(Geo g, CnDev<W> t, const double *__restrict__ dv,
+                                                   const double *__restrict__ stamp, int ns0)
+{
+    __shared__ double rho[1024], psi[1024];
+    const int ns = ns0 + blockIdx.y;
+    const double dt = dv[0], cv = dv[1];
+    if (stamp[2 * ns] == dt && stamp[2 * ns + 1] == cv) return;
+    const double beta = 0.5 * t.talp[ns] * dt * cv;
+    const int b = blockIdx.x;
+    if (b <= 1) {
+        const int n = b == 0 ? g.L : g.N;
+        const double a = -beta * (b == 0 ? g.dxqi : g.dzqi), bb = 1.0 - a - a;
+        const double r = -2.0 * a / (bb + sqrt(bb * bb - 4.0 * a * a));
+        double rn = 1.0;
+        for (int m = 0; m < n; ++m) rn *= r;
+        const double cc = 1.0 / ((1.0 - rn) * (bb + 2.0 * a * r));
+        W *out = b == 0 ? t.gx[ns] : t.gz[ns];
+        for (int d = threadIdx.x; d < n; d += blockDim.x) {
+            double rd = 1.0, rnd = 1.0;
+            for (int m = 0; m < d; ++m) rd *= r;
+            for (int m = 0; m < n - d; ++m) rnd *= r;
+            out[d] = (W)((rd + rnd) * cc);
+        }
+        return;
+    }
+    const int comp = b - 2, M = g.M, D = t.Dy[ns][comp], w = 2 * D + 1;
+    W *band = t.by[ns][comp];
+    const double *am = g.amvr + comp * (M + 2), *ac = g.acvr + comp * (M + 2), *ap = g.apvr + comp * (M + 2);
+    // 0-based row j = J - 1
+    auto sub = [&](int j) { return j == 0 ? 0.0 : -beta * am[j + 1]; };
+    auto dia = [&](int j) { return 1.0 - beta * ac[j + 1]; };
+    auto sup = [&](int j) { return j == M - 1 ? 0.0 : -beta * ap[j + 1]; };
+    if (threadIdx.x == 0) {
+        rho[0] = dia(0);
+        for (int j = 1; j < M; ++j) rho[j] = dia(j) - sub(j) * sup(j - 1) / rho[j - 1];
+    } else if (threadIdx.x == 64) {
+        psi[M - 1] = dia(M - 1);
+        for (int j = M - 2; j >= 0; --j) psi[j] = dia(j) - sup(j) * sub(j + 1) / psi[j + 1];
+    }
+    __syncthreads();
+    for (int col = threadIdx.x; col < M; col += blockDim.x) {
+        const double dg = 1.0 / (rho[col] + psi[col] - dia(col));
+        band[(size_t)col * w + D] = (W)dg;
+        double x = dg;
+        for (int i = col - 1; i >= max(0, col - D); --i) {   // rows above: inv[i][col]
+            x = -(sup(i) / rho[i]) * x;
+            band[(size_t)i * w + (col - i) + D] = (W)x;
+        }
+        x = dg;
+        for (int i = col + 1; i <= min(M - 1, col + D); ++i) {   // rows below
+            x = -(sub(i) / psi[i]) * x;
+            band[(size_t)i * w + (col - i) + D] = (W)x;
+        }
+        for (int d = -D; d <= D; ++d)   // entries of row col outside the matrix
+            if (col + d < 0 || col + d >= M) band[(size_t)col * w + d + D] = W(0);
+    }
+}
+
+// Records the (DT, CVISC) the stages ns0 .. ns0 + n - 1 now hold (see k_cn_tables); with
+// dv null, marks them as holding nothing (their band layout is about to change).
+__global__ void k_cn_stamp(const double *__restrict__ dv, double *__restrict__ stamp, int ns0, int n)
+{
+    const int t = threadIdx.x;
+    if (t < 2 * n) stamp[2 * ns0 + t] = dv ? dv[t & 1] : __builtin_nan("");
+}
+
+// The time-step rule of BCAST_COMM_STEP (SOLVE.f90:169-178) on the device: from CFLMM of the
+// current field (st[2]) and the last DT (st[0]), DT = max(min(DT0, 5 DT, CFLGV / CFLMM), DTMIN);
+// CVISC into st[1], PhyTIME += DT (st[3]); st[4] = 1 once DT / DT0 < 0.01, where the reference
+// stops (SOLVE.f90:199-207).
+__global__ void k_dt_rule(double *__restrict__ st, double dt0, double cflgv, double dtmin, double cvisc)
+{
+    const int t = threadIdx.x;
+    const double dt = fmax(fmin(fmin(dt0, 5.0 * st[0]), cflgv / st[2]), dtmin);
+    const double vals[5] = {dt, cvisc, st[2], st[3] + dt, (dt / dt0 < 0.01 || st[4] != 0.0) ? 1.0 : 0.0};
+    __syncthreads();
+    if (t < 5) st[t] = vals[t];
+}
+
+// Host-given DT and CVISC into the device step scalars.
+__global__ void k_set_step(double *__restrict__ dv, double dt, double cvisc)
+{
+    const int t = threadIdx.x;
+    if (t < 2) dv[t] = t == 0 ? dt : cvisc;
 }
 
 // Sum of n partials in a fixed order by one block; out[0] = sum * scale.
@@ -671,7 +796,7 @@ __global__ __launch_bounds__(256) void k_cn_z(Geo g, const W *__restrict__ gz, i
 template <typename W>
 __global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, int D,
                                               const W *__restrict__ rin, W *__restrict__ vrow1,
-                                              double *__restrict__ q, int c)
+                                              const double *__restrict__ qin, double *__restrict__ qout, int c)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
@@ -680,14 +805,15 @@ __global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, i
     const size_t cs = (size_t)g.N * g.M * g.L;
     const size_t hs = (size_t)g.N * (g.M + 2) * g.L;
     const W *r = rin + c * cs;
-    double *qc = q + c * hs;
+    const double *qi = qin + c * hs;
+    double *qc = qout + c * hs;
     const int M = g.M, w = 2 * D + 1;
     if (!(c == 1 && J == 1)) {   // v: rows J >= N2I = 2
         const int lo = J - D < 1 ? 1 : J - D, hi = J + D > M ? M : J + D;
         const W *bj = by + (size_t)(J - 1) * w;
         W acc = W(0);
         for (int l = lo; l <= hi; ++l) acc += bj[l - J + D] * r[cix(g, i, l, k)];
-        qc[hix(g, i, J, k)] += (double)acc;
+        qc[hix(g, i, J, k)] = qi[hix(g, i, J, k)] + (double)acc;
         if (c == 0 && J == 1 && vrow1) vrow1[cix(g, i, 1, k)] = acc;
     }
     // BC_WALL_Q (bottom and top walls)
@@ -696,83 +822,6 @@ __global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, i
         if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
     }
     if (J == M) qc[hix(g, i, M + 1, k)] = 0.0;
-}
-
-// Fused z + y Crank-Nicolson stencils, Q += dQ and wall BCs for all three components.
-// Block: an x-tile of TX columns at one z-plane k, all rows J = 1..M.  For each component
-// the z-stencil (circulant inverse) of the x-solved RHS goes to LDS column by column, then the
-// banded y-inverse is applied from LDS.  Components run u, v, w in turn so v's row J = 1
-// (the u increment left in the reference's shared RHS_tg, RHS_CvLpGpS_tg.f90:45-46) is taken
-// from the u pass of the same block.
-struct CnArgs {
-    const void *gz;
-    const void *by[3];
-    int Dz, fullz, Dy[3];
-};
-
-template <typename W>
-__global__ __launch_bounds__(256) void k_cn_zy(Geo g, CnArgs a, const W *__restrict__ rin, double *__restrict__ q,
-                                               int TX)
-{
-    extern __shared__ unsigned char smem_raw[];
-    W *col = reinterpret_cast<W *>(smem_raw);   // [M][TX]
-    W *du1 = col + (size_t)g.M * TX;             // [TX] u increment at J = 1
-    const int i0 = blockIdx.x * TX, k = blockIdx.y;
-    const int M = g.M, N = g.N;
-    const size_t cs = (size_t)N * M * g.L, hs = (size_t)N * (M + 2) * g.L;
-    const int nt = blockDim.x;
-    const W *gz = (const W *)a.gz;
-    for (int c = 0; c < 3; ++c) {
-        const W *r = rin + c * cs;
-        // z-stencil into LDS (rows J >= N2I; for v row 1 is u's increment)
-        for (int t = threadIdx.x; t < M * TX; t += nt) {
-            const int J = t / TX + 1, ii = t - (J - 1) * TX, i = i0 + ii;
-            W acc = W(0);
-            if (i < g.L) {
-                if (c == 1 && J == 1) {
-                    acc = du1[ii];
-                } else if (a.fullz) {
-                    for (int l = 0; l < N; ++l) {
-                        int d = k - l;
-                        if (d < 0) d += N;
-                        acc += gz[d] * r[cix(g, i, J, l)];
-                    }
-                } else {
-                    acc = gz[0] * r[cix(g, i, J, k)];
-                    for (int d = 1; d <= a.Dz; ++d) {
-                        int ka = k + d, kb = k - d;
-                        if (ka >= N) ka -= N;
-                        if (kb < 0) kb += N;
-                        acc += gz[d] * (r[cix(g, i, J, ka)] + r[cix(g, i, J, kb)]);
-                    }
-                }
-            }
-            col[(J - 1) * TX + ii] = acc;
-        }
-        __syncthreads();
-        // y-stencil from LDS, Q += dQ, wall BCs (BC_WALL.f90:229-251)
-        const W *by = (const W *)a.by[c];
-        const int D = a.Dy[c], w = 2 * D + 1;
-        double *qc = q + c * hs;
-        for (int t = threadIdx.x; t < M * TX; t += nt) {
-            const int J = t / TX + 1, ii = t - (J - 1) * TX, i = i0 + ii;
-            if (i >= g.L) continue;
-            if (!(c == 1 && J == 1)) {
-                const int lo = J - D < 1 ? 1 : J - D, hi = J + D > M ? M : J + D;
-                const W *bj = by + (size_t)(J - 1) * w;
-                W acc = W(0);
-                for (int l = lo; l <= hi; ++l) acc += bj[l - J + D] * col[(l - 1) * TX + ii];
-                qc[hix(g, i, J, k)] += (double)acc;
-                if (c == 0 && J == 1) du1[ii] = acc;
-            }
-            if (J == 1) {
-                qc[hix(g, i, 0, k)] = 0.0;
-                if (c == 1) qc[hix(g, i, 1, k)] = 0.0;
-            }
-            if (J == M) qc[hix(g, i, M + 1, k)] = 0.0;
-        }
-        __syncthreads();
-    }
 }
 
 // Marching forms of the z and y Crank-Nicolson stencils with the band half-width D a
@@ -838,12 +887,14 @@ __global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__
 }
 
 // y: one thread per (i, k) marches a chunk of KY rows; banded inverse by[J][d] (row J is
-// uniform across the wavefront, so its coefficients come through scalar loads); Q += dQ and
-// wall BCs (BC_WALL.f90:229-251).  For u the increment of row 1 is stored into row 1 of
+// uniform across the wavefront, so its coefficients come through scalar loads); the predicted
+// velocity Q + dQ goes to qout (the correction reads it from there and writes Q) with the wall
+// BCs (BC_WALL.f90:229-251).  For u the increment of row 1 is stored into row 1 of
 // v's input (the reference's shared RHS_tg, RHS_CvLpGpS_tg.f90:45-46).
 template <typename W, int D>
 __global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__ by, W *__restrict__ rin,
-                                                    double *__restrict__ q, int c, int KY)
+                                                    const double *__restrict__ qin, double *__restrict__ qout, int c,
+                                                    int KY)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int k = blockIdx.y * 4 + threadIdx.y;
@@ -852,7 +903,8 @@ __global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__
     const int M = g.M, w = 2 * D + 1;
     const size_t cs = (size_t)g.N * M * g.L, hs = (size_t)g.N * (M + 2) * g.L;
     const W *r = rin + c * cs + cix(g, i, 1, k);   // row J at r[(J-1)*L]
-    double *qc = q + c * hs;
+    const double *qi = qin + c * hs;
+    double *qc = qout + c * hs;
     const size_t L = g.L;
     const int J0 = (c == 1 && jc == 0) ? 2 : jc * KY + 1;
     const int J1 = min(M, jc * KY + KY);
@@ -883,14 +935,14 @@ __global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__
         for (int b = 0; b < B; ++b) {
             const int jj = J + b + D;
             nx[b] = jj <= M ? r[(size_t)(jj - 1) * L] : W(0);
-            qv[b] = qc[hix(g, i, J + b, k)];
+            qv[b] = qi[hix(g, i, J + b, k)];
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) step(J + b, nx[b], qv[b]);
     }
     for (; J <= J1; ++J) {
         const int jj = J + D;
-        step(J, jj <= M ? r[(size_t)(jj - 1) * L] : W(0), qc[hix(g, i, J, k)]);
+        step(J, jj <= M ? r[(size_t)(jj - 1) * L] : W(0), qi[hix(g, i, J, k)]);
     }
     if (jc == 0) {
         qc[hix(g, i, 0, k)] = 0.0;
@@ -919,7 +971,8 @@ __device__ __forceinline__ size_t six(const Geo &g, int J, int kx, int kz)
 // by 1/(L N) (the 1/L of FFT99 and the 1/N applied after CFFT99, possion3d_FFT99.f90:316,
 // :327-334) and written as spec[J][kx][k0..k0+TK-1] (TK consecutive values per kx).
 template <typename T>
-__global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, double coe0, const double *__restrict__ q,
+__global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, const double *__restrict__ dv, double talp,
+                                                   const double *__restrict__ q,
                                                    double *__restrict__ div, cpx<T> *__restrict__ spec,
                                                    int TK, const double *__restrict__ src)
 {
@@ -935,6 +988,7 @@ __global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, double
     const double *u = q, *v = q + hs, *w = q + 2 * hs;
     struct D2 { double d[2]; };
     const double dyf = g.dyfi[J - 1];
+    const double coe0 = dv ? 1.0 / (dv[0] * talp) : 1.0;   // 1 / (alpha DT) (DIVG_tg)
     chs::stage_items<4, D2>(
         np * L,
         [&](int t) {
@@ -947,9 +1001,9 @@ __global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, double
                     r.d[h] = src[cix(g, i, J, k)];
                 } else {
                     const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
-                    const double d = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
-                                     (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * dyf +
-                                     (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
+                    const double d = divg_cell(u[hix(g, ip, J, k)], u[hix(g, i, J, k)], v[hix(g, i, J + 1, k)],
+                                               v[hix(g, i, J, k)], w[hix(g, i, J, kp)], w[hix(g, i, J, k)], g.dxi,
+                                               dyf, g.dzi);
                     r.d[h] = d * coe0;
                 }
             }
@@ -957,7 +1011,7 @@ __global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, double
         },
         [&](int t, const D2 &r) {
             const int p = t / L, i = t - p * L;
-            if (!src) {
+            if (!src && div) {
                 div[cix(g, i, J, k0 + 2 * p)] = r.d[0];
                 div[cix(g, i, J, k0 + 2 * p + 1)] = r.d[1];
             }
@@ -1327,7 +1381,8 @@ __global__ __launch_bounds__(64) void k_fftz_r(Geo g, const cpx<T> *__restrict__
 
 // x forward: block = row J and TK = 2 LPW consecutive z-planes (plane pair per line).
 template <typename T, int L>
-__global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restrict__ tw, double coe0,
+__global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restrict__ tw, const double *__restrict__ dv,
+                                                    double talp,
                                                     const double *__restrict__ q, double *__restrict__ div,
                                                     cpx<T> *__restrict__ spec, const double *__restrict__ src)
 {
@@ -1339,6 +1394,7 @@ __global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restr
     const size_t hs = (size_t)g.N * (g.M + 2) * L;
     const double *u = q, *v = q + hs, *w = q + 2 * hs;
     const double dyf = g.dyfi[J - 1];
+    const double coe0 = dv ? 1.0 / (dv[0] * talp) : 1.0;   // 1 / (alpha DT) (DIVG_tg)
     cpx<T> x[G::E];
 #pragma unroll
     for (int b = 0; b < G::E / G::R0; ++b)
@@ -1353,11 +1409,11 @@ __global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restr
                     d[h] = src[cix(g, i, J, k)];
                 } else {
                     const int ip = (i == L - 1) ? 0 : i + 1, kp = (k == g.N - 1) ? 0 : k + 1;
-                    const double dd = (u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
-                                      (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * dyf +
-                                      (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi;
+                    const double dd = divg_cell(u[hix(g, ip, J, k)], u[hix(g, i, J, k)], v[hix(g, i, J + 1, k)],
+                                                v[hix(g, i, J, k)], w[hix(g, i, J, kp)], w[hix(g, i, J, k)], g.dxi,
+                                                dyf, g.dzi);
                     d[h] = dd * coe0;
-                    div[cix(g, i, J, k)] = d[h];
+                    if (div) div[cix(g, i, J, k)] = d[h];
                 }
             }
             x[b * G::R0 + r] = {(T)d[0], (T)d[1]};
@@ -1421,11 +1477,102 @@ __global__ __launch_bounds__(64) void k_fftx_inv_r(Geo g, const cpx<T> *__restri
     });
 }
 
-// VELOUPDT_tg (VELO_CORRECTION.f90:31-71) + PRCALC_tg (PRESSURE_CORRECTION.f90:26-36) +
-// BC_WALL_Q_tg / BC_WALL_PR_TG ghost rows.
+// Projection of a substep: VELOUPDT_tg (VELO_CORRECTION.f90:31-71), PRCALC_tg
+// (PRESSURE_CORRECTION.f90:26-36) and the ghost rows of BC_WALL_Q_tg / BC_WALL_PR_TG.  The
+// predicted velocity qs (written by the y-solve) minus alpha DT grad phi goes to q.  Because the
+// prediction has its own buffer, every neighbour is read race-free: the divergence PRCALC_tg
+// needs (RHSLLPHI, the Poisson RHS) is recomputed here rather than stored by the transform
+// kernel, and with CFL the CFL_tg value of each cell of the corrected field (CFL.f90:27-46) is
+// formed from corrected neighbours computed alike, with block maxima into part.  One thread
+// per (i, J) marches KZ planes (phi and w* of plane k+1 become the next plane's own).
+// Contraction off: a correction is the same product and difference wherever it is formed, so
+// a neighbour's corrected value here equals the one its own thread stores.
+template <bool CFL>
 __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__restrict__ phi,
-                                                const double *__restrict__ div, double *__restrict__ q,
-                                                double *__restrict__ pr, int refine)
+                                                const double *__restrict__ qs, double *__restrict__ q,
+                                                double *__restrict__ pr, int KZ, double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k0 = blockIdx.z * KZ;
+    double cmax = 0.0;
+    if (i < g.L && J <= g.M) {
+        const int L = g.L, M = g.M, N = g.N;
+        const size_t hs = (size_t)N * (M + 2) * L;
+        const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+        const int k1 = min(N, k0 + KZ);
+        const double sdt = s.dv[0], scv = s.dv[1];
+        const double ta = sdt * s.talp;
+        const double cx = ta * g.dxi, cz = ta * g.dzi, cy = ta * g.dyci[J];
+        const double cyp = J < M ? ta * g.dyci[J + 1] : 0.0;
+        const double be = 0.5 * s.talp * sdt * scv;
+        const double coe0 = 1.0 / (sdt * s.talp);
+        const double dyf = g.dyfi[J - 1];
+        auto H = [&](int ii, int jj, int kk) { return ((size_t)kk * (M + 2) + jj) * L + ii; };
+        double phm = phi[H(i, J, k0 == 0 ? N - 1 : k0 - 1)];   // phi at plane k-1
+        double phk = phi[H(i, J, k0)];
+        double wsk = qs[2 * hs + H(i, J, k0)];                 // w* at plane k
+        for (int k = k0; k < k1; ++k) {
+            const int kp = (k == N - 1) ? 0 : k + 1;
+            const size_t h0 = H(i, J, k);
+            const double php = phi[H(i, J, kp)], wsp = qs[2 * hs + H(i, J, kp)];
+            const double us = qs[h0], usp = qs[H(ip, J, k)], vs = qs[hs + h0], vsp = qs[hs + H(i, J + 1, k)];
+            const double pim = phi[H(im, J, k)], pjm = phi[H(i, J - 1, k)];
+            const double d = divg_cell(usp, us, vsp, vs, wsp, wsk, g.dxi, dyf, g.dzi) * coe0;
+            const double un = us - (phk - pim) * cx;
+            const double wn = wsk - (phk - phm) * cz;
+            const double vn = J >= 2 ? vs - (phk - pjm) * cy : 0.0;   // row 1 is the bottom wall
+            const double pn = pr[h0] + phk - be * d;
+            q[h0] = un;
+            q[hs + h0] = vn;
+            q[2 * hs + h0] = wn;
+            pr[h0] = pn;
+            if (J == 1) {
+                const size_t hb = H(i, 0, k);
+                pr[hb] = pn;
+                q[hb] = 0.0;
+                q[hs + hb] = 0.0;
+                q[2 * hs + hb] = 0.0;
+            }
+            if (J == M) {
+                const size_t ht = H(i, M + 1, k);
+                pr[ht] = pn;
+                q[ht] = 0.0;
+                q[hs + ht] = 0.0;
+                q[2 * hs + ht] = 0.0;
+            }
+            if constexpr (CFL) {
+                const double unp = usp - (phi[H(ip, J, k)] - phk) * cx;
+                const double vnp = J < M ? vsp - (phi[H(i, J + 1, k)] - phk) * cyp : 0.0;
+                const double wnp = wsp - (php - phk) * cz;
+                const double a = fabs(un + unp) * g.dxi;
+                const double b = fabs(vn + vnp) * dyf;
+                const double c = fabs(wn + wnp) * g.dzi;
+                cmax = nanmax(cmax, (a + b + c) * 0.5);
+            }
+            phm = phk;
+            phk = php;
+            wsk = wsp;
+        }
+    }
+    if constexpr (CFL) {
+        __shared__ double red[256];
+        const int t = threadIdx.y * 64 + threadIdx.x;
+        red[t] = cmax;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) red[t] = nanmax(red[t], red[t + o]);
+            __syncthreads();
+        }
+        if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+    }
+}
+
+// A refinement sweep's correction, in place: the velocity as k_update, p += phi (PRCALC_tg's
+// viscous term used the first solve's RHS only), ghost rows.
+__global__ __launch_bounds__(256) void k_update_refine(Geo g, Sub s, const double *__restrict__ phi,
+                                                       double *__restrict__ q, double *__restrict__ pr)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
@@ -1434,18 +1581,13 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__re
     const size_t hs = (size_t)g.N * (g.M + 2) * g.L;
     const int im = (i == 0) ? g.L - 1 : i - 1, km = (k == 0) ? g.N - 1 : k - 1;
     const double p0 = phi[hix(g, i, J, k)];
-    const double cx = s.dt * s.talp * g.dxi, cz = s.dt * s.talp * g.dzi;
+    const double ta = s.dv[0] * s.talp;
+    const double cx = ta * g.dxi, cz = ta * g.dzi;
     const size_t h0 = hix(g, i, J, k);
     q[h0] = q[h0] - (p0 - phi[hix(g, im, J, k)]) * cx;
     q[2 * hs + h0] = q[2 * hs + h0] - (p0 - phi[hix(g, i, J, km)]) * cz;
-    if (J >= 2) {
-        const double cy = s.dt * s.talp * g.dyci[J];
-        q[hs + h0] = q[hs + h0] - (p0 - phi[hix(g, i, J - 1, k)]) * cy;
-    }
-    const double be = 0.5 * s.talp * s.dt * s.cvisc;
-    // a refinement sweep adds its correction phi to p; the viscous term of PRCALC_tg uses the
-    // first solve's RHS (the divergence of the predicted velocity) only
-    const double pn = refine ? pr[h0] + p0 : pr[h0] + p0 - be * div[cix(g, i, J, k)];
+    if (J >= 2) q[hs + h0] = q[hs + h0] - (p0 - phi[hix(g, i, J - 1, k)]) * (ta * g.dyci[J]);
+    const double pn = pr[h0] + p0;
     pr[h0] = pn;
     if (J == 1) {
         pr[hix(g, i, 0, k)] = pn;
@@ -1540,6 +1682,7 @@ __global__ __launch_bounds__(256) void k_gauge_mean(Geo g, const double *__restr
 __global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict__ q, double *__restrict__ part,
                                                  int which)
 {
+#pragma clang fp contract(off)   // the reference's rounding: each product and sum rounded
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
     const int k = blockIdx.z;
@@ -1554,9 +1697,8 @@ __global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict
             const double c = fabs(w[hix(g, i, J, k)] + w[hix(g, i, J, kp)]) * g.dzi;
             val = (a + b + c) * 0.5;
         } else {
-            val = fabs((u[hix(g, ip, J, k)] - u[hix(g, i, J, k)]) * g.dxi +
-                       (v[hix(g, i, J + 1, k)] - v[hix(g, i, J, k)]) * g.dyfi[J - 1] +
-                       (w[hix(g, i, J, kp)] - w[hix(g, i, J, k)]) * g.dzi);
+            val = fabs(divg_cell(u[hix(g, ip, J, k)], u[hix(g, i, J, k)], v[hix(g, i, J + 1, k)], v[hix(g, i, J, k)],
+                                 w[hix(g, i, J, kp)], w[hix(g, i, J, k)], g.dxi, g.dyfi[J - 1], g.dzi));
         }
     }
     __shared__ double red[256];
@@ -1570,14 +1712,15 @@ __global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict
     if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
 }
 
-__global__ void k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
+// Maximum of n block partials by one block of blockDim.x (a power of two <= 1024) threads.
+__global__ __launch_bounds__(1024) void k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
 {
-    __shared__ double red[256];
+    __shared__ double red[1024];
     double a = 0.0;
-    for (int t = threadIdx.x; t < n; t += 256) a = nanmax(a, part[t]);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) a = nanmax(a, part[t]);
     red[threadIdx.x] = a;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) red[threadIdx.x] = nanmax(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
@@ -1717,7 +1860,6 @@ struct CnTab {
     int Dx = 0, Dz = 0, Dy[3] = {0, 0, 0};
     int fullx = 0, fullz = 0;
     void *gx = nullptr, *gz = nullptr, *by[3] = {nullptr, nullptr, nullptr};
-    size_t bycap[3] = {0, 0, 0};
 };
 
 }  // namespace
@@ -1735,8 +1877,14 @@ struct chs_ctx {
     Geo geo;
     const double *d_amph = nullptr, *d_apph = nullptr, *cf_acph = nullptr;
     double *q = nullptr, *pr = nullptr, *phi = nullptr, *div = nullptr;
+    double *qs = nullptr;   // predicted velocity of a substep (y-solve output, projection input)
     void *h0 = nullptr, *rhs = nullptr, *rhs2 = nullptr, *spec = nullptr;
     double *pp = nullptr, *partial = nullptr, *scal = nullptr;
+    double *dstep = nullptr;   // scal + 8: device step scalars [DT, CVISC, CFLMM, PhyTIME, stop]
+    double *cn_stamp = nullptr;   // (DT, CVISC) each stage's Crank-Nicolson tables were built for
+    bool dv_host = false;         // dstep holds (dv_dt, dv_cv) set from the host
+    bool cfl_valid = false;       // dstep[2] holds CFLMM_tg of the current field
+    double dv_dt = 0.0, dv_cv = 0.0;
     float *pp32 = nullptr;   // FP32 copy of the Poisson pivots (mixed form, high modes)
     double *pp00 = nullptr;  // (0,0)-mode pivots without FFT99's decoupling (FISHPACK gauge)
     int npartial = 0;
@@ -1761,6 +1909,7 @@ struct chs_ctx {
         int ytdma_bs = 64, ytdma_rb = 16;   // k_ytdma threads per block, rows per load batch
         int ytdma_c = -1;                   // k_ytdma_chunk rows per chunk (-1 auto = 16, 0 = k_ytdma)
         int conv_lds = 2;                   // k_conv_rhs_lds rows per block (1, 2, 4; 0 = k_conv_rhs)
+        int upd_kz = 16;                    // k_update z-planes per thread
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning / chs_set_refinement (graphs are re-made)
     int cn_gen = 0;     // bumped by every rebuild of a Crank-Nicolson table (its memory may move)
@@ -1775,6 +1924,9 @@ struct chs_ctx {
     hipGraphExec_t gexec = nullptr;
     double gdt = 0.0, gcv = 0.0;
     int gtune = -1, gcn = -1;   // tune_gen and cn_gen the step graph was captured with
+    int gkind = 0;              // 0: chs_step_tg's graph (host DT), 1: chs_run_tg's (device DT)
+    double grule[4] = {0, 0, 0, 0};   // DT0, CFLGV, DTMIN, CVISC of a chs_run_tg graph
+    bool dstate = false;        // chs_set_step_state has set DT and PhyTIME
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 };
 
@@ -1873,57 +2025,28 @@ void tridiag_inverse(int n, const std::vector<long double> &a, const std::vector
     }
 }
 
-int build_cn(chs_ctx *c, int ns, double dt, double cvisc)
+// Band half-widths of stage ns's Crank-Nicolson inverses at (dt, cvisc), in long double:
+// entries whose sum stays below tol of the diagonal (2^-64 FP64, 1e-12 mixed) are dropped.
+void cn_widths(const chs_ctx *c, int ns, double dt, double cvisc, CnTab &t)
 {
-    CnTab &t = c->cn[ns];
-    if (t.valid && t.dt == dt && t.cvisc == cvisc) return 0;
-    // Tables of an earlier (dt, cvisc) may still be read by enqueued kernels; captured graphs
-    // that hold their addresses are re-made (cn_gen).
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    ++c->cn_gen;
     const bool mixed = c->mode == CHS_MIXED;
     const long double tol = mixed ? 1e-12L : 1e-19L;
     const long double beta = 0.5L * (long double)c->talp[ns] * (long double)dt * (long double)cvisc;
-    auto upload = [&](void **dst, size_t *cap, const std::vector<long double> &src) -> int {
-        const size_t need = src.size() * wsize(c);
-        if (!*dst || (cap && *cap < need)) {
-            if (*dst) HIPCHK(c, hipFree(*dst));
-            HIPCHK(c, hipMalloc(dst, need));
-            if (cap) *cap = need;
-        }
-        if (mixed) {
-            std::vector<float> h(src.begin(), src.end());
-            HIPCHK(c, hipMemcpyAsync(*dst, h.data(), need, hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        } else {
-            std::vector<double> h(src.begin(), src.end());
-            HIPCHK(c, hipMemcpyAsync(*dst, h.data(), need, hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
-        return 0;
-    };
-    // x (MOMFA1_X_tg, MOMFA_tg.f90:72-82): a = c = -FACOE*DXQI, b = 1 - a - c
-    {
-        long double co = -beta * (long double)c->dxqi;
+    {   // x (MOMFA1_X_tg, MOMFA_tg.f90:72-82): a = c = -FACOE*DXQI, b = 1 - a - c
+        const long double co = -beta * (long double)c->dxqi;
         std::vector<long double> g;
         circulant_inverse(c->L, co, 1.0L - co - co, g);
         t.Dx = circulant_width(g, tol);
         t.fullx = (2 * t.Dx + 1 >= c->L) ? 1 : 0;
-        if (t.gx) { HIPCHK(c, hipFree(t.gx)); t.gx = nullptr; }
-        if (int e = upload(&t.gx, nullptr, g)) return e;
     }
-    // z (MOMFA2_Z_tg, MOMFA_tg.f90:129-146)
-    {
-        long double co = 1.0L * (-beta * (long double)c->dzqi);
+    {   // z (MOMFA2_Z_tg, MOMFA_tg.f90:129-146)
+        const long double co = -beta * (long double)c->dzqi;
         std::vector<long double> g;
         circulant_inverse(c->N, co, 1.0L - co - co, g);
         t.Dz = circulant_width(g, tol);
         t.fullz = (2 * t.Dz + 1 >= c->N) ? 1 : 0;
-        if (t.gz) { HIPCHK(c, hipFree(t.gz)); t.gz = nullptr; }
-        if (int e = upload(&t.gz, nullptr, g)) return e;
     }
-    // y (MOMFA3_Y_tg, MOMFA_tg.f90:206-216): rows J = 1..NCL2
-    const int M = c->M;
+    const int M = c->M;   // y (MOMFA3_Y_tg, MOMFA_tg.f90:206-216): rows J = 1..NCL2
     for (int comp = 0; comp < 3; ++comp) {
         std::vector<long double> a(M), b(M), cc(M), inv;
         for (int J = 1; J <= M; ++J) {
@@ -1937,7 +2060,7 @@ int build_cn(chs_ctx *c, int ns, double dt, double cvisc)
         tridiag_inverse(M, a, b, cc, inv);
         int D = 0;
         for (int J = 0; J < M; ++J) {
-            long double diag = fabsl(inv[(size_t)J * M + J]);
+            const long double diag = fabsl(inv[(size_t)J * M + J]);
             long double dropped = 0.0L;
             int dJ = M - 1;
             for (int d = M - 1; d >= 1; --d) {
@@ -1950,14 +2073,87 @@ int build_cn(chs_ctx *c, int ns, double dt, double cvisc)
             }
             if (dJ > D) D = dJ;
         }
-        const int w = 2 * D + 1;
-        std::vector<long double> band((size_t)M * w, 0.0L);
-        for (int J = 0; J < M; ++J)
-            for (int d = -D; d <= D; ++d)
-                if (J + d >= 0 && J + d < M) band[(size_t)J * w + d + D] = inv[(size_t)J * M + J + d];
         t.Dy[comp] = D;
-        if (int e = upload(&t.by[comp], &t.bycap[comp], band)) return e;
     }
+}
+
+// Device tables of every stage, at full size once (x: L entries, z: N, y: M rows of up to
+// 2M - 1), so that no rebuild moves them.
+int cn_alloc(chs_ctx *c)
+{
+    if (c->cn[0].gx) return 0;
+    const size_t ws = wsize(c), M = c->M;
+    for (auto &t : c->cn) {
+        HIPCHK(c, hipMalloc(&t.gx, ws * c->L));
+        HIPCHK(c, hipMalloc(&t.gz, ws * c->N));
+        for (auto &b : t.by) HIPCHK(c, hipMalloc(&b, ws * M * (2 * M - 1)));
+    }
+    HIPCHK(c, hipMalloc(&c->cn_stamp, sizeof(double) * 8));
+    // stamps no DT can equal: the first launch of k_cn_tables builds every stage
+    const double nan = std::nan("");
+    const double st[8] = {nan, nan, nan, nan, nan, nan, nan, nan};
+    HIPCHK(c, hipMemcpy(c->cn_stamp, st, sizeof st, hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename W>
+CnDev<W> cn_dev(const chs_ctx *c)
+{
+    CnDev<W> d{};
+    for (int ns = 0; ns < 4; ++ns) {
+        d.gx[ns] = (W *)c->cn[ns].gx;
+        d.gz[ns] = (W *)c->cn[ns].gz;
+        for (int k = 0; k < 3; ++k) {
+            d.by[ns][k] = (W *)c->cn[ns].by[k];
+            d.Dy[ns][k] = c->cn[ns].Dy[k];
+        }
+        d.talp[ns] = c->talp[ns];
+    }
+    return d;
+}
+
+// Enqueues the rebuild of stages ns0 .. ns0 + n - 1 for the device's current DT and CVISC.
+void cn_launch(chs_ctx *c, int ns0, int n)
+{
+    const dim3 grid(5, n);
+    if (c->mode == CHS_MIXED)
+        hipLaunchKernelGGL(k_cn_tables<float>, grid, dim3(256), 0, c->stream, c->geo, cn_dev<float>(c), c->dstep,
+                           c->cn_stamp, ns0);
+    else
+        hipLaunchKernelGGL(k_cn_tables<double>, grid, dim3(256), 0, c->stream, c->geo, cn_dev<double>(c), c->dstep,
+                           c->cn_stamp, ns0);
+    hipLaunchKernelGGL(k_cn_stamp, dim3(1), dim3(64), 0, c->stream, c->dstep, c->cn_stamp, ns0, n);
+}
+
+// Host-given DT and CVISC for the next launches (a no-op when the device already holds them).
+void set_step(chs_ctx *c, double dt, double cvisc)
+{
+    if (c->dv_host && c->dv_dt == dt && c->dv_cv == cvisc) return;
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, c->stream, c->dstep, dt, cvisc);
+    c->dv_host = true;
+    c->dv_dt = dt;
+    c->dv_cv = cvisc;
+}
+
+// Stage ns's tables for a host-given (dt, cvisc): widths on the host, entries on the device
+// (no synchronisation).  A change of widths changes the kernels a step launches, so captured
+// graphs are re-made (cn_gen).
+int build_cn(chs_ctx *c, int ns, double dt, double cvisc)
+{
+    CnTab &t = c->cn[ns];
+    if (t.valid && t.dt == dt && t.cvisc == cvisc) return 0;
+    if (int e = cn_alloc(c)) return e;
+    CnTab w = t;
+    cn_widths(c, ns, dt, cvisc, w);
+    const bool changed = w.Dx != t.Dx || w.Dz != t.Dz || w.fullx != t.fullx || w.fullz != t.fullz ||
+                         w.Dy[0] != t.Dy[0] || w.Dy[1] != t.Dy[1] || w.Dy[2] != t.Dy[2];
+    if (!t.valid || changed) ++c->cn_gen;
+    t.Dx = w.Dx; t.Dz = w.Dz; t.fullx = w.fullx; t.fullz = w.fullz;
+    for (int k = 0; k < 3; ++k) t.Dy[k] = w.Dy[k];
+    if (changed) hipLaunchKernelGGL(k_cn_stamp, dim3(1), dim3(64), 0, c->stream, nullptr, c->cn_stamp, ns, 1);
+    set_step(c, dt, cvisc);
+    cn_launch(c, ns, 1);
+    HIPCHK(c, hipGetLastError());
     t.dt = dt;
     t.cvisc = cvisc;
     t.valid = true;
@@ -2032,11 +2228,11 @@ void fftz_r(int N, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, cp
     }
 }
 template <typename T>
-void divg_fftx_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, double coe0, const double *q,
-                 double *div, cpx<T> *spec, const double *src)
+void divg_fftx_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, const double *dv, double talp,
+                 const double *q, double *div, cpx<T> *spec, const double *src)
 {
     switch (L) {
-#define X(n) case n: hipLaunchKernelGGL((k_divg_fftx_r<T, n>), grid, dim3(64), 0, st, g, tw, coe0, q, div, spec, src); break;
+#define X(n) case n: hipLaunchKernelGGL((k_divg_fftx_r<T, n>), grid, dim3(64), 0, st, g, tw, dv, talp, q, div, spec, src); break;
         CHS_RCASES(X)
 #undef X
     }
@@ -2064,8 +2260,14 @@ int ytdma_chunk_rows(int want, int M)
 
 // gauge 0: FFT99's (0,0) rule (row 1 decoupled), 1: TRID0's (last row pinned, FISHPACK)
 template <typename T>
-void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
+// dv: device DT for the divergence scaling 1 / (talp DT) of a substep (null: a given RHS, src).
+// The RHS is the divergence of the velocity at vel (c->q when null), stored into div (c->div
+// when null; not stored when div_out is false), or the cell field src.
+void poisson_launch(chs_ctx *c, const double *dv, double talp, const double *src, int gauge = 0,
+                    const double *vel = nullptr, bool div_out = true)
 {
+    const double *qv = vel ? vel : c->q;
+    double *dvo = div_out ? c->div : nullptr;
     const Geo &g = c->geo;
     const int L = c->L, M = c->M, N = c->N, H = c->H;
     const bool rx = c->tune.fft_reg && c->rtwx, rz = c->tune.fft_reg && c->rtwz;
@@ -2079,10 +2281,10 @@ void poisson_launch(chs_ctx *c, double coe0, const double *src, int gauge = 0)
     {
         EvScope ev(c, "divg_fftx");
         if (rx)
-            divg_fftx_r<T>(L, dim3(M, N / TKR), c->stream, g, (const cpx<T> *)c->rtwx, coe0, c->q, c->div, spec, src);
+            divg_fftx_r<T>(L, dim3(M, N / TKR), c->stream, g, (const cpx<T> *)c->rtwx, dv, talp, qv, dvo, spec, src);
         else
-            hipLaunchKernelGGL(k_divg_fftx<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa, coe0, c->q,
-                               c->div, spec, TK, src);
+            hipLaunchKernelGGL(k_divg_fftx<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa, dv, talp, qv, dvo,
+                               spec, TK, src);
     }
     {
         EvScope ev(c, "fftz_fwd");
@@ -2152,23 +2354,30 @@ void cn_xz_dispatch(int D, dim3 grid, dim3 blk, size_t sh, hipStream_t st, const
 }
 
 template <typename W>
-void cn_y_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *by, W *rin, double *q, int comp,
-                   int KY)
+void cn_y_dispatch(int D, dim3 grid, dim3 blk, hipStream_t st, const Geo &g, const W *by, W *rin, const double *qin,
+                   double *qout, int comp, int KY)
 {
     switch (D) {
-#define CHS_YCASE(d) case d: hipLaunchKernelGGL((k_cn_y_march<W, d>), grid, blk, 0, st, g, by, rin, q, comp, KY); break;
+#define CHS_YCASE(d) case d: hipLaunchKernelGGL((k_cn_y_march<W, d>), grid, blk, 0, st, g, by, rin, qin, qout, comp, KY); break;
         CHS_YCASE(0) CHS_YCASE(1) CHS_YCASE(2) CHS_YCASE(3) CHS_YCASE(4) CHS_YCASE(5) CHS_YCASE(6)
         CHS_YCASE(7) CHS_YCASE(8)
 #undef CHS_YCASE
     }
 }
 
+// One substep SOLVERRK3_MOM_tg(ns).  host_dt: DT and CVISC given by the host (tables and the
+// device step scalars are brought to them first); otherwise the device already holds them
+// (a step whose DT the device chose).  cfl: the correction also forms CFLMM_tg of the new
+// field into dstep[2] (CFL_tg, CFL.f90:13-57).
 template <typename W>
-int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
+int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = true, bool cfl = false)
 {
-    if (int e = build_cn(c, ns, dt, cvisc)) return e;
+    if (host_dt) {
+        if (int e = build_cn(c, ns, dt, cvisc)) return e;
+        set_step(c, dt, cvisc);
+    }
     const Geo &g = c->geo;
-    const Sub s = {c->tgam[ns], c->troh[ns], c->talp[ns], dt, cvisc};
+    const Sub s = {c->tgam[ns], c->troh[ns], c->talp[ns], c->dstep};
     const int L = c->L, M = c->M, N = c->N;
     const size_t cs = (size_t)N * M * L;
     W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
@@ -2257,25 +2466,43 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc)
         EvScope ev(c, "cn_y");
         if (t.Dy[comp] <= 8) {
             const dim3 gy((L + 63) / 64, (N + 3) / 4, (M + KY - 1) / KY);
-            cn_y_dispatch<W>(t.Dy[comp], gy, blk, c->stream, g, (const W *)t.by[comp], rhs2, c->q, comp, KY);
+            cn_y_dispatch<W>(t.Dy[comp], gy, blk, c->stream, g, (const W *)t.by[comp], rhs2, c->q, c->qs, comp, KY);
         } else {
             hipLaunchKernelGGL(k_cn_y<W>, grd, blk, 0, c->stream, g, (const W *)t.by[comp], t.Dy[comp], rhs2,
-                               comp == 0 ? rhs2 + cs : nullptr, c->q, comp);
+                               comp == 0 ? rhs2 + cs : nullptr, c->q, c->qs, comp);
         }
     }
-    poisson_launch<W>(c, 1.0 / (dt * c->talp[ns]), nullptr);
+    // Poisson RHS from the prediction (c->qs); the correction recomputes the divergence
+    poisson_launch<W>(c, c->dstep, c->talp[ns], nullptr, 0, c->qs, false);
+    // FP64-residual refinement (mixed form): the divergence of the corrected velocity drives
+    // another solve and correction; the CFL then comes from the final field
+    const int nref = ns == 0 ? c->nref_projection : c->nref_step;
+    const int KU = c->tune.upd_kz;
+    const dim3 gu((L + 63) / 64, (M + 3) / 4, (N + KU - 1) / KU);
     {
         EvScope ev(c, "update");
-        hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr, 0);
+        if (cfl && nref == 0)
+            hipLaunchKernelGGL(k_update<true>, gu, blk, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
+                               c->partial);
+        else
+            hipLaunchKernelGGL(k_update<false>, gu, blk, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
+                               c->partial);
     }
-    // FP64-residual refinement (mixed form): the divergence of the corrected velocity drives
-    // another solve and correction
-    const int nref = ns == 0 ? c->nref_projection : c->nref_step;
     for (int r = 0; r < nref; ++r) {
-        poisson_launch<W>(c, 1.0 / (dt * c->talp[ns]), nullptr);
+        poisson_launch<W>(c, c->dstep, c->talp[ns], nullptr, 0, c->q, false);
         EvScope ev(c, "refine_update");
-        hipLaunchKernelGGL(k_update, grd, blk, 0, c->stream, g, s, c->phi, c->div, c->q, c->pr, 1);
+        hipLaunchKernelGGL(k_update_refine, grd, blk, 0, c->stream, g, s, c->phi, c->q, c->pr);
     }
+    if (cfl) {
+        EvScope ev(c, "cfl");
+        int np = gu.x * gu.y * gu.z;
+        if (nref > 0) {
+            hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 0);
+            np = grd.x * grd.y * grd.z;
+        }
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, c->stream, c->partial, np, c->dstep + 2);
+    }
+    c->cfl_valid = cfl;
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -2290,12 +2517,12 @@ int poisson_device_impl(chs_ctx *c, const double *rhs, double *phi, int nref, do
     const dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
     const int npart = grd.x * grd.y * grd.z;
     const double *acph = c->cf_acph;
-    poisson_launch<T>(c, 1.0, rhs);
+    poisson_launch<T>(c, nullptr, 1.0, rhs);
     hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 0, (const double *)nullptr);
     for (int r = 0; r < nref; ++r) {
         hipLaunchKernelGGL(k_poisson_resid, grd, blk, 0, c->stream, g, c->d_amph, acph, c->d_apph, rhs, phi,
                            c->div, c->partial);
-        poisson_launch<T>(c, 1.0, c->div);
+        poisson_launch<T>(c, nullptr, 1.0, c->div);
         // FP32 transforms: restore the exact gauge in FP64 on the last sweep.
         const double *shift = nullptr;
         if (sizeof(T) == 4 && r == nref - 1) {
@@ -2325,7 +2552,7 @@ int poisson_impl(chs_ctx *c, const double *rhs, double *phi, int gauge)
     HIPCHK(c, hipMalloc(&src, sizeof(double) * cs));
     hipError_t e = hipMemcpyAsync(src, rhs, sizeof(double) * cs, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        poisson_launch<T>(c, 1.0, src, gauge);
+        poisson_launch<T>(c, nullptr, 1.0, src, gauge);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -2545,6 +2772,9 @@ static int create_impl(const chs_coeffs *cf, int mode, hipStream_t shared, chs_c
     c->npartial = ((L + 63) / 64) * M * N;
     CK(hipMalloc(&c->partial, sizeof(double) * c->npartial));
     CK(hipMalloc(&c->scal, sizeof(double) * 16));
+    c->dstep = c->scal + 8;
+    CK(hipMalloc(&c->qs, sizeof(double) * 3 * hs));
+    CK(hipMemsetAsync(c->qs, 0, sizeof(double) * 3 * hs, c->stream));
     CK(hipMemsetAsync(c->q, 0, sizeof(double) * 3 * hs, c->stream));
     CK(hipMemsetAsync(c->pr, 0, sizeof(double) * hs, c->stream));
     CK(hipMemsetAsync(c->phi, 0, sizeof(double) * hs, c->stream));
@@ -2609,8 +2839,8 @@ void chs_destroy(chs_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->cf, c->q, c->pr, c->phi, c->div, c->h0, c->rhs, c->rhs2, c->spec, c->pp,
-                    c->pp32, c->pp00, c->partial, c->scal, c->twx, c->twz, c->rtwx, c->rtwz};
+    void *ptrs[] = {c->cf, c->q, c->qs, c->pr, c->phi, c->div, c->h0, c->rhs, c->rhs2, c->spec, c->pp,
+                    c->pp32, c->pp00, c->partial, c->scal, c->twx, c->twz, c->rtwx, c->rtwz, c->cn_stamp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : c->cn) {
@@ -2645,6 +2875,7 @@ int chs_upload_fields(chs_ctx *c, const double *q, const double *pr, const doubl
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_fields = true;
+    c->cfl_valid = false;
     return 0;
 }
 
@@ -2681,27 +2912,36 @@ int chs_substep_tg(chs_ctx *c, int ns, double dt, double cvisc)
 int chs_step_tg(chs_ctx *c, double dt, double cvisc)
 {
     if (!c) return fail(c, CHS_E_ARG, "null ctx");
-    if (!c->use_graph || c->prof) {
-        for (int ns = 1; ns <= 3; ++ns)
-            if (int e = chs_substep_tg(c, ns, dt, cvisc)) return e;
-        return 0;
-    }
     if (!(dt > 0) || !(cvisc > 0)) return fail(c, CHS_E_ARG, "dt and cvisc must be positive");
     if (!c->have_fields) return fail(c, CHS_E_STATE, "no fields uploaded");
-    // tables first (a no-op when they are current): building them synchronises the stream,
-    // which capture forbids, and a rebuild moves them, which a captured graph cannot follow
+    auto run = [&](int ns, bool host) {
+        return c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt, cvisc, host, ns == 3)
+                                    : substep_impl<double>(c, ns, dt, cvisc, host, ns == 3);
+    };
+    if (!c->use_graph || c->prof) {
+        for (int ns = 1; ns <= 3; ++ns)
+            if (int e = run(ns, true)) return e;
+        return 0;
+    }
+    // widths and tables first (no-ops when current): a change of widths changes the kernels,
+    // which a captured graph cannot follow
     for (int ns = 1; ns <= 3; ++ns)
         if (int e = build_cn(c, ns, dt, cvisc)) return e;
-    if (!c->gexec || c->gdt != dt || c->gcv != cvisc || c->gtune != c->tune_gen || c->gcn != c->cn_gen) {
+    if (!c->gexec || c->gdt != dt || c->gcv != cvisc || c->gtune != c->tune_gen || c->gcn != c->cn_gen ||
+        c->gkind != 0) {
         if (c->gexec) {
             HIPCHK(c, hipGraphExecDestroy(c->gexec));
             c->gexec = nullptr;
         }
         hipGraph_t graph = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        // the graph sets DT, CVISC and (where their stamps differ) the tables itself, so a
+        // replay does not depend on what other calls left on the device
+        c->dv_host = false;
+        set_step(c, dt, cvisc);
+        cn_launch(c, 1, 3);
         int e = 0;
-        for (int ns = 1; ns <= 3 && !e; ++ns)
-            e = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt, cvisc) : substep_impl<double>(c, ns, dt, cvisc);
+        for (int ns = 1; ns <= 3 && !e; ++ns) e = run(ns, false);
         hipError_t ce = hipStreamEndCapture(c->stream, &graph);
         if (e) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -2715,8 +2955,114 @@ int chs_step_tg(chs_ctx *c, double dt, double cvisc)
         c->gcv = cvisc;
         c->gtune = c->tune_gen;
         c->gcn = c->cn_gen;
+        c->gkind = 0;
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->dv_host = true;
+    c->dv_dt = dt;
+    c->dv_cv = cvisc;
+    c->cfl_valid = true;
+    return 0;
+}
+
+int chs_run_tg(chs_ctx *c, int nsteps, double dt0, double cflgv, double dtmin, double cvisc)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    if (nsteps < 0 || !(dt0 > 0) || !(cflgv > 0) || !(dtmin >= 0) || !(cvisc > 0))
+        return fail(c, CHS_E_ARG, "chs_run_tg: nsteps >= 0, dt0 > 0, cflgv > 0, dtmin >= 0, cvisc > 0");
+    if (!c->have_fields) return fail(c, CHS_E_STATE, "no fields uploaded");
+    if (!c->dstate) return fail(c, CHS_E_STATE, "chs_run_tg: set the step state first (chs_set_step_state)");
+    if (int e = cn_alloc(c)) return e;
+    // widths for the largest DT the rule allows (DT0): every smaller DT needs no wider band
+    for (int ns = 1; ns <= 3; ++ns) {
+        CnTab &t = c->cn[ns];
+        CnTab w = t;
+        cn_widths(c, ns, dt0, cvisc, w);
+        const bool changed = w.Dx != t.Dx || w.Dz != t.Dz || w.fullx != t.fullx || w.fullz != t.fullz ||
+                             w.Dy[0] != t.Dy[0] || w.Dy[1] != t.Dy[1] || w.Dy[2] != t.Dy[2];
+        if (changed) {
+            ++c->cn_gen;
+            t.Dx = w.Dx; t.Dz = w.Dz; t.fullx = w.fullx; t.fullz = w.fullz;
+            for (int k = 0; k < 3; ++k) t.Dy[k] = w.Dy[k];
+            hipLaunchKernelGGL(k_cn_stamp, dim3(1), dim3(64), 0, c->stream, nullptr, c->cn_stamp, ns, 1);
+        }
+        t.valid = false;   // the device, not the host, chooses the DT these tables hold
+    }
+    if (!c->cfl_valid) {   // CFLMM of the current field for the first step's rule
+        const dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
+        hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, c->geo, c->q, c->partial, 0);
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, c->stream, c->partial,
+                           (int)(grd.x * grd.y * grd.z), c->dstep + 2);
+        c->cfl_valid = true;
+    }
+    c->dv_host = false;
+    auto step = [&]() -> int {
+        hipLaunchKernelGGL(k_dt_rule, dim3(1), dim3(64), 0, c->stream, c->dstep, dt0, cflgv, dtmin, cvisc);
+        cn_launch(c, 1, 3);
+        for (int ns = 1; ns <= 3; ++ns) {
+            const int e = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, 0.0, cvisc, false, ns == 3)
+                                               : substep_impl<double>(c, ns, 0.0, cvisc, false, ns == 3);
+            if (e) return e;
+        }
+        return 0;
+    };
+    if (!c->use_graph || c->prof) {
+        for (int n = 0; n < nsteps; ++n)
+            if (int e = step()) return e;
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    }
+    const double key[4] = {dt0, cflgv, dtmin, cvisc};
+    if (!c->gexec || c->gkind != 1 || c->gtune != c->tune_gen || c->gcn != c->cn_gen ||
+        std::memcmp(key, c->grule, sizeof key) != 0) {
+        if (c->gexec) {
+            HIPCHK(c, hipGraphExecDestroy(c->gexec));
+            c->gexec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int e = step();
+        hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (e) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return e;
+        }
+        if (ce != hipSuccess) return fail(c, CHS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        ce = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ce != hipSuccess) return fail(c, CHS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ce));
+        std::memcpy(c->grule, key, sizeof key);
+        c->gtune = c->tune_gen;
+        c->gcn = c->cn_gen;
+        c->gkind = 1;
+    }
+    for (int n = 0; n < nsteps; ++n) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    return 0;
+}
+
+int chs_set_step_state(chs_ctx *c, double dt, double phytime)
+{
+    if (!c || !(dt > 0)) return fail(c, CHS_E_ARG, "chs_set_step_state: dt must be positive");
+    const double v[3] = {dt, phytime, 0.0};
+    // DT, PhyTIME and a cleared stop flag; CVISC stays (the rule writes it each step)
+    HIPCHK(c, hipMemcpyAsync(c->dstep + 0, &v[0], sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dstep + 3, &v[1], sizeof(double) * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dv_host = false;
+    c->dstate = true;
+    return 0;
+}
+
+int chs_get_step_state(chs_ctx *c, double *dt, double *phytime, double *cflmm, int *stopped)
+{
+    if (!c) return fail(c, CHS_E_ARG, "null ctx");
+    double v[5];
+    HIPCHK(c, hipMemcpyAsync(v, c->dstep, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (dt) *dt = v[0];
+    if (phytime) *phytime = v[3];
+    if (cflmm) *cflmm = c->cfl_valid ? v[2] : std::nan("");
+    if (stopped) *stopped = v[4] != 0.0;
     return 0;
 }
 
@@ -2801,6 +3147,9 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "ytdma_bs") {
         if (value != 64 && value != 128 && value != 256) return bad("must be 64, 128 or 256");
         c->tune.ytdma_bs = value;
+    } else if (k == "upd_kz") {
+        if (value < 1 || value > N) return bad("must be 1..N");
+        c->tune.upd_kz = value;
     } else if (k == "conv_lds") {
         if (value != 0 && value != 1 && value != 2 && value != 4) return bad("must be 0, 1, 2 or 4");
         c->tune.conv_lds = value;
@@ -2850,13 +3199,16 @@ int chs_poisson_device(chs_ctx *c, const double *rhs, double *phi, int nref, dou
 int chs_cfl_tg(chs_ctx *c, double dt, double *cflmm)
 {
     if (!c || !cflmm) return fail(c, CHS_E_ARG, "null argument");
-    const Geo &g = c->geo;
-    dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
-    hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 0);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, (int)(grd.x * grd.y * grd.z),
-                       c->scal + 1);
+    if (!c->cfl_valid) {   // a step forms CFLMM_tg of its new field; otherwise compute it here
+        const Geo &g = c->geo;
+        dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
+        hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 0);
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, c->stream, c->partial,
+                           (int)(grd.x * grd.y * grd.z), c->dstep + 2);
+        c->cfl_valid = true;
+    }
     double v = 0.0;
-    HIPCHK(c, hipMemcpyAsync(&v, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&v, c->dstep + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *cflmm = v * dt;
     if (!std::isfinite(v)) return fail(c, CHS_E_NONFINITE, "chs_cfl_tg: non-finite velocity");
@@ -2869,7 +3221,7 @@ int chs_max_div(chs_ctx *c, double *maxdiv)
     const Geo &g = c->geo;
     dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
     hipLaunchKernelGGL(k_cfl_div, grd, blk, 0, c->stream, g, c->q, c->partial, 1);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, c->stream, c->partial, (int)(grd.x * grd.y * grd.z),
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, c->stream, c->partial, (int)(grd.x * grd.y * grd.z),
                        c->scal + 2);
     double v = 0.0;
     HIPCHK(c, hipMemcpyAsync(&v, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2891,6 +3243,7 @@ int chs_device_fields(chs_ctx *c, double **q, double **pr)
     if (!c) return fail(c, CHS_E_ARG, "null ctx");
     if (q) *q = c->q;
     if (pr) *pr = c->pr;
+    c->cfl_valid = false;   // the caller may write the fields
     return 0;
 }
 
@@ -3023,13 +3376,19 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
         hipError_t ce = hipEventRecord(e->ev[0], e->stream);
         for (int s = 0; s < nl && ce == hipSuccess; ++s) ce = hipStreamWaitEvent(e->lanes[s], e->ev[0], 0);
         if (ce != hipSuccess) rc = fail(nullptr, CHS_E_HIP, std::string("chs_ens_step fork: ") + hipGetErrorString(ce));
-        // RK stages outermost; channel i on lane i % nl (its stages stay in order on that lane)
+        // RK stages outermost; channel i on lane i % nl (its stages stay in order on that lane).
+        // Each channel's DT, CVISC and (stamp-guarded) tables are set inside the graph.
         for (int ns = 1; ns <= 3 && !rc; ++ns)
             for (int i = 0; i < n && !rc; ++i) {
                 chs_ctx *c = e->ch[i];
                 c->stream = e->lanes[i % nl];
-                rc = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt[i], cvisc[i])
-                                          : substep_impl<double>(c, ns, dt[i], cvisc[i]);
+                if (ns == 1) {
+                    c->dv_host = false;
+                    set_step(c, dt[i], cvisc[i]);
+                    cn_launch(c, 1, 3);
+                }
+                rc = c->mode == CHS_MIXED ? substep_impl<float>(c, ns, dt[i], cvisc[i], false, ns == 3)
+                                          : substep_impl<double>(c, ns, dt[i], cvisc[i], false, ns == 3);
             }
         for (auto *c : e->ch) c->stream = e->stream;
         // join: the origin stream waits for every lane before the capture ends
@@ -3058,6 +3417,13 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
     }
     hipError_t he = hipGraphLaunch(e->gexec, e->stream);
     if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));
+    for (int i = 0; i < n; ++i) {
+        chs_ctx *c = e->ch[i];
+        c->dv_host = true;
+        c->dv_dt = dt[i];
+        c->dv_cv = cvisc[i];
+        c->cfl_valid = true;
+    }
     return 0;
 }
 

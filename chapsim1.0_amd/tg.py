@@ -40,6 +40,9 @@ SIGNATURES = {
     "chs_download_fields": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "chs_substep_tg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
     "chs_step_tg": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "chs_run_tg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "chs_set_step_state": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "chs_get_step_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_int)]),
     "chs_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_set_refinement": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "chs_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
@@ -251,6 +254,21 @@ class ChannelTG:
 
     def rk3_step(self, dt: float, cvisc: float):
         _check(lib().chs_step_tg(self._h, float(dt), float(cvisc)), self._h)
+
+    def run(self, nsteps: int, dt0: float, cflgv: float, dtmin: float, cvisc: float):
+        """nsteps steps of SOLVE's loop with DT chosen on the device by BCAST_COMM_STEP's rule
+        (chs_run_tg); set_step_state first."""
+        _check(lib().chs_run_tg(self._h, int(nsteps), float(dt0), float(cflgv), float(dtmin), float(cvisc)),
+               self._h)
+
+    def set_step_state(self, dt: float, phytime: float = 0.0):
+        _check(lib().chs_set_step_state(self._h, float(dt), float(phytime)), self._h)
+
+    def step_state(self) -> dict:
+        """DT, PhyTIME, CFLMM (NaN until formed) and the DT / DT0 < 0.01 stop flag."""
+        dt, t, cf, st = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        _check(lib().chs_get_step_state(self._h, C.byref(dt), C.byref(t), C.byref(cf), C.byref(st)), self._h)
+        return dict(dt=dt.value, phytime=t.value, cflmm=cf.value, stopped=bool(st.value))
 
     def set_graph(self, enable: bool):
         """Replay rk3_step from a hipGraph captured per (dt, cvisc)."""

@@ -102,9 +102,26 @@ int chs_download_fields(chs_ctx *ctx, double *q, double *pr, double *convh0);
  * the context's stream; returns without waiting. */
 int chs_substep_tg(chs_ctx *ctx, int ns, double dt, double cvisc);
 
-/* One RK3 step: NS = 1, 2, 3 (SOLVE.f90:114-134, TG branch).  With graphs enabled the
- * step's launches are captured once per (dt, cvisc) into a hipGraph and replayed. */
+/* One RK3 step: NS = 1, 2, 3 (SOLVE.f90:114-134, TG branch), ending with CFLMM_tg of the new
+ * field (CFL_tg, CFL.f90:13-57) formed on the device for the next step's rule (read it with
+ * chs_cfl_tg or chs_get_step_state; no host synchronisation inside the step).  With graphs
+ * enabled the step's launches are captured once per (dt, cvisc) into a hipGraph and replayed. */
 int chs_step_tg(chs_ctx *ctx, double dt, double cvisc);
+
+/* nsteps steps of SOLVE's time loop with the time step chosen on the device (SOLVE.f90:74-134
+ * with BCAST_COMM_STEP's rule, :169-178): per step DT = max(min(DT0, 5 DT, CFLGV / CFLMM),
+ * DTMIN) from CFLMM_tg of the current field (CFL.f90:13-57) and the last DT, PhyTIME += DT,
+ * the Crank-Nicolson tables rebuilt on the device when DT changes, then the three substeps;
+ * cvisc = CVISC.  Nothing returns to the host and a captured step graph is never re-made for a
+ * new DT.  The reference stops once DT / DT0 < 0.01 (SOLVE.f90:199-207): the device records
+ * that in the step state.  Requires chs_set_step_state first.  Returns without waiting. */
+int chs_run_tg(chs_ctx *ctx, int nsteps, double dt0, double cflgv, double dtmin, double cvisc);
+
+/* The device step state of chs_run_tg: set DT (the last step's, or DT0 at the start) and
+ * PhyTIME, clearing the stop flag (synchronous); read DT, PhyTIME, CFLMM_tg of the current
+ * field (NaN if no step or chs_cfl_tg has formed it) and the stop flag (synchronous). */
+int chs_set_step_state(chs_ctx *ctx, double dt, double phytime);
+int chs_get_step_state(chs_ctx *ctx, double *dt, double *phytime, double *cflmm, int *stopped);
 
 /* Enable (1) or disable (0, default) hipGraph replay of chs_step_tg. */
 int chs_set_graph(chs_ctx *ctx, int enable);
@@ -183,7 +200,9 @@ int chs_poisson_fishpack(chs_ctx *ctx, const double *rhs, double *phi);
 int chs_poisson_device(chs_ctx *ctx, const double *rhs, double *phi, int nref, double *resmax);
 
 /* Per-step diagnostics (CFL_tg, CFL.f90:13-57, and DIVGCK_tg, DIVGCK.f90:15-66), computed
- * on the device from the current fields; synchronous (a scalar read back). */
+ * on the device from the current fields (the CFL that a step already formed is reused);
+ * synchronous (a scalar read back).  CHS_E_NONFINITE if the value is not finite (a NaN in
+ * the field propagates through the maximum). */
 int chs_cfl_tg(chs_ctx *ctx, double dt, double *cflmm);
 int chs_max_div(chs_ctx *ctx, double *maxdiv);
 

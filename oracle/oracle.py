@@ -343,6 +343,29 @@ def io_initial_field(mesh, seed, rho_amp=0.3, mu_amp=0.3):
     return dict(G=g, G0=g0, PR=pr, D0=d0, M0=m0, EXPLT0=explt0)
 
 
+def cfl_tg(mesh, q):
+    """CFL_tg (CFL.f90:13-57) restated: max over cells of
+    (|(u_i + u_i+1) DXI| + |(v_J + v_J+1) DYFI| + |(w_k + w_k+1) DZI|) / 2 in the reference's
+    operation order (RCCI1 = RCCI2 = 1 in the channel).  q [3, N, M+2, L].  Returns CFLMM_tg."""
+    M = mesh.M
+    dyfi = np.asarray(mesh.DYFI)[None, :, None]
+    u, v, w = q[0][:, 1:M + 1], q[1], q[2][:, 1:M + 1]
+    qcf = (np.abs((u + np.roll(u, -1, axis=2)) * mesh.DXI) + np.abs((v[:, 1:M + 1] + v[:, 2:M + 2]) * dyfi)
+           + np.abs((w + np.roll(w, -1, axis=0)) * mesh.DZI)) * 0.5
+    return float(np.max(qcf))
+
+
+def max_div_tg(mesh, q):
+    """DIVGCK_tg (DIVGCK.f90:15-58) restated: max over cells of
+    |(u_i+1 - u_i) DXI + (v_J+1 - v_J) DYFI + (w_k+1 - w_k) DZI|.  q [3, N, M+2, L]."""
+    M = mesh.M
+    dyfi = np.asarray(mesh.DYFI)[None, :, None]
+    u, v, w = q[0][:, 1:M + 1], q[1], q[2][:, 1:M + 1]
+    d = ((np.roll(u, -1, axis=2) - u) * mesh.DXI + (v[:, 2:M + 2] - v[:, 1:M + 1]) * dyfi
+         + (np.roll(w, -1, axis=0) - w) * mesh.DZI)
+    return float(np.max(np.abs(d)))
+
+
 def cfl_io(mesh, q, dens, visc, cvisc):
     """CFL_io (CFL.f90:59-107) restated: max over cells of the convective rate
     (|u_i + u_i+1| DXI + |v_J + v_J+1| DYFI + |w_k + w_k+1| DZI) / 2 and the viscous rate

@@ -42,6 +42,8 @@
 !   ioprops NS              DENSITY_TIME_DERIVATION(NS), DENSITY_Staggered, MU_Staggered
 !                           and, with iVisScheme = 1, DENSITY_IMPLICIT_add (SOLVE.f90:122-127)
 !   loadvisc FILE           Viscousity (the current viscosity) from a global L x (M+2) x N file
+!   cfltg FILE              CFL_tg; rank 0 writes CFLMM_tg to FILE
+!   divcktg FILE            DIVGCK_tg; rank 0 writes MAXDIVGV_tg(2) to FILE
 !   cflio FILE              VELOCITY_CALC_io and CFL_io; rank 0 writes CFLMM_io to FILE
 !   divckio FILE            DIVGCK_io; rank 0 writes MAXDIVGV_io(1:3) to FILE
 !   iocn NS DT FILE         DT = given; the momentum part of SOLVERRK3_MOM_io(NS) with the
@@ -262,6 +264,24 @@ PROGRAM ref_harness
                 OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
                      STATUS = 'replace', ACTION = 'write')
                 WRITE(80) CFLMM_io
+                CLOSE(80)
+            END IF
+        CASE ('cfltg')
+            READ(line, *) cmd, fname
+            CALL CFL_tg
+            IF (MYID == 0) THEN
+                OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+                     STATUS = 'replace', ACTION = 'write')
+                WRITE(80) CFLMM_tg
+                CLOSE(80)
+            END IF
+        CASE ('divcktg')
+            READ(line, *) cmd, fname
+            CALL DIVGCK_tg
+            IF (MYID == 0) THEN
+                OPEN(80, FILE = TRIM(fname), ACCESS = 'STREAM', FORM = 'UNFORMATTED', &
+                     STATUS = 'replace', ACTION = 'write')
+                WRITE(80) MAXDIVGV_tg(2)
                 CLOSE(80)
             END IF
         CASE ('divckio')

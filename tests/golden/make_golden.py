@@ -14,9 +14,10 @@ Also ``props_2048.npz`` (property tables, make_props), ``tdma.npz`` (the referen
 batched Thomas solvers, make_tdma) and ``fishpack.npz`` (the main-domain FISHPACK Poisson
 solve, make_fishpack), ``eng.npz`` (the energy substep, make_eng) and ``io.npz`` (the
 main-domain momentum substep, make_io) and ``meanzx.npz`` (x-z mean statistics,
-make_meanzx).
+make_meanzx) and ``cfl_div_tg.npz`` (CFL_tg and DIVGCK_tg, make_cfl).
 
-Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng] [io] [meanzx]   (needs oracle/_ref)
+Run: python tests/golden/make_golden.py [tg] [props] [tdma] [fishpack] [eng] [io] [meanzx] [cfl]
+(needs oracle/_ref)
 """
 from __future__ import annotations
 
@@ -68,6 +69,29 @@ def make_case(L, M, N, STR2, REN, dt, seed):
     finally:
         h.close()
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"tg_{L}x{M}x{N}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path))
+
+
+def make_cfl():
+    """``cfl_div_tg.npz``: CFLMM_tg (CFL_tg) and MAXDIVGV_tg (DIVGCK_tg) of the reference on
+    the fields of the tg fixtures (the seeded start, which is not divergence-free, the NS = 0
+    projection and one RK3 step), for each mesh: keys ``<L>x<M>x<N>_<field>_{cfl,div}``."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    out = {}
+    for (L, M, N, STR2, REN, dt, seed) in CASES:
+        g = dict(np.load(os.path.join(here, f"tg_{L}x{M}x{N}.npz")))
+        h = refrun.Harness(L, M, N, STR2=STR2, REN=REN, DT=dt)
+        try:
+            for tag, q, pr in (("init", g["q_init"], g["pr_init"]), ("ns0", g["q_ns0"], g["pr_ns0"]),
+                               ("step1", g["q_step1"], g["pr_step1"])):
+                h.write_fields("f.bin", q[0], q[1], q[2], pr)
+                h.run(["load f.bin", "cfltg cfl.bin", "divcktg div.bin"])
+                out[f"{L}x{M}x{N}_{tag}_cfl"] = np.fromfile(h.path("cfl.bin"), dtype=np.float64)
+                out[f"{L}x{M}x{N}_{tag}_div"] = np.fromfile(h.path("div.bin"), dtype=np.float64)
+        finally:
+            h.close()
+    path = os.path.join(here, "cfl_div_tg.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path))
 
@@ -358,7 +382,7 @@ def make_meanzx():
 
 if __name__ == "__main__":
     import sys as _sys
-    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io", "io_cn", "meanzx"]
+    which = _sys.argv[1:] or ["tg", "props", "tdma", "fishpack", "eng", "io", "io_cn", "meanzx", "cfl"]
     if "tg" in which:
         for case in CASES:
             make_case(*case)
@@ -376,3 +400,5 @@ if __name__ == "__main__":
         make_io_cn()
     if "meanzx" in which:
         make_meanzx()
+    if "cfl" in which:
+        make_cfl()

@@ -50,3 +50,27 @@ def test_poisson(golden, name):
     o, _ = make(g)
     phi = o.poisson(g["poisson_rhs"])
     assert rel(phi, g["poisson_phi"]) < 1e-10
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_cfl_and_divergence_check(golden, name):
+    """CFL_tg and DIVGCK_tg restated in the oracle equal the reference's values bit for bit on
+    a field that is not divergence-free (the seeded start), the projected one and one RK3
+    step later; a dropped term or a wrong neighbour index changes the value."""
+    g = golden(name)
+    ref = golden("cfl_div_tg.npz")
+    L, M, N, STR2 = (g["params"][i] for i in range(4))
+    L, M, N = int(L), int(M), int(N)
+    m = setup.Mesh(L, M, N, STR2=STR2)
+    m.DYFI = np.asarray(g["s_DYFI"][:M])   # the reference's own set-up (the restated one agrees to ulps)
+    for tag in ("init", "ns0", "step1"):
+        q = g[f"q_{tag}"]
+        assert oracle.cfl_tg(m, q) == ref[f"{L}x{M}x{N}_{tag}_cfl"][0], tag
+        assert oracle.max_div_tg(m, q) == ref[f"{L}x{M}x{N}_{tag}_div"][0], tag
+    # planted mistakes: a shifted v neighbour, the z term dropped
+    q = g["q_init"].copy()
+    q[1] = np.roll(q[1], 1, axis=1)
+    assert oracle.cfl_tg(m, q) != ref[f"{L}x{M}x{N}_init_cfl"][0]
+    q = g["q_init"].copy()
+    q[2] = 0.0 * q[2]
+    assert oracle.max_div_tg(m, q) != ref[f"{L}x{M}x{N}_init_div"][0]
