@@ -45,36 +45,6 @@ CFG = dict(L=256, M=192, N=256, HX=4 * np.pi, HZ=2 * np.pi, STR2=2.0, REN=6875.0
 CFG64 = dict(L=64, M=64, N=64, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8, REN=2800.0, dt=1.0e-3)
 
 
-def ren_from_retau(re_tau):
-    """Bulk Reynolds number on the half height from Re_tau by Dean's correlation
-    Re_tau = 0.09 (2 REN)^0.88 (REN = 2800 <-> Re_tau ~ 180, as the reference's templates)."""
-    return (re_tau / 0.09) ** (1.0 / 0.88) / 2.0
-
-
-def ensemble_cases(n=32, seed=4):
-    """Seeded channel sizes for the ensemble workload: L, N in [64, 192] and M in [64, 128],
-    L and N even with no prime factor above 5 (FFT99's condition, lib/fft99/FFTFAX.f90:56-91),
-    Re_tau uniform in [180, 395]."""
-    ok = [v for v in range(64, 193, 2) if _smooth(v)]
-    rng = np.random.Generator(np.random.Philox(key=seed))
-    cases = []
-    for _ in range(n):
-        L = int(rng.choice(ok))
-        N = int(rng.choice(ok))
-        M = int(rng.integers(64, 129))
-        rt = float(rng.uniform(180.0, 395.0))
-        cases.append(dict(L=L, M=M, N=N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8 + 0.2 * (rt - 180) / 215,
-                          REN=ren_from_retau(rt), dt=5.0e-4, re_tau=rt))
-    return cases
-
-
-def _smooth(v):
-    for p in (2, 3, 5):
-        while v % p == 0:
-            v //= p
-    return v == 1
-
-
 def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
     """Algorithmic HBM bytes of one launch: every array element the kernel must read or
     write, counted once.  F = L*M*N cells; fields u,v,w,p,phi,div are FP64; RHS/history and
@@ -154,12 +124,12 @@ def start_field(pkg, c):
 
 
 def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotune=False, shapes=None,
-            roundtrip=0):
+            roundtrip=0, start=None):
     c = c or CFG
     coeffs = pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"])
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
     ch = pkg.ChannelTG(coeffs, mode)
-    q, pr = start_field(pkg, c)
+    q, pr = start if start is not None else start_field(pkg, c)
     ch.upload(q, pr)
     cv = 1.0 / c["REN"]
     dt = c["dt"]
@@ -204,14 +174,17 @@ def run_gpu(pkg, form: str, steps: int, warmup: int, c=None, graph=False, autotu
     dom_avg_ms, dom_n = ch.profile_get(dom)
     if graph:                       # kernel time from the profiling pre-pass instead
         dom_avg_ms, dom_n = pre_avg, pre_n
-    cfl = ch.cfl(dt)
+    cfl = ch.cfl(dt)               # CFLMM * DT of the last step's new field, formed inside it
     div = ch.max_div()
+    mon = ch.monitor(c["REN"])
+    final = ch.download()
     rt = host_roundtrip(pkg, ch, dt, cv, roundtrip) if roundtrip else None
     ch.close()
     ms = (t1 - t0) * 1e3 / steps
     return dict(ms_per_step=ms, per_kernel_ms=per, launches=launches, dominant=dom,
                 dom_avg_ms=dom_avg_ms, dom_count=dom_n, cfl=cfl, max_div=div,
-                wbytes=4 if form == "mixed" else 8, start=(q0, p0), tuning=tuning, roundtrip_ms=rt)
+                wbytes=4 if form == "mixed" else 8, start=(q0, p0), tuning=tuning, roundtrip_ms=rt,
+                final=final, monitor=mon, steps_run=2 + warmup + (1 if graph else 0) + steps)
 
 
 def host_roundtrip(pkg, ch, dt, cv, steps):
@@ -248,6 +221,32 @@ def _monitor_cput(work, prefix):
     return cput
 
 
+def host_cores(M: int, N: int):
+    """Ranks for the reference on this host: its physical cores among the CPUs this process may
+    use (/proc/cpuinfo physical id + core id), capped by the CPU share the environment states
+    (OMP_NUM_THREADS, set to the share of one GPU on the GPU box) and by min(M, N), which the
+    reference's y / z decomposition needs (BASELINE: P = physical cores, P <= min(M, N))."""
+    aff = set(os.sched_getaffinity(0))
+    phys, cur, seen = set(), {}, False
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in list(fh) + ["\n"]:
+                if not line.strip():
+                    if "processor" in cur and int(cur["processor"]) in aff:
+                        phys.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                        seen = True
+                    cur = {}
+                elif ":" in line:
+                    k, v = line.split(":", 1)
+                    cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    n_phys = len(phys) if seen else len(aff)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None
+    P = min(n_phys, share or n_phys, M, N)
+    return max(2, P), dict(logical_cpus=len(aff), physical_cores=n_phys, cpu_share=share)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -272,8 +271,8 @@ def cpu_baseline_heated(sample_steps: int = 3, cores: int | None = None, vis_sch
         return None
     c = CFG_HEATED
     L, M, N = c["L"], c["M"], c["N"]
-    P = cores or max(2, min(16, len(os.sched_getaffinity(0))))
-    P = min(P, M, N)
+    P, hc = host_cores(M, N)
+    P = cores or P
 
     def run(steps):
         work = tempfile.mkdtemp(prefix="chs_cpu_")
@@ -301,14 +300,15 @@ def cpu_baseline_heated(sample_steps: int = 3, cores: int | None = None, vis_sch
     if res.returncode != 0 or not cput:
         return dict(error=f"reference run failed rc={res.returncode}: {res.stderr[-400:]}")
     s = float(np.median(cput))
-    return dict(value=L * M * N / s, unit="grid-point RK3-steps/s", cores=P, kind="reference",
+    return dict(value=L * M * N / s, unit="grid-point RK3-steps/s", cores=P, kind="reference", host=hc,
                 sample=f"reference CHAPSim (oracle/_ref/chapsim, amdflang -O2) mpirun -np {P} on "
-                       f"{_cpu_model()}; heated channel {L}x{M}x{N}, iVisScheme={vis_scheme}, "
+                       f"{_cpu_model()} ({hc['physical_cores']} physical cores usable, CPU share "
+                       f"{hc['cpu_share']}); heated channel {L}x{M}x{N}, iVisScheme={vis_scheme}, "
                        f"{len(cput)} RK3 steps from its own random start; median per-step CpuT {s:.3f} s; "
                        f"run wall {wall:.1f} s")
 
 
-def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = None, c=None):
+def cpu_baseline_reference(start, sample_steps: int = 6, cores: int | None = None, c=None):
     """Time the reference program on this host: restart from the projected start field
     (the reference checks max div < 1e-6 on restart, FLOWSTART.f90:41-46) written in its
     restart format (processing/WRT_VARS_TG.f90:12-104), `sample_steps` steps, median of the
@@ -319,9 +319,8 @@ def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = Non
         return None
     c = c or CFG
     L, M, N = c["L"], c["M"], c["N"]
-    ncpu = len(os.sched_getaffinity(0))
-    P = cores or max(2, min(16, ncpu))
-    P = min(P, M, N)
+    P, hc = host_cores(M, N)
+    P = cores or P
     work = tempfile.mkdtemp(prefix="chs_cpu_")
     try:
         ini = refrun.ini_text(L, M, N, HX=c["HX"], HZ=c["HZ"], STR2=c["STR2"], REN=c["REN"],
@@ -348,10 +347,12 @@ def cpu_baseline_reference(start, sample_steps: int = 3, cores: int | None = Non
             return dict(error=f"reference run failed rc={res.returncode}: {res.stderr[-400:]}")
         s = float(np.median(cput))
         cpu_model = _cpu_model()
-        return dict(value=L * M * N / s, unit="grid-point RK3-steps/s", cores=P, kind="reference",
+        return dict(value=L * M * N / s, unit="grid-point RK3-steps/s", cores=P, kind="reference", host=hc,
                     sample=f"reference CHAPSim (oracle/_ref/chapsim, amdflang -O2) mpirun -np {P} "
-                           f"on {cpu_model or 'host'}; {L}x{M}x{N}, {len(cput)} RK3 steps from a "
-                           f"restart field; median per-step CpuT {s:.3f} s; run wall {wall:.1f} s")
+                           f"on {cpu_model or 'host'} ({hc['physical_cores']} physical cores usable, CPU "
+                           f"share {hc['cpu_share']}, P = min(cores, share, M, N)); {L}x{M}x{N}, "
+                           f"{len(cput)} RK3 steps from a restart field; median per-step CpuT {s:.3f} s; "
+                           f"run wall {wall:.1f} s")
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -417,8 +418,10 @@ def bench_poisson(pkg, args):
 def bench_ensemble(pkg, args):
     """32 independent channels of seeded uneven sizes (config 4), one hipGraph per RK3 step
     of the whole ensemble; value = total grid-point RK3-steps/s."""
+    from chapsim_amd.init import ensemble_cases
     cases = ensemble_cases()
-    mode = pkg.MIXED if args.form == "mixed" else pkg.PLAIN_FP64
+    form = "plain" if args.form == "plain" else "mixed"     # the batched workload runs mixed
+    mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
     cos = [pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"]) for c in cases]
     ens = pkg.Ensemble(cos, mode)
     dt = np.array([c["dt"] for c in cases])
@@ -441,7 +444,7 @@ def bench_ensemble(pkg, args):
     print(json.dumps({"metric": "grid-point RK3-steps/s (ensemble total)", "value": pts / (ms * 1e-3),
                       "unit": "grid-point RK3-steps/s", "n_gpus": 1, "steps": args.steps,
                       "warmup": args.warmup, "ms_per_step": ms, "channels": len(cases),
-                      "grid_points": pts, "form": args.form, "max_div": max(divs),
+                      "grid_points": pts, "form": form, "max_div": max(divs),
                       "config": {"workload": "32 channels, L,N in [64,192], M in [64,128], "
                                              "Re_tau U[180,395], one hipGraph per RK3 step"}}))
 
@@ -525,7 +528,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--form", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--form", choices=["both", "plain", "mixed"], default="both",
+                    help="channel workloads: time the plain FP64 and the mixed form (value = plain), or one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--vis", choices=["cn", "explicit"], default="cn",
                     help="heated workload's viscous scheme (iVisScheme 1 or 0)")
@@ -564,66 +568,96 @@ def main():
         return bench_heated(pkg, args)
     c = CFG64 if args.workload == "channel64" else CFG
     shapes = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set}
-    r = run_gpu(pkg, args.form, args.steps, args.warmup, c, args.graph, args.autotune, shapes,
-                args.host_roundtrip)
+    start = start_field(pkg, c)
+    forms = ["plain", "mixed"] if args.form == "both" else [args.form]
+    res = {f: run_gpu(pkg, f, args.steps, args.warmup, c, args.graph, args.autotune, shapes,
+                      args.host_roundtrip if f == forms[0] else 0, start) for f in forms}
     F = c["L"] * c["M"] * c["N"]
-    value = F / (r["ms_per_step"] * 1e-3)
-    dom = r["dominant"]
-    b = alg_bytes(dom, c["L"], c["M"], c["N"], r["wbytes"])
-    achieved = b / (r["dom_avg_ms"] * 1e-3) / 1e9 if r["dom_avg_ms"] > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get(f"{args.form}:{dom}")
+
+    def form_line(f):
+        r = res[f]
+        dom = r["dominant"]
+        b = alg_bytes(dom, c["L"], c["M"], c["N"], r["wbytes"])
+        achieved = b / (r["dom_avg_ms"] * 1e-3) / 1e9 if r["dom_avg_ms"] > 0 else 0.0
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                traffic = json.load(fh).get(f"{f}:{dom}")
+        kernel_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) * n for k, n in r["launches"].items())
+        step_bytes = canonical_step_bytes(c["L"], c["M"], c["N"], r["wbytes"])
+        return dict(
+            value=F / (r["ms_per_step"] * 1e-3), ms_per_step=r["ms_per_step"],
+            roofline={"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                      "alg_bytes_per_launch": b, "avg_launch_ms": r["dom_avg_ms"], "launches": r["dom_count"]},
+            step_roofline={"alg_bytes_per_step": step_bytes,
+                           "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9,
+                           "frac": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                           "count": "48 field passes per substep (22 FP64, 26 work precision)",
+                           "kernel_sum_bytes_per_step": kernel_bytes},
+            per_kernel_ms_per_step=r["per_kernel_ms"],
+            checks={"cfl_dt": r["cfl"], "max_div": r["max_div"], "monitor": r["monitor"]},
+            tuning=r["tuning"] or "defaults")
+
+    main_form = forms[0]
+    head = form_line(main_form)
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline_reference(r["start"], c=c)
+            cpu = cpu_baseline_reference(res[main_form]["start"], c=c)
         except Exception as e:  # pragma: no cover - host-dependent
             cpu = dict(error=str(e))
         if cpu is None or "error" in cpu:
             err = cpu.get("error") if cpu else "reference build absent"
             cpu = cpu_baseline_port()
             cpu["note"] = f"reference run unavailable: {err}"
-    kernel_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) * n
-                       for k, n in r["launches"].items())
-    step_bytes = canonical_step_bytes(c["L"], c["M"], c["N"], r["wbytes"])
     out = {
         "metric": "grid-point RK3-steps/s, mixed vs plain FP64 at stated error; % of HBM peak",
-        "value": value,
+        "value": head["value"],
         "unit": "grid-point RK3-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "plain" else "f32+f64",
+        "dtype": "f64" if main_form == "plain" else "f32+f64",
         "data": "synthetic: Re_tau 180 field (64^3, 100 steps from laminar + 10% seeded noise) "
                 "interpolated to the mesh, + 5% seeded (Philox) perturbation, projected by NS=0",
         "config": {"workload": (f"periodic channel REN={c['REN']:g}, {c['L']}x{c['M']}x{c['N']}, "
                                 f"4pi x 2 x 2pi, tanh beta={c['STR2']:g}, dt={c['dt']:g}, "
-                                "RK3+CN, flow only"),
+                                "RK3+CN, flow only; a step = 3 substeps + CFL_tg of the new field"),
                    "graph": bool(args.graph),
-                   "tuning": r["tuning"] or "defaults",
-                   "form": "plain_fp64" if args.form == "plain" else "mixed",
+                   "tuning": head["tuning"],
+                   "form": "plain_fp64" if main_form == "plain" else "mixed",
                    "grid": [c["L"], c["M"], c["N"]]},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": b, "avg_launch_ms": r["dom_avg_ms"],
-                     "launches": r["dom_count"]},
-        "step_roofline": {"alg_bytes_per_step": step_bytes,
-                          "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9,
-                          "frac": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "count": "48 field passes per substep (22 FP64, 26 work precision)",
-                          "kernel_sum_bytes_per_step": kernel_bytes},
-        "per_kernel_ms_per_step": r["per_kernel_ms"],
-        "checks": {"cfl_dt": r["cfl"], "max_div": r["max_div"]},
-        "host_roundtrip_ms_per_step": r["roundtrip_ms"],
+        "roofline": head["roofline"],
+        "step_roofline": head["step_roofline"],
+        "per_kernel_ms_per_step": head["per_kernel_ms_per_step"],
+        "checks": head["checks"],
+        "host_roundtrip_ms_per_step": res[main_form]["roundtrip_ms"],
         "cpu_baseline": cpu,
     }
+    if len(forms) == 2:
+        m = form_line("mixed")
+        qp, pp = res["plain"]["final"]
+        qm, pm = res["mixed"]["final"]
+        M = c["M"]
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        ump, umm = qp[0][:, 1:M + 1].mean(axis=(0, 2)), qm[0][:, 1:M + 1].mean(axis=(0, 2))
+        mp, mm = res["plain"]["monitor"], res["mixed"]["monitor"]
+        rt = lambda mo: c["REN"] * 0.5 * (np.sqrt(abs(mo["cf_bottom"]) / 2) + np.sqrt(abs(mo["cf_top"]) / 2))
+        m["speedup_vs_plain"] = head["ms_per_step"] / m["ms_per_step"]
+        m["error_vs_plain"] = {
+            "after_steps": res["mixed"]["steps_run"],
+            "u": rel(qm[0], qp[0]), "v": rel(qm[1], qp[1]), "w": rel(qm[2], qp[2]), "p": rel(pm, pp),
+            "mean_u_profile": rel(umm, ump), "re_tau": abs(rt(mm) / rt(mp) - 1.0),
+            "max_div_mixed": res["mixed"]["max_div"], "max_div_plain": res["plain"]["max_div"]}
+        out["mixed"] = m
     print(json.dumps(out))
 
 

@@ -70,3 +70,34 @@ def add_perturbation(q, seed: int, amp: float):
     q[1, :, 2:M + 1, :] += pert[1, :, 1:, :]
     q[2, :, 1:M + 1, :] += pert[2]
     return q
+
+
+def ren_from_retau(re_tau: float) -> float:
+    """Bulk Reynolds number on the half height from Re_tau by Dean's correlation
+    Re_tau = 0.09 (2 REN)^0.88 (REN = 2800 <-> Re_tau ~ 180, as the reference's templates)."""
+    return (re_tau / 0.09) ** (1.0 / 0.88) / 2.0
+
+
+def _smooth(v: int) -> bool:
+    for p in (2, 3, 5):
+        while v % p == 0:
+            v //= p
+    return v == 1
+
+
+def ensemble_cases(n: int = 32, seed: int = 4):
+    """Seeded channel sizes of the batched ensemble workload: L, N in [64, 192] and M in
+    [64, 128], L and N even with no prime factor above 5 (FFT99's condition,
+    lib/fft99/FFTFAX.f90:56-91), Re_tau uniform in [180, 395] (REN by Dean's correlation),
+    dt = 5e-4, box 4 pi x 2 x 2 pi, tanh stretching 1.8 .. 2.0 with Re_tau."""
+    ok = [v for v in range(64, 193, 2) if _smooth(v)]
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    cases = []
+    for _ in range(n):
+        L = int(rng.choice(ok))
+        N = int(rng.choice(ok))
+        M = int(rng.integers(64, 129))
+        rt = float(rng.uniform(180.0, 395.0))
+        cases.append(dict(L=L, M=M, N=N, HX=4 * np.pi, HZ=2 * np.pi, STR2=1.8 + 0.2 * (rt - 180) / 215,
+                          REN=ren_from_retau(rt), dt=5.0e-4, re_tau=rt))
+    return cases

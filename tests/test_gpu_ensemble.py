@@ -85,3 +85,45 @@ def test_ensemble_more_channels_than_lanes(pkg):
             np.testing.assert_array_equal(p, singles[i][step][1])
             assert ch.max_div() < 1e-10
     ens.close()
+
+
+def test_config4_ensemble_mixed(pkg):
+    """The batched workload itself: 32 seeded channels of uneven size (64^3 .. 192 x 128 x 192,
+    Re_tau 180 .. 395), mixed precision, one graph per RK3 step, three steps.  Each channel
+    equals its own single-channel run bit for bit, keeps its divergence below 1e-9, its mass
+    flux (1e-9 relative) and its wall values."""
+    from chapsim_amd.init import ensemble_cases
+    cases = ensemble_cases()
+    print(f"config-4 ensemble: {len(cases)} channels, {sum(c['L'] * c['M'] * c['N'] for c in cases)} points",
+          flush=True)
+    cos = [pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"]) for c in cases]
+    dt = np.array([c["dt"] for c in cases])
+    cv = np.array([1.0 / c["REN"] for c in cases])
+    ens = pkg.Ensemble(cos, pkg.MIXED)
+    starts, bulk0 = [], []
+    for i, (ch, co) in enumerate(zip(ens.channels, cos)):
+        q, p = pkg.poiseuille_perturbed(co, 1000 + i, 0.1)
+        ch.upload(q, p)
+        ch.SOLVERRK3_MOM_tg(0, dt[i], cv[i])
+        qa, pa = ch.download()
+        starts.append((qa, pa))
+        bulk0.append(np.sum(qa[0][:, 1:co["M"] + 1] / np.asarray(co["dyfi"])[None, :, None]))
+    for _ in range(3):
+        ens.step(dt, cv)
+    ens.sync()
+    for i, (ch, co) in enumerate(zip(ens.channels, cos)):
+        M = co["M"]
+        q, p = ch.download()
+        assert ch.max_div() < 1e-9, i
+        b = np.sum(q[0][:, 1:M + 1] / np.asarray(co["dyfi"])[None, :, None])
+        assert abs(b - bulk0[i]) / abs(bulk0[i]) < 1e-9, i
+        assert np.all(q[:, :, 0] == 0) and np.all(q[:, :, M + 1] == 0) and np.all(q[1, :, 1] == 0)
+        one = pkg.ChannelTG(co, pkg.MIXED)
+        one.upload(*starts[i])
+        for _ in range(3):
+            one.rk3_step(dt[i], cv[i])
+        q1, p1 = one.download()
+        one.close()
+        np.testing.assert_array_equal(q, q1)
+        np.testing.assert_array_equal(p, p1)
+    ens.close()
