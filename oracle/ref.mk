@@ -5,6 +5,7 @@
 #                            (one rank deadlocks in INTFC_VARS3, INTFC_VARS.f90:57-79)
 #   oracle/_ref/ref_harness  oracle/ref_harness.f90 linked against the reference
 #                            objects: calls single reference routines on given inputs
+#   oracle/_ref/chapsim_gpu  the reference program on one rank with the GPU substep (below)
 # Usage: make -f oracle/ref.mk [REF=/root/reference] [-j8]
 REF     ?= /root/reference
 OUT     := oracle/_ref
@@ -43,7 +44,7 @@ SRC5 := dsyev2 dsyevc3 dsyevd3 dsyevh3 dsyevj3 dsyevq3 dsyevv3 dsytrd3 slvsec3 s
 OBJS := $(addprefix $(OBJDIR)/,$(addsuffix .o,$(SRC1) $(SRC2) $(SRC3))) $(OBJDIR)/fftpack.o \
         $(addprefix $(OBJDIR)/,$(addsuffix .o,$(SRC5)))
 
-all: $(OUT)/chapsim $(OUT)/ref_harness $(OUT)/chapsim_dropin
+all: $(OUT)/chapsim $(OUT)/ref_harness $(OUT)/chapsim_gpu
 
 $(OBJDIR):
 	mkdir -p $(OBJDIR)
@@ -65,16 +66,18 @@ $(OUT)/ref_harness: $(OBJDIR)/.stamp oracle/ref_harness.f90
 	$(FC) $(FFLAGS) -c oracle/ref_harness.f90 -o $(OBJDIR)/ref_harness.o
 	$(FC) -o $@ $(OBJDIR)/ref_harness.o $(OBJS) $(LDFLAGS)
 
-# The reference program with SOLVERRK3_MOM_tg.o replaced by the GPU drop-in
-# (chapsim1.0_amd/fortran/solverrk3_mom_tg_hip.f90 over libchapsim_hip.so): demonstrates and
-# tests the Fortran boundary end to end.
+# The reference program run as one process on the GPU: every reference object linked
+# unchanged, with the linker's --wrap sending SOLVERRK3_MOM_tg, CFL_tg, DIVGCK_tg,
+# POSTPROCESS_TG, CALL_TEC360 and INTFC_VARS3/1 to the single-rank, GPU-resident glue
+# chapsim1.0_amd/fortran/chs_tg_single_rank.f90 over libchapsim_hip.so.
 PKG     := chapsim1.0_amd
-DROPOBJ := $(filter-out $(OBJDIR)/SOLVERRK3_MOM_tg.o,$(OBJS))
-$(OUT)/chapsim_dropin: $(OBJDIR)/.stamp $(PKG)/fortran/chapsim_hip_mod.f90 $(PKG)/fortran/solverrk3_mom_tg_hip.f90 $(PKG)/libchapsim_hip.so
+comma   := ,
+WRAPS   := solverrk3_mom_tg_ cfl_tg_ divgck_tg_ postprocess_tg_ call_tec360_ intfc_vars3_ intfc_vars1_
+$(OUT)/chapsim_gpu: $(OBJDIR)/.stamp $(PKG)/fortran/chapsim_hip_mod.f90 $(PKG)/fortran/chs_tg_single_rank.f90 $(PKG)/libchapsim_hip.so
 	$(FC) $(FFLAGS) -c $(PKG)/fortran/chapsim_hip_mod.f90 -o $(OBJDIR)/chapsim_hip_mod.o
-	$(FC) $(FFLAGS) -c $(PKG)/fortran/solverrk3_mom_tg_hip.f90 -o $(OBJDIR)/solverrk3_mom_tg_hip.o
-	$(FC) -o $@ $(OBJDIR)/DNS_THERMAL.o $(DROPOBJ) $(OBJDIR)/chapsim_hip_mod.o \
-	    $(OBJDIR)/solverrk3_mom_tg_hip.o -L$(PKG) -lchapsim_hip -Wl,-rpath,'$$ORIGIN/../../$(PKG)' $(LDFLAGS)
+	$(FC) $(FFLAGS) -c $(PKG)/fortran/chs_tg_single_rank.f90 -o $(OBJDIR)/chs_tg_single_rank.o
+	$(FC) -o $@ $(OBJDIR)/DNS_THERMAL.o $(OBJS) $(OBJDIR)/chapsim_hip_mod.o $(OBJDIR)/chs_tg_single_rank.o \
+	    $(addprefix -Wl$(comma)--wrap=,$(WRAPS)) -L$(PKG) -lchapsim_hip -Wl,-rpath,'$$ORIGIN/../../$(PKG)' $(LDFLAGS)
 
 clean:
 	rm -rf $(OUT)
