@@ -7,6 +7,7 @@
 // (MOMFA_io.f90:147-153) and the triply periodic box Poisson (possion3d_FFT99.f90:355-375).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/chapsim_hip.h"
@@ -82,6 +83,34 @@ __global__ __launch_bounds__(256) void k_tdma_lines(int kind, int isz, int jsz, 
     }
 }
 
+// Work buffer of chs_tdma_lines_noalloc, one per GPU for the life of the process (16 MiB).
+// Host code never uses this name as an address: each device's address comes from
+// hipGetSymbolAddress (scratch_ptr).
+constexpr size_t kScratchDoubles = size_t(1) << 21;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with it current.
+hipError_t scratch_ptr(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -110,6 +139,40 @@ int chs_tdma_lines(int kind, int isz, int jsz, const double *a, const double *b,
     }
     if (e != hipSuccess) {
         g_tdma_err = std::string("chs_tdma_lines: ") + hipGetErrorString(e);
+        return CHS_E_HIP;
+    }
+    return CHS_OK;
+}
+
+int chs_tdma_lines_noalloc(int kind, int isz, int jsz, const double *a, const double *b, const double *c, double *r,
+                           const double *bc)
+{
+    if (kind < 0 || kind > 3) {
+        g_tdma_err = "chs_tdma_lines_noalloc: kind must be 0..3";
+        return CHS_E_ARG;
+    }
+    const bool alongI = (kind == 0 || kind == 2), cyc = (kind == 0 || kind == 3);
+    const int n = alongI ? isz : jsz, nl = alongI ? jsz : isz;
+    if (isz < 1 || jsz < 1 || n < (cyc ? 3 : 2) || !a || !b || !c || !r || (!cyc && !bc)) {
+        g_tdma_err = "chs_tdma_lines_noalloc: bad size or null array (lines need 3 points cyclic, 2 otherwise)";
+        return CHS_E_ARG;
+    }
+    if (3 * (size_t)n * (size_t)nl > kScratchDoubles) {
+        g_tdma_err = "chs_tdma_lines_noalloc: 3 * points per line * lines exceeds the " +
+                     std::to_string(kScratchDoubles) + "-double work buffer";
+        return CHS_E_ARG;
+    }
+    // every work entry a line reads was written earlier by the same line: nothing to clear
+    double *work = nullptr;
+    hipError_t e = scratch_ptr(&work);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_tdma_lines, dim3((nl + 255) / 256), dim3(256), 0, 0, kind, isz, jsz, a, b, c, r,
+                           cyc ? nullptr : bc, work);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(0);
+    }
+    if (e != hipSuccess) {
+        g_tdma_err = std::string("chs_tdma_lines_noalloc: ") + hipGetErrorString(e);
         return CHS_E_HIP;
     }
     return CHS_OK;

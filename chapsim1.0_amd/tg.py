@@ -49,6 +49,8 @@ SIGNATURES = {
     "chs_tdma_lines": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     "chs_tdma_last_error": (C.c_char_p, []),
+    "chs_tdma_lines_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
     "chs_dev_malloc": (C.c_void_p, [C.c_size_t]),
     "chs_dev_free": (C.c_int, [C.c_void_p]),
     "chs_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
@@ -411,19 +413,20 @@ class DeviceArray:
 TDMA_KINDS = {"IJI_CYC": 0, "IJJ_nonCYC": 1, "IJI_nonCYC": 2, "IJJ_CYC": 3}
 
 
-def tdma_lines(kind, A, B, C_, R, bc=None):
+def tdma_lines(kind, A, B, C_, R, bc=None, noalloc=False):
     """The reference's batched Thomas solvers (chs_tdma_lines) on host arrays given as C-order
     [jsz, isz] views of the Fortran A, B, C, R(isz, jsz); bc as [2, nlines] for the
     non-cyclic kinds.  kind: 0/"IJI_CYC" (TDMAIJI_CYC), 1/"IJJ_nonCYC", 2/"IJI_nonCYC",
-    3/"IJJ_CYC".  Returns the solution; the arrays go through device memory."""
+    3/"IJJ_CYC".  Returns the solution; the arrays go through device memory.  noalloc: the
+    entry point without allocation (chs_tdma_lines_noalloc)."""
     k = TDMA_KINDS[kind] if isinstance(kind, str) else int(kind)
     R = np.ascontiguousarray(R, dtype=np.float64)
     jsz, isz = R.shape
     dev = [DeviceArray.from_host(x) for x in (A, B, C_, R)]
     dbc = DeviceArray.from_host(bc) if k in (1, 2) else None
     try:
-        rc = lib().chs_tdma_lines(k, isz, jsz, *[C.c_void_p(d.ptr) for d in dev],
-                                  C.c_void_p(dbc.ptr) if dbc else None)
+        fn = lib().chs_tdma_lines_noalloc if noalloc else lib().chs_tdma_lines
+        rc = fn(k, isz, jsz, *[C.c_void_p(d.ptr) for d in dev], C.c_void_p(dbc.ptr) if dbc else None)
         if rc != 0:
             raise ChsError(f"chs_tdma_lines failed ({rc}): {lib().chs_tdma_last_error().decode()}")
         return dev[3].to_host()

@@ -173,6 +173,14 @@ int chs_tdma_lines(int kind, int isz, int jsz, const double *a, const double *b,
                    const double *bc);
 const char *chs_tdma_last_error(void);
 
+/* chs_tdma_lines without any allocation (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  Same
+ * arguments, kernel and results (bit-identical); its work buffer (3 * points per line * lines
+ * doubles, at most 2^21 = 16 MiB) is a static array each GPU holds for the life of the
+ * process, so calls that overlap on one GPU race.  CHS_E_ARG beyond that size. */
+int chs_tdma_lines_noalloc(int kind, int isz, int jsz, const double *a, const double *b, const double *c, double *r,
+                           const double *bc);
+
 /* Device memory helpers for hosts without their own allocator (e.g. the standalone Poisson
  * workload): plain hipMalloc / hipFree / synchronous hipMemcpy (to_device 1: host->device). */
 void *chs_dev_malloc(size_t bytes);
