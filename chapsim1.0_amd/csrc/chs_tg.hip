@@ -80,6 +80,15 @@ __device__ __forceinline__ int xcd_tile(int nxcd)
     return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
+// A wave-uniform pointer made opaque to the optimiser, so that a lane's 32-bit offset is added
+// at the access (scalar base + vector offset) instead of being folded into a loop-invariant
+// 64-bit vector address that would occupy two registers per stream.
+template <typename P> __device__ __forceinline__ __attribute__((address_space(1))) P *uniform(P *p)
+{
+    asm volatile("" : "+s"(p));
+    return (__attribute__((address_space(1))) P *)p;
+}
+
 // The cell divergence of DIVG_tg (DIVG.f90:31-46) before its 1 / (alpha DT) scaling, in one
 // fixed rounding (no contraction) wherever it is formed: the Poisson RHS and PRCALC_tg's
 // RHSLLPHI are the same numbers.
@@ -481,6 +490,244 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_lds(Geo g, Sub s, co
     constexpr int P2 = (NT & (NT - 1)) ? (NT / 3) * 2 : NT;
     static_assert((P2 & (P2 - 1)) == 0, "block size 2^n or 3 * 2^n");
     double *red = reinterpret_cast<double *>(xs);
+    const int tid = t * L + i;
+    red[tid] = mf;
+    __syncthreads();
+    if (P2 < NT) {
+        if (tid < NT - P2) red[tid] += red[tid + P2];
+        __syncthreads();
+    }
+    for (int o = P2 / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) partial[tile] = red[0];
+}
+
+// s_waitcnt vmcnt(0), resp. lgkmcnt(0), with the other counters left alone (gfx9 encoding:
+// vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int kWaitVm0 = (0x7 << 4) | (0xF << 8);
+constexpr int kWaitLgkm0 = 0xF | (0x7 << 4) | (0x3 << 14);
+
+// A workgroup barrier without the memory fence of __syncthreads: the fence would also wait for
+// direct-to-LDS loads still in flight into another buffer.  Callers wait for what the barrier
+// must cover (lgkmcnt(0) for their LDS stores); the empty asm keeps the compiler from moving
+// memory accesses across.
+__device__ __forceinline__ void bare_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// k_conv_rhs_dma: the arithmetic of k_conv_rhs_lds with the state planes moved into LDS by
+// direct global-to-LDS loads (global_load_lds_dwordx4: a lane moves 16 bytes and a wavefront
+// one 1 KB chunk, with no register in between) into three rotating plane buffers [u v w p]
+// [row J0-1 .. J0+BY][x]: while plane k is computed from the buffers of planes k and k+1, plane
+// k+2 is on its way into the third (the one plane k-1 used).  With no staging registers a block
+// holds BY = 4 rows (1024 threads at L = 256), so the two halo rows add half of the block's own
+// row loads where they add seven eighths at BY = 2.  The RHS rows of the x-solve are written
+// over plane k's buffer once every wavefront has read it.  Per plane: wait for this
+// wavefront's loads of plane k+1 (vmcnt(0)) and barrier (then every wavefront's have landed,
+// and plane k-1's buffer is free: its x-solve is done), issue plane k+2, compute, barrier
+// (plane k read), write the RHS rows, barrier, x-solve.
+template <typename W, int BY, int BXW>
+__global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_dma(Geo g, Sub s, const double *__restrict__ q,
+                                                          const double *__restrict__ pr, W *__restrict__ h0,
+                                                          W *__restrict__ rhs, double *__restrict__ partial,
+                                                          int KC, int nxcd, const W *__restrict__ gx, int Dx)
+{
+    constexpr int L = 64 * BXW, R = BY + 2, NT = L * BY, NW = NT / 64;
+    constexpr int FS = R * L, PB = 4 * FS;        // doubles per field slab, per plane buffer
+    constexpr int NCH = PB / 128;                 // 1 KB chunks per plane buffer
+    static_assert(FS % 128 == 0 && NCH % NW == 0, "a chunk lies in one field; chunks split evenly");
+    constexpr int CPW = NCH / NW;                 // chunks per wavefront per plane
+    constexpr int XH = 16, LP = L + 2 * XH;       // RHS rows carry XH wrapped points each side
+    static_assert(3 * BY * LP * sizeof(W) <= PB * sizeof(double) && NT <= PB, "RHS rows fit a buffer");
+    __shared__ double b0[PB], b1[PB], b2[PB];
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
+    const int i = threadIdx.x, t = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int lane = i & 63, wv = __builtin_amdgcn_readfirstlane((t * L + i) >> 6);
+    const int J0 = ty * BY + 1;
+    const int J = __builtin_amdgcn_readfirstlane(J0 + t);
+    const int N = g.N, M = g.M;
+    const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L;
+    const int kbeg = tz * KC, kend = min(N, kbeg + KC);
+    const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+    auto wrapk = [&](int kk) { return kk >= N ? kk - N : (kk < 0 ? kk + N : kk); };
+    auto grow = [&](int f, int jj, int kk) {
+        const double *b = (f == 3) ? pr : q + (size_t)f * hs;
+        return b + ((size_t)kk * (M + 2) + jj) * L;
+    };
+    // plane kk of rows J0-1 .. J0+BY into buffer B: chunk ch (wave-uniform) is LDS elements
+    // ch*128 .. ch*128+127, lane l moves elements ch*128 + 2l and + 2l + 1
+    auto dma = [&](double *B, int kk) {
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const int ch = wv * CPW + c;
+            const int f = (ch * 128) / FS, cf = ch - f * (FS / 128);   // field, chunk within it
+            const double *src;
+            if constexpr (L % 128 == 0) {   // a chunk is part of one row: uniform base + lane
+                constexpr int CR = L / 128;
+                src = (const double *)(uniform(grow(f, J0 - 1 + cf / CR, kk) + (cf % CR) * 128) + 2 * lane);
+            } else {                        // a chunk spans rows: row and x per lane
+                const int e = cf * 128 + 2 * lane, rr = e / L;
+                src = grow(f, J0 - 1 + rr, kk) + (e - rr * L);
+            }
+            __builtin_amdgcn_global_load_lds((const void *)src,
+                                             (__attribute__((address_space(3))) void *)(B + ch * 128), 16, 0, 0);
+        }
+    };
+    auto put = [&](W *xs, int cc, W v) {
+        W *row = xs + (cc * BY + t) * LP + XH;
+        row[i] = v;
+        if (i < XH) row[L + i] = v;
+        if (i >= L - XH) row[i - L] = v;
+    };
+    double mf = 0.0;
+    {
+        const double wffJ = g.wff[J], wfbJ = g.wfb[J], wffP = g.wff[J + 1], wfbP = g.wfb[J + 1];
+        const double dyf = g.dyfi[J - 1], dyc = g.dyci[J];
+        const int o1 = M + 2, o2 = 2 * (M + 2);
+        const double apu = g.apvr[J], acu = g.acvr[J], amu = g.amvr[J];
+        const double apv = g.apvr[o1 + J], acv = g.acvr[o1 + J], amv = g.amvr[o1 + J];
+        const double apw = g.apvr[o2 + J], acw = g.acvr[o2 + J], amw = g.amvr[o2 + J];
+        const double sdt = s.dv[0], scv = s.dv[1];
+        const W tgam = (W)s.tgam, troh = (W)s.troh, dt = (W)sdt;
+        const double coev = s.talp * scv, coep = s.talp * sdt;
+        const W cu1 = (W)(0.25 * g.dxi), cu2 = (W)(dyf * 0.5), cu3 = (W)(0.25 * g.dzi);
+        const W cv1 = (W)(g.dxi * 0.5), cv2 = (W)(dyc * 0.25), cv3 = (W)(g.dzi * 0.5);
+        // prologue: planes kbeg, kbeg+1 on their way into buffers 0, 1; this point's plane
+        // kbeg-1 values into registers
+        dma(b0, kbeg);
+        dma(b1, wrapk(kbeg + 1));
+        const int km0 = wrapk(kbeg - 1);
+        double u_m = grow(0, J, km0)[i], v_m = grow(1, J, km0)[i], w_m = grow(2, J, km0)[i];
+        double p_m = grow(3, J, km0)[i], uip_m = grow(0, J, km0)[ip], vjp_m = grow(1, J + 1, km0)[i];
+        const int r = t + 1;
+        double rsum = 0.0;
+        W hn[3];
+        auto hload = [&](int kk) {
+            const size_t cc0 = ((size_t)kk * M + (J - 1)) * L;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) hn[f] = uniform(h0 + f * cs + cc0)[i];
+        };
+        hload(kbeg);
+        auto plane = [&](int k, auto cbc) {
+            constexpr int cb = decltype(cbc)::value;
+            double *A = cb == 0 ? b0 : (cb == 1 ? b1 : b2);
+            double *Bn = cb == 0 ? b1 : (cb == 1 ? b2 : b0);
+            double *Cn = cb == 0 ? b2 : (cb == 1 ? b0 : b1);
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            __syncthreads();                                  // planes k, k+1 in LDS
+            if (k + 2 <= kend) dma(Cn, wrapk(k + 2));         // plane kend is the last one read
+            const size_t c0 = ((size_t)k * M + (J - 1)) * L;
+            auto hr0 = uniform(h0 + c0), hr1 = uniform(h0 + cs + c0), hr2 = uniform(h0 + 2 * cs + c0);
+            const W hu = hn[0], hv = hn[1], hw = hn[2];
+            const double *U = A, *V = A + FS, *Wf = A + 2 * FS, *Pf = A + 3 * FS;
+            const double *Un = Bn, *Vn = Bn + FS, *Wn = Bn + 2 * FS;
+            auto at = [&](const double *F, int rr, int x) { return F[rr * L + x]; };
+            const double u_c = at(U, r, i), v_c = at(V, r, i), w_c = at(Wf, r, i), p0 = at(Pf, r, i);
+            const double uip = at(U, r, ip), vjp = at(V, r + 1, i), ujm = at(U, r - 1, i), wim_c = at(Wf, r, im);
+            const double wjm_c = at(Wf, r - 1, i), w_p = at(Wn, r, i);
+            W cu, cv, cw, ru, rv, rw;
+            // ---- u (CONVECTION_X.f90:31-59) ----
+            {
+                const double uim = at(U, r, im), ujp = at(U, r + 1, i), vimjp = at(V, r + 1, im), vim = at(V, r, im);
+                const double u_p = at(Un, r, i), wim_p = at(Wn, r, im), pim = at(Pf, r, im);
+                const W a1 = (W)(uip + u_c), a2 = (W)(uim + u_c);
+                const W H11 = (a1 * a1 - a2 * a2) * cu1;
+                const W H12 = ((W)(vjp + vimjp) * (W)(wffP * ujp + wfbP * u_c) -
+                               (W)(v_c + vim) * (W)(wffJ * u_c + wfbJ * ujm)) * cu2;
+                const W H13 = ((W)(w_p + wim_p) * (W)(u_p + u_c) - (W)(w_c + wim_c) * (W)(u_c + u_m)) * cu3;
+                const W conv = -(H11 + H12 + H13);
+                const W rhsc = tgam * conv + troh * hu;
+                const double lap = (uip - 2.0 * u_c + uim) * g.dxqi + (u_p - 2.0 * u_c + u_m) * g.dzqi +
+                                   (ujp * apu + u_c * acu + ujm * amu);
+                cu = conv;
+                const W pgm = (W)((p0 - pim) * g.dxi * coep);
+                ru = (rhsc + (W)(coev * lap)) * dt - pgm;
+                rsum += (double)ru;
+            }
+            asm volatile("" ::: "memory");
+            // ---- v (CONVECTION_Y.f90:37-84), stored for rows J >= 2 ----
+            {
+                const double uipjm = at(U, r - 1, ip), vip = at(V, r, ip), vim = at(V, r, im), vjm = at(V, r - 1, i);
+                const double v_p = at(Vn, r, i), wjm_p = at(Wn, r - 1, i), pjm = at(Pf, r - 1, i);
+                const W H21 = ((W)(wffJ * uip + wfbJ * uipjm) * (W)(vip + v_c) -
+                               (W)(wffJ * u_c + wfbJ * ujm) * (W)(v_c + vim)) * cv1;
+                const W b1v = (W)(vjp + v_c), b2v = (W)(v_c + vjm);
+                const W H22 = (b1v * b1v - b2v * b2v) * cv2;
+                const W H23 = ((W)(wffJ * w_p + wfbJ * wjm_p) * (W)(v_p + v_c) -
+                               (W)(wffJ * w_c + wfbJ * wjm_c) * (W)(v_c + v_m)) * cv3;
+                const W conv = -(H21 + H22 + H23);
+                const W rhsc = tgam * conv + troh * hv;
+                const double lap = (vip - 2.0 * v_c + vim) * g.dxqi + (v_p - 2.0 * v_c + v_m) * g.dzqi +
+                                   (vjp * apv + v_c * acv + vjm * amv);
+                cv = conv;
+                const W pgm = (W)((p0 - pjm) * dyc * coep);
+                rv = (rhsc + (W)(coev * lap)) * dt - pgm;
+            }
+            asm volatile("" ::: "memory");
+            // ---- w (CONVECTION_Z.f90:33-69) ----
+            {
+                const double wip = at(Wf, r, ip), wjp = at(Wf, r + 1, i);
+                const W H31 = ((W)(uip + uip_m) * (W)(wip + w_c) - (W)(u_c + u_m) * (W)(w_c + wim_c)) * cu1;
+                const W H32 = ((W)(vjp + vjp_m) * (W)(wffP * wjp + wfbP * w_c) -
+                               (W)(v_c + v_m) * (W)(wffJ * w_c + wfbJ * wjm_c)) * cu2;
+                const W c1 = (W)(w_p + w_c), c2 = (W)(w_c + w_m);
+                const W H33 = (c1 * c1 - c2 * c2) * cu3;
+                const W conv = -(H31 + H32 + H33);
+                const W rhsc = tgam * conv + troh * hw;
+                const double lap = (wip - 2.0 * w_c + wim_c) * g.dxqi + (w_p - 2.0 * w_c + w_m) * g.dzqi +
+                                   (wjp * apw + w_c * acw + wjm_c * amw);
+                cw = conv;
+                const W pgm = (W)((p0 - p_m) * g.dzi * coep);
+                rw = (rhsc + (W)(coev * lap)) * dt - pgm;
+            }
+            hr0[i] = cu;
+            if (J >= 2) hr1[i] = cv;
+            hr2[i] = cw;
+            u_m = u_c; v_m = v_c; w_m = w_c; p_m = p0; uip_m = uip; vjp_m = vjp;
+            bare_barrier();   // plane k read by every wavefront (its reads have returned: their
+                              // values are used above), so its buffer takes the RHS rows
+            W *xs = reinterpret_cast<W *>(A);
+            put(xs, 0, ru);
+            put(xs, 1, rv);
+            put(xs, 2, rw);
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+            bare_barrier();
+            // x-solve (MOMFA1_X_tg): circulant inverse gx of half-width Dx along the line
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                if (cc == 1 && J == 1) continue;   // v: rows J >= N2I = 2 (MOMFA_tg.f90:34-35)
+                const W *ln = xs + (cc * BY + t) * LP + XH + i;
+                W acc = gx[0] * ln[0];
+                for (int d = 1; d <= Dx; ++d) acc += gx[d] * (ln[d] + ln[-d]);   // Dx <= XH
+                uniform(rhs + cc * cs + c0)[i] = acc;
+            }
+            // plane k+1's history values, loaded a plane ahead: no vector-memory wait inside a
+            // plane is then behind that plane's in-flight state loads
+            if (k + 1 < kend) hload(k + 1);
+        };
+        int k = kbeg;
+        for (; k + 2 < kend; k += 3) {
+            plane(k, std::integral_constant<int, 0>{});
+            plane(k + 1, std::integral_constant<int, 1>{});
+            plane(k + 2, std::integral_constant<int, 2>{});
+        }
+        if (k < kend) plane(k, std::integral_constant<int, 0>{});
+        if (k + 1 < kend) plane(k + 1, std::integral_constant<int, 1>{});
+        mf = rsum / dyf;
+    }
+    // block reduction of the mass-flux partial in a fixed order, in buffer 0 once every
+    // wavefront is past its last x-solve (no plane load is outstanding: the last plane issues
+    // none)
+    constexpr int P2 = (NT & (NT - 1)) ? (NT / 3) * 2 : NT;
+    static_assert((P2 & (P2 - 1)) == 0, "block size 2^n or 3 * 2^n");
+    __syncthreads();
+    double *red = b0;
     const int tid = t * L + i;
     red[tid] = mf;
     __syncthreads();
@@ -951,6 +1198,86 @@ __global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__
     if (J1 == M) qc[hix(g, i, M + 1, k)] = 0.0;
 }
 
+// The three y-solves in one launch (blockIdx.z = component + 3 chunk), the components' bands
+// of one half-width D.  v's row 1 input, u's increment at row 1 in the reference's shared
+// RHS_tg (RHS_CvLpGpS_tg.f90:45-46), is formed by the v thread itself from u's input rows, as
+// u's thread forms it, so the launch has no order between components.
+template <typename W, int D>
+__global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict__ by0, const W *__restrict__ by1,
+                                                     const W *__restrict__ by2, const W *__restrict__ rin,
+                                                     const double *__restrict__ qin, double *__restrict__ qout, int KY)
+{
+    // a wavefront is 64 consecutive i of one z-plane k: the plane's bases are wave-uniform
+    // (scalar registers) and a lane's offset (J - 1) L + i is a 32-bit vector value
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int k = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+    const int c = blockIdx.z % 3, jc = blockIdx.z / 3;
+    if (i >= g.L || k >= g.N) return;
+    // the component's band (read-only restrict arguments: its coefficients, uniform per row,
+    // come through scalar loads)
+    const W *__restrict__ by = c == 0 ? by0 : (c == 1 ? by1 : by2);
+    const int M = g.M, w = 2 * D + 1, L = g.L;
+    const size_t cs = (size_t)g.N * M * L, hs = (size_t)g.N * (M + 2) * L;
+    const auto rb0 = uniform(rin + (size_t)k * M * L), rbc = uniform(rin + c * cs + (size_t)k * M * L);
+    const auto qib = uniform(qin + c * hs + (size_t)k * (M + 2) * L);
+    const auto qob = uniform(qout + c * hs + (size_t)k * (M + 2) * L);
+    auto rrow = [&](int cc, int JJ) { return (cc == 0 ? rb0 : rbc) + (unsigned)((JJ - 1) * L); };
+    auto qirow = [&](int JJ) { return qib + (unsigned)(JJ * L); };
+    auto qorow = [&](int JJ) { return qob + (unsigned)(JJ * L); };
+    const unsigned lane = i;
+    const int J0 = (c == 1 && jc == 0) ? 2 : jc * KY + 1;
+    const int J1 = min(M, jc * KY + KY);
+    W win[2 * D + 1];
+#pragma unroll
+    for (int d = -D; d < D; ++d) {
+        const int jj = J0 + d;
+        win[d + D] = (jj >= 1 && jj <= M) ? rrow(c, jj)[lane] : W(0);
+    }
+    if constexpr (D > 0) {
+        if (c == 1 && jc == 0) {
+            // u's increment at row 1: rows 1..1+Du of u's input against its band row 1, whose
+            // entries below the diagonal are zero
+            W a = W(0);
+#pragma unroll
+            for (int d = 0; d <= D; ++d) a += by0[D + d] * rrow(0, 1 + d)[lane];
+            win[D - 1] = a;
+        }
+    }
+    constexpr int B = 8;
+    auto step = [&](int J, W nx, double qv) {
+        win[2 * D] = nx;
+        const W *bj = by + (size_t)(J - 1) * w;
+        W acc = W(0);
+#pragma unroll
+        for (int d = 0; d < w; ++d) acc += bj[d] * win[d];   // out-of-range rows hold 0 and 0 coefficients
+        qorow(J)[lane] = qv + (double)acc;
+#pragma unroll
+        for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+    };
+    int J = J0;
+    for (; J + B - 1 <= J1; J += B) {
+        W nx[B];
+        double qv[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int jj = J + b + D;
+            nx[b] = jj <= M ? rrow(c, jj)[lane] : W(0);
+            qv[b] = qirow(J + b)[lane];
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) step(J + b, nx[b], qv[b]);
+    }
+    for (; J <= J1; ++J) {
+        const int jj = J + D;
+        step(J, jj <= M ? rrow(c, jj)[lane] : W(0), qirow(J)[lane]);
+    }
+    if (jc == 0) {
+        qorow(0)[lane] = 0.0;
+        if (c == 1) qorow(1)[lane] = 0.0;
+    }
+    if (J1 == M) qorow(M + 1)[lane] = 0.0;
+}
+
 // ---------------------------------------------------------------------------------------
 // Poisson.  Spectral data is complex, layout [J][kx][kz] (kz fastest), so z-lines are
 // contiguous and the per-mode y-Thomas reads consecutive kz across a wavefront.
@@ -1203,10 +1530,11 @@ __global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__res
 {
     __shared__ double xr[16][64], xi[16][64], xl[16][64];
     const int HN = g.H * g.N, M = g.M;
-    const int lane = threadIdx.x, q = threadIdx.y, P = blockDim.y;
-    const int t = blockIdx.x * 64 + lane;
+    const int lane = threadIdx.x, q = __builtin_amdgcn_readfirstlane(threadIdx.y), P = blockDim.y;
+    const int t0 = blockIdx.x * 64, t = t0 + lane;
     const bool act = t < HN;
     const int tt = act ? t : HN - 1;   // lanes past the last line solve a copy of it, unstored
+    const unsigned lo = (unsigned)(tt - t0);   // lane offset from the wavefront's first line
     const int kx = tt / g.N, kz = tt - kx * g.N;
     const bool zero = kx == 0 && kz == 0;
     const bool pin = zero && pp00;      // TRID0: x_M = 0, unmodified pivots pp00[J-1]
@@ -1215,16 +1543,17 @@ __global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__res
     const bool low = !pp32 || (kx <= 1 && (kz <= 1 || kz == g.N - 1));
     const int J0 = q * C + 1, n = min(C, M - q * C);   // rows of this chunk (n >= 1)
     const size_t st = (size_t)HN;
-    cpx<T> *s = spec + (size_t)(J0 - 1) * st + tt;
+    // row bases are wave-uniform (scalar registers); a lane adds its 32-bit offset lo
+    auto srow = [&](int u) { return spec + (size_t)(J0 - 1 + u) * st + t0; };
     double rr[C], ri[C], pv[C];
 #pragma unroll
     for (int u = 0; u < C; ++u)
         if (u < n) {
-            const cpx<T> v = s[(size_t)u * st];
+            const cpx<T> v = srow(u)[lo];
             rr[u] = (double)v.x;
             ri[u] = (double)v.y;
-            const size_t o = (size_t)(J0 - 1 + u) * st + tt;
-            pv[u] = pin ? pp00[J0 - 1 + u] : (low ? pp[o] : (double)pp32[o]);
+            const size_t o = (size_t)(J0 - 1 + u) * st + t0;
+            pv[u] = pin ? pp00[J0 - 1 + u] : (low ? (pp + o)[lo] : (double)(pp32 + o)[lo]);
         }
     // forward sweep from g_in = 0
     double gr = 0.0, gi = 0.0, lam = 1.0;
@@ -1301,7 +1630,7 @@ __global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__res
     if (act) {
 #pragma unroll
         for (int u = 0; u < C; ++u)
-            if (u < n) s[(size_t)u * st] = {(T)rr[u], (T)ri[u]};
+            if (u < n) srow(u)[lo] = {(T)rr[u], (T)ri[u]};
     }
 }
 
@@ -1384,13 +1713,16 @@ template <typename T, int L>
 __global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restrict__ tw, const double *__restrict__ dv,
                                                     double talp,
                                                     const double *__restrict__ q, double *__restrict__ div,
-                                                    cpx<T> *__restrict__ spec, const double *__restrict__ src)
+                                                    cpx<T> *__restrict__ spec, const double *__restrict__ src,
+                                                    int nxcd)
 {
     using G = RGeom<L>;
     constexpr int TK = 2 * G::LPW;
     __shared__ cpx<T> lds[G::LPW * G::LS];
     const int li = threadIdx.x / G::TL, t = threadIdx.x % G::TL;
-    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK, ka = k0 + 2 * li;
+    // rows J and J+1 (whose v this block also reads) on one XCD, so the re-read hits its L2
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int J = tile % gridDim.x + 1, k0 = (tile / gridDim.x) * TK, ka = k0 + 2 * li;
     const size_t hs = (size_t)g.N * (g.M + 2) * L;
     const double *u = q, *v = q + hs, *w = q + 2 * hs;
     const double dyf = g.dyfi[J - 1];
@@ -1494,13 +1826,14 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__re
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y + 1;
+    // a wavefront is one row J: row bases are wave-uniform (scalar), lanes add i, ip or im
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
     const int k0 = blockIdx.z * KZ;
     double cmax = 0.0;
     if (i < g.L && J <= g.M) {
         const int L = g.L, M = g.M, N = g.N;
         const size_t hs = (size_t)N * (M + 2) * L;
-        const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
+        const unsigned ui = i, ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
         const int k1 = min(N, k0 + KZ);
         const double sdt = s.dv[0], scv = s.dv[1];
         const double ta = sdt * s.talp;
@@ -1509,42 +1842,44 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__re
         const double be = 0.5 * s.talp * sdt * scv;
         const double coe0 = 1.0 / (sdt * s.talp);
         const double dyf = g.dyfi[J - 1];
-        auto H = [&](int ii, int jj, int kk) { return ((size_t)kk * (M + 2) + jj) * L + ii; };
-        double phm = phi[H(i, J, k0 == 0 ? N - 1 : k0 - 1)];   // phi at plane k-1
-        double phk = phi[H(i, J, k0)];
-        double wsk = qs[2 * hs + H(i, J, k0)];                 // w* at plane k
+        auto R = [&](int jj, int kk) { return ((size_t)kk * (M + 2) + jj) * L; };
+        auto cr = [&](const double *f, int jj, int kk) { return uniform(f + R(jj, kk)); };
+        auto wr = [&](double *f, int jj, int kk) { return uniform(f + R(jj, kk)); };
+        double phm = cr(phi, J, k0 == 0 ? N - 1 : k0 - 1)[ui];   // phi at plane k-1
+        double phk = cr(phi, J, k0)[ui];
+        double wsk = cr(qs + 2 * hs, J, k0)[ui];                 // w* at plane k
         for (int k = k0; k < k1; ++k) {
             const int kp = (k == N - 1) ? 0 : k + 1;
-            const size_t h0 = H(i, J, k);
-            const double php = phi[H(i, J, kp)], wsp = qs[2 * hs + H(i, J, kp)];
-            const double us = qs[h0], usp = qs[H(ip, J, k)], vs = qs[hs + h0], vsp = qs[hs + H(i, J + 1, k)];
-            const double pim = phi[H(im, J, k)], pjm = phi[H(i, J - 1, k)];
+            const auto phr = cr(phi, J, k);
+            const double php = cr(phi, J, kp)[ui], wsp = cr(qs + 2 * hs, J, kp)[ui];
+            const auto usr = cr(qs, J, k);
+            const double us = usr[ui], usp = usr[ip], vs = cr(qs + hs, J, k)[ui], vsp = cr(qs + hs, J + 1, k)[ui];
+            const double pim = phr[im], pjm = cr(phi, J - 1, k)[ui];
             const double d = divg_cell(usp, us, vsp, vs, wsp, wsk, g.dxi, dyf, g.dzi) * coe0;
             const double un = us - (phk - pim) * cx;
             const double wn = wsk - (phk - phm) * cz;
             const double vn = J >= 2 ? vs - (phk - pjm) * cy : 0.0;   // row 1 is the bottom wall
-            const double pn = pr[h0] + phk - be * d;
-            q[h0] = un;
-            q[hs + h0] = vn;
-            q[2 * hs + h0] = wn;
-            pr[h0] = pn;
+            const auto prr = wr(pr, J, k);
+            const double pn = prr[ui] + phk - be * d;
+            wr(q, J, k)[ui] = un;
+            wr(q + hs, J, k)[ui] = vn;
+            wr(q + 2 * hs, J, k)[ui] = wn;
+            prr[ui] = pn;
             if (J == 1) {
-                const size_t hb = H(i, 0, k);
-                pr[hb] = pn;
-                q[hb] = 0.0;
-                q[hs + hb] = 0.0;
-                q[2 * hs + hb] = 0.0;
+                wr(pr, 0, k)[ui] = pn;
+                wr(q, 0, k)[ui] = 0.0;
+                wr(q + hs, 0, k)[ui] = 0.0;
+                wr(q + 2 * hs, 0, k)[ui] = 0.0;
             }
             if (J == M) {
-                const size_t ht = H(i, M + 1, k);
-                pr[ht] = pn;
-                q[ht] = 0.0;
-                q[hs + ht] = 0.0;
-                q[2 * hs + ht] = 0.0;
+                wr(pr, M + 1, k)[ui] = pn;
+                wr(q, M + 1, k)[ui] = 0.0;
+                wr(q + hs, M + 1, k)[ui] = 0.0;
+                wr(q + 2 * hs, M + 1, k)[ui] = 0.0;
             }
             if constexpr (CFL) {
-                const double unp = usp - (phi[H(ip, J, k)] - phk) * cx;
-                const double vnp = J < M ? vsp - (phi[H(i, J + 1, k)] - phk) * cyp : 0.0;
+                const double unp = usp - (phr[ip] - phk) * cx;
+                const double vnp = J < M ? vsp - (cr(phi, J + 1, k)[ui] - phk) * cyp : 0.0;
                 const double wnp = wsp - (php - phk) * cz;
                 const double a = fabs(un + unp) * g.dxi;
                 const double b = fabs(vn + vnp) * dyf;
@@ -1896,7 +2231,7 @@ struct chs_ctx {
     // (defaults: the fastest shapes on MI355X for a 256 x 192 x 256 channel; cnx_nlx 0 means
     // 2048 / L lines)
     struct {
-        int conv_by = 4, conv_kc = 32;  // k_conv_rhs rows per block, z-planes per thread
+        int conv_by = 4, conv_kc = 16;  // k_conv_rhs rows per block, z-planes per thread
         int conv_bx = 1;                // k_conv_rhs wavefronts along x per block (1 or 4)
         int conv_fence = 0;             // 1: all of a plane's loads stay ahead of its arithmetic
         int cnx_nlx = 0;                // k_cn_x lines per block
@@ -1909,7 +2244,10 @@ struct chs_ctx {
         int ytdma_bs = 64, ytdma_rb = 16;   // k_ytdma threads per block, rows per load batch
         int ytdma_c = -1;                   // k_ytdma_chunk rows per chunk (-1 auto = 16, 0 = k_ytdma)
         int conv_lds = 2;                   // k_conv_rhs_lds rows per block (1, 2, 4; 0 = k_conv_rhs)
-        int upd_kz = 16;                    // k_update z-planes per thread
+        int conv_dma = 4;                   // k_conv_rhs_dma rows per block (2, 4; 0 = k_conv_rhs_lds)
+        int divg_nxcd = 8;                  // XCDs the k_divg_fftx_r rows are grouped for (1 = off)
+        int cny3_ky = 48;                   // k_cn_y3_march rows per chunk (0 = M; -1 = one launch per component)
+        int upd_kz = 8;                     // k_update z-planes per thread
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning / chs_set_refinement (graphs are re-made)
     int cn_gen = 0;     // bumped by every rebuild of a Crank-Nicolson table (its memory may move)
@@ -2075,6 +2413,10 @@ void cn_widths(const chs_ctx *c, int ns, double dt, double cvisc, CnTab &t)
         }
         t.Dy[comp] = D;
     }
+    // the three components share the widest half-width (entries past a component's own width
+    // are below the tolerance), so that one launch solves all three with one window
+    const int Dm = std::max(t.Dy[0], std::max(t.Dy[1], t.Dy[2]));
+    for (int comp = 0; comp < 3; ++comp) t.Dy[comp] = Dm;
 }
 
 // Device tables of every stage, at full size once (x: L entries, z: N, y: M rows of up to
@@ -2229,10 +2571,10 @@ void fftz_r(int N, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, cp
 }
 template <typename T>
 void divg_fftx_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw, const double *dv, double talp,
-                 const double *q, double *div, cpx<T> *spec, const double *src)
+                 const double *q, double *div, cpx<T> *spec, const double *src, int nxcd)
 {
     switch (L) {
-#define X(n) case n: hipLaunchKernelGGL((k_divg_fftx_r<T, n>), grid, dim3(64), 0, st, g, tw, dv, talp, q, div, spec, src); break;
+#define X(n) case n: hipLaunchKernelGGL((k_divg_fftx_r<T, n>), grid, dim3(64), 0, st, g, tw, dv, talp, q, div, spec, src, nxcd); break;
         CHS_RCASES(X)
 #undef X
     }
@@ -2281,7 +2623,8 @@ void poisson_launch(chs_ctx *c, const double *dv, double talp, const double *src
     {
         EvScope ev(c, "divg_fftx");
         if (rx)
-            divg_fftx_r<T>(L, dim3(M, N / TKR), c->stream, g, (const cpx<T> *)c->rtwx, dv, talp, qv, dvo, spec, src);
+            divg_fftx_r<T>(L, dim3(M, N / TKR), c->stream, g, (const cpx<T> *)c->rtwx, dv, talp, qv, dvo, spec, src,
+                           c->tune.divg_nxcd);
         else
             hipLaunchKernelGGL(k_divg_fftx<T>, dim3(M, N / TK), dim3(256), shx, c->stream, g, pa, dv, talp, qv, dvo,
                                spec, TK, src);
@@ -2396,13 +2739,25 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
                     t.Dx <= 16;
     // state staged through LDS (k_conv_rhs_lds): LY rows of whole x-lines per block
     const int LY = c->tune.conv_lds;
-    const bool lds = fx && LY > 0 && M % LY == 0 && L <= 256 && L * LY <= 1024;
-    const dim3 blkc = lds ? dim3(L, LY) : fx ? dim3(L, FY) : dim3(64 * BX, BY);
-    const dim3 grdc = lds  ? dim3(1, M / LY, (N + KC - 1) / KC)
-                      : fx ? dim3(1, M / FY, (N + KC - 1) / KC)
-                           : dim3((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
+    // state planes moved into LDS by direct loads (k_conv_rhs_dma): DY rows per block
+    const int DY = c->tune.conv_dma;
+    const bool dma = fx && DY > 0 && M % DY == 0 && L <= 256 && L * DY <= 1024;
+    const bool lds = !dma && fx && LY > 0 && M % LY == 0 && L <= 256 && L * LY <= 1024;
+    const int RY = dma ? DY : LY;
+    const dim3 blkc = (dma || lds) ? dim3(L, RY) : fx ? dim3(L, FY) : dim3(64 * BX, BY);
+    const dim3 grdc = (dma || lds) ? dim3(1, M / RY, (N + KC - 1) / KC)
+                      : fx         ? dim3(1, M / FY, (N + KC - 1) / KC)
+                                   : dim3((L + 64 * BX - 1) / (64 * BX), (M + BY - 1) / BY, (N + KC - 1) / KC);
     const int npart = grdc.x * grdc.y * grdc.z;
-    if (lds) {
+    if (dma) {
+        EvScope ev(c, "conv_rhs");
+        switch (DY * 10 + L / 64) {
+#define CHS_DCASE(by, bx) case by * 10 + bx: hipLaunchKernelGGL((k_conv_rhs_dma<W, by, bx>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd, (const W *)t.gx, t.Dx); break;
+            CHS_DCASE(2, 1) CHS_DCASE(2, 2) CHS_DCASE(2, 3) CHS_DCASE(2, 4)
+            CHS_DCASE(4, 1) CHS_DCASE(4, 2) CHS_DCASE(4, 3) CHS_DCASE(4, 4)
+#undef CHS_DCASE
+        }
+    } else if (lds) {
         EvScope ev(c, "conv_rhs");
         switch (LY * 10 + L / 64) {
 #define CHS_LCASE(by, bx) case by * 10 + bx: hipLaunchKernelGGL((k_conv_rhs_lds<W, by, bx>), grdc, blkc, 0, c->stream, g, s, c->q, c->pr, h0, rhs, c->partial, KC, c->tune.nxcd, (const W *)t.gx, t.Dx); break;
@@ -2462,7 +2817,19 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
             hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz, rhs, rhs2, comp);
         }
     }
-    for (int comp = 0; comp < 3; ++comp) {
+    const int Dym = t.Dy[0];
+    const int KY3 = c->tune.cny3_ky > 0 ? c->tune.cny3_ky : M;
+    if (c->tune.cny3_ky >= 0 && Dym <= 8 && KY3 > Dym && t.Dy[1] == Dym && t.Dy[2] == Dym) {
+        EvScope ev(c, "cn_y");
+        const dim3 gy((L + 63) / 64, (N + 3) / 4, 3 * ((M + KY3 - 1) / KY3));
+        const W *b0 = (const W *)t.by[0], *b1 = (const W *)t.by[1], *b2 = (const W *)t.by[2];
+        switch (Dym) {
+#define CHS_Y3CASE(d) case d: hipLaunchKernelGGL((k_cn_y3_march<W, d>), gy, blk, 0, c->stream, g, b0, b1, b2, rhs2, c->q, c->qs, KY3); break;
+            CHS_Y3CASE(0) CHS_Y3CASE(1) CHS_Y3CASE(2) CHS_Y3CASE(3) CHS_Y3CASE(4) CHS_Y3CASE(5) CHS_Y3CASE(6)
+            CHS_Y3CASE(7) CHS_Y3CASE(8)
+#undef CHS_Y3CASE
+        }
+    } else for (int comp = 0; comp < 3; ++comp) {
         EvScope ev(c, "cn_y");
         if (t.Dy[comp] <= 8) {
             const dim3 gy((L + 63) / 64, (N + 3) / 4, (M + KY - 1) / KY);
@@ -3153,6 +3520,15 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "conv_lds") {
         if (value != 0 && value != 1 && value != 2 && value != 4) return bad("must be 0, 1, 2 or 4");
         c->tune.conv_lds = value;
+    } else if (k == "divg_nxcd") {
+        if (value < 1 || value > 64) return bad("must be 1..64");
+        c->tune.divg_nxcd = value;
+    } else if (k == "cny3_ky") {
+        if (value < -1 || value > M) return bad("must be -1..M");
+        c->tune.cny3_ky = value;
+    } else if (k == "conv_dma") {
+        if (value != 0 && value != 2 && value != 4) return bad("must be 0, 2 or 4");
+        c->tune.conv_dma = value;
     } else if (k == "ytdma_c") {
         if (value != -1 && value != 0 && value != 4 && value != 8 && value != 16) return bad("must be -1, 0, 4, 8 or 16");
         c->tune.ytdma_c = value;

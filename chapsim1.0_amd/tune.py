@@ -4,7 +4,7 @@ Each key is swept over its candidates with the others fixed, timing only the ker
 key shapes (HIP events of chs_profile_*), and the fastest value is kept.  Fields are saved
 before and restored after, so the channel's state is unchanged; the step graph is re-made
 on the next step.  Shapes change no result except the summation order of the mass-flux
-partials (conv_by, conv_kc, conv_lds) and the rounding of the chunked y-Thomas (ytdma_c).
+partials (conv_by, conv_kc, conv_lds, conv_dma) and the rounding of the chunked y-Thomas (ytdma_c).
 """
 from __future__ import annotations
 
@@ -18,6 +18,7 @@ KERNELS = {
     "fftz_nl": ("fftz_fwd", "fftz_inv"), "nxcd": ("conv_rhs",),
     "fft_reg": ("divg_fftx", "fftz_fwd", "fftz_inv", "fftx_inv"),
     "ytdma_bs": ("ytdma",), "ytdma_rb": ("ytdma",), "ytdma_c": ("ytdma",), "conv_lds": ("conv_rhs", "cn_x"),
+    "conv_dma": ("conv_rhs", "cn_x"), "cny3_ky": ("cn_y",), "divg_nxcd": ("divg_fftx",),
 }
 
 
@@ -42,6 +43,9 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "ytdma_rb": [8, 16],
         "ytdma_c": [c for c in (0, 4, 8, 16) if c == 0 or (M <= 256 and -(-M // c) <= 16)],
         "conv_lds": [v for v in (0, 1, 2, 4) if v == 0 or (L % 64 == 0 and L <= 256 and M % v == 0 and L * v <= 1024)],
+        "conv_dma": [v for v in (0, 2, 4) if v == 0 or (L % 64 == 0 and L <= 256 and M % v == 0 and L * v <= 1024)],
+        "cny3_ky": [v for v in (-1, 0, 48, 64, 96) if v <= M],
+        "divg_nxcd": [1, 8],
     }
     return {k: v for k, v in c.items() if v}
 
