@@ -104,9 +104,11 @@ template <int A, int... B> struct first_of { static constexpr int value = A; };
 
 // One pass and the rest, recursively.  v holds this pass's inputs: v[b R + r] =
 // x[t + b TL + r N / R].  Off = offset of this pass's twiddles in the table.  The last pass
-// hands each output (natural index, value) to out.
-template <typename T, int N, int E, int PS, int Ns, int Off, int R, int... Rest, typename Out>
-__device__ __forceinline__ void rfft_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, const cpx<T> *__restrict__ tw,
+// hands each output (natural index, value) to out and leaves it in v.  lds is the line's
+// buffer: cpx<T>[LS], or with SPLIT T[LS], through which the real parts go first and then the
+// imaginary parts (half the LDS of a line, two more wavefront barriers per exchange).
+template <bool SPLIT, typename T, int N, int E, int PS, int Ns, int Off, int R, int... Rest, typename Out>
+__device__ __forceinline__ void rfft_pass(cpx<T> (&v)[E], void *lds, int t, const cpx<T> *__restrict__ tw,
                                           int sign, Out &&out)
 {
     constexpr int TL = N / E, NB = E / R;
@@ -133,23 +135,46 @@ __device__ __forceinline__ void rfft_pass(cpx<T> (&v)[E], cpx<T> *lds, int t, co
 #pragma unroll
             for (int r = 0; r < R; ++r) out(t + b * TL + r * Ns, v[b * R + r]);
     } else {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = t + b * TL;
-            const int k = j % Ns;
-            const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) lds[rpad<PS>(base + r * Ns)] = v[b * R + r];
-        }
-        wave_lds_sync();
         constexpr int R2 = first_of<Rest...>::value;
         constexpr int NB2 = E / R2, NR2 = N / R2;
+        auto slot = [&](int b, int r) { return ((t + b * TL) / Ns) * Ns * R + (t + b * TL) % Ns + r * Ns; };
+        if constexpr (!SPLIT) {
+            cpx<T> *l = static_cast<cpx<T> *>(lds);
 #pragma unroll
-        for (int b = 0; b < NB2; ++b)
+            for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds[rpad<PS>(t + b * TL + r * NR2)];
-        wave_lds_sync();
-        rfft_pass<T, N, E, PS, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), Rest...>(v, lds, t, tw, sign, out);
+                for (int r = 0; r < R; ++r) l[rpad<PS>(slot(b, r))] = v[b * R + r];
+            wave_lds_sync();
+#pragma unroll
+            for (int b = 0; b < NB2; ++b)
+#pragma unroll
+                for (int r = 0; r < R2; ++r) v[b * R2 + r] = l[rpad<PS>(t + b * TL + r * NR2)];
+            wave_lds_sync();
+        } else {
+            T *l = static_cast<T *>(lds);
+            T xin[E];
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < R; ++r) l[rpad<PS>(slot(b, r))] = v[b * R + r].x;
+            wave_lds_sync();
+#pragma unroll
+            for (int b = 0; b < NB2; ++b)
+#pragma unroll
+                for (int r = 0; r < R2; ++r) xin[b * R2 + r] = l[rpad<PS>(t + b * TL + r * NR2)];
+            wave_lds_sync();
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < R; ++r) l[rpad<PS>(slot(b, r))] = v[b * R + r].y;
+            wave_lds_sync();
+#pragma unroll
+            for (int b = 0; b < NB2; ++b)
+#pragma unroll
+                for (int r = 0; r < R2; ++r) v[b * R2 + r] = {xin[b * R2 + r], l[rpad<PS>(t + b * TL + r * NR2)]};
+            wave_lds_sync();
+        }
+        rfft_pass<SPLIT, T, N, E, PS, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), Rest...>(v, lds, t, tw, sign, out);
     }
 }
 
@@ -162,17 +187,27 @@ template <> struct RPlan<512> { static constexpr int E = 8, PS = 3; };
 template <> struct RPlan<1024> { static constexpr int E = 16, PS = 4; };
 
 // Full transform of one line: v holds the first pass's inputs (v[b R0 + r] =
-// x[t + b TL + r N / R0], TL = N / E); out(index, value) receives the natural-order result.
-template <typename T, int N, typename Out>
-__device__ __forceinline__ void rfft_line(cpx<T> (&v)[RPlan<N>::E], cpx<T> *lds, int t, const cpx<T> *__restrict__ tw,
+// x[t + b TL + r N / R0], TL = N / E); out(index, value) receives the natural-order result,
+// which also stays in v (v[e] is element rout_index<N>(t, e)).  lds: cpx<T>[LS], or T[LS] with
+// SPLIT.
+template <typename T, int N, bool SPLIT = false, typename Out>
+__device__ __forceinline__ void rfft_line(cpx<T> (&v)[RPlan<N>::E], void *lds, int t, const cpx<T> *__restrict__ tw,
                                           int sign, Out &&out)
 {
     constexpr int E = RPlan<N>::E, PS = RPlan<N>::PS;
-    if constexpr (N == 64) rfft_pass<T, N, E, PS, 1, 0, 8, 8>(v, lds, t, tw, sign, out);
-    else if constexpr (N == 128) rfft_pass<T, N, E, PS, 1, 0, 16, 8>(v, lds, t, tw, sign, out);
-    else if constexpr (N == 256) rfft_pass<T, N, E, PS, 1, 0, 16, 16>(v, lds, t, tw, sign, out);
-    else if constexpr (N == 512) rfft_pass<T, N, E, PS, 1, 0, 8, 8, 8>(v, lds, t, tw, sign, out);
-    else rfft_pass<T, N, E, PS, 1, 0, 16, 16, 4>(v, lds, t, tw, sign, out);
+    if constexpr (N == 64) rfft_pass<SPLIT, T, N, E, PS, 1, 0, 8, 8>(v, lds, t, tw, sign, out);
+    else if constexpr (N == 128) rfft_pass<SPLIT, T, N, E, PS, 1, 0, 16, 8>(v, lds, t, tw, sign, out);
+    else if constexpr (N == 256) rfft_pass<SPLIT, T, N, E, PS, 1, 0, 16, 16>(v, lds, t, tw, sign, out);
+    else if constexpr (N == 512) rfft_pass<SPLIT, T, N, E, PS, 1, 0, 8, 8, 8>(v, lds, t, tw, sign, out);
+    else rfft_pass<SPLIT, T, N, E, PS, 1, 0, 16, 16, 4>(v, lds, t, tw, sign, out);
+}
+
+// Last radix of the plan for N, and the natural index of output slot e of lane t.
+template <int N> constexpr int rlast() { return N == 128 ? 8 : (N == 1024 ? 4 : (N == 512 ? 8 : (N == 64 ? 8 : 16))); }
+template <int N> __device__ __forceinline__ int rout_index(int t, int e)
+{
+    constexpr int R = rlast<N>(), TL = N / RPlan<N>::E, Ns = N / R;
+    return t + (e / R) * TL + (e % R) * Ns;
 }
 
 // First radix of the plan (pass-0 input stride is N / rfirst).

@@ -80,13 +80,20 @@ __device__ __forceinline__ int xcd_tile(int nxcd)
     return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
-// A wave-uniform pointer made opaque to the optimiser, so that a lane's 32-bit offset is added
-// at the access (scalar base + vector offset) instead of being folded into a loop-invariant
-// 64-bit vector address that would occupy two registers per stream.
+// A wave-uniform pointer (the caller guarantees it is the same in every lane) taken into scalar
+// registers and made opaque to the optimiser, so that a lane's 32-bit offset is added at the
+// access (scalar base + vector offset) instead of being folded into a loop-invariant 64-bit
+// vector address that would occupy two registers per stream.
 template <typename P> __device__ __forceinline__ __attribute__((address_space(1))) P *uniform(P *p)
 {
-    asm volatile("" : "+s"(p));
-    return (__attribute__((address_space(1))) P *)p;
+    // readfirstlane returns int: each half goes through unsigned so that the low word is not
+    // sign-extended into the high one
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    unsigned long long u = (unsigned long long)lo | ((unsigned long long)hi << 32);
+    asm volatile("" : "+s"(u));
+    return (__attribute__((address_space(1))) P *)u;
 }
 
 // The cell divergence of DIVG_tg (DIVG.f90:31-46) before its 1 / (alpha DT) scaling, in one
@@ -1796,15 +1803,17 @@ __global__ __launch_bounds__(64) void k_fftx_inv_r(Geo g, const cpx<T> *__restri
 #pragma unroll
         for (int r = 0; r < G::R0; ++r) x[b * G::R0 + r] = line[chs::rpad<G::PS>(t + b * G::TL + r * (L / G::R0))];
     chs::wave_lds_sync();
-    const int ka = k0 + 2 * li;
+    // the block's planes k0 .. k0+TK-1 from one wave-uniform base: a lane's offset is 32-bit
+    const auto pb = uniform(phi + (size_t)k0 * (M + 2) * L);
+    const unsigned pl = (unsigned)(2 * li) * (unsigned)((M + 2) * L), ps = (unsigned)((M + 2) * L);
     chs::rfft_line<T, L>(x, line, t, tw, +1, [&](int i, const cpx<T> &z) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int k = ka + h;
+            const unsigned o = pl + h * ps + (unsigned)i;
             const double val = h == 0 ? (double)z.x : (double)z.y;
-            phi[hix(g, i, J, k)] = val;
-            if (J == 1) phi[hix(g, i, 0, k)] = val;
-            if (J == M) phi[hix(g, i, M + 1, k)] = val;
+            pb[o + (unsigned)(J * L)] = val;
+            if (J == 1) pb[o] = val;
+            if (J == M) pb[o + (unsigned)((M + 1) * L)] = val;
         }
     });
 }

@@ -21,7 +21,7 @@ void wave_lds_sync() { g_bar->arrive_and_wait(); }
 
 #include "../../chapsim1.0_amd/csrc/chs_fftr.h"
 
-template <typename T, int N> double run(int sign, unsigned seed)
+template <typename T, int N, bool SPLIT> double run(int sign, unsigned seed)
 {
     using C = chs::cpx<T>;
     constexpr int E = chs::RPlan<N>::E, PS = chs::RPlan<N>::PS, TL = N / E, R0 = chs::rfirst<N>();
@@ -30,7 +30,7 @@ template <typename T, int N> double run(int sign, unsigned seed)
     std::vector<C> x(N), out(N, C{T(0), T(0)});
     for (auto &e : x) e = {(T)U(rng), (T)U(rng)};
     const std::vector<C> tw = chs::rplan_twiddles<T>(N);
-    std::vector<C> lds(chs::rline_stride<N, PS>());
+    std::vector<C> lds(chs::rline_stride<N, PS>());   // SPLIT uses the first half as T[LS]
     std::barrier<> bar(TL);
     std::vector<std::thread> th;
     for (int t = 0; t < TL; ++t)
@@ -39,7 +39,12 @@ template <typename T, int N> double run(int sign, unsigned seed)
             C v[E];
             for (int b = 0; b < E / R0; ++b)
                 for (int r = 0; r < R0; ++r) v[b * R0 + r] = x[t + b * TL + r * (N / R0)];
-            chs::rfft_line<T, N>(v, lds.data(), t, tw.data(), sign, [&](int idx, C val) { out[idx] = val; });
+            chs::rfft_line<T, N, SPLIT>(v, lds.data(), t, tw.data(), sign, [&](int idx, C val) { out[idx] = val; });
+            // the outputs also stay in v, slot e at its natural index rout_index
+            for (int e = 0; e < E; ++e) {
+                const C w = out[chs::rout_index<N>(t, e)];
+                if (w.x != v[e].x || w.y != v[e].y) out[chs::rout_index<N>(t, e)] = {T(1e30), T(1e30)};
+            }
         });
     for (auto &h : th) h.join();
     const long double pi = 3.141592653589793238462643383279502884L;
@@ -62,10 +67,13 @@ template <int N> int check()
 {
     int bad = 0;
     for (int sign : {-1, 1}) {
-        const double e64 = run<double, N>(sign, 7 + N), e32 = run<float, N>(sign, 11 + N);
-        const bool ok = e64 < 1e-14 && e32 < 2e-6;
-        std::printf("N=%d sign=%+d fp64 %.3e fp32 %.3e %s\n", N, sign, e64, e32, ok ? "ok" : "FAIL");
-        bad += !ok;
+        for (int split = 0; split < 2; ++split) {
+            const double e64 = split ? run<double, N, true>(sign, 7 + N) : run<double, N, false>(sign, 7 + N);
+            const double e32 = split ? run<float, N, true>(sign, 11 + N) : run<float, N, false>(sign, 11 + N);
+            const bool ok = e64 < 1e-14 && e32 < 2e-6;
+            std::printf("N=%d sign=%+d split=%d fp64 %.3e fp32 %.3e %s\n", N, sign, split, e64, e32, ok ? "ok" : "FAIL");
+            bad += !ok;
+        }
     }
     return bad;
 }
