@@ -60,18 +60,19 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
         "cn_z": 6 * F * wbytes,
         # x and z together, three components: R + W RHS x3
         "cn_xz": 6 * F * wbytes,
-        # y stencil + Q update, one launch per component: R RHS, R + W q
+        # y stencil + Q update of one component: R RHS, R q, W predicted q (one launch per
+        # component, or the three in one launch)
         "cn_y": F * wbytes + 2 * F * 8,
-        # R u,v,w; W div (FP64) + spectrum
-        "divg_fftx": 3 * F * 8 + F * 8 + S * 2 * wbytes,
+        # R predicted u,v,w; W spectrum (the correction recomputes the divergence)
+        "divg_fftx": 3 * F * 8 + S * 2 * wbytes,
         "fftz_fwd": 2 * S * 2 * wbytes,
         # R spectrum + pivots (work precision; FP64 for six low modes), W spectrum
         "ytdma": 2 * S * 2 * wbytes + S * wbytes,
         "fftz_inv": 2 * S * 2 * wbytes,
         # R spectrum, W phi
         "fftx_inv": S * 2 * wbytes + F * 8,
-        # R phi, div, u, v, w, p; W u, v, w, p
-        "update": 10 * F * 8,
+        # R phi, predicted u, v, w, p; W u, v, w, p
+        "update": 9 * F * 8,
         # refinement correction: R phi, u, v, w, p; W u, v, w, p
         "refine_update": 9 * F * 8,
     }
@@ -584,7 +585,9 @@ def main():
         if os.path.exists(tpath):
             with open(tpath) as fh:
                 traffic = json.load(fh).get(f"{f}:{dom}")
-        kernel_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) * n for k, n in r["launches"].items())
+        # per step: the y-solves count three components per substep however they are launched
+        kernel_bytes = sum(alg_bytes(k, c["L"], c["M"], c["N"], r["wbytes"]) * (9 if k == "cn_y" and n else n)
+                           for k, n in r["launches"].items())
         step_bytes = canonical_step_bytes(c["L"], c["M"], c["N"], r["wbytes"])
         return dict(
             value=F / (r["ms_per_step"] * 1e-3), ms_per_step=r["ms_per_step"],

@@ -1829,14 +1829,14 @@ __global__ __launch_bounds__(64) void k_fftx_inv_r(Geo g, const cpx<T> *__restri
 // Contraction off: a correction is the same product and difference wherever it is formed, so
 // a neighbour's corrected value here equals the one its own thread stores.
 template <bool CFL>
-__global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__restrict__ phi,
+__global__ __launch_bounds__(1024) void k_update(Geo g, Sub s, const double *__restrict__ phi,
                                                 const double *__restrict__ qs, double *__restrict__ q,
                                                 double *__restrict__ pr, int KZ, double *__restrict__ part)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 64 + threadIdx.x;
     // a wavefront is one row J: row bases are wave-uniform (scalar), lanes add i, ip or im
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y + 1);
     const int k0 = blockIdx.z * KZ;
     double cmax = 0.0;
     if (i < g.L && J <= g.M) {
@@ -1901,11 +1901,11 @@ __global__ __launch_bounds__(256) void k_update(Geo g, Sub s, const double *__re
         }
     }
     if constexpr (CFL) {
-        __shared__ double red[256];
-        const int t = threadIdx.y * 64 + threadIdx.x;
+        __shared__ double red[1024];
+        const int t = threadIdx.y * 64 + threadIdx.x;   // blockDim = (64, 2^n)
         red[t] = cmax;
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
+        for (int o = 32 * blockDim.y; o > 0; o >>= 1) {
             if (t < o) red[t] = nanmax(red[t], red[t + o]);
             __syncthreads();
         }
@@ -2257,6 +2257,7 @@ struct chs_ctx {
         int divg_nxcd = 8;                  // XCDs the k_divg_fftx_r rows are grouped for (1 = off)
         int cny3_ky = 48;                   // k_cn_y3_march rows per chunk (0 = M; -1 = one launch per component)
         int upd_kz = 8;                     // k_update z-planes per thread
+        int upd_by = 4;                     // k_update rows per block (1, 2, 4, 8, 16)
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning / chs_set_refinement (graphs are re-made)
     int cn_gen = 0;     // bumped by every rebuild of a Crank-Nicolson table (its memory may move)
@@ -2853,15 +2854,15 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
     // FP64-residual refinement (mixed form): the divergence of the corrected velocity drives
     // another solve and correction; the CFL then comes from the final field
     const int nref = ns == 0 ? c->nref_projection : c->nref_step;
-    const int KU = c->tune.upd_kz;
-    const dim3 gu((L + 63) / 64, (M + 3) / 4, (N + KU - 1) / KU);
+    const int KU = c->tune.upd_kz, UY = c->tune.upd_by;
+    const dim3 gu((L + 63) / 64, (M + UY - 1) / UY, (N + KU - 1) / KU), blku(64, UY);
     {
         EvScope ev(c, "update");
         if (cfl && nref == 0)
-            hipLaunchKernelGGL(k_update<true>, gu, blk, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
+            hipLaunchKernelGGL(k_update<true>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
                                c->partial);
         else
-            hipLaunchKernelGGL(k_update<false>, gu, blk, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
+            hipLaunchKernelGGL(k_update<false>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
                                c->partial);
     }
     for (int r = 0; r < nref; ++r) {
@@ -3523,6 +3524,9 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "ytdma_bs") {
         if (value != 64 && value != 128 && value != 256) return bad("must be 64, 128 or 256");
         c->tune.ytdma_bs = value;
+    } else if (k == "upd_by") {
+        if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return bad("must be 1, 2, 4, 8 or 16");
+        c->tune.upd_by = value;
     } else if (k == "upd_kz") {
         if (value < 1 || value > N) return bad("must be 1..N");
         c->tune.upd_kz = value;

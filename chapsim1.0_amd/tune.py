@@ -19,6 +19,7 @@ KERNELS = {
     "fft_reg": ("divg_fftx", "fftz_fwd", "fftz_inv", "fftx_inv"),
     "ytdma_bs": ("ytdma",), "ytdma_rb": ("ytdma",), "ytdma_c": ("ytdma",), "conv_lds": ("conv_rhs", "cn_x"),
     "conv_dma": ("conv_rhs", "cn_x"), "cny3_ky": ("cn_y",), "divg_nxcd": ("divg_fftx",),
+    "upd_kz": ("update",), "upd_by": ("update",),
 }
 
 
@@ -46,6 +47,8 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "conv_dma": [v for v in (0, 2, 4) if v == 0 or (L % 64 == 0 and L <= 256 and M % v == 0 and L * v <= 1024)],
         "cny3_ky": [v for v in (-1, 0, 48, 64, 96) if v <= M],
         "divg_nxcd": [1, 8],
+        "upd_kz": [k for k in (4, 8, 16) if k <= N],
+        "upd_by": [1, 2, 4, 8, 16],
     }
     return {k: v for k, v in c.items() if v}
 
