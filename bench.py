@@ -425,6 +425,7 @@ def bench_ensemble(pkg, args):
     mode = pkg.MIXED if form == "mixed" else pkg.PLAIN_FP64
     cos = [pkg.channel_setup(c["L"], c["M"], c["N"], c["HX"], c["HZ"], c["STR2"]) for c in cases]
     ens = pkg.Ensemble(cos, mode)
+    ens.set_batched(args.ens_batched)
     dt = np.array([c["dt"] for c in cases])
     cv = np.array([1.0 / c["REN"] for c in cases])
     for i, (ch, co) in enumerate(zip(ens.channels, cos)):
@@ -446,6 +447,8 @@ def bench_ensemble(pkg, args):
                       "unit": "grid-point RK3-steps/s", "n_gpus": 1, "steps": args.steps,
                       "warmup": args.warmup, "ms_per_step": ms, "channels": len(cases),
                       "grid_points": pts, "form": form, "max_div": max(divs),
+                      "launches": "batched: one per kernel kind and stage over all channels"
+                                  if args.ens_batched else "per-channel, on capture lanes",
                       "config": {"workload": "32 channels, L,N in [64,192], M in [64,128], "
                                              "Re_tau U[180,395], one hipGraph per RK3 step"}}))
 
@@ -537,6 +540,9 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each RK3 step from a hipGraph")
     ap.add_argument("--autotune", action="store_true",
                     help="search the kernels' launch shapes (chapsim_amd.tune) before timing")
+    ap.add_argument("--ens-batched", action="store_true",
+                    help="ensemble: one launch per kernel kind over all channels instead of each channel's "
+                         "own launches on capture lanes")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="launch shape for chs_set_tuning (repeatable), e.g. --set conv_bx=4")
     ap.add_argument("--workload", choices=["channel", "channel64", "poisson", "ensemble", "heated"],

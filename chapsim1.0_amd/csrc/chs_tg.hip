@@ -70,11 +70,22 @@ __device__ __forceinline__ size_t cix(const Geo &g, int i, int J, int k)
 // on the same XCD.  Logical tile l = the id's position within its XCD's share, so XCD x gets
 // the contiguous tiles [start_x, start_x + count_x) (x fastest, then y, then z).  A bijection
 // for any nxcd >= 1 (1 = identity).
-__device__ __forceinline__ int xcd_tile(int nxcd)
+// A block's position in its grid.  Kernel bodies (the *_b functions) read it from here rather
+// than from the built-ins, so that a body runs under its own launch (own_blk) or as one item
+// of a launch that covers many channels' grids (k_multi).
+struct Blk {
+    unsigned x, y, z, gx, gy, gz;
+};
+__device__ __forceinline__ Blk own_blk()
 {
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int nb = gx * gy * gridDim.z;
-    const int p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    return {blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.y, gridDim.z};
+}
+
+__device__ __forceinline__ int xcd_tile(int nxcd, const Blk blk)
+{
+    const int gx = blk.gx, gy = blk.gy;
+    const int nb = gx * gy * blk.gz;
+    const int p = blk.x + gx * (blk.y + gy * blk.z);
     if (nxcd <= 1) return p;
     const int xcd = p % nxcd, slot = p / nxcd, per = nb / nxcd, rem = nb % nxcd;
     return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
@@ -128,18 +139,18 @@ struct Sub {
 // after the z-solve instead: the cyclic x and z factors map a constant to itself, so only the
 // rounding changes.  The partial sums use the unsolved u-row, as the reference's.
 template <typename W, int BY, int BXW, bool FENCE, bool FX = false>
-__global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
+__device__ __forceinline__ void k_conv_rhs_b(Geo g, Sub s, const double *__restrict__ q,
                                                       const double *__restrict__ pr,
                                                       W *__restrict__ h0, W *__restrict__ rhs,
                                                       double *__restrict__ partial, int KC, int nxcd,
-                                                      const W *__restrict__ gx = nullptr, int Dx = 0)
+                                                      const W *__restrict__ gx, int Dx, const Blk blk)
 {
     // FX: whole x-lines, BY rows; the block's rows are all inside the mesh (M % BY == 0)
     constexpr int LXB = 64 * BXW * BY;               // one plane's row-block of one component
     __shared__ W lxs[FX ? 2 * 3 * LXB : 1];           // two planes' u, v, w rows (double buffer)
     const int lrow = threadIdx.y * 64 * BXW;
-    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
-    const int tx = tile % gridDim.x, ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd, blk));
+    const int tx = tile % blk.gx, ty = (tile / blk.gx) % blk.gy, tz = tile / (blk.gx * blk.gy);
     const int i = tx * (64 * BXW) + threadIdx.x;
     // blockDim = (64 BXW, BY): a wavefront is one row J, so J-indexed coefficients are uniform
     const int J = __builtin_amdgcn_readfirstlane(ty * BY + threadIdx.y + 1);
@@ -301,6 +312,15 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const 
     }
     if (t == 0) partial[tile] = red[0];
 }
+template <typename W, int BY, int BXW, bool FENCE, bool FX = false>
+__global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs(Geo g, Sub s, const double *__restrict__ q,
+                                                      const double *__restrict__ pr,
+                                                      W *__restrict__ h0, W *__restrict__ rhs,
+                                                      double *__restrict__ partial, int KC, int nxcd,
+                                                      const W *__restrict__ gx = nullptr, int Dx = 0)
+{
+    k_conv_rhs_b<W, BY, BXW, FENCE, FX>(g, s, q, pr, h0, rhs, partial, KC, nxcd, gx, Dx, own_blk());
+}
 
 // k_conv_rhs_lds: the k_conv_rhs FX form (whole x-lines, x-solve applied in LDS, same
 // arithmetic) with the state staged through LDS, so that the stencils' neighbours cost LDS
@@ -323,7 +343,7 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_lds(Geo g, Sub s, co
     constexpr int XH = 16, LP = L + 2 * XH;            // RHS rows carry XH wrapped points each side
     __shared__ double st[2][4][R][L];                  // [plane buffer][u v w p][row][x]
     __shared__ W xs[3 * BY * LP];                       // one plane's three RHS row-blocks
-    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd, own_blk()));
     const int ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
     // blockDim.x = L is a multiple of 64, so the row t is wave-uniform: scalar branches and bases
     const int i = threadIdx.x, t = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -552,7 +572,7 @@ __global__ __launch_bounds__(64 * BXW * BY) void k_conv_rhs_dma(Geo g, Sub s, co
     constexpr int XH = 16, LP = L + 2 * XH;       // RHS rows carry XH wrapped points each side
     static_assert(3 * BY * LP * sizeof(W) <= PB * sizeof(double) && NT <= PB, "RHS rows fit a buffer");
     __shared__ double b0[PB], b1[PB], b2[PB];
-    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd, own_blk()));
     const int ty = (tile / gridDim.x) % gridDim.y, tz = tile / (gridDim.x * gridDim.y);
     const int i = threadIdx.x, t = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int lane = i & 63, wv = __builtin_amdgcn_readfirstlane((t * L + i) >> 6);
@@ -774,15 +794,15 @@ template <typename W> struct CnDev {
 };
 
 template <typename W>
-__global__ __launch_bounds__(256) void k_cn_tables(Geo g, CnDev<W> t, const double *__restrict__ dv,
-                                                   const double *__restrict__ stamp, int ns0)
+__device__ __forceinline__ void k_cn_tables_b(Geo g, CnDev<W> t, const double *__restrict__ dv,
+                                                   const double *__restrict__ stamp, int ns0, const Blk blk)
 {
     __shared__ double rho[1024], psi[1024];
-    const int ns = ns0 + blockIdx.y;
+    const int ns = ns0 + blk.y;
     const double dt = dv[0], cv = dv[1];
     if (stamp[2 * ns] == dt && stamp[2 * ns + 1] == cv) return;
     const double beta = 0.5 * t.talp[ns] * dt * cv;
-    const int b = blockIdx.x;
+    const int b = blk.x;
     if (b <= 1) {
         const int n = b == 0 ? g.L : g.N;
         const double a = -beta * (b == 0 ? g.dxqi : g.dzqi), bb = 1.0 - a - a;
@@ -831,13 +851,23 @@ __global__ __launch_bounds__(256) void k_cn_tables(Geo g, CnDev<W> t, const doub
             if (col + d < 0 || col + d >= M) band[(size_t)col * w + d + D] = W(0);
     }
 }
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_tables(Geo g, CnDev<W> t, const double *__restrict__ dv,
+                                                   const double *__restrict__ stamp, int ns0)
+{
+    k_cn_tables_b<W>(g, t, dv, stamp, ns0, own_blk());
+}
 
 // Records the (DT, CVISC) the stages ns0 .. ns0 + n - 1 now hold (see k_cn_tables); with
 // dv null, marks them as holding nothing (their band layout is about to change).
-__global__ void k_cn_stamp(const double *__restrict__ dv, double *__restrict__ stamp, int ns0, int n)
+__device__ __forceinline__ void k_cn_stamp_b(const double *__restrict__ dv, double *__restrict__ stamp, int ns0, int n, const Blk blk)
 {
     const int t = threadIdx.x;
     if (t < 2 * n) stamp[2 * ns0 + t] = dv ? dv[t & 1] : __builtin_nan("");
+}
+__global__ void k_cn_stamp(const double *__restrict__ dv, double *__restrict__ stamp, int ns0, int n)
+{
+    k_cn_stamp_b(dv, stamp, ns0, n, own_blk());
 }
 
 // The time-step rule of BCAST_COMM_STEP (SOLVE.f90:169-178) on the device: from CFLMM of the
@@ -854,15 +884,19 @@ __global__ void k_dt_rule(double *__restrict__ st, double dt0, double cflgv, dou
 }
 
 // Host-given DT and CVISC into the device step scalars.
-__global__ void k_set_step(double *__restrict__ dv, double dt, double cvisc)
+__device__ __forceinline__ void k_set_step_b(double *__restrict__ dv, double dt, double cvisc, const Blk blk)
 {
     const int t = threadIdx.x;
     if (t < 2) dv[t] = t == 0 ? dt : cvisc;
 }
+__global__ void k_set_step(double *__restrict__ dv, double dt, double cvisc)
+{
+    k_set_step_b(dv, dt, cvisc, own_blk());
+}
 
 // Sum of n partials in a fixed order by one block; out[0] = sum * scale.
-__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int n, double scale,
-                                                double *__restrict__ out)
+__device__ __forceinline__ void k_reduce_b(const double *__restrict__ partial, int n, double scale,
+                                                double *__restrict__ out, const Blk blk)
 {
     __shared__ double red[256];
     double a = 0.0;
@@ -875,6 +909,11 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
     }
     if (threadIdx.x == 0) out[0] = red[0] * scale;
 }
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int n, double scale,
+                                                double *__restrict__ out)
+{
+    k_reduce_b(partial, n, scale, out, own_blk());
+}
 
 // ---------------------------------------------------------------------------------------
 // Crank-Nicolson line solves as exact inverses.
@@ -882,16 +921,16 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ parti
 // column gx (computed on the host in long double).  Entries decay geometrically; D is the
 // half-width beyond which all entries are below 2^-64 of the diagonal (D = L/2 keeps all).
 template <typename W>
-__global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, int D, int full,
+__device__ __forceinline__ void k_cn_x_b(Geo g, const W *__restrict__ gx, int D, int full,
                                               W *__restrict__ rhs, const double *__restrict__ dpg,
-                                              int NLX)
+                                              int NLX, const Blk blk)
 {
-    // block: NLX consecutive x-lines (J, k) of component blockIdx.y
+    // block: NLX consecutive x-lines (J, k) of component blk.y
     extern __shared__ unsigned char smem_raw[];
     W *line = reinterpret_cast<W *>(smem_raw);
-    const int c = blockIdx.y;
+    const int c = blk.y;
     const int L = g.L, nl = g.M * g.N;
-    const int l0 = blockIdx.x * NLX;
+    const int l0 = blk.x * NLX;
     const size_t cs = (size_t)g.N * g.M * g.L;
     W *r = rhs + c * cs + (size_t)l0 * L;     // line l = k*M + J-1 starts at l*L
     const W sub = (c == 0) ? (W)dpg[0] : W(0);
@@ -923,6 +962,13 @@ __global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, i
         }
         r[t] = acc;
     }
+}
+template <typename W>
+__global__ __launch_bounds__(256) void k_cn_x(Geo g, const W *__restrict__ gx, int D, int full,
+                                              W *__restrict__ rhs, const double *__restrict__ dpg,
+                                              int NLX)
+{
+    k_cn_x_b<W>(g, gx, D, full, rhs, dpg, NLX, own_blk());
 }
 
 // x and z together (MOMFA1_X_tg then MOMFA2_Z_tg): a block holds whole x-lines, one row J per
@@ -1083,12 +1129,12 @@ __global__ __launch_bounds__(256) void k_cn_y(Geo g, const W *__restrict__ by, i
 //
 // z: one thread per (i, J) marches KZ consecutive planes k; circulant inverse gz (cyclic k).
 template <typename W, int D>
-__global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__ gz, const W *__restrict__ rin,
-                                                    W *__restrict__ rout, int KZ, const double *__restrict__ dpg)
+__device__ __forceinline__ void k_cn_z_march_b(Geo g, const W *__restrict__ gz, const W *__restrict__ rin,
+                                                    W *__restrict__ rout, int KZ, const double *__restrict__ dpg, const Blk blk)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y + 1);
-    const int c = blockIdx.z % 3, kc = blockIdx.z / 3;
+    const int i = blk.x * 64 + threadIdx.x;
+    const int J = __builtin_amdgcn_readfirstlane(blk.y * 4 + threadIdx.y + 1);
+    const int c = blk.z % 3, kc = blk.z / 3;
     if (i >= g.L || J > g.M) return;
     if (c == 1 && J == 1) return;
     const int N = g.N;
@@ -1138,6 +1184,12 @@ __global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__
         if (kk >= N) kk -= N;
         step(k, r[(size_t)kk * ks]);
     }
+}
+template <typename W, int D>
+__global__ __launch_bounds__(256) void k_cn_z_march(Geo g, const W *__restrict__ gz, const W *__restrict__ rin,
+                                                    W *__restrict__ rout, int KZ, const double *__restrict__ dpg)
+{
+    k_cn_z_march_b<W, D>(g, gz, rin, rout, KZ, dpg, own_blk());
 }
 
 // y: one thread per (i, k) marches a chunk of KY rows; banded inverse by[J][d] (row J is
@@ -1210,15 +1262,15 @@ __global__ __launch_bounds__(256) void k_cn_y_march(Geo g, const W *__restrict__
 // RHS_tg (RHS_CvLpGpS_tg.f90:45-46), is formed by the v thread itself from u's input rows, as
 // u's thread forms it, so the launch has no order between components.
 template <typename W, int D>
-__global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict__ by0, const W *__restrict__ by1,
+__device__ __forceinline__ void k_cn_y3_march_b(Geo g, const W *__restrict__ by0, const W *__restrict__ by1,
                                                      const W *__restrict__ by2, const W *__restrict__ rin,
-                                                     const double *__restrict__ qin, double *__restrict__ qout, int KY)
+                                                     const double *__restrict__ qin, double *__restrict__ qout, int KY, const Blk blk)
 {
     // a wavefront is 64 consecutive i of one z-plane k: the plane's bases are wave-uniform
     // (scalar registers) and a lane's offset (J - 1) L + i is a 32-bit vector value
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int k = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
-    const int c = blockIdx.z % 3, jc = blockIdx.z / 3;
+    const int i = blk.x * 64 + threadIdx.x;
+    const int k = __builtin_amdgcn_readfirstlane(blk.y * 4 + threadIdx.y);
+    const int c = blk.z % 3, jc = blk.z / 3;
     if (i >= g.L || k >= g.N) return;
     // the component's band (read-only restrict arguments: its coefficients, uniform per row,
     // come through scalar loads)
@@ -1284,6 +1336,13 @@ __global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict_
     }
     if (J1 == M) qorow(M + 1)[lane] = 0.0;
 }
+template <typename W, int D>
+__global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict__ by0, const W *__restrict__ by1,
+                                                     const W *__restrict__ by2, const W *__restrict__ rin,
+                                                     const double *__restrict__ qin, double *__restrict__ qout, int KY)
+{
+    k_cn_y3_march_b<W, D>(g, by0, by1, by2, rin, qin, qout, KY, own_blk());
+}
 
 // ---------------------------------------------------------------------------------------
 // Poisson.  Spectral data is complex, layout [J][kx][kz] (kz fastest), so z-lines are
@@ -1305,10 +1364,10 @@ __device__ __forceinline__ size_t six(const Geo &g, int J, int kx, int kz)
 // by 1/(L N) (the 1/L of FFT99 and the 1/N applied after CFFT99, possion3d_FFT99.f90:316,
 // :327-334) and written as spec[J][kx][k0..k0+TK-1] (TK consecutive values per kx).
 template <typename T>
-__global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, const double *__restrict__ dv, double talp,
+__device__ __forceinline__ void k_divg_fftx_b(Geo g, PoisArgs pa, const double *__restrict__ dv, double talp,
                                                    const double *__restrict__ q,
                                                    double *__restrict__ div, cpx<T> *__restrict__ spec,
-                                                   int TK, const double *__restrict__ src)
+                                                   int TK, const double *__restrict__ src, const Blk blk)
 {
     extern __shared__ unsigned char smem_raw[];
     const int L = g.L, H = g.H, ls = L + 1;
@@ -1316,7 +1375,7 @@ __global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, const 
     cpx<T> *a = tw + L;
     const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twx);
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
-    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK;
+    const int J = blk.x + 1, k0 = blk.y * TK;
     const int np = TK / 2;
     const size_t hs = (size_t)g.N * (g.M + 2) * L;
     const double *u = q, *v = q + hs, *w = q + 2 * hs;
@@ -1366,10 +1425,18 @@ __global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, const 
         spec[six(g, J, kx, k0 + ln)] = o;
     }
 }
+template <typename T>
+__global__ __launch_bounds__(256, 4) void k_divg_fftx(Geo g, PoisArgs pa, const double *__restrict__ dv, double talp,
+                                                   const double *__restrict__ q,
+                                                   double *__restrict__ div, cpx<T> *__restrict__ spec,
+                                                   int TK, const double *__restrict__ src)
+{
+    k_divg_fftx_b<T>(g, pa, dv, talp, q, div, spec, TK, src, own_blk());
+}
 
 // Complex z-transform of NL contiguous lines [J][kx][:] (both directions).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__restrict__ spec, int NL, int sign)
+__device__ __forceinline__ void k_fftz_b(Geo g, PoisArgs pa, cpx<T> *__restrict__ spec, int NL, int sign, const Blk blk)
 {
     extern __shared__ unsigned char smem_raw[];
     const int N = g.N, ls = N + 1;
@@ -1378,7 +1445,7 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
     const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twz);
     for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
     const int nlines = g.M * g.H;
-    const int l0 = blockIdx.x * NL;
+    const int l0 = blk.x * NL;
     const int nl = min(NL, nlines - l0);
     cpx<T> *base = spec + (size_t)l0 * N;
     chs::stage_items<4, cpx<T>>(
@@ -1393,6 +1460,11 @@ __global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__rest
         const int ln = t / N, kz = t - ln * N;
         base[t] = a[ln * ls + kz];
     }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_fftz(Geo g, PoisArgs pa, cpx<T> *__restrict__ spec, int NL, int sign)
+{
+    k_fftz_b<T>(g, pa, spec, NL, sign, own_blk());
 }
 
 // Per-mode y-Thomas (TDMAIJJ_nonCYC.f90:84-120 on the FFT99 coefficients,
@@ -1530,15 +1602,17 @@ __global__ __launch_bounds__(256) void k_ytdma(Geo g, const double *__restrict__
 // and the last chunk x_in = 0 (x_{M+1} = 0 gives x_M = g_M), so these are the recurrences of
 // ytdma_line with the sweeps split by linearity; only rounding differs.  Gauges as ytdma_line.
 template <typename T, int C>
-__global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__restrict__ amph,
+__device__ __forceinline__ void k_ytdma_chunk_b(Geo g, const double *__restrict__ amph,
                                                       const double *__restrict__ apph, const double *__restrict__ pp,
                                                       const float *__restrict__ pp32, const double *__restrict__ pp00,
-                                                      cpx<T> *__restrict__ spec)
+                                                      cpx<T> *__restrict__ spec, int P, const Blk blk)
 {
     __shared__ double xr[16][64], xi[16][64], xl[16][64];
     const int HN = g.H * g.N, M = g.M;
-    const int lane = threadIdx.x, q = __builtin_amdgcn_readfirstlane(threadIdx.y), P = blockDim.y;
-    const int t0 = blockIdx.x * 64, t = t0 + lane;
+    // P: chunks of this launch's lines (blockDim.y on its own launch; an ensemble launch sizes
+    // its blocks for its largest M, and the chunks q >= P of a smaller mesh idle)
+    const int lane = threadIdx.x, q = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int t0 = blk.x * 64, t = t0 + lane;
     const bool act = t < HN;
     const int tt = act ? t : HN - 1;   // lanes past the last line solve a copy of it, unstored
     const unsigned lo = (unsigned)(tt - t0);   // lane offset from the wavefront's first line
@@ -1640,13 +1714,21 @@ __global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__res
             if (u < n) srow(u)[lo] = {(T)rr[u], (T)ri[u]};
     }
 }
+template <typename T, int C>
+__global__ __launch_bounds__(1024) void k_ytdma_chunk(Geo g, const double *__restrict__ amph,
+                                                      const double *__restrict__ apph, const double *__restrict__ pp,
+                                                      const float *__restrict__ pp32, const double *__restrict__ pp00,
+                                                      cpx<T> *__restrict__ spec)
+{
+    k_ytdma_chunk_b<T, C>(g, amph, apph, pp, pp32, pp00, spec, int(blockDim.y), own_blk());
+}
 
 // Inverse real x-transform of TK z-planes at row J -> phi (halo field) and the Neumann
 // ghost rows of BC_WALL_DPH_tg (BC_WALL.f90:369-386).  Z = A + i B with the Hermitian
 // extension; imaginary parts at kx = 0 and L/2 are not used (FFT99.f90:84-86).
 template <typename T>
-__global__ __launch_bounds__(256, 4) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
-                                                  double *__restrict__ phi, int TK)
+__device__ __forceinline__ void k_fftx_inv_b(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
+                                                  double *__restrict__ phi, int TK, const Blk blk)
 {
     extern __shared__ unsigned char smem_raw[];
     const int L = g.L, M = g.M, H = g.H, ls = L + 1;
@@ -1654,7 +1736,7 @@ __global__ __launch_bounds__(256, 4) void k_fftx_inv(Geo g, PoisArgs pa, const c
     cpx<T> *a = tw + L;
     const cpx<T> *twg = reinterpret_cast<const cpx<T> *>(pa.twx);
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
-    const int J = blockIdx.x + 1, k0 = blockIdx.y * TK;
+    const int J = blk.x + 1, k0 = blk.y * TK;
     const int np = TK / 2;
     struct C2 { cpx<T> A, B; };
     chs::stage_items<4, C2>(
@@ -1682,6 +1764,12 @@ __global__ __launch_bounds__(256, 4) void k_fftx_inv(Geo g, PoisArgs pa, const c
             if (J == M) phi[hix(g, i, M + 1, k)] = val;
         }
     }
+}
+template <typename T>
+__global__ __launch_bounds__(256, 4) void k_fftx_inv(Geo g, PoisArgs pa, const cpx<T> *__restrict__ spec,
+                                                  double *__restrict__ phi, int TK)
+{
+    k_fftx_inv_b<T>(g, pa, spec, phi, TK, own_blk());
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1728,7 +1816,7 @@ __global__ __launch_bounds__(64) void k_divg_fftx_r(Geo g, const cpx<T> *__restr
     __shared__ cpx<T> lds[G::LPW * G::LS];
     const int li = threadIdx.x / G::TL, t = threadIdx.x % G::TL;
     // rows J and J+1 (whose v this block also reads) on one XCD, so the re-read hits its L2
-    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd));
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd, own_blk()));
     const int J = tile % gridDim.x + 1, k0 = (tile / gridDim.x) * TK, ka = k0 + 2 * li;
     const size_t hs = (size_t)g.N * (g.M + 2) * L;
     const double *u = q, *v = q + hs, *w = q + 2 * hs;
@@ -1829,15 +1917,15 @@ __global__ __launch_bounds__(64) void k_fftx_inv_r(Geo g, const cpx<T> *__restri
 // Contraction off: a correction is the same product and difference wherever it is formed, so
 // a neighbour's corrected value here equals the one its own thread stores.
 template <bool CFL>
-__global__ __launch_bounds__(1024) void k_update(Geo g, Sub s, const double *__restrict__ phi,
+__device__ __forceinline__ void k_update_b(Geo g, Sub s, const double *__restrict__ phi,
                                                 const double *__restrict__ qs, double *__restrict__ q,
-                                                double *__restrict__ pr, int KZ, double *__restrict__ part)
+                                                double *__restrict__ pr, int KZ, double *__restrict__ part, const Blk blk)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int i = blk.x * 64 + threadIdx.x;
     // a wavefront is one row J: row bases are wave-uniform (scalar), lanes add i, ip or im
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y + 1);
-    const int k0 = blockIdx.z * KZ;
+    const int J = __builtin_amdgcn_readfirstlane(blk.y * blockDim.y + threadIdx.y + 1);
+    const int k0 = blk.z * KZ;
     double cmax = 0.0;
     if (i < g.L && J <= g.M) {
         const int L = g.L, M = g.M, N = g.N;
@@ -1909,8 +1997,15 @@ __global__ __launch_bounds__(1024) void k_update(Geo g, Sub s, const double *__r
             if (t < o) red[t] = nanmax(red[t], red[t + o]);
             __syncthreads();
         }
-        if (t == 0) part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+        if (t == 0) part[((size_t)blk.z * blk.gy + blk.y) * blk.gx + blk.x] = red[0];
     }
+}
+template <bool CFL>
+__global__ __launch_bounds__(1024) void k_update(Geo g, Sub s, const double *__restrict__ phi,
+                                                const double *__restrict__ qs, double *__restrict__ q,
+                                                double *__restrict__ pr, int KZ, double *__restrict__ part)
+{
+    k_update_b<CFL>(g, s, phi, qs, q, pr, KZ, part, own_blk());
 }
 
 // A refinement sweep's correction, in place: the velocity as k_update, p += phi (PRCALC_tg's
@@ -2057,7 +2152,7 @@ __global__ __launch_bounds__(256) void k_cfl_div(Geo g, const double *__restrict
 }
 
 // Maximum of n block partials by one block of blockDim.x (a power of two <= 1024) threads.
-__global__ __launch_bounds__(1024) void k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
+__device__ __forceinline__ void k_reduce_max_b(const double *__restrict__ part, int n, double *__restrict__ out, const Blk blk)
 {
     __shared__ double red[1024];
     double a = 0.0;
@@ -2069,6 +2164,10 @@ __global__ __launch_bounds__(1024) void k_reduce_max(const double *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
+}
+__global__ __launch_bounds__(1024) void k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+    k_reduce_max_b(part, n, out, own_blk());
 }
 
 // x-z mean statistics, PP_MEAN_ZX_FLOW_TG (processing/PP_MEAN_ZX.f90:14-307), channel.  Per
@@ -2197,6 +2296,215 @@ bool fft_ok(int n, chs::FftPlan &p)
     while (m % 5 == 0) { p.radix[p.npass++] = 5; m /= 5; }
     return m == 1 && p.npass <= 16;
 }
+
+// ---------------------------------------------------------------------------------------
+// Launches over many channels (ensembles).  A kind K names a kernel body and its arguments;
+// a launch of k_multi<K> runs the body for n items (channels), item i over its own grid
+// (gx, gy, gz).  The blocks are numbered over the items in turn (bstart[i] = first block of
+// item i); a block finds its item by a binary search (wave-uniform) and runs the body with
+// its position in that item's grid.  Every item of a launch has the same block shape and
+// dynamic LDS size (the largest any item needs).
+template <class A> struct MItem {
+    A a;
+    unsigned gx, gy, gz;
+};
+
+template <class K>
+__global__ __launch_bounds__(K::NT, K::MINW) void k_multi(const MItem<typename K::Args> *__restrict__ it,
+                                                          const int *__restrict__ bstart, int n)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const MItem<typename K::Args> &m = it[lo];
+    const unsigned lb = (unsigned)(b - bstart[lo]), gx = m.gx, gy = m.gy, gz = m.gz;
+    K::run(m.a, Blk{lb % gx, (lb / gx) % gy, lb / (gx * gy), gx, gy, gz});
+}
+
+template <typename W, bool FENCE> struct KConv {   // k_conv_rhs, 64 x 4 blocks, no x-solve
+    struct Args {
+        Geo g;
+        Sub s;
+        const double *q, *pr;
+        W *h0, *rhs;
+        double *partial;
+        int KC;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        k_conv_rhs_b<W, 4, 1, FENCE, false>(a.g, a.s, a.q, a.pr, a.h0, a.rhs, a.partial, a.KC, 1, nullptr, 0, blk);
+    }
+};
+struct KReduce {
+    struct Args {
+        const double *partial;
+        int n;
+        double scale;
+        double *out;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_reduce_b(a.partial, a.n, a.scale, a.out, blk); }
+};
+template <typename W> struct KCnX {
+    struct Args {
+        Geo g;
+        const W *gx;
+        int D, full;
+        W *rhs;
+        const double *dpg;
+        int NLX;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        k_cn_x_b<W>(a.g, a.gx, a.D, a.full, a.rhs, a.dpg, a.NLX, blk);
+    }
+};
+template <typename W> struct KCnZ {
+    struct Args {
+        Geo g;
+        const W *gz, *rin;
+        W *rout;
+        int KZ, D;
+        const double *dpg;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        switch (a.D) {
+#define CHS_MZ(d) case d: k_cn_z_march_b<W, d>(a.g, a.gz, a.rin, a.rout, a.KZ, a.dpg, blk); break;
+            CHS_MZ(0) CHS_MZ(1) CHS_MZ(2) CHS_MZ(3) CHS_MZ(4) CHS_MZ(5) CHS_MZ(6) CHS_MZ(7) CHS_MZ(8)
+#undef CHS_MZ
+        }
+    }
+};
+template <typename W> struct KCnY3 {
+    struct Args {
+        Geo g;
+        const W *by0, *by1, *by2, *rin;
+        const double *qin;
+        double *qout;
+        int KY, D;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        switch (a.D) {
+#define CHS_MY(d) case d: k_cn_y3_march_b<W, d>(a.g, a.by0, a.by1, a.by2, a.rin, a.qin, a.qout, a.KY, blk); break;
+            CHS_MY(0) CHS_MY(1) CHS_MY(2) CHS_MY(3) CHS_MY(4) CHS_MY(5) CHS_MY(6) CHS_MY(7) CHS_MY(8)
+#undef CHS_MY
+        }
+    }
+};
+template <typename T> struct KDivg {
+    struct Args {
+        Geo g;
+        PoisArgs pa;
+        const double *dv;
+        double talp;
+        const double *q;
+        cpx<T> *spec;
+        int TK;
+    };
+    static constexpr int NT = 256, MINW = 4;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        k_divg_fftx_b<T>(a.g, a.pa, a.dv, a.talp, a.q, nullptr, a.spec, a.TK, nullptr, blk);
+    }
+};
+template <typename T> struct KFftz {
+    struct Args {
+        Geo g;
+        PoisArgs pa;
+        cpx<T> *spec;
+        int NL, sign;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_fftz_b<T>(a.g, a.pa, a.spec, a.NL, a.sign, blk); }
+};
+template <typename T> struct KYtdma {   // 16-row chunks; blocks of 64 x (largest chunk count)
+    struct Args {
+        Geo g;
+        const double *amph, *apph, *pp;
+        const float *pp32;
+        cpx<T> *spec;
+        int P;
+    };
+    static constexpr int NT = 1024, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        k_ytdma_chunk_b<T, 16>(a.g, a.amph, a.apph, a.pp, a.pp32, nullptr, a.spec, a.P, blk);
+    }
+};
+template <typename T> struct KFftxInv {
+    struct Args {
+        Geo g;
+        PoisArgs pa;
+        const cpx<T> *spec;
+        double *phi;
+        int TK;
+    };
+    static constexpr int NT = 256, MINW = 4;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_fftx_inv_b<T>(a.g, a.pa, a.spec, a.phi, a.TK, blk); }
+};
+struct UpdArgs {
+    Geo g;
+    Sub s;
+    const double *phi, *qs;
+    double *q, *pr;
+    int KZ;
+    double *part;
+};
+template <bool CFL> struct KUpdate {   // 64 x 4 blocks
+    using Args = UpdArgs;
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk)
+    {
+        k_update_b<CFL>(a.g, a.s, a.phi, a.qs, a.q, a.pr, a.KZ, a.part, blk);
+    }
+};
+struct KReduceMax {
+    struct Args {
+        const double *part;
+        int n;
+        double *out;
+    };
+    static constexpr int NT = 1024, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_reduce_max_b(a.part, a.n, a.out, blk); }
+};
+struct KSetStep {
+    struct Args {
+        double *dv;
+        double dt, cvisc;
+    };
+    static constexpr int NT = 64, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_set_step_b(a.dv, a.dt, a.cvisc, blk); }
+};
+template <typename W> struct KCnTables {
+    struct Args {
+        Geo g;
+        CnDev<W> t;
+        const double *dv;
+        double *stamp;
+        int ns0;
+    };
+    static constexpr int NT = 256, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_cn_tables_b<W>(a.g, a.t, a.dv, a.stamp, a.ns0, blk); }
+};
+struct KCnStamp {
+    struct Args {
+        const double *dv;
+        double *stamp;
+        int ns0, n;
+    };
+    static constexpr int NT = 64, MINW = 1;
+    static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_cn_stamp_b(a.dv, a.stamp, a.ns0, a.n, blk); }
+};
 
 struct CnTab {
     bool valid = false;
@@ -3670,6 +3978,12 @@ int chs_profile_get(chs_ctx *c, const char *kernel, double *avg_ms, int *count)
 struct chs_ens {
     std::vector<chs_ctx *> ch;
     hipStream_t stream = nullptr;
+    // batched: one launch per kernel kind and stage over all channels (k_multi); otherwise
+    // (the default: faster on the uneven workload, see chs_ens_set_batched) each channel's own
+    // launches on the lanes below
+    bool batched = false;
+    bool gbatched = false;          // the current graph is the batched one
+    std::vector<void *> mbufs;      // device item tables of the batched graph
     // Capture lanes: channel i's launches go to lane i % lanes.size(), so the step graph holds
     // independent chains that the GPU runs side by side (small channels alone fill little of it).
     std::vector<hipStream_t> lanes;
@@ -3719,6 +4033,7 @@ void chs_ens_destroy(chs_ens *e)
 {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (void *b : e->mbufs) (void)hipFree(b);
     for (auto *c : e->ch) chs_destroy(c);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     for (auto st : e->lanes) (void)hipStreamDestroy(st);
@@ -3735,6 +4050,252 @@ int chs_ens_channel(chs_ens *e, int i, chs_ctx **ctx)
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One k_multi launch: the channels' items and first blocks in device memory (owned by the
+// ensemble), the block shape and the dynamic LDS size.
+template <class K> struct MLaunch {
+    const MItem<typename K::Args> *it = nullptr;
+    const int *bstart = nullptr;
+    int n = 0, nblk = 0;
+    dim3 blk;
+    size_t sh = 0;
+    void launch(hipStream_t st) const
+    {
+        hipLaunchKernelGGL(k_multi<K>, dim3(nblk), blk, sh, st, it, bstart, n);
+    }
+};
+
+template <class K>
+int mlaunch_make(chs_ens *e, const std::vector<typename K::Args> &args, const std::vector<dim3> &grids, dim3 blk,
+                 size_t sh, MLaunch<K> &ml)
+{
+    using It = MItem<typename K::Args>;
+    const int n = (int)args.size();
+    std::vector<It> items(n);
+    std::vector<int> first(n);
+    long long nb = 0;
+    for (int i = 0; i < n; ++i) {
+        items[i].a = args[i];
+        items[i].gx = grids[i].x;
+        items[i].gy = grids[i].y;
+        items[i].gz = grids[i].z;
+        first[i] = (int)nb;
+        nb += (long long)grids[i].x * grids[i].y * grids[i].z;
+    }
+    if (nb < 1 || nb > 0x7fffffffLL) return fail(nullptr, CHS_E_ARG, "ensemble launch: block count out of range");
+    const size_t bi = (sizeof(It) * n + 255) / 256 * 256;
+    void *d = nullptr;
+    hipError_t he = hipMalloc(&d, bi + sizeof(int) * n);
+    if (he == hipSuccess) e->mbufs.push_back(d);
+    if (he == hipSuccess) he = hipMemcpy(d, items.data(), sizeof(It) * n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy((char *)d + bi, first.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("ensemble launch tables: ") + hipGetErrorString(he));
+    ml.it = (const It *)d;
+    ml.bstart = (const int *)((char *)d + bi);
+    ml.n = n;
+    ml.nblk = (int)nb;
+    ml.blk = blk;
+    ml.sh = sh;
+    return 0;
+}
+
+// Whether every channel's step fits the batched launches: banded z and y solves (one y width),
+// chunked y-Thomas (M <= 256), no refinement sweeps inside a step, the fused y-solve enabled.
+bool ens_batchable(const chs_ens *e)
+{
+    for (const chs_ctx *c : e->ch)
+        for (int ns = 1; ns <= 3; ++ns) {
+            const CnTab &t = c->cn[ns];
+            if (t.fullz || t.Dz > 8 || t.Dy[0] > 8 || t.Dy[1] != t.Dy[0] || t.Dy[2] != t.Dy[0]) return false;
+            if (c->M > 256 || c->nref_step != 0 || c->tune.cny3_ky < 0 || c->tune.ytdma_c == 0) return false;
+            if (c->tune.cny3_ky > 0 && c->tune.cny3_ky <= t.Dy[0]) return false;
+        }
+    return true;
+}
+
+// The batched step graph's launches, in a channel's own order (substep_impl with the x-solve in
+// its own kernel, the LDS transforms, 64 x 4 stencil blocks): per stage convection, mass-flux
+// sum, x-, z-, y-solves, divergence + x-transform, z-transform, y-Thomas, inverse z- and
+// x-transforms, correction (with the CFL on the last stage), and first the step scalars and
+// the stamp-guarded tables.  Channel i of each launch is item i.
+template <typename W> struct EnsLaunches {
+    MLaunch<KSetStep> set;
+    MLaunch<KCnTables<W>> tab;
+    MLaunch<KCnStamp> stamp;
+    MLaunch<KConv<W, std::is_same<W, float>::value>> conv[3];
+    MLaunch<KReduce> red[3];
+    MLaunch<KCnX<W>> cnx[3];
+    MLaunch<KCnZ<W>> cnz[3];
+    MLaunch<KCnY3<W>> cny[3];
+    MLaunch<KDivg<W>> divg[3];
+    MLaunch<KFftz<W>> fz[3], fzi[3];
+    MLaunch<KYtdma<W>> yt[3];
+    MLaunch<KFftxInv<W>> fxi[3];
+    MLaunch<KUpdate<false>> upd[2];
+    MLaunch<KUpdate<true>> updc;
+    MLaunch<KReduceMax> rmax;
+};
+
+template <typename W>
+int ens_make_launches(chs_ens *e, const double *dt, const double *cvisc, EnsLaunches<W> &el)
+{
+    const int n = (int)e->ch.size();
+    const bool mixed = std::is_same<W, float>::value;
+    int rc = 0;
+    {
+        std::vector<KSetStep::Args> a;
+        std::vector<typename KCnTables<W>::Args> b;
+        std::vector<KCnStamp::Args> d;
+        std::vector<dim3> g1, g2;
+        for (int i = 0; i < n; ++i) {
+            chs_ctx *c = e->ch[i];
+            a.push_back({c->dstep, dt[i], cvisc[i]});
+            b.push_back({c->geo, cn_dev<W>(c), c->dstep, c->cn_stamp, 1});
+            d.push_back({c->dstep, c->cn_stamp, 1, 3});
+            g1.push_back(dim3(1));
+            g2.push_back(dim3(5, 3));
+        }
+        if (!rc) rc = mlaunch_make(e, a, g1, dim3(64), 0, el.set);
+        if (!rc) rc = mlaunch_make(e, b, g2, dim3(256), 0, el.tab);
+        if (!rc) rc = mlaunch_make(e, d, g1, dim3(64), 0, el.stamp);
+    }
+    for (int ns = 1; ns <= 3 && !rc; ++ns) {
+        std::vector<typename KConv<W, std::is_same<W, float>::value>::Args> aconv;
+        std::vector<KReduce::Args> ared;
+        std::vector<typename KCnX<W>::Args> acnx;
+        std::vector<typename KCnZ<W>::Args> acnz;
+        std::vector<typename KCnY3<W>::Args> acny;
+        std::vector<typename KDivg<W>::Args> adiv;
+        std::vector<typename KFftz<W>::Args> afz, afzi;
+        std::vector<typename KYtdma<W>::Args> ayt;
+        std::vector<typename KFftxInv<W>::Args> afxi;
+        std::vector<KUpdate<false>::Args> aupd;
+        std::vector<KReduceMax::Args> armax;
+        std::vector<dim3> gconv, gred, gcnx, gcnz, gcny, gdiv, gfz, gyt, gupd;
+        size_t shx = 0, shz = 0, shcx = 0;
+        int Pmax = 1;
+        for (int i = 0; i < n; ++i) {
+            chs_ctx *c = e->ch[i];
+            const CnTab &t = c->cn[ns];
+            const Geo &g = c->geo;
+            const Sub s = {c->tgam[ns], c->troh[ns], c->talp[ns], c->dstep};
+            const int L = c->L, M = c->M, N = c->N, H = c->H;
+            W *h0 = (W *)c->h0, *rhs = (W *)c->rhs, *rhs2 = (W *)c->rhs2;
+            cpx<W> *spec = (cpx<W> *)c->spec;
+            const int KC = c->tune.conv_kc, KZ = c->tune.cnz_kz, KU = c->tune.upd_kz;
+            const int KY3 = c->tune.cny3_ky > 0 ? c->tune.cny3_ky : M;
+            const int NLX = c->tune.cnx_nlx > 0 ? c->tune.cnx_nlx : ((2048 / L) < 1 ? 1 : 2048 / L);
+            const int TK = c->tune.fftx_tk > 0 ? c->tune.fftx_tk : tk_for(L, N);
+            const int NL = c->tune.fftz_nl > 0 ? c->tune.fftz_nl : nlz_for(N);
+            const PoisArgs pa{c->px, c->pz, c->twx, c->twz};
+            const dim3 gc((L + 63) / 64, (M + 3) / 4, (N + KC - 1) / KC);
+            aconv.push_back({g, s, c->q, c->pr, h0, rhs, c->partial, KC});
+            gconv.push_back(gc);
+            ared.push_back({c->partial, (int)(gc.x * gc.y * gc.z), 1.0 / c->volm, c->scal});
+            gred.push_back(dim3(1));
+            acnx.push_back({g, (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, NLX});
+            gcnx.push_back(dim3((M * N + NLX - 1) / NLX, 3));
+            shcx = std::max(shcx, sizeof(W) * NLX * L);
+            acnz.push_back({g, (const W *)t.gz, rhs, rhs2, KZ, t.Dz, nullptr});
+            gcnz.push_back(dim3((L + 63) / 64, (M + 3) / 4, 3 * ((N + KZ - 1) / KZ)));
+            acny.push_back({g, (const W *)t.by[0], (const W *)t.by[1], (const W *)t.by[2], rhs2, c->q, c->qs, KY3,
+                            t.Dy[0]});
+            gcny.push_back(dim3((L + 63) / 64, (N + 3) / 4, 3 * ((M + KY3 - 1) / KY3)));
+            adiv.push_back({g, pa, c->dstep, c->talp[ns], c->qs, spec, TK});
+            gdiv.push_back(dim3(M, N / TK));
+            shx = std::max(shx, sizeof(cpx<W>) * (L + (TK / 2) * (L + 1)));
+            afz.push_back({g, pa, spec, NL, -1});
+            afzi.push_back({g, pa, spec, NL, +1});
+            gfz.push_back(dim3((M * H + NL - 1) / NL));
+            shz = std::max(shz, sizeof(cpx<W>) * (N + NL * (N + 1)));
+            const int P = (M + 15) / 16;
+            Pmax = std::max(Pmax, P);
+            ayt.push_back({g, c->d_amph, c->d_apph, c->pp, mixed ? c->pp32 : (const float *)nullptr, spec, P});
+            gyt.push_back(dim3((H * N + 63) / 64));
+            afxi.push_back({g, pa, (const cpx<W> *)spec, c->phi, TK});
+            const dim3 gu((L + 63) / 64, (M + 3) / 4, (N + KU - 1) / KU);
+            aupd.push_back({g, s, c->phi, c->qs, c->q, c->pr, KU, c->partial});
+            gupd.push_back(gu);
+            armax.push_back({c->partial, (int)(gu.x * gu.y * gu.z), c->dstep + 2});
+        }
+        const int k = ns - 1;
+        if (!rc) rc = mlaunch_make(e, aconv, gconv, dim3(64, 4), 0, el.conv[k]);
+        if (!rc) rc = mlaunch_make(e, ared, gred, dim3(256), 0, el.red[k]);
+        if (!rc) rc = mlaunch_make(e, acnx, gcnx, dim3(256), shcx, el.cnx[k]);
+        if (!rc) rc = mlaunch_make(e, acnz, gcnz, dim3(64, 4), 0, el.cnz[k]);
+        if (!rc) rc = mlaunch_make(e, acny, gcny, dim3(64, 4), 0, el.cny[k]);
+        if (!rc) rc = mlaunch_make(e, adiv, gdiv, dim3(256), shx, el.divg[k]);
+        if (!rc) rc = mlaunch_make(e, afz, gfz, dim3(256), shz, el.fz[k]);
+        if (!rc) rc = mlaunch_make(e, ayt, gyt, dim3(64, Pmax), 0, el.yt[k]);
+        if (!rc) rc = mlaunch_make(e, afzi, gfz, dim3(256), shz, el.fzi[k]);
+        if (!rc) rc = mlaunch_make(e, afxi, gdiv, dim3(256), shx, el.fxi[k]);
+        if (ns < 3) {
+            if (!rc) rc = mlaunch_make(e, aupd, gupd, dim3(64, 4), 0, el.upd[k]);
+        } else {
+            if (!rc) rc = mlaunch_make(e, aupd, gupd, dim3(64, 4), 0, el.updc);
+            if (!rc) rc = mlaunch_make(e, armax, gred, dim3(1024), 0, el.rmax);
+        }
+    }
+    (void)mixed;
+    return rc;
+}
+
+template <typename W> void ens_record(const EnsLaunches<W> &el, hipStream_t st)
+{
+    el.set.launch(st);
+    el.tab.launch(st);
+    el.stamp.launch(st);
+    for (int k = 0; k < 3; ++k) {
+        el.conv[k].launch(st);
+        el.red[k].launch(st);
+        el.cnx[k].launch(st);
+        el.cnz[k].launch(st);
+        el.cny[k].launch(st);
+        el.divg[k].launch(st);
+        el.fz[k].launch(st);
+        el.yt[k].launch(st);
+        el.fzi[k].launch(st);
+        el.fxi[k].launch(st);
+        if (k < 2) el.upd[k].launch(st);
+    }
+    el.updc.launch(st);
+    el.rmax.launch(st);
+}
+
+// Capture of the batched step graph into e->gexec (the item tables are made first: their
+// uploads cannot be captured).
+template <typename W> int ens_capture_batched(chs_ens *e, const double *dt, const double *cvisc)
+{
+    for (void *b : e->mbufs) (void)hipFree(b);
+    e->mbufs.clear();
+    EnsLaunches<W> el;
+    if (int rc = ens_make_launches<W>(e, dt, cvisc, el)) return rc;
+    hipGraph_t graph = nullptr;
+    hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(he));
+    ens_record<W>(el, e->stream);
+    he = hipStreamEndCapture(e->stream, &graph);
+    if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+    he = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chs_ens_set_batched(chs_ens *e, int batched)
+{
+    if (!e || (batched != 0 && batched != 1)) return fail(nullptr, CHS_E_ARG, "chs_ens_set_batched: bad argument");
+    e->batched = batched == 1;
+    return 0;
+}
+
 int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
 {
     if (!e || !dt || !cvisc) return fail(nullptr, CHS_E_ARG, "chs_ens_step: bad argument");
@@ -3747,15 +4308,34 @@ int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc)
         for (int ns = 1; ns <= 3; ++ns)
             if (int rc = build_cn(c, ns, dt[i], cvisc[i])) return rc;
     }
-    bool same = e->gexec && (int)e->gdt.size() == n;
+    const bool batched = e->batched && ens_batchable(e);
+    bool same = e->gexec && (int)e->gdt.size() == n && e->gbatched == batched;
     for (int i = 0; same && i < n; ++i)
         same = e->gdt[i] == dt[i] && e->gcv[i] == cvisc[i] && e->gtune[i] == e->ch[i]->tune_gen &&
                e->gcn[i] == e->ch[i]->cn_gen;
-    if (!same) {
+    if (!same && batched) {
         if (e->gexec) {
             (void)hipGraphExecDestroy(e->gexec);
             e->gexec = nullptr;
         }
+        int rc = e->ch[0]->mode == CHS_MIXED ? ens_capture_batched<float>(e, dt, cvisc)
+                                             : ens_capture_batched<double>(e, dt, cvisc);
+        if (rc) return rc;
+        e->gbatched = true;
+        e->gdt.assign(dt, dt + n);
+        e->gcv.assign(cvisc, cvisc + n);
+        e->gtune.resize(n);
+        e->gcn.resize(n);
+        for (int i = 0; i < n; ++i) {
+            e->gtune[i] = e->ch[i]->tune_gen;
+            e->gcn[i] = e->ch[i]->cn_gen;
+        }
+    } else if (!same) {
+        if (e->gexec) {
+            (void)hipGraphExecDestroy(e->gexec);
+            e->gexec = nullptr;
+        }
+        e->gbatched = false;
         hipGraph_t graph = nullptr;
         hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
         if (he != hipSuccess) return fail(nullptr, CHS_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(he));

@@ -68,6 +68,7 @@ SIGNATURES = {
     "chs_ens_channel": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "chs_ens_step": (C.c_int, [C.c_void_p, _dp, _dp]),
     "chs_ens_sync": (C.c_int, [C.c_void_p]),
+    "chs_ens_set_batched": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_prop_table_setup": (C.c_int, [C.c_int, _dp, C.c_double, _dp, _dp, C.POINTER(C.c_int), _dp]),
     "chs_prop_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "chs_prop_destroy": (None, [C.c_void_p]),
@@ -361,6 +362,11 @@ class Ensemble:
 
     def sync(self):
         _check(lib().chs_ens_sync(self._h))
+
+    def set_batched(self, batched: bool):
+        """One launch per kernel kind over all channels (True) or each channel's own launches
+        on parallel capture lanes (False, the default)."""
+        _check(lib().chs_ens_set_batched(self._h, 1 if batched else 0))
 
     def close(self):
         if getattr(self, "_h", None):

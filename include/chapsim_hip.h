@@ -250,8 +250,19 @@ void chs_ens_destroy(chs_ens *e);
 int chs_ens_channel(chs_ens *e, int i, chs_ctx **ctx);
 
 /* One RK3 step of every channel (dt[i], cvisc[i]) replayed from a single hipGraph, captured
- * once per set of (dt, cvisc); stage-major order (all channels' NS = 1, then NS = 2, 3). */
+ * once per set of (dt, cvisc); stage-major order (all channels' NS = 1, then NS = 2, 3).
+ * By default each channel's own launches go to parallel capture lanes (a channel's result
+ * equals its single-channel step bit for bit).  Batched (chs_ens_set_batched): one launch per
+ * kernel kind and stage covers every channel, each block running one channel's tile, with the
+ * x-solve in its own kernel and the LDS transforms for every mesh, so a channel equals its
+ * single-channel step with the tunings conv_fx 0 and fft_reg 0 bit for bit; channels outside
+ * the batched kernels' limits (M > 256, a full z or y inverse, refinement sweeps inside a
+ * step) make the step use the lanes. */
 int chs_ens_step(chs_ens *e, const double *dt, const double *cvisc);
+
+/* Batched launches (1) or each channel's own launches on capture lanes (0, default); no
+ * counterpart in the reference. */
+int chs_ens_set_batched(chs_ens *e, int batched);
 int chs_ens_sync(chs_ens *e);
 
 /* ---- Thermophysical property tables (heated channel, table mode) ---------------------- */
