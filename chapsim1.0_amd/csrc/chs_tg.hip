@@ -2427,18 +2427,22 @@ template <typename T> struct KFftz {
     static constexpr int NT = 256, MINW = 1;
     static __device__ __forceinline__ void run(const Args &a, Blk blk) { k_fftz_b<T>(a.g, a.pa, a.spec, a.NL, a.sign, blk); }
 };
-template <typename T> struct KYtdma {   // 16-row chunks; blocks of 64 x (largest chunk count)
+template <typename T> struct KYtdma {   // C-row chunks; blocks of 64 x (largest chunk count)
     struct Args {
         Geo g;
         const double *amph, *apph, *pp;
         const float *pp32;
         cpx<T> *spec;
-        int P;
+        int P, C;
     };
     static constexpr int NT = 1024, MINW = 1;
     static __device__ __forceinline__ void run(const Args &a, Blk blk)
     {
-        k_ytdma_chunk_b<T, 16>(a.g, a.amph, a.apph, a.pp, a.pp32, nullptr, a.spec, a.P, blk);
+        switch (a.C) {
+        case 4: k_ytdma_chunk_b<T, 4>(a.g, a.amph, a.apph, a.pp, a.pp32, nullptr, a.spec, a.P, blk); break;
+        case 8: k_ytdma_chunk_b<T, 8>(a.g, a.amph, a.apph, a.pp, a.pp32, nullptr, a.spec, a.P, blk); break;
+        default: k_ytdma_chunk_b<T, 16>(a.g, a.amph, a.apph, a.pp, a.pp32, nullptr, a.spec, a.P, blk); break;
+        }
     }
 };
 template <typename T> struct KFftxInv {
@@ -2909,12 +2913,17 @@ void fftx_inv_r(int L, dim3 grid, hipStream_t st, const Geo &g, const cpx<T> *tw
 #undef CHS_RCASES
 
 // Rows per chunk of k_ytdma_chunk (4, 8 or 16; at most 16 chunks, so M <= 256 — longer
-// chunks exceed the 128 registers a 1024-thread block allows): the tuning value if it fits,
-// else 16; 0, or a mesh with M > 256, selects the one-thread-per-line k_ytdma.
-int ytdma_chunk_rows(int want, int M)
+// chunks exceed the 128 registers a 1024-thread block allows): the tuning value if it fits;
+// automatic (-1): 16, or on meshes with fewer than 256 blocks of 64 lines the shortest
+// chunks that fit (more wavefronts per block where blocks are too few to fill the GPU);
+// 0, or a mesh with M > 256, selects the one-thread-per-line k_ytdma.
+int ytdma_chunk_rows(int want, int M, int lines)
 {
     if (want == 0 || M > 16 * 16) return 0;
     if ((want == 4 || want == 8) && (M + want - 1) / want <= 16) return want;
+    if (want < 0 && (lines + 63) / 64 < 256)
+        for (int c : {4, 8})
+            if ((M + c - 1) / c <= 16) return c;
     return 16;
 }
 
@@ -2958,7 +2967,7 @@ void poisson_launch(chs_ctx *c, const double *dv, double talp, const double *src
         EvScope ev(c, "ytdma");
         const float *p32 = c->mode == CHS_MIXED ? c->pp32 : (const float *)nullptr;
         const double *p00 = gauge ? (const double *)c->pp00 : (const double *)nullptr;
-        const int C = ytdma_chunk_rows(c->tune.ytdma_c, M);
+        const int C = ytdma_chunk_rows(c->tune.ytdma_c, M, H * N);
         if (C > 0) {
             const dim3 grid((H * N + 63) / 64), blk(64, (M + C - 1) / C);
             switch (C) {
@@ -4110,7 +4119,9 @@ bool ens_batchable(const chs_ens *e)
         for (int ns = 1; ns <= 3; ++ns) {
             const CnTab &t = c->cn[ns];
             if (t.fullz || t.Dz > 8 || t.Dy[0] > 8 || t.Dy[1] != t.Dy[0] || t.Dy[2] != t.Dy[0]) return false;
-            if (c->M > 256 || c->nref_step != 0 || c->tune.cny3_ky < 0 || c->tune.ytdma_c == 0) return false;
+            if (c->M > 256 || c->nref_step != 0 || c->tune.cny3_ky < 0 ||
+                ytdma_chunk_rows(c->tune.ytdma_c, c->M, c->H * c->N) == 0)
+                return false;
             if (c->tune.cny3_ky > 0 && c->tune.cny3_ky <= t.Dy[0]) return false;
         }
     return true;
@@ -4211,9 +4222,9 @@ int ens_make_launches(chs_ens *e, const double *dt, const double *cvisc, EnsLaun
             afzi.push_back({g, pa, spec, NL, +1});
             gfz.push_back(dim3((M * H + NL - 1) / NL));
             shz = std::max(shz, sizeof(cpx<W>) * (N + NL * (N + 1)));
-            const int P = (M + 15) / 16;
+            const int C = ytdma_chunk_rows(c->tune.ytdma_c, M, H * N), P = (M + C - 1) / C;
             Pmax = std::max(Pmax, P);
-            ayt.push_back({g, c->d_amph, c->d_apph, c->pp, mixed ? c->pp32 : (const float *)nullptr, spec, P});
+            ayt.push_back({g, c->d_amph, c->d_apph, c->pp, mixed ? c->pp32 : (const float *)nullptr, spec, P, C});
             gyt.push_back(dim3((H * N + 63) / 64));
             afxi.push_back({g, pa, (const cpx<W> *)spec, c->phi, TK});
             const dim3 gu((L + 63) / 64, (M + 3) / 4, (N + KU - 1) / KU);
