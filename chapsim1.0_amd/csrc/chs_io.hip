@@ -185,6 +185,59 @@ __device__ double block_sum256(double v, double *sh)
     return sh[0];
 }
 
+// Fields the momentum stencil reads: G (the previous mass flux, G0_io), Q, the staggered
+// viscosities MU_STG (x-y, x-z, y-z), the viscosity (Viscousity0 for the explicit scheme,
+// the current Viscousity for Crank-Nicolson), the pressure, and the explicit scheme's
+// divergence (a cell field, rows 1..M).
+enum IoField { FG0, FG1, FG2, FQ0, FQ1, FQ2, FMS0, FMS1, FMS2, FM0, FM1, FPR, FDU };
+
+// Mesh coefficients of row J the stencil reads: DYFI(J), DYFI(J-1), DYCI(J), DYCI(J+1),
+// YCL2ND_WFF/WFB(J) and (J+1).
+enum IoRowC { RDYFI, RDYFIM, RDYCI, RDYCIP, RWFF, RWFB, RWFFP, RWFBP, IO_NROW };
+
+// ac(f, di, dj, dk) = field f at (i + di, J + dj, k + dk), x and z periodic, read from HBM.
+struct IoAtHbm {
+    const IoDev &d;
+    const double *gs;
+    int i, J, k, ip, im, kp, km;
+    __device__ __forceinline__ double operator()(int f, int di, int dj, int dk) const
+    {
+        const int ii = di > 0 ? ip : (di < 0 ? im : i), kk = dk > 0 ? kp : (dk < 0 ? km : k), jj = J + dj;
+        const size_t hs = hsz(d), o = hx(d, ii, jj, kk);
+        switch (f) {
+        case FG0: return gs[o];
+        case FG1: return gs[hs + o];
+        case FG2: return gs[2 * hs + o];
+        case FQ0: return d.q[o];
+        case FQ1: return d.q[hs + o];
+        case FQ2: return d.q[2 * hs + o];
+        case FMS0: return d.mstg[o];
+        case FMS1: return d.mstg[hs + o];
+        case FMS2: return d.mstg[2 * hs + o];
+        case FM0: return d.m0[o];
+        case FM1: return d.m1[o];
+        case FPR: return d.pr[o];
+        default: return d.divu[cx(d, ii, jj, kk)];
+        }
+    }
+    __device__ __forceinline__ double rw(int c) const
+    {
+        switch (c) {
+        case RDYFI: return d.dyfi[J];
+        case RDYFIM: return d.dyfi[J - 1];
+        case RDYCI: return d.dyci[J];
+        case RDYCIP: return d.dyci[J + 1];
+        case RWFF: return d.wff[J];
+        case RWFB: return d.wfb[J];
+        case RWFFP: return d.wff[J + 1];
+        default: return d.wfb[J + 1];
+        }
+    }
+    // EXPLT0 of component c and the staggered density of component c at the point
+    __device__ __forceinline__ double ex(int c) const { return d.explt0[(size_t)c * csz(d) + cx(d, i, J, k)]; }
+    __device__ __forceinline__ double ds(int c) const { return d.dstg[(size_t)c * hsz(d) + hx(d, i, J, k)]; }
+};
+
 // One momentum component C (0 x, 1 y, 2 z) at a point: convection (CONVECTION_X/Y/Z_io),
 // the viscous terms and RHS_MOM_EXPLICIT_io(NS, C+1): RK combination, EXPLT0 update, pressure
 // gradient, buoyancy.  Explicit scheme (CN false): the full variable-mu stress divergence
@@ -196,239 +249,227 @@ __device__ double block_sum256(double v, double *sh)
 // multiply DRHOI_STG, which DENSITY_IMPLICIT_add evaluates as 1/D_STG minus the reciprocal of
 // D_STG's own expression of the same DENSITY0 (MASSFLUX_CALC_IO.f90:332-416): exactly zero.
 // Rows 1..M (2..M for v).  Returns RHS / DYFI for C = 0 (the mass-flux partial), 0 otherwise.
-template <int C, bool CN>
-__device__ __forceinline__ double io_mom_point(const IoDev &d, const double *__restrict__ gs, int i, int J, int k,
-                                               double tg, double tr, double ta, double dt)
+template <int C, bool CN, typename Acc>
+__device__ __forceinline__ double io_mom_point(const IoDev &d, const Acc &ac, int i, int J, int k, double tg,
+                                               double tr, double ta, double dt)
 {
 #pragma clang fp contract(off)
     {
-        const int L = d.L, N = d.N;
-        const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
-        const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
-        const size_t hs = hsz(d);
-        const double *G0 = gs, *G1 = gs + hs, *G2 = gs + 2 * hs;
-        const double *Q0 = d.q, *Q1 = d.q + hs, *Q2 = d.q + 2 * hs;
-        const double *V = d.m0, *DU = d.divu;
         const double hf = 0.5, cv = d.cvisc, dxi = d.dxi, dzi = d.dzi;
-        const double wffp = d.wff[J + 1], wfbp = d.wfb[J + 1], wff = d.wff[J], wfb = d.wfb[J];
-        auto H = [&](int ii, int jj, int kk) { return hx(d, ii, jj, kk); };
-        auto CI = [&](int ii, int jj, int kk) { return cx(d, ii, jj, kk); };
+        const double wffp = ac.rw(RWFFP), wfbp = ac.rw(RWFBP), wff = ac.rw(RWFF), wfb = ac.rw(RWFB);
         double acc, pgm, imp = 0.0;
         (void)imp;
         if constexpr (C == 0) {
-            const double c1 = dxi * hf * hf, c2 = d.dyfi[J] * hf, c3 = dzi * hf * hf;
-            double f = (G0[H(ip, J, k)] + G0[H(i, J, k)]) * (Q0[H(ip, J, k)] + Q0[H(i, J, k)]);
-            double b = (G0[H(i, J, k)] + G0[H(im, J, k)]) * (Q0[H(i, J, k)] + Q0[H(im, J, k)]);
+            const double c1 = dxi * hf * hf, c2 = ac.rw(RDYFI) * hf, c3 = dzi * hf * hf;
+            double f = (ac(FG0, 1, 0, 0) + ac(FG0, 0, 0, 0)) * (ac(FQ0, 1, 0, 0) + ac(FQ0, 0, 0, 0));
+            double b = (ac(FG0, 0, 0, 0) + ac(FG0, -1, 0, 0)) * (ac(FQ0, 0, 0, 0) + ac(FQ0, -1, 0, 0));
             const double h11 = (f - b) * c1;
-            f = (G1[H(i, J + 1, k)] + G1[H(im, J + 1, k)]) * (wffp * Q0[H(i, J + 1, k)] + wfbp * Q0[H(i, J, k)]);
-            b = (G1[H(i, J, k)] + G1[H(im, J, k)]) * (wff * Q0[H(i, J, k)] + wfb * Q0[H(i, J - 1, k)]);
+            f = (ac(FG1, 0, 1, 0) + ac(FG1, -1, 1, 0)) * (wffp * ac(FQ0, 0, 1, 0) + wfbp * ac(FQ0, 0, 0, 0));
+            b = (ac(FG1, 0, 0, 0) + ac(FG1, -1, 0, 0)) * (wff * ac(FQ0, 0, 0, 0) + wfb * ac(FQ0, 0, -1, 0));
             const double h12 = (f - b) * c2;
-            f = (G2[H(i, J, kp)] + G2[H(im, J, kp)]) * (Q0[H(i, J, kp)] + Q0[H(i, J, k)]);
-            b = (G2[H(i, J, k)] + G2[H(im, J, k)]) * (Q0[H(i, J, k)] + Q0[H(i, J, km)]);
+            f = (ac(FG2, 0, 0, 1) + ac(FG2, -1, 0, 1)) * (ac(FQ0, 0, 0, 1) + ac(FQ0, 0, 0, 0));
+            b = (ac(FG2, 0, 0, 0) + ac(FG2, -1, 0, 0)) * (ac(FQ0, 0, 0, 0) + ac(FQ0, 0, 0, -1));
             const double h13 = (f - b) * c3;
             acc = -(h11 + h12 + h13);
             if constexpr (CN) {
-                const double *V1 = d.m1;
-                const double e1 = -2.0 / 3.0 * dxi * cv, e2 = d.dyfi[J] * cv, e3 = dzi * cv;
-                double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
-                double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
-                double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
-                const double t11fe = V1[H(i, J, k)] * (dvdy + dwdz), t11fi = V1[H(i, J, k)] * dudx;
-                dudx = (Q0[H(i, J, k)] - Q0[H(im, J, k)]) * dxi;
-                dvdy = (Q1[H(im, J + 1, k)] - Q1[H(im, J, k)]) * d.dyfi[J];
-                dwdz = (Q2[H(im, J, kp)] - Q2[H(im, J, k)]) * dzi;
-                const double t11be = V1[H(im, J, k)] * (dvdy + dwdz), t11bi = V1[H(im, J, k)] * dudx;
+                const double e1 = -2.0 / 3.0 * dxi * cv, e2 = ac.rw(RDYFI) * cv, e3 = dzi * cv;
+                double dudx = (ac(FQ0, 1, 0, 0) - ac(FQ0, 0, 0, 0)) * dxi;
+                double dvdy = (ac(FQ1, 0, 1, 0) - ac(FQ1, 0, 0, 0)) * ac.rw(RDYFI);
+                double dwdz = (ac(FQ2, 0, 0, 1) - ac(FQ2, 0, 0, 0)) * dzi;
+                const double t11fe = ac(FM1, 0, 0, 0) * (dvdy + dwdz), t11fi = ac(FM1, 0, 0, 0) * dudx;
+                dudx = (ac(FQ0, 0, 0, 0) - ac(FQ0, -1, 0, 0)) * dxi;
+                dvdy = (ac(FQ1, -1, 1, 0) - ac(FQ1, -1, 0, 0)) * ac.rw(RDYFI);
+                dwdz = (ac(FQ2, -1, 0, 1) - ac(FQ2, -1, 0, 0)) * dzi;
+                const double t11be = ac(FM1, -1, 0, 0) * (dvdy + dwdz), t11bi = ac(FM1, -1, 0, 0) * dudx;
                 const double d11e = (t11fe - t11be) * e1, d11i = (t11fi - t11bi) * e1 * (-2.0);
-                double vis = d.mstg[H(i, J + 1, k)];
-                double dvdx = (Q1[H(i, J + 1, k)] - Q1[H(im, J + 1, k)]) * dxi;
-                double dudy = (Q0[H(i, J + 1, k)] - Q0[H(i, J, k)]) * d.dyci[J + 1];
+                double vis = ac(FMS0, 0, 1, 0);
+                double dvdx = (ac(FQ1, 0, 1, 0) - ac(FQ1, -1, 1, 0)) * dxi;
+                double dudy = (ac(FQ0, 0, 1, 0) - ac(FQ0, 0, 0, 0)) * ac.rw(RDYCIP);
                 const double t12fe = vis * dvdx, t12fi = vis * dudy;
-                vis = d.mstg[H(i, J, k)];
-                dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
-                dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
+                vis = ac(FMS0, 0, 0, 0);
+                dudy = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, -1, 0)) * ac.rw(RDYCI);
+                dvdx = (ac(FQ1, 0, 0, 0) - ac(FQ1, -1, 0, 0)) * dxi;
                 const double t12be = vis * dvdx, t12bi = vis * dudy;
                 const double d12e = (t12fe - t12be) * e2, d12i = (t12fi - t12bi) * e2;
-                vis = d.mstg[hs + H(i, J, kp)];
-                double dudz = (Q0[H(i, J, kp)] - Q0[H(i, J, k)]) * dzi;
-                double dwdx = (Q2[H(i, J, kp)] - Q2[H(im, J, kp)]) * dxi;
+                vis = ac(FMS1, 0, 0, 1);
+                double dudz = (ac(FQ0, 0, 0, 1) - ac(FQ0, 0, 0, 0)) * dzi;
+                double dwdx = (ac(FQ2, 0, 0, 1) - ac(FQ2, -1, 0, 1)) * dxi;
                 const double t13fe = vis * dwdx, t13fi = vis * dudz;
-                vis = d.mstg[hs + H(i, J, k)];
-                dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
-                dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
+                vis = ac(FMS1, 0, 0, 0);
+                dudz = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, 0, -1)) * dzi;
+                dwdx = (ac(FQ2, 0, 0, 0) - ac(FQ2, -1, 0, 0)) * dxi;
                 const double t13be = vis * dwdx, t13bi = vis * dudz;
                 const double d13e = (t13fe - t13be) * e3, d13i = (t13fi - t13bi) * e3;
                 acc = d11e + d12e + d13e + acc;
                 imp = d11i + d12i + d13i;
             } else {
-            const double v1 = 2.0 * dxi * cv, v2 = d.dyfi[J] * cv, v3 = dzi * cv;
-            double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
-            const double t11f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dudx - DU[CI(i, J, k)]);
-            dudx = (Q0[H(i, J, k)] - Q0[H(im, J, k)]) * dxi;
-            const double t11b = 0.5 * (V[H(im, J, k)] + V[H(im, J, k)]) * (dudx - DU[CI(im, J, k)]);
+            const double v1 = 2.0 * dxi * cv, v2 = ac.rw(RDYFI) * cv, v3 = dzi * cv;
+            double dudx = (ac(FQ0, 1, 0, 0) - ac(FQ0, 0, 0, 0)) * dxi;
+            const double t11f = 0.5 * (ac(FM0, 0, 0, 0) + ac(FM0, 0, 0, 0)) * (dudx - ac(FDU, 0, 0, 0));
+            dudx = (ac(FQ0, 0, 0, 0) - ac(FQ0, -1, 0, 0)) * dxi;
+            const double t11b = 0.5 * (ac(FM0, -1, 0, 0) + ac(FM0, -1, 0, 0)) * (dudx - ac(FDU, -1, 0, 0));
             const double d11 = (t11f - t11b) * v1;
-            double dudy = (Q0[H(i, J + 1, k)] - Q0[H(i, J, k)]) * d.dyci[J + 1];
-            double dvdx = (Q1[H(i, J + 1, k)] - Q1[H(im, J + 1, k)]) * dxi;
-            const double t12f = d.mstg[H(i, J + 1, k)] * (dudy + dvdx);
-            dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
-            dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
-            const double t12b = d.mstg[H(i, J, k)] * (dudy + dvdx);
+            double dudy = (ac(FQ0, 0, 1, 0) - ac(FQ0, 0, 0, 0)) * ac.rw(RDYCIP);
+            double dvdx = (ac(FQ1, 0, 1, 0) - ac(FQ1, -1, 1, 0)) * dxi;
+            const double t12f = ac(FMS0, 0, 1, 0) * (dudy + dvdx);
+            dudy = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, -1, 0)) * ac.rw(RDYCI);
+            dvdx = (ac(FQ1, 0, 0, 0) - ac(FQ1, -1, 0, 0)) * dxi;
+            const double t12b = ac(FMS0, 0, 0, 0) * (dudy + dvdx);
             const double d12 = (t12f - t12b) * v2;
-            double dudz = (Q0[H(i, J, kp)] - Q0[H(i, J, k)]) * dzi;
-            double dwdx = (Q2[H(i, J, kp)] - Q2[H(im, J, kp)]) * dxi;
-            const double t13f = d.mstg[hs + H(i, J, kp)] * (dudz + dwdx);
-            dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
-            dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
-            const double t13b = d.mstg[hs + H(i, J, k)] * (dudz + dwdx);
+            double dudz = (ac(FQ0, 0, 0, 1) - ac(FQ0, 0, 0, 0)) * dzi;
+            double dwdx = (ac(FQ2, 0, 0, 1) - ac(FQ2, -1, 0, 1)) * dxi;
+            const double t13f = ac(FMS1, 0, 0, 1) * (dudz + dwdx);
+            dudz = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, 0, -1)) * dzi;
+            dwdx = (ac(FQ2, 0, 0, 0) - ac(FQ2, -1, 0, 0)) * dxi;
+            const double t13b = ac(FMS1, 0, 0, 0) * (dudz + dwdx);
             const double d13 = (t13f - t13b) * v3;
             acc = acc + d11 + d12 + d13;
             }
-            pgm = (d.pr[H(i, J, k)] - d.pr[H(im, J, k)]) * (ta * dt * dxi);
+            pgm = (ac(FPR, 0, 0, 0) - ac(FPR, -1, 0, 0)) * (ta * dt * dxi);
         } else if constexpr (C == 1) {
-            const double c1 = dxi * hf, c2 = d.dyci[J] * hf * hf, c3 = dzi * hf;
-            double f = (wff * G0[H(ip, J, k)] + wfb * G0[H(ip, J - 1, k)]) * (Q1[H(ip, J, k)] + Q1[H(i, J, k)]);
-            double b = (wff * G0[H(i, J, k)] + wfb * G0[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(im, J, k)]);
+            const double c1 = dxi * hf, c2 = ac.rw(RDYCI) * hf * hf, c3 = dzi * hf;
+            double f = (wff * ac(FG0, 1, 0, 0) + wfb * ac(FG0, 1, -1, 0)) * (ac(FQ1, 1, 0, 0) + ac(FQ1, 0, 0, 0));
+            double b = (wff * ac(FG0, 0, 0, 0) + wfb * ac(FG0, 0, -1, 0)) * (ac(FQ1, 0, 0, 0) + ac(FQ1, -1, 0, 0));
             const double h21 = (f - b) * c1;
-            f = (G1[H(i, J + 1, k)] + G1[H(i, J, k)]) * (Q1[H(i, J + 1, k)] + Q1[H(i, J, k)]);
-            b = (G1[H(i, J, k)] + G1[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(i, J - 1, k)]);
+            f = (ac(FG1, 0, 1, 0) + ac(FG1, 0, 0, 0)) * (ac(FQ1, 0, 1, 0) + ac(FQ1, 0, 0, 0));
+            b = (ac(FG1, 0, 0, 0) + ac(FG1, 0, -1, 0)) * (ac(FQ1, 0, 0, 0) + ac(FQ1, 0, -1, 0));
             const double h22 = (f - b) * c2;
-            f = (wff * G2[H(i, J, kp)] + wfb * G2[H(i, J - 1, kp)]) * (Q1[H(i, J, kp)] + Q1[H(i, J, k)]);
-            b = (wff * G2[H(i, J, k)] + wfb * G2[H(i, J - 1, k)]) * (Q1[H(i, J, k)] + Q1[H(i, J, km)]);
+            f = (wff * ac(FG2, 0, 0, 1) + wfb * ac(FG2, 0, -1, 1)) * (ac(FQ1, 0, 0, 1) + ac(FQ1, 0, 0, 0));
+            b = (wff * ac(FG2, 0, 0, 0) + wfb * ac(FG2, 0, -1, 0)) * (ac(FQ1, 0, 0, 0) + ac(FQ1, 0, 0, -1));
             const double h23 = (f - b) * c3;
             acc = -(h21 + h22 + h23);
             if constexpr (CN) {
-                const double *V1 = d.m1;
-                const double e1 = dxi * cv, e3 = dzi * cv, e2 = -2.0 / 3.0 * d.dyci[J] * cv;
-                double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
-                double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
-                double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
-                const double t22fe = V1[H(i, J, k)] * (dudx + dwdz), t22fi = V1[H(i, J, k)] * dvdy;
-                dudx = (Q0[H(ip, J - 1, k)] - Q0[H(i, J - 1, k)]) * dxi;
-                dvdy = (Q1[H(i, J, k)] - Q1[H(i, J - 1, k)]) * d.dyfi[J - 1];
-                dwdz = (Q2[H(i, J - 1, kp)] - Q2[H(i, J - 1, k)]) * dzi;
-                const double t22be = V1[H(i, J - 1, k)] * (dudx + dwdz), t22bi = V1[H(i, J - 1, k)] * dvdy;
+                const double e1 = dxi * cv, e3 = dzi * cv, e2 = -2.0 / 3.0 * ac.rw(RDYCI) * cv;
+                double dudx = (ac(FQ0, 1, 0, 0) - ac(FQ0, 0, 0, 0)) * dxi;
+                double dvdy = (ac(FQ1, 0, 1, 0) - ac(FQ1, 0, 0, 0)) * ac.rw(RDYFI);
+                double dwdz = (ac(FQ2, 0, 0, 1) - ac(FQ2, 0, 0, 0)) * dzi;
+                const double t22fe = ac(FM1, 0, 0, 0) * (dudx + dwdz), t22fi = ac(FM1, 0, 0, 0) * dvdy;
+                dudx = (ac(FQ0, 1, -1, 0) - ac(FQ0, 0, -1, 0)) * dxi;
+                dvdy = (ac(FQ1, 0, 0, 0) - ac(FQ1, 0, -1, 0)) * ac.rw(RDYFIM);
+                dwdz = (ac(FQ2, 0, -1, 1) - ac(FQ2, 0, -1, 0)) * dzi;
+                const double t22be = ac(FM1, 0, -1, 0) * (dudx + dwdz), t22bi = ac(FM1, 0, -1, 0) * dvdy;
                 const double d22e = (t22fe - t22be) * e2, d22i = (t22fi - t22bi) * e2 * (-2.0);
-                double vis = d.mstg[H(ip, J, k)];
-                double dvdx = (Q1[H(ip, J, k)] - Q1[H(i, J, k)]) * dxi;
-                double dudy = (Q0[H(ip, J, k)] - Q0[H(ip, J - 1, k)]) * d.dyci[J];
+                double vis = ac(FMS0, 1, 0, 0);
+                double dvdx = (ac(FQ1, 1, 0, 0) - ac(FQ1, 0, 0, 0)) * dxi;
+                double dudy = (ac(FQ0, 1, 0, 0) - ac(FQ0, 1, -1, 0)) * ac.rw(RDYCI);
                 const double t21fe = vis * dudy, t21fi = vis * dvdx;
-                vis = d.mstg[H(i, J, k)];
-                dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
-                dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
+                vis = ac(FMS0, 0, 0, 0);
+                dvdx = (ac(FQ1, 0, 0, 0) - ac(FQ1, -1, 0, 0)) * dxi;
+                dudy = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, -1, 0)) * ac.rw(RDYCI);
                 const double t21be = vis * dudy, t21bi = vis * dvdx;
                 const double d21e = (t21fe - t21be) * e1, d21i = (t21fi - t21bi) * e1;
-                vis = d.mstg[2 * hs + H(i, J, kp)];
-                double dvdz = (Q1[H(i, J, kp)] - Q1[H(i, J, k)]) * dzi;
-                double dwdy = (Q2[H(i, J, kp)] - Q2[H(i, J - 1, kp)]) * d.dyci[J];
+                vis = ac(FMS2, 0, 0, 1);
+                double dvdz = (ac(FQ1, 0, 0, 1) - ac(FQ1, 0, 0, 0)) * dzi;
+                double dwdy = (ac(FQ2, 0, 0, 1) - ac(FQ2, 0, -1, 1)) * ac.rw(RDYCI);
                 const double t23fe = vis * dwdy, t23fi = vis * dvdz;
-                vis = d.mstg[2 * hs + H(i, J, k)];
-                dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
-                dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
+                vis = ac(FMS2, 0, 0, 0);
+                dvdz = (ac(FQ1, 0, 0, 0) - ac(FQ1, 0, 0, -1)) * dzi;
+                dwdy = (ac(FQ2, 0, 0, 0) - ac(FQ2, 0, -1, 0)) * ac.rw(RDYCI);
                 const double t23be = vis * dwdy, t23bi = vis * dvdz;
                 const double d23e = (t23fe - t23be) * e3, d23i = (t23fi - t23bi) * e3;
                 acc = d21e + d22e + d23e + acc;
                 imp = d21i + d22i + d23i;
             } else {
-            const double v1 = dxi * cv, v2 = 2.0 * d.dyci[J] * cv, v3 = dzi * cv;
-            double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
-            const double t22f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dvdy - DU[CI(i, J, k)]);
-            dvdy = (Q1[H(i, J, k)] - Q1[H(i, J - 1, k)]) * d.dyfi[J - 1];
-            const double t22b = 0.5 * (V[H(i, J - 1, k)] + V[H(i, J - 1, k)]) * (dvdy - DU[CI(i, J - 1, k)]);
+            const double v1 = dxi * cv, v2 = 2.0 * ac.rw(RDYCI) * cv, v3 = dzi * cv;
+            double dvdy = (ac(FQ1, 0, 1, 0) - ac(FQ1, 0, 0, 0)) * ac.rw(RDYFI);
+            const double t22f = 0.5 * (ac(FM0, 0, 0, 0) + ac(FM0, 0, 0, 0)) * (dvdy - ac(FDU, 0, 0, 0));
+            dvdy = (ac(FQ1, 0, 0, 0) - ac(FQ1, 0, -1, 0)) * ac.rw(RDYFIM);
+            const double t22b = 0.5 * (ac(FM0, 0, -1, 0) + ac(FM0, 0, -1, 0)) * (dvdy - ac(FDU, 0, -1, 0));
             const double d22 = (t22f - t22b) * v2;
-            double dvdx = (Q1[H(ip, J, k)] - Q1[H(i, J, k)]) * dxi;
-            double dudy = (Q0[H(ip, J, k)] - Q0[H(ip, J - 1, k)]) * d.dyci[J];
-            const double t21f = d.mstg[H(ip, J, k)] * (dvdx + dudy);
-            dvdx = (Q1[H(i, J, k)] - Q1[H(im, J, k)]) * dxi;
-            dudy = (Q0[H(i, J, k)] - Q0[H(i, J - 1, k)]) * d.dyci[J];
-            const double t21b = d.mstg[H(i, J, k)] * (dvdx + dudy);
+            double dvdx = (ac(FQ1, 1, 0, 0) - ac(FQ1, 0, 0, 0)) * dxi;
+            double dudy = (ac(FQ0, 1, 0, 0) - ac(FQ0, 1, -1, 0)) * ac.rw(RDYCI);
+            const double t21f = ac(FMS0, 1, 0, 0) * (dvdx + dudy);
+            dvdx = (ac(FQ1, 0, 0, 0) - ac(FQ1, -1, 0, 0)) * dxi;
+            dudy = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, -1, 0)) * ac.rw(RDYCI);
+            const double t21b = ac(FMS0, 0, 0, 0) * (dvdx + dudy);
             const double d21 = (t21f - t21b) * v1;
-            double dvdz = (Q1[H(i, J, kp)] - Q1[H(i, J, k)]) * dzi;
-            double dwdy = (Q2[H(i, J, kp)] - Q2[H(i, J - 1, kp)]) * d.dyci[J];
-            const double t23f = d.mstg[2 * hs + H(i, J, kp)] * (dvdz + dwdy);
-            dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
-            dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
-            const double t23b = d.mstg[2 * hs + H(i, J, k)] * (dvdz + dwdy);
+            double dvdz = (ac(FQ1, 0, 0, 1) - ac(FQ1, 0, 0, 0)) * dzi;
+            double dwdy = (ac(FQ2, 0, 0, 1) - ac(FQ2, 0, -1, 1)) * ac.rw(RDYCI);
+            const double t23f = ac(FMS2, 0, 0, 1) * (dvdz + dwdy);
+            dvdz = (ac(FQ1, 0, 0, 0) - ac(FQ1, 0, 0, -1)) * dzi;
+            dwdy = (ac(FQ2, 0, 0, 0) - ac(FQ2, 0, -1, 0)) * ac.rw(RDYCI);
+            const double t23b = ac(FMS2, 0, 0, 0) * (dvdz + dwdy);
             const double d23 = (t23f - t23b) * v3;
             acc = acc + d21 + d22 + d23;
             }
-            pgm = (d.pr[H(i, J, k)] - d.pr[H(i, J - 1, k)]) * (ta * dt * d.dyci[J]);
+            pgm = (ac(FPR, 0, 0, 0) - ac(FPR, 0, -1, 0)) * (ta * dt * ac.rw(RDYCI));
         } else {
-            const double c1 = dxi * hf * hf, c2 = d.dyfi[J] * hf, c3 = dzi * hf * hf;
-            double f = (G0[H(ip, J, k)] + G0[H(ip, J, km)]) * (Q2[H(ip, J, k)] + Q2[H(i, J, k)]);
-            double b = (G0[H(i, J, k)] + G0[H(i, J, km)]) * (Q2[H(i, J, k)] + Q2[H(im, J, k)]);
+            const double c1 = dxi * hf * hf, c2 = ac.rw(RDYFI) * hf, c3 = dzi * hf * hf;
+            double f = (ac(FG0, 1, 0, 0) + ac(FG0, 1, 0, -1)) * (ac(FQ2, 1, 0, 0) + ac(FQ2, 0, 0, 0));
+            double b = (ac(FG0, 0, 0, 0) + ac(FG0, 0, 0, -1)) * (ac(FQ2, 0, 0, 0) + ac(FQ2, -1, 0, 0));
             const double h31 = (f - b) * c1;
-            f = (G1[H(i, J + 1, k)] + G1[H(i, J + 1, km)]) * (wffp * Q2[H(i, J + 1, k)] + wfbp * Q2[H(i, J, k)]);
-            b = (G1[H(i, J, k)] + G1[H(i, J, km)]) * (wff * Q2[H(i, J, k)] + wfb * Q2[H(i, J - 1, k)]);
+            f = (ac(FG1, 0, 1, 0) + ac(FG1, 0, 1, -1)) * (wffp * ac(FQ2, 0, 1, 0) + wfbp * ac(FQ2, 0, 0, 0));
+            b = (ac(FG1, 0, 0, 0) + ac(FG1, 0, 0, -1)) * (wff * ac(FQ2, 0, 0, 0) + wfb * ac(FQ2, 0, -1, 0));
             const double h32 = (f - b) * c2;
-            f = (G2[H(i, J, kp)] + G2[H(i, J, k)]) * (Q2[H(i, J, kp)] + Q2[H(i, J, k)]);
-            b = (G2[H(i, J, k)] + G2[H(i, J, km)]) * (Q2[H(i, J, k)] + Q2[H(i, J, km)]);
+            f = (ac(FG2, 0, 0, 1) + ac(FG2, 0, 0, 0)) * (ac(FQ2, 0, 0, 1) + ac(FQ2, 0, 0, 0));
+            b = (ac(FG2, 0, 0, 0) + ac(FG2, 0, 0, -1)) * (ac(FQ2, 0, 0, 0) + ac(FQ2, 0, 0, -1));
             const double h33 = (f - b) * c3;
             acc = -(h31 + h32 + h33);
             if constexpr (CN) {
-                const double *V1 = d.m1;
-                const double e1 = dxi * cv, e3 = -2.0 / 3.0 * dzi * cv, e2 = d.dyfi[J] * cv;
-                double dudx = (Q0[H(ip, J, k)] - Q0[H(i, J, k)]) * dxi;
-                double dvdy = (Q1[H(i, J + 1, k)] - Q1[H(i, J, k)]) * d.dyfi[J];
-                double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
-                const double t33fe = V1[H(i, J, k)] * (dudx + dvdy), t33fi = V1[H(i, J, k)] * dwdz;
-                dudx = (Q0[H(ip, J, km)] - Q0[H(i, J, km)]) * dxi;
-                dvdy = (Q1[H(i, J + 1, km)] - Q1[H(i, J, km)]) * d.dyfi[J];
-                dwdz = (Q2[H(i, J, k)] - Q2[H(i, J, km)]) * dzi;
-                const double t33be = V1[H(i, J, km)] * (dudx + dvdy), t33bi = V1[H(i, J, km)] * dwdz;
+                const double e1 = dxi * cv, e3 = -2.0 / 3.0 * dzi * cv, e2 = ac.rw(RDYFI) * cv;
+                double dudx = (ac(FQ0, 1, 0, 0) - ac(FQ0, 0, 0, 0)) * dxi;
+                double dvdy = (ac(FQ1, 0, 1, 0) - ac(FQ1, 0, 0, 0)) * ac.rw(RDYFI);
+                double dwdz = (ac(FQ2, 0, 0, 1) - ac(FQ2, 0, 0, 0)) * dzi;
+                const double t33fe = ac(FM1, 0, 0, 0) * (dudx + dvdy), t33fi = ac(FM1, 0, 0, 0) * dwdz;
+                dudx = (ac(FQ0, 1, 0, -1) - ac(FQ0, 0, 0, -1)) * dxi;
+                dvdy = (ac(FQ1, 0, 1, -1) - ac(FQ1, 0, 0, -1)) * ac.rw(RDYFI);
+                dwdz = (ac(FQ2, 0, 0, 0) - ac(FQ2, 0, 0, -1)) * dzi;
+                const double t33be = ac(FM1, 0, 0, -1) * (dudx + dvdy), t33bi = ac(FM1, 0, 0, -1) * dwdz;
                 const double d33e = (t33fe - t33be) * e3, d33i = (t33fi - t33bi) * e3 * (-2.0);
-                double vis = d.mstg[hs + H(ip, J, k)];
-                double dwdx = (Q2[H(ip, J, k)] - Q2[H(i, J, k)]) * dxi;
-                double dudz = (Q0[H(ip, J, k)] - Q0[H(ip, J, km)]) * dzi;
+                double vis = ac(FMS1, 1, 0, 0);
+                double dwdx = (ac(FQ2, 1, 0, 0) - ac(FQ2, 0, 0, 0)) * dxi;
+                double dudz = (ac(FQ0, 1, 0, 0) - ac(FQ0, 1, 0, -1)) * dzi;
                 const double t31fe = vis * dudz, t31fi = vis * dwdx;
-                vis = d.mstg[hs + H(i, J, k)];
-                dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
-                dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
+                vis = ac(FMS1, 0, 0, 0);
+                dwdx = (ac(FQ2, 0, 0, 0) - ac(FQ2, -1, 0, 0)) * dxi;
+                dudz = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, 0, -1)) * dzi;
                 const double t31be = vis * dudz, t31bi = vis * dwdx;
                 const double d31e = (t31fe - t31be) * e1, d31i = (t31fi - t31bi) * e1;
-                vis = d.mstg[2 * hs + H(i, J + 1, k)];
-                double dwdy = (Q2[H(i, J + 1, k)] - Q2[H(i, J, k)]) * d.dyci[J + 1];
-                double dvdz = (Q1[H(i, J + 1, k)] - Q1[H(i, J + 1, km)]) * dzi;
+                vis = ac(FMS2, 0, 1, 0);
+                double dwdy = (ac(FQ2, 0, 1, 0) - ac(FQ2, 0, 0, 0)) * ac.rw(RDYCIP);
+                double dvdz = (ac(FQ1, 0, 1, 0) - ac(FQ1, 0, 1, -1)) * dzi;
                 const double t32fe = vis * dvdz, t32fi = vis * dwdy;
-                vis = d.mstg[2 * hs + H(i, J, k)];
-                dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
-                dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
+                vis = ac(FMS2, 0, 0, 0);
+                dwdy = (ac(FQ2, 0, 0, 0) - ac(FQ2, 0, -1, 0)) * ac.rw(RDYCI);
+                dvdz = (ac(FQ1, 0, 0, 0) - ac(FQ1, 0, 0, -1)) * dzi;
                 const double t32be = vis * dvdz, t32bi = vis * dwdy;
                 const double d32e = (t32fe - t32be) * e2, d32i = (t32fi - t32bi) * e2;
                 acc = d31e + d32e + d33e + acc;
                 imp = d31i + d32i + d33i;
             } else {
-            const double v1 = dxi * cv, v2 = d.dyfi[J] * cv, v3 = 2.0 * dzi * cv;
-            double dwdz = (Q2[H(i, J, kp)] - Q2[H(i, J, k)]) * dzi;
-            const double t33f = 0.5 * (V[H(i, J, k)] + V[H(i, J, k)]) * (dwdz - DU[CI(i, J, k)]);
-            dwdz = (Q2[H(i, J, k)] - Q2[H(i, J, km)]) * dzi;
-            const double t33b = 0.5 * (V[H(i, J, km)] + V[H(i, J, km)]) * (dwdz - DU[CI(i, J, km)]);
+            const double v1 = dxi * cv, v2 = ac.rw(RDYFI) * cv, v3 = 2.0 * dzi * cv;
+            double dwdz = (ac(FQ2, 0, 0, 1) - ac(FQ2, 0, 0, 0)) * dzi;
+            const double t33f = 0.5 * (ac(FM0, 0, 0, 0) + ac(FM0, 0, 0, 0)) * (dwdz - ac(FDU, 0, 0, 0));
+            dwdz = (ac(FQ2, 0, 0, 0) - ac(FQ2, 0, 0, -1)) * dzi;
+            const double t33b = 0.5 * (ac(FM0, 0, 0, -1) + ac(FM0, 0, 0, -1)) * (dwdz - ac(FDU, 0, 0, -1));
             const double d33 = (t33f - t33b) * v3;
-            double dwdx = (Q2[H(ip, J, k)] - Q2[H(i, J, k)]) * dxi;
-            double dudz = (Q0[H(ip, J, k)] - Q0[H(ip, J, km)]) * dzi;
-            const double t31f = d.mstg[hs + H(ip, J, k)] * (dwdx + dudz);
-            dwdx = (Q2[H(i, J, k)] - Q2[H(im, J, k)]) * dxi;
-            dudz = (Q0[H(i, J, k)] - Q0[H(i, J, km)]) * dzi;
-            const double t31b = d.mstg[hs + H(i, J, k)] * (dwdx + dudz);
+            double dwdx = (ac(FQ2, 1, 0, 0) - ac(FQ2, 0, 0, 0)) * dxi;
+            double dudz = (ac(FQ0, 1, 0, 0) - ac(FQ0, 1, 0, -1)) * dzi;
+            const double t31f = ac(FMS1, 1, 0, 0) * (dwdx + dudz);
+            dwdx = (ac(FQ2, 0, 0, 0) - ac(FQ2, -1, 0, 0)) * dxi;
+            dudz = (ac(FQ0, 0, 0, 0) - ac(FQ0, 0, 0, -1)) * dzi;
+            const double t31b = ac(FMS1, 0, 0, 0) * (dwdx + dudz);
             const double d31 = (t31f - t31b) * v1;
-            double dwdy = (Q2[H(i, J + 1, k)] - Q2[H(i, J, k)]) * d.dyci[J + 1];
-            double dvdz = (Q1[H(i, J + 1, k)] - Q1[H(i, J + 1, km)]) * dzi;
-            const double t32f = d.mstg[2 * hs + H(i, J + 1, k)] * (dwdy + dvdz);
-            dwdy = (Q2[H(i, J, k)] - Q2[H(i, J - 1, k)]) * d.dyci[J];
-            dvdz = (Q1[H(i, J, k)] - Q1[H(i, J, km)]) * dzi;
-            const double t32b = d.mstg[2 * hs + H(i, J, k)] * (dwdy + dvdz);
+            double dwdy = (ac(FQ2, 0, 1, 0) - ac(FQ2, 0, 0, 0)) * ac.rw(RDYCIP);
+            double dvdz = (ac(FQ1, 0, 1, 0) - ac(FQ1, 0, 1, -1)) * dzi;
+            const double t32f = ac(FMS2, 0, 1, 0) * (dwdy + dvdz);
+            dwdy = (ac(FQ2, 0, 0, 0) - ac(FQ2, 0, -1, 0)) * ac.rw(RDYCI);
+            dvdz = (ac(FQ1, 0, 0, 0) - ac(FQ1, 0, 0, -1)) * dzi;
+            const double t32b = ac(FMS2, 0, 0, 0) * (dwdy + dvdz);
             const double d32 = (t32f - t32b) * v2;
             acc = acc + d31 + d32 + d33;
             }
-            pgm = (d.pr[H(i, J, k)] - d.pr[H(i, J, km)]) * (ta * dt * dzi);
+            pgm = (ac(FPR, 0, 0, 0) - ac(FPR, 0, 0, -1)) * (ta * dt * dzi);
         }
-        const size_t o = (size_t)C * csz(d) + CI(i, J, k);
+        const size_t o = (size_t)C * csz(d) + cx(d, i, J, k);
         double r;
-        if constexpr (CN) r = (tg * acc + tr * d.explt0[o]) * dt + (ta * dt) * imp;
-        else r = (tg * acc + tr * d.explt0[o]) * dt;
+        if constexpr (CN) r = (tg * acc + tr * ac.ex(C)) * dt + (ta * dt) * imp;
+        else r = (tg * acc + tr * ac.ex(C)) * dt;
         d.explt0[o] = acc;
         r = r - pgm;
-        if (d.gravity == C + 1) r = r + d.dstg[C * hs + H(i, J, k)] * (ta * dt * d.f_a);
+        if (d.gravity == C + 1) r = r + ac.ds(C) * (ta * dt * d.f_a);
         d.h[o] = r;
-        return C == 0 ? r / d.dyfi[J] : 0.0;
+        return C == 0 ? r / ac.rw(RDYFI) : 0.0;
     }
 }
 
@@ -448,13 +489,220 @@ __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restric
     const int L = d.L, M = d.M, N = d.N;
     double contrib = 0.0;
     if (i < L && J <= M && k < N) {
-        contrib = io_mom_point<0, CN>(d, gs, i, J, k, tg, tr, ta, dt);
-        if (J >= 2) io_mom_point<1, CN>(d, gs, i, J, k, tg, tr, ta, dt);
-        io_mom_point<2, CN>(d, gs, i, J, k, tg, tr, ta, dt);
+        const IoAtHbm ac{d, gs, i, J, k, i == L - 1 ? 0 : i + 1, i == 0 ? L - 1 : i - 1, k == N - 1 ? 0 : k + 1,
+                         k == 0 ? N - 1 : k - 1};
+        contrib = io_mom_point<0, CN>(d, ac, i, J, k, tg, tr, ta, dt);
+        if (J >= 2) io_mom_point<1, CN>(d, ac, i, J, k, tg, tr, ta, dt);
+        io_mom_point<2, CN>(d, ac, i, J, k, tg, tr, ta, dt);
     }
     const double s = block_sum256(contrib, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0)
         d.partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// The same three components from LDS, marching z.  A block of 64 x BY threads owns columns
+// i0..i0+63 and rows J0..J0+BY-1 and marches KZ z-planes.  Each field the stencil reads is
+// staged per plane as a (BY + 2) x 66 tile (a halo row and column each side, x periodic) in a
+// ring of two plane slots, planes k and k+1; the plane-(k-1) values the stencil reads stay in
+// registers from the previous plane.  So a value is loaded from HBM once per block and plane
+// (halo cells by the neighbouring blocks as well) rather than once per stencil use, and the
+// stencil's ~110 loads of a point become LDS reads.  Plane k+2 is loaded into registers while
+// plane k is computed and stored into plane k's slot after it.  Identical arithmetic to
+// k_io_mom; the block's partial sums its points of all its planes.
+template <bool CN> struct IoStage {
+    static constexpr int NF = CN ? 11 : 12;   // staged fields: G, Q, MU_STG (3 each), viscosity, p (, div)
+    static constexpr int NC = CN ? 12 : 13;   // plane-(k-1) values carried in registers
+    // staged slot of field f
+    static __device__ __forceinline__ constexpr int slot(int f)
+    {
+        return f <= FMS2 ? f : (f == FPR ? 10 : (f == FDU ? 11 : 9));
+    }
+    static __device__ __forceinline__ constexpr int field(int s)
+    {
+        return s <= FMS2 ? s : (s == 9 ? (CN ? FM1 : FM0) : (s == 10 ? FPR : FDU));
+    }
+    // register of the carried value (f, di, dj) at plane k-1, -1 if it is not carried
+    static __device__ __forceinline__ constexpr int carry(int f, int di, int dj)
+    {
+        return (f == FQ0 && di == 0 && dj == 0) ? 0 : (f == FQ0 && di == 1 && dj == 0) ? 1
+             : (f == FQ1 && di == 0 && dj == 0) ? 2 : (f == FQ1 && di == 0 && dj == 1) ? 3
+             : (f == FQ2 && di == 0 && dj == 0) ? 4 : (f == FG0 && di == 0 && dj == 0) ? 5
+             : (f == FG0 && di == 1 && dj == 0) ? 6 : (f == FG1 && di == 0 && dj == 0) ? 7
+             : (f == FG1 && di == 0 && dj == 1) ? 8 : (f == FG2 && di == 0 && dj == 0) ? 9
+             : (f == FPR && di == 0 && dj == 0) ? 10 : (f == (CN ? FM1 : FM0) && di == 0 && dj == 0) ? 11
+             : (!CN && f == FDU && di == 0 && dj == 0) ? 12 : -1;
+    }
+    static __device__ __forceinline__ constexpr int carry_f(int c)
+    {
+        return c <= 1 ? FQ0 : c <= 3 ? FQ1 : c == 4 ? FQ2 : c <= 6 ? FG0 : c <= 8 ? FG1 : c == 9 ? FG2
+             : c == 10 ? FPR : c == 11 ? (CN ? FM1 : FM0) : FDU;
+    }
+    static __device__ __forceinline__ constexpr int carry_di(int c) { return (c == 1 || c == 6) ? 1 : 0; }
+    static __device__ __forceinline__ constexpr int carry_dj(int c) { return (c == 3 || c == 8) ? 1 : 0; }
+};
+
+template <bool CN, int BY> struct IoAtLds {
+    static constexpr int TW = 66, TS = (BY + 2) * TW;
+    const double *c0, *c1;   // this point's cell in the tiles of planes k and k+1
+    const double *cr;        // the carried plane-(k-1) values
+    const double *xr;        // EXPLT0 of the three components and the buoyancy component's D_STG
+    const double *rr;        // the row's mesh coefficients (IoRowC), loaded once
+    IoAtHbm hb;              // any other plane-(k-1) value (the stencil reads none)
+    __device__ __forceinline__ double rw(int c) const { return rr[c]; }
+    __device__ __forceinline__ double ex(int c) const { return xr[c]; }
+    __device__ __forceinline__ double ds(int) const { return xr[3]; }
+    __device__ __forceinline__ double operator()(int f, int di, int dj, int dk) const
+    {
+        if (dk < 0) {
+            const int c = IoStage<CN>::carry(f, di, dj);
+            return c >= 0 ? cr[c] : hb(f, di, dj, dk);
+        }
+        return (dk > 0 ? c1 : c0)[IoStage<CN>::slot(f) * TS + dj * TW + di];
+    }
+};
+
+// Tile of launch slot p of nb: the hardware deals slots to the XCDs in turn (slot p to XCD
+// p mod nxcd); each XCD gets a contiguous run of tiles, so that the halo rows a tile shares
+// with the tiles above and below it are read through the same L2.
+__device__ __forceinline__ int io_xcd_tile(int p, int nb, int nxcd)
+{
+    if (nxcd <= 1) return p;
+    const int xcd = p % nxcd, slot = p / nxcd, per = nb / nxcd, rem = nb % nxcd;
+    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+template <bool CN, int BY>
+__global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *__restrict__ gs, double tg, double tr,
+                                                         double ta, double dt, int KZ, int nxcd)
+{
+    using S = IoStage<CN>;
+    constexpr int NT = 64 * BY, TW = 66, TS = (BY + 2) * TW, NF = S::NF, NCL = (TS + NT - 1) / NT;
+    __shared__ double tiles[2][NF * TS];
+    __shared__ double sh[NT];
+    const int L = d.L, M = d.M, N = d.N;
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), t = ty * 64 + tx;
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int tile = io_xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z, nxcd);
+    const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
+    const int i0 = bx * 64, J0 = 1 + by * BY, k0 = bz * KZ, k1 = min(N, k0 + KZ);
+    const int i = i0 + tx, J = J0 + ty;
+    const bool act = i < L && J <= M;
+    const size_t hs = hsz(d), ph = (size_t)(M + 2) * L, pc = (size_t)M * L;
+    // in-plane offsets of the tile cells this thread stages; a row past the field's last (or
+    // before a cell field's first) reads the nearest row instead, which no point of the block
+    // uses, so that every staging load is unconditional
+    unsigned oh[NCL], oc[NCL];
+#pragma unroll
+    for (int s = 0; s < NCL; ++s) {
+        const int n = t + s * NT, r = n / TW, c = n % TW, jj = J0 - 1 + r;
+        int ii = i0 - 1 + c;
+        ii = ii < 0 ? L - 1 : ii % L;
+        oh[s] = (unsigned)(min(jj, M + 1) * L + ii);
+        oc[s] = (unsigned)((min(max(jj, 1), M) - 1) * L + ii);
+    }
+    auto base = [&](int f) -> const double * {
+        switch (f) {
+        case FG0: return gs;
+        case FG1: return gs + hs;
+        case FG2: return gs + 2 * hs;
+        case FQ0: return d.q;
+        case FQ1: return d.q + hs;
+        case FQ2: return d.q + 2 * hs;
+        case FMS0: return d.mstg;
+        case FMS1: return d.mstg + hs;
+        case FMS2: return d.mstg + 2 * hs;
+        case FM0: return d.m0;
+        case FM1: return d.m1;
+        case FPR: return d.pr;
+        default: return d.divu;
+        }
+    };
+    double pre[NF][NCL];
+    auto load = [&](int kk) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const int fld = S::field(f);
+            const bool cell = fld == FDU;
+            const double *b = base(fld) + (size_t)kk * (cell ? pc : ph);
+#pragma unroll
+            for (int s = 0; s < NCL; ++s) pre[f][s] = b[cell ? oc[s] : oh[s]];
+        }
+    };
+    auto store = [&](int sl) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int s = 0; s < NCL; ++s) {
+                const int n = t + s * NT;
+                if (n < TS) tiles[sl][f * TS + n] = pre[f][s];
+            }
+    };
+    const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
+    double cr[S::NC];
+    {
+        const int km = k0 == 0 ? N - 1 : k0 - 1;
+        const IoAtHbm hb{d, gs, act ? i : 0, act ? J : 1, km, act ? ip : 0, act ? im : 0, km, km};
+#pragma unroll
+        for (int c = 0; c < S::NC; ++c) cr[c] = hb(S::carry_f(c), S::carry_di(c), S::carry_dj(c), 0);
+    }
+    // EXPLT0 of the three components and the buoyancy component's D_STG at this point, loaded
+    // a plane ahead and before that plane's tiles: the vector-memory counter retires loads in
+    // issue order, so a load issued after the tile loads would make the plane wait for them
+    const int gc = d.gravity - 1;
+    double xr[4] = {0.0, 0.0, 0.0, 0.0}, xn[4] = {0.0, 0.0, 0.0, 0.0};
+    auto loadx = [&](int kk, double (&x)[4]) {
+        if (act) {
+            const size_t o = cx(d, i, J, kk);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[c] = d.explt0[(size_t)c * pc * N + o];
+            if (gc >= 0 && gc <= 2) x[3] = d.dstg[(size_t)gc * hs + hx(d, i, J, kk)];
+        }
+    };
+    double rr[IO_NROW];
+    {
+        const int Jr = min(J, M);   // rows past M (no points) take row M's
+        const IoAtHbm hb{d, gs, 0, Jr, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < IO_NROW; ++c) rr[c] = (c == RDYFIM && Jr < 2) ? 0.0 : hb.rw(c);
+    }
+    loadx(k0, xr);
+    load(k0);
+    store(0);
+    load(k0 + 1 == N ? 0 : k0 + 1);
+    store(1);
+    __syncthreads();
+    const int cell = (ty + 1) * TW + tx + 1;
+    double contrib = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const int cur = (k - k0) & 1;
+        const bool more = k + 1 < k1;
+        if (more) {
+            loadx(k + 1, xn);
+            load(k + 2 >= N ? k + 2 - N : k + 2);
+        }
+        if (act) {
+            const IoAtLds<CN, BY> ac{&tiles[cur][cell], &tiles[cur ^ 1][cell], cr, xr, rr,
+                                     IoAtHbm{d, gs, i, J, k, ip, im, k == N - 1 ? 0 : k + 1, k == 0 ? N - 1 : k - 1}};
+            contrib = contrib + io_mom_point<0, CN>(d, ac, i, J, k, tg, tr, ta, dt);
+            if (J >= 2) io_mom_point<1, CN>(d, ac, i, J, k, tg, tr, ta, dt);
+            io_mom_point<2, CN>(d, ac, i, J, k, tg, tr, ta, dt);
+        }
+#pragma unroll
+        for (int c = 0; c < S::NC; ++c)
+            cr[c] = tiles[cur][S::slot(S::carry_f(c)) * TS + cell + S::carry_dj(c) * TW + S::carry_di(c)];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) xr[c] = xn[c];
+        __syncthreads();
+        if (more) store(cur);
+        __syncthreads();
+    }
+    sh[t] = contrib;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    if (t == 0) d.partial[tile] = sh[0];
 }
 
 // DPGRNS = sum of the partials / VOLM_io, one block, fixed order.
@@ -1221,6 +1469,9 @@ struct chs_io {
     double *buf = nullptr;
     int npart = 0, flow_driven = 1;
     int mom_rows = 4;   // rows per momentum block (4, 2 or 1; the rest of its 4 rows of threads are z-planes)
+    int mom_kz = 16;    // z-planes per block of the LDS-staged momentum kernel (0: the per-point kernel)
+    int mom_by = 4;     // rows per block of the LDS-staged momentum kernel (4 or 8)
+    int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
     int cn_cols = 16;   // most x-columns per block of the z and y Jacobi line solves (16: 1.7 %
                         // faster per heated step than 64 at 256 x 128 x 256)
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
@@ -1434,6 +1685,21 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
         m->mom_rows = value;
         return CHS_OK;
     }
+    if (k == "mom_kz") {
+        if (value < 0 || value > 1024) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: mom_kz must be 0..1024");
+        m->mom_kz = value;
+        return CHS_OK;
+    }
+    if (k == "mom_nxcd") {
+        if (value < 1 || value > 64) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: mom_nxcd must be 1..64");
+        m->mom_nxcd = value;
+        return CHS_OK;
+    }
+    if (k == "mom_by") {
+        if (value != 4 && value != 8) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: mom_by must be 4 or 8");
+        m->mom_by = value;
+        return CHS_OK;
+    }
     if (k == "cn_cols") {
         if (value < 1 || value > 64) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: cn_cols must be 1..64");
         m->cn_cols = value;
@@ -1454,13 +1720,20 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
     if (!m->q_fresh) hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
     m->q_fresh = false;
-    const int TY = m->mom_rows, TZ = 4 / TY;
-    const dim3 gmom((m->L + 63) / 64, (m->M + TY - 1) / TY, (m->N + TZ - 1) / TZ);
-    const int npart = (int)(gmom.x * gmom.y * gmom.z);
-    if (m->cn) {
+    const int TY = m->mom_rows, TZ = 4 / TY, KZ = m->mom_kz, BY = m->mom_by, NX = m->mom_nxcd;
+    const dim3 gmom = KZ > 0 ? dim3((m->L + 63) / 64, (m->M + BY - 1) / BY, (m->N + KZ - 1) / KZ)
+                             : dim3((m->L + 63) / 64, (m->M + TY - 1) / TY, (m->N + TZ - 1) / TZ);
+    const int npart = (int)(gmom.x * gmom.y * gmom.z);   // at most the 64 x 4 row-block count
+    if (!m->cn) hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
+    if (KZ > 0) {
+        const dim3 bz(64, BY);
+        if (m->cn && BY == 4) hipLaunchKernelGGL((k_io_mom_z<true, 4>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        else if (m->cn) hipLaunchKernelGGL((k_io_mom_z<true, 8>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        else if (BY == 4) hipLaunchKernelGGL((k_io_mom_z<false, 4>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        else hipLaunchKernelGGL((k_io_mom_z<false, 8>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+    } else if (m->cn) {
         hipLaunchKernelGGL(k_io_mom<true>, gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
     } else {
-        hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
         hipLaunchKernelGGL(k_io_mom<false>, gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
     }
     const double *dpg_dev = nullptr;

@@ -172,15 +172,19 @@ def test_gpu_cn_substeps_match_reference(pkg, golden):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("L,M,N,steps,rows,cols", [(64, 48, 32, 3, 4, 16), (48, 36, 40, 3, 4, 16),
-                                                   (128, 64, 96, 1, 4, 16), (64, 48, 32, 1, 2, 64),
-                                                   (48, 37, 30, 1, 2, 8), (48, 37, 30, 1, 1, 64),
-                                                   (128, 64, 96, 1, 4, 64)])
-def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows, cols):
+@pytest.mark.parametrize("L,M,N,steps,mom,cols", [(64, 48, 32, 3, {}, 16), (48, 36, 40, 3, {}, 16),
+                                                  (128, 64, 96, 1, dict(mom_by=8, mom_kz=5), 16),
+                                                  (64, 48, 32, 1, dict(mom_kz=0, mom_rows=2), 64),
+                                                  (48, 37, 30, 1, dict(mom_kz=0, mom_rows=2), 8),
+                                                  (48, 37, 30, 1, dict(mom_kz=0, mom_rows=1), 64),
+                                                  (48, 37, 30, 1, dict(mom_by=8, mom_kz=7), 16),
+                                                  (128, 64, 96, 1, dict(mom_kz=0, mom_rows=4), 64)])
+def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, mom, cols):
     """Larger seeded states with a noisy current viscosity against the oracle (gravity and
-    constant mass flux on), with each momentum block shape (rows per block; ragged M, N) and
-    several line-solve block widths (x-columns per block)."""
-    print(f"io cn oracle {L}x{M}x{N} rows {rows} cols {cols}", flush=True)
+    constant mass flux on), with each momentum kernel and block shape (the LDS-staged z-march,
+    mom_kz planes of mom_by rows, or the per-point kernel, mom_kz = 0, with mom_rows rows;
+    ragged M, N) and several line-solve block widths (x-columns per block)."""
+    print(f"io cn oracle {L}x{M}x{N} mom {mom} cols {cols}", flush=True)
     mesh = setup.Mesh(L, M, N, STR2=2.0)
     st = oracle.io_initial_field(mesh, 90 + L)
     m1 = cn_viscosity(st, 90 + L, False)
@@ -188,7 +192,7 @@ def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, rows, cols):
     o = oracle.OracleIO(mesh, REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
     o.set_state(dict(st, M1=m1))
     ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), REN, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=1)
-    ch.set_tuning(mom_rows=rows, cn_cols=cols)
+    ch.set_tuning(cn_cols=cols, **mom)
     ch.upload(**st, M1=m1)
     for _ in range(steps):
         for ns in (1, 2, 3):
@@ -392,7 +396,7 @@ def test_gpu_explicit_momentum_block_shapes(pkg, rows):
     for r in (4, rows):
         ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=1, f_a=0.02, flow_driven=1,
                            vis_scheme=0)
-        ch.set_tuning(mom_rows=r)
+        ch.set_tuning(mom_rows=r, mom_kz=0)
         ch.upload(**st)
         for ns in (1, 2, 3):
             ch.ioprops(ns)
@@ -404,6 +408,45 @@ def test_gpu_explicit_momentum_block_shapes(pkg, rows):
             assert rel(out[1][k][c], out[0][k][c]) < 1e-13, (k, c)
     assert rel(out[1]["PR"], out[0]["PR"]) < 1e-11
     ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0)
-    with pytest.raises(pkg.ChsError):
-        ch.set_tuning(mom_rows=3)
+    for bad in (dict(mom_rows=3), dict(mom_by=2), dict(mom_kz=-1), dict(mom_nxcd=0)):
+        with pytest.raises(pkg.ChsError):
+            ch.set_tuning(**bad)
     ch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vis", [1, 0])
+def test_gpu_momentum_lds_march_equals_per_point_kernel(pkg, vis):
+    """The LDS-staged z-march momentum kernel (mom_kz > 0) against the per-point kernel
+    (mom_kz = 0) under a constant Cf, where the mass-flux partials are not used: bit-identical
+    substeps (the same expressions from the same values), for both block heights, march
+    lengths 1, 3, 5, 8, N and beyond N, tiles grouped for 8, 3 or 1 XCDs, and ragged meshes (L below or not a multiple of 64, M not a
+    multiple of the block height, N not a multiple of the march length).  With a constant mass
+    flux only the partials' summation order differs."""
+    for L, M, N in [(48, 37, 30), (80, 21, 10), (16, 9, 6)]:
+        print(f"io momentum kernels {L}x{M}x{N} vis {vis}", flush=True)
+        mesh = setup.Mesh(L, M, N, STR2=2.0)
+        st = oracle.io_initial_field(mesh, 55 + L)
+        if vis == 1:
+            st = dict(st, M1=cn_viscosity(st, 55 + L, False))
+        for fd in (2, 1):
+            out = []
+            for mom in (dict(mom_kz=0), dict(mom_kz=8), dict(mom_kz=3, mom_by=8), dict(mom_kz=1),
+                        dict(mom_kz=N, mom_by=8), dict(mom_kz=64), dict(mom_kz=8, mom_nxcd=1),
+                        dict(mom_kz=5, mom_nxcd=3)):
+                ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=2, f_a=0.02,
+                                   flow_driven=fd, cf_given=8.18e-3, vis_scheme=vis)
+                ch.set_tuning(**mom)
+                ch.upload(**st)
+                for ns in (1, 2, 3):
+                    ch.ioprops(ns)
+                    ch.SOLVERRK3_MOM_io(ns, 5e-4)
+                out.append((mom, ch.download()))
+                ch.close()
+            ref = out[0][1]
+            for mom, f in out[1:]:
+                for k, v in ref.items():
+                    if fd == 2:
+                        assert np.array_equal(f[k], v), (mom, fd, k)
+                    else:
+                        assert rel(f[k], v) < 1e-11, (mom, fd, k, rel(f[k], v))
