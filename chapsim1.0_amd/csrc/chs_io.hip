@@ -969,6 +969,88 @@ __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double fa
     if (valid) d.h[C * cs + cx(d, e, J, k)] = x;
 }
 
+// MOMFA1_X_io for L = 64 P (P = 1..8): one wavefront per x-line, lane l holding points
+// l P .. l P + P - 1; the neighbours across lanes come by cross-lane shuffles, so a line needs
+// no block barrier and a wavefront's loads are all issued together.  Coefficients and sweeps
+// as io_cn_x_jac (rho is the line's own maximum).
+template <int C, int P>
+__device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double facoe)
+{
+#pragma clang fp contract(off)
+    const int M = d.M, j0 = C == 1 ? 2 : 1, nj = M - j0 + 1, nl = nj * d.N;
+    const int lane = threadIdx.x & 63;
+    const int line = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (line >= nl) return;   // the whole wavefront
+    const int J = j0 + line % nj, k = line / nj;
+    const size_t hs = hsz(d), cs = csz(d);
+    const int lm = (lane + 63) & 63, lp = (lane + 1) & 63;
+    const double *F0 = (C == 0 ? d.m1 : d.mstg + (C - 1) * hs) + hx(d, lane * P, J, k);
+    const double *F1 = d.dstg + C * hs + hx(d, lane * P, J, k);
+    double *R = d.h + C * cs + cx(d, lane * P, J, k);
+    double f0[P], f1[P], r[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        f0[j] = F0[j];
+        f1[j] = F1[j];
+        r[j] = R[j];
+        if constexpr (C == 0) r[j] = r[j] - dp;
+    }
+    const double f0m = __shfl(f0[P - 1], lm), f0p = __shfl(f0[0], lp);
+    const double f1m = __shfl(f1[P - 1], lm), f1p = __shfl(f1[0], lp);
+    double a[P], c[P], ib[P], rho = 0.0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const double vm = j > 0 ? f0[j - 1] : f0m, vp = j < P - 1 ? f0[j + 1] : f0p;
+        const double dm = j > 0 ? f1[j - 1] : f1m, dq = j < P - 1 ? f1[j + 1] : f1p;
+        double amu, apu;
+        if constexpr (C == 0) {
+            const double coe1 = facoe * 4.0 / 3.0 * d.dxqi;
+            amu = -coe1 * vm;
+            apu = -coe1 * f0[j];
+        } else {
+            const double coe2 = facoe * d.dxqi;
+            amu = -coe2 * f0[j];
+            apu = -coe2 * vp;
+        }
+        a[j] = amu / dm;
+        c[j] = apu / dq;
+        const double b = 1.0 - (amu + apu) / f1[j];
+        ib[j] = 1.0 / b;
+        rho = nanmax(rho, (fabs(a[j]) + fabs(c[j])) * fabs(ib[j]));
+    }
+    for (int o = 32; o > 0; o >>= 1) rho = nanmax(rho, __shfl_xor(rho, o));
+    if (!(rho <= 0.25)) {
+        if (lane == 0) io_cn_x_line<C>(d, line, dp, facoe);
+        return;
+    }
+    const int K = jacobi_sweeps(rho);
+    double x[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) x[j] = r[j] * ib[j];
+    for (int it = 0; it < K; ++it) {
+        const double xl = __shfl(x[P - 1], lm), xr = __shfl(x[0], lp);
+        double xn[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const double xm = j > 0 ? x[j - 1] : xl, xp = j < P - 1 ? x[j + 1] : xr;
+            xn[j] = (r[j] - a[j] * xm - c[j] * xp) * ib[j];
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) x[j] = xn[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) R[j] = x[j];
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_io_cn_x_wave(IoDev d, const double *__restrict__ dpg_dev, double dpg,
+                                                      double facoe)
+{
+    if (blockIdx.y == 0) io_cn_x_wave<0, P>(d, dpg_dev ? dpg_dev[0] : dpg, facoe);
+    else if (blockIdx.y == 1) io_cn_x_wave<1, P>(d, 0.0, facoe);
+    else io_cn_x_wave<2, P>(d, 0.0, facoe);
+}
+
 __global__ __launch_bounds__(512) void k_io_cn_x_jac(IoDev d, const double *__restrict__ dpg_dev, double dpg,
                                                      double facoe)
 {
@@ -1472,6 +1554,7 @@ struct chs_io {
     int mom_kz = 16;    // z-planes per block of the LDS-staged momentum kernel (0: the per-point kernel)
     int mom_by = 4;     // rows per block of the LDS-staged momentum kernel (4 or 8)
     int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
+    int cnx_wave = 1;   // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
     int cn_cols = 16;   // most x-columns per block of the z and y Jacobi line solves (16: 1.7 %
                         // faster per heated step than 64 at 256 x 128 x 256)
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
@@ -1700,6 +1783,11 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
         m->mom_by = value;
         return CHS_OK;
     }
+    if (k == "cnx_wave") {
+        if (value != 0 && value != 1) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: cnx_wave must be 0 or 1");
+        m->cnx_wave = value;
+        return CHS_OK;
+    }
     if (k == "cn_cols") {
         if (value < 1 || value > 64) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: cn_cols must be 1..64");
         m->cn_cols = value;
@@ -1749,7 +1837,14 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
         // here (each reads Q, the properties and its own increment), so they run side by side.
         const double facoe = 0.5 * ta * dt * d.cvisc;
         const int L = m->L, M = m->M, N = m->N;
-        if ((L & (L - 1)) == 0 && L >= 4 && L <= 512) {
+        const int PX = L / 64;   // x-line points per lane of the wavefront-per-line solve
+        if (m->cnx_wave && L % 64 == 0 && (PX == 1 || PX == 2 || PX == 4 || PX == 8)) {
+            const dim3 gx((M * N + 3) / 4, 3);
+            if (PX == 1) hipLaunchKernelGGL(k_io_cn_x_wave<1>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else if (PX == 2) hipLaunchKernelGGL(k_io_cn_x_wave<2>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else if (PX == 4) hipLaunchKernelGGL(k_io_cn_x_wave<4>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else hipLaunchKernelGGL(k_io_cn_x_wave<8>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+        } else if ((L & (L - 1)) == 0 && L >= 4 && L <= 512) {
             const int T = L >= 256 ? 1 : 256 / L;   // x-lines per block
             hipLaunchKernelGGL(k_io_cn_x_jac, dim3((M * N + T - 1) / T, 3), dim3(T * L),
                                sizeof(double) * (2 * T * L + 1), m->stream, d, dpg_dev, dpg, facoe);

@@ -44,6 +44,12 @@ struct PropDev {
     int n;
     SplineDev dhh, ht, hd, hm, hk, hcp, hb;
     BucketDev bdh, bh;   // over LIST_DH (dhh.x) and LIST_H (ht.x and the other H splines)
+    // The same coefficients as records, one per table index, for the property update's
+    // evaluation (a cell's reads of one index then come in a few 16-byte loads):
+    // recd[8 i] = LIST_DH(i), then H's y, b, c, d (2 doubles of padding);
+    // rech[24 i] = LIST_H(i), then y, b, c, d of the H->T, H->M, H->K, H->D, H->Cp splines
+    // (3 of padding).
+    const double *recd, *rech;
 };
 
 // The bisection of spline_interpolation_* (I = 1, J = N+1, thermo_properties_table_searching.f90:
@@ -273,6 +279,14 @@ __device__ __forceinline__ double spl_at(const SplineDev &s, int I, double xv)
     return s.y[I - 1] + dx * (s.b[I - 1] + dx * (s.c[I - 1] + dx * s.d[I - 1]));
 }
 
+// spl_at from a record's x and y, b, c, d (the same expression).
+__device__ __forceinline__ double spl_rec(double x, double y, double b, double c, double d, double xv)
+{
+#pragma clang fp contract(off)
+    const double dx = xv - x;
+    return y + dx * (b + dx * (c + dx * d));
+}
+
 // spl_index_b for C values in lockstep: the bisection steps are branch-free (a finished search
 // re-reads its own entry and keeps its bounds), so the C searches' loads go out together.
 // Two lockstep steps cover a bucket's index range unless the table clusters; the rest, if
@@ -341,7 +355,14 @@ __device__ void eng_cells_props(const EngDev &e, const PropDev &p, const size_t 
     }
     spl_index_lock<C>(p.bdh, xdh, p.n, dv, I);
 #pragma unroll
-    for (int c = 0; c < C; ++c) hv[c] = act[c] ? spl_at(p.dhh, I[c], dv[c]) : p.ht.x[0];
+    for (int c = 0; c < C; ++c) {
+        hv[c] = 0.0;   // an inactive cell searches from any value
+        if (act[c]) {
+            const double2 *r = reinterpret_cast<const double2 *>(p.recd + (size_t)8 * (I[c] - 1));
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+            hv[c] = spl_rec(r0.x, r0.y, r1.x, r1.y, r2.x, dv[c]);
+        }
+    }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         if (!act[c]) continue;
@@ -353,12 +374,17 @@ __device__ void eng_cells_props(const EngDev &e, const PropDev &p, const size_t 
     for (int c = 0; c < C; ++c) {
         if (!act[c]) continue;
         const size_t o = h0[c];
-        e.h[o] = hv[c];
-        e.t[o] = spl_at(p.ht, I[c], hv[c]);
-        e.d[o] = spl_at(p.hm, I[c], hv[c]);    // D from H->M, as the reference
-        e.m[o] = spl_at(p.hk, I[c], hv[c]);    // M from H->K
-        e.k[o] = spl_at(p.hd, I[c], hv[c]);    // K from H->D
-        e.cp[o] = spl_at(p.hcp, I[c], hv[c]);
+        const double2 *r = reinterpret_cast<const double2 *>(p.rech + (size_t)24 * (I[c] - 1));
+        double2 v[11];
+#pragma unroll
+        for (int q = 0; q < 11; ++q) v[q] = r[q];
+        const double x = v[0].x, h = hv[c];
+        e.h[o] = h;
+        e.t[o] = spl_rec(x, v[0].y, v[1].x, v[1].y, v[2].x, h);
+        e.d[o] = spl_rec(x, v[2].y, v[3].x, v[3].y, v[4].x, h);    // D from H->M, as the reference
+        e.m[o] = spl_rec(x, v[4].y, v[5].x, v[5].y, v[6].x, h);    // M from H->K
+        e.k[o] = spl_rec(x, v[6].y, v[7].x, v[7].y, v[8].x, h);    // K from H->D
+        e.cp[o] = spl_rec(x, v[8].y, v[9].x, v[9].y, v[10].x, h);
     }
 }
 
@@ -432,6 +458,7 @@ struct chs_prop {
     int n = 0;
     double *dev = nullptr;   // all table arrays, packed
     int *buckets = nullptr;  // bucket indexes of LIST_DH and LIST_H
+    double *recs = nullptr;  // PropDev::recd and rech
     int *err = nullptr;
     PropDev p;
     hipStream_t stream = nullptr;
@@ -783,6 +810,30 @@ int chs_prop_create(const chs_prop_table *t, chs_prop **out)
     }
     p->p.bdh.lohi = p->buckets;
     p->p.bh.lohi = p->buckets + 2 * (size_t)nb;
+    // evaluation records (PropDev::recd, rech)
+    std::vector<double> rec((size_t)32 * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double *rd = rec.data() + (size_t)8 * i, *rh = rec.data() + (size_t)8 * n + (size_t)24 * i;
+        const double vd[5] = {t->dh[i], t->h[i], t->dhh_b[i], t->dhh_c[i], t->dhh_d[i]};
+        for (int q = 0; q < 5; ++q) rd[q] = vd[q];
+        rh[0] = t->h[i];
+        const double *hs[5][4] = {{t->t, t->ht_b, t->ht_c, t->ht_d}, {t->m, t->hm_b, t->hm_c, t->hm_d},
+                                  {t->k, t->hk_b, t->hk_c, t->hk_d}, {t->d, t->hd_b, t->hd_c, t->hd_d},
+                                  {t->cp, t->hcp_b, t->hcp_c, t->hcp_d}};
+        for (int sp = 0; sp < 5; ++sp)
+            for (int q = 0; q < 4; ++q) rh[1 + 4 * sp + q] = hs[sp][q][i];
+    }
+    e = hipMalloc(&p->recs, sizeof(double) * rec.size());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->recs, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) {
+        g_prop_err = std::string("chs_prop_create: ") + hipGetErrorString(e);
+        chs_prop_destroy(p);
+        return CHS_E_HIP;
+    }
+    p->p.recd = p->recs;
+    p->p.rech = p->recs + (size_t)8 * n;
     *out = p;
     return CHS_OK;
 }
@@ -793,6 +844,7 @@ void chs_prop_destroy(chs_prop *p)
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->dev) (void)hipFree(p->dev);
     if (p->buckets) (void)hipFree(p->buckets);
+    if (p->recs) (void)hipFree(p->recs);
     if (p->err) (void)hipFree(p->err);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;

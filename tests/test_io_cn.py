@@ -450,3 +450,33 @@ def test_gpu_momentum_lds_march_equals_per_point_kernel(pkg, vis):
                         assert np.array_equal(f[k], v), (mom, fd, k)
                     else:
                         assert rel(f[k], v) < 1e-11, (mom, fd, k, rel(f[k], v))
+
+
+@pytest.mark.gpu
+def test_gpu_cn_x_wave_equals_block_solve(pkg):
+    """The wavefront-per-line Crank-Nicolson x-solve (cnx_wave = 1, lane l holding L/64
+    consecutive points) against the block-per-line one (cnx_wave = 0): at L = 256 both take
+    the same per-line sweep count and give the same substeps bit for bit; at L = 64, 128, 512
+    the block kernel's count comes from several lines (or L = 512 from one 512-thread line) and
+    the results agree to rounding."""
+    for L, M, N in [(256, 12, 8), (128, 10, 6), (64, 9, 6), (512, 6, 4)]:
+        print(f"io cn x wave {L}x{M}x{N}", flush=True)
+        mesh = setup.Mesh(L, M, N, STR2=2.0)
+        st = oracle.io_initial_field(mesh, 31 + L)
+        st = dict(st, M1=cn_viscosity(st, 31 + L, False))
+        out = []
+        for w in (0, 1):
+            ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=1, f_a=0.02, flow_driven=1,
+                               vis_scheme=1)
+            ch.set_tuning(cnx_wave=w)
+            ch.upload(**st)
+            for ns in (1, 2, 3):
+                ch.ioprops(ns)
+                ch.SOLVERRK3_MOM_io(ns, 5e-4)
+            out.append(ch.download())
+            ch.close()
+        for k, v in out[0].items():
+            if L == 256:
+                assert np.array_equal(out[1][k], v), (L, k)
+            else:
+                assert rel(out[1][k], v) < 1e-13, (L, k, rel(out[1][k], v))
