@@ -461,8 +461,10 @@ def bench_heated(pkg, args):
     """Quasi-incompressible heated channel on the main domain (config 5): per RK3 stage the
     energy substep (table properties, iso-flux walls), the staggered properties and the
     variable-property momentum substep, Crank-Nicolson viscous scheme (iVisScheme = 1; --vis
-    explicit for iVisScheme = 0), in SOLVE's order (chs_heated_substep).  Plain FP64.
-    value = grid-point RK3-steps/s."""
+    explicit for iVisScheme = 0), in SOLVE's order (chs_heated_substep).  --form both (default):
+    the plain FP64 step and the mixed form (EXPLT0 and the momentum increment in FP32,
+    chs_io_set_mixed), each from the same start, with the mixed fields' error against the
+    plain ones; value = the first form's grid-point RK3-steps/s."""
     from chapsim_amd import init, thermo
     c = CFG_HEATED
     L, M, N = c["L"], c["M"], c["N"]
@@ -471,39 +473,58 @@ def bench_heated(pkg, args):
     dh, g, pr = init.heated_channel_initial(co, t, seed=180)
     cthecd = 1.0 / c["REN"] / (t["M0"] * t["CP0"] / t["K0"])
     dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
-    eng = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, c["wallflux"])
-    eng.upload(dh, g)
-    f0 = eng.download()
     vis = 1 if args.vis == "cn" else 0
-    io = pkg.ChannelIO(co, c["REN"], flow_driven=1, vis_scheme=vis)
     io_shapes = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set}
-    if io_shapes:
-        io.set_tuning(**io_shapes)
-    io.upload(G=g, PR=pr, D0=f0["D"], M0=f0["M"])
     dt = c["dt"]
 
-    def step():
-        for ns in (1, 2, 3):
-            pkg.heated_substep(eng, io, ns, dt)
+    def run(mixed):
+        eng = thermo.EnergyTG(co, t, (1, 1), cthecd, dhmin, dhmax, c["wallflux"])
+        eng.upload(dh, g)
+        f0 = eng.download()
+        io = pkg.ChannelIO(co, c["REN"], flow_driven=1, vis_scheme=vis)
+        if io_shapes:
+            io.set_tuning(**io_shapes)
+        io.set_mixed(mixed)
+        io.upload(G=g, PR=pr, D0=f0["D"], M0=f0["M"])
 
-    for _ in range(args.warmup):
-        step()
-    io.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    io.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    f = io.download()
-    fe = eng.download()
+        def step():
+            for ns in (1, 2, 3):
+                pkg.heated_substep(eng, io, ns, dt)
+
+        for _ in range(args.warmup):
+            step()
+        io.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        io.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        f, fe = io.download(), eng.download()
+        eng.close()
+        io.close()
+        return ms, f, fe
+
+    forms = {"both": (False, True), "plain": (False,), "mixed": (True,)}[args.form]
+    res = {mx: run(mx) for mx in forms}
+    ms, f, fe = res[forms[0]]
+    mixed_leg = None
+    if True in res:
+        msm, fm, fem = res[True]
+        mixed_leg = {"ms_per_step": msm, "value": L * M * N / (msm * 1e-3)}
+        if False in res:
+            def rerr(a, b):
+                return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+            mixed_leg["speedup_vs_plain"] = ms / msm
+            mixed_leg["rel_err_vs_plain"] = {
+                "u": rerr(fm["G"][0], f["G"][0]), "v": rerr(fm["G"][1], f["G"][1]),
+                "w": rerr(fm["G"][2], f["G"][2]), "p": rerr(fm["PR"], f["PR"]), "T": rerr(fem["T"], fe["T"]),
+                "after_steps": args.warmup + args.steps}
     # per-step algorithmic bytes, FP64 field passes per stage: Crank-Nicolson 138, the fixed
     # count of compulsory passes (energy 32: RHS + update 10, property update 7, d rho / dt 7,
     # rho / mu staggering 8; momentum 106: u = G / rho 9, explicit terms 23, x / z / y line
     # solves 12 + 12 + 15, divergence + forward transforms 9, y-Thomas 2, inverse transforms 4,
     # p / G / u updates 20); explicit 148, the kernel list of chs_io.hip / chs_thermo.hip
     alg = 3 * (138 if vis else 148) * 8.0 * L * M * N
-    eng.close()
-    io.close()
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -522,6 +543,8 @@ def bench_heated(pkg, args):
         "step_roofline": {"alg_bytes_per_step": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
                           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "form": "plain FP64" if not forms[0] else "mixed (EXPLT0 and momentum increment FP32)",
+        "mixed": mixed_leg,
         "checks": {"T_range": [float(fe["T"][:, 1:M + 1].min()), float(fe["T"][:, 1:M + 1].max())],
                    "bulk_G": float(np.mean(f["G"][0][:, 1:M + 1] / np.asarray(co["dyfi"])[None, :, None]))},
     }))

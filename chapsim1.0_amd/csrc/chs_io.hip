@@ -41,6 +41,7 @@ struct IoDev {
     double *g, *g0, *q, *dstg, *mstg;         // 3 halo fields each
     double *pr, *d0, *m0;                     // halo fields
     double *explt0, *h;                       // 3 cell fields each
+    float *explt0f, *hf;                      // the same in the mixed form (work precision FP32)
     double *drho, *divu, *rhsp, *dph;         // cell fields
     double *partial, *scal;
     // Crank-Nicolson scheme (iVisScheme = 1)
@@ -54,6 +55,14 @@ __device__ __forceinline__ size_t hx(const IoDev &d, int i, int J, int k) { retu
 __device__ __forceinline__ size_t cx(const IoDev &d, int i, int J, int k) { return ((size_t)k * d.M + (J - 1)) * d.L + i; }
 __device__ __forceinline__ size_t hsz(const IoDev &d) { return (size_t)d.N * (d.M + 2) * d.L; }
 __device__ __forceinline__ size_t csz(const IoDev &d) { return (size_t)d.N * d.M * d.L; }
+
+// EXPLT0 and the increment in the work precision W: double (plain) or float (mixed form).
+template <typename W> __device__ __forceinline__ W *ioh(const IoDev &d);
+template <> __device__ __forceinline__ double *ioh<double>(const IoDev &d) { return d.h; }
+template <> __device__ __forceinline__ float *ioh<float>(const IoDev &d) { return d.hf; }
+template <typename W> __device__ __forceinline__ W *ioex(const IoDev &d);
+template <> __device__ __forceinline__ double *ioex<double>(const IoDev &d) { return d.explt0; }
+template <> __device__ __forceinline__ float *ioex<float>(const IoDev &d) { return d.explt0f; }
 
 // Thread (i, J = j0 + row, k) of a 64 x 4 block on a (ceil(L/64), ceil(rows/4), N) grid.
 #define IO_THREAD(j0)                                        \
@@ -195,8 +204,9 @@ enum IoField { FG0, FG1, FG2, FQ0, FQ1, FQ2, FMS0, FMS1, FMS2, FM0, FM1, FPR, FD
 // YCL2ND_WFF/WFB(J) and (J+1).
 enum IoRowC { RDYFI, RDYFIM, RDYCI, RDYCIP, RWFF, RWFB, RWFFP, RWFBP, IO_NROW };
 
-// ac(f, di, dj, dk) = field f at (i + di, J + dj, k + dk), x and z periodic, read from HBM.
-struct IoAtHbm {
+// ac(f, di, dj, dk) = field f at (i + di, J + dj, k + dk), x and z periodic, read from HBM;
+// EXPLT0 and the increment in the work precision W.
+template <typename W = double> struct IoAtHbm {
     const IoDev &d;
     const double *gs;
     int i, J, k, ip, im, kp, km;
@@ -234,7 +244,14 @@ struct IoAtHbm {
         }
     }
     // EXPLT0 of component c and the staggered density of component c at the point
-    __device__ __forceinline__ double ex(int c) const { return d.explt0[(size_t)c * csz(d) + cx(d, i, J, k)]; }
+    __device__ __forceinline__ double ex(int c) const { return ioex<W>(d)[(size_t)c * csz(d) + cx(d, i, J, k)]; }
+    // the new EXPLT0 and the increment of component c at the point
+    __device__ __forceinline__ void put(int c, double acc, double r) const
+    {
+        const size_t o = (size_t)c * csz(d) + cx(d, i, J, k);
+        ioex<W>(d)[o] = (W)acc;
+        ioh<W>(d)[o] = (W)r;
+    }
     __device__ __forceinline__ double ds(int c) const { return d.dstg[(size_t)c * hsz(d) + hx(d, i, J, k)]; }
 };
 
@@ -461,14 +478,13 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const Acc &ac, in
             }
             pgm = (ac(FPR, 0, 0, 0) - ac(FPR, 0, 0, -1)) * (ta * dt * dzi);
         }
-        const size_t o = (size_t)C * csz(d) + cx(d, i, J, k);
         double r;
         if constexpr (CN) r = (tg * acc + tr * ac.ex(C)) * dt + (ta * dt) * imp;
         else r = (tg * acc + tr * ac.ex(C)) * dt;
-        d.explt0[o] = acc;
+
         r = r - pgm;
         if (d.gravity == C + 1) r = r + ac.ds(C) * (ta * dt * d.f_a);
-        d.h[o] = r;
+        ac.put(C, acc, r);
         return C == 0 ? r / ac.rw(RDYFI) : 0.0;
     }
 }
@@ -478,7 +494,7 @@ __device__ __forceinline__ double io_mom_point(const IoDev &d, const Acc &ac, in
 // mass-flux correction (RHS_MOM_EXPLICIT_io.f90:229-265) is written as well.
 // A 64 x 4 block covers TY rows of TZ = 4 / TY z-planes (TY 4, 2 or 1), so the z-neighbour
 // values a point reads belong to the block's own other planes when TY < 4.
-template <bool CN>
+template <bool CN, typename W>
 __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restrict__ gs, double tg, double tr, double ta,
                                                 double dt, int TY)
 {
@@ -489,7 +505,7 @@ __global__ __launch_bounds__(256) void k_io_mom(IoDev d, const double *__restric
     const int L = d.L, M = d.M, N = d.N;
     double contrib = 0.0;
     if (i < L && J <= M && k < N) {
-        const IoAtHbm ac{d, gs, i, J, k, i == L - 1 ? 0 : i + 1, i == 0 ? L - 1 : i - 1, k == N - 1 ? 0 : k + 1,
+        const IoAtHbm<W> ac{d, gs, i, J, k, i == L - 1 ? 0 : i + 1, i == 0 ? L - 1 : i - 1, k == N - 1 ? 0 : k + 1,
                          k == 0 ? N - 1 : k - 1};
         contrib = io_mom_point<0, CN>(d, ac, i, J, k, tg, tr, ta, dt);
         if (J >= 2) io_mom_point<1, CN>(d, ac, i, J, k, tg, tr, ta, dt);
@@ -541,16 +557,17 @@ template <bool CN> struct IoStage {
     static __device__ __forceinline__ constexpr int carry_dj(int c) { return (c == 3 || c == 8) ? 1 : 0; }
 };
 
-template <bool CN, int BY> struct IoAtLds {
+template <bool CN, int BY, typename W> struct IoAtLds {
     static constexpr int TW = 66, TS = (BY + 2) * TW;
     const double *c0, *c1;   // this point's cell in the tiles of planes k and k+1
     const double *cr;        // the carried plane-(k-1) values
     const double *xr;        // EXPLT0 of the three components and the buoyancy component's D_STG
     const double *rr;        // the row's mesh coefficients (IoRowC), loaded once
-    IoAtHbm hb;              // any other plane-(k-1) value (the stencil reads none)
+    IoAtHbm<W> hb;           // any other plane-(k-1) value (the stencil reads none); the stores
     __device__ __forceinline__ double rw(int c) const { return rr[c]; }
     __device__ __forceinline__ double ex(int c) const { return xr[c]; }
     __device__ __forceinline__ double ds(int) const { return xr[3]; }
+    __device__ __forceinline__ void put(int c, double acc, double r) const { hb.put(c, acc, r); }
     __device__ __forceinline__ double operator()(int f, int di, int dj, int dk) const
     {
         if (dk < 0) {
@@ -571,7 +588,7 @@ __device__ __forceinline__ int io_xcd_tile(int p, int nb, int nxcd)
     return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
-template <bool CN, int BY>
+template <bool CN, int BY, typename W>
 __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *__restrict__ gs, double tg, double tr,
                                                          double ta, double dt, int KZ, int nxcd)
 {
@@ -641,7 +658,7 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
     double cr[S::NC];
     {
         const int km = k0 == 0 ? N - 1 : k0 - 1;
-        const IoAtHbm hb{d, gs, act ? i : 0, act ? J : 1, km, act ? ip : 0, act ? im : 0, km, km};
+        const IoAtHbm<> hb{d, gs, act ? i : 0, act ? J : 1, km, act ? ip : 0, act ? im : 0, km, km};
 #pragma unroll
         for (int c = 0; c < S::NC; ++c) cr[c] = hb(S::carry_f(c), S::carry_di(c), S::carry_dj(c), 0);
     }
@@ -654,14 +671,14 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
         if (act) {
             const size_t o = cx(d, i, J, kk);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) x[c] = d.explt0[(size_t)c * pc * N + o];
+            for (int c = 0; c < 3; ++c) x[c] = ioex<W>(d)[(size_t)c * pc * N + o];
             if (gc >= 0 && gc <= 2) x[3] = d.dstg[(size_t)gc * hs + hx(d, i, J, kk)];
         }
     };
     double rr[IO_NROW];
     {
         const int Jr = min(J, M);   // rows past M (no points) take row M's
-        const IoAtHbm hb{d, gs, 0, Jr, 0, 0, 0, 0, 0};
+        const IoAtHbm<> hb{d, gs, 0, Jr, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int c = 0; c < IO_NROW; ++c) rr[c] = (c == RDYFIM && Jr < 2) ? 0.0 : hb.rw(c);
     }
@@ -681,8 +698,8 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
             load(k + 2 >= N ? k + 2 - N : k + 2);
         }
         if (act) {
-            const IoAtLds<CN, BY> ac{&tiles[cur][cell], &tiles[cur ^ 1][cell], cr, xr, rr,
-                                     IoAtHbm{d, gs, i, J, k, ip, im, k == N - 1 ? 0 : k + 1, k == 0 ? N - 1 : k - 1}};
+            const IoAtLds<CN, BY, W> ac{&tiles[cur][cell], &tiles[cur ^ 1][cell], cr, xr, rr,
+                                        IoAtHbm<W>{d, gs, i, J, k, ip, im, k == N - 1 ? 0 : k + 1, k == 0 ? N - 1 : k - 1}};
             contrib = contrib + io_mom_point<0, CN>(d, ac, i, J, k, tg, tr, ta, dt);
             if (J >= 2) io_mom_point<1, CN>(d, ac, i, J, k, tg, tr, ta, dt);
             io_mom_point<2, CN>(d, ac, i, J, k, tg, tr, ta, dt);
@@ -724,6 +741,7 @@ __global__ __launch_bounds__(256) void k_io_reduce(const double *__restrict__ pa
 // The streamwise correction (RHS_MOM_EXPLICIT_io.f90:259-265) and MASSFLUX_CALC_io
 // (MASSFLUX_CALC_IO.f90:15-52) with BC_WALL_G_io: rows 0..M+1.
 // gs: G at the start of the substep (G0_io); writes every row of the new G.
+template <typename W>
 __global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__restrict__ gs,
                                                      const double *__restrict__ dpg_dev, double dpg)
 {
@@ -737,9 +755,10 @@ __global__ __launch_bounds__(256) void k_io_update_g(IoDev d, const double *__re
     }
     const double dp = dpg_dev ? dpg_dev[0] : dpg;
     const size_t ci = cx(d, i, J, k), cs = csz(d);
-    d.g[c0] = gs[c0] + (d.h[ci] - dp);
-    d.g[2 * hs + c0] = gs[2 * hs + c0] + d.h[2 * cs + ci];
-    d.g[hs + c0] = J == 1 ? 0.0 : gs[hs + c0] + d.h[cs + ci];
+    const W *h = ioh<W>(d);
+    d.g[c0] = gs[c0] + ((double)h[ci] - dp);
+    d.g[2 * hs + c0] = gs[2 * hs + c0] + (double)h[2 * cs + ci];
+    d.g[hs + c0] = J == 1 ? 0.0 : gs[hs + c0] + (double)h[cs + ci];
 }
 
 // ---- Crank-Nicolson line solves (MOMFA_io, MOMFA_io.f90:18-406) --------------------------
@@ -836,7 +855,7 @@ __device__ __forceinline__ void thomas_line(int n, const Coef &coef, const Rhs &
 // MOMFA1_X_io (MOMFA_io.f90:77-176): cyclic x-lines of component C over rows J0..M (2..M for v)
 // with FACOE = alpha dt / (2 Re); the mass-flux correction of the streamwise increment
 // (RHS_MOM_EXPLICIT_io.f90:259-265) is applied as the line is read.  In place on d.h[C].
-template <int C>
+template <int C, typename W>
 __device__ __forceinline__ void io_cn_x_line(const IoDev &d, int l, double dp, double facoe)
 {
 #pragma clang fp contract(off)
@@ -845,8 +864,8 @@ __device__ __forceinline__ void io_cn_x_line(const IoDev &d, int l, double dp, d
     const int J = j0 + l % nj, k = l / nj;
     const size_t hs = hsz(d), cs = csz(d);
     const double *DS = d.dstg + C * hs;
-    double *R = d.h + C * cs;
-    double *W = d.work + (size_t)C * 3 * cs + l;
+    W *R = ioh<W>(d) + C * cs;
+    double *Wk = d.work + (size_t)C * 3 * cs + l;
     const double coe1 = facoe * 4.0 / 3.0 * d.dxqi, coe2 = facoe * d.dxqi;
     auto coef = [&](int i, double &a, double &b, double &c) {
         const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
@@ -866,19 +885,20 @@ __device__ __forceinline__ void io_cn_x_line(const IoDev &d, int l, double dp, d
         b = 1.0 - (amu + apu) / DS[hx(d, i, J, k)];
     };
     auto rhs = [&](int i) {
-        if constexpr (C == 0) return R[cx(d, i, J, k)] - dp;
-        else return R[cx(d, i, J, k)];
+        if constexpr (C == 0) return (double)R[cx(d, i, J, k)] - dp;
+        else return (double)R[cx(d, i, J, k)];
     };
-    auto out = [&](int i, double x) { R[cx(d, i, J, k)] = x; };
-    thomas_cyc(L, coef, rhs, out, W, W + cs, W + 2 * cs, (size_t)nl);
+    auto out = [&](int i, double x) { R[cx(d, i, J, k)] = (W)x; };
+    thomas_cyc(L, coef, rhs, out, Wk, Wk + cs, Wk + 2 * cs, (size_t)nl);
 }
 
+template <typename W>
 __global__ __launch_bounds__(128) void k_io_cn_x(IoDev d, const double *__restrict__ dpg_dev, double dpg, double facoe)
 {
     const int l = blockIdx.x * 128 + threadIdx.x;
-    if (blockIdx.y == 0) io_cn_x_line<0>(d, l, dpg_dev ? dpg_dev[0] : dpg, facoe);
-    else if (blockIdx.y == 1) io_cn_x_line<1>(d, l, 0.0, facoe);
-    else io_cn_x_line<2>(d, l, 0.0, facoe);
+    if (blockIdx.y == 0) io_cn_x_line<0, W>(d, l, dpg_dev ? dpg_dev[0] : dpg, facoe);
+    else if (blockIdx.y == 1) io_cn_x_line<1, W>(d, l, 0.0, facoe);
+    else io_cn_x_line<2, W>(d, l, 0.0, facoe);
 }
 
 // Block maximum of a non-negative value: a wave maximum by shuffles, one 64-bit atomic per
@@ -911,7 +931,7 @@ __device__ __forceinline__ int jacobi_sweeps(double rho)
 // MOMFA1_X_io for L a power of two (4 <= L <= 512): T = blockDim.x / L x-lines per block, one
 // thread per point, coalesced; the coefficients are built from the staged property rows as
 // io_cn_x_line builds them.  Lines past the end carry a dummy identity row.
-template <int C>
+template <int C, typename W>
 __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double facoe, double *lds)
 {
 #pragma clang fp contract(off)
@@ -930,7 +950,7 @@ __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double fa
         const size_t o = hx(d, e, J, k);
         f0 = C == 0 ? d.m1[o] : d.mstg[(C - 1) * hs + o];
         f1 = d.dstg[C * hs + o];
-        r = d.h[C * cs + cx(d, e, J, k)];
+        r = ioh<W>(d)[C * cs + cx(d, e, J, k)];
         if constexpr (C == 0) r = r - dp;
     }
     s0[threadIdx.x] = f0;
@@ -954,7 +974,7 @@ __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double fa
     const double ib = 1.0 / b;
     const double rho = block_max_nonneg((fabs(a) + fabs(c)) * fabs(ib), word);
     if (!(rho <= 0.25)) {   // the same branch for the whole block
-        if (valid && e == 0) io_cn_x_line<C>(d, line, dp, facoe);
+        if (valid && e == 0) io_cn_x_line<C, W>(d, line, dp, facoe);
         return;
     }
     const int K = jacobi_sweeps(rho);
@@ -966,14 +986,14 @@ __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double fa
         __syncthreads();
         x = (r - a * xm - c * xp) * ib;
     }
-    if (valid) d.h[C * cs + cx(d, e, J, k)] = x;
+    if (valid) ioh<W>(d)[C * cs + cx(d, e, J, k)] = (W)x;
 }
 
 // MOMFA1_X_io for L = 64 P (P = 1..8): one wavefront per x-line, lane l holding points
 // l P .. l P + P - 1; the neighbours across lanes come by cross-lane shuffles, so a line needs
 // no block barrier and a wavefront's loads are all issued together.  Coefficients and sweeps
 // as io_cn_x_jac (rho is the line's own maximum).
-template <int C, int P>
+template <int C, int P, typename W>
 __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double facoe)
 {
 #pragma clang fp contract(off)
@@ -986,7 +1006,7 @@ __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double f
     const int lm = (lane + 63) & 63, lp = (lane + 1) & 63;
     const double *F0 = (C == 0 ? d.m1 : d.mstg + (C - 1) * hs) + hx(d, lane * P, J, k);
     const double *F1 = d.dstg + C * hs + hx(d, lane * P, J, k);
-    double *R = d.h + C * cs + cx(d, lane * P, J, k);
+    W *R = ioh<W>(d) + C * cs + cx(d, lane * P, J, k);
     double f0[P], f1[P], r[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
@@ -1020,7 +1040,7 @@ __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double f
     }
     for (int o = 32; o > 0; o >>= 1) rho = nanmax(rho, __shfl_xor(rho, o));
     if (!(rho <= 0.25)) {
-        if (lane == 0) io_cn_x_line<C>(d, line, dp, facoe);
+        if (lane == 0) io_cn_x_line<C, W>(d, line, dp, facoe);
         return;
     }
     const int K = jacobi_sweeps(rho);
@@ -1039,29 +1059,30 @@ __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double f
         for (int j = 0; j < P; ++j) x[j] = xn[j];
     }
 #pragma unroll
-    for (int j = 0; j < P; ++j) R[j] = x[j];
+    for (int j = 0; j < P; ++j) R[j] = (W)x[j];
 }
 
-template <int P>
+template <int P, typename W>
 __global__ __launch_bounds__(256) void k_io_cn_x_wave(IoDev d, const double *__restrict__ dpg_dev, double dpg,
                                                       double facoe)
 {
-    if (blockIdx.y == 0) io_cn_x_wave<0, P>(d, dpg_dev ? dpg_dev[0] : dpg, facoe);
-    else if (blockIdx.y == 1) io_cn_x_wave<1, P>(d, 0.0, facoe);
-    else io_cn_x_wave<2, P>(d, 0.0, facoe);
+    if (blockIdx.y == 0) io_cn_x_wave<0, P, W>(d, dpg_dev ? dpg_dev[0] : dpg, facoe);
+    else if (blockIdx.y == 1) io_cn_x_wave<1, P, W>(d, 0.0, facoe);
+    else io_cn_x_wave<2, P, W>(d, 0.0, facoe);
 }
 
+template <typename W>
 __global__ __launch_bounds__(512) void k_io_cn_x_jac(IoDev d, const double *__restrict__ dpg_dev, double dpg,
                                                      double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_x_jac<0>(d, dpg_dev ? dpg_dev[0] : dpg, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_x_jac<1>(d, 0.0, facoe, lds);
-    else io_cn_x_jac<2>(d, 0.0, facoe, lds);
+    if (blockIdx.y == 0) io_cn_x_jac<0, W>(d, dpg_dev ? dpg_dev[0] : dpg, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_x_jac<1, W>(d, 0.0, facoe, lds);
+    else io_cn_x_jac<2, W>(d, 0.0, facoe, lds);
 }
 
 // MOMFA2_Z_io (MOMFA_io.f90:179-265): cyclic z-lines (RCCI2 = RNDI2 = 1 on the channel).
-template <int C>
+template <int C, typename W>
 __device__ __forceinline__ void io_cn_z_line(const IoDev &d, int l, double facoe)
 {
 #pragma clang fp contract(off)
@@ -1070,8 +1091,8 @@ __device__ __forceinline__ void io_cn_z_line(const IoDev &d, int l, double facoe
     const int i = l % L, J = j0 + l / L;
     const size_t hs = hsz(d), cs = csz(d);
     const double *DS = d.dstg + C * hs;
-    double *R = d.h + C * cs;
-    double *W = d.work + (size_t)C * 3 * cs + l;
+    W *R = ioh<W>(d) + C * cs;
+    double *Wk = d.work + (size_t)C * 3 * cs + l;
     const double coe1 = facoe * 4.0 / 3.0 * d.dzqi, coe2 = facoe * d.dzqi;
     auto coef = [&](int k, double &a, double &b, double &c) {
         const int kp = k == N - 1 ? 0 : k + 1, km = k == 0 ? N - 1 : k - 1;
@@ -1090,17 +1111,18 @@ __device__ __forceinline__ void io_cn_z_line(const IoDev &d, int l, double facoe
         c = apu / DS[hx(d, i, J, kp)];
         b = 1.0 - (amu + apu) / DS[hx(d, i, J, k)];
     };
-    auto rhs = [&](int k) { return R[cx(d, i, J, k)]; };
-    auto out = [&](int k, double x) { R[cx(d, i, J, k)] = x; };
-    thomas_cyc(N, coef, rhs, out, W, W + cs, W + 2 * cs, (size_t)nl);
+    auto rhs = [&](int k) { return (double)R[cx(d, i, J, k)]; };
+    auto out = [&](int k, double x) { R[cx(d, i, J, k)] = (W)x; };
+    thomas_cyc(N, coef, rhs, out, Wk, Wk + cs, Wk + 2 * cs, (size_t)nl);
 }
 
+template <typename W>
 __global__ __launch_bounds__(128) void k_io_cn_z(IoDev d, double facoe)
 {
     const int l = blockIdx.x * 128 + threadIdx.x;
-    if (blockIdx.y == 0) io_cn_z_line<0>(d, l, facoe);
-    else if (blockIdx.y == 1) io_cn_z_line<1>(d, l, facoe);
-    else io_cn_z_line<2>(d, l, facoe);
+    if (blockIdx.y == 0) io_cn_z_line<0, W>(d, l, facoe);
+    else if (blockIdx.y == 1) io_cn_z_line<1, W>(d, l, facoe);
+    else io_cn_z_line<2, W>(d, l, facoe);
 }
 
 // MOMFA3_Y_io (MOMFA_io.f90:268-406): y-lines over rows 1..M with zero boundary values, then
@@ -1108,7 +1130,7 @@ __global__ __launch_bounds__(128) void k_io_cn_z(IoDev d, double facoe)
 // line's new G.  u, w: the wall rows' viscosity is the mean of the wall cell and its ghost in
 // the line's own z-plane (the one-rank form of MOMFA_io.f90:342-352).  v: row 1 is the wall,
 // AMVR = APVR = 0 there, RHS 0.
-template <int C>
+template <int C, typename W>
 __device__ __forceinline__ void io_cn_y_line(const IoDev &d, const double *__restrict__ gs, int l, double facoe)
 {
 #pragma clang fp contract(off)
@@ -1118,8 +1140,8 @@ __device__ __forceinline__ void io_cn_y_line(const IoDev &d, const double *__res
     const size_t hs = hsz(d), cs = csz(d);
     const double *DS = d.dstg + C * hs, *FM = C == 1 ? d.m1 : d.mstg + C * hs;
     const double *am = d.amvr + C * (M + 2), *ap = d.apvr + C * (M + 2);
-    const double *R = d.h + C * cs;
-    double *W = d.work + (size_t)C * 3 * cs + l;
+    const W *R = ioh<W>(d) + C * cs;
+    double *Wk = d.work + (size_t)C * 3 * cs + l;
     const double coe1 = facoe * 4.0 / 3.0, coe2 = facoe;
     auto coef = [&](int jj, double &a, double &b, double &c) {
         const int J = jj + 1;
@@ -1151,7 +1173,7 @@ __device__ __forceinline__ void io_cn_y_line(const IoDev &d, const double *__res
         }
         b = 1.0 - (amu + apu) / DS[hx(d, i, J, k)];
     };
-    auto rhs = [&](int jj) { return (C == 1 && jj == 0) ? 0.0 : R[cx(d, i, jj + 1, k)]; };
+    auto rhs = [&](int jj) { return (C == 1 && jj == 0) ? 0.0 : (double)R[cx(d, i, jj + 1, k)]; };
     double *G = d.g + C * hs;
     const double *G0 = gs + C * hs;
     auto out = [&](int jj, double x) {
@@ -1161,22 +1183,23 @@ __device__ __forceinline__ void io_cn_y_line(const IoDev &d, const double *__res
     };
     G[hx(d, i, 0, k)] = 0.0;
     G[hx(d, i, M + 1, k)] = 0.0;
-    thomas_line(M, coef, rhs, out, W, W + cs, (size_t)nl);
+    thomas_line(M, coef, rhs, out, Wk, Wk + cs, (size_t)nl);
 }
 
+template <typename W>
 __global__ __launch_bounds__(128) void k_io_cn_y(IoDev d, const double *__restrict__ gs, double facoe)
 {
     const int l = blockIdx.x * 128 + threadIdx.x;
-    if (blockIdx.y == 0) io_cn_y_line<0>(d, gs, l, facoe);
-    else if (blockIdx.y == 1) io_cn_y_line<1>(d, gs, l, facoe);
-    else io_cn_y_line<2>(d, gs, l, facoe);
+    if (blockIdx.y == 0) io_cn_y_line<0, W>(d, gs, l, facoe);
+    else if (blockIdx.y == 1) io_cn_y_line<1, W>(d, gs, l, facoe);
+    else io_cn_y_line<2, W>(d, gs, l, facoe);
 }
 
 // MOMFA2_Z_io with P consecutive z-points per thread (N a multiple of P): a block holds T
 // x-columns of one row J, thread (lt, chunk) at lt + T chunk, so a wave reads runs of T
 // consecutive x; only the chunk edges go through LDS (coefficient neighbours, then the Jacobi
 // sweeps' neighbours).
-template <int C, int P>
+template <int C, int P, typename W>
 __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double *lds)
 {
 #pragma clang fp contract(off)
@@ -1201,7 +1224,7 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
             const size_t o = hx(d, i, J, k0 + j);
             f0[j] = C == 2 ? d.m1[o] : d.mstg[(C + 1) * hs + o];
             f1[j] = d.dstg[C * hs + o];
-            r[j] = d.h[C * cs + cx(d, i, J, k0 + j)];
+            r[j] = ioh<W>(d)[C * cs + cx(d, i, J, k0 + j)];
         }
     }
     e0[threadIdx.x] = f0[0];
@@ -1237,7 +1260,7 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
     }
     rho = block_max_nonneg(rho, word);   // also orders the edge reads above before the sweeps
     if (!(rho <= 0.25)) {
-        if (valid && tt == 0) io_cn_z_line<C>(d, (J - j0) * L + i, facoe);
+        if (valid && tt == 0) io_cn_z_line<C, W>(d, (J - j0) * L + i, facoe);
         return;
     }
     const int K = jacobi_sweeps(rho);
@@ -1261,23 +1284,24 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
     }
     if (valid) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) d.h[C * cs + cx(d, i, J, k0 + j)] = x[j];
+        for (int j = 0; j < P; ++j) ioh<W>(d)[C * cs + cx(d, i, J, k0 + j)] = (W)x[j];
     }
 }
 
+template <typename W>
 __global__ __launch_bounds__(1024) void k_io_cn_z_jac(IoDev d, double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_z_jac<0, 4>(d, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_z_jac<1, 4>(d, facoe, lds);
-    else io_cn_z_jac<2, 4>(d, facoe, lds);
+    if (blockIdx.y == 0) io_cn_z_jac<0, 4, W>(d, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_z_jac<1, 4, W>(d, facoe, lds);
+    else io_cn_z_jac<2, 4, W>(d, facoe, lds);
 }
 
 // MOMFA3_Y_io + the G update with P consecutive rows per thread (NT = ceil(M / P) chunks per
 // column, rows past M are identity rows): a block holds T x-columns of one z-plane, thread
 // (lt, chunk) at lt + T chunk; coefficients as io_cn_y_line builds them; writes every row of
 // the new G of its columns.
-template <int C, int P>
+template <int C, int P, typename W>
 __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__restrict__ gs, double facoe, double *lds)
 {
 #pragma clang fp contract(off)
@@ -1304,7 +1328,7 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
             const size_t o = hx(d, i, J, k);
             f0[j] = FM[o];
             f1[j] = DS[o];
-            r[j] = (C == 1 && J == 1) ? 0.0 : d.h[C * cs + cx(d, i, J, k)];
+            r[j] = (C == 1 && J == 1) ? 0.0 : (double)ioh<W>(d)[C * cs + cx(d, i, J, k)];
         }
     }
     e0[threadIdx.x] = f0[0];
@@ -1360,7 +1384,7 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
     }
     rho = block_max_nonneg(rho, word);
     if (!(rho <= 0.25)) {
-        if (valid && tt == 0) io_cn_y_line<C>(d, gs, k * L + i, facoe);
+        if (valid && tt == 0) io_cn_y_line<C, W>(d, gs, k * L + i, facoe);
         return;
     }
     const int K = jacobi_sweeps(rho);
@@ -1398,12 +1422,13 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
     }
 }
 
+template <typename W>
 __global__ __launch_bounds__(1024) void k_io_cn_y_jac(IoDev d, const double *__restrict__ gs, double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_y_jac<0, 4>(d, gs, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_y_jac<1, 4>(d, gs, facoe, lds);
-    else io_cn_y_jac<2, 4>(d, gs, facoe, lds);
+    if (blockIdx.y == 0) io_cn_y_jac<0, 4, W>(d, gs, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_y_jac<1, 4, W>(d, gs, facoe, lds);
+    else io_cn_y_jac<2, 4, W>(d, gs, facoe, lds);
 }
 
 // DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
@@ -1533,6 +1558,19 @@ __global__ __launch_bounds__(256) void k_io_reduce_max(const double *__restrict_
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_io_cvt(const A *__restrict__ a, B *__restrict__ b, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) b[t] = (B)a[t];
+}
+
+template <typename A, typename B> hipError_t io_convert(const A *a, B *b, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_io_cvt<A, B>), dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0, st,
+                       a, b, n);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_io_fill(double *__restrict__ p, size_t n, double v)
 {
     const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -1554,12 +1592,15 @@ struct chs_io {
     int mom_kz = 16;    // z-planes per block of the LDS-staged momentum kernel (0: the per-point kernel)
     int mom_by = 4;     // rows per block of the LDS-staged momentum kernel (4 or 8)
     int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
+    int cnz_cols = 0, cny_cols = 0;   // cn_cols for the z or the y solve alone (0: cn_cols)
     int cnx_wave = 1;   // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
     int cn_cols = 16;   // most x-columns per block of the z and y Jacobi line solves (16: 1.7 %
                         // faster per heated step than 64 at 256 x 128 x 256)
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
     double *own_m1 = nullptr;                       // d.m1 likewise (Crank-Nicolson only)
     double *cnbuf = nullptr;                        // m1, work, amvr, apvr
+    float *wbuf = nullptr;                          // d.explt0f, d.hf (mixed form)
+    int mixed = 0;                                  // EXPLT0 and the increment in FP32
     int cn = 0;                                     // iVisScheme
     bool q_fresh = false;   // Q = G / D_STG of the current G and D_STG (written by chs_io_props)
     double cf_given = 0.0, volm = 0.0;
@@ -1690,6 +1731,7 @@ void chs_io_destroy(chs_io *m)
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->buf) (void)hipFree(m->buf);
     if (m->cnbuf) (void)hipFree(m->cnbuf);
+    if (m->wbuf) (void)hipFree(m->wbuf);
     if (m->pois) chs_destroy(m->pois);
     delete m;
 }
@@ -1717,8 +1759,31 @@ int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr
     if (he == hipSuccess)
         he = explt0 ? hipMemcpyAsync(d.explt0, explt0, 3 * cb, hipMemcpyHostToDevice, m->stream)
                     : hipMemsetAsync(d.explt0, 0, 3 * cb, m->stream);
+    if (he == hipSuccess && m->mixed) he = io_convert(d.explt0, d.explt0f, 3 * cb / sizeof(double), m->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
     return io_hip(m, he, "chs_io_upload");
+}
+
+int chs_io_set_mixed(chs_io *m, int mixed)
+{
+    if (!m || (mixed != 0 && mixed != 1)) return io_fail(m, CHS_E_ARG, "chs_io_set_mixed: bad argument");
+    const size_t n = (size_t)3 * m->N * m->M * m->L;
+    hipError_t he = hipSuccess;
+    if (mixed && !m->wbuf) {
+        he = hipMalloc(&m->wbuf, sizeof(float) * 2 * n);
+        if (he != hipSuccess) {
+            m->wbuf = nullptr;
+            return io_hip(m, he, "chs_io_set_mixed");
+        }
+        m->d.explt0f = m->wbuf;
+        m->d.hf = m->wbuf + n;
+    }
+    // the RK history carries over in the new precision
+    if (mixed && !m->mixed) he = io_convert(m->d.explt0, m->d.explt0f, n, m->stream);
+    if (!mixed && m->mixed) he = io_convert(m->d.explt0f, m->d.explt0, n, m->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
+    if (he == hipSuccess) m->mixed = mixed;
+    return io_hip(m, he, "chs_io_set_mixed");
 }
 
 int chs_io_set_properties(chs_io *m, const double *d0_dev, const double *m0_dev)
@@ -1783,6 +1848,11 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
         m->mom_by = value;
         return CHS_OK;
     }
+    if (k == "cnz_cols" || k == "cny_cols") {
+        if (value < 0 || value > 64) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: " + k + " must be 0..64");
+        (k == "cnz_cols" ? m->cnz_cols : m->cny_cols) = value;
+        return CHS_OK;
+    }
     if (k == "cnx_wave") {
         if (value != 0 && value != 1) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: cnx_wave must be 0 or 1");
         m->cnx_wave = value;
@@ -1796,15 +1866,16 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
     return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: unknown key " + k);
 }
 
-int chs_substep_io_momentum(chs_io *m, int ns, double dt)
+}  // extern "C"
+
+namespace {
+
+// The momentum terms and solves of a substep in the work precision W (EXPLT0 and the
+// increment): double for the plain form, float for the mixed form.
+template <typename W>
+void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, double ta, double dt)
 {
-    if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io_momentum: bad argument");
-    // G0_io <- G_io (SOLVERRK3_MOM_IO.f90:25) by exchanging the two arrays: the terms read the
-    // old G from g0 and k_io_update_g / k_io_cn_y write every row of the new G.
-    std::swap(m->d.g, m->d.g0);
     const IoDev &d = m->d;
-    const double *gs = d.g0;
-    const double tg = m->tgam[ns], tr = m->troh[ns], ta = m->talp[ns];
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
     if (!m->q_fresh) hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
     m->q_fresh = false;
@@ -1815,14 +1886,14 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
     if (!m->cn) hipLaunchKernelGGL(k_io_divu, gin, blk, 0, m->stream, d);
     if (KZ > 0) {
         const dim3 bz(64, BY);
-        if (m->cn && BY == 4) hipLaunchKernelGGL((k_io_mom_z<true, 4>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
-        else if (m->cn) hipLaunchKernelGGL((k_io_mom_z<true, 8>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
-        else if (BY == 4) hipLaunchKernelGGL((k_io_mom_z<false, 4>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
-        else hipLaunchKernelGGL((k_io_mom_z<false, 8>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        if (m->cn && BY == 4) hipLaunchKernelGGL((k_io_mom_z<true, 4, W>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        else if (m->cn) hipLaunchKernelGGL((k_io_mom_z<true, 8, W>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        else if (BY == 4) hipLaunchKernelGGL((k_io_mom_z<false, 4, W>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
+        else hipLaunchKernelGGL((k_io_mom_z<false, 8, W>), gmom, bz, 0, m->stream, d, gs, tg, tr, ta, dt, KZ, NX);
     } else if (m->cn) {
-        hipLaunchKernelGGL(k_io_mom<true>, gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
+        hipLaunchKernelGGL((k_io_mom<true, W>), gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
     } else {
-        hipLaunchKernelGGL(k_io_mom<false>, gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
+        hipLaunchKernelGGL((k_io_mom<false, W>), gmom, blk, 0, m->stream, d, gs, tg, tr, ta, dt, TY);
     }
     const double *dpg_dev = nullptr;
     double dpg = 0.0;
@@ -1840,42 +1911,58 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt)
         const int PX = L / 64;   // x-line points per lane of the wavefront-per-line solve
         if (m->cnx_wave && L % 64 == 0 && (PX == 1 || PX == 2 || PX == 4 || PX == 8)) {
             const dim3 gx((M * N + 3) / 4, 3);
-            if (PX == 1) hipLaunchKernelGGL(k_io_cn_x_wave<1>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
-            else if (PX == 2) hipLaunchKernelGGL(k_io_cn_x_wave<2>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
-            else if (PX == 4) hipLaunchKernelGGL(k_io_cn_x_wave<4>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
-            else hipLaunchKernelGGL(k_io_cn_x_wave<8>, gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            if (PX == 1) hipLaunchKernelGGL((k_io_cn_x_wave<1, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else if (PX == 2) hipLaunchKernelGGL((k_io_cn_x_wave<2, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else if (PX == 4) hipLaunchKernelGGL((k_io_cn_x_wave<4, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else hipLaunchKernelGGL((k_io_cn_x_wave<8, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
         } else if ((L & (L - 1)) == 0 && L >= 4 && L <= 512) {
             const int T = L >= 256 ? 1 : 256 / L;   // x-lines per block
-            hipLaunchKernelGGL(k_io_cn_x_jac, dim3((M * N + T - 1) / T, 3), dim3(T * L),
+            hipLaunchKernelGGL(k_io_cn_x_jac<W>, dim3((M * N + T - 1) / T, 3), dim3(T * L),
                                sizeof(double) * (2 * T * L + 1), m->stream, d, dpg_dev, dpg, facoe);
         } else {
-            hipLaunchKernelGGL(k_io_cn_x, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg,
+            hipLaunchKernelGGL(k_io_cn_x<W>, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg,
                                facoe);
         }
         constexpr int P = 4;   // line points per thread of the z and y kernels
         const int NTz = N / P;   // z: block = Tz x-columns x NTz chunks, whole waves
-        int Tz = (N % P == 0 && NTz >= 2 && NTz <= 1024) ? std::min(m->cn_cols, 1024 / NTz) : 0;
+        int Tz = (N % P == 0 && NTz >= 2 && NTz <= 1024) ? std::min(m->cnz_cols ? m->cnz_cols : m->cn_cols, 1024 / NTz) : 0;
         if (Tz > 0) Tz -= Tz % (64 / std::gcd(NTz, 64));
         if (Tz > 0) {
             const int ng = (L + Tz - 1) / Tz;
-            hipLaunchKernelGGL(k_io_cn_z_jac, dim3(ng * M, 3), dim3(Tz * NTz), sizeof(double) * (4 * Tz * NTz + 1),
+            hipLaunchKernelGGL(k_io_cn_z_jac<W>, dim3(ng * M, 3), dim3(Tz * NTz), sizeof(double) * (4 * Tz * NTz + 1),
                                m->stream, d, facoe);
         } else {
-            hipLaunchKernelGGL(k_io_cn_z, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
+            hipLaunchKernelGGL(k_io_cn_z<W>, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
         }
         const int NTy = (M + P - 1) / P;   // y: block = Ty x-columns x NTy chunks, whole waves
-        int Ty = NTy <= 1024 ? std::min(m->cn_cols, 1024 / NTy) : 0;
+        int Ty = NTy <= 1024 ? std::min(m->cny_cols ? m->cny_cols : m->cn_cols, 1024 / NTy) : 0;
         if (Ty > 0) Ty -= Ty % (64 / std::gcd(NTy, 64));
         if (Ty > 0) {
             const int ng = (L + Ty - 1) / Ty;
-            hipLaunchKernelGGL(k_io_cn_y_jac, dim3(ng * N, 3), dim3(Ty * NTy), sizeof(double) * (4 * Ty * NTy + 1),
+            hipLaunchKernelGGL(k_io_cn_y_jac<W>, dim3(ng * N, 3), dim3(Ty * NTy), sizeof(double) * (4 * Ty * NTy + 1),
                                m->stream, d, gs, facoe);
         } else {
-            hipLaunchKernelGGL(k_io_cn_y, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
+            hipLaunchKernelGGL(k_io_cn_y<W>, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
         }
     } else {
-        hipLaunchKernelGGL(k_io_update_g, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
+        hipLaunchKernelGGL(k_io_update_g<W>, gall, blk, 0, m->stream, d, gs, dpg_dev, dpg);
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+int chs_substep_io_momentum(chs_io *m, int ns, double dt)
+{
+    if (!m || ns < 1 || ns > 3 || !(dt > 0)) return io_fail(m, CHS_E_ARG, "chs_substep_io_momentum: bad argument");
+    // G0_io <- G_io (SOLVERRK3_MOM_IO.f90:25) by exchanging the two arrays: the terms read the
+    // old G from g0 and k_io_update_g / k_io_cn_y write every row of the new G.
+    std::swap(m->d.g, m->d.g0);
+    const double *gs = m->d.g0;
+    const double tg = m->tgam[ns], tr = m->troh[ns], ta = m->talp[ns];
+    if (m->mixed) io_momentum_launch<float>(m, gs, tg, tr, ta, dt);
+    else io_momentum_launch<double>(m, gs, tg, tr, ta, dt);
     return io_hip(m, hipSuccess, "chs_substep_io_momentum");
 }
 
@@ -1904,6 +1991,7 @@ int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, do
     const IoDev &d = m->d;
     std::vector<double> ph;
     hipError_t he = hipSuccess;
+    if (explt0 && m->mixed) he = io_convert(d.explt0f, d.explt0, 3 * cs, m->stream);   // the FP32 history
     struct { double *dst; const double *src; size_t bytes; } cp[] = {
         {g, d.g, 3 * hb}, {q, d.q, 3 * hb}, {pr, d.pr, hb}, {explt0, d.explt0, 3 * cb},
         {drho, d.drho, cb}, {rhsp, d.rhsp, cb}, {dstg, d.dstg, 3 * hb}, {mstg, d.mstg, 3 * hb}};

@@ -499,6 +499,14 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt);
  * or value. */
 int chs_io_set_tuning(chs_io *m, const char *key, int value);
 
+/* Mixed form of the main-domain momentum substep (mixed = 1): EXPLT0_io (the RK history of
+ * the explicit terms) and the momentum increment RHS_io, which the line solves transform in
+ * place, are kept in FP32; the terms, the solves and every state and property field stay
+ * FP64.  mixed = 0 returns to the plain FP64 form.  The current EXPLT0 carries over (rounded
+ * to FP32, or widened back); uploads and downloads keep FP64 arrays.  No counterpart in the
+ * reference, which computes in FP64 throughout.  Synchronous; CHS_E_ARG for another value. */
+int chs_io_set_mixed(chs_io *m, int mixed);
+
 /* Copy fields back (NULL skips one): g, q, dstg, mstg [3][N][M+2][L]; pr, dph [N][M+2][L]
  * (dph's ghost rows copy the wall cells, BC_WALL_DPH_io); explt0 [3][N][M][L]; drho, rhsp
  * [N][M][L].  Synchronous. */

@@ -480,3 +480,63 @@ def test_gpu_cn_x_wave_equals_block_solve(pkg):
                 assert np.array_equal(out[1][k], v), (L, k)
             else:
                 assert rel(out[1][k], v) < 1e-13, (L, k, rel(out[1][k], v))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vis", [1, 0])
+def test_gpu_io_mixed_form(pkg, vis):
+    """Mixed form (chs_io_set_mixed: EXPLT0 and the momentum increment in FP32) against the
+    plain FP64 form and the oracle over three RK3 steps from a seeded state with a noisy
+    viscosity: the error is that of rounding the increments to FP32 (bounded here by 1e-6
+    relative in G and 1e-4 in p); switching to the mixed form carries EXPLT0 over, and an
+    upload into a mixed context reads the FP64 EXPLT0 given."""
+    L, M, N = 64, 48, 32
+    print(f"io mixed vis {vis}", flush=True)
+    mesh = setup.Mesh(L, M, N, STR2=2.0)
+    st = oracle.io_initial_field(mesh, 77)
+    if vis == 1:
+        st = dict(st, M1=cn_viscosity(st, 77, False))
+    o = oracle.OracleIO(mesh, 4000.0, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=vis)
+    o.set_state(st)
+    out = {}
+    for mixed in (False, True):
+        ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=1, f_a=0.02, flow_driven=1,
+                           vis_scheme=vis)
+        ch.upload(**st)
+        if mixed:
+            ch.set_mixed(True)       # EXPLT0 carried over
+        for step in range(3):
+            for ns in (1, 2, 3):
+                if not mixed:
+                    o.props(ns)
+                    o.substep(ns, 5e-4)
+                ch.ioprops(ns)
+                ch.SOLVERRK3_MOM_io(ns, 5e-4)
+        out[mixed] = ch.download()
+        ch.close()
+    p, m = out[False], out[True]
+    errs = {f"G{c}": rel(m["G"][c], p["G"][c]) for c in range(3)}
+    errs["PR"] = rel(m["PR"], p["PR"])
+    print("mixed vs plain", errs, flush=True)
+    for c in range(3):
+        assert rel(p["G"][c], o.f["G"][c]) < 1e-11
+        assert errs[f"G{c}"] < 1e-6, errs
+        assert rel(m["G"][c], o.f["G"][c]) < 1e-6
+        assert errs[f"G{c}"] > 0.0   # the FP32 path ran
+    assert errs["PR"] < 1e-4, errs
+    # an upload into a context already in the mixed form
+    ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=1, f_a=0.02, flow_driven=1,
+                       vis_scheme=vis)
+    ch.set_mixed(True)
+    ch.upload(**st)
+    for ns in (1, 2, 3):
+        ch.ioprops(ns)
+        ch.SOLVERRK3_MOM_io(ns, 5e-4)
+    f = ch.download()
+    ch.set_mixed(False)
+    ch.close()
+    assert np.isfinite(f["G"]).all()
+    ch2 = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0)
+    with pytest.raises(pkg.ChsError):
+        ch2._check(pkg.lib().chs_io_set_mixed(ch2._h, 2), "chs_io_set_mixed")
+    ch2.close()
