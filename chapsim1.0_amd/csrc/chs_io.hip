@@ -921,6 +921,21 @@ __device__ __forceinline__ double block_max_nonneg(double v, unsigned long long 
 // the least count with rho^(K+1) <= 1e-17, e.g. 3 sweeps at rho = 1e-4.  The line kernels below
 // solve this way and hand a block whose rho exceeds 1/4 to the thread-per-line recurrences
 // (io_cn_x_line, io_cn_z_line, io_cn_y_line).
+// 1/x from the hardware estimate and two Newton-Raphson steps r <- r + r (1 - x r), each of
+// which doubles the correct bits: within about an ulp of the correctly rounded reciprocal.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// a / b for the Jacobi line solves' coefficients: the IEEE quotient, or with FD a times
+// rcp_nr(b) (one reciprocal per density value, shared by the neighbouring rows' coefficients).
+// The sweeps approximate the exact solve to 1e-17 relative, so an ulp in a coefficient is
+// below what they resolve.
+template <bool FD> __device__ __forceinline__ double qd(double a, double b) { return FD ? a * rcp_nr(b) : a / b; }
+
 __device__ __forceinline__ int jacobi_sweeps(double rho)
 {
     int K = 0;
@@ -993,7 +1008,7 @@ __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double fa
 // l P .. l P + P - 1; the neighbours across lanes come by cross-lane shuffles, so a line needs
 // no block barrier and a wavefront's loads are all issued together.  Coefficients and sweeps
 // as io_cn_x_jac (rho is the line's own maximum).
-template <int C, int P, typename W>
+template <int C, int P, typename W, bool FD>
 __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double facoe)
 {
 #pragma clang fp contract(off)
@@ -1032,10 +1047,10 @@ __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double f
             amu = -coe2 * f0[j];
             apu = -coe2 * vp;
         }
-        a[j] = amu / dm;
-        c[j] = apu / dq;
-        const double b = 1.0 - (amu + apu) / f1[j];
-        ib[j] = 1.0 / b;
+        a[j] = qd<FD>(amu, dm);
+        c[j] = qd<FD>(apu, dq);
+        const double b = 1.0 - qd<FD>(amu + apu, f1[j]);
+        ib[j] = qd<FD>(1.0, b);
         rho = nanmax(rho, (fabs(a[j]) + fabs(c[j])) * fabs(ib[j]));
     }
     for (int o = 32; o > 0; o >>= 1) rho = nanmax(rho, __shfl_xor(rho, o));
@@ -1062,13 +1077,13 @@ __device__ __forceinline__ void io_cn_x_wave(const IoDev &d, double dp, double f
     for (int j = 0; j < P; ++j) R[j] = (W)x[j];
 }
 
-template <int P, typename W>
+template <int P, typename W, bool FD>
 __global__ __launch_bounds__(256) void k_io_cn_x_wave(IoDev d, const double *__restrict__ dpg_dev, double dpg,
                                                       double facoe)
 {
-    if (blockIdx.y == 0) io_cn_x_wave<0, P, W>(d, dpg_dev ? dpg_dev[0] : dpg, facoe);
-    else if (blockIdx.y == 1) io_cn_x_wave<1, P, W>(d, 0.0, facoe);
-    else io_cn_x_wave<2, P, W>(d, 0.0, facoe);
+    if (blockIdx.y == 0) io_cn_x_wave<0, P, W, FD>(d, dpg_dev ? dpg_dev[0] : dpg, facoe);
+    else if (blockIdx.y == 1) io_cn_x_wave<1, P, W, FD>(d, 0.0, facoe);
+    else io_cn_x_wave<2, P, W, FD>(d, 0.0, facoe);
 }
 
 template <typename W>
@@ -1199,7 +1214,7 @@ __global__ __launch_bounds__(128) void k_io_cn_y(IoDev d, const double *__restri
 // x-columns of one row J, thread (lt, chunk) at lt + T chunk, so a wave reads runs of T
 // consecutive x; only the chunk edges go through LDS (coefficient neighbours, then the Jacobi
 // sweeps' neighbours).
-template <int C, int P, typename W>
+template <int C, int P, typename W, bool FD>
 __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double *lds)
 {
 #pragma clang fp contract(off)
@@ -1248,14 +1263,14 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
             amu = -coe2 * f0[j];
             apu = -coe2 * vp;
         }
-        a[j] = amu / dm;
-        c[j] = apu / dq;
-        double b = 1.0 - (amu + apu) / f1[j];
+        a[j] = qd<FD>(amu, dm);
+        c[j] = qd<FD>(apu, dq);
+        double b = 1.0 - qd<FD>(amu + apu, f1[j]);
         if (!valid) {
             a[j] = c[j] = 0.0;
             b = 1.0;
         }
-        ib[j] = 1.0 / b;
+        ib[j] = qd<FD>(1.0, b);
         rho = nanmax(rho, (fabs(a[j]) + fabs(c[j])) * fabs(ib[j]));
     }
     rho = block_max_nonneg(rho, word);   // also orders the edge reads above before the sweeps
@@ -1288,20 +1303,20 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
     }
 }
 
-template <typename W>
+template <typename W, bool FD>
 __global__ __launch_bounds__(1024) void k_io_cn_z_jac(IoDev d, double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_z_jac<0, 4, W>(d, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_z_jac<1, 4, W>(d, facoe, lds);
-    else io_cn_z_jac<2, 4, W>(d, facoe, lds);
+    if (blockIdx.y == 0) io_cn_z_jac<0, 4, W, FD>(d, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_z_jac<1, 4, W, FD>(d, facoe, lds);
+    else io_cn_z_jac<2, 4, W, FD>(d, facoe, lds);
 }
 
 // MOMFA3_Y_io + the G update with P consecutive rows per thread (NT = ceil(M / P) chunks per
 // column, rows past M are identity rows): a block holds T x-columns of one z-plane, thread
 // (lt, chunk) at lt + T chunk; coefficients as io_cn_y_line builds them; writes every row of
 // the new G of its columns.
-template <int C, int P, typename W>
+template <int C, int P, typename W, bool FD>
 __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__restrict__ gs, double facoe, double *lds)
 {
 #pragma clang fp contract(off)
@@ -1356,30 +1371,30 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
                     const double muqt = (d.m1[hx(d, i, 1, k)] + d.m1[hx(d, i, 0, k)]) * 0.5;
                     amu = -coe2 * d.amvr1 * muqt;
                     apu = -coe2 * ap[J] * fp;
-                    cc = apu / dq;
+                    cc = qd<FD>(apu, dq);
                 } else if (J == M) {
                     const double muqt = (d.m1[hx(d, i, M, k)] + d.m1[hx(d, i, M + 1, k)]) * 0.5;
                     amu = -coe2 * am[J] * f0[j];
                     apu = -coe2 * d.apvrn * muqt;
-                    aa = amu / dm;
+                    aa = qd<FD>(amu, dm);
                 } else {
                     amu = -coe2 * am[J] * f0[j];
                     apu = -coe2 * ap[J] * fp;
-                    aa = amu / dm;
-                    cc = apu / dq;
+                    aa = qd<FD>(amu, dm);
+                    cc = qd<FD>(apu, dq);
                 }
             } else {
                 const double coe1 = facoe * 4.0 / 3.0;
                 amu = -coe1 * am[J] * fm;
                 apu = -coe1 * ap[J] * f0[j];
-                aa = J == 1 ? 0.0 : amu / dm;
-                cc = apu / (J == 1 ? dq : f1[j]);
+                aa = J == 1 ? 0.0 : qd<FD>(amu, dm);
+                cc = qd<FD>(apu, J == 1 ? dq : f1[j]);
             }
-            b = 1.0 - (amu + apu) / f1[j];
+            b = 1.0 - qd<FD>(amu + apu, f1[j]);
         }
         a[j] = aa;
         c[j] = cc;
-        ib[j] = 1.0 / b;
+        ib[j] = qd<FD>(1.0, b);
         rho = nanmax(rho, (fabs(aa) + fabs(cc)) * fabs(ib[j]));
     }
     rho = block_max_nonneg(rho, word);
@@ -1422,13 +1437,13 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
     }
 }
 
-template <typename W>
+template <typename W, bool FD>
 __global__ __launch_bounds__(1024) void k_io_cn_y_jac(IoDev d, const double *__restrict__ gs, double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_y_jac<0, 4, W>(d, gs, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_y_jac<1, 4, W>(d, gs, facoe, lds);
-    else io_cn_y_jac<2, 4, W>(d, gs, facoe, lds);
+    if (blockIdx.y == 0) io_cn_y_jac<0, 4, W, FD>(d, gs, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_y_jac<1, 4, W, FD>(d, gs, facoe, lds);
+    else io_cn_y_jac<2, 4, W, FD>(d, gs, facoe, lds);
 }
 
 // DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
@@ -1593,7 +1608,9 @@ struct chs_io {
     int mom_by = 4;     // rows per block of the LDS-staged momentum kernel (4 or 8)
     int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
     int cnz_cols = 0, cny_cols = 0;   // cn_cols for the z or the y solve alone (0: cn_cols)
-    int cnx_wave = 1;   // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
+    int cnx_wave = 1;
+    int cn_fastdiv = 1;   // z and y Jacobi coefficients by Newton reciprocals (the sweeps resolve
+                          // no finer; 0.26 -> 0.24 and 0.29 -> 0.28 ms per stage at 256 x 128 x 256)   // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
     int cn_cols = 16;   // most x-columns per block of the z and y Jacobi line solves (16: 1.7 %
                         // faster per heated step than 64 at 256 x 128 x 256)
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
@@ -1853,6 +1870,11 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
         (k == "cnz_cols" ? m->cnz_cols : m->cny_cols) = value;
         return CHS_OK;
     }
+    if (k == "cn_fastdiv") {
+        if (value != 0 && value != 1) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: cn_fastdiv must be 0 or 1");
+        m->cn_fastdiv = value;
+        return CHS_OK;
+    }
     if (k == "cnx_wave") {
         if (value != 0 && value != 1) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: cnx_wave must be 0 or 1");
         m->cnx_wave = value;
@@ -1911,10 +1933,10 @@ void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, doubl
         const int PX = L / 64;   // x-line points per lane of the wavefront-per-line solve
         if (m->cnx_wave && L % 64 == 0 && (PX == 1 || PX == 2 || PX == 4 || PX == 8)) {
             const dim3 gx((M * N + 3) / 4, 3);
-            if (PX == 1) hipLaunchKernelGGL((k_io_cn_x_wave<1, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
-            else if (PX == 2) hipLaunchKernelGGL((k_io_cn_x_wave<2, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
-            else if (PX == 4) hipLaunchKernelGGL((k_io_cn_x_wave<4, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
-            else hipLaunchKernelGGL((k_io_cn_x_wave<8, W>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            if (PX == 1) hipLaunchKernelGGL((k_io_cn_x_wave<1, W, false>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else if (PX == 2) hipLaunchKernelGGL((k_io_cn_x_wave<2, W, false>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else if (PX == 4) hipLaunchKernelGGL((k_io_cn_x_wave<4, W, false>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
+            else hipLaunchKernelGGL((k_io_cn_x_wave<8, W, false>), gx, dim3(256), 0, m->stream, d, dpg_dev, dpg, facoe);
         } else if ((L & (L - 1)) == 0 && L >= 4 && L <= 512) {
             const int T = L >= 256 ? 1 : 256 / L;   // x-lines per block
             hipLaunchKernelGGL(k_io_cn_x_jac<W>, dim3((M * N + T - 1) / T, 3), dim3(T * L),
@@ -1929,7 +1951,7 @@ void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, doubl
         if (Tz > 0) Tz -= Tz % (64 / std::gcd(NTz, 64));
         if (Tz > 0) {
             const int ng = (L + Tz - 1) / Tz;
-            hipLaunchKernelGGL(k_io_cn_z_jac<W>, dim3(ng * M, 3), dim3(Tz * NTz), sizeof(double) * (4 * Tz * NTz + 1),
+            hipLaunchKernelGGL((m->cn_fastdiv ? k_io_cn_z_jac<W, true> : k_io_cn_z_jac<W, false>), dim3(ng * M, 3), dim3(Tz * NTz), sizeof(double) * (4 * Tz * NTz + 1),
                                m->stream, d, facoe);
         } else {
             hipLaunchKernelGGL(k_io_cn_z<W>, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
@@ -1939,7 +1961,7 @@ void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, doubl
         if (Ty > 0) Ty -= Ty % (64 / std::gcd(NTy, 64));
         if (Ty > 0) {
             const int ng = (L + Ty - 1) / Ty;
-            hipLaunchKernelGGL(k_io_cn_y_jac<W>, dim3(ng * N, 3), dim3(Ty * NTy), sizeof(double) * (4 * Ty * NTy + 1),
+            hipLaunchKernelGGL((m->cn_fastdiv ? k_io_cn_y_jac<W, true> : k_io_cn_y_jac<W, false>), dim3(ng * N, 3), dim3(Ty * NTy), sizeof(double) * (4 * Ty * NTy + 1),
                                m->stream, d, gs, facoe);
         } else {
             hipLaunchKernelGGL(k_io_cn_y<W>, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
