@@ -557,8 +557,12 @@ template <bool CN> struct IoStage {
     static __device__ __forceinline__ constexpr int carry_dj(int c) { return (c == 3 || c == 8) ? 1 : 0; }
 };
 
+// Tile width of the staged planes: columns i0-2 .. i0+65 (the halo column each side and one
+// more on the left, so that a tile row is 34 aligned pairs of columns, one 16-byte load each).
+constexpr int IO_TW = 68;
+
 template <bool CN, int BY, typename W> struct IoAtLds {
-    static constexpr int TW = 66, TS = (BY + 2) * TW;
+    static constexpr int TW = IO_TW, TS = (BY + 2) * TW;
     const double *c0, *c1;   // this point's cell in the tiles of planes k and k+1
     const double *cr;        // the carried plane-(k-1) values
     const double *xr;        // EXPLT0 of the three components and the buoyancy component's D_STG
@@ -593,8 +597,9 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
                                                          double ta, double dt, int KZ, int nxcd)
 {
     using S = IoStage<CN>;
-    constexpr int NT = 64 * BY, TW = 66, TS = (BY + 2) * TW, NF = S::NF, NCL = (TS + NT - 1) / NT;
-    __shared__ double tiles[2][NF * TS];
+    constexpr int NT = 64 * BY, TW = IO_TW, TS = (BY + 2) * TW, NF = S::NF, NCH = TS / 2;
+    static_assert(NCH <= NT, "one column pair per thread");
+    __shared__ __align__(16) double tiles[2][NF * TS];
     __shared__ double sh[NT];
     const int L = d.L, M = d.M, N = d.N;
     const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), t = ty * 64 + tx;
@@ -605,17 +610,18 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
     const int i = i0 + tx, J = J0 + ty;
     const bool act = i < L && J <= M;
     const size_t hs = hsz(d), ph = (size_t)(M + 2) * L, pc = (size_t)M * L;
-    // in-plane offsets of the tile cells this thread stages; a row past the field's last (or
-    // before a cell field's first) reads the nearest row instead, which no point of the block
-    // uses, so that every staging load is unconditional
-    unsigned oh[NCL], oc[NCL];
-#pragma unroll
-    for (int s = 0; s < NCL; ++s) {
-        const int n = t + s * NT, r = n / TW, c = n % TW, jj = J0 - 1 + r;
-        int ii = i0 - 1 + c;
-        ii = ii < 0 ? L - 1 : ii % L;
-        oh[s] = (unsigned)(min(jj, M + 1) * L + ii);
-        oc[s] = (unsigned)((min(max(jj, 1), M) - 1) * L + ii);
+    // in-plane offset of the pair of tile columns this thread stages (pair q of row r: columns
+    // i0-2+2q and the next, x periodic; L is even, so a pair never straddles the wrap and
+    // starts 16-byte aligned); a row past the field's last (or before a cell field's first)
+    // reads the nearest row instead, which no point of the block uses, so that every staging
+    // load is unconditional
+    const bool stg = t < NCH;
+    unsigned oh, oc;
+    {
+        const int n = stg ? t : 0, r = n / (TW / 2), q = n % (TW / 2), jj = J0 - 1 + r;
+        const int ii = ((i0 - 2 + 2 * q) % L + L) % L;
+        oh = (unsigned)(min(jj, M + 1) * L + ii);
+        oc = (unsigned)((min(max(jj, 1), M) - 1) * L + ii);
     }
     auto base = [&](int f) -> const double * {
         switch (f) {
@@ -634,25 +640,24 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
         default: return d.divu;
         }
     };
-    double pre[NF][NCL];
+    double pre0[NF], pre1[NF];   // the staged pair (two arrays: a double2 array went to scratch)
     auto load = [&](int kk) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             const int fld = S::field(f);
             const bool cell = fld == FDU;
             const double *b = base(fld) + (size_t)kk * (cell ? pc : ph);
-#pragma unroll
-            for (int s = 0; s < NCL; ++s) pre[f][s] = b[cell ? oc[s] : oh[s]];
+            const double2 v = *reinterpret_cast<const double2 *>(b + (cell ? oc : oh));
+            pre0[f] = v.x;
+            pre1[f] = v.y;
         }
     };
     auto store = [&](int sl) {
+        if (stg) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f)
-#pragma unroll
-            for (int s = 0; s < NCL; ++s) {
-                const int n = t + s * NT;
-                if (n < TS) tiles[sl][f * TS + n] = pre[f][s];
-            }
+            for (int f = 0; f < NF; ++f)
+                reinterpret_cast<double2 *>(&tiles[sl][f * TS])[t] = make_double2(pre0[f], pre1[f]);
+        }
     };
     const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
     double cr[S::NC];
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *
     load(k0 + 1 == N ? 0 : k0 + 1);
     store(1);
     __syncthreads();
-    const int cell = (ty + 1) * TW + tx + 1;
+    const int cell = (ty + 1) * TW + tx + 2;
     double contrib = 0.0;
     for (int k = k0; k < k1; ++k) {
         const int cur = (k - k0) & 1;
@@ -1604,7 +1609,7 @@ struct chs_io {
     double *buf = nullptr;
     int npart = 0, flow_driven = 1;
     int mom_rows = 4;   // rows per momentum block (4, 2 or 1; the rest of its 4 rows of threads are z-planes)
-    int mom_kz = 16;    // z-planes per block of the LDS-staged momentum kernel (0: the per-point kernel)
+    int mom_kz = 32;    // z-planes per block of the LDS-staged momentum kernel (0: the per-point kernel)
     int mom_by = 4;     // rows per block of the LDS-staged momentum kernel (4 or 8)
     int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
     int cnz_cols = 0, cny_cols = 0;   // cn_cols for the z or the y solve alone (0: cn_cols)
@@ -1901,7 +1906,11 @@ void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, doubl
     const dim3 blk(64, 4), gall = io_grid(m, m->M + 2), gin = io_grid(m, m->M);
     if (!m->q_fresh) hipLaunchKernelGGL(k_io_velocity, gall, blk, 0, m->stream, d, gs);
     m->q_fresh = false;
-    const int TY = m->mom_rows, TZ = 4 / TY, KZ = m->mom_kz, BY = m->mom_by, NX = m->mom_nxcd;
+    // the LDS-staged kernel loads column pairs as 16-byte words: every staged field 16-byte aligned
+    auto a16 = [](const void *q) { return q == nullptr || ((uintptr_t)q & 15) == 0; };
+    const bool aligned = m->L % 2 == 0 && a16(gs) && a16(d.q) && a16(d.mstg) && a16(d.m0) && a16(d.m1) &&
+                         a16(d.pr) && a16(d.divu);
+    const int TY = m->mom_rows, TZ = 4 / TY, KZ = aligned ? m->mom_kz : 0, BY = m->mom_by, NX = m->mom_nxcd;
     const dim3 gmom = KZ > 0 ? dim3((m->L + 63) / 64, (m->M + BY - 1) / BY, (m->N + KZ - 1) / KZ)
                              : dim3((m->L + 63) / 64, (m->M + TY - 1) / TY, (m->N + TZ - 1) / TZ);
     const int npart = (int)(gmom.x * gmom.y * gmom.z);   // at most the 64 x 4 row-block count

@@ -489,7 +489,7 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt);
 
 /* Launch shape of the main-domain momentum kernel, as chs_set_tuning is for the turbulence
  * generator; no counterpart in the reference.  Key "mom_kz": z-planes each block of the
- * LDS-staged momentum kernel marches (default 16; 0 selects the per-point kernel);
+ * LDS-staged momentum kernel marches (default 32; 0 selects the per-point kernel);
  * "mom_by": its rows per block (4 default, or 8); "mom_nxcd": XCDs its tiles are grouped for
  * (default 8; 1 keeps launch order); "mom_rows": rows per 64 x 4 block of the per-point kernel (4
  * default, 2, 1; the block's other rows of threads take further z-planes); "cn_cols": most
