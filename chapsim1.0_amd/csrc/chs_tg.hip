@@ -2000,12 +2000,22 @@ __device__ __forceinline__ void k_update_b(Geo g, Sub s, const double *__restric
         if (t == 0) part[((size_t)blk.z * blk.gy + blk.y) * blk.gx + blk.x] = red[0];
     }
 }
+// Tiles dealt to the XCDs in contiguous runs (nxcd > 1), so that the rows a tile reads from
+// the tiles above and below it (phi at J-1, v* at J+1) come through the same L2.
 template <bool CFL>
 __global__ __launch_bounds__(1024) void k_update(Geo g, Sub s, const double *__restrict__ phi,
                                                 const double *__restrict__ qs, double *__restrict__ q,
-                                                double *__restrict__ pr, int KZ, double *__restrict__ part)
+                                                double *__restrict__ pr, int KZ, double *__restrict__ part,
+                                                int nxcd)
 {
-    k_update_b<CFL>(g, s, phi, qs, q, pr, KZ, part, own_blk());
+    Blk b = own_blk();
+    if (nxcd > 1) {
+        const unsigned t = (unsigned)__builtin_amdgcn_readfirstlane(xcd_tile(nxcd, b));
+        b.x = t % b.gx;
+        b.y = (t / b.gx) % b.gy;
+        b.z = t / (b.gx * b.gy);
+    }
+    k_update_b<CFL>(g, s, phi, qs, q, pr, KZ, part, b);
 }
 
 // A refinement sweep's correction, in place: the velocity as k_update, p += phi (PRCALC_tg's
@@ -2570,6 +2580,8 @@ struct chs_ctx {
         int cny3_ky = 48;                   // k_cn_y3_march rows per chunk (0 = M; -1 = one launch per component)
         int upd_kz = 8;                     // k_update z-planes per thread
         int upd_by = 4;                     // k_update rows per block (1, 2, 4, 8, 16)
+        int upd_nxcd = 1;                   // k_update tiles grouped for this many XCDs (1: launch order;
+                                            // 8 measured equal at 256x192x256)
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning / chs_set_refinement (graphs are re-made)
     int cn_gen = 0;     // bumped by every rebuild of a Crank-Nicolson table (its memory may move)
@@ -3177,10 +3189,10 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
         EvScope ev(c, "update");
         if (cfl && nref == 0)
             hipLaunchKernelGGL(k_update<true>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
-                               c->partial);
+                               c->partial, c->tune.upd_nxcd);
         else
             hipLaunchKernelGGL(k_update<false>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
-                               c->partial);
+                               c->partial, c->tune.upd_nxcd);
     }
     for (int r = 0; r < nref; ++r) {
         poisson_launch<W>(c, c->dstep, c->talp[ns], nullptr, 0, c->q, false);
@@ -3844,6 +3856,9 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "upd_by") {
         if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return bad("must be 1, 2, 4, 8 or 16");
         c->tune.upd_by = value;
+    } else if (k == "upd_nxcd") {
+        if (value < 1 || value > 64) return bad("must be 1..64");
+        c->tune.upd_nxcd = value;
     } else if (k == "upd_kz") {
         if (value < 1 || value > N) return bad("must be 1..N");
         c->tune.upd_kz = value;

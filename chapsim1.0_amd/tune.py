@@ -19,7 +19,7 @@ KERNELS = {
     "fft_reg": ("divg_fftx", "fftz_fwd", "fftz_inv", "fftx_inv"),
     "ytdma_bs": ("ytdma",), "ytdma_rb": ("ytdma",), "ytdma_c": ("ytdma",), "conv_lds": ("conv_rhs", "cn_x"),
     "conv_dma": ("conv_rhs", "cn_x"), "cny3_ky": ("cn_y",), "divg_nxcd": ("divg_fftx",),
-    "upd_kz": ("update",), "upd_by": ("update",),
+    "upd_kz": ("update",), "upd_by": ("update",), "upd_nxcd": ("update",),
 }
 
 
@@ -49,6 +49,7 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "divg_nxcd": [1, 8],
         "upd_kz": [k for k in (4, 8, 16) if k <= N],
         "upd_by": [1, 2, 4, 8, 16],
+        "upd_nxcd": [1, 8],
     }
     return {k: v for k, v in c.items() if v}
 

@@ -151,7 +151,12 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * short, 0 for the separate x kernel (with conv_fx > 0, conv_bx / conv_by do not apply; the
  * mass-flux correction moves after the z-solve, which changes rounding only), "ytdma_bs"
  * threads per block of the Poisson y-solve (64 default, 128, 256) and "ytdma_rb" its rows per
- * load batch (8, 16 default).  Results do not
+ * load batch (8, 16 default), "ytdma_c" rows per chunk of the chunked y-Thomas (-1 = chosen by
+ * the mesh, 0 = unchunked), "conv_dma" rows per block of the direct-to-LDS convection kernel
+ * (4 default, 2; 0 = off), "cny3_ky" rows per chunk of the fused three-component y-solve (48;
+ * 0 = M, -1 = one launch per component), "divg_nxcd" and "upd_nxcd" the XCDs the divergence
+ * transform's and the correction's tiles are grouped for (8 and 1; 1 = dispatch order), "upd_kz"
+ * z-planes each correction thread marches (8) and "upd_by" its rows per block (4).  Results do not
  * depend on the shapes except through the summation order of the mass-flux partials
  * (conv_by, conv_kc, conv_fx) and conv_fx's rounding.  A captured step graph is re-made on
  * the next step.  CHS_E_ARG for an unknown key or a value out of range. */
