@@ -482,8 +482,11 @@ def bench_heated(pkg, args):
         eng.upload(dh, g)
         f0 = eng.download()
         io = pkg.ChannelIO(co, c["REN"], flow_driven=1, vis_scheme=vis)
-        if io_shapes:
-            io.set_tuning(**io_shapes)
+        eshapes = {k: v for k, v in io_shapes.items() if k.startswith("rhs_")}   # energy-substep keys
+        if eshapes:
+            eng.set_tuning(**eshapes)
+        if len(eshapes) < len(io_shapes):
+            io.set_tuning(**{k: v for k, v in io_shapes.items() if k not in eshapes})
         io.set_mixed(mixed)
         io.upload(G=g, PR=pr, D0=f0["D"], M0=f0["M"])
 

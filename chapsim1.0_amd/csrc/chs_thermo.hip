@@ -200,75 +200,218 @@ __device__ __forceinline__ size_t ehx(const EngDev &e, int i, int J, int k)
     return ((size_t)k * (e.M + 2) + J) * e.L + i;
 }
 
-// RHS_ENERGY_EXPLICIT (RHS_ENERGY_EXPLICIT.f90:43-150) and the RK update with the table
-// limiter (SOLVERRK3_ENG_IO.f90:35-47): DH += ca R + cb R0, clamp, R0 = R.  Each thread
-// marches KC z-planes: H, K, T at (i, J) of plane k and the plane-(k+1) values of H, K, T,
-// G_z read for plane k's z-differences stay in registers for the next plane.
-__global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, double cb, int KC)
+// Row-J coefficients of RHS_ENERGY_EXPLICIT (RHS_ENERGY_EXPLICIT.f90:43-150).
+struct EngRow {
+    double coe1, coe11, coe2, coe3, coe33, coe22, coe221, coe222, wffp, wfbp, wff, wfb;
+};
+__device__ __forceinline__ EngRow eng_row(const EngDev &e, int J)
 {
 #pragma clang fp contract(off)
+    const int M = e.M;
+    const double one = 1.0, xnd2cl = 0.5, znd2cl = 0.5;
+    EngRow c;
+    c.coe1 = e.dxi * xnd2cl * 0.5;
+    c.coe11 = e.dxqi * xnd2cl * e.cthecd;
+    c.coe2 = e.dyfi[J] * one * 0.5;
+    c.coe3 = e.dzi * znd2cl * one * 0.5;
+    c.coe33 = e.dzqi * znd2cl * one * e.cthecd;
+    c.coe22 = e.dyfi[J] * one * e.cthecd;
+    if (J == 1) {
+        c.coe221 = e.dyci[J + 1] / one;
+        c.coe222 = e.wall_type0 == 1 ? 1.0 / one : e.dyci[J] / one;
+    } else if (J == M) {
+        c.coe222 = e.dyci[J] / one;
+        c.coe221 = e.wall_type1 == 1 ? 1.0 / one : e.dyci[J + 1] / one;
+    } else {
+        c.coe221 = e.dyci[J + 1] / one;
+        c.coe222 = e.dyci[J] / one;
+    }
+    c.wffp = e.wff[J + 1]; c.wfbp = e.wfb[J + 1]; c.wff = e.wff[J]; c.wfb = e.wfb[J];
+    return c;
+}
+
+// Fields the energy stencil reads in plane k: X(f, di, dj) = f at (i + di, J + dj, k).
+enum EngF { EH, EK, ET, EG1, EG2 };
+
+// R of RHS_ENERGY_EXPLICIT at (i, J, k) of row J < or = M: convection of H with G and the
+// conduction K grad T; the z-neighbours come as values (m: k-1, c: k, p: k+1), wfl is the
+// iso-flux wall's heat flux at (i, k) where row J is a wall row of that kind.
+template <typename A>
+__device__ __forceinline__ double eng_rhs_point(const EngDev &e, const EngRow &c, const A &X, int J, double Hm,
+                                                double Hc, double Hp, double Km, double Kc, double Kp, double Tm,
+                                                double Tc, double Tp, double G3c, double G3p, double wfl)
+{
+#pragma clang fp contract(off)
+    const int M = e.M;
+    double dx_dhu = (X(EH, 1, 0) + Hc) * (X(EG1, 1, 0) + X(EG1, 1, 0)) - (Hc + X(EH, -1, 0)) * (X(EG1, 0, 0) + X(EG1, 0, 0));
+    dx_dhu = dx_dhu * c.coe1;
+    double dy_dhv = (c.wffp * X(EH, 0, 1) + c.wfbp * Hc) * (X(EG2, 0, 1) + X(EG2, 0, 1)) -
+                    (c.wff * Hc + c.wfb * X(EH, 0, -1)) * (X(EG2, 0, 0) + X(EG2, 0, 0));
+    dy_dhv = dy_dhv * c.coe2;
+    double dz_dhw = (Hp + Hc) * (G3p + G3p) - (Hc + Hm) * (G3c + G3c);
+    dz_dhw = dz_dhw * c.coe3;
+    double dx_kdt = (X(EK, 1, 0) + Kc) * (X(ET, 1, 0) - Tc) - (Kc + X(EK, -1, 0)) * (Tc - X(ET, -1, 0));
+    dx_kdt = dx_kdt * c.coe11;
+    double kdty1 = (c.wffp * X(EK, 0, 1) + c.wfbp * Kc) * (X(ET, 0, 1) - Tc) * c.coe221;
+    double kdty2 = (c.wff * Kc + c.wfb * X(EK, 0, -1)) * (Tc - X(ET, 0, -1)) * c.coe222;
+    if (J == 1 && e.wall_type0 == 1)
+        kdty2 = -wfl * c.coe222;
+    else if (J == M && e.wall_type1 == 1)
+        kdty1 = -wfl * c.coe221;
+    const double dy_kdt = (kdty1 - kdty2) * c.coe22;
+    double dz_kdt = (Kp + Kc) * (Tp - Tc) - (Km + Kc) * (Tc - Tm);
+    dz_kdt = dz_kdt * c.coe33;
+    return -dx_dhu - dy_dhv - dz_dhw + dx_kdt + dy_kdt + dz_kdt;
+}
+
+// The iso-flux wall's heat flux the point's row reads (0 where the row is no such wall).
+__device__ __forceinline__ double eng_wall_flux(const EngDev &e, int i, int J, int k)
+{
+    if (J == 1 && e.wall_type0 == 1) return e.wallflux[(size_t)k * e.L + i];
+    if (J == e.M && e.wall_type1 == 1) return e.wallflux[((size_t)e.N + k) * e.L + i];
+    return 0.0;
+}
+
+// The RK update with the table limiter (SOLVERRK3_ENG_IO.f90:35-47): DH += ca R + cb R0,
+// clamp, R0 = R.
+__device__ __forceinline__ void eng_update(const EngDev &e, size_t h0, size_t c0, double r, double dh0, double r0,
+                                           double ca, double cb)
+{
+#pragma clang fp contract(off)
+    double dh = dh0 + ca * r + cb * r0;
+    dh = fmin(e.dhmax - 1.0e-14, dh);   // REALMIN (modules.f90:56)
+    dh = fmax(e.dhmin + 1.0e-14, dh);
+    e.dh[h0] = dh;
+    e.rhs0[c0] = r;
+}
+
+// RHS_ENERGY_EXPLICIT (RHS_ENERGY_EXPLICIT.f90:43-150) and the RK update, one thread per
+// (i, J) marching KC z-planes: H, K, T at (i, J) of plane k and the plane-(k+1) values of H,
+// K, T, G_z read for plane k's z-differences stay in registers for the next plane.
+__global__ __launch_bounds__(256) void k_eng_rhs_update(EngDev e, double ca, double cb, int KC)
+{
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y + 1;
     if (i >= e.L || J > e.M) return;
     const int L = e.L, N = e.N, M = e.M;
     const int k0 = blockIdx.z * KC, k1 = min(N, k0 + KC);
     const int ip = (i == L - 1) ? 0 : i + 1, im = (i == 0) ? L - 1 : i - 1;
-    const double one = 1.0, xnd2cl = 0.5, znd2cl = 0.5;
-    const double coe1 = e.dxi * xnd2cl * 0.5;
-    const double coe11 = e.dxqi * xnd2cl * e.cthecd;
-    const double coe2 = e.dyfi[J] * one * 0.5;
-    const double coe3 = e.dzi * znd2cl * one * 0.5;
-    const double coe33 = e.dzqi * znd2cl * one * e.cthecd;
-    const double coe22 = e.dyfi[J] * one * e.cthecd;
-    double coe221, coe222;
-    if (J == 1) {
-        coe221 = e.dyci[J + 1] / one;
-        coe222 = e.wall_type0 == 1 ? 1.0 / one : e.dyci[J] / one;
-    } else if (J == M) {
-        coe222 = e.dyci[J] / one;
-        coe221 = e.wall_type1 == 1 ? 1.0 / one : e.dyci[J + 1] / one;
-    } else {
-        coe221 = e.dyci[J + 1] / one;
-        coe222 = e.dyci[J] / one;
-    }
-    const double wffp = e.wff[J + 1], wfbp = e.wfb[J + 1], wff = e.wff[J], wfb = e.wfb[J];
+    const EngRow c = eng_row(e, J);
     const double *H = e.h, *T = e.t, *K = e.k;
-    auto X = [&](const double *f, int ii, int jj, int kk) { return f[ehx(e, ii, jj, kk)]; };
+    auto F = [&](const double *f, int ii, int jj, int kk) { return f[ehx(e, ii, jj, kk)]; };
     const int kb = k0 == 0 ? N - 1 : k0 - 1;
-    double Hm = X(H, i, J, kb), Km = X(K, i, J, kb), Tm = X(T, i, J, kb);
-    double Hc = X(H, i, J, k0), Kc = X(K, i, J, k0), Tc = X(T, i, J, k0), G3c = X(e.g3, i, J, k0);
+    double Hm = F(H, i, J, kb), Km = F(K, i, J, kb), Tm = F(T, i, J, kb);
+    double Hc = F(H, i, J, k0), Kc = F(K, i, J, k0), Tc = F(T, i, J, k0), G3c = F(e.g3, i, J, k0);
     for (int k = k0; k < k1; ++k) {
         const int kp = (k == N - 1) ? 0 : k + 1;
-        const double Hp = X(H, i, J, kp), Kp = X(K, i, J, kp), Tp = X(T, i, J, kp), G3p = X(e.g3, i, J, kp);
-        double dx_dhu = (X(H, ip, J, k) + Hc) * (X(e.g1, ip, J, k) + X(e.g1, ip, J, k)) -
-                        (Hc + X(H, im, J, k)) * (X(e.g1, i, J, k) + X(e.g1, i, J, k));
-        dx_dhu = dx_dhu * coe1;
-        double dy_dhv = (wffp * X(H, i, J + 1, k) + wfbp * Hc) * (X(e.g2, i, J + 1, k) + X(e.g2, i, J + 1, k)) -
-                        (wff * Hc + wfb * X(H, i, J - 1, k)) * (X(e.g2, i, J, k) + X(e.g2, i, J, k));
-        dy_dhv = dy_dhv * coe2;
-        double dz_dhw = (Hp + Hc) * (G3p + G3p) - (Hc + Hm) * (G3c + G3c);
-        dz_dhw = dz_dhw * coe3;
-        double dx_kdt = (X(K, ip, J, k) + Kc) * (X(T, ip, J, k) - Tc) - (Kc + X(K, im, J, k)) * (Tc - X(T, im, J, k));
-        dx_kdt = dx_kdt * coe11;
-        double kdty1 = (wffp * X(K, i, J + 1, k) + wfbp * Kc) * (X(T, i, J + 1, k) - Tc) * coe221;
-        double kdty2 = (wff * Kc + wfb * X(K, i, J - 1, k)) * (Tc - X(T, i, J - 1, k)) * coe222;
-        if (J == 1 && e.wall_type0 == 1)
-            kdty2 = -e.wallflux[(size_t)k * L + i] * coe222;
-        else if (J == M && e.wall_type1 == 1)
-            kdty1 = -e.wallflux[((size_t)N + k) * L + i] * coe221;
-        const double dy_kdt = (kdty1 - kdty2) * coe22;
-        double dz_kdt = (Kp + Kc) * (Tp - Tc) - (Km + Kc) * (Tc - Tm);
-        dz_kdt = dz_kdt * coe33;
-        const double r = -dx_dhu - dy_dhv - dz_dhw + dx_kdt + dy_kdt + dz_kdt;
+        const double Hp = F(H, i, J, kp), Kp = F(K, i, J, kp), Tp = F(T, i, J, kp), G3p = F(e.g3, i, J, kp);
+        auto X = [&](int f, int di, int dj) {
+            const double *a = f == EH ? H : f == EK ? K : f == ET ? T : f == EG1 ? e.g1 : e.g2;
+            return F(a, di > 0 ? ip : (di < 0 ? im : i), J + dj, k);
+        };
+        const double r = eng_rhs_point(e, c, X, J, Hm, Hc, Hp, Km, Kc, Kp, Tm, Tc, Tp, G3c, G3p, eng_wall_flux(e, i, J, k));
         const size_t c0 = ((size_t)k * M + (J - 1)) * L + i;
         const size_t h0 = ehx(e, i, J, k);
-        double dh = e.dh[h0] + ca * r + cb * e.rhs0[c0];
-        dh = fmin(e.dhmax - 1.0e-14, dh);   // REALMIN (modules.f90:56)
-        dh = fmax(e.dhmin + 1.0e-14, dh);
-        e.dh[h0] = dh;
-        e.rhs0[c0] = r;
+        eng_update(e, h0, c0, r, e.dh[h0], e.rhs0[c0], ca, cb);
         Hm = Hc; Km = Kc; Tm = Tc;
         Hc = Hp; Kc = Kp; Tc = Tp; G3c = G3p;
+    }
+}
+
+// The same from LDS: a 64 x BY block owns columns i0..i0+63 and rows J0..J0+BY-1 and marches
+// KC z-planes; H, K, T, G_x, G_y are staged per plane as (BY + 2) x 68 tiles (columns
+// i0-2 .. i0+65, x periodic, loaded as 16-byte column pairs) in a ring of two plane slots,
+// plane k-1's H, K, T stay in registers, and the point's DH, R0, G_z and wall flux are loaded a
+// plane ahead of the tile loads.  Identical arithmetic to k_eng_rhs_update.
+__device__ __forceinline__ int eng_xcd_tile(int p, int nb, int nxcd)
+{
+    if (nxcd <= 1) return p;
+    const int xcd = p % nxcd, slot = p / nxcd, per = nb / nxcd, rem = nb % nxcd;
+    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void k_eng_rhs_lds(EngDev e, double ca, double cb, int KC, int nxcd)
+{
+    constexpr int NT = 64 * BY, TW = 68, TS = (BY + 2) * TW, NF = 5, NCH = TS / 2;
+    static_assert(NCH <= NT, "one column pair per thread");
+    __shared__ __align__(16) double tiles[2][NF * TS];
+    const int L = e.L, M = e.M, N = e.N;
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), t = ty * 64 + tx;
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int tile = eng_xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z, nxcd);
+    const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
+    const int i0 = bx * 64, J0 = 1 + by * BY, k0 = bz * KC, k1 = min(N, k0 + KC);
+    const int i = i0 + tx, J = J0 + ty;
+    const bool act = i < L && J <= M;
+    const size_t ph = (size_t)(M + 2) * L;
+    // the column pair this thread stages (rows past M + 1 read row M + 1 instead)
+    const bool stg = t < NCH;
+    unsigned oh;
+    {
+        const int n = stg ? t : 0, r = n / (TW / 2), q = n % (TW / 2), jj = J0 - 1 + r;
+        oh = (unsigned)(min(jj, M + 1) * L + ((i0 - 2 + 2 * q) % L + L) % L);
+    }
+    const double *base[NF] = {e.h, e.k, e.t, e.g1, e.g2};
+    double pre0[NF], pre1[NF];
+    auto load = [&](int kk) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const double2 v = *reinterpret_cast<const double2 *>(base[f] + (size_t)kk * ph + oh);
+            pre0[f] = v.x;
+            pre1[f] = v.y;
+        }
+    };
+    auto store = [&](int sl) {
+        if (stg) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                reinterpret_cast<double2 *>(&tiles[sl][f * TS])[t] = make_double2(pre0[f], pre1[f]);
+        }
+    };
+    const int Jr = min(J, M), ir = act ? i : 0;
+    const EngRow c = eng_row(e, Jr);
+    // the point's own values a plane ahead: DH, R0, G_z of the next plane, the wall flux
+    double own[4] = {0.0, 0.0, 0.0, 0.0}, ownn[4] = {0.0, 0.0, 0.0, 0.0};
+    auto loadown = [&](int kk, double (&o)[4]) {
+        if (act) {
+            o[0] = e.dh[ehx(e, i, J, kk)];
+            o[1] = e.rhs0[((size_t)kk * M + (J - 1)) * L + i];
+            o[2] = e.g3[ehx(e, i, J, kk == N - 1 ? 0 : kk + 1)];
+            o[3] = eng_wall_flux(e, i, J, kk);
+        }
+    };
+    const int kb = k0 == 0 ? N - 1 : k0 - 1;
+    double Hm = e.h[ehx(e, ir, Jr, kb)], Km = e.k[ehx(e, ir, Jr, kb)], Tm = e.t[ehx(e, ir, Jr, kb)];
+    double G3c = e.g3[ehx(e, ir, Jr, k0)];
+    loadown(k0, own);
+    load(k0);
+    store(0);
+    load(k0 + 1 == N ? 0 : k0 + 1);
+    store(1);
+    __syncthreads();
+    const int cell = (ty + 1) * TW + tx + 2;
+    for (int k = k0; k < k1; ++k) {
+        const int cur = (k - k0) & 1;
+        const bool more = k + 1 < k1;
+        if (more) {
+            loadown(k + 1, ownn);
+            load(k + 2 >= N ? k + 2 - N : k + 2);
+        }
+        const double *c0p = &tiles[cur][cell], *c1p = &tiles[cur ^ 1][cell];
+        const double Hc = c0p[EH * TS], Kc = c0p[EK * TS], Tc = c0p[ET * TS];
+        const double Hp = c1p[EH * TS], Kp = c1p[EK * TS], Tp = c1p[ET * TS];
+        if (act) {
+            auto X = [&](int f, int di, int dj) { return c0p[f * TS + dj * TW + di]; };
+            const double r = eng_rhs_point(e, c, X, J, Hm, Hc, Hp, Km, Kc, Kp, Tm, Tc, Tp, G3c, own[2], own[3]);
+            eng_update(e, ehx(e, i, J, k), ((size_t)k * M + (J - 1)) * L + i, r, own[0], own[1], ca, cb);
+        }
+        Hm = Hc; Km = Kc; Tm = Tc; G3c = own[2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) own[q] = ownn[q];
+        __syncthreads();
+        if (more) store(cur);
+        __syncthreads();
     }
 }
 
@@ -476,6 +619,9 @@ struct chs_eng {
     size_t hs = 0;             // halo field size
     double *d0 = nullptr, *m0 = nullptr;   // DENSITY0, Viscousity0 (swapped with e.d, e.m each substep)
     double *gown = nullptr;                 // own G (e.g1.. may point at a momentum context's G)
+    int rhs_lds = 1;                        // energy RHS from LDS-staged planes (k_eng_rhs_lds)
+    int rhs_kc = 32;                        // its z-planes per block
+    int rhs_nxcd = 8;                       // XCDs its tiles are grouped for
     std::string errmsg;
 };
 
@@ -638,11 +784,32 @@ int chs_substep_eng(chs_eng *e, int ns, double dt)
     // (:27) is never read by the reference and is not kept.
     std::swap(e->e.d, e->d0);
     std::swap(e->e.m, e->m0);
-    constexpr int KC = 8;
-    const dim3 blk(64, 4), grd((e->L + 63) / 64, (e->M + 3) / 4, (e->N + KC - 1) / KC);
-    hipLaunchKernelGGL(k_eng_rhs_update, grd, blk, 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt, KC);
+    // the LDS kernel loads column pairs as 16-byte words: the staged fields 16-byte aligned
+    auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    if (e->rhs_lds && e->L % 2 == 0 && a16(e->e.h) && a16(e->e.k) && a16(e->e.t) && a16(e->e.g1) && a16(e->e.g2) &&
+        (e->hs % 2) == 0) {
+        const int KC = e->rhs_kc;
+        const dim3 grd((e->L + 63) / 64, (e->M + 3) / 4, (e->N + KC - 1) / KC);
+        hipLaunchKernelGGL(k_eng_rhs_lds<4>, grd, dim3(64, 4), 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt,
+                           KC, e->rhs_nxcd);
+    } else {
+        constexpr int KC = 8;
+        const dim3 blk(64, 4), grd((e->L + 63) / 64, (e->M + 3) / 4, (e->N + KC - 1) / KC);
+        hipLaunchKernelGGL(k_eng_rhs_update, grd, blk, 0, e->stream, e->e, e->tgam[ns] * dt, e->troh[ns] * dt, KC);
+    }
     eng_props_and_walls(e, 1, e->M);
     return eng_check(e, "chs_substep_eng");
+}
+
+int chs_eng_set_tuning(chs_eng *e, const char *key, int value)
+{
+    if (!e || !key) return eng_fail(e, CHS_E_ARG, "chs_eng_set_tuning: bad argument");
+    const std::string k(key);
+    if (k == "rhs_lds" && (value == 0 || value == 1)) e->rhs_lds = value;
+    else if (k == "rhs_kc" && value >= 1 && value <= 1024) e->rhs_kc = value;
+    else if (k == "rhs_nxcd" && value >= 1 && value <= 64) e->rhs_nxcd = value;
+    else return eng_fail(e, CHS_E_ARG, "chs_eng_set_tuning: unknown key or value out of range (" + k + ")");
+    return CHS_OK;
 }
 
 int chs_eng_download(chs_eng *e, double *dh, double *h, double *t, double *d, double *m, double *k, double *cp,

@@ -79,6 +79,7 @@ SIGNATURES = {
     "chs_eng_last_error": (C.c_char_p, [C.c_void_p]),
     "chs_eng_upload": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int]),
     "chs_eng_set_mass_flux": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "chs_eng_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "chs_substep_eng": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_eng_download": (C.c_int, [C.c_void_p] + [_dp] * 8),
     "chs_mean_zx_tg": (C.c_int, [C.c_void_p, _dp, _dp]),

@@ -206,6 +206,11 @@ class EnergyTG:
     def set_mass_flux(self, g_dev_ptr: int):
         self._check(lib().chs_eng_set_mass_flux(self._h, C.c_void_p(g_dev_ptr)), "chs_eng_set_mass_flux")
 
+    def set_tuning(self, **kv):
+        """Launch shapes (chs_eng_set_tuning), e.g. rhs_lds=0."""
+        for k, v in kv.items():
+            self._check(lib().chs_eng_set_tuning(self._h, k.encode(), int(v)), "chs_eng_set_tuning")
+
     def SOLVERRK3_ENG_io(self, ns, dt):
         self._check(lib().chs_substep_eng(self._h, int(ns), float(dt)), "chs_substep_eng")
 

@@ -352,6 +352,13 @@ int chs_eng_set_mass_flux(chs_eng *e, const double *g_dev);
  * CHS_E_RANGE if DH leaves the table (the reference STOPs).  Synchronous. */
 int chs_substep_eng(chs_eng *e, int ns, double dt);
 
+/* Launch shape of the energy substep's right-hand side (no counterpart in the reference): key
+ * "rhs_lds" 1 (default) for the kernel that stages H, K, T, G_x, G_y per z-plane in LDS, 0 for
+ * the one reading them per point (identical results); "rhs_kc" its z-planes per block (1..1024,
+ * default 32); "rhs_nxcd" the XCDs its tiles are grouped for (1..64, default 8).  CHS_E_ARG for
+ * an unknown key or a value out of range. */
+int chs_eng_set_tuning(chs_eng *e, const char *key, int value);
+
 /* Copy fields back (NULL skips one): halo arrays (N, M+2, L), rhs0 (N, M, L). */
 int chs_eng_download(chs_eng *e, double *dh, double *h, double *t, double *d, double *m, double *k, double *cp,
                      double *rhs0);

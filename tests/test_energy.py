@@ -162,3 +162,44 @@ def test_gpu_cell_props_equal_lookup(pkg, table):
     pt.close()
     for i, kk in enumerate(("H", "T", "D", "M", "K", "CP")):
         np.testing.assert_array_equal(out[kk][:, 1:M + 1].ravel(), ref[i], err_msg=kk)
+
+
+@pytest.mark.gpu
+def test_gpu_energy_rhs_lds_equals_per_point(pkg, table):
+    """The energy right-hand side from LDS-staged planes (rhs_lds = 1) against the per-point
+    kernel (rhs_lds = 0): bit-identical substeps (the same expressions from the same values),
+    for ragged meshes (L below or not a multiple of 64, M not a multiple of 4, N not a
+    multiple of the march), march lengths 1, 3, 32, beyond N, tiles grouped for 8, 3 or 1
+    XCDs, both wall kinds on either side."""
+    thermo, t = table
+    REN, STR2, dt = 2800.0, 1.9, 5e-4
+    cthecd = 1.0 / REN / (t["M0"] * t["CP0"] / t["K0"])
+    dhmin, dhmax = t["DH"][t["IMIN_DH"] - 1], t["DH"][t["IMAX_DH"] - 1]
+    for L, M, N in [(64, 48, 32), (48, 37, 30), (80, 21, 10), (16, 9, 6)]:
+        print(f"energy rhs kernels {L}x{M}x{N}", flush=True)
+        mesh = setup.Mesh(L, M, N, STR2=STR2)
+        dh, gg = oracle.eng_initial_field(mesh, t, 50 + L)
+        Tw = 675.0 / t["T0"]
+        hw = float(np.interp(Tw, t["T"], t["H"]))
+        ws = [[Tw, hw, float(np.interp(hw, t["H"], t["D"])), float(np.interp(hw, t["H"], t["M"])),
+               float(np.interp(hw, t["H"], t["K"]))]] * 2
+        for wt in [(1, 2), (2, 1), (1, 1)]:
+            out = []
+            for shape in (dict(rhs_lds=0), dict(rhs_lds=1), dict(rhs_lds=1, rhs_kc=3, rhs_nxcd=3),
+                          dict(rhs_lds=1, rhs_kc=1, rhs_nxcd=1), dict(rhs_lds=1, rhs_kc=64)):
+                e = thermo.EnergyTG(pkg.channel_setup(L, M, N, STR2=STR2), t, wt, cthecd, dhmin, dhmax, (0.1, -0.05),
+                                    wall_state=ws)
+                e.set_tuning(**shape)
+                e.upload(dh, gg)
+                for ns in (1, 2, 3):
+                    e.SOLVERRK3_ENG_io(ns, dt)
+                out.append((shape, e.download()))
+                e.close()
+            for shape, f in out[1:]:
+                for k, v in out[0][1].items():
+                    assert np.array_equal(f[k], v), (L, M, N, wt, shape, k)
+    e = thermo.EnergyTG(pkg.channel_setup(16, 9, 6, STR2=STR2), t, (1, 1), cthecd, dhmin, dhmax, (0.1, -0.05))
+    for bad in (dict(rhs_lds=2), dict(rhs_kc=0), dict(rhs_nxcd=0), dict(nope=1)):
+        with pytest.raises(pkg.ChsError):
+            e.set_tuning(**bad)
+    e.close()
