@@ -1000,10 +1000,10 @@ __device__ __forceinline__ void io_cn_x_jac(const IoDev &d, double dp, double fa
     const int K = jacobi_sweeps(rho);
     double x = r * ib;
     for (int it = 0; it < K; ++it) {
-        s0[threadIdx.x] = x;
+        double *sb = (it & 1) ? s1 : s0;   // alternating buffers: one barrier per sweep
+        sb[threadIdx.x] = x;
         __syncthreads();
-        const double xm = s0[sm], xp = s0[sp];
-        __syncthreads();
+        const double xm = sb[sm], xp = sb[sp];
         x = (r - a * xm - c * xp) * ib;
     }
     if (valid) ioh<W>(d)[C * cs + cx(d, e, J, k)] = (W)x;
@@ -1288,11 +1288,13 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
 #pragma unroll
     for (int j = 0; j < P; ++j) x[j] = r[j] * ib[j];
     for (int it = 0; it < K; ++it) {
-        e0[threadIdx.x] = x[0];
-        e1[threadIdx.x] = x[P - 1];
+        // alternating edge buffers: a sweep's writes never meet the previous sweep's reads, so
+        // one barrier per sweep
+        double *w0 = (it & 1) ? e2 : e0, *w1 = (it & 1) ? e3 : e1;
+        w0[threadIdx.x] = x[0];
+        w1[threadIdx.x] = x[P - 1];
         __syncthreads();
-        const double xl = e1[sl], xr = e0[sr];
-        __syncthreads();
+        const double xl = w1[sl], xr = w0[sr];
         double xn[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
@@ -1412,11 +1414,11 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
 #pragma unroll
     for (int j = 0; j < P; ++j) x[j] = r[j] * ib[j];
     for (int it = 0; it < K; ++it) {
-        e0[threadIdx.x] = x[0];
-        e1[threadIdx.x] = x[P - 1];
+        double *w0 = (it & 1) ? e2 : e0, *w1 = (it & 1) ? e3 : e1;   // alternating: one barrier per sweep
+        w0[threadIdx.x] = x[0];
+        w1[threadIdx.x] = x[P - 1];
         __syncthreads();
-        const double xl = sl >= 0 ? e1[sl] : 0.0, xr = sr >= 0 ? e0[sr] : 0.0;
-        __syncthreads();
+        const double xl = sl >= 0 ? w1[sl] : 0.0, xr = sr >= 0 ? w0[sr] : 0.0;
         double xn[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
