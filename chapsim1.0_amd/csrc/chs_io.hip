@@ -64,6 +64,21 @@ template <typename W> __device__ __forceinline__ W *ioex(const IoDev &d);
 template <> __device__ __forceinline__ double *ioex<double>(const IoDev &d) { return d.explt0; }
 template <> __device__ __forceinline__ float *ioex<float>(const IoDev &d) { return d.explt0f; }
 
+// Tile width of the staged planes: columns i0-2 .. i0+65 (the halo column each side and one
+// more on the left, so that a tile row is 34 aligned pairs of columns, one 16-byte load each).
+constexpr int IO_TW = 68;
+
+// Tile of launch slot p of nb: the hardware deals slots to the XCDs in turn (slot p to XCD
+// p mod nxcd); each XCD gets a contiguous run of tiles, so that the halo rows a tile shares
+// with the tiles above and below it are read through the same L2.
+__device__ __forceinline__ int io_xcd_tile(int p, int nb, int nxcd)
+{
+    if (nxcd <= 1) return p;
+    const int xcd = p % nxcd, slot = p / nxcd, per = nb / nxcd, rem = nb % nxcd;
+    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+
 // Thread (i, J = j0 + row, k) of a 64 x 4 block on a (ceil(L/64), ceil(rows/4), N) grid.
 #define IO_THREAD(j0)                                        \
     const int i = blockIdx.x * 64 + threadIdx.x;             \
@@ -78,9 +93,66 @@ template <> __device__ __forceinline__ float *ioex<float>(const IoDev &d) { retu
 // point's G and staggered density.  Each thread marches KC z-planes: the values plane k+1
 // needs from plane k (D and V at (i, J), (i-1, J), (i, J-1); G_z and G0_z at (i, J)) stay in
 // registers, so a plane costs 8 state loads instead of 14.
-__global__ __launch_bounds__(256) void k_io_props_z(IoDev d, double tg, double tr, int KC)
+// Plane-k fields of the staggered-property point: X(f, di, dj) = f at (i + di, J + dj, k).
+enum IoPF { PD, PV, PGX, PG0X, PGY, PG0Y };
+
+// One point (i, J, k), J = 1..M+1: the plane-(k-1) values (D, V at (i, J); V at (i-1, J) and
+// (i, J-1)) and G_z, G0_z at (i, J) of planes k and k+1 come as values.
+template <typename A>
+__device__ __forceinline__ void io_props_point(const IoDev &d, const A &X, int i, int J, int k, double tg, double tr,
+                                               double Dm, double Vm, double Vmim, double Vmb, double g2c,
+                                               double g2p, double g02c, double g02p)
 {
 #pragma clang fp contract(off)
+    const int M = d.M;
+    const size_t hs = hsz(d), c0 = hx(d, i, J, k);
+    const double h = 0.5;
+    const double coe2 = d.dyfi[min(J, M)], coe3 = d.dzi, wff = d.wff[J], wfb = d.wfb[J];
+    const double Dc = X(PD, 0, 0), Vc = X(PV, 0, 0), Vim = X(PV, -1, 0);
+    const double Vb = X(PV, 0, -1), Vimb = X(PV, -1, -1);
+    double gx = 0.0, gy = 0.0, gz = g2c, dsx = 0.0, dsy, dsz = 0.0;
+    if (J <= M) {
+        gx = X(PGX, 0, 0);
+        gy = X(PGY, 0, 0);
+        const double divx = (X(PGX, 1, 0) - gx) * d.dxi;
+        const double divx0 = (X(PG0X, 1, 0) - X(PG0X, 0, 0)) * d.dxi;
+        const double divy = (X(PGY, 0, 1) - gy) * coe2;
+        const double divy0 = (X(PG0Y, 0, 1) - X(PG0Y, 0, 0)) * coe2;
+        const double divz = (g2p - g2c) * coe3;
+        const double divz0 = (g02p - g02c) * coe3;
+        d.drho[cx(d, i, J, k)] = -tg * (divx + divy + divz) - tr * (divx0 + divy0 + divz0);
+        dsx = (Dc + X(PD, -1, 0)) * h;
+        dsz = (Dc + Dm) * h;
+        d.dstg[c0] = dsx;
+        d.dstg[2 * hs + c0] = dsz;
+        d.mstg[hs + c0] = (Vc + Vim + Vm + Vmim) * h * h;
+    }
+    if (J == 1) {
+        dsy = X(PD, 0, -1);
+        d.dstg[hs + c0] = dsy;
+        d.mstg[c0] = Vb;
+        d.mstg[2 * hs + c0] = Vb;
+    } else if (J == M + 1) {
+        dsy = Dc;
+        d.dstg[hs + c0] = dsy;
+        d.mstg[c0] = Vc;
+        d.mstg[2 * hs + c0] = Vc;
+    } else {
+        dsy = wff * Dc + wfb * X(PD, 0, -1);
+        d.dstg[hs + c0] = dsy;
+        d.mstg[c0] = ((Vc + Vim) * wff + (Vb + Vimb) * wfb) * h;
+        d.mstg[2 * hs + c0] = ((Vm + Vc) * wff + (Vmb + Vb) * wfb) * h;
+    }
+    // the velocity the next momentum substep starts from (VELOCITY_CALC_io, BC_WALL_Q_io)
+    d.q[c0] = J == M + 1 ? 0.0 : gx / dsx;
+    d.q[hs + c0] = (J == M + 1 || J == 1) ? 0.0 : gy / dsy;
+    d.q[2 * hs + c0] = J == M + 1 ? 0.0 : gz / dsz;
+    if (J == 1)
+        for (int c = 0; c < 3; ++c) d.q[c * hs + hx(d, i, 0, k)] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_io_props_z(IoDev d, double tg, double tr, int KC)
+{
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int J = (int)(blockIdx.y * 4 + threadIdx.y) + 1;
     const int L = d.L, M = d.M, N = d.N;
@@ -88,7 +160,6 @@ __global__ __launch_bounds__(256) void k_io_props_z(IoDev d, double tg, double t
     const int k0 = blockIdx.z * KC, k1 = min(N, k0 + KC);
     const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
     const size_t hs = hsz(d);
-    const double h = 0.5;
     const double *D = d.d0, *V = d.m0, *G = d.g, *G0 = d.g0;
     const int kb = k0 == 0 ? N - 1 : k0 - 1;
     double Dm = D[hx(d, i, J, kb)], Vm = V[hx(d, i, J, kb)], Vmim = V[hx(d, im, J, kb)];
@@ -98,58 +169,109 @@ __global__ __launch_bounds__(256) void k_io_props_z(IoDev d, double tg, double t
         g2c = G[2 * hs + hx(d, i, J, k0)];
         g02c = G0[2 * hs + hx(d, i, J, k0)];
     }
-    const double coe2 = d.dyfi[min(J, M)], coe3 = d.dzi, wff = d.wff[J], wfb = d.wfb[J];
     for (int k = k0; k < k1; ++k) {
         const int kp = k == N - 1 ? 0 : k + 1;
-        const size_t c0 = hx(d, i, J, k), b0 = hx(d, i, J - 1, k);
-        const double Dc = D[c0], Vc = V[c0], Vim = V[hx(d, im, J, k)];
-        const double Vb = V[b0], Vimb = V[hx(d, im, J - 1, k)];
-        double gx = 0.0, gy = 0.0, gz = g2c, dsx = 0.0, dsy, dsz = 0.0;
+        double g2p = 0.0, g02p = 0.0;
         if (J <= M) {
-            gx = G[c0];
-            gy = G[hs + c0];
-            const double g2p = G[2 * hs + hx(d, i, J, kp)], g02p = G0[2 * hs + hx(d, i, J, kp)];
-            const double divx = (G[hx(d, ip, J, k)] - gx) * d.dxi;
-            const double divx0 = (G0[hx(d, ip, J, k)] - G0[c0]) * d.dxi;
-            const double divy = (G[hs + hx(d, i, J + 1, k)] - gy) * coe2;
-            const double divy0 = (G0[hs + hx(d, i, J + 1, k)] - G0[hs + c0]) * coe2;
-            const double divz = (g2p - g2c) * coe3;
-            const double divz0 = (g02p - g02c) * coe3;
-            d.drho[cx(d, i, J, k)] = -tg * (divx + divy + divz) - tr * (divx0 + divy0 + divz0);
-            dsx = (Dc + D[hx(d, im, J, k)]) * h;
-            dsz = (Dc + Dm) * h;
-            d.dstg[c0] = dsx;
-            d.dstg[2 * hs + c0] = dsz;
-            d.mstg[hs + c0] = (Vc + Vim + Vm + Vmim) * h * h;
-            g2c = g2p;
-            g02c = g02p;
+            g2p = G[2 * hs + hx(d, i, J, kp)];
+            g02p = G0[2 * hs + hx(d, i, J, kp)];
         }
-        if (J == 1) {
-            dsy = D[b0];
-            d.dstg[hs + c0] = dsy;
-            d.mstg[c0] = Vb;
-            d.mstg[2 * hs + c0] = Vb;
-        } else if (J == M + 1) {
-            dsy = Dc;
-            d.dstg[hs + c0] = dsy;
-            d.mstg[c0] = Vc;
-            d.mstg[2 * hs + c0] = Vc;
-        } else {
-            dsy = wff * Dc + wfb * D[b0];
-            d.dstg[hs + c0] = dsy;
-            d.mstg[c0] = ((Vc + Vim) * wff + (Vb + Vimb) * wfb) * h;
-            d.mstg[2 * hs + c0] = ((Vm + Vc) * wff + (Vmb + Vb) * wfb) * h;
+        auto X = [&](int f, int di, int dj) {
+            const double *a = f == PD ? D : f == PV ? V : f == PGX ? G : f == PG0X ? G0 : f == PGY ? G + hs : G0 + hs;
+            return a[hx(d, di > 0 ? ip : (di < 0 ? im : i), J + dj, k)];
+        };
+        io_props_point(d, X, i, J, k, tg, tr, Dm, Vm, Vmim, Vmb, g2c, g2p, g02c, g02p);
+        Dm = X(PD, 0, 0);
+        Vm = X(PV, 0, 0);
+        Vmim = X(PV, -1, 0);
+        Vmb = X(PV, 0, -1);
+        g2c = g2p;
+        g02c = g02p;
+    }
+}
+
+// The same from LDS: a 64 x 4 block owns columns i0..i0+63 and rows J0..J0+3 (J <= M+1) and
+// marches KC z-planes; D, V and the x and y components of G and G0 are staged per plane as
+// 6 x 68 tiles (columns i0-2 .. i0+65, loaded as 16-byte column pairs), plane k+1's tiles
+// loaded while plane k is computed; the plane-(k-1) values stay in registers, G_z and G0_z of
+// the point are loaded a plane ahead.  Identical arithmetic to k_io_props_z.
+__global__ __launch_bounds__(256) void k_io_props_lds(IoDev d, double tg, double tr, int KC, int nxcd)
+{
+    constexpr int BY = 4, NT = 64 * BY, TW = IO_TW, TS = (BY + 2) * TW, NF = 6, NCH = TS / 2;
+    static_assert(NCH <= NT, "one column pair per thread");
+    __shared__ __align__(16) double tiles[NF * TS];
+    const int L = d.L, M = d.M, N = d.N;
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), t = ty * 64 + tx;
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int tile = io_xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z, nxcd);
+    const int bx = tile % gx, by = (tile / gx) % gy, bz = tile / (gx * gy);
+    const int i0 = bx * 64, J0 = 1 + by * BY, k0 = bz * KC, k1 = min(N, k0 + KC);
+    const int i = i0 + tx, J = J0 + ty;
+    const bool act = i < L && J <= M + 1;
+    const size_t hs = hsz(d), ph = (size_t)(M + 2) * L;
+    const bool stg = t < NCH;
+    unsigned oh;
+    {
+        const int n = stg ? t : 0, r = n / (TW / 2), q = n % (TW / 2), jj = J0 - 1 + r;
+        oh = (unsigned)(min(jj, M + 1) * L + ((i0 - 2 + 2 * q) % L + L) % L);
+    }
+    const double *base[NF] = {d.d0, d.m0, d.g, d.g0, d.g + hs, d.g0 + hs};
+    double pre0[NF], pre1[NF];
+    auto load = [&](int kk) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const double2 v = *reinterpret_cast<const double2 *>(base[f] + (size_t)kk * ph + oh);
+            pre0[f] = v.x;
+            pre1[f] = v.y;
         }
-        // the velocity the next momentum substep starts from (VELOCITY_CALC_io, BC_WALL_Q_io)
-        d.q[c0] = J == M + 1 ? 0.0 : gx / dsx;
-        d.q[hs + c0] = (J == M + 1 || J == 1) ? 0.0 : gy / dsy;
-        d.q[2 * hs + c0] = J == M + 1 ? 0.0 : gz / dsz;
-        if (J == 1)
-            for (int c = 0; c < 3; ++c) d.q[c * hs + hx(d, i, 0, k)] = 0.0;
-        Dm = Dc;
-        Vm = Vc;
-        Vmim = Vim;
-        Vmb = Vb;
+    };
+    auto store = [&]() {
+        if (stg) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                reinterpret_cast<double2 *>(&tiles[f * TS])[t] = make_double2(pre0[f], pre1[f]);
+        }
+    };
+    const int ir = act ? i : 0, Jr = act ? J : 1, imr = ir == 0 ? L - 1 : ir - 1;
+    const bool gz = act && J <= M;   // rows with a z mass flux
+    const int kb = k0 == 0 ? N - 1 : k0 - 1;
+    double Dm = d.d0[hx(d, ir, Jr, kb)], Vm = d.m0[hx(d, ir, Jr, kb)], Vmim = d.m0[hx(d, imr, Jr, kb)];
+    double Vmb = d.m0[hx(d, ir, Jr - 1, kb)];
+    double g2c = 0.0, g02c = 0.0, g2p = 0.0, g02p = 0.0, g2n = 0.0, g02n = 0.0;
+    if (gz) {
+        const int kp = k0 == N - 1 ? 0 : k0 + 1;
+        g2c = d.g[2 * hs + hx(d, i, J, k0)];
+        g02c = d.g0[2 * hs + hx(d, i, J, k0)];
+        g2p = d.g[2 * hs + hx(d, i, J, kp)];
+        g02p = d.g0[2 * hs + hx(d, i, J, kp)];
+    }
+    load(k0);
+    store();
+    __syncthreads();
+    const int cell = (ty + 1) * TW + tx + 2;
+    for (int k = k0; k < k1; ++k) {
+        const bool more = k + 1 < k1;
+        if (more) {
+            if (gz) {   // G_z, G0_z of plane k+2: the next plane's k+1 values
+                const int kn = k + 2 >= N ? k + 2 - N : k + 2;
+                g2n = d.g[2 * hs + hx(d, i, J, kn)];
+                g02n = d.g0[2 * hs + hx(d, i, J, kn)];
+            }
+            load(k + 1 == N ? 0 : k + 1);
+        }
+        auto X = [&](int f, int di, int dj) { return tiles[f * TS + cell + dj * TW + di]; };
+        if (act) io_props_point(d, X, i, J, k, tg, tr, Dm, Vm, Vmim, Vmb, g2c, g2p, g02c, g02p);
+        Dm = X(PD, 0, 0);
+        Vm = X(PV, 0, 0);
+        Vmim = X(PV, -1, 0);
+        Vmb = X(PV, 0, -1);
+        g2c = g2p;
+        g02c = g02p;
+        g2p = g2n;
+        g02p = g02n;
+        __syncthreads();
+        if (more) store();
+        __syncthreads();
     }
 }
 
@@ -557,10 +679,6 @@ template <bool CN> struct IoStage {
     static __device__ __forceinline__ constexpr int carry_dj(int c) { return (c == 3 || c == 8) ? 1 : 0; }
 };
 
-// Tile width of the staged planes: columns i0-2 .. i0+65 (the halo column each side and one
-// more on the left, so that a tile row is 34 aligned pairs of columns, one 16-byte load each).
-constexpr int IO_TW = 68;
-
 template <bool CN, int BY, typename W> struct IoAtLds {
     static constexpr int TW = IO_TW, TS = (BY + 2) * TW;
     const double *c0, *c1;   // this point's cell in the tiles of planes k and k+1
@@ -581,16 +699,6 @@ template <bool CN, int BY, typename W> struct IoAtLds {
         return (dk > 0 ? c1 : c0)[IoStage<CN>::slot(f) * TS + dj * TW + di];
     }
 };
-
-// Tile of launch slot p of nb: the hardware deals slots to the XCDs in turn (slot p to XCD
-// p mod nxcd); each XCD gets a contiguous run of tiles, so that the halo rows a tile shares
-// with the tiles above and below it are read through the same L2.
-__device__ __forceinline__ int io_xcd_tile(int p, int nb, int nxcd)
-{
-    if (nxcd <= 1) return p;
-    const int xcd = p % nxcd, slot = p / nxcd, per = nb / nxcd, rem = nb % nxcd;
-    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-}
 
 template <bool CN, int BY, typename W>
 __global__ __launch_bounds__(64 * BY, 2) void k_io_mom_z(IoDev d, const double *__restrict__ gs, double tg, double tr,
@@ -1614,6 +1722,7 @@ struct chs_io {
     int mom_kz = 32;    // z-planes per block of the LDS-staged momentum kernel (0: the per-point kernel)
     int mom_by = 4;     // rows per block of the LDS-staged momentum kernel (4 or 8)
     int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
+    int props_kc = 16;  // z-planes per block of the LDS-staged staggered properties (0: per-point kernel)
     int cnz_cols = 0, cny_cols = 0;   // cn_cols for the z or the y solve alone (0: cn_cols)
     int cnx_wave = 1;
     int cn_fastdiv = 1;   // z and y Jacobi coefficients by Newton reciprocals (the sweeps resolve
@@ -1838,10 +1947,16 @@ int chs_io_upload_viscosity(chs_io *m, const double *visc)
 int chs_io_props(chs_io *m, int ns)
 {
     if (!m || ns < 1 || ns > 3) return io_fail(m, CHS_E_ARG, "chs_io_props: bad argument (NS 1..3)");
-    {
+    const IoDev &d = m->d;
+    auto a16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    const dim3 g = io_grid(m, m->M + 1);
+    if (m->props_kc > 0 && m->L % 2 == 0 && a16(d.d0) && a16(d.m0) && a16(d.g) && a16(d.g0)) {
+        const int KC = m->props_kc;
+        hipLaunchKernelGGL(k_io_props_lds, dim3(g.x, g.y, (m->N + KC - 1) / KC), dim3(64, 4), 0, m->stream, d,
+                           m->tgam[ns], m->troh[ns], KC, m->mom_nxcd);
+    } else {
         constexpr int KC = 8;
-        const dim3 g = io_grid(m, m->M + 1);
-        hipLaunchKernelGGL(k_io_props_z, dim3(g.x, g.y, (m->N + KC - 1) / KC), dim3(64, 4), 0, m->stream, m->d,
+        hipLaunchKernelGGL(k_io_props_z, dim3(g.x, g.y, (m->N + KC - 1) / KC), dim3(64, 4), 0, m->stream, d,
                            m->tgam[ns], m->troh[ns], KC);
     }
     m->q_fresh = true;
@@ -1860,6 +1975,11 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
     if (k == "mom_kz") {
         if (value < 0 || value > 1024) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: mom_kz must be 0..1024");
         m->mom_kz = value;
+        return CHS_OK;
+    }
+    if (k == "props_kc") {
+        if (value < 0 || value > 1024) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: props_kc must be 0..1024");
+        m->props_kc = value;
         return CHS_OK;
     }
     if (k == "mom_nxcd") {

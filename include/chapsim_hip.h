@@ -503,7 +503,9 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt);
  * generator; no counterpart in the reference.  Key "mom_kz": z-planes each block of the
  * LDS-staged momentum kernel marches (default 32; 0 selects the per-point kernel);
  * "mom_by": its rows per block (4 default, or 8); "mom_nxcd": XCDs its tiles are grouped for
- * (default 8; 1 keeps launch order); "mom_rows": rows per 64 x 4 block of the per-point kernel (4
+ * (default 8; 1 keeps launch order; also used by the staggered properties); "props_kc": z-planes
+ * per block of the LDS-staged staggered-property kernel of chs_io_props (default 16; 0 selects
+ * the per-point kernel); "mom_rows": rows per 64 x 4 block of the per-point kernel (4
  * default, 2, 1; the block's other rows of threads take further z-planes); "cn_cols": most
  * x-columns per block of the Crank-Nicolson z and y Jacobi solves (1..64, default 16; rounded
  * down to whole wavefronts, the sequential solves where that leaves none), "cnz_cols" and

@@ -410,7 +410,7 @@ def test_gpu_explicit_momentum_block_shapes(pkg, rows):
             assert rel(out[1][k][c], out[0][k][c]) < 1e-13, (k, c)
     assert rel(out[1]["PR"], out[0]["PR"]) < 1e-11
     ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0)
-    for bad in (dict(mom_rows=3), dict(mom_by=2), dict(mom_kz=-1), dict(mom_nxcd=0)):
+    for bad in (dict(mom_rows=3), dict(mom_by=2), dict(mom_kz=-1), dict(mom_nxcd=0), dict(props_kc=-1)):
         with pytest.raises(pkg.ChsError):
             ch.set_tuning(**bad)
     ch.close()
@@ -419,8 +419,9 @@ def test_gpu_explicit_momentum_block_shapes(pkg, rows):
 @pytest.mark.gpu
 @pytest.mark.parametrize("vis", [1, 0])
 def test_gpu_momentum_lds_march_equals_per_point_kernel(pkg, vis):
-    """The LDS-staged z-march momentum kernel (mom_kz > 0) against the per-point kernel
-    (mom_kz = 0) under a constant Cf, where the mass-flux partials are not used: bit-identical
+    """The LDS-staged z-march momentum kernel (mom_kz > 0) and staggered-property kernel
+    (props_kc > 0) against the per-point kernels (mom_kz = 0, props_kc = 0) under a constant Cf,
+    where the mass-flux partials are not used: bit-identical
     substeps (the same expressions from the same values), for both block heights, march
     lengths 1, 3, 5, 8, N and beyond N, tiles grouped for 8, 3 or 1 XCDs, and ragged meshes (L below or not a multiple of 64, M not a
     multiple of the block height, N not a multiple of the march length).  With a constant mass
@@ -433,9 +434,10 @@ def test_gpu_momentum_lds_march_equals_per_point_kernel(pkg, vis):
             st = dict(st, M1=cn_viscosity(st, 55 + L, False))
         for fd in (2, 1):
             out = []
-            for mom in (dict(mom_kz=0), dict(mom_kz=8), dict(mom_kz=3, mom_by=8), dict(mom_kz=1),
+            for mom in (dict(mom_kz=0, props_kc=0), dict(mom_kz=8), dict(mom_kz=3, mom_by=8), dict(mom_kz=1),
                         dict(mom_kz=N, mom_by=8), dict(mom_kz=64), dict(mom_kz=8, mom_nxcd=1),
-                        dict(mom_kz=5, mom_nxcd=3)):
+                        dict(mom_kz=5, mom_nxcd=3), dict(mom_kz=0, props_kc=3), dict(props_kc=1, mom_nxcd=1),
+                        dict(props_kc=64)):
                 ch = pkg.ChannelIO(pkg.channel_setup(L, M, N, STR2=2.0), 4000.0, gravity=2, f_a=0.02,
                                    flow_driven=fd, cf_given=8.18e-3, vis_scheme=vis)
                 ch.set_tuning(**mom)
