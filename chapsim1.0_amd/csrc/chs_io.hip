@@ -1418,13 +1418,13 @@ __device__ __forceinline__ void io_cn_z_jac(const IoDev &d, double facoe, double
     }
 }
 
-template <typename W, bool FD>
+template <typename W, bool FD, int P>
 __global__ __launch_bounds__(1024) void k_io_cn_z_jac(IoDev d, double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_z_jac<0, 4, W, FD>(d, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_z_jac<1, 4, W, FD>(d, facoe, lds);
-    else io_cn_z_jac<2, 4, W, FD>(d, facoe, lds);
+    if (blockIdx.y == 0) io_cn_z_jac<0, P, W, FD>(d, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_z_jac<1, P, W, FD>(d, facoe, lds);
+    else io_cn_z_jac<2, P, W, FD>(d, facoe, lds);
 }
 
 // MOMFA3_Y_io + the G update with P consecutive rows per thread (NT = ceil(M / P) chunks per
@@ -1552,13 +1552,13 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
     }
 }
 
-template <typename W, bool FD>
+template <typename W, bool FD, int P>
 __global__ __launch_bounds__(1024) void k_io_cn_y_jac(IoDev d, const double *__restrict__ gs, double facoe)
 {
     extern __shared__ double lds[];
-    if (blockIdx.y == 0) io_cn_y_jac<0, 4, W, FD>(d, gs, facoe, lds);
-    else if (blockIdx.y == 1) io_cn_y_jac<1, 4, W, FD>(d, gs, facoe, lds);
-    else io_cn_y_jac<2, 4, W, FD>(d, gs, facoe, lds);
+    if (blockIdx.y == 0) io_cn_y_jac<0, P, W, FD>(d, gs, facoe, lds);
+    else if (blockIdx.y == 1) io_cn_y_jac<1, P, W, FD>(d, gs, facoe, lds);
+    else io_cn_y_jac<2, P, W, FD>(d, gs, facoe, lds);
 }
 
 // DIVG_io (DIVG.f90:54-137), iWeightedPre = 0: rows 1..M.
@@ -1724,6 +1724,8 @@ struct chs_io {
     int mom_nxcd = 8;   // XCDs the LDS-staged momentum kernel groups its tiles for (1: launch order)
     int props_kc = 16;  // z-planes per block of the LDS-staged staggered properties (0: per-point kernel)
     int cnz_cols = 0, cny_cols = 0;   // cn_cols for the z or the y solve alone (0: cn_cols)
+    int cnz_p = 8, cny_p = 8;         // line points per thread of the z and y solves (4 or 8; 8:
+                                      // 6.21 -> 6.09 (z) and 6.14 (y) ms per heated CN step)
     int cnx_wave = 1;
     int cn_fastdiv = 1;   // z and y Jacobi coefficients by Newton reciprocals (the sweeps resolve
                           // no finer; 0.26 -> 0.24 and 0.29 -> 0.28 ms per stage at 256 x 128 x 256)   // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
@@ -1992,6 +1994,11 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value)
         m->mom_by = value;
         return CHS_OK;
     }
+    if (k == "cnz_p" || k == "cny_p") {
+        if (value != 4 && value != 8) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: " + k + " must be 4 or 8");
+        (k == "cnz_p" ? m->cnz_p : m->cny_p) = value;
+        return CHS_OK;
+    }
     if (k == "cnz_cols" || k == "cny_cols") {
         if (value < 0 || value > 64) return io_fail(m, CHS_E_ARG, "chs_io_set_tuning: " + k + " must be 0..64");
         (k == "cnz_cols" ? m->cnz_cols : m->cny_cols) = value;
@@ -2076,24 +2083,33 @@ void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, doubl
             hipLaunchKernelGGL(k_io_cn_x<W>, dim3((M * N + 127) / 128, 3), dim3(128), 0, m->stream, d, dpg_dev, dpg,
                                facoe);
         }
-        constexpr int P = 4;   // line points per thread of the z and y kernels
-        const int NTz = N / P;   // z: block = Tz x-columns x NTz chunks, whole waves
-        int Tz = (N % P == 0 && NTz >= 2 && NTz <= 1024) ? std::min(m->cnz_cols ? m->cnz_cols : m->cn_cols, 1024 / NTz) : 0;
+        // line points per thread of the z and y kernels (4 or 8)
+        const int Pz = m->cnz_p, Py = m->cny_p;
+        const int NTz = N / Pz;   // z: block = Tz x-columns x NTz chunks, whole waves
+        int Tz = (N % Pz == 0 && NTz >= 2 && NTz <= 1024) ? std::min(m->cnz_cols ? m->cnz_cols : m->cn_cols, 1024 / NTz) : 0;
         if (Tz > 0) Tz -= Tz % (64 / std::gcd(NTz, 64));
         if (Tz > 0) {
             const int ng = (L + Tz - 1) / Tz;
-            hipLaunchKernelGGL((m->cn_fastdiv ? k_io_cn_z_jac<W, true> : k_io_cn_z_jac<W, false>), dim3(ng * M, 3), dim3(Tz * NTz), sizeof(double) * (4 * Tz * NTz + 1),
-                               m->stream, d, facoe);
+            const dim3 gz(ng * M, 3), bz(Tz * NTz);
+            const size_t sz = sizeof(double) * (4 * Tz * NTz + 1);
+            if (Pz == 8 && m->cn_fastdiv) hipLaunchKernelGGL((k_io_cn_z_jac<W, true, 8>), gz, bz, sz, m->stream, d, facoe);
+            else if (Pz == 8) hipLaunchKernelGGL((k_io_cn_z_jac<W, false, 8>), gz, bz, sz, m->stream, d, facoe);
+            else if (m->cn_fastdiv) hipLaunchKernelGGL((k_io_cn_z_jac<W, true, 4>), gz, bz, sz, m->stream, d, facoe);
+            else hipLaunchKernelGGL((k_io_cn_z_jac<W, false, 4>), gz, bz, sz, m->stream, d, facoe);
         } else {
             hipLaunchKernelGGL(k_io_cn_z<W>, dim3((M * L + 127) / 128, 3), dim3(128), 0, m->stream, d, facoe);
         }
-        const int NTy = (M + P - 1) / P;   // y: block = Ty x-columns x NTy chunks, whole waves
+        const int NTy = (M + Py - 1) / Py;   // y: block = Ty x-columns x NTy chunks, whole waves
         int Ty = NTy <= 1024 ? std::min(m->cny_cols ? m->cny_cols : m->cn_cols, 1024 / NTy) : 0;
         if (Ty > 0) Ty -= Ty % (64 / std::gcd(NTy, 64));
         if (Ty > 0) {
             const int ng = (L + Ty - 1) / Ty;
-            hipLaunchKernelGGL((m->cn_fastdiv ? k_io_cn_y_jac<W, true> : k_io_cn_y_jac<W, false>), dim3(ng * N, 3), dim3(Ty * NTy), sizeof(double) * (4 * Ty * NTy + 1),
-                               m->stream, d, gs, facoe);
+            const dim3 gy(ng * N, 3), by(Ty * NTy);
+            const size_t sy = sizeof(double) * (4 * Ty * NTy + 1);
+            if (Py == 8 && m->cn_fastdiv) hipLaunchKernelGGL((k_io_cn_y_jac<W, true, 8>), gy, by, sy, m->stream, d, gs, facoe);
+            else if (Py == 8) hipLaunchKernelGGL((k_io_cn_y_jac<W, false, 8>), gy, by, sy, m->stream, d, gs, facoe);
+            else if (m->cn_fastdiv) hipLaunchKernelGGL((k_io_cn_y_jac<W, true, 4>), gy, by, sy, m->stream, d, gs, facoe);
+            else hipLaunchKernelGGL((k_io_cn_y_jac<W, false, 4>), gy, by, sy, m->stream, d, gs, facoe);
         } else {
             hipLaunchKernelGGL(k_io_cn_y<W>, dim3((L * N + 127) / 128, 3), dim3(128), 0, m->stream, d, gs, facoe);
         }

@@ -509,7 +509,8 @@ int chs_substep_io_momentum(chs_io *m, int ns, double dt);
  * default, 2, 1; the block's other rows of threads take further z-planes); "cn_cols": most
  * x-columns per block of the Crank-Nicolson z and y Jacobi solves (1..64, default 16; rounded
  * down to whole wavefronts, the sequential solves where that leaves none), "cnz_cols" and
- * "cny_cols" the same for one direction (0: cn_cols); "cnx_wave": the x-solve one wavefront
+ * "cny_cols" the same for one direction (0: cn_cols), "cnz_p" / "cny_p" their line points per
+ * thread (8 default, or 4); "cnx_wave": the x-solve one wavefront
  * per line (1, default, for L = 64, 128, 256, 512) or one block per line (0); "cn_fastdiv":
  * the z and y solves' coefficients by Newton reciprocals (1, default) or IEEE division (0),
  * which changes them by an ulp, below the 1e-17 the sweeps resolve.  Otherwise results change

@@ -179,13 +179,16 @@ def test_gpu_cn_substeps_match_reference(pkg, golden):
                                                   (48, 37, 30, 1, dict(mom_kz=0, mom_rows=1), 64),
                                                   (48, 37, 30, 1, dict(mom_by=8, mom_kz=7), 16),
                                                   (128, 64, 96, 1, dict(mom_kz=0, mom_rows=4), 64),
-                                                  (64, 48, 32, 2, dict(cn_fastdiv=0), 16)])
+                                                  (64, 48, 32, 2, dict(cn_fastdiv=0), 16),
+                                                  (64, 48, 32, 1, dict(cnz_p=4, cny_p=4), 16),
+                                                  (48, 37, 30, 1, dict(cnz_p=4, cny_p=8), 8)])
 def test_gpu_cn_matches_oracle(pkg, L, M, N, steps, mom, cols):
     """Larger seeded states with a noisy current viscosity against the oracle (gravity and
     constant mass flux on), with each momentum kernel and block shape (the LDS-staged z-march,
     mom_kz planes of mom_by rows, or the per-point kernel, mom_kz = 0, with mom_rows rows;
-    ragged M, N), several line-solve block widths (x-columns per block), and the z and y
-    coefficients by IEEE division (cn_fastdiv = 0) as well as by Newton reciprocals."""
+    ragged M, N), several line-solve block widths (x-columns per block) and points per thread
+    (cnz_p, cny_p 4 or 8), and the z and y coefficients by IEEE division (cn_fastdiv = 0) as
+    well as by Newton reciprocals."""
     print(f"io cn oracle {L}x{M}x{N} mom {mom} cols {cols}", flush=True)
     mesh = setup.Mesh(L, M, N, STR2=2.0)
     st = oracle.io_initial_field(mesh, 90 + L)
