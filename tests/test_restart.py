@@ -64,3 +64,40 @@ def test_interchange_with_reference(pkg, golden):
         np.testing.assert_array_equal(pr, p2)
     finally:
         h.close()
+
+
+@pytest.mark.gpu
+def test_gpu_resume_from_restart_files(pkg, tmp_path):
+    """Checkpoint from device state and resume: 2 RK3 steps on the GPU, the fields downloaded
+    and written as the reference's restart files, read back into a fresh context (CONVH0
+    zero, as after the reference's restart), 2 more steps; bit-identical to 4 uninterrupted
+    steps (the first substep after a restart weights CONVH0 by TROH(1) = 0)."""
+    L, M, N, REN, dt = 32, 24, 16, 2800.0, 2e-3
+    print(f"resume {L}x{M}x{N}", flush=True)
+    co = pkg.channel_setup(L, M, N, STR2=1.8)
+    q0, p0 = pkg.poiseuille_perturbed(co, 5, 0.1)
+    ref = pkg.ChannelTG(co, pkg.PLAIN_FP64)
+    ref.upload(q0, p0)
+    ref.SOLVERRK3_MOM_tg(0, dt, 1.0 / REN)
+    for _ in range(4):
+        ref.rk3_step(dt, 1.0 / REN)
+    qr, pr = ref.download()
+    ref.close()
+    a = pkg.ChannelTG(co, pkg.PLAIN_FP64)
+    a.upload(q0, p0)
+    a.SOLVERRK3_MOM_tg(0, dt, 1.0 / REN)
+    for _ in range(2):
+        a.rk3_step(dt, 1.0 / REN)
+    qa, pa = a.download()
+    a.close()
+    pkg.restart_write_tg(tmp_path, qa, pa, 2, 2 * dt, REN, dt)
+    q2, p2, it, t = pkg.restart_read_tg(tmp_path, 2 * dt, L, M, N)
+    assert it == 2 and t == 2 * dt
+    b = pkg.ChannelTG(co, pkg.PLAIN_FP64)
+    b.upload(q2, p2)
+    for _ in range(2):
+        b.rk3_step(dt, 1.0 / REN)
+    qb, pb = b.download()
+    b.close()
+    np.testing.assert_array_equal(qb, qr)
+    np.testing.assert_array_equal(pb, pr)
