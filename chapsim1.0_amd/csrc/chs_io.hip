@@ -1465,10 +1465,16 @@ __device__ __forceinline__ void io_cn_y_jac(const IoDev &d, const double *__rest
     e1[threadIdx.x] = f0[P - 1];
     e2[threadIdx.x] = f1[0];
     e3[threadIdx.x] = f1[P - 1];
+    // the component's AMVR / APVR rows in LDS: a wavefront spans several row chunks, so their
+    // per-row reads are not wave-uniform
+    double *am = lds + 4 * S + 1, *ap = am + (M + 2);
+    for (int q = threadIdx.x; q < M + 2; q += S) {
+        am[q] = d.amvr[C * (M + 2) + q];
+        ap[q] = d.apvr[C * (M + 2) + q];
+    }
     __syncthreads();
     const double f0m = sl >= 0 ? e1[sl] : 0.0, f1m = sl >= 0 ? e3[sl] : 1.0;
     const double f0p = sr >= 0 ? e0[sr] : 0.0, f1p = sr >= 0 ? e2[sr] : 1.0;
-    const double *am = d.amvr + C * (M + 2), *ap = d.apvr + C * (M + 2);
     double a[P], c[P], ib[P], rho = 0.0;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
@@ -2105,7 +2111,7 @@ void io_momentum_launch(chs_io *m, const double *gs, double tg, double tr, doubl
         if (Ty > 0) {
             const int ng = (L + Ty - 1) / Ty;
             const dim3 gy(ng * N, 3), by(Ty * NTy);
-            const size_t sy = sizeof(double) * (4 * Ty * NTy + 1);
+            const size_t sy = sizeof(double) * (4 * Ty * NTy + 1 + 2 * (M + 2));
             if (Py == 8 && m->cn_fastdiv) hipLaunchKernelGGL((k_io_cn_y_jac<W, true, 8>), gy, by, sy, m->stream, d, gs, facoe);
             else if (Py == 8) hipLaunchKernelGGL((k_io_cn_y_jac<W, false, 8>), gy, by, sy, m->stream, d, gs, facoe);
             else if (m->cn_fastdiv) hipLaunchKernelGGL((k_io_cn_y_jac<W, true, 4>), gy, by, sy, m->stream, d, gs, facoe);
