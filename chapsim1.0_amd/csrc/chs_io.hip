@@ -98,16 +98,25 @@ enum IoPF { PD, PV, PGX, PG0X, PGY, PG0Y };
 
 // One point (i, J, k), J = 1..M+1: the plane-(k-1) values (D, V at (i, J); V at (i-1, J) and
 // (i, J-1)) and G_z, G0_z at (i, J) of planes k and k+1 come as values.
+// Row J's mesh coefficients of the staggered-property point: DYFI(min(J, M)), YCL2ND_WFF/WFB(J).
+struct IoPropRow {
+    double coe2, wff, wfb;
+};
+__device__ __forceinline__ IoPropRow io_prop_row(const IoDev &d, int J)
+{
+    return {d.dyfi[min(J, d.M)], d.wff[J], d.wfb[J]};
+}
+
 template <typename A>
-__device__ __forceinline__ void io_props_point(const IoDev &d, const A &X, int i, int J, int k, double tg, double tr,
-                                               double Dm, double Vm, double Vmim, double Vmb, double g2c,
-                                               double g2p, double g02c, double g02p)
+__device__ __forceinline__ void io_props_point(const IoDev &d, const A &X, const IoPropRow &rc, int i, int J, int k,
+                                               double tg, double tr, double Dm, double Vm, double Vmim, double Vmb,
+                                               double g2c, double g2p, double g02c, double g02p)
 {
 #pragma clang fp contract(off)
     const int M = d.M;
     const size_t hs = hsz(d), c0 = hx(d, i, J, k);
     const double h = 0.5;
-    const double coe2 = d.dyfi[min(J, M)], coe3 = d.dzi, wff = d.wff[J], wfb = d.wfb[J];
+    const double coe2 = rc.coe2, coe3 = d.dzi, wff = rc.wff, wfb = rc.wfb;
     const double Dc = X(PD, 0, 0), Vc = X(PV, 0, 0), Vim = X(PV, -1, 0);
     const double Vb = X(PV, 0, -1), Vimb = X(PV, -1, -1);
     double gx = 0.0, gy = 0.0, gz = g2c, dsx = 0.0, dsy, dsz = 0.0;
@@ -161,6 +170,7 @@ __global__ __launch_bounds__(256) void k_io_props_z(IoDev d, double tg, double t
     const int ip = i == L - 1 ? 0 : i + 1, im = i == 0 ? L - 1 : i - 1;
     const size_t hs = hsz(d);
     const double *D = d.d0, *V = d.m0, *G = d.g, *G0 = d.g0;
+    const IoPropRow rc = io_prop_row(d, J);
     const int kb = k0 == 0 ? N - 1 : k0 - 1;
     double Dm = D[hx(d, i, J, kb)], Vm = V[hx(d, i, J, kb)], Vmim = V[hx(d, im, J, kb)];
     double Vmb = V[hx(d, i, J - 1, kb)];
@@ -180,7 +190,7 @@ __global__ __launch_bounds__(256) void k_io_props_z(IoDev d, double tg, double t
             const double *a = f == PD ? D : f == PV ? V : f == PGX ? G : f == PG0X ? G0 : f == PGY ? G + hs : G0 + hs;
             return a[hx(d, di > 0 ? ip : (di < 0 ? im : i), J + dj, k)];
         };
-        io_props_point(d, X, i, J, k, tg, tr, Dm, Vm, Vmim, Vmb, g2c, g2p, g02c, g02p);
+        io_props_point(d, X, rc, i, J, k, tg, tr, Dm, Vm, Vmim, Vmb, g2c, g2p, g02c, g02p);
         Dm = X(PD, 0, 0);
         Vm = X(PV, 0, 0);
         Vmim = X(PV, -1, 0);
@@ -234,6 +244,7 @@ __global__ __launch_bounds__(256) void k_io_props_lds(IoDev d, double tg, double
     };
     const int ir = act ? i : 0, Jr = act ? J : 1, imr = ir == 0 ? L - 1 : ir - 1;
     const bool gz = act && J <= M;   // rows with a z mass flux
+    const IoPropRow rc = io_prop_row(d, act ? J : 1);
     const int kb = k0 == 0 ? N - 1 : k0 - 1;
     double Dm = d.d0[hx(d, ir, Jr, kb)], Vm = d.m0[hx(d, ir, Jr, kb)], Vmim = d.m0[hx(d, imr, Jr, kb)];
     double Vmb = d.m0[hx(d, ir, Jr - 1, kb)];
@@ -260,7 +271,7 @@ __global__ __launch_bounds__(256) void k_io_props_lds(IoDev d, double tg, double
             load(k + 1 == N ? 0 : k + 1);
         }
         auto X = [&](int f, int di, int dj) { return tiles[f * TS + cell + dj * TW + di]; };
-        if (act) io_props_point(d, X, i, J, k, tg, tr, Dm, Vm, Vmim, Vmb, g2c, g2p, g02c, g02p);
+        if (act) io_props_point(d, X, rc, i, J, k, tg, tr, Dm, Vm, Vmim, Vmb, g2c, g2p, g02c, g02p);
         Dm = X(PD, 0, 0);
         Vm = X(PV, 0, 0);
         Vmim = X(PV, -1, 0);
