@@ -21,6 +21,7 @@
 #include "../../include/chapsim_hip.h"
 
 hipStream_t chs_internal_stream(chs_ctx *c);
+int chs_internal_poisson_device_fishpack(chs_ctx *c, const double *rhs, double *phi);
 void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **m, double **g);
 void chs_internal_eng_set_g(chs_eng *e, const double *g);
 double *chs_internal_eng_density(chs_eng *e);
@@ -1752,6 +1753,7 @@ struct chs_io {
     double *own_m1 = nullptr;                       // d.m1 likewise (Crank-Nicolson only)
     double *cnbuf = nullptr;                        // m1, work, amvr, apvr
     float *wbuf = nullptr;                          // d.explt0f, d.hf (mixed form)
+    int pois_fishpack = 0;                          // pressure solve with FISHPACK's (0,0)-mode gauge
     int mixed = 0;                                  // EXPLT0 and the increment in FP32
     int cn = 0;                                     // iVisScheme
     bool q_fresh = false;   // Q = G / D_STG of the current G and D_STG (written by chs_io_props)
@@ -1914,6 +1916,13 @@ int chs_io_upload(chs_io *m, const double *g, const double *g0, const double *pr
     if (he == hipSuccess && m->mixed) he = io_convert(d.explt0, d.explt0f, 3 * cb / sizeof(double), m->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
     return io_hip(m, he, "chs_io_upload");
+}
+
+int chs_io_set_poisson(chs_io *m, int fishpack)
+{
+    if (!m || (fishpack != 0 && fishpack != 1)) return io_fail(m, CHS_E_ARG, "chs_io_set_poisson: bad argument");
+    m->pois_fishpack = fishpack;
+    return CHS_OK;
 }
 
 int chs_io_set_mixed(chs_io *m, int mixed)
@@ -2162,7 +2171,8 @@ int chs_substep_io(chs_io *m, int ns, double dt)
     hipLaunchKernelGGL(k_io_divg, gin, blk, 0, m->stream, d, 1.0 / dt / ta);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
-    rc = chs_poisson_device(m->pois, d.rhsp, d.dph, 0, nullptr);
+    rc = m->pois_fishpack ? chs_internal_poisson_device_fishpack(m->pois, d.rhsp, d.dph)
+                          : chs_poisson_device(m->pois, d.rhsp, d.dph, 0, nullptr);
     if (rc) return io_fail(m, rc, std::string("chs_substep_io: Poisson: ") + chs_last_error(m->pois));
     hipLaunchKernelGGL(k_io_correct, gall, blk, 0, m->stream, d, dt * ta, m->cn, 0.5 * ta * dt * d.cvisc);
     return io_hip(m, hipSuccess, "chs_substep_io");

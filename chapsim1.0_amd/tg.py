@@ -102,6 +102,7 @@ SIGNATURES = {
     "chs_substep_io_momentum": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "chs_io_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "chs_io_set_mixed": (C.c_int, [C.c_void_p, C.c_int]),
+    "chs_io_set_poisson": (C.c_int, [C.c_void_p, C.c_int]),
     "chs_io_upload_viscosity": (C.c_int, [C.c_void_p, _dp]),
     "chs_cfl_io": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
     "chs_max_div_io": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
@@ -508,6 +509,10 @@ class ChannelIO:
     def momentum(self, ns, dt):
         """The momentum part of the substep alone: G before the projection."""
         self._check(lib().chs_substep_io_momentum(self._h, int(ns), float(dt)), "chs_substep_io_momentum")
+
+    def set_poisson(self, fishpack: bool):
+        """Pressure solver (chs_io_set_poisson): FFT99's gauge (False) or FISHPACK's (True)."""
+        self._check(lib().chs_io_set_poisson(self._h, int(bool(fishpack))), "chs_io_set_poisson")
 
     def set_mixed(self, mixed: bool):
         """Mixed form (chs_io_set_mixed): EXPLT0 and the momentum increment in FP32."""

@@ -526,6 +526,16 @@ int chs_io_set_tuning(chs_io *m, const char *key, int value);
  * reference, which computes in FP64 throughout.  Synchronous; CHS_E_ARG for another value. */
 int chs_io_set_mixed(chs_io *m, int mixed);
 
+/* Pressure solver of the main-domain substep, the reference's is_FFT_FFT99 / is_FFT_FISHPACK
+ * switch (modules.f90:49-50, SOLVERRK3_MOM_IO.f90:89-94): fishpack = 0 (default) solves with
+ * FFT99_POIS3D_periodicxz's gauge of the (0,0) mode, 1 with FISHPACK_POIS3D_SIMPLE's (its last
+ * row pinned, possion3d_fishpack.f90:329-331; the solver of chs_poisson_fishpack).  The two
+ * differ by a constant in DPH and hence in the pressure.  The reference's own FISHPACK call in
+ * the substep passes DPH_io(NCL1_io, 0:N2DO+1, NCL3) to the solver's dummy VAR(NCL1_io, N2DO,
+ * NCL3), so its solution lands one row shifted (SOLVERRK3_MOM_IO.f90:89-94); that is not
+ * reproduced.  CHS_E_ARG for another value. */
+int chs_io_set_poisson(chs_io *m, int fishpack);
+
 /* Copy fields back (NULL skips one): g, q, dstg, mstg [3][N][M+2][L]; pr, dph [N][M+2][L]
  * (dph's ghost rows copy the wall cells, BC_WALL_DPH_io); explt0 [3][N][M][L]; drho, rhsp
  * [N][M][L].  Synchronous. */

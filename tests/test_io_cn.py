@@ -547,3 +547,46 @@ def test_gpu_io_mixed_form(pkg, vis):
     with pytest.raises(pkg.ChsError):
         ch2._check(pkg.lib().chs_io_set_mixed(ch2._h, 2), "chs_io_set_mixed")
     ch2.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("vis", [0, 1])
+def test_gpu_io_fishpack_gauge(pkg, vis):
+    """chs_io_set_poisson(1): the substep's pressure solve with FISHPACK_POIS3D_SIMPLE's gauge of
+    the (0,0) mode.  G and Q agree with the FFT99-gauge substeps to rounding and the pressure
+    differs by a constant (the gauges fix only that mode's constant), and DPH equals what
+    chs_poisson_fishpack (pinned against the reference's FISHPACK) gives for the substep's
+    Poisson right-hand side."""
+    L, M, N = 64, 48, 32
+    print(f"io fishpack gauge vis {vis}", flush=True)
+    mesh = setup.Mesh(L, M, N, STR2=2.0)
+    st = oracle.io_initial_field(mesh, 61)
+    if vis == 1:
+        st = dict(st, M1=cn_viscosity(st, 61, False))
+    co = pkg.channel_setup(L, M, N, STR2=2.0)
+    out = {}
+    for fish in (False, True):
+        ch = pkg.ChannelIO(co, 4000.0, gravity=1, f_a=0.02, flow_driven=1, vis_scheme=vis)
+        ch.set_poisson(fish)
+        ch.upload(**st)
+        for ns in (1, 2, 3):
+            ch.ioprops(ns)
+            ch.SOLVERRK3_MOM_io(ns, 5e-4)
+        out[fish] = ch.download()
+        ch.close()
+    f0, f1 = out[False], out[True]
+    for k in ("G", "Q"):
+        for c in range(3):
+            assert rel(f1[k][c], f0[k][c]) < 1e-12, (k, c, rel(f1[k][c], f0[k][c]))
+    dp = f1["PR"] - f0["PR"]
+    spread = float(dp.max() - dp.min()) / float(np.max(np.abs(f0["PR"])))
+    print(f"  pressure offset {float(dp.mean()):.3e}, spread {spread:.2e}", flush=True)
+    assert spread < 1e-10
+    tg = pkg.ChannelTG(co, pkg.PLAIN_FP64)
+    phi = tg.FISHPACK_POIS3D_SIMPLE(f1["RHSP"])
+    tg.close()
+    assert rel(f1["DPH"][:, 1:M + 1], phi) < 1e-13, rel(f1["DPH"][:, 1:M + 1], phi)
+    ch = pkg.ChannelIO(co, 4000.0)
+    with pytest.raises(pkg.ChsError):
+        ch._check(pkg.lib().chs_io_set_poisson(ch._h, 2), "chs_io_set_poisson")
+    ch.close()
