@@ -259,3 +259,37 @@ def test_autotune_restores_state(pkg):
     for c in range(3):
         assert rel(gq[c], o.q[c]) < 1e-10
     ch.close()
+
+
+def test_config1_100_steps_match_oracle(pkg):
+    """The CPU-runnable configuration as configured: 64^3, Re_tau 180 (REN 2800), tanh beta 1.8,
+    laminar profile + 10 % seeded perturbations projected by NS = 0, dt 1e-3, 100 RK3 steps,
+    plain FP64, against the oracle after every 10 steps (velocity, pressure, x-z mean u,
+    maximum divergence)."""
+    L, M, N, STR2, REN, dt = 64, 64, 64, 1.8, 2800.0, 1.0e-3
+    print(f"config 1 {L}x{M}x{N}, 100 steps", flush=True)
+    mesh, ch, o = make(pkg, L, M, N, STR2, REN, seed=180)
+    cv = 1.0 / REN
+    ch.SOLVERRK3_MOM_tg(0, dt, cv)
+    o.substep(0, dt)
+    worst = dict(u=0.0, p=0.0, mean=0.0)
+    for step in range(1, 101):
+        ch.rk3_step(dt, cv)
+        for ns in (1, 2, 3):
+            o.substep(ns, dt)
+        if step % 10 == 0:
+            gq, gp = ch.download()
+            e = max(rel(gq[c], o.q[c]) for c in range(3))
+            e_p = rel(gp, o.pr)
+            um_g = gq[0][:, 1:M + 1].mean(axis=(0, 2))
+            um_o = o.q[0][:, 1:M + 1].mean(axis=(0, 2))
+            e_m = float(np.max(np.abs(um_g - um_o)) / np.max(np.abs(um_o)))
+            worst = dict(u=max(worst["u"], e), p=max(worst["p"], e_p), mean=max(worst["mean"], e_m))
+            print(f"  step {step}: velocity {e:.2e}, pressure {e_p:.2e}, mean u {e_m:.2e}", flush=True)
+    div = ch.max_div()
+    print(f"  worst {worst}; max div {div:.2e}", flush=True)
+    # measured on MI355X: velocity 7.6e-15, pressure 3.4e-12, mean u 4.4e-16, div 1.6e-15
+    assert worst["u"] < 1e-13 and worst["mean"] < 1e-14, worst
+    assert worst["p"] < 1e-10, worst
+    assert div < 1e-12
+    ch.close()
