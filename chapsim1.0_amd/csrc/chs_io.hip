@@ -21,7 +21,8 @@
 #include "../../include/chapsim_hip.h"
 
 hipStream_t chs_internal_stream(chs_ctx *c);
-int chs_internal_poisson_device_fishpack(chs_ctx *c, const double *rhs, double *phi);
+int chs_internal_poisson_in_place(chs_ctx *c, const double *rhs, int gauge);
+const double *chs_internal_phi(chs_ctx *c);
 void chs_internal_eng_fields(chs_eng *e, hipStream_t *s, double **d0, double **m0, double **m, double **g);
 void chs_internal_eng_set_g(chs_eng *e, const double *g);
 double *chs_internal_eng_density(chs_eng *e);
@@ -43,7 +44,8 @@ struct IoDev {
     double *pr, *d0, *m0;                     // halo fields
     double *explt0, *h;                       // 3 cell fields each
     float *explt0f, *hf;                      // the same in the mixed form (work precision FP32)
-    double *drho, *divu, *rhsp, *dph;         // cell fields
+    double *drho, *divu, *rhsp;               // cell fields
+    const double *dph;                        // DPH_io: the Poisson context's solution, halo field
     double *partial, *scal;
     // Crank-Nicolson scheme (iVisScheme = 1)
     double *m1;                               // Viscousity, the current viscosity (halo field)
@@ -1605,7 +1607,7 @@ __global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt, int cn
     if (i >= L || J > M + 1) return;
     const size_t hs = hsz(d), c0 = hx(d, i, J, k);
     if (J >= 1 && J <= M) {   // the wall cells' threads also write the ghost-row copies
-        double p = d.pr[c0] + d.dph[cx(d, i, J, k)];
+        double p = d.pr[c0] + d.dph[c0];
         if (cn) p = p - be * d.rhsp[cx(d, i, J, k)];
         d.pr[c0] = p;
         if (J == 1) d.pr[hx(d, i, 0, k)] = p;
@@ -1619,9 +1621,9 @@ __global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt, int cn
         return;
     }
     const int im = i == 0 ? L - 1 : i - 1, km = k == 0 ? N - 1 : k - 1;
-    const double ph = d.dph[cx(d, i, J, k)];
-    const double gx = d.g[c0] - (ph - d.dph[cx(d, im, J, k)]) * (tadt * d.dxi);
-    const double gz = d.g[2 * hs + c0] - (ph - d.dph[cx(d, i, J, km)]) * (tadt * d.dzi);
+    const double ph = d.dph[c0];
+    const double gx = d.g[c0] - (ph - d.dph[hx(d, im, J, k)]) * (tadt * d.dxi);
+    const double gz = d.g[2 * hs + c0] - (ph - d.dph[hx(d, i, J, km)]) * (tadt * d.dzi);
     d.g[c0] = gx;
     d.g[2 * hs + c0] = gz;
     d.q[c0] = gx / d.dstg[c0];
@@ -1630,7 +1632,7 @@ __global__ __launch_bounds__(256) void k_io_correct(IoDev d, double tadt, int cn
         d.g[hs + c0] = 0.0;
         d.q[hs + c0] = 0.0;
     } else {
-        const double gy = d.g[hs + c0] - (ph - d.dph[cx(d, i, J - 1, k)]) * (tadt * d.dyci[J]);
+        const double gy = d.g[hs + c0] - (ph - d.dph[hx(d, i, J - 1, k)]) * (tadt * d.dyci[J]);
         d.g[hs + c0] = gy;
         d.q[hs + c0] = gy / d.dstg[hs + c0];
     }
@@ -1754,6 +1756,7 @@ struct chs_io {
     double *cnbuf = nullptr;                        // m1, work, amvr, apvr
     float *wbuf = nullptr;                          // d.explt0f, d.hf (mixed form)
     int pois_fishpack = 0;                          // pressure solve with FISHPACK's (0,0)-mode gauge
+    bool dph_valid = false;                         // a substep has solved for DPH
     int mixed = 0;                                  // EXPLT0 and the increment in FP32
     int cn = 0;                                     // iVisScheme
     bool q_fresh = false;   // Q = G / D_STG of the current G and D_STG (written by chs_io_props)
@@ -1808,7 +1811,7 @@ int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
     const size_t npart_alloc = (size_t)((L + 63) / 64) *
         std::max({(size_t)((M + 3) / 4) * N, (size_t)((M + 1) / 2) * ((N + 1) / 2), (size_t)M * ((N + 3) / 4)});
     const size_t ng = 4 * (size_t)(M + 2);
-    const size_t nb = 18 * hs + 10 * cs + ng + npart_alloc + 4;
+    const size_t nb = 18 * hs + 9 * cs + ng + npart_alloc + 4;
     hipError_t he = hipMalloc(&m->buf, sizeof(double) * nb);
     if (he == hipSuccess) he = hipMemsetAsync(m->buf, 0, sizeof(double) * nb, m->stream);
     IoDev &d = m->d;
@@ -1817,8 +1820,9 @@ int chs_io_create(const chs_coeffs *c, const chs_io_params *p, chs_io **out)
     d.pr = f + 15 * hs; d.d0 = f + 16 * hs; d.m0 = f + 17 * hs;
     m->own_d0 = d.d0; m->own_m0 = d.m0;
     f += 18 * hs;
-    d.explt0 = f; d.h = f + 3 * cs; d.drho = f + 6 * cs; d.divu = f + 7 * cs; d.rhsp = f + 8 * cs; d.dph = f + 9 * cs;
-    f += 10 * cs;
+    d.explt0 = f; d.h = f + 3 * cs; d.drho = f + 6 * cs; d.divu = f + 7 * cs; d.rhsp = f + 8 * cs;
+    f += 9 * cs;
+    d.dph = chs_internal_phi(m->pois);
     double *geo = f;
     d.partial = f + ng;
     d.scal = d.partial + npart_alloc;
@@ -2171,8 +2175,8 @@ int chs_substep_io(chs_io *m, int ns, double dt)
     hipLaunchKernelGGL(k_io_divg, gin, blk, 0, m->stream, d, 1.0 / dt / ta);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return io_hip(m, he, "chs_substep_io");
-    rc = m->pois_fishpack ? chs_internal_poisson_device_fishpack(m->pois, d.rhsp, d.dph)
-                          : chs_poisson_device(m->pois, d.rhsp, d.dph, 0, nullptr);
+    rc = chs_internal_poisson_in_place(m->pois, d.rhsp, m->pois_fishpack);   // DPH_io is its solution array
+    m->dph_valid = true;
     if (rc) return io_fail(m, rc, std::string("chs_substep_io: Poisson: ") + chs_last_error(m->pois));
     hipLaunchKernelGGL(k_io_correct, gall, blk, 0, m->stream, d, dt * ta, m->cn, 0.5 * ta * dt * d.cvisc);
     return io_hip(m, hipSuccess, "chs_substep_io");
@@ -2185,7 +2189,6 @@ int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, do
     const int L = m->L, M = m->M, N = m->N;
     const size_t hs = (size_t)N * (M + 2) * L, cs = (size_t)N * M * L, hb = sizeof(double) * hs, cb = sizeof(double) * cs;
     const IoDev &d = m->d;
-    std::vector<double> ph;
     hipError_t he = hipSuccess;
     if (explt0 && m->mixed) he = io_convert(d.explt0f, d.explt0, 3 * cs, m->stream);   // the FP32 history
     struct { double *dst; const double *src; size_t bytes; } cp[] = {
@@ -2193,18 +2196,16 @@ int chs_io_download(chs_io *m, double *g, double *q, double *pr, double *dph, do
         {drho, d.drho, cb}, {rhsp, d.rhsp, cb}, {dstg, d.dstg, 3 * hb}, {mstg, d.mstg, 3 * hb}};
     for (auto &x : cp)
         if (x.dst && he == hipSuccess) he = hipMemcpyAsync(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost, m->stream);
-    if (dph && he == hipSuccess) {
-        ph.resize(cs);
-        he = hipMemcpyAsync(ph.data(), d.dph, cb, hipMemcpyDeviceToHost, m->stream);
-    }
+    if (dph && m->dph_valid && he == hipSuccess) he = hipMemcpyAsync(dph, d.dph, hb, hipMemcpyDeviceToHost, m->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(m->stream);
     if (he != hipSuccess) return io_hip(m, he, "chs_io_download");
-    if (dph) {   // halo layout, ghost rows copy the wall cells (BC_WALL_DPH_io)
+    if (dph && !m->dph_valid) std::fill(dph, dph + hs, 0.0);   // no solve yet
+    if (dph) {   // ghost rows copy the wall cells (BC_WALL_DPH_io)
         for (int k = 0; k < N; ++k)
-            for (int J = 0; J <= M + 1; ++J) {
-                const int Js = J == 0 ? 1 : (J == M + 1 ? M : J);
-                for (int i = 0; i < L; ++i)
-                    dph[((size_t)k * (M + 2) + J) * L + i] = ph[((size_t)k * M + Js - 1) * L + i];
+            for (int J = 0; J <= M + 1; J += M + 1) {
+                const int Js = J == 0 ? 1 : M;
+                std::copy(dph + ((size_t)k * (M + 2) + Js) * L, dph + ((size_t)k * (M + 2) + Js + 1) * L,
+                          dph + ((size_t)k * (M + 2) + J) * L);
             }
     }
     return CHS_OK;

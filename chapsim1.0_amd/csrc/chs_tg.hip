@@ -3217,13 +3217,13 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
 // sweeps: r = rhs - L phi in FP64, phi += solve(r).  resmax (optional) gets max |r| after
 // the last sweep.
 template <typename T>
-int poisson_device_impl(chs_ctx *c, const double *rhs, double *phi, int nref, double *resmax, int gauge = 0)
+int poisson_device_impl(chs_ctx *c, const double *rhs, double *phi, int nref, double *resmax)
 {
     const Geo &g = c->geo;
     const dim3 blk(64, 4), grd((c->L + 63) / 64, (c->M + 3) / 4, c->N);
     const int npart = grd.x * grd.y * grd.z;
     const double *acph = c->cf_acph;
-    poisson_launch<T>(c, nullptr, 1.0, rhs, gauge);
+    poisson_launch<T>(c, nullptr, 1.0, rhs);
     hipLaunchKernelGGL(k_phi_accum, grd, blk, 0, c->stream, g, c->phi, phi, 0, (const double *)nullptr);
     for (int r = 0; r < nref; ++r) {
         hipLaunchKernelGGL(k_poisson_resid, grd, blk, 0, c->stream, g, c->d_amph, acph, c->d_apph, rhs, phi,
@@ -4436,15 +4436,17 @@ int chs_ens_sync(chs_ens *e)
 // solves (the main-domain momentum substep uses a context's Poisson solver).
 __attribute__((visibility("hidden"))) hipStream_t chs_internal_stream(chs_ctx *c) { return c ? c->stream : nullptr; }
 
-// The device Poisson solve with FISHPACK_POIS3D_SIMPLE's gauge of the (0,0) mode (TRID0 pins its
-// last row) instead of FFT99's, for the main-domain substep's solver switch; no refinement.
-__attribute__((visibility("hidden"))) int chs_internal_poisson_device_fishpack(chs_ctx *c, const double *rhs,
-                                                                                double *phi)
+// The device Poisson solve of a plain FP64 context left in the context's own solution array
+// (halo layout [N][M+2][L], ghost rows of BC_WALL_DPH), which chs_internal_phi returns: for a
+// caller that reads the solution in place instead of a copy.  gauge 1: FISHPACK's (0,0)-mode gauge.
+__attribute__((visibility("hidden"))) int chs_internal_poisson_in_place(chs_ctx *c, const double *rhs, int gauge)
 {
-    if (!c || !rhs || !phi) return fail(c, CHS_E_ARG, "null argument");
-    return c->mode == CHS_MIXED ? poisson_device_impl<float>(c, rhs, phi, 0, nullptr, 1)
-                                : poisson_device_impl<double>(c, rhs, phi, 0, nullptr, 1);
+    if (!c || !rhs || c->mode != CHS_PLAIN_FP64) return fail(c, CHS_E_ARG, "in-place Poisson: bad argument");
+    poisson_launch<double>(c, nullptr, 1.0, rhs, gauge);
+    HIPCHK(c, hipGetLastError());
+    return 0;
 }
+__attribute__((visibility("hidden"))) const double *chs_internal_phi(chs_ctx *c) { return c ? c->phi : nullptr; }
 
 extern "C" int chs_mean_zx_tg(chs_ctx *c, double *stats, double *scal)
 {
