@@ -273,6 +273,23 @@ MODULE chapsim_hip
             INTEGER(C_INT) :: rc
         END FUNCTION
 
+        ! pressure solver of the substep: 0 FFT99's gauge, 1 FISHPACK_POIS3D_SIMPLE's
+        ! (is_FFT_FFT99 / is_FFT_FISHPACK, modules.f90:49-50)
+        FUNCTION chs_io_set_poisson(io, fishpack) BIND(C, NAME='chs_io_set_poisson') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: io
+            INTEGER(C_INT), VALUE :: fishpack
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
+        ! mixed form: EXPLT0_io and RHS_io in FP32 (1) or FP64 throughout (0)
+        FUNCTION chs_io_set_mixed(io, mixed) BIND(C, NAME='chs_io_set_mixed') RESULT(rc)
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: io
+            INTEGER(C_INT), VALUE :: mixed
+            INTEGER(C_INT) :: rc
+        END FUNCTION
+
         FUNCTION chs_io_props(io, ns) BIND(C, NAME='chs_io_props') RESULT(rc)
             IMPORT :: C_INT, C_PTR
             TYPE(C_PTR), VALUE :: io
