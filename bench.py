@@ -63,6 +63,8 @@ def alg_bytes(kernel: str, L: int, M: int, N: int, wbytes: int) -> float:
         # y stencil + Q update of one component: R RHS, R q, W predicted q (one launch per
         # component, or the three in one launch)
         "cn_y": F * wbytes + 2 * F * 8,
+        # z and y solves in one pass, three components: R RHS x3, R q x3, W predicted q x3
+        "cn_zy": 3 * F * wbytes + 6 * F * 8,
         # R predicted u,v,w; W spectrum (the correction recomputes the divergence)
         "divg_fftx": 3 * F * 8 + S * 2 * wbytes,
         "fftz_fwd": 2 * S * 2 * wbytes,
@@ -91,7 +93,7 @@ def canonical_step_bytes(L: int, M: int, N: int, wbytes: int) -> float:
     return 3 * (22 * 8 + 26 * wbytes) * F
 
 
-KERNELS = ["conv_rhs", "cn_x", "cn_z", "cn_xz", "cn_y", "divg_fftx", "fftz_fwd", "ytdma",
+KERNELS = ["conv_rhs", "cn_x", "cn_z", "cn_xz", "cn_y", "cn_zy", "divg_fftx", "fftz_fwd", "ytdma",
            "fftz_inv", "fftx_inv", "update", "refine_update"]
 
 

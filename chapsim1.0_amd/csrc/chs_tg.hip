@@ -1344,6 +1344,145 @@ __global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict_
     k_cn_y3_march_b<W, D>(g, by0, by1, by2, rin, qin, qout, KY, own_blk());
 }
 
+// The z and y Crank-Nicolson inverses in one pass (MOMFA2_Z_tg, then MOMFA3_Y_tg with
+// TDMAIJJ_nonCYC, Q + dQ and BC_WALL_Q), so that the z-solved increments never reach HBM.  A
+// block is 64 columns i by KB planes k of one component and one chunk of rows; each thread
+// (i, k) marches its chunk with the y-band window in registers, as k_cn_y3_march does, and the
+// row entering the window is z-solved on the fly from an LDS slab of that row that holds the
+// block's KB planes and Dz planes either side (cyclic in k; the first 2 Dz wavefronts load
+// the side planes).  Rows go in batches of BT: the registers of the next batch are loaded
+// while the current one is solved, and the batch's slabs go to one of two LDS buffers taken
+// in turns, one barrier per batch.  The arithmetic is k_cn_z_march's and k_cn_y3_march's, in
+// their order.  v's row 1 input (u's increment at row 1, RHS_CvLpGpS_tg.f90:45-46) is formed
+// by the v block of the first chunk from u's rows 1..1+D, z-solved the same way, before its
+// own rows.  Slab plane s holds plane k0 + s - DZM; Dz <= DZM and 2 Dz <= KB.
+template <typename W, int D, int KB, int BT, int DZM>
+__global__ __launch_bounds__(64 * KB) void k_cn_zy_march(Geo g, const W *__restrict__ gz, int Dz,
+                                                         const W *__restrict__ by0, const W *__restrict__ by1,
+                                                         const W *__restrict__ by2, const W *__restrict__ rin,
+                                                         const double *__restrict__ qin, double *__restrict__ qout,
+                                                         int KY, const double *__restrict__ dpg, int nxcd)
+{
+    constexpr int SP = KB + 2 * DZM, w = 2 * D + 1, BUF = BT * SP * 64;
+    __shared__ W S[2 * BUF];
+    const int L = g.L, M = g.M, N = g.N;
+    const int nkt = N / KB, nit = L / 64;
+    const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd, own_blk()));
+    const int kt = tile % nkt, it = (tile / nkt) % nit, cz = tile / (nkt * nit);
+    const int c = cz % 3, jc = cz / 3;
+    const int lane = threadIdx.x, kw = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int k0 = kt * KB, k = k0 + kw;
+    const size_t cs = (size_t)N * M * L, hs = (size_t)N * (M + 2) * L;
+    const W *__restrict__ by = c == 0 ? by0 : (c == 1 ? by1 : by2);
+    // the side plane this wavefront loads (first 2 Dz wavefronts) and its slab position
+    const bool side = kw < 2 * Dz;
+    int kh = kw < Dz ? k0 - Dz + kw : k0 + KB + kw - Dz;
+    kh = kh < 0 ? kh + N : (kh >= N ? kh - N : kh);
+    const int sh = kw < Dz ? DZM - Dz + kw : DZM + KB + kw - Dz;
+    const int so = DZM + kw;
+    const W sub0 = dpg ? (W)dpg[0] : W(0);   // u's deferred -DPGRNS (see k_cn_z_march)
+    W cz0[DZM + 1];
+#pragma unroll
+    for (int d = 0; d <= DZM; ++d) cz0[d] = d <= Dz ? gz[d] : W(0);
+    const int J0 = (c == 1 && jc == 0) ? 2 : jc * KY + 1;
+    const int J1 = min(M, jc * KY + KY);
+    // items: P rows of u for v's row-1 input, then the rows entering the window, J0-D .. J1+D
+    const int P = (c == 1 && jc == 0) ? D + 1 : 0;
+    const int eS = J0 - D, NI = P + (J1 + D - eS + 1);
+    // kind 0: zero (outside 1..M), 1: loaded and z-solved, 2: v's row 1 (u's increment)
+    auto item = [&](int t, int &cc, int &row, int &kind) {
+        if (t < P) { cc = 0; row = 1 + t; kind = 1; return; }
+        const int e = eS + (t - P);
+        cc = c; row = e;
+        kind = (e < 1 || e > M) ? 0 : ((c == 1 && e == 1) ? 2 : 1);
+    };
+    auto rrow = [&](int cc, int row, int kk) {
+        return uniform(rin + cc * cs + ((size_t)kk * M + (row - 1)) * L + it * 64);
+    };
+    auto qirow = [&](int J) { return uniform(qin + c * hs + ((size_t)k * (M + 2) + J) * L + it * 64); };
+    auto qorow = [&](int J) { return uniform(qout + c * hs + ((size_t)k * (M + 2) + J) * L + it * 64); };
+    struct Regs { W own[BT], hal[BT]; double qv[BT]; };
+    auto fetch = [&](int t0, Regs &r) {
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            const int t = t0 + b;
+            if (t >= NI) break;
+            int cc, row, kind;
+            item(t, cc, row, kind);
+            if (kind == 1) {
+                r.own[b] = rrow(cc, row, k)[lane];
+                if (side) r.hal[b] = rrow(cc, row, kh)[lane];
+            }
+            if (t >= P) {
+                const int J = eS + (t - P) - D;
+                if (J >= J0) r.qv[b] = qirow(J)[lane];
+            }
+        }
+    };
+    W win[2 * D + 1];
+    W a = W(0);
+    auto batch = [&](int t0, W *Sb, Regs &cur, Regs &nxt) {
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            const int t = t0 + b;
+            if (t >= NI) break;
+            int cc, row, kind;
+            item(t, cc, row, kind);
+            if (kind == 1) {
+                Sb[(b * SP + so) * 64 + lane] = cur.own[b];
+                if (side) Sb[(b * SP + sh) * 64 + lane] = cur.hal[b];
+            }
+        }
+        __syncthreads();
+        fetch(t0 + BT, nxt);
+#pragma unroll
+        for (int b = 0; b < BT; ++b) {
+            const int t = t0 + b;
+            if (t >= NI) break;
+            int cc, row, kind;
+            item(t, cc, row, kind);
+            W zs = W(0);
+            if (kind == 1) {
+                const W *sr = Sb + (b * SP + so) * 64 + lane;
+                W acc = cz0[0] * sr[0];
+#pragma unroll
+                for (int d = 1; d <= DZM; ++d)
+                    if (d <= Dz) acc += cz0[d] * (sr[d * 64] + sr[-d * 64]);
+                zs = acc - (cc == 0 ? sub0 : W(0));
+            } else if (kind == 2) {
+                zs = a;
+            }
+            if (t < P) {
+                a += by0[D + t] * zs;   // u's band row 1, entries on and above the diagonal
+            } else {
+                win[2 * D] = zs;
+                const int J = row - D;
+                if (J >= J0) {
+                    const W *bj = by + (size_t)(J - 1) * w;
+                    W acc = W(0);
+#pragma unroll
+                    for (int d = 0; d < w; ++d) acc += bj[d] * win[d];   // out-of-range rows hold 0 and 0 coefficients
+                    qorow(J)[lane] = cur.qv[b] + (double)acc;
+                }
+#pragma unroll
+                for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
+            }
+        }
+    };
+    Regs ra, rb;
+    fetch(0, ra);
+    for (int t0 = 0; t0 < NI; t0 += 2 * BT) {
+        batch(t0, S, ra, rb);
+        if (t0 + BT < NI) batch(t0 + BT, S + BUF, rb, ra);
+    }
+    double *q0 = qout + c * hs + (size_t)k * (M + 2) * L + it * 64;
+    if (jc == 0) {
+        q0[lane] = 0.0;
+        if (c == 1) q0[L + lane] = 0.0;
+    }
+    if (J1 == M) q0[(size_t)(M + 1) * L + lane] = 0.0;
+}
+
 // ---------------------------------------------------------------------------------------
 // Poisson.  Spectral data is complex, layout [J][kx][kz] (kz fastest), so z-lines are
 // contiguous and the per-mode y-Thomas reads consecutive kz across a wavefront.
@@ -2582,6 +2721,9 @@ struct chs_ctx {
         int upd_by = 4;                     // k_update rows per block (1, 2, 4, 8, 16)
         int upd_nxcd = 1;                   // k_update tiles grouped for this many XCDs (1: launch order;
                                             // 8 measured equal at 256x192x256)
+        int cnzy = 1;                       // z and y solves fused (k_cn_zy_march; 0 = two kernels)
+        int cnzy_ky = 48;                   // k_cn_zy_march rows per chunk (0 = M)
+        int cnzy_nxcd = 8;                  // k_cn_zy_march tiles grouped for this many XCDs
     } tune;
     int tune_gen = 0;   // bumped by every chs_set_tuning / chs_set_refinement (graphs are re-made)
     int cn_gen = 0;     // bumped by every rebuild of a Crank-Nicolson table (its memory may move)
@@ -3144,7 +3286,34 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
         hipLaunchKernelGGL(k_cn_x<W>, dim3((M * N + NLX - 1) / NLX, 3), dim3(256), sizeof(W) * NLX * L, c->stream,
                            g, (const W *)t.gx, t.Dx, t.fullx, rhs, c->scal, NLX);
     }
-    if (xz) {
+    const int Dym = t.Dy[0];
+    const bool ysame = t.Dy[1] == Dym && t.Dy[2] == Dym;
+    // z and y fused (k_cn_zy_march): whole wavefronts of x, blocks of 16 planes, z-halo within
+    // the block's side planes
+    constexpr int ZYKB = 16, ZYBT = 4;
+    const int KZY = c->tune.cnzy_ky > 0 ? c->tune.cnzy_ky : M;
+    const bool zy = c->tune.cnzy && !xz && zmarch && L % 64 == 0 && N % ZYKB == 0 && 2 * t.Dz <= ZYKB &&
+                    t.Dz <= 8 && ysame && Dym <= 8 && KZY > Dym;
+    if (zy) {
+        EvScope ev(c, "cn_zy");
+        const int nb = (L / 64) * (N / ZYKB) * 3 * ((M + KZY - 1) / KZY);
+        const W *b0 = (const W *)t.by[0], *b1 = (const W *)t.by[1], *b2 = (const W *)t.by[2];
+        const double *dpg = fx ? c->scal : nullptr;
+        const int nx = c->tune.cnzy_nxcd;
+        if (t.Dz <= 4) {
+            switch (Dym) {
+#define CHS_ZY4(d) case d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 4>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break;
+                CHS_ZY4(0) CHS_ZY4(1) CHS_ZY4(2) CHS_ZY4(3) CHS_ZY4(4) CHS_ZY4(5) CHS_ZY4(6) CHS_ZY4(7) CHS_ZY4(8)
+#undef CHS_ZY4
+            }
+        } else {
+            switch (Dym) {
+#define CHS_ZY8(d) case d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 8>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break;
+                CHS_ZY8(0) CHS_ZY8(1) CHS_ZY8(2) CHS_ZY8(3) CHS_ZY8(4) CHS_ZY8(5) CHS_ZY8(6) CHS_ZY8(7) CHS_ZY8(8)
+#undef CHS_ZY8
+            }
+        }
+    } else if (xz) {
         // rhs2 already holds the x- and z-solved increments
     } else if (zmarch) {
         EvScope ev(c, "cn_z");
@@ -3156,9 +3325,10 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
             hipLaunchKernelGGL(k_cn_z<W>, grd, blk, 0, c->stream, g, (const W *)t.gz, t.Dz, t.fullz, rhs, rhs2, comp);
         }
     }
-    const int Dym = t.Dy[0];
     const int KY3 = c->tune.cny3_ky > 0 ? c->tune.cny3_ky : M;
-    if (c->tune.cny3_ky >= 0 && Dym <= 8 && KY3 > Dym && t.Dy[1] == Dym && t.Dy[2] == Dym) {
+    if (zy) {
+        // the y-solves ran inside k_cn_zy_march
+    } else if (c->tune.cny3_ky >= 0 && Dym <= 8 && KY3 > Dym && ysame) {
         EvScope ev(c, "cn_y");
         const dim3 gy((L + 63) / 64, (N + 3) / 4, 3 * ((M + KY3 - 1) / KY3));
         const W *b0 = (const W *)t.by[0], *b1 = (const W *)t.by[1], *b2 = (const W *)t.by[2];
@@ -3868,6 +4038,15 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "divg_nxcd") {
         if (value < 1 || value > 64) return bad("must be 1..64");
         c->tune.divg_nxcd = value;
+    } else if (k == "cnzy") {
+        if (value < 0 || value > 1) return bad("must be 0 or 1");
+        c->tune.cnzy = value;
+    } else if (k == "cnzy_ky") {
+        if (value < 0 || value > M) return bad("must be 0..M");
+        c->tune.cnzy_ky = value;
+    } else if (k == "cnzy_nxcd") {
+        if (value < 1 || value > 64) return bad("must be 1..64");
+        c->tune.cnzy_nxcd = value;
     } else if (k == "cny3_ky") {
         if (value < -1 || value > M) return bad("must be -1..M");
         c->tune.cny3_ky = value;

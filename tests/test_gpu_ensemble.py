@@ -8,7 +8,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-BATCHED_SHAPES = dict(conv_fx=0, fft_reg=0)
+BATCHED_SHAPES = dict(conv_fx=0, fft_reg=0, cnzy=0)
 
 
 @pytest.mark.parametrize("batched", [True, False])

@@ -225,6 +225,63 @@ def test_fused_x_solve_matches_oracle(pkg, L, M, N, fx):
     ch.close()
 
 
+@pytest.mark.parametrize("L,M,N,REN,dt,ky", [(64, 48, 32, 4000.0, 8.0e-4, 0), (64, 48, 32, 4000.0, 8.0e-4, 16),
+                                              (128, 40, 48, 6875.0, 5.0e-4, 7), (64, 24, 16, 2800.0, 2.0e-3, 20),
+                                              (192, 36, 64, 2800.0, 1.0e-3, 11)])
+def test_fused_zy_solve_matches_two_kernels_and_oracle(pkg, L, M, N, REN, dt, ky):
+    """k_cn_zy_march (the z and y Crank-Nicolson inverses in one pass, chunks of ky rows, 0 =
+    whole lines, ragged last chunks) gives the separate z and y kernels' result to rounding and
+    the oracle's to 1e-10, in FP64 and in the mixed form (against the mixed two-kernel path)."""
+    STR2 = 2.0
+    print(f"fused zy {L}x{M}x{N} ky {ky}", flush=True)
+    mesh, ch, o = make(pkg, L, M, N, STR2, REN)
+    two = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=STR2), pkg.PLAIN_FP64)
+    q0, p0 = oracle.initial_field(mesh, 3)
+    two.upload(q0, p0)
+    ch.set_tuning(cnzy=1, cnzy_ky=ky)
+    two.set_tuning(cnzy=0)
+    cv = 1.0 / REN
+    for c in (ch, two):
+        c.SOLVERRK3_MOM_tg(0, dt, cv)
+        for _ in range(2):
+            c.rk3_step(dt, cv)
+    o.substep(0, dt)
+    for _ in range(2):
+        for ns in (1, 2, 3):
+            o.substep(ns, dt)
+    gq, gp = ch.download()
+    tq, tp = two.download()
+    dq, dp = max(rel(gq[c], tq[c]) for c in range(3)), rel(gp, tp)
+    e, ep = max(rel(gq[c], o.q[c]) for c in range(3)), rel(gp, o.pr)
+    print(f"  fused vs two kernels velocity {dq:.2e} pressure {dp:.2e}; vs oracle velocity {e:.2e} "
+          f"pressure {ep:.2e}")
+    # rounding only (contractions differ between the kernels); the pressure carries the
+    # Poisson solve's amplification of it
+    assert dq < 1e-13 and dp < 1e-10
+    assert e < 1e-10 and ep < 1e-10
+    assert ch.max_div() < 1e-10
+    ch.close()
+    two.close()
+    mixed = [pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=STR2), pkg.MIXED) for _ in range(2)]
+    for m, on in zip(mixed, (1, 0)):
+        m.upload(q0, p0)
+        m.set_tuning(cnzy=on, cnzy_ky=ky)
+        m.SOLVERRK3_MOM_tg(0, dt, cv)
+        for _ in range(2):
+            m.rk3_step(dt, cv)
+    (aq, ap), (bq, bp) = mixed[0].download(), mixed[1].download()
+    # the mixed form's error against FP64, fused and with two kernels: FP32 rounding of the
+    # same size
+    ea, eb = max(rel(aq[c], gq[c]) for c in range(3)), max(rel(bq[c], gq[c]) for c in range(3))
+    pa, pb = rel(ap, gp), rel(bp, gp)
+    print(f"  mixed vs FP64: fused velocity {ea:.2e} pressure {pa:.2e}; two kernels velocity {eb:.2e} "
+          f"pressure {pb:.2e}")
+    assert ea < 1e-5 and ea < 3 * eb + 1e-7
+    assert pa < 3 * pb + 1e-6
+    for m in mixed:
+        m.close()
+
+
 def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
     for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(cn_xz=2), dict(cnxz_by=3), dict(conv_kc=41), dict(fftx_tk=3),
