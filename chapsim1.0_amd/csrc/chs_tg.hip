@@ -1357,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict_
 // by the v block of the first chunk from u's rows 1..1+D, z-solved the same way, before its
 // own rows.  Slab plane s holds plane k0 + s - DZM; Dz <= DZM and 2 Dz <= KB.
 template <typename W, int D, int KB, int BT, int DZM>
-__global__ __launch_bounds__(64 * KB) void k_cn_zy_march(Geo g, const W *__restrict__ gz, int Dz,
+__global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k_cn_zy_march(Geo g, const W *__restrict__ gz, int Dz,
                                                          const W *__restrict__ by0, const W *__restrict__ by1,
                                                          const W *__restrict__ by2, const W *__restrict__ rin,
                                                          const double *__restrict__ qin, double *__restrict__ qout,
@@ -3289,8 +3289,9 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
     const int Dym = t.Dy[0];
     const bool ysame = t.Dy[1] == Dym && t.Dy[2] == Dym;
     // z and y fused (k_cn_zy_march): whole wavefronts of x, blocks of 16 planes, z-halo within
-    // the block's side planes
-    constexpr int ZYKB = 16, ZYBT = 4;
+    // the block's side planes; batches of 2 FP64 / 4 FP32 rows, so that two blocks fit a CU
+    // (48 KB of LDS each at Dz <= 4, and at most 64 VGPRs)
+    constexpr int ZYKB = 16, ZYBT = sizeof(W) == 8 ? 2 : 4;
     const int KZY = c->tune.cnzy_ky > 0 ? c->tune.cnzy_ky : M;
     const bool zy = c->tune.cnzy && !xz && zmarch && L % 64 == 0 && N % ZYKB == 0 && 2 * t.Dz <= ZYKB &&
                     t.Dz <= 8 && ysame && Dym <= 8 && KZY > Dym;
