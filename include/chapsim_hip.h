@@ -154,7 +154,10 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * load batch (8, 16 default), "ytdma_c" rows per chunk of the chunked y-Thomas (-1 = chosen by
  * the mesh, 0 = unchunked), "conv_dma" rows per block of the direct-to-LDS convection kernel
  * (4 default, 2; 0 = off), "cny3_ky" rows per chunk of the fused three-component y-solve (48;
- * 0 = M, -1 = one launch per component), "divg_nxcd" and "upd_nxcd" the XCDs the divergence
+ * 0 = M, -1 = one launch per component), "cnzy" 1 (default) to run the z and y Crank-Nicolson
+ * solves in one pass where L is a multiple of 64, N of 16 and the z band's half-width at most 8
+ * (0: the two kernels), "cnzy_ky" its rows per chunk (48; 0 = M), "cnzy_nxcd" the XCDs its tiles
+ * are grouped for (8), "divg_nxcd" and "upd_nxcd" the XCDs the divergence
  * transform's and the correction's tiles are grouped for (8 and 1; 1 = dispatch order), "upd_kz"
  * z-planes each correction thread marches (8) and "upd_by" its rows per block (4).  Results do not
  * depend on the shapes except through the summation order of the mass-flux partials
