@@ -1355,7 +1355,9 @@ __global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict_
 // in turns, one barrier per batch.  The arithmetic is k_cn_z_march's and k_cn_y3_march's, in
 // their order.  v's row 1 input (u's increment at row 1, RHS_CvLpGpS_tg.f90:45-46) is formed
 // by the v block of the first chunk from u's rows 1..1+D, z-solved the same way, before its
-// own rows.  Slab plane s holds plane k0 + s - DZM; Dz <= DZM and 2 Dz <= KB.
+// own rows.  Slab plane s holds plane (k0 + s - DZM) mod N; Dz <= DZM and 2 Dz <= KB.  Lanes past
+// L and planes past N (ragged last tiles) fill their slab slots from wrapped planes and store
+// nothing.
 template <typename W, int D, int KB, int BT, int DZM>
 __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k_cn_zy_march(Geo g, const W *__restrict__ gz, int Dz,
                                                          const W *__restrict__ by0, const W *__restrict__ by1,
@@ -1366,18 +1368,20 @@ __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k
     constexpr int SP = KB + 2 * DZM, w = 2 * D + 1, BUF = BT * SP * 64;
     __shared__ W S[2 * BUF];
     const int L = g.L, M = g.M, N = g.N;
-    const int nkt = N / KB, nit = L / 64;
+    const int nkt = (N + KB - 1) / KB, nit = (L + 63) / 64;
     const int tile = __builtin_amdgcn_readfirstlane(xcd_tile(nxcd, own_blk()));
     const int kt = tile % nkt, it = (tile / nkt) % nit, cz = tile / (nkt * nit);
     const int c = cz % 3, jc = cz / 3;
     const int lane = threadIdx.x, kw = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int k0 = kt * KB, k = k0 + kw;
+    const bool iv = it * 64 + lane < L, kv = k < N;
+    const int kown = kv ? k : k % N;
     const size_t cs = (size_t)N * M * L, hs = (size_t)N * (M + 2) * L;
     const W *__restrict__ by = c == 0 ? by0 : (c == 1 ? by1 : by2);
     // the side plane this wavefront loads (first 2 Dz wavefronts) and its slab position
     const bool side = kw < 2 * Dz;
     int kh = kw < Dz ? k0 - Dz + kw : k0 + KB + kw - Dz;
-    kh = kh < 0 ? kh + N : (kh >= N ? kh - N : kh);
+    kh = ((kh % N) + N) % N;
     const int sh = kw < Dz ? DZM - Dz + kw : DZM + KB + kw - Dz;
     const int so = DZM + kw;
     const W sub0 = dpg ? (W)dpg[0] : W(0);   // u's deferred -DPGRNS (see k_cn_z_march)
@@ -1409,11 +1413,11 @@ __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k
             if (t >= NI) break;
             int cc, row, kind;
             item(t, cc, row, kind);
-            if (kind == 1) {
-                r.own[b] = rrow(cc, row, k)[lane];
+            if (kind == 1 && iv) {
+                r.own[b] = rrow(cc, row, kown)[lane];
                 if (side) r.hal[b] = rrow(cc, row, kh)[lane];
             }
-            if (t >= P) {
+            if (t >= P && iv && kv) {
                 const int J = eS + (t - P) - D;
                 if (J >= J0) r.qv[b] = qirow(J)[lane];
             }
@@ -1462,7 +1466,7 @@ __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k
                     W acc = W(0);
 #pragma unroll
                     for (int d = 0; d < w; ++d) acc += bj[d] * win[d];   // out-of-range rows hold 0 and 0 coefficients
-                    qorow(J)[lane] = cur.qv[b] + (double)acc;
+                    if (iv && kv) qorow(J)[lane] = cur.qv[b] + (double)acc;
                 }
 #pragma unroll
                 for (int d = 0; d < 2 * D; ++d) win[d] = win[d + 1];
@@ -1475,6 +1479,7 @@ __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k
         batch(t0, S, ra, rb);
         if (t0 + BT < NI) batch(t0 + BT, S + BUF, rb, ra);
     }
+    if (!iv || !kv) return;
     double *q0 = qout + c * hs + (size_t)k * (M + 2) * L + it * 64;
     if (jc == 0) {
         q0[lane] = 0.0;
@@ -2721,7 +2726,8 @@ struct chs_ctx {
         int upd_by = 4;                     // k_update rows per block (1, 2, 4, 8, 16)
         int upd_nxcd = 1;                   // k_update tiles grouped for this many XCDs (1: launch order;
                                             // 8 measured equal at 256x192x256)
-        int cnzy = 1;                       // z and y solves fused (k_cn_zy_march; 0 = two kernels)
+        int cnzy = 1;                       // z and y solves fused (k_cn_zy_march): 1 where L % 64 == 0
+                                            // and N % 16 == 0, 2 on every mesh, 0 = two kernels
         int cnzy_ky = 48;                   // k_cn_zy_march rows per chunk (0 = M)
         int cnzy_nxcd = 8;                  // k_cn_zy_march tiles grouped for this many XCDs
     } tune;
@@ -3288,16 +3294,19 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
     }
     const int Dym = t.Dy[0];
     const bool ysame = t.Dy[1] == Dym && t.Dy[2] == Dym;
-    // z and y fused (k_cn_zy_march): whole wavefronts of x, blocks of 16 planes, z-halo within
+    // z and y fused (k_cn_zy_march): wavefronts of 64 x, blocks of 16 planes, z-halo within
     // the block's side planes; batches of 2 FP64 / 4 FP32 rows, so that two blocks fit a CU
     // (48 KB of LDS each at Dz <= 4, and at most 64 VGPRs)
     constexpr int ZYKB = 16, ZYBT = sizeof(W) == 8 ? 2 : 4;
     const int KZY = c->tune.cnzy_ky > 0 ? c->tune.cnzy_ky : M;
-    const bool zy = c->tune.cnzy && !xz && zmarch && L % 64 == 0 && N % ZYKB == 0 && 2 * t.Dz <= ZYKB &&
+    // (by default only on whole tiles: on the ensemble's uneven meshes the ragged tiles made the
+    // pass slower than the two kernels)
+    const bool full = L % 64 == 0 && N % ZYKB == 0;
+    const bool zy = (c->tune.cnzy == 2 || (c->tune.cnzy == 1 && full)) && !xz && zmarch && 2 * t.Dz <= ZYKB &&
                     t.Dz <= 8 && ysame && Dym <= 8 && KZY > Dym;
     if (zy) {
         EvScope ev(c, "cn_zy");
-        const int nb = (L / 64) * (N / ZYKB) * 3 * ((M + KZY - 1) / KZY);
+        const int nb = ((L + 63) / 64) * ((N + ZYKB - 1) / ZYKB) * 3 * ((M + KZY - 1) / KZY);
         const W *b0 = (const W *)t.by[0], *b1 = (const W *)t.by[1], *b2 = (const W *)t.by[2];
         const double *dpg = fx ? c->scal : nullptr;
         const int nx = c->tune.cnzy_nxcd;
@@ -4040,7 +4049,7 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
         if (value < 1 || value > 64) return bad("must be 1..64");
         c->tune.divg_nxcd = value;
     } else if (k == "cnzy") {
-        if (value < 0 || value > 1) return bad("must be 0 or 1");
+        if (value < 0 || value > 2) return bad("must be 0, 1 or 2");
         c->tune.cnzy = value;
     } else if (k == "cnzy_ky") {
         if (value < 0 || value > M) return bad("must be 0..M");

@@ -51,7 +51,7 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "upd_kz": [k for k in (4, 8, 16) if k <= N],
         "upd_by": [1, 2, 4, 8, 16],
         "upd_nxcd": [1, 8],
-        "cnzy": [0, 1],
+        "cnzy": [0, 1, 2],
         "cnzy_ky": [v for v in (0, 24, 48, 96) if v <= M],
         "cnzy_nxcd": [1, 8],
     }

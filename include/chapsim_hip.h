@@ -155,8 +155,9 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * the mesh, 0 = unchunked), "conv_dma" rows per block of the direct-to-LDS convection kernel
  * (4 default, 2; 0 = off), "cny3_ky" rows per chunk of the fused three-component y-solve (48;
  * 0 = M, -1 = one launch per component), "cnzy" 1 (default) to run the z and y Crank-Nicolson
- * solves in one pass where L is a multiple of 64, N of 16 and the z band's half-width at most 8
- * (0: the two kernels), "cnzy_ky" its rows per chunk (48; 0 = M), "cnzy_nxcd" the XCDs its tiles
+ * solves in one pass where L is a multiple of 64 and N of 16, 2 on every mesh (ragged tiles), 0
+ * for the two kernels (the pass needs z and y band half-widths of at most 8), "cnzy_ky" its rows
+ * per chunk (48; 0 = M), "cnzy_nxcd" the XCDs its tiles
  * are grouped for (8), "divg_nxcd" and "upd_nxcd" the XCDs the divergence
  * transform's and the correction's tiles are grouped for (8 and 1; 1 = dispatch order), "upd_kz"
  * z-planes each correction thread marches (8) and "upd_by" its rows per block (4).  Results do not

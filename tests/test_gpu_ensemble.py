@@ -32,6 +32,8 @@ def test_ensemble_matches_single_channels(pkg, batched):
     ens = pkg.Ensemble(cos, pkg.PLAIN_FP64)
     ens.set_batched(batched)
     for ch, (q, p), d, v in zip(ens.channels, fields, dt, cv):
+        if batched:
+            ch.set_tuning(**BATCHED_SHAPES)   # the NS = 0 substep runs the channel's own kernels
         ch.upload(q, p)
         ch.SOLVERRK3_MOM_tg(0, d, v)
     for _ in range(2):
@@ -87,6 +89,8 @@ def test_ensemble_more_channels_than_lanes(pkg, batched):
     ens = pkg.Ensemble(cos, pkg.PLAIN_FP64)
     ens.set_batched(batched)
     for ch, (q, p), d, v in zip(ens.channels, fields, dt, cv):
+        if batched:
+            ch.set_tuning(**BATCHED_SHAPES)   # the NS = 0 substep runs the channel's own kernels
         ch.upload(q, p)
         ch.SOLVERRK3_MOM_tg(0, d, v)
     for step in range(3):

@@ -227,10 +227,12 @@ def test_fused_x_solve_matches_oracle(pkg, L, M, N, fx):
 
 @pytest.mark.parametrize("L,M,N,REN,dt,ky", [(64, 48, 32, 4000.0, 8.0e-4, 0), (64, 48, 32, 4000.0, 8.0e-4, 16),
                                               (128, 40, 48, 6875.0, 5.0e-4, 7), (64, 24, 16, 2800.0, 2.0e-3, 20),
-                                              (192, 36, 64, 2800.0, 1.0e-3, 11)])
+                                              (192, 36, 64, 2800.0, 1.0e-3, 11), (96, 36, 40, 4000.0, 8.0e-4, 16),
+                                              (20, 18, 30, 6875.0, 5.0e-4, 0)])
 def test_fused_zy_solve_matches_two_kernels_and_oracle(pkg, L, M, N, REN, dt, ky):
     """k_cn_zy_march (the z and y Crank-Nicolson inverses in one pass, chunks of ky rows, 0 =
-    whole lines, ragged last chunks) gives the separate z and y kernels' result to rounding and
+    whole lines, ragged last chunks; ragged x and z tiles where L is not a multiple of 64 or N
+    of 16) gives the separate z and y kernels' result to rounding and
     the oracle's to 1e-10, in FP64 and in the mixed form (against the mixed two-kernel path)."""
     STR2 = 2.0
     print(f"fused zy {L}x{M}x{N} ky {ky}", flush=True)
@@ -238,7 +240,7 @@ def test_fused_zy_solve_matches_two_kernels_and_oracle(pkg, L, M, N, REN, dt, ky
     two = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=STR2), pkg.PLAIN_FP64)
     q0, p0 = oracle.initial_field(mesh, 3)
     two.upload(q0, p0)
-    ch.set_tuning(cnzy=1, cnzy_ky=ky)
+    ch.set_tuning(cnzy=2, cnzy_ky=ky)
     two.set_tuning(cnzy=0)
     cv = 1.0 / REN
     for c in (ch, two):
@@ -265,7 +267,7 @@ def test_fused_zy_solve_matches_two_kernels_and_oracle(pkg, L, M, N, REN, dt, ky
     mixed = [pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=STR2), pkg.MIXED) for _ in range(2)]
     for m, on in zip(mixed, (1, 0)):
         m.upload(q0, p0)
-        m.set_tuning(cnzy=on, cnzy_ky=ky)
+        m.set_tuning(cnzy=2 * on, cnzy_ky=ky)
         m.SOLVERRK3_MOM_tg(0, dt, cv)
         for _ in range(2):
             m.rk3_step(dt, cv)
@@ -286,6 +288,7 @@ def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
     for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(cn_xz=2), dict(cnxz_by=3), dict(conv_kc=41), dict(fftx_tk=3),
                 dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6), dict(conv_fx=3),
+                dict(cnzy=3), dict(cnzy_ky=37), dict(cnzy_nxcd=0),
                 dict(ytdma_c=12), dict(ytdma_c=-2), dict(no_such_key=1)):
         with pytest.raises(pkg.ChsError):
             ch.set_tuning(**bad)
