@@ -2162,6 +2162,132 @@ __global__ __launch_bounds__(1024) void k_update(Geo g, Sub s, const double *__r
     k_update_b<CFL>(g, s, phi, qs, q, pr, KZ, part, b);
 }
 
+// k_update with two adjacent points per lane (16-byte loads and stores: half the vector-memory
+// instructions, which bound k_update): lane l of a wavefront holds x = x0 + 2l and x0 + 2l + 1
+// of a 128-point stretch of row J.  The x-neighbours across lanes come by cross-lane moves; the
+// stretch's outer ones (phi at x0 - 1, u* and phi at x0 + 128, cyclic) are loaded by the edge
+// lanes.  Same arithmetic as k_update_b point by point (contraction off): bit-identical.  L is a
+// multiple of 128.
+template <bool CFL>
+__global__ __launch_bounds__(1024) void k_update2(Geo g, Sub s, const double *__restrict__ phi,
+                                                  const double *__restrict__ qs, double *__restrict__ q,
+                                                  double *__restrict__ pr, int KZ, double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    const Blk blk = own_blk();
+    const int lane = threadIdx.x;
+    const int x0 = blk.x * 128;
+    const int J = __builtin_amdgcn_readfirstlane(blk.y * blockDim.y + threadIdx.y + 1);
+    const int k0 = blk.z * KZ;
+    double cmax = 0.0;
+    if (J <= g.M) {
+        const int L = g.L, M = g.M, N = g.N;
+        const size_t hs = (size_t)N * (M + 2) * L;
+        const unsigned xm = x0 == 0 ? L - 1 : x0 - 1, xp = x0 + 128 == L ? 0 : x0 + 128;
+        const int k1 = min(N, k0 + KZ);
+        const double sdt = s.dv[0], scv = s.dv[1];
+        const double ta = sdt * s.talp;
+        const double cx = ta * g.dxi, cz = ta * g.dzi, cy = ta * g.dyci[J];
+        const double cyp = J < M ? ta * g.dyci[J + 1] : 0.0;
+        const double be = 0.5 * s.talp * sdt * scv;
+        const double coe0 = 1.0 / (sdt * s.talp);
+        const double dyf = g.dyfi[J - 1];
+        auto R = [&](int jj, int kk) { return ((size_t)kk * (M + 2) + jj) * L; };
+        auto row = [&](const double *f, int jj, int kk) { return uniform(f + R(jj, kk)); };
+        typedef double dv2 __attribute__((ext_vector_type(2)));
+        auto ld2 = [&](const double *f, int jj, int kk) {
+            return reinterpret_cast<const __attribute__((address_space(1))) dv2 *>(row(f, jj, kk) + x0)[lane];
+        };
+        auto st2 = [&](double *f, int jj, int kk, double a, double b) {
+            dv2 v;
+            v.x = a;
+            v.y = b;
+            reinterpret_cast<__attribute__((address_space(1))) dv2 *>(uniform(f + R(jj, kk)) + x0)[lane] = v;
+        };
+        const auto pm0 = ld2(phi, J, k0 == 0 ? N - 1 : k0 - 1);
+        double phm[2] = {pm0.x, pm0.y};
+        const auto pk0 = ld2(phi, J, k0), wk0 = ld2(qs + 2 * hs, J, k0);
+        double phk[2] = {pk0.x, pk0.y}, wsk[2] = {wk0.x, wk0.y};
+        for (int k = k0; k < k1; ++k) {
+            const int kp = (k == N - 1) ? 0 : k + 1;
+            const auto pp2 = ld2(phi, J, kp), wp2 = ld2(qs + 2 * hs, J, kp);
+            const auto u2 = ld2(qs, J, k), v2 = ld2(qs + hs, J, k), vp2 = ld2(qs + hs, J + 1, k);
+            const auto pj2 = ld2(phi, J - 1, k), pr2 = ld2(pr, J, k);
+            double pjp[2] = {0.0, 0.0};
+            if constexpr (CFL) {
+                if (J < M) {
+                    const auto t = ld2(phi, J + 1, k);
+                    pjp[0] = t.x; pjp[1] = t.y;
+                }
+            }
+            // x-neighbours: u*(x+1) and phi(x+1) of the pair's second point from the next lane
+            // (the stretch's last lane loads x0 + 128), phi(x-1) of the first from the previous
+            double unx = __shfl_down(u2.x, 1), pnx = __shfl_down(phk[0], 1), ppv = __shfl_up(phk[1], 1);
+            if (lane == 63) {
+                unx = row(qs, J, k)[xp];
+                if constexpr (CFL) pnx = row(phi, J, k)[xp];
+            }
+            if (lane == 0) ppv = row(phi, J, k)[xm];
+            const double us[2] = {u2.x, u2.y}, usp[2] = {u2.y, unx};
+            const double vs[2] = {v2.x, v2.y}, vsp[2] = {vp2.x, vp2.y};
+            const double php[2] = {pp2.x, pp2.y}, wsp[2] = {wp2.x, wp2.y};
+            const double pim[2] = {ppv, phk[0]}, pjm[2] = {pj2.x, pj2.y}, pip[2] = {phk[1], pnx};
+            const double pold[2] = {pr2.x, pr2.y};
+            double un[2], vn[2], wn[2], pn[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double d = divg_cell(usp[h], us[h], vsp[h], vs[h], wsp[h], wsk[h], g.dxi, dyf, g.dzi) * coe0;
+                un[h] = us[h] - (phk[h] - pim[h]) * cx;
+                wn[h] = wsk[h] - (phk[h] - phm[h]) * cz;
+                vn[h] = J >= 2 ? vs[h] - (phk[h] - pjm[h]) * cy : 0.0;   // row 1 is the bottom wall
+                pn[h] = pold[h] + phk[h] - be * d;
+                if constexpr (CFL) {
+                    const double unp = usp[h] - (pip[h] - phk[h]) * cx;
+                    const double vnp = J < M ? vsp[h] - (pjp[h] - phk[h]) * cyp : 0.0;
+                    const double wnp = wsp[h] - (php[h] - phk[h]) * cz;
+                    const double a = fabs(un[h] + unp) * g.dxi;
+                    const double b = fabs(vn[h] + vnp) * dyf;
+                    const double c = fabs(wn[h] + wnp) * g.dzi;
+                    cmax = nanmax(cmax, (a + b + c) * 0.5);
+                }
+            }
+            st2(q, J, k, un[0], un[1]);
+            st2(q + hs, J, k, vn[0], vn[1]);
+            st2(q + 2 * hs, J, k, wn[0], wn[1]);
+            st2(pr, J, k, pn[0], pn[1]);
+            if (J == 1) {
+                st2(pr, 0, k, pn[0], pn[1]);
+                st2(q, 0, k, 0.0, 0.0);
+                st2(q + hs, 0, k, 0.0, 0.0);
+                st2(q + 2 * hs, 0, k, 0.0, 0.0);
+            }
+            if (J == M) {
+                st2(pr, M + 1, k, pn[0], pn[1]);
+                st2(q, M + 1, k, 0.0, 0.0);
+                st2(q + hs, M + 1, k, 0.0, 0.0);
+                st2(q + 2 * hs, M + 1, k, 0.0, 0.0);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                phm[h] = phk[h];
+                phk[h] = php[h];
+                wsk[h] = wsp[h];
+            }
+        }
+    }
+    if constexpr (CFL) {
+        __shared__ double red[1024];
+        const int t = threadIdx.y * 64 + threadIdx.x;   // blockDim = (64, 2^n)
+        red[t] = cmax;
+        __syncthreads();
+        for (int o = 32 * blockDim.y; o > 0; o >>= 1) {
+            if (t < o) red[t] = nanmax(red[t], red[t + o]);
+            __syncthreads();
+        }
+        if (t == 0) part[((size_t)blk.z * blk.gy + blk.y) * blk.gx + blk.x] = red[0];
+    }
+}
+
 // A refinement sweep's correction, in place: the velocity as k_update, p += phi (PRCALC_tg's
 // viscous term used the first solve's RHS only), ghost rows.
 __global__ __launch_bounds__(256) void k_update_refine(Geo g, Sub s, const double *__restrict__ phi,
@@ -2726,6 +2852,7 @@ struct chs_ctx {
         int upd_by = 4;                     // k_update rows per block (1, 2, 4, 8, 16)
         int upd_nxcd = 1;                   // k_update tiles grouped for this many XCDs (1: launch order;
                                             // 8 measured equal at 256x192x256)
+        int upd_w2 = 1;                     // k_update2 (two points per lane) where L % 128 == 0 (0 = k_update)
         int cnzy = 1;                       // z and y solves fused (k_cn_zy_march): 1 where L % 64 == 0
                                             // and N % 16 == 0, 2 on every mesh, 0 = two kernels
         int cnzy_ky = 48;                   // k_cn_zy_march rows per chunk (0 = M)
@@ -3364,8 +3491,17 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
     // another solve and correction; the CFL then comes from the final field
     const int nref = ns == 0 ? c->nref_projection : c->nref_step;
     const int KU = c->tune.upd_kz, UY = c->tune.upd_by;
-    const dim3 gu((L + 63) / 64, (M + UY - 1) / UY, (N + KU - 1) / KU), blku(64, UY);
-    {
+    const bool u2 = c->tune.upd_w2 && L % 128 == 0;   // two points per lane
+    const dim3 gu(u2 ? L / 128 : (L + 63) / 64, (M + UY - 1) / UY, (N + KU - 1) / KU), blku(64, UY);
+    if (u2) {
+        EvScope ev(c, "update");
+        if (cfl && nref == 0)
+            hipLaunchKernelGGL(k_update2<true>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
+                               c->partial);
+        else
+            hipLaunchKernelGGL(k_update2<false>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
+                               c->partial);
+    } else {
         EvScope ev(c, "update");
         if (cfl && nref == 0)
             hipLaunchKernelGGL(k_update<true>, gu, blku, 0, c->stream, g, s, c->phi, c->qs, c->q, c->pr, KU,
@@ -4048,6 +4184,9 @@ int chs_set_tuning(chs_ctx *c, const char *key, int value)
     } else if (k == "divg_nxcd") {
         if (value < 1 || value > 64) return bad("must be 1..64");
         c->tune.divg_nxcd = value;
+    } else if (k == "upd_w2") {
+        if (value < 0 || value > 1) return bad("must be 0 or 1");
+        c->tune.upd_w2 = value;
     } else if (k == "cnzy") {
         if (value < 0 || value > 2) return bad("must be 0, 1 or 2");
         c->tune.cnzy = value;

@@ -20,7 +20,7 @@ KERNELS = {
     "ytdma_bs": ("ytdma",), "ytdma_rb": ("ytdma",), "ytdma_c": ("ytdma",), "conv_lds": ("conv_rhs", "cn_x"),
     "conv_dma": ("conv_rhs", "cn_x"), "cny3_ky": ("cn_y",), "divg_nxcd": ("divg_fftx",),
     "upd_kz": ("update",), "upd_by": ("update",), "upd_nxcd": ("update",),
-    "cnzy": ("cn_z", "cn_y", "cn_zy"), "cnzy_ky": ("cn_zy",), "cnzy_nxcd": ("cn_zy",),
+    "upd_w2": ("update",), "cnzy": ("cn_z", "cn_y", "cn_zy"), "cnzy_ky": ("cn_zy",), "cnzy_nxcd": ("cn_zy",),
 }
 
 
@@ -51,6 +51,7 @@ def candidates(L: int, M: int, N: int) -> dict[str, list[int]]:
         "upd_kz": [k for k in (4, 8, 16) if k <= N],
         "upd_by": [1, 2, 4, 8, 16],
         "upd_nxcd": [1, 8],
+        "upd_w2": [0, 1],
         "cnzy": [0, 1, 2],
         "cnzy_ky": [v for v in (0, 24, 48, 96) if v <= M],
         "cnzy_nxcd": [1, 8],

@@ -284,11 +284,34 @@ def test_fused_zy_solve_matches_two_kernels_and_oracle(pkg, L, M, N, REN, dt, ky
         m.close()
 
 
+@pytest.mark.parametrize("L,M,N,mode", [(128, 40, 48, "plain"), (256, 24, 32, "plain"), (128, 20, 24, "mixed")])
+def test_update_two_points_per_lane_bit_identical(pkg, L, M, N, mode):
+    """k_update2 (two points per lane, neighbours across lanes and the stretch's edges loaded)
+    gives k_update's fields and CFL bit for bit, with and without the CFL reduction."""
+    print(f"update2 {L}x{M}x{N} {mode}", flush=True)
+    mesh = setup.Mesh(L, M, N, STR2=2.0)
+    q, pr = oracle.initial_field(mesh, 5)
+    m = pkg.PLAIN_FP64 if mode == "plain" else pkg.MIXED
+    out = []
+    for w2 in (1, 0):
+        ch = pkg.ChannelTG(pkg.channel_setup(L, M, N, STR2=2.0), m)
+        ch.set_tuning(upd_w2=w2)
+        ch.upload(q, pr)
+        ch.SOLVERRK3_MOM_tg(0, 5e-4, 1 / 4000.0)
+        for _ in range(2):
+            ch.rk3_step(5e-4, 1 / 4000.0)
+        out.append((*ch.download(), ch.step_state()["cflmm"]))
+        ch.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] and np.isfinite(out[0][2])
+
+
 def test_tuning_rejects_bad_shapes(pkg):
     ch = pkg.ChannelTG(pkg.channel_setup(48, 36, 40, STR2=2.0), pkg.PLAIN_FP64)
     for bad in (dict(conv_by=3), dict(conv_bx=2), dict(conv_kc=0), dict(fft_reg=2), dict(nxcd=0), dict(cn_xz=2), dict(cnxz_by=3), dict(conv_kc=41), dict(fftx_tk=3),
                 dict(fftx_tk=6), dict(fftz_nl=100), dict(cny_ky=37), dict(cnx_nlx=10**6), dict(conv_fx=3),
-                dict(cnzy=3), dict(cnzy_ky=37), dict(cnzy_nxcd=0),
+                dict(cnzy=3), dict(cnzy_ky=37), dict(cnzy_nxcd=0), dict(upd_w2=2),
                 dict(ytdma_c=12), dict(ytdma_c=-2), dict(no_such_key=1)):
         with pytest.raises(pkg.ChsError):
             ch.set_tuning(**bad)
