@@ -1355,10 +1355,10 @@ __global__ __launch_bounds__(256) void k_cn_y3_march(Geo g, const W *__restrict_
 // in turns, one barrier per batch.  The arithmetic is k_cn_z_march's and k_cn_y3_march's, in
 // their order.  v's row 1 input (u's increment at row 1, RHS_CvLpGpS_tg.f90:45-46) is formed
 // by the v block of the first chunk from u's rows 1..1+D, z-solved the same way, before its
-// own rows.  Slab plane s holds plane (k0 + s - DZM) mod N; Dz <= DZM and 2 Dz <= KB.  Lanes past
-// L and planes past N (ragged last tiles) fill their slab slots from wrapped planes and store
-// nothing.
-template <typename W, int D, int KB, int BT, int DZM>
+// own rows.  Slab plane s holds plane (k0 + s - DZM) mod N; Dz <= DZM and 2 Dz <= KB.  With RG
+// (ragged last tiles), lanes past L and planes past N fill their slab slots from wrapped planes
+// and store nothing; without it L is a multiple of 64 and N of KB (no per-lane conditions).
+template <typename W, int D, int KB, int BT, int DZM, bool RG>
 __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k_cn_zy_march(Geo g, const W *__restrict__ gz, int Dz,
                                                          const W *__restrict__ by0, const W *__restrict__ by1,
                                                          const W *__restrict__ by2, const W *__restrict__ rin,
@@ -1374,14 +1374,14 @@ __global__ __launch_bounds__(64 * KB, (sizeof(W) == 8 && D >= 7) ? 4 : 8) void k
     const int c = cz % 3, jc = cz / 3;
     const int lane = threadIdx.x, kw = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int k0 = kt * KB, k = k0 + kw;
-    const bool iv = it * 64 + lane < L, kv = k < N;
-    const int kown = kv ? k : k % N;
+    const bool iv = !RG || it * 64 + lane < L, kv = !RG || k < N;
+    const int kown = RG ? k % N : k;
     const size_t cs = (size_t)N * M * L, hs = (size_t)N * (M + 2) * L;
     const W *__restrict__ by = c == 0 ? by0 : (c == 1 ? by1 : by2);
     // the side plane this wavefront loads (first 2 Dz wavefronts) and its slab position
     const bool side = kw < 2 * Dz;
     int kh = kw < Dz ? k0 - Dz + kw : k0 + KB + kw - Dz;
-    kh = ((kh % N) + N) % N;
+    kh = RG ? ((kh % N) + N) % N : (kh < 0 ? kh + N : (kh >= N ? kh - N : kh));
     const int sh = kw < Dz ? DZM - Dz + kw : DZM + KB + kw - Dz;
     const int so = DZM + kw;
     const W sub0 = dpg ? (W)dpg[0] : W(0);   // u's deferred -DPGRNS (see k_cn_z_march)
@@ -3437,18 +3437,15 @@ int substep_impl(chs_ctx *c, int ns, double dt, double cvisc, bool host_dt = tru
         const W *b0 = (const W *)t.by[0], *b1 = (const W *)t.by[1], *b2 = (const W *)t.by[2];
         const double *dpg = fx ? c->scal : nullptr;
         const int nx = c->tune.cnzy_nxcd;
-        if (t.Dz <= 4) {
-            switch (Dym) {
-#define CHS_ZY4(d) case d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 4>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break;
-                CHS_ZY4(0) CHS_ZY4(1) CHS_ZY4(2) CHS_ZY4(3) CHS_ZY4(4) CHS_ZY4(5) CHS_ZY4(6) CHS_ZY4(7) CHS_ZY4(8)
-#undef CHS_ZY4
-            }
-        } else {
-            switch (Dym) {
-#define CHS_ZY8(d) case d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 8>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break;
-                CHS_ZY8(0) CHS_ZY8(1) CHS_ZY8(2) CHS_ZY8(3) CHS_ZY8(4) CHS_ZY8(5) CHS_ZY8(6) CHS_ZY8(7) CHS_ZY8(8)
-#undef CHS_ZY8
-            }
+        const int sel = (t.Dz <= 4 ? 0 : 1) + (full ? 0 : 2);
+        switch (sel * 16 + Dym) {
+#define CHS_ZY(d) \
+    case 0 * 16 + d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 4, false>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break; \
+    case 1 * 16 + d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 8, false>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break; \
+    case 2 * 16 + d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 4, true>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break; \
+    case 3 * 16 + d: hipLaunchKernelGGL((k_cn_zy_march<W, d, ZYKB, ZYBT, 8, true>), dim3(nb), dim3(64, ZYKB), 0, c->stream, g, (const W *)t.gz, t.Dz, b0, b1, b2, rhs, c->q, c->qs, KZY, dpg, nx); break;
+            CHS_ZY(0) CHS_ZY(1) CHS_ZY(2) CHS_ZY(3) CHS_ZY(4) CHS_ZY(5) CHS_ZY(6) CHS_ZY(7) CHS_ZY(8)
+#undef CHS_ZY
         }
     } else if (xz) {
         // rhs2 already holds the x- and z-solved increments
