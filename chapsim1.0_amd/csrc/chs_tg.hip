@@ -2848,7 +2848,7 @@ struct chs_ctx {
         int conv_dma = 4;                   // k_conv_rhs_dma rows per block (2, 4; 0 = k_conv_rhs_lds)
         int divg_nxcd = 8;                  // XCDs the k_divg_fftx_r rows are grouped for (1 = off)
         int cny3_ky = 48;                   // k_cn_y3_march rows per chunk (0 = M; -1 = one launch per component)
-        int upd_kz = 8;                     // k_update z-planes per thread
+        int upd_kz = 4;                     // k_update z-planes per thread
         int upd_by = 4;                     // k_update rows per block (1, 2, 4, 8, 16)
         int upd_nxcd = 1;                   // k_update tiles grouped for this many XCDs (1: launch order;
                                             // 8 measured equal at 256x192x256)

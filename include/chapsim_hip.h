@@ -160,7 +160,7 @@ int chs_set_refinement(chs_ctx *ctx, int nref_step, int nref_projection);
  * per chunk (48; 0 = M), "cnzy_nxcd" the XCDs its tiles
  * are grouped for (8), "divg_nxcd" and "upd_nxcd" the XCDs the divergence
  * transform's and the correction's tiles are grouped for (8 and 1; 1 = dispatch order), "upd_kz"
- * z-planes each correction thread marches (8) and "upd_by" its rows per block (4), "upd_w2" 1
+ * z-planes each correction thread marches (4) and "upd_by" its rows per block (4), "upd_w2" 1
  * (default) for the correction with two points per lane where L is a multiple of 128 (0: one).  Results do not
  * depend on the shapes except through the summation order of the mass-flux partials
  * (conv_by, conv_kc, conv_fx) and conv_fx's rounding.  A captured step graph is re-made on
