@@ -1746,9 +1746,9 @@ struct chs_io {
     int cnz_cols = 0, cny_cols = 0;   // cn_cols for the z or the y solve alone (0: cn_cols)
     int cnz_p = 8, cny_p = 8;         // line points per thread of the z and y solves (4 or 8; 8:
                                       // 6.21 -> 6.09 (z) and 6.14 (y) ms per heated CN step)
-    int cnx_wave = 1;
+    int cnx_wave = 1;     // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
     int cn_fastdiv = 1;   // z and y Jacobi coefficients by Newton reciprocals (the sweeps resolve
-                          // no finer; 0.26 -> 0.24 and 0.29 -> 0.28 ms per stage at 256 x 128 x 256)   // x-lines of the Crank-Nicolson solve one per wavefront (L = 64, 128, 256, 512)
+                          // no finer; 0.26 -> 0.24 and 0.29 -> 0.28 ms per stage at 256 x 128 x 256)
     int cn_cols = 16;   // most x-columns per block of the z and y Jacobi line solves (16: 1.7 %
                         // faster per heated step than 64 at 256 x 128 x 256)
     double *own_d0 = nullptr, *own_m0 = nullptr;   // d.d0 / d.m0 may alias an energy context's
